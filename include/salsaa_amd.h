@@ -1,0 +1,243 @@
+/*
+ * salsaa_amd.h -- C ABI of the MI355X (gfx950) SALSAA prover hot path.
+ *
+ * Two groups of entry points:
+ *
+ * 1. The legacy hexl_wrapper symbols, with the names, signatures and semantics
+ *    the reference's Rust FFI binds (src/hexl/bindings.rs:3-73, defined in
+ *    hexl-bindings/hexl_wrapper.cpp).  They take host pointers to single
+ *    vectors (one ring element or one scalar at a time), exactly as the
+ *    reference calls them, and run on the host: this keeps the unchanged
+ *    reference crate linkable.  They are thread-safe.
+ *
+ * 2. Batched device entry points that replace the reference's operator API
+ *    (src/arithmetic.rs and the files in src/subroutines/) on whole matrices resident in
+ *    HBM.  All pointers named d_* are device pointers.  Every function returns
+ *    an int status (SALSAA_OK = 0, negative on error) and never throws;
+ *    salsaa_last_error() gives the message.  Work is enqueued on the context's
+ *    stream; a call returns once enqueued unless stated otherwise.
+ *
+ * Data layout: a ring element is d consecutive uint64 canonical residues in
+ * [0, q); a ring matrix is [rows][cols][d] row-major.  Representation tags:
+ * SALSAA_REP_COEFF (Coefficient), SALSAA_REP_INCOMPLETE (IncompleteNTT: data =
+ * NTT_{d/2}(even coeffs) || NTT_{d/2}(odd coeffs), HEXL convention), and
+ * SALSAA_REP_FULL (NTT: NTT_d of the coefficients)
+ * (reference: src/cyclotomic_ring.rs:13-18, 483-556).
+ *
+ * The device kernels support d = 64 and any prime q < 2^50 with q = 1 mod 128.
+ */
+#ifndef SALSAA_AMD_H
+#define SALSAA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* 1. legacy hexl_wrapper ABI (host)                                        */
+/* ------------------------------------------------------------------------ */
+
+/* hexl_wrapper.cpp:34-36, bindings.rs:3 */
+uint64_t multiply_mod(uint64_t a, uint64_t b, uint64_t modulus);
+/* hexl_wrapper.cpp:61-63, bindings.rs:5 */
+uint64_t power_mod(uint64_t a, uint64_t b, uint64_t modulus);
+/* hexl_wrapper.cpp:38-41, bindings.rs:7 */
+uint64_t add_mod(uint64_t a, uint64_t b, uint64_t modulus);
+/* hexl_wrapper.cpp:43-45, bindings.rs:9 */
+uint64_t sub_mod(uint64_t a, uint64_t b, uint64_t modulus);
+/* hexl_wrapper.cpp:65-67, bindings.rs:11-17 */
+void eltwise_mult_mod(uint64_t *result, const uint64_t *operand1, const uint64_t *operand2,
+                      uint64_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:47-50, bindings.rs:19: bit-reversed psi powers (n entries),
+ * owned by a process-lifetime cache */
+const uint64_t *get_roots(uint64_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:52-54, bindings.rs:20 */
+uint64_t inv_mod(uint64_t a, uint64_t modulus);
+/* hexl_wrapper.cpp:56-59, bindings.rs:21 */
+const uint64_t *get_inv_roots(uint64_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:262-268, bindings.rs:23 */
+uint64_t sum_sq(const uint64_t *operand1, uint64_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:271-276, bindings.rs:24 */
+uint64_t sum_sq_fast(const uint64_t *operand1, uint64_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:231-237, bindings.rs:25 */
+uint64_t sum(const uint64_t *operand1, uint64_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:239-260, bindings.rs:26 (defined result: sum mod q) */
+uint64_t sum_fast(const uint64_t *operand1, uint64_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:68-70, bindings.rs:27-34: result = operand1 * operand2 + operand3 */
+void eltwise_fma_mod(uint64_t *result, const uint64_t *operand1, uint64_t operand2,
+                     const uint64_t *operand3, uint64_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:188-196, bindings.rs:36-42 */
+void eltwise_add_mod(uint64_t *result, const uint64_t *operand1, const uint64_t *operand2,
+                     uint64_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:199-207, bindings.rs:44-50 */
+void eltwise_sub_mod(uint64_t *result, const uint64_t *operand1, const uint64_t *operand2,
+                     uint64_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:73-93, bindings.rs:52-58: product mod X^n + 1 */
+void multiply_poly(uint64_t *result, const uint64_t *operand1, const uint64_t *operand2,
+                   uint64_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:119-126, bindings.rs:60: result = operand mod q, any input */
+void eltwise_reduce_mod(uint64_t *result, const uint64_t *operand, uint64_t n,
+                        uint64_t modulus);
+/* hexl_wrapper.cpp:158-186, bindings.rs:61 */
+void eltwise_reduce_mod_naive(uint64_t *result, const uint64_t *operand, uint64_t n,
+                              uint64_t modulus);
+/* hexl_wrapper.cpp:95-117, bindings.rs:63-69 */
+void polynomial_multiply_cyclotomic_mod(uint64_t *result, const uint64_t *operand1,
+                                        const uint64_t *operand2, uint64_t phi,
+                                        uint64_t mod_q);
+/* hexl_wrapper.cpp:213-220, bindings.rs:71: HEXL negacyclic NTT, in place */
+void ntt_forward_in_place(uint64_t *operand, size_t n, uint64_t modulus);
+/* hexl_wrapper.cpp:222-229, bindings.rs:73 */
+void ntt_inverse_in_place(uint64_t *operand, size_t n, uint64_t modulus);
+
+/* ------------------------------------------------------------------------ */
+/* 2. batched device API                                                    */
+/* ------------------------------------------------------------------------ */
+
+#define SALSAA_OK 0
+#define SALSAA_ERR_ARG -1
+#define SALSAA_ERR_HIP -2
+#define SALSAA_ERR_NOMEM -3
+#define SALSAA_ERR_CHECK -4 /* a prover-side self-check failed */
+
+#define SALSAA_REP_COEFF 0
+#define SALSAA_REP_FULL 1
+#define SALSAA_REP_INCOMPLETE 2
+
+typedef struct salsaa_ctx salsaa_ctx;
+typedef struct salsaa_key salsaa_key;
+
+/* Message of the last failing call on this thread. */
+const char *salsaa_last_error(void);
+/* Library version string. */
+const char *salsaa_version(void);
+
+/* Context for modulus q and degree d on the current HIP device: builds the
+ * NTT tables (the role of the reference's NTTCache, hexl_wrapper.cpp:11-32)
+ * and one default-priority stream.  d must be 64. */
+int salsaa_ctx_create(uint64_t q, uint32_t d, salsaa_ctx **out);
+int salsaa_ctx_destroy(salsaa_ctx *ctx);
+int salsaa_ctx_sync(salsaa_ctx *ctx);
+/* The context's stream (hipStream_t) so callers can order their own work. */
+void *salsaa_ctx_stream(salsaa_ctx *ctx);
+
+/* Device memory and copies (stream-ordered on the context stream). */
+int salsaa_dev_alloc(salsaa_ctx *ctx, size_t bytes, void **d_ptr);
+int salsaa_dev_free(salsaa_ctx *ctx, void *d_ptr);
+int salsaa_memcpy_h2d(salsaa_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int salsaa_memcpy_d2h(salsaa_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+
+/* Events on the context stream, for timing enqueued work. */
+int salsaa_event_create(void **ev);
+int salsaa_event_destroy(void *ev);
+int salsaa_event_record(salsaa_ctx *ctx, void *ev);
+int salsaa_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* Seeded sampling on the device, element i of the stream drawn from
+ * SplitMix64(seed, stream, i) (the reference samplers, cyclotomic_ring.rs:
+ * 257-317, are unseeded).  uniform: [0, q); bounded: uniform over the
+ * 2*bound+1 integers [-bound, bound], mapped mod q. */
+int salsaa_sample_uniform(salsaa_ctx *ctx, uint64_t *d_out, size_t n, uint64_t seed,
+                          uint64_t stream);
+int salsaa_sample_bounded(salsaa_ctx *ctx, uint64_t *d_out, size_t n, uint64_t bound,
+                          uint64_t seed, uint64_t stream);
+/* Sumcheck challenges cs[reps][nrounds][2]: coordinate 0 (and 1 when fq2) is
+ * uniform mod q from stream 0x5C000 + rep, index 2*round (+1). */
+int salsaa_sample_challenges(salsaa_ctx *ctx, uint64_t *d_cs, size_t reps, size_t nrounds,
+                             int fq2, uint64_t seed);
+
+/* Batched representation change of `count` ring elements in place:
+ * to_incomplete_ntt_representation / to_ntt_representation /
+ * to_coeff_representation (cyclotomic_ring.rs:483-556), one thread per
+ * length-d/2 or length-d transform.  from_rep, to_rep: SALSAA_REP_*. */
+int salsaa_convert(salsaa_ctx *ctx, uint64_t *d_data, size_t count, int from_rep,
+                   int to_rep);
+
+/* Batched ring product out[i] = a[i] * b[i] (both Coefficient), result in
+ * IncompleteNTT form: incomplete_ntt_multiplication (cyclotomic_ring.rs:
+ * 699-791) for `count` pairs. */
+int salsaa_ring_mul(salsaa_ctx *ctx, const uint64_t *d_a, const uint64_t *d_b,
+                    uint64_t *d_out, size_t count);
+
+/* Commitment key: A [kappa][m][d] Coefficient, made resident in HBM in the
+ * slot-major full-NTT layout the commit kernel reads.  The reference stores
+ * the key as Vec<PowerSeries> (arithmetic.rs:146-150); a uniform key has one
+ * layer of length m; commits with a shorter witness use the prefix layer. */
+int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t m,
+                      salsaa_key **out);
+int salsaa_key_destroy(salsaa_key *key);
+
+/* parallel_dot_series_matrix (arithmetic.rs:459-505): commit a witness
+ * W [r][m][d] (Coefficient) under the key's length-m prefix:
+ * out[j][k] = sum_i A_j[i] * W_k[i], out [kappa][r][d] IncompleteNTT. */
+int salsaa_parallel_dot_series_matrix(salsaa_ctx *ctx, const salsaa_key *key,
+                                      const uint64_t *d_W, size_t r, size_t m,
+                                      uint64_t *d_out);
+
+/* split (split.rs:45-76): the rows [L_0..L_{r-1}, R_0..R_{r-1}] of the halves
+ * of W [r][m][d] (a view, not copied) are committed under the length-m/2
+ * prefix: d_out [kappa][2r][d] IncompleteNTT. */
+int salsaa_split(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_W, size_t r,
+                 size_t m, uint64_t *d_out);
+
+/* Norm check = norm_check_1 + norm_check_2 + norm_check_3 with
+ * verify_norm_check_2/3 (fast_norm_check.rs:282-462) for every row of
+ * w [rows][L] (Coefficient data flattened, L = m*d a power of two), `reps`
+ * repetitions (protocol.rs:222-259) with the given challenges
+ * cs [reps][log2 L][2].  fq2 = 0: challenges in F_q (reference semantics);
+ * fq2 = 1: challenges in F_{q^2} = F_q[Y]/(Y^2 - nr).  Outputs (device):
+ *   claims [rows]                  sum of squares mod q (norm_check_1)
+ *   msgs   [reps][log2 L][rows][3][2]  round messages (f0, f1, f2), canonical
+ *   finals [reps][rows][2]         verifier claims after the last round
+ *   evals  [reps][rows][2]         folded witness <w, tensor(cs)>
+ *   rhs    [reps][rows][2][d]      norm_check_3 rhs = base * V, IncompleteNTT
+ *                                  (component 1 is zero for F_q challenges);
+ *                                  requires L >= d
+ *   ok     [reps][rows]            1 if every verifier check passed
+ * Any output pointer except ok may be NULL to skip it. */
+int salsaa_norm_check(salsaa_ctx *ctx, const uint64_t *d_w, size_t rows, size_t L,
+                      const uint64_t *d_cs, size_t reps, int fq2, uint64_t nr,
+                      uint64_t *d_claims, uint64_t *d_msgs, uint64_t *d_finals,
+                      uint64_t *d_evals, uint64_t *d_rhs, int32_t *d_ok);
+
+/* decompose_matrix_by_chunks (arithmetic.rs:1572-1593): W [r][m][d]
+ * Coefficient -> d_out [r*parts][m][d], limb t of row rho in row rho*parts+t.
+ * radix = 0 selects the reference's auto radix 2^ceil(bw/parts); the radix
+ * used is written to *d_radix (device). */
+int salsaa_decompose(salsaa_ctx *ctx, const uint64_t *d_W, size_t r, size_t m,
+                     size_t parts, uint64_t radix, uint64_t *d_out, uint64_t *d_radix);
+
+/* decomp_ell (decomp.rs:32-67): decompose, then commit the limbs:
+ * d_new_W [r*parts][m][d], d_rhs [kappa][r*parts][d] IncompleteNTT. */
+int salsaa_decomp_ell(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_W,
+                      size_t r, size_t m, size_t parts, uint64_t radix, uint64_t *d_new_W,
+                      uint64_t *d_rhs, uint64_t *d_radix);
+
+/* compose_with_radix_mod (arithmetic.rs:1719-1755) on the device:
+ * in [rows][n*parts][d] -> out [rows][n][d], radix read from *d_radix. */
+int salsaa_compose(salsaa_ctx *ctx, const uint64_t *d_in, size_t rows, size_t n,
+                   size_t parts, const uint64_t *d_radix, uint64_t *d_out);
+
+/* verify_decomp (decomp.rs:95-112): counts entries where
+ * compose(d_rhs_dec [kappa][r*parts][d]) differs from d_rhs [kappa][r][d];
+ * with rows_mode = 1 instead checks the witness limbs
+ * d_rhs_dec = limbs [r*parts][m][d] against d_rhs = W [r][m][d] (pass
+ * kappa = r, r = m).  *d_mismatch (device uint32) is incremented. */
+int salsaa_verify_decomp(salsaa_ctx *ctx, const uint64_t *d_rhs_dec, const uint64_t *d_rhs,
+                         size_t kappa, size_t r, size_t parts, const uint64_t *d_radix,
+                         int rows_mode, uint32_t *d_mismatch);
+
+/* fold (fold.rs:22-27 -> parallel_dot_matrix_matrix, arithmetic.rs:429-457):
+ * out[j][i] = sum_k W[k][i] * C[j][k]; W [r_in][m][d] and C [r_out][r_in][d]
+ * Coefficient, out [r_out][m][d] IncompleteNTT. */
+int salsaa_parallel_dot_matrix_matrix(salsaa_ctx *ctx, const uint64_t *d_W, size_t r_in,
+                                      size_t m, const uint64_t *d_C, size_t r_out,
+                                      uint64_t *d_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

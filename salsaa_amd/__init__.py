@@ -1,0 +1,333 @@
+"""salsaa_amd -- MI355X (gfx950) implementation of SALSAA's prover hot path.
+
+The compute lives in libsalsaa_amd.so (hand-written HIP kernels behind the C
+ABI declared in include/salsaa_amd.h).  This module is the host-side mirror
+of the reference's operator API over that ABI: it loads the library (and
+fails loudly if it has not been built), manages device buffers, and exposes
+the reference's function names (parallel_dot_series_matrix, norm_check,
+decomp_ell, parallel_dot_matrix_matrix, split) on HBM-resident matrices.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsalsaa_amd.so")
+
+Q = 1125899906840833  # largest prime < 2^50 with q = 1 mod 128
+D = 64
+NR = 5  # smallest quadratic non-residue mod Q
+
+REP_COEFF, REP_FULL, REP_INCOMPLETE = 0, 1, 2
+
+# sampler streams used by the configs (any distinct values work)
+STREAM_A, STREAM_W, STREAM_NTT, STREAM_FOLD = 0xA, 0xB, 0xC, 0xF
+
+_u64 = ctypes.c_uint64
+_sz = ctypes.c_size_t
+_vp = ctypes.c_void_p
+_int = ctypes.c_int
+_PP = ctypes.POINTER(ctypes.c_void_p)
+
+
+class SalsaaError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libsalsaa_amd.so is not built (expected at %s); run "
+            "`python -c 'import __graft_entry__ as g; g.build()'` or "
+            "`make -C salsaa_amd/csrc`" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        # legacy hexl_wrapper ABI
+        "multiply_mod": (_u64, [_u64, _u64, _u64]),
+        "power_mod": (_u64, [_u64, _u64, _u64]),
+        "add_mod": (_u64, [_u64, _u64, _u64]),
+        "sub_mod": (_u64, [_u64, _u64, _u64]),
+        "inv_mod": (_u64, [_u64, _u64]),
+        "eltwise_mult_mod": (None, [_vp, _vp, _vp, _u64, _u64]),
+        "eltwise_fma_mod": (None, [_vp, _vp, _u64, _vp, _u64, _u64]),
+        "eltwise_add_mod": (None, [_vp, _vp, _vp, _u64, _u64]),
+        "eltwise_sub_mod": (None, [_vp, _vp, _vp, _u64, _u64]),
+        "eltwise_reduce_mod": (None, [_vp, _vp, _u64, _u64]),
+        "eltwise_reduce_mod_naive": (None, [_vp, _vp, _u64, _u64]),
+        "get_roots": (ctypes.POINTER(_u64), [_u64, _u64]),
+        "get_inv_roots": (ctypes.POINTER(_u64), [_u64, _u64]),
+        "sum": (_u64, [_vp, _u64, _u64]),
+        "sum_fast": (_u64, [_vp, _u64, _u64]),
+        "sum_sq": (_u64, [_vp, _u64, _u64]),
+        "sum_sq_fast": (_u64, [_vp, _u64, _u64]),
+        "multiply_poly": (None, [_vp, _vp, _vp, _u64, _u64]),
+        "polynomial_multiply_cyclotomic_mod": (None, [_vp, _vp, _vp, _u64, _u64]),
+        "ntt_forward_in_place": (None, [_vp, _sz, _u64]),
+        "ntt_inverse_in_place": (None, [_vp, _sz, _u64]),
+        # device API
+        "salsaa_last_error": (ctypes.c_char_p, []),
+        "salsaa_version": (ctypes.c_char_p, []),
+        "salsaa_ctx_create": (_int, [_u64, ctypes.c_uint32, _PP]),
+        "salsaa_ctx_destroy": (_int, [_vp]),
+        "salsaa_ctx_sync": (_int, [_vp]),
+        "salsaa_ctx_stream": (_vp, [_vp]),
+        "salsaa_dev_alloc": (_int, [_vp, _sz, _PP]),
+        "salsaa_dev_free": (_int, [_vp, _vp]),
+        "salsaa_memcpy_h2d": (_int, [_vp, _vp, _vp, _sz]),
+        "salsaa_memcpy_d2h": (_int, [_vp, _vp, _vp, _sz]),
+        "salsaa_event_create": (_int, [_PP]),
+        "salsaa_event_destroy": (_int, [_vp]),
+        "salsaa_event_record": (_int, [_vp, _vp]),
+        "salsaa_event_elapsed_ms": (_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+        "salsaa_sample_uniform": (_int, [_vp, _vp, _sz, _u64, _u64]),
+        "salsaa_sample_bounded": (_int, [_vp, _vp, _sz, _u64, _u64, _u64]),
+        "salsaa_sample_challenges": (_int, [_vp, _vp, _sz, _sz, _int, _u64]),
+        "salsaa_convert": (_int, [_vp, _vp, _sz, _int, _int]),
+        "salsaa_ring_mul": (_int, [_vp, _vp, _vp, _vp, _sz]),
+        "salsaa_key_create": (_int, [_vp, _vp, _sz, _sz, _PP]),
+        "salsaa_key_destroy": (_int, [_vp]),
+        "salsaa_parallel_dot_series_matrix": (_int, [_vp, _vp, _vp, _sz, _sz, _vp]),
+        "salsaa_split": (_int, [_vp, _vp, _vp, _sz, _sz, _vp]),
+        "salsaa_norm_check": (_int, [_vp, _vp, _sz, _sz, _vp, _sz, _int, _u64, _vp, _vp, _vp,
+                                     _vp, _vp, _vp]),
+        "salsaa_decompose": (_int, [_vp, _vp, _sz, _sz, _sz, _u64, _vp, _vp]),
+        "salsaa_decomp_ell": (_int, [_vp, _vp, _vp, _sz, _sz, _sz, _u64, _vp, _vp, _vp]),
+        "salsaa_compose": (_int, [_vp, _vp, _sz, _sz, _sz, _vp, _vp]),
+        "salsaa_verify_decomp": (_int, [_vp, _vp, _vp, _sz, _sz, _sz, _vp, _int, _vp]),
+        "salsaa_parallel_dot_matrix_matrix": (_int, [_vp, _vp, _sz, _sz, _vp, _sz, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+#: every C-ABI symbol declared in include/salsaa_amd.h
+EXPORTED = None
+
+
+def _check(status):
+    if status != 0:
+        msg = lib.salsaa_last_error()
+        raise SalsaaError("salsaa call failed (%d): %s" % (status, msg.decode() if msg else ""))
+
+
+class DeviceArray:
+    """A device buffer of uint64 (or int32/uint32) owned by a Context."""
+
+    def __init__(self, ctx, shape, dtype=np.uint64):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib.salsaa_dev_alloc(ctx.h, max(self.nbytes, 1), ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes, (arr.shape, self.shape)
+        _check(lib.salsaa_memcpy_h2d(self.ctx.h, self.ptr, arr.ctypes.data, self.nbytes))
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib.salsaa_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr is not None:
+            lib.salsaa_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Key:
+    """Resident commitment key (A in the slot-major full-NTT layout)."""
+
+    def __init__(self, ctx, A_dev, kappa, m):
+        self.ctx, self.kappa, self.m = ctx, kappa, m
+        p = ctypes.c_void_p()
+        _check(lib.salsaa_key_create(ctx.h, A_dev.ptr, kappa, m, ctypes.byref(p)))
+        self.h = p.value
+
+    def close(self):
+        if self.h:
+            lib.salsaa_key_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """A modulus/degree context on the current GPU (one stream)."""
+
+    def __init__(self, q=Q, d=D):
+        p = ctypes.c_void_p()
+        _check(lib.salsaa_ctx_create(q, d, ctypes.byref(p)))
+        self.h, self.q, self.d = p.value, q, d
+
+    def close(self):
+        if self.h:
+            lib.salsaa_ctx_destroy(self.h)
+            self.h = None
+
+    def sync(self):
+        _check(lib.salsaa_ctx_sync(self.h))
+
+    # ---- buffers
+    def empty(self, shape, dtype=np.uint64):
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, arr, dtype=np.uint64):
+        arr = np.ascontiguousarray(arr, dtype=dtype)
+        return DeviceArray(self, arr.shape, dtype).upload(arr)
+
+    def uniform(self, shape, seed, stream):
+        a = self.empty(shape)
+        _check(lib.salsaa_sample_uniform(self.h, a.ptr, int(np.prod(shape)), seed, stream))
+        return a
+
+    def bounded(self, shape, bound, seed, stream):
+        a = self.empty(shape)
+        _check(lib.salsaa_sample_bounded(self.h, a.ptr, int(np.prod(shape)), bound, seed, stream))
+        return a
+
+    def challenges(self, reps, nrounds, fq2, seed):
+        cs = self.empty((reps, nrounds, 2))
+        _check(lib.salsaa_sample_challenges(self.h, cs.ptr, reps, nrounds, int(fq2), seed))
+        return cs
+
+    # ---- events
+    def event(self):
+        p = ctypes.c_void_p()
+        _check(lib.salsaa_event_create(ctypes.byref(p)))
+        return p.value
+
+    def record(self, ev):
+        _check(lib.salsaa_event_record(self.h, ev))
+
+    @staticmethod
+    def elapsed_ms(a, b):
+        ms = ctypes.c_float()
+        _check(lib.salsaa_event_elapsed_ms(a, b, ctypes.byref(ms)))
+        return ms.value
+
+    # ---- operators (reference names)
+    def convert(self, arr, from_rep, to_rep):
+        """to_{incomplete_ntt,ntt,coeff}_representation on every element, in place."""
+        count = int(np.prod(arr.shape[:-1]))
+        _check(lib.salsaa_convert(self.h, arr.ptr, count, from_rep, to_rep))
+        return arr
+
+    def ring_mul(self, a, b):
+        out = self.empty(a.shape)
+        _check(lib.salsaa_ring_mul(self.h, a.ptr, b.ptr, out.ptr, int(np.prod(a.shape[:-1]))))
+        return out
+
+    def key(self, A_dev):
+        kappa, m, d = A_dev.shape
+        assert d == self.d
+        return Key(self, A_dev, kappa, m)
+
+    def parallel_dot_series_matrix(self, key, W, out=None):
+        r, m, d = W.shape
+        out = out or self.empty((key.kappa, r, d))
+        _check(lib.salsaa_parallel_dot_series_matrix(self.h, key.h, W.ptr, r, m, out.ptr))
+        return out
+
+    def split(self, key, W, out=None):
+        r, m, d = W.shape
+        out = out or self.empty((key.kappa, 2 * r, d))
+        _check(lib.salsaa_split(self.h, key.h, W.ptr, r, m, out.ptr))
+        return out
+
+    def norm_check(self, w, cs, fq2, nr=NR, want_rhs=True, outs=None):
+        """norm_check_1 + norm_check_2 (+ verifier) + norm_check_3 for every
+        row of w [rows][L] (any shape whose leading axis is the row)."""
+        rows = w.shape[0]
+        L = int(np.prod(w.shape[1:]))
+        reps, nrounds, _ = cs.shape
+        if outs is None:
+            outs = dict(
+                claims=self.empty((rows,)),
+                msgs=self.empty((reps, nrounds, rows, 3, 2)),
+                finals=self.empty((reps, rows, 2)),
+                evals=self.empty((reps, rows, 2)),
+                rhs=self.empty((reps, rows, 2, self.d)) if want_rhs else None,
+                ok=self.empty((reps, rows), np.int32),
+            )
+        rhs = outs["rhs"].ptr if outs.get("rhs") is not None else None
+        _check(lib.salsaa_norm_check(self.h, w.ptr, rows, L, cs.ptr, reps, int(fq2), nr,
+                                     outs["claims"].ptr, outs["msgs"].ptr, outs["finals"].ptr,
+                                     outs["evals"].ptr, rhs, outs["ok"].ptr))
+        return outs
+
+    def decompose(self, W, parts, radix=0, out=None, d_radix=None):
+        r, m, d = W.shape
+        out = out or self.empty((r * parts, m, d))
+        d_radix = d_radix or self.empty((1,))
+        _check(lib.salsaa_decompose(self.h, W.ptr, r, m, parts, radix, out.ptr, d_radix.ptr))
+        return out, d_radix
+
+    def decomp_ell(self, key, W, parts, radix=0, new_W=None, rhs=None, d_radix=None):
+        r, m, d = W.shape
+        new_W = new_W or self.empty((r * parts, m, d))
+        rhs = rhs or self.empty((key.kappa, r * parts, d))
+        d_radix = d_radix or self.empty((1,))
+        _check(lib.salsaa_decomp_ell(self.h, key.h, W.ptr, r, m, parts, radix, new_W.ptr, rhs.ptr,
+                                     d_radix.ptr))
+        return new_W, rhs, d_radix
+
+    def compose(self, x, parts, d_radix):
+        rows, np_, d = x.shape
+        out = self.empty((rows, np_ // parts, d))
+        _check(lib.salsaa_compose(self.h, x.ptr, rows, np_ // parts, parts, d_radix.ptr, out.ptr))
+        return out
+
+    def verify_decomp(self, rhs_dec, rhs, parts, d_radix, rows_mode=False, mismatch=None):
+        """Counts mismatches of compose(rhs_dec) against rhs on the device."""
+        if rows_mode:
+            kappa, r = rhs.shape[0], rhs.shape[1]
+        else:
+            kappa, r = rhs.shape[0], rhs.shape[1]
+        if mismatch is None:
+            mismatch = self.to_device(np.zeros(1, np.uint32), np.uint32)
+        _check(lib.salsaa_verify_decomp(self.h, rhs_dec.ptr, rhs.ptr, kappa, r, parts,
+                                        d_radix.ptr, int(rows_mode), mismatch.ptr))
+        return mismatch
+
+    def parallel_dot_matrix_matrix(self, W, C, out=None):
+        """fold: out[j][i] = sum_k W[k][i] * C[j][k] (IncompleteNTT)."""
+        r_in, m, d = W.shape
+        r_out = C.shape[0]
+        out = out or self.empty((r_out, m, d))
+        _check(lib.salsaa_parallel_dot_matrix_matrix(self.h, W.ptr, r_in, m, C.ptr, r_out,
+                                                     out.ptr))
+        return out
+
+    fold = parallel_dot_matrix_matrix
+
+
+def header_symbols():
+    """Names of every function declared in include/salsaa_amd.h."""
+    import re
+    root = os.path.dirname(_HERE)
+    with open(os.path.join(root, "include", "salsaa_amd.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)) -
+                  {"if", "defined"})

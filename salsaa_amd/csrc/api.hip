@@ -1,0 +1,397 @@
+// api.hip -- the batched device C ABI: contexts, keys, memory helpers and the
+// operator entry points that replace the reference's arithmetic.rs and
+// subroutines/*.rs functions on HBM-resident matrices.
+#include <cstring>
+#include <new>
+
+#include "common.hpp"
+
+struct salsaa_ctx {
+    salsaa::Context c;
+};
+struct salsaa_key {
+    salsaa::Key k;
+};
+
+namespace salsaa {
+
+namespace {
+thread_local std::string tl_error;
+
+uint64_t h_min_root(uint64_t order, uint64_t q) {
+    uint64_t w = 0;
+    for (uint64_t g = 2; g < q && !w; g++) {
+        uint64_t c = h_powmod(g, (q - 1) / order, q);
+        if (h_powmod(c, order / 2, q) == q - 1) w = c;
+    }
+    if (!w) return 0;
+    uint64_t w2 = h_mulmod(w, w, q), cur = w, best = w;
+    for (uint64_t i = 0; i < order / 2; i++) {
+        if (cur < best) best = cur;
+        cur = h_mulmod(cur, w2, q);
+    }
+    return best;
+}
+
+uint64_t h_brv(uint64_t x, int bits) {
+    uint64_t r = 0;
+    for (int i = 0; i < bits; i++) r |= ((x >> i) & 1ULL) << (bits - 1 - i);
+    return r;
+}
+
+bool is_prime_u64(uint64_t n) {
+    if (n < 2) return false;
+    for (uint64_t p : {2ULL, 3ULL, 5ULL, 7ULL, 11ULL, 13ULL, 17ULL, 19ULL, 23ULL, 29ULL, 31ULL, 37ULL}) {
+        if (n % p == 0) return n == p;
+    }
+    uint64_t d = n - 1;
+    int s = 0;
+    while ((d & 1) == 0) {
+        d >>= 1;
+        s++;
+    }
+    for (uint64_t a : {2ULL, 3ULL, 5ULL, 7ULL, 11ULL, 13ULL, 17ULL, 19ULL, 23ULL, 29ULL, 31ULL, 37ULL}) {
+        uint64_t x = h_powmod(a, d, n);
+        if (x == 1 || x == n - 1) continue;
+        bool comp = true;
+        for (int r = 1; r < s; r++) {
+            x = h_mulmod(x, x, n);
+            if (x == n - 1) {
+                comp = false;
+                break;
+            }
+        }
+        if (comp) return false;
+    }
+    return true;
+}
+
+// HEXL-convention tables for n = 32 and n = 64 plus the slot moduli omega.
+void build_tables(uint64_t q, std::vector<double> &tab) {
+    tab.assign(TAB_SIZE, 0.0);
+    auto fill = [&](int n, int fwd_off, int inv_off, int ninv_off) {
+        int bits = 0;
+        while ((1 << bits) < n) bits++;
+        uint64_t psi = h_min_root(2 * (uint64_t)n, q);
+        std::vector<uint64_t> roots(n);
+        uint64_t p = 1;
+        for (int i = 0; i < n; i++) {
+            roots[h_brv(i, bits)] = p;
+            p = h_mulmod(p, psi, q);
+        }
+        uint64_t ninv = h_invmod((uint64_t)n, q);
+        for (int i = 0; i < n; i++) {
+            tab[fwd_off + i] = (double)roots[i];
+            tab[inv_off + i] = (double)h_invmod(roots[i], q);
+        }
+        // last inverse stage uses inv(roots[1]) * n^{-1}
+        tab[inv_off + 0] = (double)h_mulmod(h_invmod(roots[1], q), ninv, q);
+        tab[ninv_off] = (double)ninv;
+        return roots;
+    };
+    std::vector<uint64_t> r32 = fill(32, TAB_FWD32, TAB_INV32, TAB_NINV32);
+    fill(64, TAB_FWD64, TAB_INV64, TAB_NINV64);
+    // omega = NTT_32(Y): the forward transform of (0, 1, 0, ...) is
+    // omega_j = psi^(2 brv(j) + 1) (evaluation at the slot roots)
+    uint64_t psi32 = h_min_root(64, q);
+    for (int j = 0; j < 32; j++) tab[TAB_OMEGA + j] = (double)h_powmod(psi32, 2 * h_brv(j, 5) + 1, q);
+    (void)r32;
+}
+
+}  // namespace
+
+void set_error(const std::string &msg) { tl_error = msg; }
+
+int scratch_get(Context *ctx, int slot, size_t bytes, void **out) {
+    ScratchBuf &b = ctx->scratch[slot];
+    if (b.bytes < bytes) {
+        if (b.ptr) SALSAA_HIP(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+        size_t want = bytes + bytes / 8 + 256;
+        hipError_t e = hipMalloc(&b.ptr, want);
+        if (e != hipSuccess) {
+            set_error(std::string("scratch allocation failed: ") + hipGetErrorString(e));
+            return SALSAA_ERR_NOMEM;
+        }
+        b.bytes = want;
+    }
+    *out = b.ptr;
+    return SALSAA_OK;
+}
+
+}  // namespace salsaa
+
+using namespace salsaa;
+
+#define CTX_OK(ctx)                                                  \
+    do {                                                             \
+        if (!(ctx)) {                                                \
+            set_error("null context");                               \
+            return SALSAA_ERR_ARG;                                   \
+        }                                                            \
+        SALSAA_HIP(hipSetDevice((ctx)->c.device));                   \
+    } while (0)
+
+extern "C" {
+
+const char *salsaa_last_error(void) { return tl_error.c_str(); }
+
+const char *salsaa_version(void) { return "salsaa_amd 0.1 gfx950"; }
+
+int salsaa_ctx_create(uint64_t q, uint32_t d, salsaa_ctx **out) {
+    SALSAA_CHECK(out != nullptr, "null output pointer");
+    SALSAA_CHECK(d == (uint32_t)kD, "device kernels support d = 64 only");
+    SALSAA_CHECK(q > 2 && q < (1ULL << 50), "q must be below 2^50");
+    SALSAA_CHECK((q - 1) % 128 == 0, "q must be 1 mod 128 (2d-th roots of unity)");
+    SALSAA_CHECK(is_prime_u64(q), "q must be prime");
+    salsaa_ctx *c = new (std::nothrow) salsaa_ctx();
+    if (!c) return SALSAA_ERR_NOMEM;
+    Context &x = c->c;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error(std::string("no HIP device: ") + hipGetErrorString(e));
+        delete c;
+        return SALSAA_ERR_HIP;
+    }
+    x.device = dev;
+    x.q = q;
+    x.d = (int)d;
+    x.qd = (double)q;
+    x.qinv = 1.0 / (double)q;
+    build_tables(q, x.h_tables);
+    if (hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&x.d_tables, TAB_SIZE * sizeof(double)) != hipSuccess ||
+        hipMemcpy(x.d_tables, x.h_tables.data(), TAB_SIZE * sizeof(double),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("context setup failed: " + std::string(hipGetErrorString(hipGetLastError())));
+        if (x.stream) (void)hipStreamDestroy(x.stream);
+        if (x.d_tables) (void)hipFree(x.d_tables);
+        delete c;
+        return SALSAA_ERR_HIP;
+    }
+    *out = c;
+    return SALSAA_OK;
+}
+
+int salsaa_ctx_destroy(salsaa_ctx *ctx) {
+    if (!ctx) return SALSAA_OK;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    for (auto &b : ctx->c.scratch)
+        if (b.ptr) (void)hipFree(b.ptr);
+    (void)hipFree(ctx->c.d_tables);
+    (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+    return SALSAA_OK;
+}
+
+int salsaa_ctx_sync(salsaa_ctx *ctx) {
+    CTX_OK(ctx);
+    SALSAA_HIP(hipStreamSynchronize(ctx->c.stream));
+    return SALSAA_OK;
+}
+
+void *salsaa_ctx_stream(salsaa_ctx *ctx) { return ctx ? (void *)ctx->c.stream : nullptr; }
+
+int salsaa_dev_alloc(salsaa_ctx *ctx, size_t bytes, void **d_ptr) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(d_ptr != nullptr, "null output pointer");
+    hipError_t e = hipMalloc(d_ptr, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return SALSAA_ERR_NOMEM;
+    }
+    return SALSAA_OK;
+}
+
+int salsaa_dev_free(salsaa_ctx *ctx, void *d_ptr) {
+    CTX_OK(ctx);
+    if (d_ptr) SALSAA_HIP(hipFree(d_ptr));
+    return SALSAA_OK;
+}
+
+int salsaa_memcpy_h2d(salsaa_ctx *ctx, void *d_dst, const void *h_src, size_t bytes) {
+    CTX_OK(ctx);
+    SALSAA_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+    SALSAA_HIP(hipStreamSynchronize(ctx->c.stream));
+    return SALSAA_OK;
+}
+
+int salsaa_memcpy_d2h(salsaa_ctx *ctx, void *h_dst, const void *d_src, size_t bytes) {
+    CTX_OK(ctx);
+    SALSAA_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+    SALSAA_HIP(hipStreamSynchronize(ctx->c.stream));
+    return SALSAA_OK;
+}
+
+int salsaa_event_create(void **ev) {
+    SALSAA_CHECK(ev != nullptr, "null output pointer");
+    hipEvent_t e;
+    SALSAA_HIP(hipEventCreate(&e));
+    *ev = (void *)e;
+    return SALSAA_OK;
+}
+
+int salsaa_event_destroy(void *ev) {
+    if (ev) SALSAA_HIP(hipEventDestroy((hipEvent_t)ev));
+    return SALSAA_OK;
+}
+
+int salsaa_event_record(salsaa_ctx *ctx, void *ev) {
+    CTX_OK(ctx);
+    SALSAA_HIP(hipEventRecord((hipEvent_t)ev, ctx->c.stream));
+    return SALSAA_OK;
+}
+
+int salsaa_event_elapsed_ms(void *start, void *stop, float *ms) {
+    SALSAA_HIP(hipEventSynchronize((hipEvent_t)stop));
+    SALSAA_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return SALSAA_OK;
+}
+
+int salsaa_sample_uniform(salsaa_ctx *ctx, uint64_t *d_out, size_t n, uint64_t seed,
+                          uint64_t stream) {
+    CTX_OK(ctx);
+    return launch_sample_uniform(&ctx->c, d_out, n, seed, stream);
+}
+
+int salsaa_sample_bounded(salsaa_ctx *ctx, uint64_t *d_out, size_t n, uint64_t bound,
+                          uint64_t seed, uint64_t stream) {
+    CTX_OK(ctx);
+    return launch_sample_bounded(&ctx->c, d_out, n, bound, seed, stream);
+}
+
+int salsaa_sample_challenges(salsaa_ctx *ctx, uint64_t *d_cs, size_t reps, size_t nrounds,
+                             int fq2, uint64_t seed) {
+    CTX_OK(ctx);
+    return launch_challenges(&ctx->c, d_cs, reps, nrounds, fq2, seed);
+}
+
+int salsaa_convert(salsaa_ctx *ctx, uint64_t *d_data, size_t count, int from_rep, int to_rep) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(from_rep >= 0 && from_rep <= 2 && to_rep >= 0 && to_rep <= 2,
+                 "unknown representation");
+    if (from_rep == to_rep) return SALSAA_OK;
+    // NTT <-> IncompleteNTT goes through Coefficient, as the reference does
+    // (cyclotomic_ring.rs:488-490, 519-521)
+    if (from_rep != SALSAA_REP_COEFF)
+        SALSAA_TRY(launch_ntt(&ctx->c, d_data, count, from_rep, 1));
+    if (to_rep != SALSAA_REP_COEFF) SALSAA_TRY(launch_ntt(&ctx->c, d_data, count, to_rep, 0));
+    return SALSAA_OK;
+}
+
+int salsaa_ring_mul(salsaa_ctx *ctx, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out,
+                    size_t count) {
+    CTX_OK(ctx);
+    return launch_ring_mul(&ctx->c, d_a, d_b, d_out, count);
+}
+
+int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t m,
+                      salsaa_key **out) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(out != nullptr, "null output pointer");
+    SALSAA_CHECK(kappa >= 1 && m >= 1, "key needs kappa, m >= 1");
+    salsaa_key *k = new (std::nothrow) salsaa_key();
+    if (!k) return SALSAA_ERR_NOMEM;
+    k->k.ctx = &ctx->c;
+    k->k.kappa = kappa;
+    k->k.m = m;
+    hipError_t e = hipMalloc(&k->k.d_A, (size_t)kD * kappa * m * sizeof(double));
+    if (e != hipSuccess) {
+        delete k;
+        set_error(std::string("key allocation: ") + hipGetErrorString(e));
+        return SALSAA_ERR_NOMEM;
+    }
+    int s = launch_ntt_full_to_slotmajor(&ctx->c, d_A, kappa, m, kappa, m * kD, 0, k->k.d_A);
+    if (s != SALSAA_OK) {
+        (void)hipFree(k->k.d_A);
+        delete k;
+        return s;
+    }
+    *out = k;
+    return SALSAA_OK;
+}
+
+int salsaa_key_destroy(salsaa_key *key) {
+    if (!key) return SALSAA_OK;
+    (void)hipStreamSynchronize(key->k.ctx->stream);
+    (void)hipFree(key->k.d_A);
+    delete key;
+    return SALSAA_OK;
+}
+
+static int commit_rows(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_W, size_t rows,
+                       size_t m, size_t row_lo, size_t stride_lo, size_t stride_hi,
+                       uint64_t *d_out) {
+    SALSAA_CHECK(key != nullptr, "null key");
+    SALSAA_CHECK(key->k.ctx == &ctx->c, "key belongs to another context");
+    SALSAA_CHECK(m >= 1 && m <= key->k.m, "witness longer than the key");
+    SALSAA_CHECK(rows >= 1, "commit needs at least one witness row");
+    double *w_sm = nullptr;
+    SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * rows * m * sizeof(double), (void **)&w_sm));
+    SALSAA_TRY(launch_ntt_full_to_slotmajor(&ctx->c, d_W, rows, m, row_lo, stride_lo, stride_hi,
+                                            w_sm));
+    return launch_commit_mac(&ctx->c, key->k.d_A, key->k.kappa, key->k.m, w_sm, rows, m, d_out);
+}
+
+int salsaa_parallel_dot_series_matrix(salsaa_ctx *ctx, const salsaa_key *key,
+                                      const uint64_t *d_W, size_t r, size_t m, uint64_t *d_out) {
+    CTX_OK(ctx);
+    return commit_rows(ctx, key, d_W, r, m, r, m * kD, 0, d_out);
+}
+
+int salsaa_split(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_W, size_t r, size_t m,
+                 uint64_t *d_out) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(m >= 2 && m % 2 == 0, "split needs an even witness length");
+    // row k < r is L_k = W[k][0 .. m/2), row r + k is R_k = W[k][m/2 .. m)
+    return commit_rows(ctx, key, d_W, 2 * r, m / 2, r, m * kD, (m / 2) * kD, d_out);
+}
+
+int salsaa_norm_check(salsaa_ctx *ctx, const uint64_t *d_w, size_t rows, size_t L,
+                      const uint64_t *d_cs, size_t reps, int fq2, uint64_t nr, uint64_t *d_claims,
+                      uint64_t *d_msgs, uint64_t *d_finals, uint64_t *d_evals, uint64_t *d_rhs,
+                      int32_t *d_ok) {
+    CTX_OK(ctx);
+    return launch_norm_check(&ctx->c, d_w, rows, L, d_cs, reps, fq2, nr, d_claims, d_msgs,
+                             d_finals, d_evals, d_rhs, d_ok);
+}
+
+int salsaa_decompose(salsaa_ctx *ctx, const uint64_t *d_W, size_t r, size_t m, size_t parts,
+                     uint64_t radix, uint64_t *d_out, uint64_t *d_radix) {
+    CTX_OK(ctx);
+    return launch_decompose(&ctx->c, d_W, r, m, parts, radix, d_out, d_radix);
+}
+
+int salsaa_decomp_ell(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_W, size_t r,
+                      size_t m, size_t parts, uint64_t radix, uint64_t *d_new_W, uint64_t *d_rhs,
+                      uint64_t *d_radix) {
+    CTX_OK(ctx);
+    SALSAA_TRY(launch_decompose(&ctx->c, d_W, r, m, parts, radix, d_new_W, d_radix));
+    return commit_rows(ctx, key, d_new_W, r * parts, m, r * parts, m * kD, 0, d_rhs);
+}
+
+int salsaa_compose(salsaa_ctx *ctx, const uint64_t *d_in, size_t rows, size_t n, size_t parts,
+                   const uint64_t *d_radix, uint64_t *d_out) {
+    CTX_OK(ctx);
+    return launch_compose(&ctx->c, d_in, rows, n, parts, d_radix, d_out);
+}
+
+int salsaa_verify_decomp(salsaa_ctx *ctx, const uint64_t *d_rhs_dec, const uint64_t *d_rhs,
+                         size_t kappa, size_t r, size_t parts, const uint64_t *d_radix,
+                         int rows_mode, uint32_t *d_mismatch) {
+    CTX_OK(ctx);
+    return launch_compose_check(&ctx->c, d_rhs_dec, kappa, r, parts, d_radix, d_rhs, rows_mode,
+                                d_mismatch);
+}
+
+int salsaa_parallel_dot_matrix_matrix(salsaa_ctx *ctx, const uint64_t *d_W, size_t r_in,
+                                      size_t m, const uint64_t *d_C, size_t r_out,
+                                      uint64_t *d_out) {
+    CTX_OK(ctx);
+    return launch_fold(&ctx->c, d_W, r_in, m, d_C, r_out, d_out);
+}
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+// common.hpp -- shared definitions for the SALSAA MI355X library: status
+// codes, the context, exact fp64 modular arithmetic (q < 2^50) and the seeded
+// counter-based sampler used by both host and device code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/salsaa_amd.h"
+
+namespace salsaa {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+
+#define SALSAA_HIP(expr)                                                        \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            ::salsaa::set_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+            return SALSAA_ERR_HIP;                                              \
+        }                                                                       \
+    } while (0)
+
+#define SALSAA_CHECK(cond, msg)                                                 \
+    do {                                                                        \
+        if (!(cond)) {                                                          \
+            ::salsaa::set_error(msg);                                           \
+            return SALSAA_ERR_ARG;                                              \
+        }                                                                       \
+    } while (0)
+
+#define SALSAA_TRY(expr)                                                        \
+    do {                                                                        \
+        int s_ = (expr);                                                        \
+        if (s_ != SALSAA_OK) return s_;                                         \
+    } while (0)
+
+// ---------------------------------------------------------------- constants
+constexpr int kD = 64;      // ring degree supported by the device kernels
+constexpr int kH = kD / 2;  // slots of the IncompleteNTT representation
+
+// Device table layout (doubles) inside ctx->d_tables.
+enum TableOffset : int {
+    TAB_FWD32 = 0,    // [32]  psi_64 powers, bit-reversed (HEXL roots, n = 32)
+    TAB_INV32 = 32,   // [32]  their inverses; [32+1] pre-scaled by 32^{-1}
+    TAB_FWD64 = 64,   // [64]  psi_128 powers, bit-reversed (n = 64)
+    TAB_INV64 = 128,  // [64]  inverses; [128+1] pre-scaled by 64^{-1}
+    TAB_OMEGA = 192,  // [32]  omega_j = NTT_32(Y)[j] (slot moduli X^2 - omega_j)
+    TAB_NINV32 = 224, // 32^{-1}
+    TAB_NINV64 = 225, // 64^{-1}
+    TAB_SIZE = 256,
+};
+
+// ---------------------------------------------------------------- context
+struct ScratchBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct Context {
+    int device = 0;
+    uint64_t q = 0;
+    int d = kD;
+    double qd = 0.0, qinv = 0.0;
+    hipStream_t stream = nullptr;
+    double *d_tables = nullptr;     // TAB_SIZE doubles
+    std::vector<double> h_tables;   // host copy
+    ScratchBuf scratch[8];          // grown on demand, owned by the context
+    std::mutex lock;                // one host thread drives a context at a time
+};
+
+struct Key {
+    Context *ctx = nullptr;
+    size_t kappa = 0, m = 0;
+    double *d_A = nullptr;  // slot-major full-NTT [64][kappa][m] as exact fp64
+};
+
+int scratch_get(Context *ctx, int slot, size_t bytes, void **out);
+
+// ---------------------------------------------------------------- host arithmetic
+inline uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t q) {
+    return (uint64_t)((unsigned __int128)a * b % q);
+}
+inline uint64_t h_powmod(uint64_t a, uint64_t e, uint64_t q) {
+    uint64_t r = 1 % q;
+    a %= q;
+    while (e) {
+        if (e & 1) r = h_mulmod(r, a, q);
+        a = h_mulmod(a, a, q);
+        e >>= 1;
+    }
+    return r;
+}
+inline uint64_t h_invmod(uint64_t a, uint64_t q) { return h_powmod(a, q - 2, q); }
+
+// ---------------------------------------------------------------- sampler
+// Counter-based SplitMix64: element `idx` of stream `stream` under `seed`.
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t stream_key(uint64_t seed, uint64_t stream) {
+    return mix64(seed ^ (stream * 0xD1B54A32D192ED03ULL));
+}
+__host__ __device__ inline uint64_t draw(uint64_t key, uint64_t idx) {
+    return mix64(key + (idx + 1) * 0x9E3779B97F4A7C15ULL);
+}
+__host__ __device__ inline uint64_t mulhi64(uint64_t a, uint64_t b) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+// ---------------------------------------------------------------- fp64 modular arithmetic
+// Residues live in registers as exact integer-valued doubles in [0, q).
+// For a, b < q < 2^50: hi = fl(a*b), lo = a*b - hi exactly (fma), the quotient
+// estimate rint(hi/q) is within 0.875 of a*b/q, so r = (hi - qt*q) + lo is an
+// exact integer in (-q, q) (every intermediate is an integer below 2^52).
+struct Mod {
+    double q, qinv;
+};
+
+__device__ __forceinline__ double u2d(uint64_t x) {
+    // exact for x < 2^52: the double 2^52 + x has mantissa x
+    return __longlong_as_double((long long)(x | 0x4330000000000000ULL)) - 4503599627370496.0;
+}
+__device__ __forceinline__ uint64_t d2u(double v) {
+    return (uint64_t)__double_as_longlong(v + 4503599627370496.0) & 0x000FFFFFFFFFFFFFULL;
+}
+
+__device__ __forceinline__ double mul_lazy(double a, double b, Mod M) {
+    double hi = a * b;
+    double qt = __builtin_rint(hi * M.qinv);
+    double lo = __fma_rn(a, b, -hi);
+    return __fma_rn(-qt, M.q, hi) + lo;  // in (-q, q)
+}
+__device__ __forceinline__ double mulmod(double a, double b, Mod M) {
+    double r = mul_lazy(a, b, M);
+    return r < 0.0 ? r + M.q : r;
+}
+__device__ __forceinline__ double addmod(double a, double b, Mod M) {
+    double s = a + b;
+    return s >= M.q ? s - M.q : s;
+}
+__device__ __forceinline__ double submod(double a, double b, Mod M) {
+    double s = a - b;
+    return s < 0.0 ? s + M.q : s;
+}
+// Reduce an integer-valued |x| < 2^53 to [0, q).
+__device__ __forceinline__ double reduce(double x, Mod M) {
+    double t = __fma_rn(-__builtin_rint(x * M.qinv), M.q, x);
+    return t < 0.0 ? t + M.q : t;
+}
+// Reduce to (-q/2 - 1, q/2 + 1) without the final correction (lazy).
+__device__ __forceinline__ double reduce_lazy(double x, Mod M) {
+    return __fma_rn(-__builtin_rint(x * M.qinv), M.q, x);
+}
+
+// F_{q^2} = F_q[Y]/(Y^2 - nr), element a + bY
+struct F2 {
+    double a, b;
+};
+__device__ __forceinline__ F2 f2_add(F2 x, F2 y, Mod M) { return {addmod(x.a, y.a, M), addmod(x.b, y.b, M)}; }
+__device__ __forceinline__ F2 f2_sub(F2 x, F2 y, Mod M) { return {submod(x.a, y.a, M), submod(x.b, y.b, M)}; }
+__device__ __forceinline__ F2 f2_mul(F2 x, F2 y, double nr, Mod M) {
+    double bd = mulmod(x.b, y.b, M);
+    double re = reduce(mul_lazy(x.a, y.a, M) + mul_lazy(nr, bd, M), M);
+    double im = reduce(mul_lazy(x.a, y.b, M) + mul_lazy(x.b, y.a, M), M);
+    return {re, im};
+}
+
+// ---------------------------------------------------------------- launch helpers
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// entry points implemented in the kernel files
+int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse);
+int launch_ntt_full_to_slotmajor(Context *ctx, const uint64_t *src, size_t rows, size_t m,
+                                 size_t row_lo, size_t stride_lo, size_t stride_hi,
+                                 double *dst);
+int launch_commit_mac(Context *ctx, const double *A_sm, size_t kappa, size_t mA,
+                      const double *W_sm, size_t r, size_t m, uint64_t *out);
+int launch_sample_uniform(Context *ctx, uint64_t *out, size_t n, uint64_t seed, uint64_t stream);
+int launch_sample_bounded(Context *ctx, uint64_t *out, size_t n, uint64_t bound, uint64_t seed,
+                          uint64_t stream);
+int launch_challenges(Context *ctx, uint64_t *cs, size_t reps, size_t nrounds, int fq2,
+                      uint64_t seed);
+int launch_norm_check(Context *ctx, const uint64_t *w, size_t rows, size_t L,
+                      const uint64_t *cs, size_t reps, int fq2, uint64_t nr, uint64_t *claims,
+                      uint64_t *msgs, uint64_t *finals, uint64_t *evals, uint64_t *rhs,
+                      int32_t *ok);
+int launch_decompose(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
+                     uint64_t radix, uint64_t *out, uint64_t *d_radix);
+int launch_compose_check(Context *ctx, const uint64_t *in, size_t rows, size_t n, size_t parts,
+                         const uint64_t *d_radix, const uint64_t *expect, int rows_mode,
+                         uint32_t *d_mismatch);
+int launch_compose(Context *ctx, const uint64_t *in, size_t rows, size_t n, size_t parts,
+                   const uint64_t *d_radix, uint64_t *out);
+int launch_ring_mul(Context *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out,
+                    size_t count);
+int launch_fold(Context *ctx, const uint64_t *W, size_t r_in, size_t m, const uint64_t *C,
+                size_t r_out, uint64_t *out);
+
+}  // namespace salsaa

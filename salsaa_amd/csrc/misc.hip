@@ -1,0 +1,304 @@
+// misc.hip -- seeded samplers, decomposition (decomp_ell), recomposition
+// checks (verify_decomp), batched ring products and the fold kernel.
+#include "ntt_device.hpp"
+
+namespace salsaa {
+
+namespace {
+
+__global__ void k_sample_uniform(uint64_t *__restrict__ out, size_t n, uint64_t key, uint64_t q) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        out[i] = mulhi64(draw(key, i), q);
+}
+
+__global__ void k_sample_bounded(uint64_t *__restrict__ out, size_t n, uint64_t key,
+                                 uint64_t bound, uint64_t q) {
+    const uint64_t span = 2 * bound + 1;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t t = mulhi64(draw(key, i), span);
+        out[i] = t >= bound ? t - bound : q - (bound - t);
+    }
+}
+
+// cs[rep][round][2]: coordinate k of round t from stream 0x5C000 + rep, index 2t + k
+__global__ void k_challenges(uint64_t *__restrict__ cs, size_t reps, size_t nrounds, int fq2,
+                             uint64_t seed, uint64_t q) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= reps * nrounds * 2) return;
+    const size_t rep = i / (nrounds * 2), rem = i % (nrounds * 2);
+    const size_t t = rem / 2, k = rem % 2;
+    uint64_t v = 0;
+    if (k == 0 || fq2) v = mulhi64(draw(stream_key(seed, 0x5C000ULL + rep), 2 * t + k), q);
+    cs[i] = v;
+}
+
+// ---- decomposition (arithmetic.rs:1461-1593)
+__global__ void k_bitor(const uint64_t *__restrict__ W, size_t n, uint64_t q,
+                        unsigned long long *__restrict__ acc) {
+    uint64_t a = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t x = W[i];
+        a |= (2 * x > q) ? q - x : x;
+    }
+    for (int off = 32; off >= 1; off >>= 1) a |= __shfl_xor(a, off, 64);
+    if ((threadIdx.x & 63) == 0 && a) atomicOr(acc, (unsigned long long)a);
+}
+
+__global__ void k_zero1(unsigned long long *p) { *p = 0ULL; }
+
+// gadget weight 2^(lg * t) mod q; 0 once the shift leaves 64 bits, as the
+// reference's wrapping 2u64.pow does (arithmetic.rs:1739)
+__device__ __forceinline__ double gadget_weight(unsigned lg, size_t t, uint64_t q) {
+    const size_t s = (size_t)lg * t;
+    return s >= 64 ? 0.0 : u2d((1ULL << s) % q);
+}
+
+// radix = 2^ceil(bw / parts) from the OR accumulator (or the explicit radix)
+__global__ void k_set_radix(unsigned long long *__restrict__ acc, uint64_t explicit_radix,
+                            size_t parts, uint64_t *__restrict__ d_radix) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (explicit_radix) {
+        *d_radix = explicit_radix;
+        return;
+    }
+    const uint64_t a = *acc;
+    const unsigned bw = a ? 64u - (unsigned)__clzll((long long)a) : 0u;
+    const unsigned lg = (unsigned)((bw + parts - 1) / parts);
+    *d_radix = 1ULL << lg;
+}
+
+__global__ void k_decompose(const uint64_t *__restrict__ W, size_t r, size_t md, size_t parts,
+                            const uint64_t *__restrict__ d_radix, uint64_t q,
+                            uint64_t *__restrict__ out) {
+    const uint64_t radix = *d_radix;
+    const unsigned lg = (unsigned)__ffsll((long long)radix) - 1u;
+    const uint64_t mask = radix - 1;
+    const size_t n = r * md;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t x = W[idx];
+        const bool neg = 2 * x > q;
+        const uint64_t v = neg ? q - x : x;
+        const size_t rho = idx / md, rem = idx % md;
+        uint64_t *o = out + rho * parts * md + rem;
+        for (size_t t = 0; t < parts; t++) {
+            const unsigned s = (unsigned)(lg * t);
+            const uint64_t dg = (lg == 0) ? (v & mask) : (s >= 64 ? 0 : (v >> s) & mask);
+            o[t * md] = neg ? (dg ? q - dg : 0) : dg;
+        }
+    }
+}
+
+// compose_with_radix_mod: in [rows][n*parts][d] -> out [rows][n][d]
+__global__ void k_compose(const uint64_t *__restrict__ in, size_t rows, size_t n, size_t parts,
+                          const uint64_t *__restrict__ d_radix, uint64_t *__restrict__ out,
+                          uint64_t q, Mod M) {
+    const uint64_t radix = *d_radix;
+    const unsigned lg = (unsigned)__ffsll((long long)radix) - 1u;
+    const size_t total = rows * n * kD;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = idx % kD, ri = idx / kD;
+        const size_t row = ri / n, i = ri % n;
+        double acc = 0.0;
+        for (size_t t = 0; t < parts; t++) {
+            const double x = u2d(in[((row * n * parts) + i * parts + t) * kD + c]);
+            acc = addmod(acc, mulmod(x, gadget_weight(lg, t, q), M), M);
+        }
+        out[idx] = d2u(acc);
+    }
+}
+
+// rows_mode 0: compare compose(in [rows][n*parts]) (columns grouped) with
+// expect [rows][n]; rows_mode 1: in = limbs [n_rows*parts][n][d] grouped by
+// rows, expect [n_rows][n][d].
+__global__ void k_compose_check(const uint64_t *__restrict__ in, size_t rows, size_t n,
+                                size_t parts, const uint64_t *__restrict__ d_radix,
+                                const uint64_t *__restrict__ expect, int rows_mode,
+                                uint32_t *__restrict__ mismatch, uint64_t q, Mod M) {
+    const uint64_t radix = *d_radix;
+    const unsigned lg = (unsigned)__ffsll((long long)radix) - 1u;
+    const size_t total = rows * n * kD;
+    uint32_t bad = 0;
+    for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = idx % kD, ri = idx / kD;
+        const size_t row = ri / n, i = ri % n;
+        double acc = 0.0;
+        for (size_t t = 0; t < parts; t++) {
+            const size_t src = rows_mode ? (((row * parts + t) * n + i) * kD + c)
+                                         : (((row * n * parts) + i * parts + t) * kD + c);
+            acc = addmod(acc, mulmod(u2d(in[src]), gadget_weight(lg, t, q), M), M);
+        }
+        if (d2u(acc) != expect[idx]) bad++;
+    }
+    if (bad) atomicAdd(mismatch, bad);
+}
+
+// ---- IncompleteNTT slot products
+// out[e] = a[e] * b[e], all IncompleteNTT; thread per (element, slot)
+__global__ void k_slot_mul(const uint64_t *__restrict__ a, const uint64_t *__restrict__ b,
+                           uint64_t *__restrict__ out, size_t count,
+                           const double *__restrict__ tab, Mod M) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= count * kH) return;
+    const size_t e = gid / kH;
+    const int j = (int)(gid % kH);
+    const uint64_t *ae = a + e * kD, *be = b + e * kD;
+    const double a0 = u2d(ae[j]), a1 = u2d(ae[kH + j]);
+    const double b0 = u2d(be[j]), b1 = u2d(be[kH + j]);
+    const double om = tab[TAB_OMEGA + j];
+    const double re = reduce(mul_lazy(a0, b0, M) + mul_lazy(mulmod(a1, b1, M), om, M), M);
+    const double ro = reduce(mul_lazy(a1, b0, M) + mul_lazy(a0, b1, M), M);
+    out[e * kD + j] = d2u(re);
+    out[e * kD + kH + j] = d2u(ro);
+}
+
+// fold MAC in the IncompleteNTT domain:
+// out[j][i] = sum_k Wn[k][i] * Cn[j][k]; thread per (j, i, slot)
+__global__ void k_fold_mac(const uint64_t *__restrict__ Wn, size_t r_in, size_t m,
+                           const uint64_t *__restrict__ Cn, size_t r_out,
+                           uint64_t *__restrict__ out, const double *__restrict__ tab, Mod M) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= r_out * m * kH) return;
+    const int s = (int)(gid % kH);
+    const size_t ji = gid / kH;
+    const size_t j = ji / m, i = ji % m;
+    const double om = tab[TAB_OMEGA + s];
+    double re = 0.0, ro = 0.0;
+    for (size_t k = 0; k < r_in; k++) {
+        const uint64_t *we = Wn + (k * m + i) * kD;
+        const uint64_t *ce = Cn + (j * r_in + k) * kD;
+        const double w0 = u2d(we[s]), w1 = u2d(we[kH + s]);
+        const double c0 = u2d(ce[s]), c1 = u2d(ce[kH + s]);
+        re += mul_lazy(w0, c0, M) + mul_lazy(mulmod(w1, c1, M), om, M);
+        ro += mul_lazy(w1, c0, M) + mul_lazy(w0, c1, M);
+        re = reduce_lazy(re, M);
+        ro = reduce_lazy(ro, M);
+    }
+    uint64_t *o = out + (j * m + i) * kD;
+    o[s] = d2u(reduce(re, M));
+    o[kH + s] = d2u(reduce(ro, M));
+}
+
+unsigned grid_for(size_t n, unsigned block) {
+    size_t g = (n + block - 1) / block;
+    if (g > 65536) g = 65536;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+int launch_sample_uniform(Context *ctx, uint64_t *out, size_t n, uint64_t seed, uint64_t stream) {
+    if (!n) return SALSAA_OK;
+    hipLaunchKernelGGL(k_sample_uniform, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, out, n,
+                       stream_key(seed, stream), ctx->q);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_sample_bounded(Context *ctx, uint64_t *out, size_t n, uint64_t bound, uint64_t seed,
+                          uint64_t stream) {
+    if (!n) return SALSAA_OK;
+    SALSAA_CHECK(bound < ctx->q / 2, "bound must be below q/2");
+    hipLaunchKernelGGL(k_sample_bounded, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, out, n,
+                       stream_key(seed, stream), bound, ctx->q);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_challenges(Context *ctx, uint64_t *cs, size_t reps, size_t nrounds, int fq2,
+                      uint64_t seed) {
+    const size_t n = reps * nrounds * 2;
+    if (!n) return SALSAA_OK;
+    hipLaunchKernelGGL(k_challenges, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, cs, reps,
+                       nrounds, fq2, seed, ctx->q);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_decompose(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
+                     uint64_t radix, uint64_t *out, uint64_t *d_radix) {
+    SALSAA_CHECK(parts >= 1 && parts <= 64, "parts must be in [1, 64]");
+    SALSAA_CHECK(radix == 0 || (radix & (radix - 1)) == 0, "radix must be a power of two");
+    const size_t md = m * kD, n = r * md;
+    unsigned long long *acc = nullptr;
+    SALSAA_TRY(scratch_get(ctx, 3, sizeof(unsigned long long), (void **)&acc));
+    if (radix == 0) {
+        // the OR accumulator starts at zero (kernel write, safe under capture)
+        hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, acc);
+        hipLaunchKernelGGL(k_bitor, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, W, n, ctx->q,
+                           acc);
+    }
+    hipLaunchKernelGGL(k_set_radix, dim3(1), dim3(1), 0, ctx->stream, acc, radix, parts, d_radix);
+    hipLaunchKernelGGL(k_decompose, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, W, r, md,
+                       parts, d_radix, ctx->q, out);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_compose(Context *ctx, const uint64_t *in, size_t rows, size_t n, size_t parts,
+                   const uint64_t *d_radix, uint64_t *out) {
+    Mod M{ctx->qd, ctx->qinv};
+    const size_t total = rows * n * kD;
+    if (!total) return SALSAA_OK;
+    hipLaunchKernelGGL(k_compose, dim3(grid_for(total, 256)), dim3(256), 0, ctx->stream, in, rows, n,
+                       parts, d_radix, out, ctx->q, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_compose_check(Context *ctx, const uint64_t *in, size_t rows, size_t n, size_t parts,
+                         const uint64_t *d_radix, const uint64_t *expect, int rows_mode,
+                         uint32_t *d_mismatch) {
+    Mod M{ctx->qd, ctx->qinv};
+    const size_t total = rows * n * kD;
+    if (!total) return SALSAA_OK;
+    hipLaunchKernelGGL(k_compose_check, dim3(grid_for(total, 256)), dim3(256), 0, ctx->stream, in,
+                       rows, n, parts, d_radix, expect, rows_mode, d_mismatch, ctx->q, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_ring_mul(Context *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out,
+                    size_t count) {
+    if (!count) return SALSAA_OK;
+    Mod M{ctx->qd, ctx->qinv};
+    uint64_t *ta = nullptr;
+    SALSAA_TRY(scratch_get(ctx, 4, 2 * count * kD * sizeof(uint64_t), (void **)&ta));
+    uint64_t *tb = ta + count * kD;
+    SALSAA_HIP(hipMemcpyAsync(ta, a, count * kD * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    SALSAA_HIP(hipMemcpyAsync(tb, b, count * kD * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    SALSAA_TRY(launch_ntt(ctx, ta, count, SALSAA_REP_INCOMPLETE, 0));
+    SALSAA_TRY(launch_ntt(ctx, tb, count, SALSAA_REP_INCOMPLETE, 0));
+    hipLaunchKernelGGL(k_slot_mul, dim3(ceil_div(count * kH, 256)), dim3(256), 0, ctx->stream, ta,
+                       tb, out, count, ctx->d_tables, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_fold(Context *ctx, const uint64_t *W, size_t r_in, size_t m, const uint64_t *C,
+                size_t r_out, uint64_t *out) {
+    if (!r_out || !m) return SALSAA_OK;
+    SALSAA_CHECK(r_in >= 1, "fold needs r_in >= 1");
+    Mod M{ctx->qd, ctx->qinv};
+    uint64_t *wn = nullptr;
+    const size_t nw = r_in * m, nc = r_out * r_in;
+    SALSAA_TRY(scratch_get(ctx, 5, (nw + nc) * kD * sizeof(uint64_t), (void **)&wn));
+    uint64_t *cn = wn + nw * kD;
+    SALSAA_HIP(hipMemcpyAsync(wn, W, nw * kD * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    SALSAA_HIP(hipMemcpyAsync(cn, C, nc * kD * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    SALSAA_TRY(launch_ntt(ctx, wn, nw, SALSAA_REP_INCOMPLETE, 0));
+    SALSAA_TRY(launch_ntt(ctx, cn, nc, SALSAA_REP_INCOMPLETE, 0));
+    const size_t threads = r_out * m * kH;
+    hipLaunchKernelGGL(k_fold_mac, dim3(ceil_div(threads, 256)), dim3(256), 0, ctx->stream, wn, r_in,
+                       m, cn, r_out, out, ctx->d_tables, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+}  // namespace salsaa

@@ -1,0 +1,263 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle: bit-exact
+on seeded inputs at sizes the oracle finishes in seconds, and size-independent
+properties at the configs' full sizes."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import NR, Q
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def sa():
+    import salsaa_amd
+    return salsaa_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(sa):
+    c = sa.Context(Q, 64)
+    yield c
+    c.close()
+
+
+def rand_ring(orc, shape, seed, stream, bound=None):
+    if bound is None:
+        return orc.sample_uniform(seed, stream, shape, Q)
+    return orc.sample_bounded(seed, stream, shape, bound, Q)
+
+
+# ------------------------------------------------------------------ sampling
+def test_samplers_match_oracle(ctx, orc):
+    say("case: samplers")
+    u = ctx.uniform((300, 64), 7, 11).download()
+    assert np.array_equal(u, orc.sample_uniform(7, 11, (300, 64), Q))
+    b = ctx.bounded((5000,), 16, 3, 12).download()
+    assert np.array_equal(b, orc.sample_bounded(3, 12, (5000,), 16, Q))
+    for fq2 in (0, 1):
+        cs = ctx.challenges(3, 9, fq2, 42).download()
+        for rep in range(3):
+            want = orc.sample_uniform(42, 0x5C000 + rep, (18,), Q).reshape(9, 2)
+            if not fq2:
+                want[:, 1] = 0
+            assert np.array_equal(cs[rep], want)
+
+
+# ------------------------------------------------------------------ NTT
+@pytest.mark.parametrize("count", [1, 127, 128, 1000])
+def test_ntt_incomplete_and_full_vs_oracle(sa, ctx, orc, count):
+    say("case: ntt count=%d" % count)
+    x = rand_ring(orc, (count, 64), SEED, 3)
+    dx = ctx.to_device(x)
+    ctx.convert(dx, sa.REP_COEFF, sa.REP_INCOMPLETE)
+    inc = dx.download()
+    assert np.array_equal(inc, orc.to_incomplete(x, Q))
+    ctx.convert(dx, sa.REP_INCOMPLETE, sa.REP_FULL)  # via Coefficient
+    full = dx.download()
+    assert np.array_equal(full, orc.to_full(x, Q))
+    ctx.convert(dx, sa.REP_FULL, sa.REP_COEFF)
+    assert np.array_equal(dx.download(), x)
+
+
+def test_ntt_edge_values(sa, ctx, orc):
+    say("case: ntt edge values")
+    x = np.zeros((8, 64), np.uint64)
+    x[1] = Q - 1
+    x[2, 0] = 1
+    x[3, 63] = Q - 1
+    x[4] = np.arange(64, dtype=np.uint64)
+    x[5] = Q - 1 - np.arange(64, dtype=np.uint64)
+    x[6, ::2] = Q - 1
+    x[7, 1::2] = (Q - 1) // 2
+    dx = ctx.to_device(x)
+    ctx.convert(dx, sa.REP_COEFF, sa.REP_INCOMPLETE)
+    assert np.array_equal(dx.download(), orc.to_incomplete(x, Q))
+    ctx.convert(dx, sa.REP_INCOMPLETE, sa.REP_COEFF)
+    assert np.array_equal(dx.download(), x)
+
+
+def test_ring_mul_vs_oracle(ctx, orc):
+    say("case: ring_mul")
+    a = rand_ring(orc, (200, 64), 1, 1)
+    b = rand_ring(orc, (200, 64), 1, 2)
+    out = ctx.ring_mul(ctx.to_device(a), ctx.to_device(b)).download()
+    for i in range(0, 200, 13):
+        assert np.array_equal(out[i], orc.ring_mul_incomplete(a[i], b[i], Q))
+
+
+# ------------------------------------------------------------------ commit
+@pytest.mark.parametrize("kappa,r,m,bound", [(4, 3, 40, None), (16, 16, 256, 8), (8, 1, 1024, 16),
+                                             (32, 8, 64, 16), (5, 9, 300, 4)])
+def test_commit_vs_oracle(ctx, orc, kappa, r, m, bound):
+    say("case: commit kappa=%d r=%d m=%d" % (kappa, r, m))
+    A = rand_ring(orc, (kappa, m, 64), SEED, 0xA)
+    W = rand_ring(orc, (r, m, 64), SEED, 0xB, bound)
+    key = ctx.key(ctx.to_device(A))
+    got = ctx.parallel_dot_series_matrix(key, ctx.to_device(W)).download()
+    want = orc.commit(A, W, Q, threads=8)
+    assert np.array_equal(got, want)
+    # a shorter witness uses the key's prefix layer
+    m2 = m // 2
+    got2 = ctx.parallel_dot_series_matrix(key, ctx.to_device(np.ascontiguousarray(W[:, :m2]))).download()
+    want2 = orc.commit(np.ascontiguousarray(A[:, :m2]), np.ascontiguousarray(W[:, :m2]), Q, 8)
+    assert np.array_equal(got2, want2)
+    key.close()
+
+
+def test_split_vs_oracle(ctx, orc):
+    say("case: split")
+    kappa, r, m = 6, 3, 128
+    A = rand_ring(orc, (kappa, m, 64), 5, 0xA)
+    W = rand_ring(orc, (r, m, 64), 5, 0xB, 16)
+    key = ctx.key(ctx.to_device(A))
+    got = ctx.split(key, ctx.to_device(W)).download()
+    halves = np.concatenate([W[:, : m // 2], W[:, m // 2:]], axis=0)
+    want = orc.commit(np.ascontiguousarray(A[:, : m // 2]), halves, Q, 8)
+    assert np.array_equal(got, want)
+    key.close()
+
+
+# ------------------------------------------------------------------ norm check
+def _oracle_cs(reps, nrounds, fq2, seed):
+    from oracle import oracle as orc
+    cs = np.stack([orc.sample_uniform(seed, 0x5C000 + rep, (nrounds * 2,), Q).reshape(nrounds, 2)
+                   for rep in range(reps)])
+    if not fq2:
+        cs[:, :, 1] = 0
+    return cs
+
+
+@pytest.mark.parametrize("fq2", [0, 1])
+@pytest.mark.parametrize("rows,L,reps", [(2, 64, 1), (3, 1024, 2), (2, 2048, 2), (3, 1 << 14, 3),
+                                         (1, 1 << 16, 2), (2, 1 << 13, 1)])
+def test_norm_check_vs_oracle(ctx, orc, fq2, rows, L, reps):
+    say("case: norm_check fq2=%d rows=%d L=%d reps=%d" % (fq2, rows, L, reps))
+    n = L.bit_length() - 1
+    w = rand_ring(orc, (rows, L), 9, 0xB, 16)
+    cs = ctx.challenges(reps, n, fq2, 77)
+    assert np.array_equal(cs.download(), _oracle_cs(reps, n, fq2, 77))
+    outs = ctx.norm_check(ctx.to_device(w), cs, fq2)
+    want = orc.sumcheck(w, cs.download(), Q, fq2=bool(fq2), nr=NR, threads=8)
+    assert np.array_equal(outs["claims"].download(), want["claims"])
+    assert np.array_equal(outs["msgs"].download(), want["msgs"])
+    assert np.array_equal(outs["finals"].download(), want["finals"])
+    assert np.array_equal(outs["evals"].download(), want["evals"])
+    assert outs["ok"].download().all() and want["ok"].all()
+    rhs = outs["rhs"].download()
+    m = L // 64
+    W3 = w.reshape(rows, m, 64)
+    csd = cs.download()
+    for rep in range(reps):
+        if fq2:
+            want_rhs = orc.norm_check_3_fq2(W3, csd[rep], Q, NR)
+            assert np.array_equal(rhs[rep], want_rhs)
+        else:
+            want_rhs = orc.norm_check_3(W3, csd[rep, :, 0], Q)
+            assert np.array_equal(rhs[rep, :, 0], want_rhs)
+            assert not rhs[rep, :, 1].any()
+
+
+def test_norm_check_small_lengths(ctx, orc):
+    for L in (2, 4, 32):
+        say("case: norm_check L=%d (no rhs)" % L)
+        n = L.bit_length() - 1
+        w = rand_ring(orc, (2, L), 1, 2)
+        cs = ctx.challenges(2, n, 1, 5)
+        outs = ctx.norm_check(ctx.to_device(w), cs, 1, want_rhs=False)
+        want = orc.sumcheck(w, cs.download(), Q, fq2=True, nr=NR)
+        assert np.array_equal(outs["msgs"].download(), want["msgs"])
+        assert np.array_equal(outs["finals"].download(), want["finals"])
+        assert outs["ok"].download().all()
+
+
+def test_norm_check_rejects_wrong_claim_path(ctx, orc):
+    # a witness row and its negation have equal claims and messages: the
+    # verifier replay must accept both (sign symmetry of the square)
+    say("case: norm_check sign symmetry")
+    L = 4096
+    w = rand_ring(orc, (1, L), 4, 4)
+    wn = np.where(w == 0, w, np.uint64(Q) - w)
+    cs = ctx.challenges(1, 12, 0, 3)
+    a = ctx.norm_check(ctx.to_device(w), cs, 0)
+    b = ctx.norm_check(ctx.to_device(wn), cs, 0)
+    assert np.array_equal(a["msgs"].download(), b["msgs"].download())
+    assert a["ok"].download().all() and b["ok"].download().all()
+
+
+# ------------------------------------------------------------------ decomposition
+@pytest.mark.parametrize("r,m,parts,bound,radix", [(8, 32, 4, 4096, 0), (8, 16, 6, 16, 256),
+                                                   (2, 8, 2, 3, 0), (3, 5, 3, 1 << 20, 0)])
+def test_decomp_ell_vs_oracle(ctx, orc, r, m, parts, bound, radix):
+    say("case: decomp r=%d m=%d parts=%d radix=%d" % (r, m, parts, radix))
+    kappa = 4
+    A = rand_ring(orc, (kappa, m, 64), 2, 0xA)
+    W = rand_ring(orc, (r, m, 64), 2, 0xB, bound)
+    key = ctx.key(ctx.to_device(A))
+    dW = ctx.to_device(W)
+    new_W, rhs_dec, d_radix = ctx.decomp_ell(key, dW, parts, radix)
+    want_limbs, want_radix = orc.decompose(W, parts, Q, radix)
+    assert int(d_radix.download()[0]) == want_radix
+    assert np.array_equal(new_W.download(), want_limbs)
+    assert np.array_equal(rhs_dec.download(), orc.commit(A, want_limbs, Q, 8))
+    rhs = ctx.parallel_dot_series_matrix(key, dW)
+    mism = ctx.verify_decomp(rhs_dec, rhs, parts, d_radix)
+    assert int(mism.download()[0]) == 0
+    assert np.array_equal(ctx.compose(rhs_dec, parts, d_radix).download(), rhs.download())
+    # witness recomposition (rows mode)
+    mism2 = ctx.verify_decomp(new_W, dW, parts, d_radix, rows_mode=True)
+    assert int(mism2.download()[0]) == 0
+    # a corrupted limb commitment is counted
+    bad = rhs_dec.download()
+    bad[0, 0, 5] = (int(bad[0, 0, 5]) + 1) % Q
+    mism3 = ctx.verify_decomp(ctx.to_device(bad), rhs, parts, d_radix)
+    assert int(mism3.download()[0]) == 1
+    key.close()
+
+
+# ------------------------------------------------------------------ fold
+@pytest.mark.parametrize("r_in,r_out,m", [(4, 2, 16), (32, 8, 64), (3, 5, 7)])
+def test_fold_vs_oracle(ctx, orc, r_in, r_out, m):
+    say("case: fold r_in=%d r_out=%d m=%d" % (r_in, r_out, m))
+    W = rand_ring(orc, (r_in, m, 64), 6, 0xB, 4096)
+    C = rand_ring(orc, (r_out, r_in, 64), 6, 0xF, 1)
+    got = ctx.fold(ctx.to_device(W), ctx.to_device(C)).download()
+    assert np.array_equal(got, orc.fold(W, C, Q, threads=8))
+
+
+# ------------------------------------------------------------------ full-size properties
+def test_cfg2_ntt_full_size_roundtrip(sa, ctx, orc):
+    say("case: 2^20-element NTT round trip")
+    n = 1 << 20
+    dx = ctx.uniform((n, 64), 0, 0xC)
+    ctx.convert(dx, sa.REP_COEFF, sa.REP_INCOMPLETE)
+    inc = dx.download()
+    x = orc.sample_uniform(0, 0xC, (n, 64), Q)
+    idx = np.random.default_rng(0).choice(n, 64, replace=False)
+    assert np.array_equal(inc[idx], orc.to_incomplete(x[idx], Q))
+    ctx.convert(dx, sa.REP_INCOMPLETE, sa.REP_COEFF)
+    assert np.array_equal(dx.download(), x)
+
+
+def test_cfg2_norm_check_full_size(ctx, orc):
+    say("case: 8 x 2^20 norm check, F_q^2, 2 repetitions")
+    rows, L = 8, 1 << 20
+    w = ctx.bounded((rows, L), 16, 0, 0xB)
+    cs = ctx.challenges(2, 20, 1, 1)
+    outs = ctx.norm_check(w, cs, 1)
+    assert outs["ok"].download().all()
+    # rows 0 and 5 in full against the oracle
+    wh = w.download()
+    want = orc.sumcheck(wh[[0, 5]], cs.download(), Q, fq2=True, nr=NR, threads=2)
+    got_msgs = outs["msgs"].download()[:, :, [0, 5]]
+    assert np.array_equal(got_msgs, want["msgs"])
+    assert np.array_equal(outs["evals"].download()[:, [0, 5]], want["evals"])
