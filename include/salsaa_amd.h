@@ -129,6 +129,9 @@ int salsaa_dev_alloc(salsaa_ctx *ctx, size_t bytes, void **d_ptr);
 int salsaa_dev_free(salsaa_ctx *ctx, void *d_ptr);
 int salsaa_memcpy_h2d(salsaa_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
 int salsaa_memcpy_d2h(salsaa_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+/* Stream-ordered device copy (also the plain copy kernel the traffic
+ * counters are calibrated against). */
+int salsaa_memcpy_d2d(salsaa_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 
 /* Events on the context stream, for timing enqueued work. */
 int salsaa_event_create(void **ev);

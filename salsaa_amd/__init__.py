@@ -76,6 +76,7 @@ def _load():
         "salsaa_dev_free": (_int, [_vp, _vp]),
         "salsaa_memcpy_h2d": (_int, [_vp, _vp, _vp, _sz]),
         "salsaa_memcpy_d2h": (_int, [_vp, _vp, _vp, _sz]),
+        "salsaa_memcpy_d2d": (_int, [_vp, _vp, _vp, _sz]),
         "salsaa_event_create": (_int, [_PP]),
         "salsaa_event_destroy": (_int, [_vp]),
         "salsaa_event_record": (_int, [_vp, _vp]),
@@ -195,6 +196,11 @@ class Context:
     def to_device(self, arr, dtype=np.uint64):
         arr = np.ascontiguousarray(arr, dtype=dtype)
         return DeviceArray(self, arr.shape, dtype).upload(arr)
+
+    def copy(self, dst, src):
+        assert dst.nbytes == src.nbytes
+        _check(lib.salsaa_memcpy_d2d(self.h, dst.ptr, src.ptr, src.nbytes))
+        return dst
 
     def uniform(self, shape, seed, stream):
         a = self.empty(shape)

@@ -96,6 +96,23 @@ void build_tables(uint64_t q, std::vector<double> &tab) {
     uint64_t psi32 = h_min_root(64, q);
     for (int j = 0; j < 32; j++) tab[TAB_OMEGA + j] = (double)h_powmod(psi32, 2 * h_brv(j, 5) + 1, q);
     (void)r32;
+    // commit output map: a vector of 64 full-NTT slot values to the
+    // IncompleteNTT form of the ring element it represents (linear in the slots)
+    for (int s = 0; s < 64; s++) {
+        uint64_t e[64] = {0}, ev[32], od[32];
+        e[s] = 1;
+        ntt_inverse_in_place(e, 64, q);
+        for (int i = 0; i < 32; i++) {
+            ev[i] = e[2 * i];
+            od[i] = e[2 * i + 1];
+        }
+        ntt_forward_in_place(ev, 32, q);
+        ntt_forward_in_place(od, 32, q);
+        for (int c = 0; c < 32; c++) {
+            tab[TAB_FIN + s * 64 + c] = (double)ev[c];
+            tab[TAB_FIN + s * 64 + 32 + c] = (double)od[c];
+        }
+    }
 }
 
 }  // namespace
@@ -223,6 +240,12 @@ int salsaa_memcpy_d2h(salsaa_ctx *ctx, void *h_dst, const void *d_src, size_t by
     CTX_OK(ctx);
     SALSAA_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
     SALSAA_HIP(hipStreamSynchronize(ctx->c.stream));
+    return SALSAA_OK;
+}
+
+int salsaa_memcpy_d2d(salsaa_ctx *ctx, void *d_dst, const void *d_src, size_t bytes) {
+    CTX_OK(ctx);
+    SALSAA_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->c.stream));
     return SALSAA_OK;
 }
 
@@ -369,8 +392,17 @@ int salsaa_decomp_ell(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_
                       size_t m, size_t parts, uint64_t radix, uint64_t *d_new_W, uint64_t *d_rhs,
                       uint64_t *d_radix) {
     CTX_OK(ctx);
-    SALSAA_TRY(launch_decompose(&ctx->c, d_W, r, m, parts, radix, d_new_W, d_radix));
-    return commit_rows(ctx, key, d_new_W, r * parts, m, r * parts, m * kD, 0, d_rhs);
+    SALSAA_CHECK(key != nullptr, "null key");
+    SALSAA_CHECK(key->k.ctx == &ctx->c, "key belongs to another context");
+    SALSAA_CHECK(m >= 1 && m <= key->k.m, "witness longer than the key");
+    SALSAA_CHECK(r >= 1, "decomp_ell needs at least one witness row");
+    SALSAA_TRY(launch_radix(&ctx->c, d_W, r, m, parts, radix, d_radix));
+    // one pass over W writes the limb witness and the limbs' slot-major NTT
+    double *w_sm = nullptr;
+    SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * r * parts * m * sizeof(double),
+                           (void **)&w_sm));
+    SALSAA_TRY(launch_decomp_to_slotmajor(&ctx->c, d_W, r, m, parts, d_radix, d_new_W, w_sm));
+    return launch_commit_mac(&ctx->c, key->k.d_A, key->k.kappa, key->k.m, w_sm, r * parts, m, d_rhs);
 }
 
 int salsaa_compose(salsaa_ctx *ctx, const uint64_t *d_in, size_t rows, size_t n, size_t parts,

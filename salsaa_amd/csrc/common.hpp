@@ -54,7 +54,9 @@ enum TableOffset : int {
     TAB_OMEGA = 192,  // [32]  omega_j = NTT_32(Y)[j] (slot moduli X^2 - omega_j)
     TAB_NINV32 = 224, // 32^{-1}
     TAB_NINV64 = 225, // 64^{-1}
-    TAB_SIZE = 256,
+    TAB_FIN = 256,    // [64][64] transposed map full-NTT slots -> IncompleteNTT:
+                      // fin[s * 64 + c] = (IncNTT(INTT_64(e_s)))[c]
+    TAB_SIZE = 256 + 64 * 64,
 };
 
 // ---------------------------------------------------------------- context
@@ -186,6 +188,10 @@ int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse
 int launch_ntt_full_to_slotmajor(Context *ctx, const uint64_t *src, size_t rows, size_t m,
                                  size_t row_lo, size_t stride_lo, size_t stride_hi,
                                  double *dst);
+int launch_decomp_to_slotmajor(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
+                               const uint64_t *d_radix, uint64_t *new_W, double *dst);
+int launch_radix(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
+                 uint64_t radix, uint64_t *d_radix);
 int launch_commit_mac(Context *ctx, const double *A_sm, size_t kappa, size_t mA,
                       const double *W_sm, size_t r, size_t m, uint64_t *out);
 int launch_sample_uniform(Context *ctx, uint64_t *out, size_t n, uint64_t seed, uint64_t stream);

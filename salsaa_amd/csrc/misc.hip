@@ -221,11 +221,11 @@ int launch_challenges(Context *ctx, uint64_t *cs, size_t reps, size_t nrounds, i
     return SALSAA_OK;
 }
 
-int launch_decompose(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
-                     uint64_t radix, uint64_t *out, uint64_t *d_radix) {
+int launch_radix(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
+                 uint64_t radix, uint64_t *d_radix) {
     SALSAA_CHECK(parts >= 1 && parts <= 64, "parts must be in [1, 64]");
     SALSAA_CHECK(radix == 0 || (radix & (radix - 1)) == 0, "radix must be a power of two");
-    const size_t md = m * kD, n = r * md;
+    const size_t n = r * m * kD;
     unsigned long long *acc = nullptr;
     SALSAA_TRY(scratch_get(ctx, 3, sizeof(unsigned long long), (void **)&acc));
     if (radix == 0) {
@@ -235,6 +235,14 @@ int launch_decompose(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t
                            acc);
     }
     hipLaunchKernelGGL(k_set_radix, dim3(1), dim3(1), 0, ctx->stream, acc, radix, parts, d_radix);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_decompose(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
+                     uint64_t radix, uint64_t *out, uint64_t *d_radix) {
+    SALSAA_TRY(launch_radix(ctx, W, r, m, parts, radix, d_radix));
+    const size_t md = m * kD, n = r * md;
     hipLaunchKernelGGL(k_decompose, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, W, r, md,
                        parts, d_radix, ctx->q, out);
     SALSAA_HIP(hipGetLastError());

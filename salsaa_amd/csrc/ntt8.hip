@@ -1,0 +1,351 @@
+// ntt8.hip -- batched NTT kernels for d = 64 with 8 threads per ring element
+// (see ntt8.hpp), a block of 1024 threads staging 128 elements through LDS
+// with coalesced 16-byte global accesses.
+//
+//   k_ntt_inc<false/true>  Coefficient <-> IncompleteNTT in place
+//                          (cyclotomic_ring.rs:483-511, 527-549)
+//   k_ntt_full<false/true> Coefficient <-> NTT in place (cyclotomic_ring.rs:513-525)
+//   k_ntt_full_sm          Coefficient rows -> full NTT written slot-major as
+//                          exact doubles [64][rows][m] (the commit's W layout)
+//   k_decomp_sm            decompose_matrix_by_chunks fused with the limbs'
+//                          full NTT: writes the limb witness (Coefficient) and
+//                          the limbs' slot-major NTT in one pass over W
+#include "ntt8.hpp"
+
+namespace salsaa {
+
+namespace {
+
+constexpr int EL = 128;      // ring elements per block
+constexpr int NT = 1024;     // threads per block (8 per element)
+constexpr int SROW = 72;     // LDS row stride in u64 (64 + 8 pad)
+
+__device__ __forceinline__ uint64_t dbits(double v) { return (uint64_t)__double_as_longlong(v); }
+__device__ __forceinline__ double bitsd(uint64_t b) { return __longlong_as_double((long long)b); }
+
+// LDS row layouts (row stride SROW u64): "plain" keeps entry k at k; "skew"
+// puts entry k at k + k/8, which makes the 8-consecutive-entry accesses of
+// phase B and the stride-4/8 accesses of phase A bank-conflict free.
+__device__ __forceinline__ int skew(int k) { return k + (k >> 3); }
+
+template <bool SKEW>
+__device__ __forceinline__ void load_block(uint64_t *lds, const uint64_t *__restrict__ src, int nel,
+                                           int tid) {
+    const ulonglong2 *s2 = reinterpret_cast<const ulonglong2 *>(src);
+    for (int c = tid; c < nel * 32; c += NT) {
+        const int e = c >> 5, k = (c & 31) * 2;
+        const ulonglong2 v = s2[c];
+        if (SKEW) {
+            lds[e * SROW + skew(k)] = v.x;
+            lds[e * SROW + skew(k + 1)] = v.y;
+        } else {
+            *reinterpret_cast<ulonglong2 *>(&lds[e * SROW + k]) = v;
+        }
+    }
+}
+
+template <bool SKEW>
+__device__ __forceinline__ void store_block(uint64_t *__restrict__ dst, const uint64_t *lds, int nel,
+                                            int tid) {
+    ulonglong2 *d2 = reinterpret_cast<ulonglong2 *>(dst);
+    for (int c = tid; c < nel * 32; c += NT) {
+        const int e = c >> 5, k = (c & 31) * 2;
+        ulonglong2 v;
+        if (SKEW) {
+            v.x = lds[e * SROW + skew(k)];
+            v.y = lds[e * SROW + skew(k + 1)];
+        } else {
+            v = *reinterpret_cast<const ulonglong2 *>(&lds[e * SROW + k]);
+        }
+        d2[c] = v;
+    }
+}
+
+// IncompleteNTT: thread h of element e works on half p = h >> 2 as t = h & 3.
+// Half p, position i lives at 36 p + skew(i) in the skewed half layout.
+template <bool INV>
+__global__ __launch_bounds__(NT) void k_ntt_inc(uint64_t *__restrict__ data, size_t count,
+                                                const double *__restrict__ tab, Mod M) {
+    __shared__ uint64_t lds[EL * SROW];
+    const size_t e0 = (size_t)blockIdx.x * EL;
+    const int nel = (int)min((size_t)EL, count - e0);
+    const int tid = threadIdx.x, e = tid >> 3, h = tid & 7, p = h >> 2, t = h & 3;
+    const bool live = e < nel;
+    uint64_t *row = lds + e * SROW;
+    double v[8];
+    if (!INV) {
+        load_block<false>(lds, data + e0 * kD, nel, tid);
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = u2d(row[8 * k + 2 * t + p]);  // coeff 2(t+4k)+p
+            ntt32_fwd_a(v, tab + TAB_FWD32, M);
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) row[36 * p + skew(t + 4 * k)] = dbits(v[k]);
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int x = 0; x < 8; x++) v[x] = bitsd(row[36 * p + skew(8 * t + x)]);
+            ntt32_fwd_b(v, tab + TAB_FWD32, t, M);
+            // output entry 32 p + 8 t + x, skewed: same slots this thread read
+#pragma unroll
+            for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = d2u(v[x]);
+        }
+        __syncthreads();
+        store_block<true>(data + e0 * kD, lds, nel, tid);
+    } else {
+        load_block<true>(lds, data + e0 * kD, nel, tid);
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int x = 0; x < 8; x++) v[x] = u2d(row[36 * p + skew(8 * t + x)]);
+            ntt32_inv_b(v, tab + TAB_INV32, t, M);
+#pragma unroll
+            for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
+            ntt32_inv_a(v, tab + TAB_INV32, tab[TAB_NINV32], M);
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) row[8 * k + 2 * t + p] = d2u(v[k]);
+        }
+        __syncthreads();
+        store_block<false>(data + e0 * kD, lds, nel, tid);
+    }
+}
+
+template <bool INV>
+__global__ __launch_bounds__(NT) void k_ntt_full(uint64_t *__restrict__ data, size_t count,
+                                                 const double *__restrict__ tab, Mod M) {
+    __shared__ uint64_t lds[EL * SROW];
+    const size_t e0 = (size_t)blockIdx.x * EL;
+    const int nel = (int)min((size_t)EL, count - e0);
+    const int tid = threadIdx.x, e = tid >> 3, t = tid & 7;
+    const bool live = e < nel;
+    uint64_t *row = lds + e * SROW;
+    double v[8];
+    if (!INV) {
+        load_block<false>(lds, data + e0 * kD, nel, tid);
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = u2d(row[t + 8 * k]);
+            ntt64_fwd_a(v, tab + TAB_FWD64, M);
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) row[skew(t + 8 * k)] = dbits(v[k]);
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int x = 0; x < 8; x++) v[x] = bitsd(row[skew(8 * t + x)]);
+            ntt64_fwd_b(v, tab + TAB_FWD64, t, M);
+#pragma unroll
+            for (int x = 0; x < 8; x++) row[skew(8 * t + x)] = d2u(v[x]);
+        }
+        __syncthreads();
+        store_block<true>(data + e0 * kD, lds, nel, tid);
+    } else {
+        load_block<true>(lds, data + e0 * kD, nel, tid);
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int x = 0; x < 8; x++) v[x] = u2d(row[skew(8 * t + x)]);
+            ntt64_inv_b(v, tab + TAB_INV64, t, M);
+#pragma unroll
+            for (int x = 0; x < 8; x++) row[skew(8 * t + x)] = dbits(v[x]);
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = bitsd(row[skew(t + 8 * k)]);
+            ntt64_inv_a(v, tab + TAB_INV64, tab[TAB_NINV64], M);
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) row[t + 8 * k] = d2u(v[k]);
+        }
+        __syncthreads();
+        store_block<false>(data + e0 * kD, lds, nel, tid);
+    }
+}
+
+// Full forward NTT from phase-A inputs v (positions t + 8k); uses `xch` (a
+// 128 x SROW region) for the exchange and leaves the 64 slots transposed in
+// out[s * EL + e] (64 x 128 u64; may alias xch).
+__device__ __forceinline__ void full_fwd_to_work(double (&v)[8], uint64_t *xch, uint64_t *out,
+                                                 int e, int t, bool live,
+                                                 const double *__restrict__ tab, Mod M) {
+    if (live) ntt64_fwd_a(v, tab + TAB_FWD64, M);
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) xch[e * SROW + skew(t + 8 * k)] = dbits(v[k]);
+    }
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int x = 0; x < 8; x++) v[x] = bitsd(xch[e * SROW + skew(8 * t + x)]);
+        ntt64_fwd_b(v, tab + TAB_FWD64, t, M);
+    }
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int x = 0; x < 8; x++) out[(8 * t + x) * EL + e] = dbits(v[x]);
+    }
+    __syncthreads();
+}
+
+// slot-major store: dst_row(s)[i0 + e] for s < 64, e < nel
+__device__ __forceinline__ void store_slotmajor(double *__restrict__ dst, const uint64_t *work,
+                                                size_t rows, size_t row, size_t m, size_t i0,
+                                                int nel, int tid) {
+    for (int c = tid; c < 64 * EL; c += NT) {
+        const int s = c >> 7, e = c & (EL - 1);
+        if (e < nel) dst[((size_t)s * rows + row) * m + i0 + e] = bitsd(work[c]);
+    }
+}
+
+// grid (ceil(m / 128), rows).  Source row `row` starts at
+// (row % row_lo) * stride_lo + (row / row_lo) * stride_hi (u64 units), which
+// covers plain matrices and the halves of split (split.rs:21-28).
+__global__ __launch_bounds__(NT) void k_ntt_full_sm(const uint64_t *__restrict__ src, size_t rows,
+                                                    size_t m, size_t row_lo, size_t stride_lo,
+                                                    size_t stride_hi, double *__restrict__ dst,
+                                                    const double *__restrict__ tab, Mod M) {
+    __shared__ uint64_t lds[EL * SROW];
+    const size_t row = blockIdx.y;
+    const size_t i0 = (size_t)blockIdx.x * EL;
+    const int nel = (int)min((size_t)EL, m - i0);
+    const int tid = threadIdx.x, e = tid >> 3, t = tid & 7;
+    const uint64_t *base = src + (row % row_lo) * stride_lo + (row / row_lo) * stride_hi;
+    load_block<false>(lds, base + i0 * kD, nel, tid);
+    __syncthreads();
+    double v[8];
+    const bool live = e < nel;
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = u2d(lds[e * SROW + t + 8 * k]);
+    }
+    full_fwd_to_work(v, lds, lds, e, t, live, tab, M);
+    store_slotmajor(dst, lds, rows, row, m, i0, nel, tid);
+}
+
+// signed digit t of x (arithmetic.rs:1496-1570): x is split into pos (2x <= q)
+// or neg (q - x); digit = (|.| >> t lg) & (radix - 1); limb = +-digit mod q
+__device__ __forceinline__ uint64_t limb_of(uint64_t x, uint64_t q, unsigned lg, uint64_t mask,
+                                            int tl) {
+    const bool neg = 2 * x > q;
+    const uint64_t v = neg ? q - x : x;
+    const unsigned s = lg * (unsigned)tl;
+    const uint64_t dg = (lg == 0) ? (v & mask) : (s >= 64 ? 0 : (v >> s) & mask);
+    return neg ? (dg ? q - dg : 0) : dg;
+}
+
+// grid (ceil(m / 128), r).  Row rho of W (Coefficient) -> limb rows
+// rho * parts + tl of new_W (Coefficient) and of dst (slot-major full NTT,
+// rows = r * parts in total).
+__global__ __launch_bounds__(NT) void k_decomp_sm(const uint64_t *__restrict__ W, size_t r,
+                                                  size_t m, size_t parts,
+                                                  const uint64_t *__restrict__ d_radix,
+                                                  uint64_t *__restrict__ new_W,
+                                                  double *__restrict__ dst, uint64_t q,
+                                                  const double *__restrict__ tab, Mod M) {
+    __shared__ uint64_t raw[EL * SROW];
+    __shared__ uint64_t work[EL * SROW];
+    const size_t rho = blockIdx.y;
+    const size_t i0 = (size_t)blockIdx.x * EL;
+    const int nel = (int)min((size_t)EL, m - i0);
+    const int tid = threadIdx.x, e = tid >> 3, t = tid & 7;
+    const uint64_t radix = *d_radix;
+    const unsigned lg = (unsigned)__ffsll((long long)radix) - 1u;
+    const uint64_t mask = radix - 1;
+    load_block<false>(raw, W + (rho * m + i0) * kD, nel, tid);
+    __syncthreads();
+    const bool live = e < nel;
+    const size_t rows = r * parts;
+    for (int tl = 0; tl < (int)parts; tl++) {
+        const size_t orow = rho * parts + tl;
+        // limb witness row, Coefficient form
+        ulonglong2 *nw = reinterpret_cast<ulonglong2 *>(new_W + (orow * m + i0) * kD);
+        for (int c = tid; c < nel * 32; c += NT) {
+            const int ee = c >> 5, k = (c & 31) * 2;
+            ulonglong2 o;
+            o.x = limb_of(raw[ee * SROW + k], q, lg, mask, tl);
+            o.y = limb_of(raw[ee * SROW + k + 1], q, lg, mask, tl);
+            nw[c] = o;
+        }
+        double v[8];
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = u2d(limb_of(raw[e * SROW + t + 8 * k], q, lg, mask, tl));
+        }
+        full_fwd_to_work(v, work, work, e, t, live, tab, M);
+        store_slotmajor(dst, work, rows, orow, m, i0, nel, tid);
+        __syncthreads();  // work is rewritten by the next limb
+    }
+}
+
+}  // namespace
+
+int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse) {
+    if (count == 0) return SALSAA_OK;
+    Mod M{ctx->qd, ctx->qinv};
+    const unsigned grid = ceil_div(count, EL);
+    if (mode == SALSAA_REP_INCOMPLETE) {
+        if (inverse)
+            hipLaunchKernelGGL(k_ntt_inc<true>, dim3(grid), dim3(NT), 0, ctx->stream, data, count,
+                               ctx->d_tables, M);
+        else
+            hipLaunchKernelGGL(k_ntt_inc<false>, dim3(grid), dim3(NT), 0, ctx->stream, data, count,
+                               ctx->d_tables, M);
+    } else {
+        if (inverse)
+            hipLaunchKernelGGL(k_ntt_full<true>, dim3(grid), dim3(NT), 0, ctx->stream, data, count,
+                               ctx->d_tables, M);
+        else
+            hipLaunchKernelGGL(k_ntt_full<false>, dim3(grid), dim3(NT), 0, ctx->stream, data, count,
+                               ctx->d_tables, M);
+    }
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_ntt_full_to_slotmajor(Context *ctx, const uint64_t *src, size_t rows, size_t m,
+                                 size_t row_lo, size_t stride_lo, size_t stride_hi,
+                                 double *dst) {
+    if (rows == 0 || m == 0) return SALSAA_OK;
+    SALSAA_CHECK(rows < 65536, "too many rows for one launch");
+    Mod M{ctx->qd, ctx->qinv};
+    dim3 grid(ceil_div(m, EL), (unsigned)rows);
+    hipLaunchKernelGGL(k_ntt_full_sm, grid, dim3(NT), 0, ctx->stream, src, rows, m, row_lo,
+                       stride_lo, stride_hi, dst, ctx->d_tables, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_decomp_to_slotmajor(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
+                               const uint64_t *d_radix, uint64_t *new_W, double *dst) {
+    if (r == 0 || m == 0) return SALSAA_OK;
+    SALSAA_CHECK(r < 65536, "too many rows for one launch");
+    Mod M{ctx->qd, ctx->qinv};
+    dim3 grid(ceil_div(m, EL), (unsigned)r);
+    hipLaunchKernelGGL(k_decomp_sm, grid, dim3(NT), 0, ctx->stream, W, r, m, parts, d_radix, new_W,
+                       dst, ctx->q, ctx->d_tables, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+}  // namespace salsaa

@@ -1,0 +1,129 @@
+// ntt8.hpp -- negacyclic NTTs with 8 threads per ring element (d = 64).
+//
+// Each thread holds 8 of an element's values.  The first three butterfly
+// stages pair values in the same residue class (positions t + stride * k),
+// so they run in registers; one exchange through LDS regroups the values as
+// 8 consecutive positions per thread (8t + x) for the remaining stages.  With
+// ~40 VGPRs per thread and 73.7 KiB of LDS per 128-element block a CU keeps
+// 32 waves in flight, which is what an HBM-bound transform needs.
+//
+// Conventions (HEXL): tw[m + i] = psi^{brv(m + i)}, Cooley-Tukey forward,
+// Gentleman-Sande inverse with inverse twiddles, the inverse's last stage
+// folding n^{-1} (twi[0] = twi[1] * n^{-1}).
+#pragma once
+
+#include "common.hpp"
+
+namespace salsaa {
+
+// forward butterfly X, Y <- X + wY, X - wY
+__device__ __forceinline__ void bf_fwd(double &X, double &Y, double w, Mod M) {
+    const double T = mulmod(Y, w, M);
+    const double x = X;
+    X = addmod(x, T, M);
+    Y = submod(x, T, M);
+}
+// inverse butterfly X, Y <- X + Y, (X - Y) w
+__device__ __forceinline__ void bf_inv(double &X, double &Y, double w, Mod M) {
+    const double x = X, y = Y;
+    X = addmod(x, y, M);
+    Y = mulmod(submod(x, y, M), w, M);
+}
+
+// ---------------------------------------------------------------- n = 32
+// Thread t (0..3) of a half; phase A positions t + 4k (k < 8), phase B 8t + x.
+__device__ __forceinline__ void ntt32_fwd_a(double (&v)[8], const double *__restrict__ tw, Mod M) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) bf_fwd(v[k], v[k + 4], tw[1], M);  // distance 16
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if ((k & 2) == 0) bf_fwd(v[k], v[k + 2], tw[2 + (k >> 2)], M);  // distance 8
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) bf_fwd(v[k], v[k + 1], tw[4 + (k >> 1)], M);  // distance 4
+}
+__device__ __forceinline__ void ntt32_fwd_b(double (&v)[8], const double *__restrict__ tw, int t,
+                                            Mod M) {
+    const double w8a = tw[8 + 2 * t], w8b = tw[8 + 2 * t + 1];
+#pragma unroll
+    for (int x = 0; x < 8; x++)
+        if ((x & 2) == 0) bf_fwd(v[x], v[x + 2], (x & 4) ? w8b : w8a, M);  // distance 2
+#pragma unroll
+    for (int x = 0; x < 8; x += 2) bf_fwd(v[x], v[x + 1], tw[16 + 4 * t + (x >> 1)], M);  // distance 1
+}
+__device__ __forceinline__ void ntt32_inv_b(double (&v)[8], const double *__restrict__ twi, int t,
+                                            Mod M) {
+#pragma unroll
+    for (int x = 0; x < 8; x += 2) bf_inv(v[x], v[x + 1], twi[16 + 4 * t + (x >> 1)], M);
+    const double w8a = twi[8 + 2 * t], w8b = twi[8 + 2 * t + 1];
+#pragma unroll
+    for (int x = 0; x < 8; x++)
+        if ((x & 2) == 0) bf_inv(v[x], v[x + 2], (x & 4) ? w8b : w8a, M);
+}
+__device__ __forceinline__ void ntt32_inv_a(double (&v)[8], const double *__restrict__ twi,
+                                            double ninv, Mod M) {
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) bf_inv(v[k], v[k + 1], twi[4 + (k >> 1)], M);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if ((k & 2) == 0) bf_inv(v[k], v[k + 2], twi[2 + (k >> 2)], M);
+    const double wl = twi[0];  // twi[1] * n^{-1}
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double x = v[k], y = v[k + 4];
+        v[k] = mulmod(addmod(x, y, M), ninv, M);
+        v[k + 4] = mulmod(submod(x, y, M), wl, M);
+    }
+}
+
+// ---------------------------------------------------------------- n = 64
+// Thread t (0..7); phase A positions t + 8k, phase B 8t + x.
+__device__ __forceinline__ void ntt64_fwd_a(double (&v)[8], const double *__restrict__ tw, Mod M) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) bf_fwd(v[k], v[k + 4], tw[1], M);  // distance 32
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if ((k & 2) == 0) bf_fwd(v[k], v[k + 2], tw[2 + (k >> 2)], M);  // distance 16
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) bf_fwd(v[k], v[k + 1], tw[4 + (k >> 1)], M);  // distance 8
+}
+__device__ __forceinline__ void ntt64_fwd_b(double (&v)[8], const double *__restrict__ tw, int t,
+                                            Mod M) {
+    const double w8 = tw[8 + t];
+#pragma unroll
+    for (int x = 0; x < 4; x++) bf_fwd(v[x], v[x + 4], w8, M);  // distance 4
+    const double w16a = tw[16 + 2 * t], w16b = tw[16 + 2 * t + 1];
+#pragma unroll
+    for (int x = 0; x < 8; x++)
+        if ((x & 2) == 0) bf_fwd(v[x], v[x + 2], (x & 4) ? w16b : w16a, M);  // distance 2
+#pragma unroll
+    for (int x = 0; x < 8; x += 2) bf_fwd(v[x], v[x + 1], tw[32 + 4 * t + (x >> 1)], M);  // distance 1
+}
+__device__ __forceinline__ void ntt64_inv_b(double (&v)[8], const double *__restrict__ twi, int t,
+                                            Mod M) {
+#pragma unroll
+    for (int x = 0; x < 8; x += 2) bf_inv(v[x], v[x + 1], twi[32 + 4 * t + (x >> 1)], M);
+    const double w16a = twi[16 + 2 * t], w16b = twi[16 + 2 * t + 1];
+#pragma unroll
+    for (int x = 0; x < 8; x++)
+        if ((x & 2) == 0) bf_inv(v[x], v[x + 2], (x & 4) ? w16b : w16a, M);
+    const double w8 = twi[8 + t];
+#pragma unroll
+    for (int x = 0; x < 4; x++) bf_inv(v[x], v[x + 4], w8, M);
+}
+__device__ __forceinline__ void ntt64_inv_a(double (&v)[8], const double *__restrict__ twi,
+                                            double ninv, Mod M) {
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) bf_inv(v[k], v[k + 1], twi[4 + (k >> 1)], M);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if ((k & 2) == 0) bf_inv(v[k], v[k + 2], twi[2 + (k >> 2)], M);
+    const double wl = twi[0];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double x = v[k], y = v[k + 4];
+        v[k] = mulmod(addmod(x, y, M), ninv, M);
+        v[k + 4] = mulmod(submod(x, y, M), wl, M);
+    }
+}
+
+}  // namespace salsaa

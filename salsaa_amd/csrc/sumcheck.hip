@@ -86,6 +86,12 @@ __device__ __forceinline__ void block_sum(double (&x)[NV], double *red, Mod M) {
     __syncthreads();
 }
 
+// x mod q for any u64 x: x = hi * 2^32 + lo with hi, lo < 2^32 < q
+__device__ __forceinline__ double u64_mod(unsigned long long x, Mod M) {
+    const double hi = u2d(x >> 32), lo = u2d(x & 0xFFFFFFFFULL);
+    return addmod(mulmod(hi, 4294967296.0, M), lo, M);
+}
+
 __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) {
     atomicAdd(p, (unsigned long long)d2u(v));
 }
@@ -381,24 +387,30 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
         }
     }
 
-    // ---- verifier replay (fast_norm_check.rs:352-399, 443-462)
+    // ---- verifier replay (fast_norm_check.rs:352-399, 443-462): first reduce
+    // the accumulated messages of the earlier passes in parallel
+    __shared__ double msg_pre[48][6];
+    __shared__ double claim_red;
+    for (int x = tid; x < t0 * 6; x += SC_THREADS) {
+        const int tt = x / 6, k = x % 6;
+        msg_pre[tt][k] = u64_mod(macc[macc_idx(rep, tt, row, nrounds, rows) + k], M);
+    }
+    if (tid == SC_THREADS - 1) claim_red = FIRST ? claim_local : u64_mod(cacc[row], M);
+    __syncthreads();
+    if (msgs_out) {
+        for (int x = tid; x < nrounds * 6; x += SC_THREADS) {
+            const int tt = x / 6, k = x % 6;
+            const double f = tt < t0 ? msg_pre[tt][k] : msg_local[tt - t0][k];
+            msgs_out[(((size_t)rep * nrounds + tt) * rows + row) * 6 + k] = d2u(f);
+        }
+    }
     if (tid == 0) {
-        const double claim0 = FIRST ? claim_local : u2d(cacc[row] % q);
+        const double claim0 = claim_red;
         if (rep == 0 && claims_out) claims_out[row] = d2u(claim0);
         F2 cl2 = {claim0, 0.0};
         int good = 1;
         for (int tt = 0; tt < nrounds; tt++) {
-            double f[6];
-            if (tt < t0) {
-                const unsigned long long *p = macc + macc_idx(rep, tt, row, nrounds, rows);
-                for (int k = 0; k < 6; k++) f[k] = u2d(p[k] % q);
-            } else {
-                for (int k = 0; k < 6; k++) f[k] = msg_local[tt - t0][k];
-            }
-            if (msgs_out) {
-                uint64_t *mo = msgs_out + (((size_t)rep * nrounds + tt) * rows + row) * 6;
-                for (int k = 0; k < 6; k++) mo[k] = d2u(f[k]);
-            }
+            const double *f = tt < t0 ? msg_pre[tt] : msg_local[tt - t0];
             const F2 f0 = {f[0], f[1]}, f1 = {f[2], f[3]}, f2 = {f[4], f[5]};
             const F2 lhs = x_add<true>(x_add<true>(x_add<true>(f0, f0, M), x_add<true>(f1, f1, M), M),
                                        f2, M);
