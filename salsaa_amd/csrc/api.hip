@@ -1,6 +1,7 @@
 // api.hip -- the batched device C ABI: contexts, keys, memory helpers and the
 // operator entry points that replace the reference's arithmetic.rs and
 // subroutines/*.rs functions on HBM-resident matrices.
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -98,6 +99,7 @@ void build_tables(uint64_t q, std::vector<double> &tab) {
     (void)r32;
     // commit output map: a vector of 64 full-NTT slot values to the
     // IncompleteNTT form of the ring element it represents (linear in the slots)
+    for (int t = 0; t <= 12; t++) tab[TAB_W256 + t] = (double)h_powmod(2, 8 * (uint64_t)t, q);
     for (int s = 0; s < 64; s++) {
         uint64_t e[64] = {0}, ev[32], od[32];
         e[s] = 1;
@@ -177,6 +179,8 @@ int salsaa_ctx_create(uint64_t q, uint32_t d, salsaa_ctx **out) {
     x.d = (int)d;
     x.qd = (double)q;
     x.qinv = 1.0 / (double)q;
+    const char *path = getenv("SALSAA_COMMIT");
+    x.mfma_commit = !(path && std::string(path) == "valu");
     build_tables(q, x.h_tables);
     if (hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&x.d_tables, TAB_SIZE * sizeof(double)) != hipSuccess ||
@@ -321,15 +325,29 @@ int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t
     k->k.ctx = &ctx->c;
     k->k.kappa = kappa;
     k->k.m = m;
-    hipError_t e = hipMalloc(&k->k.d_A, (size_t)kD * kappa * m * sizeof(double));
-    if (e != hipSuccess) {
-        delete k;
-        set_error(std::string("key allocation: ") + hipGetErrorString(e));
-        return SALSAA_ERR_NOMEM;
+    int s = SALSAA_OK;
+    if (ctx->c.mfma_commit) {
+        k->k.pstride = (m + 127) / 128 * 128;
+        hipError_t e = hipMalloc(&k->k.d_AB, (size_t)kD * kappa * 7 * k->k.pstride);
+        if (e != hipSuccess) {
+            delete k;
+            set_error(std::string("key allocation: ") + hipGetErrorString(e));
+            return SALSAA_ERR_NOMEM;
+        }
+        s = launch_ntt_full_to_planes(&ctx->c, d_A, kappa, m, kappa, m * kD, 0, k->k.d_AB,
+                                      k->k.pstride);
+    } else {
+        hipError_t e = hipMalloc(&k->k.d_A, (size_t)kD * kappa * m * sizeof(double));
+        if (e != hipSuccess) {
+            delete k;
+            set_error(std::string("key allocation: ") + hipGetErrorString(e));
+            return SALSAA_ERR_NOMEM;
+        }
+        s = launch_ntt_full_to_slotmajor(&ctx->c, d_A, kappa, m, kappa, m * kD, 0, k->k.d_A);
     }
-    int s = launch_ntt_full_to_slotmajor(&ctx->c, d_A, kappa, m, kappa, m * kD, 0, k->k.d_A);
     if (s != SALSAA_OK) {
         (void)hipFree(k->k.d_A);
+        (void)hipFree(k->k.d_AB);
         delete k;
         return s;
     }
@@ -340,7 +358,8 @@ int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t
 int salsaa_key_destroy(salsaa_key *key) {
     if (!key) return SALSAA_OK;
     (void)hipStreamSynchronize(key->k.ctx->stream);
-    (void)hipFree(key->k.d_A);
+    if (key->k.d_A) (void)hipFree(key->k.d_A);
+    if (key->k.d_AB) (void)hipFree(key->k.d_AB);
     delete key;
     return SALSAA_OK;
 }
@@ -352,6 +371,15 @@ static int commit_rows(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d
     SALSAA_CHECK(key->k.ctx == &ctx->c, "key belongs to another context");
     SALSAA_CHECK(m >= 1 && m <= key->k.m, "witness longer than the key");
     SALSAA_CHECK(rows >= 1, "commit needs at least one witness row");
+    if (ctx->c.mfma_commit) {
+        const size_t ps = (m + 127) / 128 * 128;
+        int8_t *wb = nullptr;
+        SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * rows * 7 * ps, (void **)&wb));
+        SALSAA_TRY(launch_ntt_full_to_planes(&ctx->c, d_W, rows, m, row_lo, stride_lo, stride_hi,
+                                             wb, ps));
+        return launch_commit_mfma(&ctx->c, key->k.d_AB, key->k.kappa, key->k.pstride, wb, rows, m,
+                                  ps, d_out);
+    }
     double *w_sm = nullptr;
     SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * rows * m * sizeof(double), (void **)&w_sm));
     SALSAA_TRY(launch_ntt_full_to_slotmajor(&ctx->c, d_W, rows, m, row_lo, stride_lo, stride_hi,
@@ -398,11 +426,20 @@ int salsaa_decomp_ell(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_
     SALSAA_CHECK(r >= 1, "decomp_ell needs at least one witness row");
     SALSAA_TRY(launch_radix(&ctx->c, d_W, r, m, parts, radix, d_radix));
     // one pass over W writes the limb witness and the limbs' slot-major NTT
+    const size_t rows = r * parts;
+    if (ctx->c.mfma_commit) {
+        const size_t ps = (m + 127) / 128 * 128;
+        int8_t *wb = nullptr;
+        SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * rows * 7 * ps, (void **)&wb));
+        SALSAA_TRY(launch_decomp_to_slotmajor(&ctx->c, d_W, r, m, parts, d_radix, d_new_W, wb, 1,
+                                              ps));
+        return launch_commit_mfma(&ctx->c, key->k.d_AB, key->k.kappa, key->k.pstride, wb, rows, m,
+                                  ps, d_rhs);
+    }
     double *w_sm = nullptr;
-    SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * r * parts * m * sizeof(double),
-                           (void **)&w_sm));
-    SALSAA_TRY(launch_decomp_to_slotmajor(&ctx->c, d_W, r, m, parts, d_radix, d_new_W, w_sm));
-    return launch_commit_mac(&ctx->c, key->k.d_A, key->k.kappa, key->k.m, w_sm, r * parts, m, d_rhs);
+    SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * rows * m * sizeof(double), (void **)&w_sm));
+    SALSAA_TRY(launch_decomp_to_slotmajor(&ctx->c, d_W, r, m, parts, d_radix, d_new_W, w_sm, 0, 0));
+    return launch_commit_mac(&ctx->c, key->k.d_A, key->k.kappa, key->k.m, w_sm, rows, m, d_rhs);
 }
 
 int salsaa_compose(salsaa_ctx *ctx, const uint64_t *d_in, size_t rows, size_t n, size_t parts,

@@ -56,7 +56,8 @@ enum TableOffset : int {
     TAB_NINV64 = 225, // 64^{-1}
     TAB_FIN = 256,    // [64][64] transposed map full-NTT slots -> IncompleteNTT:
                       // fin[s * 64 + c] = (IncNTT(INTT_64(e_s)))[c]
-    TAB_SIZE = 256 + 64 * 64,
+    TAB_W256 = 256 + 64 * 64,  // [13] 2^(8t) mod q, digit weights of the int8 commit
+    TAB_SIZE = 256 + 64 * 64 + 16,
 };
 
 // ---------------------------------------------------------------- context
@@ -74,13 +75,16 @@ struct Context {
     double *d_tables = nullptr;     // TAB_SIZE doubles
     std::vector<double> h_tables;   // host copy
     ScratchBuf scratch[8];          // grown on demand, owned by the context
+    bool mfma_commit = true;        // int8 MFMA commit (false: fp64 VALU commit)
     std::mutex lock;                // one host thread drives a context at a time
 };
 
 struct Key {
     Context *ctx = nullptr;
     size_t kappa = 0, m = 0;
-    double *d_A = nullptr;  // slot-major full-NTT [64][kappa][m] as exact fp64
+    double *d_A = nullptr;   // slot-major full-NTT [64][kappa][m] as exact fp64 (VALU path)
+    int8_t *d_AB = nullptr;  // digit planes [64][kappa * 7][pstride] (MFMA path)
+    size_t pstride = 0;      // plane row length: m rounded up to 128
 };
 
 int scratch_get(Context *ctx, int slot, size_t bytes, void **out);
@@ -189,7 +193,15 @@ int launch_ntt_full_to_slotmajor(Context *ctx, const uint64_t *src, size_t rows,
                                  size_t row_lo, size_t stride_lo, size_t stride_hi,
                                  double *dst);
 int launch_decomp_to_slotmajor(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
-                               const uint64_t *d_radix, uint64_t *new_W, double *dst);
+                               const uint64_t *d_radix, uint64_t *new_W, void *dst, int planes,
+                               size_t pstride);
+int launch_ntt_full_to_planes(Context *ctx, const uint64_t *src, size_t rows, size_t m,
+                              size_t row_lo, size_t stride_lo, size_t stride_hi, int8_t *dst,
+                              size_t pstride);
+int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t strideA,
+                       const int8_t *WB, size_t r, size_t m, size_t strideW, uint64_t *out);
+int launch_commit_finalize(Context *ctx, const double *partial, size_t ksplit, size_t kappa,
+                           size_t r, uint64_t *out);
 int launch_radix(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
                  uint64_t radix, uint64_t *d_radix);
 int launch_commit_mac(Context *ctx, const double *A_sm, size_t kappa, size_t mA,

@@ -218,17 +218,61 @@ __device__ __forceinline__ void store_slotmajor(double *__restrict__ dst, const 
     }
 }
 
+// Signed base-256 digits of a residue x < 2^50: x = sum_u d_u 256^u with
+// d_u in [-128, 127], 7 digits (the top one is in [0, 4]).
+__device__ __forceinline__ void digits7(uint64_t x, int (&d)[7]) {
+    long long y = (long long)x;
+#pragma unroll
+    for (int u = 0; u < 7; u++) {
+        int b = (int)(y & 255);
+        b -= (b >= 128) ? 256 : 0;
+        d[u] = b;
+        y = (y - b) >> 8;
+    }
+}
+
+// byte-plane store for the int8 MFMA commit: value (s, row, i) contributes
+// digit u to plane row (s, row * 7 + u), column i (row length `stride`,
+// a multiple of 128).  Each thread packs 4 consecutive columns into one
+// 32-bit store; columns past nel are written as zero (padding).
+__device__ __forceinline__ void store_planes(int8_t *__restrict__ dst, const uint64_t *work,
+                                             size_t rows, size_t row, size_t stride, size_t i0,
+                                             int nel, int tid) {
+    for (int c = tid; c < 64 * (EL / 4); c += NT) {
+        const int s = c >> 5, e4 = (c & 31) * 4;
+        int d[4][7];
+#pragma unroll
+        for (int q4 = 0; q4 < 4; q4++) {
+            const int e = e4 + q4;
+            const uint64_t x = e < nel ? d2u(bitsd(work[s * EL + e])) : 0;
+            digits7(x, d[q4]);
+        }
+        int8_t *base = dst + ((size_t)s * rows * 7 + row * 7) * stride + i0 + e4;
+#pragma unroll
+        for (int u = 0; u < 7; u++) {
+            const uint32_t packed = (uint32_t)(uint8_t)d[0][u] | ((uint32_t)(uint8_t)d[1][u] << 8) |
+                                    ((uint32_t)(uint8_t)d[2][u] << 16) |
+                                    ((uint32_t)(uint8_t)d[3][u] << 24);
+            *reinterpret_cast<uint32_t *>(base + (size_t)u * stride) = packed;
+        }
+    }
+}
+
 // grid (ceil(m / 128), rows).  Source row `row` starts at
 // (row % row_lo) * stride_lo + (row / row_lo) * stride_hi (u64 units), which
 // covers plain matrices and the halves of split (split.rs:21-28).
+// PLANES = false: exact doubles dst[(s * rows + row) * m + i];
+// PLANES = true: int8 digit planes (store_planes) with row length `pstride`.
+template <bool PLANES>
 __global__ __launch_bounds__(NT) void k_ntt_full_sm(const uint64_t *__restrict__ src, size_t rows,
                                                     size_t m, size_t row_lo, size_t stride_lo,
-                                                    size_t stride_hi, double *__restrict__ dst,
-                                                    const double *__restrict__ tab, Mod M) {
+                                                    size_t stride_hi, void *__restrict__ dst,
+                                                    size_t pstride, const double *__restrict__ tab,
+                                                    Mod M) {
     __shared__ uint64_t lds[EL * SROW];
     const size_t row = blockIdx.y;
     const size_t i0 = (size_t)blockIdx.x * EL;
-    const int nel = (int)min((size_t)EL, m - i0);
+    const int nel = i0 < m ? (int)min((size_t)EL, m - i0) : 0;
     const int tid = threadIdx.x, e = tid >> 3, t = tid & 7;
     const uint64_t *base = src + (row % row_lo) * stride_lo + (row / row_lo) * stride_hi;
     load_block<false>(lds, base + i0 * kD, nel, tid);
@@ -240,7 +284,10 @@ __global__ __launch_bounds__(NT) void k_ntt_full_sm(const uint64_t *__restrict__
         for (int k = 0; k < 8; k++) v[k] = u2d(lds[e * SROW + t + 8 * k]);
     }
     full_fwd_to_work(v, lds, lds, e, t, live, tab, M);
-    store_slotmajor(dst, lds, rows, row, m, i0, nel, tid);
+    if (PLANES)
+        store_planes((int8_t *)dst, lds, rows, row, pstride, i0, nel, tid);
+    else
+        store_slotmajor((double *)dst, lds, rows, row, m, i0, nel, tid);
 }
 
 // signed digit t of x (arithmetic.rs:1496-1570): x is split into pos (2x <= q)
@@ -257,17 +304,19 @@ __device__ __forceinline__ uint64_t limb_of(uint64_t x, uint64_t q, unsigned lg,
 // grid (ceil(m / 128), r).  Row rho of W (Coefficient) -> limb rows
 // rho * parts + tl of new_W (Coefficient) and of dst (slot-major full NTT,
 // rows = r * parts in total).
+template <bool PLANES>
 __global__ __launch_bounds__(NT) void k_decomp_sm(const uint64_t *__restrict__ W, size_t r,
                                                   size_t m, size_t parts,
                                                   const uint64_t *__restrict__ d_radix,
                                                   uint64_t *__restrict__ new_W,
-                                                  double *__restrict__ dst, uint64_t q,
-                                                  const double *__restrict__ tab, Mod M) {
+                                                  void *__restrict__ dst, size_t pstride,
+                                                  uint64_t q, const double *__restrict__ tab,
+                                                  Mod M) {
     __shared__ uint64_t raw[EL * SROW];
     __shared__ uint64_t work[EL * SROW];
     const size_t rho = blockIdx.y;
     const size_t i0 = (size_t)blockIdx.x * EL;
-    const int nel = (int)min((size_t)EL, m - i0);
+    const int nel = i0 < m ? (int)min((size_t)EL, m - i0) : 0;
     const int tid = threadIdx.x, e = tid >> 3, t = tid & 7;
     const uint64_t radix = *d_radix;
     const unsigned lg = (unsigned)__ffsll((long long)radix) - 1u;
@@ -293,7 +342,10 @@ __global__ __launch_bounds__(NT) void k_decomp_sm(const uint64_t *__restrict__ W
             for (int k = 0; k < 8; k++) v[k] = u2d(limb_of(raw[e * SROW + t + 8 * k], q, lg, mask, tl));
         }
         full_fwd_to_work(v, work, work, e, t, live, tab, M);
-        store_slotmajor(dst, work, rows, orow, m, i0, nel, tid);
+        if (PLANES)
+            store_planes((int8_t *)dst, work, rows, orow, pstride, i0, nel, tid);
+        else
+            store_slotmajor((double *)dst, work, rows, orow, m, i0, nel, tid);
         __syncthreads();  // work is rewritten by the next limb
     }
 }
@@ -330,20 +382,42 @@ int launch_ntt_full_to_slotmajor(Context *ctx, const uint64_t *src, size_t rows,
     SALSAA_CHECK(rows < 65536, "too many rows for one launch");
     Mod M{ctx->qd, ctx->qinv};
     dim3 grid(ceil_div(m, EL), (unsigned)rows);
-    hipLaunchKernelGGL(k_ntt_full_sm, grid, dim3(NT), 0, ctx->stream, src, rows, m, row_lo,
-                       stride_lo, stride_hi, dst, ctx->d_tables, M);
+    hipLaunchKernelGGL(k_ntt_full_sm<false>, grid, dim3(NT), 0, ctx->stream, src, rows, m, row_lo,
+                       stride_lo, stride_hi, (void *)dst, (size_t)0, ctx->d_tables, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_ntt_full_to_planes(Context *ctx, const uint64_t *src, size_t rows, size_t m,
+                              size_t row_lo, size_t stride_lo, size_t stride_hi, int8_t *dst,
+                              size_t pstride) {
+    if (rows == 0 || m == 0) return SALSAA_OK;
+    SALSAA_CHECK(rows < 65536, "too many rows for one launch");
+    SALSAA_CHECK(pstride % EL == 0 && pstride >= m, "plane stride must be a multiple of 128 >= m");
+    Mod M{ctx->qd, ctx->qinv};
+    dim3 grid((unsigned)(pstride / EL), (unsigned)rows);
+    hipLaunchKernelGGL(k_ntt_full_sm<true>, grid, dim3(NT), 0, ctx->stream, src, rows, m, row_lo,
+                       stride_lo, stride_hi, (void *)dst, pstride, ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }
 
 int launch_decomp_to_slotmajor(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
-                               const uint64_t *d_radix, uint64_t *new_W, double *dst) {
+                               const uint64_t *d_radix, uint64_t *new_W, void *dst, int planes,
+                               size_t pstride) {
     if (r == 0 || m == 0) return SALSAA_OK;
     SALSAA_CHECK(r < 65536, "too many rows for one launch");
     Mod M{ctx->qd, ctx->qinv};
-    dim3 grid(ceil_div(m, EL), (unsigned)r);
-    hipLaunchKernelGGL(k_decomp_sm, grid, dim3(NT), 0, ctx->stream, W, r, m, parts, d_radix, new_W,
-                       dst, ctx->q, ctx->d_tables, M);
+    if (planes) {
+        SALSAA_CHECK(pstride % EL == 0 && pstride >= m, "plane stride must be a multiple of 128");
+        dim3 grid((unsigned)(pstride / EL), (unsigned)r);
+        hipLaunchKernelGGL(k_decomp_sm<true>, grid, dim3(NT), 0, ctx->stream, W, r, m, parts,
+                           d_radix, new_W, dst, pstride, ctx->q, ctx->d_tables, M);
+    } else {
+        dim3 grid(ceil_div(m, EL), (unsigned)r);
+        hipLaunchKernelGGL(k_decomp_sm<false>, grid, dim3(NT), 0, ctx->stream, W, r, m, parts,
+                           d_radix, new_W, dst, (size_t)0, ctx->q, ctx->d_tables, M);
+    }
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }

@@ -127,6 +127,12 @@ int launch_commit_mac(Context *ctx, const double *A_sm, size_t kappa, size_t mA,
     hipLaunchKernelGGL(k_commit_mac, grid, dim3(CM_THREADS), 0, ctx->stream, A_sm, (int)kappa, mA,
                        W_sm, (int)r, m, chunk, partial, M);
     SALSAA_HIP(hipGetLastError());
+    return launch_commit_finalize(ctx, partial, ksplit, kappa, r, out);
+}
+
+int launch_commit_finalize(Context *ctx, const double *partial, size_t ksplit, size_t kappa,
+                           size_t r, uint64_t *out) {
+    Mod M{ctx->qd, ctx->qinv};
     hipLaunchKernelGGL(k_commit_finalize, dim3((unsigned)(kappa * r)), dim3(64), 0, ctx->stream,
                        partial, (int)ksplit, (int)kappa, (int)r, out, ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());

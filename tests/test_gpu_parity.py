@@ -261,3 +261,42 @@ def test_cfg2_norm_check_full_size(ctx, orc):
     got_msgs = outs["msgs"].download()[:, :, [0, 5]]
     assert np.array_equal(got_msgs, want["msgs"])
     assert np.array_equal(outs["evals"].download()[:, [0, 5]], want["evals"])
+
+
+def test_commit_valu_path_matches_oracle(sa, orc, monkeypatch):
+    # the fp64 VALU commit (selected per context) against the oracle, and the
+    # default int8-MFMA path against it on the same inputs
+    say("case: commit VALU path")
+    kappa, r, m = 8, 5, 300
+    A = rand_ring(orc, (kappa, m, 64), 3, 0xA)
+    W = rand_ring(orc, (r, m, 64), 3, 0xB, 16)
+    want = orc.commit(A, W, Q, threads=8)
+    monkeypatch.setenv("SALSAA_COMMIT", "valu")
+    cv = sa.Context(Q, 64)
+    kv = cv.key(cv.to_device(A))
+    got_v = cv.parallel_dot_series_matrix(kv, cv.to_device(W)).download()
+    limbs_v = cv.decomp_ell(kv, cv.to_device(W), 3, 0)[1].download()
+    kv.close()
+    cv.close()
+    monkeypatch.delenv("SALSAA_COMMIT")
+    cm = sa.Context(Q, 64)
+    km = cm.key(cm.to_device(A))
+    got_m = cm.parallel_dot_series_matrix(km, cm.to_device(W)).download()
+    limbs_m = cm.decomp_ell(km, cm.to_device(W), 3, 0)[1].download()
+    km.close()
+    cm.close()
+    assert np.array_equal(got_v, want)
+    assert np.array_equal(got_m, want)
+    assert np.array_equal(limbs_v, limbs_m)
+
+
+@pytest.mark.parametrize("kappa,r,m", [(40, 3, 200), (32, 48, 130)])
+def test_commit_large_tiles_vs_oracle(ctx, orc, kappa, r, m):
+    # more than 36 key rows (several M blocks) and more than 8 witness rows
+    say("case: commit tiles kappa=%d r=%d m=%d" % (kappa, r, m))
+    A = rand_ring(orc, (kappa, m, 64), 8, 0xA)
+    W = rand_ring(orc, (r, m, 64), 8, 0xB)
+    key = ctx.key(ctx.to_device(A))
+    got = ctx.parallel_dot_series_matrix(key, ctx.to_device(W)).download()
+    assert np.array_equal(got, orc.commit(A, W, Q, threads=8))
+    key.close()
