@@ -139,6 +139,32 @@ int scratch_get(Context *ctx, int slot, size_t bytes, void **out) {
     return SALSAA_OK;
 }
 
+__global__ void k_fill_zero(uint64_t *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+int zero_row(Context *ctx, size_t bytes, const int8_t **out) {
+    ScratchBuf &b = ctx->zeros;
+    if (b.bytes < bytes) {
+        if (b.ptr) SALSAA_HIP(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+        size_t want = (bytes + 4095) / 4096 * 4096 * 2;
+        hipError_t e = hipMalloc(&b.ptr, want);
+        if (e != hipSuccess) {
+            set_error(std::string("zero buffer allocation failed: ") + hipGetErrorString(e));
+            return SALSAA_ERR_NOMEM;
+        }
+        hipLaunchKernelGGL(k_fill_zero, dim3(ceil_div(want / 8, 256)), dim3(256), 0, ctx->stream,
+                           (uint64_t *)b.ptr, want / 8);
+        SALSAA_HIP(hipGetLastError());
+        b.bytes = want;
+    }
+    *out = (const int8_t *)b.ptr;
+    return SALSAA_OK;
+}
+
 }  // namespace salsaa
 
 using namespace salsaa;
@@ -202,6 +228,7 @@ int salsaa_ctx_destroy(salsaa_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->c.stream);
     for (auto &b : ctx->c.scratch)
         if (b.ptr) (void)hipFree(b.ptr);
+    if (ctx->c.zeros.ptr) (void)hipFree(ctx->c.zeros.ptr);
     (void)hipFree(ctx->c.d_tables);
     (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
@@ -327,7 +354,7 @@ int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t
     k->k.m = m;
     int s = SALSAA_OK;
     if (ctx->c.mfma_commit) {
-        k->k.pstride = (m + 127) / 128 * 128;
+        k->k.pstride = plane_stride(m);
         hipError_t e = hipMalloc(&k->k.d_AB, (size_t)kD * kappa * 7 * k->k.pstride);
         if (e != hipSuccess) {
             delete k;
@@ -372,7 +399,7 @@ static int commit_rows(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d
     SALSAA_CHECK(m >= 1 && m <= key->k.m, "witness longer than the key");
     SALSAA_CHECK(rows >= 1, "commit needs at least one witness row");
     if (ctx->c.mfma_commit) {
-        const size_t ps = (m + 127) / 128 * 128;
+        const size_t ps = plane_stride(m);
         int8_t *wb = nullptr;
         SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * rows * 7 * ps, (void **)&wb));
         SALSAA_TRY(launch_ntt_full_to_planes(&ctx->c, d_W, rows, m, row_lo, stride_lo, stride_hi,
@@ -428,7 +455,7 @@ int salsaa_decomp_ell(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_
     // one pass over W writes the limb witness and the limbs' slot-major NTT
     const size_t rows = r * parts;
     if (ctx->c.mfma_commit) {
-        const size_t ps = (m + 127) / 128 * 128;
+        const size_t ps = plane_stride(m);
         int8_t *wb = nullptr;
         SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * rows * 7 * ps, (void **)&wb));
         SALSAA_TRY(launch_decomp_to_slotmajor(&ctx->c, d_W, r, m, parts, d_radix, d_new_W, wb, 1,

@@ -76,6 +76,7 @@ struct Context {
     std::vector<double> h_tables;   // host copy
     ScratchBuf scratch[8];          // grown on demand, owned by the context
     bool mfma_commit = true;        // int8 MFMA commit (false: fp64 VALU commit)
+    ScratchBuf zeros;               // zero-filled, grown on demand
     std::mutex lock;                // one host thread drives a context at a time
 };
 
@@ -84,10 +85,18 @@ struct Key {
     size_t kappa = 0, m = 0;
     double *d_A = nullptr;   // slot-major full-NTT [64][kappa][m] as exact fp64 (VALU path)
     int8_t *d_AB = nullptr;  // digit planes [64][kappa * 7][pstride] (MFMA path)
-    size_t pstride = 0;      // plane row length: m rounded up to 128
+    size_t pstride = 0;      // plane row length (plane_stride(m))
 };
 
+// Digit-plane row length for a witness of length m: m rounded up to 128 plus
+// 64 bytes, so consecutive rows are an odd multiple of 64 bytes apart and the
+// 16+ rows a wave streams concurrently spread over HBM channels instead of
+// all sitting at the same offset modulo a power of two.
+inline size_t plane_stride(size_t m) { return (m + 127) / 128 * 128 + 64; }
+
 int scratch_get(Context *ctx, int slot, size_t bytes, void **out);
+// A device buffer of at least `bytes` zero bytes owned by the context.
+int zero_row(Context *ctx, size_t bytes, const int8_t **out);
 
 // ---------------------------------------------------------------- host arithmetic
 inline uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t q) {

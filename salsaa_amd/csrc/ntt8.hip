@@ -218,41 +218,37 @@ __device__ __forceinline__ void store_slotmajor(double *__restrict__ dst, const 
     }
 }
 
-// Signed base-256 digits of a residue x < 2^50: x = sum_u d_u 256^u with
-// d_u in [-128, 127], 7 digits (the top one is in [0, 4]).
-__device__ __forceinline__ void digits7(uint64_t x, int (&d)[7]) {
-    long long y = (long long)x;
-#pragma unroll
-    for (int u = 0; u < 7; u++) {
-        int b = (int)(y & 255);
-        b -= (b >= 128) ? 256 : 0;
-        d[u] = b;
-        y = (y - b) >> 8;
-    }
-}
+// Balanced base-256 digits of a residue x < 2^50: x = sum_u d_u 256^u with
+// d_u in [-128, 127] (7 digits).  With y = x + sum_u 128 * 256^u the bytes
+// b_u of y give d_u = b_u - 128, whose int8 bit pattern is b_u XOR 0x80.
+constexpr uint64_t DIGIT_BIAS = 0x0080808080808080ULL;
 
 // byte-plane store for the int8 MFMA commit: value (s, row, i) contributes
 // digit u to plane row (s, row * 7 + u), column i (row length `stride`,
-// a multiple of 128).  Each thread packs 4 consecutive columns into one
-// 32-bit store; columns past nel are written as zero (padding).
+// a multiple of 64 covering m rounded up to 128).  Each thread packs 4
+// consecutive columns into one 32-bit store; columns past nel are written as
+// zero, so the plane is zero from m up to the next multiple of 128.
 __device__ __forceinline__ void store_planes(int8_t *__restrict__ dst, const uint64_t *work,
                                              size_t rows, size_t row, size_t stride, size_t i0,
                                              int nel, int tid) {
     for (int c = tid; c < 64 * (EL / 4); c += NT) {
         const int s = c >> 5, e4 = (c & 31) * 4;
-        int d[4][7];
+        uint32_t lo[4], hi[4];
 #pragma unroll
         for (int q4 = 0; q4 < 4; q4++) {
             const int e = e4 + q4;
-            const uint64_t x = e < nel ? d2u(bitsd(work[s * EL + e])) : 0;
-            digits7(x, d[q4]);
+            uint64_t y = 0;  // zero padding: every digit 0
+            if (e < nel) y = (d2u(bitsd(work[s * EL + e])) + DIGIT_BIAS) ^ 0x8080808080808080ULL;
+            lo[q4] = (uint32_t)y;
+            hi[q4] = (uint32_t)(y >> 32);
         }
         int8_t *base = dst + ((size_t)s * rows * 7 + row * 7) * stride + i0 + e4;
 #pragma unroll
         for (int u = 0; u < 7; u++) {
-            const uint32_t packed = (uint32_t)(uint8_t)d[0][u] | ((uint32_t)(uint8_t)d[1][u] << 8) |
-                                    ((uint32_t)(uint8_t)d[2][u] << 16) |
-                                    ((uint32_t)(uint8_t)d[3][u] << 24);
+            const int sh = 8 * (u & 3);
+            const uint32_t *src = u < 4 ? lo : hi;
+            const uint32_t packed = ((src[0] >> sh) & 0xFF) | (((src[1] >> sh) & 0xFF) << 8) |
+                                    (((src[2] >> sh) & 0xFF) << 16) | (((src[3] >> sh) & 0xFF) << 24);
             *reinterpret_cast<uint32_t *>(base + (size_t)u * stride) = packed;
         }
     }
@@ -393,9 +389,10 @@ int launch_ntt_full_to_planes(Context *ctx, const uint64_t *src, size_t rows, si
                               size_t pstride) {
     if (rows == 0 || m == 0) return SALSAA_OK;
     SALSAA_CHECK(rows < 65536, "too many rows for one launch");
-    SALSAA_CHECK(pstride % EL == 0 && pstride >= m, "plane stride must be a multiple of 128 >= m");
+    SALSAA_CHECK(pstride % 64 == 0 && pstride >= (m + EL - 1) / EL * EL,
+                 "plane stride must be a multiple of 64 covering m rounded up to 128");
     Mod M{ctx->qd, ctx->qinv};
-    dim3 grid((unsigned)(pstride / EL), (unsigned)rows);
+    dim3 grid(ceil_div(m, EL), (unsigned)rows);
     hipLaunchKernelGGL(k_ntt_full_sm<true>, grid, dim3(NT), 0, ctx->stream, src, rows, m, row_lo,
                        stride_lo, stride_hi, (void *)dst, pstride, ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());
@@ -409,8 +406,9 @@ int launch_decomp_to_slotmajor(Context *ctx, const uint64_t *W, size_t r, size_t
     SALSAA_CHECK(r < 65536, "too many rows for one launch");
     Mod M{ctx->qd, ctx->qinv};
     if (planes) {
-        SALSAA_CHECK(pstride % EL == 0 && pstride >= m, "plane stride must be a multiple of 128");
-        dim3 grid((unsigned)(pstride / EL), (unsigned)r);
+        SALSAA_CHECK(pstride % 64 == 0 && pstride >= (m + EL - 1) / EL * EL,
+                     "plane stride must be a multiple of 64 covering m rounded up to 128");
+        dim3 grid(ceil_div(m, EL), (unsigned)r);
         hipLaunchKernelGGL(k_decomp_sm<true>, grid, dim3(NT), 0, ctx->stream, W, r, m, parts,
                            d_radix, new_W, dst, pstride, ctx->q, ctx->d_tables, M);
     } else {

@@ -24,11 +24,20 @@ namespace {
 constexpr int SC_THREADS = 256;
 constexpr int SC_FINAL_MAX = 1024;
 
-// ---- field helpers: F_q (FQ2 = false, imaginary parts constant zero) or F_{q^2}
-template <bool FQ2>
+// ---- field helpers: F_q (FQ2 = false, imaginary parts constant zero) or
+// F_{q^2}.  SNR: the non-residue is below 8, so nr * t for a lazy product
+// |t| < q is an exact double product (|nr t| < 8q < 2^53) and needs no
+// reduction of its own.
+template <bool SNR>
+__device__ __forceinline__ double nr_times(double nr, double t, Mod M) {
+    return SNR ? nr * t : mul_lazy(nr, t, M);
+}
+template <bool FQ2, bool SNR>
 __device__ __forceinline__ F2 x_mul(F2 x, F2 y, double nr, Mod M) {
     if (!FQ2) return {mulmod(x.a, y.a, M), 0.0};
-    return f2_mul(x, y, nr, M);
+    const double re = mul_lazy(x.a, y.a, M) + nr_times<SNR>(nr, mul_lazy(x.b, y.b, M), M);
+    const double im = mul_lazy(x.a, y.b, M) + mul_lazy(x.b, y.a, M);
+    return {reduce(re, M), reduce(im, M)};
 }
 template <bool FQ2>
 __device__ __forceinline__ F2 x_add(F2 x, F2 y, Mod M) {
@@ -41,14 +50,26 @@ __device__ __forceinline__ F2 x_sub(F2 x, F2 y, Mod M) {
     return f2_sub(x, y, M);
 }
 // acc += x * y, lazily (|acc| stays below q/2 + 1 after each step)
-template <bool FQ2>
+template <bool FQ2, bool SNR>
 __device__ __forceinline__ void x_mac(F2 &acc, F2 x, F2 y, double nr, Mod M) {
     if (!FQ2) {
         acc.a = reduce_lazy(acc.a + mul_lazy(x.a, y.a, M), M);
         return;
     }
-    double re = mul_lazy(x.a, y.a, M) + mul_lazy(nr, mul_lazy(x.b, y.b, M), M);
-    double im = mul_lazy(x.a, y.b, M) + mul_lazy(x.b, y.a, M);
+    const double re = mul_lazy(x.a, y.a, M) + nr_times<SNR>(nr, mul_lazy(x.b, y.b, M), M);
+    const double im = mul_lazy(x.a, y.b, M) + mul_lazy(x.b, y.a, M);
+    acc.a = reduce_lazy(acc.a + re, M);
+    acc.b = reduce_lazy(acc.b + im, M);
+}
+// acc += x^2 = (a^2 + nr b^2) + 2ab Y
+template <bool FQ2, bool SNR>
+__device__ __forceinline__ void x_sq_mac(F2 &acc, F2 x, double nr, Mod M) {
+    if (!FQ2) {
+        acc.a = reduce_lazy(acc.a + mul_lazy(x.a, x.a, M), M);
+        return;
+    }
+    const double re = mul_lazy(x.a, x.a, M) + nr_times<SNR>(nr, mul_lazy(x.b, x.b, M), M);
+    const double im = 2.0 * mul_lazy(x.a, x.b, M);
     acc.a = reduce_lazy(acc.a + re, M);
     acc.b = reduce_lazy(acc.b + im, M);
 }
@@ -99,7 +120,7 @@ __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) 
 // ---------------------------------------------------------------- first pass
 // F_q input w [rows][L], rounds 0..3 for REPS repetitions (reps rep0..).
 // Round 0's messages and the norm_check_1 claim do not depend on challenges.
-template <int REPS, bool FQ2>
+template <int REPS, bool FQ2, bool SNR>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
     const uint64_t *__restrict__ w, size_t L, int rows, int rep0, int reps_total,
     int add_claim, const uint64_t *__restrict__ cs, int nrounds, double nr, int cpt,
@@ -162,10 +183,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
                 for (int y = 0; y < half; y++) {
                     F2 w0 = cur[y];
                     F2 d = x_sub<FQ2>(cur[y + half], w0, M);
-                    x_mac<FQ2>(acc[r][t - 1][0], w0, w0, nr, M);
-                    x_mac<FQ2>(acc[r][t - 1][1], w0, d, nr, M);
-                    x_mac<FQ2>(acc[r][t - 1][2], d, d, nr, M);
-                    cur[y] = x_add<FQ2>(w0, x_mul<FQ2>(c[r][t], d, nr, M), M);
+                    x_sq_mac<FQ2, SNR>(acc[r][t - 1][0], w0, nr, M);
+                    x_mac<FQ2, SNR>(acc[r][t - 1][1], w0, d, nr, M);
+                    x_sq_mac<FQ2, SNR>(acc[r][t - 1][2], d, nr, M);
+                    cur[y] = x_add<FQ2>(w0, x_mul<FQ2, SNR>(c[r][t], d, nr, M), M);
                 }
             }
             double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
@@ -205,7 +226,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
 
 // ---------------------------------------------------------------- later passes
 // F2 input in [reps][rows][len] (pairs of doubles), rounds t0..t0+3.
-template <bool FQ2>
+template <bool FQ2, bool SNR>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
     const double *__restrict__ in, size_t len, int rows, int t0, const uint64_t *__restrict__ cs,
     int nrounds, double nr, int cpt, double *__restrict__ out,
@@ -239,10 +260,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
             for (int y = 0; y < half; y++) {
                 F2 w0 = cur[y];
                 F2 d = x_sub<FQ2>(cur[y + half], w0, M);
-                x_mac<FQ2>(acc[t][0], w0, w0, nr, M);
-                x_mac<FQ2>(acc[t][1], w0, d, nr, M);
-                x_mac<FQ2>(acc[t][2], d, d, nr, M);
-                cur[y] = x_add<FQ2>(w0, x_mul<FQ2>(c[t], d, nr, M), M);
+                x_sq_mac<FQ2, SNR>(acc[t][0], w0, nr, M);
+                x_mac<FQ2, SNR>(acc[t][1], w0, d, nr, M);
+                x_sq_mac<FQ2, SNR>(acc[t][2], d, nr, M);
+                cur[y] = x_add<FQ2>(w0, x_mul<FQ2, SNR>(c[t], d, nr, M), M);
             }
         }
         double *o = out + (((size_t)rep * rows + row) * S + x) * 2;
@@ -268,7 +289,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
 
 // ---------------------------------------------------------------- final block
 // One block per (row, rep): remaining rounds t0.. in LDS, norm_check_3, verifier.
-template <bool FQ2, bool FIRST>
+template <bool FQ2, bool FIRST, bool SNR>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
     const uint64_t *__restrict__ w, const double *__restrict__ in, size_t len, int rows, int t0,
     const uint64_t *__restrict__ cs, int nrounds, double nr,
@@ -284,8 +305,13 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
     __shared__ double red[8 * 64];
     __shared__ double msg_local[32][6];
     __shared__ double claim_local;
+    __shared__ F2 chal[48];
     const int row = blockIdx.x, rep = blockIdx.y;
     const int tid = threadIdx.x;
+    // this repetition's challenges, loaded once (the round loop and the
+    // verifier replay below would otherwise wait on one load per round)
+    for (int tt = tid; tt < nrounds; tt += SC_THREADS) chal[tt] = load_chal<FQ2>(cs, rep, nrounds, tt);
+    __syncthreads();
 
     for (size_t x = tid; x < len; x += SC_THREADS) {
         if (FIRST) {
@@ -316,15 +342,15 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
             for (int x = tid; x < kD; x += SC_THREADS) V[x] = cur[x];
         }
         const size_t half = cl >> 1;
-        const F2 c = load_chal<FQ2>(cs, rep, nrounds, t);
+        const F2 c = chal[t];
         F2 a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, a2 = {0.0, 0.0};
         for (size_t x = tid; x < half; x += SC_THREADS) {
             F2 w0 = cur[x];
             F2 d = x_sub<FQ2>(cur[x + half], w0, M);
-            x_mac<FQ2>(a0, w0, w0, nr, M);
-            x_mac<FQ2>(a1, w0, d, nr, M);
-            x_mac<FQ2>(a2, d, d, nr, M);
-            cur[x] = x_add<FQ2>(w0, x_mul<FQ2>(c, d, nr, M), M);
+            x_sq_mac<FQ2, SNR>(a0, w0, nr, M);
+            x_mac<FQ2, SNR>(a1, w0, d, nr, M);
+            x_sq_mac<FQ2, SNR>(a2, d, nr, M);
+            cur[x] = x_add<FQ2>(w0, x_mul<FQ2, SNR>(c, d, nr, M), M);
         }
         double vals[6] = {reduce(a0.a, M), reduce(a0.b, M), reduce(a1.a, M),
                           reduce(a1.b, M), reduce(a2.a, M), reduce(a2.b, M)};
@@ -341,10 +367,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
         if (tid < kD) {
             F2 tk = {1.0, 0.0};
             for (int b = 0; b < 6; b++) {
-                const F2 cb = load_chal<FQ2>(cs, rep, nrounds, nrounds - 6 + b);
+                const F2 cb = chal[nrounds - 6 + b];
                 const int bit = (tid >> (5 - b)) & 1;
                 F2 f = bit ? cb : x_sub<FQ2>(F2{1.0, 0.0}, cb, M);
-                tk = x_mul<FQ2>(tk, f, nr, M);
+                tk = x_mul<FQ2, SNR>(tk, f, nr, M);
             }
             // conj: base[0] = T[0], base[i] = -T[d - i]; thread tid owns T[tid]
             const int dst = tid == 0 ? 0 : kD - tid;
@@ -364,7 +390,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
                 if (!FQ2) {
                     p0 = mul_lazy(b.a, v.a, M);
                 } else {
-                    p0 = mul_lazy(b.a, v.a, M) + mul_lazy(nr, mul_lazy(b.b, v.b, M), M);
+                    p0 = mul_lazy(b.a, v.a, M) + nr_times<SNR>(nr, mul_lazy(b.b, v.b, M), M);
                     p1 = mul_lazy(b.a, v.b, M) + mul_lazy(b.b, v.a, M);
                 }
                 r0 = reduce_lazy(wrap ? r0 - p0 : r0 + p0, M);
@@ -415,18 +441,18 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
             const F2 lhs = x_add<true>(x_add<true>(x_add<true>(f0, f0, M), x_add<true>(f1, f1, M), M),
                                        f2, M);
             if (lhs.a != cl2.a || lhs.b != cl2.b) good = 0;
-            const F2 c = load_chal<FQ2>(cs, rep, nrounds, tt);
-            const F2 c2 = x_mul<true>(c, c, nr, M);
-            cl2 = x_add<true>(x_add<true>(f0, x_mul<true>(x_add<true>(f1, f1, M), c, nr, M), M),
-                              x_mul<true>(f2, c2, nr, M), M);
+            const F2 c = chal[tt];
+            const F2 c2 = x_mul<true, SNR>(c, c, nr, M);
+            cl2 = x_add<true>(x_add<true>(f0, x_mul<true, SNR>(x_add<true>(f1, f1, M), c, nr, M), M),
+                              x_mul<true, SNR>(f2, c2, nr, M), M);
         }
-        const F2 ev2 = x_mul<true>(ev, ev, nr, M);
+        const F2 ev2 = x_mul<true, SNR>(ev, ev, nr, M);
         if (ev2.a != cl2.a || ev2.b != cl2.b) good = 0;
         if (want_rhs) {
             // ct(rhs)^2 == final claim (verify_norm_check_3): ct from the
             // coefficient form computed above
             const F2 ct = {rc[0][0], rc[1][0]};
-            const F2 ct2 = x_mul<true>(ct, ct, nr, M);
+            const F2 ct2 = x_mul<true, SNR>(ct, ct, nr, M);
             if (ct2.a != cl2.a || ct2.b != cl2.b) good = 0;
         }
         const size_t o = (size_t)rep * rows + row;
@@ -447,7 +473,7 @@ __global__ void k_zero_u64(unsigned long long *p, size_t n) {
     if (i < n) p[i] = 0ULL;
 }
 
-template <bool FQ2>
+template <bool FQ2, bool SNR>
 int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L,
                     const uint64_t *cs, size_t reps, double nr, uint64_t *claims,
                     uint64_t *msgs, uint64_t *finals, uint64_t *evals, uint64_t *rhs,
@@ -463,7 +489,7 @@ int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L,
                        macc, nmacc + rows);
     SALSAA_HIP(hipGetLastError());
     if (L <= (size_t)SC_FINAL_MAX) {
-        hipLaunchKernelGGL((k_sc_final<FQ2, true>), dim3((unsigned)rows, (unsigned)reps),
+        hipLaunchKernelGGL((k_sc_final<FQ2, true, SNR>), dim3((unsigned)rows, (unsigned)reps),
                            dim3(SC_THREADS), 0, ctx->stream, w, nullptr, L, (int)rows, 0, cs, n, nr,
                            macc, cacc, claims, msgs, finals, evals, rhs, ok, want_rhs, ctx->q,
                            ctx->d_tables, M);
@@ -484,11 +510,11 @@ int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L,
             const int nrep = (reps - r0) >= 2 ? 2 : 1;
             dim3 grid(bx, (unsigned)rows);
             if (nrep == 2)
-                hipLaunchKernelGGL((k_sc_first<2, FQ2>), grid, dim3(SC_THREADS), 0, ctx->stream, w,
+                hipLaunchKernelGGL((k_sc_first<2, FQ2, SNR>), grid, dim3(SC_THREADS), 0, ctx->stream, w,
                                    L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, n, nr, cpt, bufA,
                                    macc, cacc, M);
             else
-                hipLaunchKernelGGL((k_sc_first<1, FQ2>), grid, dim3(SC_THREADS), 0, ctx->stream, w,
+                hipLaunchKernelGGL((k_sc_first<1, FQ2, SNR>), grid, dim3(SC_THREADS), 0, ctx->stream, w,
                                    L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, n, nr, cpt, bufA,
                                    macc, cacc, M);
             SALSAA_HIP(hipGetLastError());
@@ -501,7 +527,7 @@ int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L,
         const size_t S = len >> 4;
         const int cpt = S >= (size_t)SC_THREADS * 64 ? 4 : 1;
         dim3 grid(ceil_div(S, (size_t)SC_THREADS * cpt), (unsigned)rows, (unsigned)reps);
-        hipLaunchKernelGGL((k_sc_pass<FQ2>), grid, dim3(SC_THREADS), 0, ctx->stream, src, len,
+        hipLaunchKernelGGL((k_sc_pass<FQ2, SNR>), grid, dim3(SC_THREADS), 0, ctx->stream, src, len,
                            (int)rows, t0, cs, n, nr, cpt, dst, macc, M);
         SALSAA_HIP(hipGetLastError());
         len = S;
@@ -510,7 +536,7 @@ int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L,
         src = dst;
         dst = tmp;
     }
-    hipLaunchKernelGGL((k_sc_final<FQ2, false>), dim3((unsigned)rows, (unsigned)reps),
+    hipLaunchKernelGGL((k_sc_final<FQ2, false, SNR>), dim3((unsigned)rows, (unsigned)reps),
                        dim3(SC_THREADS), 0, ctx->stream, nullptr, src, len, (int)rows, t0, cs, n, nr,
                        macc, cacc, claims, msgs, finals, evals, rhs, ok, want_rhs, ctx->q,
                        ctx->d_tables, M);
@@ -532,11 +558,14 @@ int launch_norm_check(Context *ctx, const uint64_t *w, size_t rows, size_t L,
     SALSAA_CHECK(nr < ctx->q, "non-residue must be reduced mod q");
     SALSAA_CHECK(ok != nullptr, "ok output is required");
     // the final block keeps at most 32 rounds of messages locally
+    if (fq2 && nr < 8)
+        return norm_check_impl<true, true>(ctx, w, rows, L, cs, reps, (double)nr, claims, msgs,
+                                           finals, evals, rhs, ok);
     if (fq2)
-        return norm_check_impl<true>(ctx, w, rows, L, cs, reps, (double)nr, claims, msgs, finals,
-                                     evals, rhs, ok);
-    return norm_check_impl<false>(ctx, w, rows, L, cs, reps, (double)nr, claims, msgs, finals,
-                                  evals, rhs, ok);
+        return norm_check_impl<true, false>(ctx, w, rows, L, cs, reps, (double)nr, claims, msgs,
+                                            finals, evals, rhs, ok);
+    return norm_check_impl<false, true>(ctx, w, rows, L, cs, reps, (double)nr, claims, msgs,
+                                        finals, evals, rhs, ok);
 }
 
 }  // namespace salsaa
