@@ -102,6 +102,18 @@ def algorithmic(step):
     }
 
 
+def traffic_bytes():
+    """HBM bytes per launch of the NTT forward kernel, measured by rocprofv3
+    FETCH_SIZE and WRITE_SIZE (each scaled by a plain-copy calibration) and
+    committed in profiles/ntt_inc_fwd_cfg2_traffic.json."""
+    path = os.path.join(ROOT, "profiles", "ntt_inc_fwd_cfg2_traffic.json")
+    try:
+        with open(path) as f:
+            return int(json.load(f)["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(budget_s=20.0):
     """The oracle (single thread) on a bounded sample of the same step, scaled
     to one full step: each phase runs on a fraction of its work and the time is
@@ -109,14 +121,15 @@ def cpu_baseline(budget_s=20.0):
     from oracle import oracle as orc
     q = 1125899906840833
     t = {}
-    # NTT: 4096 of the 2^20 elements, forward + inverse
-    x = orc.sample_uniform(0, 0xC, (4096, 64), q)
+    # NTT: 65536 of the 2^20 elements, forward + inverse
+    nn = 65536
+    x = orc.sample_uniform(0, 0xC, (nn, 64), q)
     t0 = time.perf_counter()
     y = orc.to_incomplete(x, q)
     orc.from_incomplete(y, q)
-    t["ntt"] = (time.perf_counter() - t0) * (NTT_COUNT / 4096)
-    # commit: full kappa x r, 16 of the 16384 columns (per-product conversion)
-    ms = 16
+    t["ntt"] = (time.perf_counter() - t0) * (NTT_COUNT / nn)
+    # commit: full kappa x r, 64 of the 16384 columns (per-product conversion)
+    ms = 128
     A = orc.sample_uniform(0, 0xA, (KAPPA, ms, 64), q)
     W = orc.sample_bounded(0, 0xB, (R, ms, 64), BOUND, q)
     t0 = time.perf_counter()
@@ -139,8 +152,8 @@ def cpu_baseline(budget_s=20.0):
         "unit": "prover-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": ("oracle, 1 thread: NTT on 4096 of 2^20 elements; commit and decomp_ell "
-                   "with all 32x8 (32x48) rows on 16 of 16384 columns; norm check on 1 of 8 "
+        "sample": ("oracle, 1 thread: NTT on 65536 of 2^20 elements; commit and decomp_ell "
+                   "with all 32x8 (32x48) rows on 128 of 16384 columns; norm check on 1 of 8 "
                    "rows; each phase scaled to the full step (%s)" %
                    ", ".join("%s %.1fs" % kv for kv in t.items())),
     }
@@ -205,9 +218,10 @@ def main():
     ntt_ms = phase_avg["ntt_fwd"]
     ntt_gbs = alg["ntt_fwd"]["bytes"] / (ntt_ms * 1e-3) / 1e9
     peak = 8000.0
-    roofline = {"bound": "hbm", "kernel": "ntt_incomplete_fwd", "achieved": round(ntt_gbs, 1),
-                "peak": peak, "unit": "GB/s", "frac": round(ntt_gbs / peak, 4), "traffic": None,
-                "dominant_phase": dom}
+    roofline = {"bound": "hbm", "kernel": "k_ntt_inc<false> (IncompleteNTT forward)",
+                "achieved": round(ntt_gbs, 1), "peak": peak, "unit": "GB/s",
+                "frac": round(ntt_gbs / peak, 4), "traffic": traffic_bytes(),
+                "algorithmic_bytes": alg["ntt_fwd"]["bytes"], "dominant_phase": dom}
     phases = {}
     for k, v in phase_avg.items():
         a = alg[k]
