@@ -201,6 +201,9 @@ int salsaa_ctx_create(uint64_t q, uint32_t d, salsaa_ctx **out) {
         return SALSAA_ERR_HIP;
     }
     x.device = dev;
+    if (hipDeviceGetAttribute(&x.ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        x.ncu < 1)
+        x.ncu = 256;
     x.q = q;
     x.d = (int)d;
     x.qd = (double)q;
@@ -353,9 +356,9 @@ int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t
     k->k.kappa = kappa;
     k->k.m = m;
     int s = SALSAA_OK;
-    if (ctx->c.mfma_commit) {
-        k->k.pstride = plane_stride(m);
-        hipError_t e = hipMalloc(&k->k.d_AB, (size_t)kD * kappa * 7 * k->k.pstride);
+    if (ctx->c.mfma_commit && kappa <= kMfmaMaxKappa) {
+        k->k.pstride = plane_ktiles(m);
+        hipError_t e = hipMalloc(&k->k.d_AB, plane_bytes(kappa, m));
         if (e != hipSuccess) {
             delete k;
             set_error(std::string("key allocation: ") + hipGetErrorString(e));
@@ -398,10 +401,10 @@ static int commit_rows(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d
     SALSAA_CHECK(key->k.ctx == &ctx->c, "key belongs to another context");
     SALSAA_CHECK(m >= 1 && m <= key->k.m, "witness longer than the key");
     SALSAA_CHECK(rows >= 1, "commit needs at least one witness row");
-    if (ctx->c.mfma_commit) {
-        const size_t ps = plane_stride(m);
+    if (key->k.d_AB) {
+        const size_t ps = plane_ktiles(m);
         int8_t *wb = nullptr;
-        SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * rows * 7 * ps, (void **)&wb));
+        SALSAA_TRY(scratch_get(&ctx->c, 6, plane_bytes(rows, m), (void **)&wb));
         SALSAA_TRY(launch_ntt_full_to_planes(&ctx->c, d_W, rows, m, row_lo, stride_lo, stride_hi,
                                              wb, ps));
         return launch_commit_mfma(&ctx->c, key->k.d_AB, key->k.kappa, key->k.pstride, wb, rows, m,
@@ -454,10 +457,10 @@ int salsaa_decomp_ell(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_
     SALSAA_TRY(launch_radix(&ctx->c, d_W, r, m, parts, radix, d_radix));
     // one pass over W writes the limb witness and the limbs' slot-major NTT
     const size_t rows = r * parts;
-    if (ctx->c.mfma_commit) {
-        const size_t ps = plane_stride(m);
+    if (key->k.d_AB) {
+        const size_t ps = plane_ktiles(m);
         int8_t *wb = nullptr;
-        SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * rows * 7 * ps, (void **)&wb));
+        SALSAA_TRY(scratch_get(&ctx->c, 6, plane_bytes(rows, m), (void **)&wb));
         SALSAA_TRY(launch_decomp_to_slotmajor(&ctx->c, d_W, r, m, parts, d_radix, d_new_W, wb, 1,
                                               ps));
         return launch_commit_mfma(&ctx->c, key->k.d_AB, key->k.kappa, key->k.pstride, wb, rows, m,

@@ -68,6 +68,7 @@ struct ScratchBuf {
 
 struct Context {
     int device = 0;
+    int ncu = 256;                  // compute units of the device
     uint64_t q = 0;
     int d = kD;
     double qd = 0.0, qinv = 0.0;
@@ -84,15 +85,20 @@ struct Key {
     Context *ctx = nullptr;
     size_t kappa = 0, m = 0;
     double *d_A = nullptr;   // slot-major full-NTT [64][kappa][m] as exact fp64 (VALU path)
-    int8_t *d_AB = nullptr;  // digit planes [64][kappa * 7][pstride] (MFMA path)
-    size_t pstride = 0;      // plane row length (plane_stride(m))
+    int8_t *d_AB = nullptr;  // fragment-tiled digit planes (MFMA path, kappa <= kMfmaMaxKappa)
+    size_t pstride = 0;      // K-tiles of the planes (plane_ktiles(m))
 };
 
-// Digit-plane row length for a witness of length m: m rounded up to 128 plus
-// 64 bytes, so consecutive rows are an odd multiple of 64 bytes apart and the
-// 16+ rows a wave streams concurrently spread over HBM channels instead of
-// all sitting at the same offset modulo a power of two.
-inline size_t plane_stride(size_t m) { return (m + 127) / 128 * 128 + 64; }
+// The MFMA commit keeps all of a key's digit rows in one block (16 row tiles).
+constexpr size_t kMfmaMaxKappa = 36;
+
+// Fragment-tiled digit planes (see store_planes in ntt8.hip): K-tiles of 64
+// columns for a witness of length m, m rounded up to 128.
+inline size_t plane_ktiles(size_t m) { return (m + 127) / 128 * 2; }
+// bytes of the planes of a matrix with `rows` value rows
+inline size_t plane_bytes(size_t rows, size_t m) {
+    return (size_t)64 * ((rows * 7 + 15) / 16) * plane_ktiles(m) * 1024;
+}
 
 int scratch_get(Context *ctx, int slot, size_t bytes, void **out);
 // A device buffer of at least `bytes` zero bytes owned by the context.

@@ -3,15 +3,27 @@
 // In the full NTT domain the commit is, for every slot s, C_s = A_s . W_s over
 // F_q with A_s (kappa x m) and W_s (m x r) holding 50-bit residues.  Each
 // residue x is written as 7 signed base-256 digits x = sum_u d_u 256^u,
-// d_u in [-128, 127], stored as int8 planes [64][rows * 7][stride] (row
-// rows*7 + u holds digit u).  Then
+// d_u in [-128, 127]; digit row r*7 + u of a matrix holds digit u of value
+// row r.  Then
 //     C_s[j][k] = sum_{u,v} 256^(u+v) * (A_s,u . W_s,v)[j][k],
 // and the 49 digit products are ONE integer GEMM per slot with M = 7 kappa,
 // N = 7 r, K = m on v_mfma_i32_16x16x64_i8: every digit product is at most
 // 2^14 in magnitude, so int32 accumulation over a K chunk of at most 2^16 is
-// exact.  A block owns a K chunk of one slot (layout below); the int32 tiles
-// go through LDS and each (j, k) is recombined with the weights 2^(8t) mod q
-// in exact fp64 into the same per-slot partial sums the VALU kernel produces.
+// exact.
+//
+// Operands are fragment-tiled (store_planes in ntt8.hip): the 16 digit rows x
+// 64 columns one MFMA reads are one contiguous KiB, lane-ordered, so a wave
+// moves a whole fragment with one 16-byte load per lane, and the row tiles of
+// one K step are consecutive, so a block's loads form one sequential stream
+// per operand.  A block owns one
+// slot, one K chunk, all of the key's digit rows (at most 16 row tiles) and
+// 16 witness rows (7 column tiles).  Per K step its threads stage the step's
+// fragments from HBM through registers into LDS two steps ahead (double
+// buffered), and every wave multiplies its own register tile of up to 4 x 4
+// fragments out of LDS; HBM sees each fragment once per block.  At the end the
+// int32 tiles of each group of 16 key rows go through LDS and each (j, k) is
+// recombined with the weights 2^(8t) mod q in exact fp64 into the per-slot
+// partial sums that k_commit_finalize reduces over K chunks.
 #include "common.hpp"
 
 namespace salsaa {
@@ -20,177 +32,251 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int MJ = 9;        // key rows per M strip (63 digit rows, 4 MFMA row tiles)
-constexpr int NK = 8;        // witness rows per N strip (56 digit rows)
-constexpr int KSTEP = 64;    // K per MFMA
+constexpr int FRAG = 1024;  // bytes of one 16 x 64 int8 fragment
+constexpr int GT = 7;       // tiles per recombination group (16 rows of values)
 
-// Block = 4 M-strip waves x NWN N-groups; a wave owns one M strip (4 row
-// tiles) and NSPW consecutive N strips (NT = ceil(56 NSPW / 16) column tiles:
-// 4 for one strip, 7 for two).  All waves of a block walk the same K range,
-// so the key rows and witness rows they share come from L1 and each byte
-// plane is streamed from HBM once per block.
-template <int NSPW, int NWN>
-__global__ __launch_bounds__(4 * NWN * 64) void k_commit_mfma(
-    const int8_t *__restrict__ AB, int kappa, size_t strideA, const int8_t *__restrict__ WB, int r,
-    size_t strideW, size_t kchunk, size_t kend, int nbx, int mbx, const int8_t *__restrict__ zrow,
-    double *__restrict__ partial, const double *__restrict__ tab, Mod M) {
-    constexpr int NT = (56 * NSPW + 15) / 16;
-    constexpr int NROWS = 56 * NSPW;    // digit columns owned by a wave
-    constexpr int NW = 4 * NWN;         // waves per block
-    constexpr int LDSW = NROWS + 1;
-    __shared__ int Dl[2][64][LDSW];     // 2 waves' tiles at a time
+// TM: row tiles per wave (wave wm owns key tiles wm + 4a, a < TM);
+// WN x TN: witness tiles (wave wn owns tiles wn + WN b of the block's 7).
+template <int TM, int TN, int WN>
+struct CommitShape {
+    static constexpr int NW = 4 * WN, NTHR = 64 * NW;
+    static constexpr int SA = 4 * TM, SB = WN * TN;          // fragment slots per stage
+    static constexpr int STAGE = (SA + SB) * FRAG;
+    static constexpr int CPT = (SA + (SB < GT ? SB : GT) + NW - 1) / NW;  // 16-B chunks per thread
+    static constexpr int DR = (SA < GT ? SA : GT) * 16;      // recombination rows
+    static constexpr int DC = (SB < GT ? SB : GT) * 16 + 1;  // recombination row length
+    static constexpr int LDS_BYTES = 2 * STAGE > DR * DC * 4 ? 2 * STAGE : DR * DC * 4;
+};
+
+template <int TM, int TN, int WN>
+__global__ __launch_bounds__(256 * WN) void k_commit_mfma(
+    const int8_t *AB, int kappa, int ktA, const int8_t *WB, int r,
+    int ktW, int kchunk, int ktend, int nbx, double *__restrict__ partial,
+    const double *__restrict__ tab, Mod M) {
+    using S = CommitShape<TM, TN, WN>;
+    __shared__ __align__(16) char lds[S::LDS_BYTES];
     // XCD-aware block order: the 8 XCDs take consecutive runs of logical
-    // blocks, so the blocks sharing one key chunk share an L2
+    // blocks, so the blocks sharing one slot's key chunk share an L2
     const unsigned nblk = gridDim.x;
     const unsigned pid = blockIdx.x;
     const unsigned lid = (nblk % 8 == 0) ? (pid % 8) * (nblk / 8) + pid / 8 : pid;
     const int nb = (int)(lid % (unsigned)nbx);
-    unsigned rest = lid / (unsigned)nbx;
-    const int mb = (int)(rest % (unsigned)mbx);
-    rest /= (unsigned)mbx;
+    const unsigned rest = lid / (unsigned)nbx;
     const int s = (int)(rest % 64u);
     const int z = (int)(rest / 64u);
 
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = w & 3, wn = w >> 2;
-    const int lr = lane & 15, g = lane >> 4;
-    const int j0 = mb * (4 * MJ) + wm * MJ;
-    const int k0 = (nb * NWN + wn) * NSPW * NK;
-    const int rowsA = 7 * kappa, rowsB = 7 * r;
-    const size_t kb = (size_t)z * kchunk;
-    const size_t ke = min(kend, kb + kchunk);
+    const int rtA = (7 * kappa + 15) / 16, rtB = (7 * r + 15) / 16;
+    const int nbv = min(GT, rtB - GT * nb);  // witness tiles of this block
+    const int nchunk = (rtA + nbv) * 64;
+    const int ktb = z * kchunk;
+    const int nsteps = min(ktend, ktb + kchunk) - ktb;  // even
 
-    // fragment rows: out-of-range rows read the zero row `zrow` (at least
-    // kend + 64 bytes of zeros), so every load is unconditional and feeds the
-    // matrix cores directly
-    const int8_t *pa[4];
-    const int8_t *pb[NT];
+    // staging: chunk c = tid + NTHR i is 16 bytes of fragment c / 64 (key
+    // tiles first, then the block's witness tiles); one wave moves one KiB.
+    // Chunks past the block's tiles repeat the same-lane chunk of key tile 0
+    // (the same bytes to the same LDS address), so loads and stores carry no
+    // branches and the wait counts can keep two steps in flight.
+    const int8_t *gp[S::CPT];
+    int gs[S::CPT], lo[S::CPT];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const int mr = 16 * t + lr, ga = 7 * j0 + mr;
-        const bool ok = mr < 7 * MJ && ga < rowsA;
-        pa[t] = (ok ? AB + ((size_t)s * rowsA + ga) * strideA : zrow) + 16 * g;
+    for (int i = 0; i < S::CPT; i++) {
+        const int c0 = tid + S::NTHR * i;
+        const int c = c0 < nchunk ? c0 : (c0 & 63);
+        const int t = c >> 6, off = (c & 63) * 16;
+        if (t < rtA) {
+            gp[i] = AB + (((size_t)s * ktA + ktb) * rtA + t) * FRAG + off;
+            gs[i] = rtA * FRAG;
+            lo[i] = t * FRAG + off;
+        } else {
+            const int tb = t - rtA;
+            gp[i] = WB + (((size_t)s * ktW + ktb) * rtB + GT * nb + tb) * FRAG + off;
+            gs[i] = rtB * FRAG;
+            lo[i] = (S::SA + tb) * FRAG + off;
+        }
     }
+    v4i R0[S::CPT], R1[S::CPT];
+    // the compiler and scheduling barriers keep each step's loads together
+    // and in program order, which is what lets the wait before a store skip
+    // the newer step's loads (AB and WB are not __restrict__ so that the
+    // compiler does not treat these loads as invariant and move them freely)
+    auto gload = [&](v4i(&R)[S::CPT], int st) {
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-        const int nr = 16 * t + lr, gb = 7 * k0 + nr;
-        const bool ok = nr < NROWS && gb < rowsB;
-        pb[t] = (ok ? WB + ((size_t)s * rowsB + gb) * strideW : zrow) + 16 * g;
-    }
-    v4i acc[4][NT];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < NT; b++) acc[a][b] = (v4i){0, 0, 0, 0};
-    // Two-stage ping-pong over K (ke - kb is a multiple of 128): buffer 1's
-    // loads are in flight while buffer 0 feeds the matrix cores and vice
-    // versa; the last refill of buffer 0 re-reads a valid step and is unused.
-    v4i a0[4], b0[NT], a1[4], b1[NT];
-    auto load = [&](v4i(&af)[4], v4i(&bf)[NT], size_t k) {
-#pragma unroll
-        for (int t = 0; t < 4; t++) af[t] = *reinterpret_cast<const v4i *>(pa[t] + k);
-#pragma unroll
-        for (int t = 0; t < NT; t++) bf[t] = *reinterpret_cast<const v4i *>(pb[t] + k);
+        for (int i = 0; i < S::CPT; i++)
+            R[i] = *reinterpret_cast<const v4i *>(gp[i] + (size_t)(unsigned)st * (unsigned)gs[i]);
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("" ::: "memory");
     };
-    auto mma = [&](v4i(&af)[4], v4i(&bf)[NT]) {
+    auto sstore = [&](const v4i(&R)[S::CPT], int buf) {
 #pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int b = 0; b < NT; b++)
-                acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[a], bf[b], acc[a][b], 0, 0, 0);
+        for (int i = 0; i < S::CPT; i++)
+            *reinterpret_cast<v4i *>(lds + buf * S::STAGE + lo[i]) = R[i];
     };
-    if (kb < ke) load(a0, b0, kb);
-    for (size_t kk = kb; kk < ke; kk += 2 * KSTEP) {
-        load(a1, b1, kk + KSTEP);
-        mma(a0, b0);
-        load(a0, b0, kk + 2 * KSTEP < ke ? kk + 2 * KSTEP : kk);
-        mma(a1, b1);
-    }
-    // recombination through LDS, 2 waves at a time
-    const double *wt = tab + TAB_W256;
-    for (int grp = 0; grp < NW; grp += 2) {
+    v4i acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++) acc[a][b] = (v4i){0, 0, 0, 0};
+    // fragment slots past the block's tiles hold stale bytes; they only feed
+    // rows or columns of the product that are never recombined
+    auto compute = [&](int buf) {
+        const char *base = lds + buf * S::STAGE + lane * 16;
+        v4i fb[TN];
+#pragma unroll
+        for (int b = 0; b < TN; b++)
+            fb[b] = *reinterpret_cast<const v4i *>(base + (S::SA + wn + WN * b) * FRAG);
+#pragma unroll
+        for (int a = 0; a < TM; a++) {
+            const v4i fa = *reinterpret_cast<const v4i *>(base + (wm + 4 * a) * FRAG);
+#pragma unroll
+            for (int b = 0; b < TN; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[b], acc[a][b], 0, 0, 0);
+        }
+    };
+    // pipeline: step i is in LDS buffer i & 1 while the registers carry the
+    // loads of steps i + 1 and i + 2 (nsteps is even and at least 2; loads
+    // past the last step repeat it, and the last store into buffer 0 is
+    // never read)
+    const int last = nsteps - 1;
+    gload(R0, 0);
+    gload(R1, 1);
+    sstore(R0, 0);
+    gload(R0, min(2, last));
+    __syncthreads();
+    for (int i = 0; i < nsteps; i += 2) {
+        compute(0);
+        sstore(R1, 1);
+        gload(R1, min(i + 3, last));
         __syncthreads();
-        const bool mine = w >= grp && w < grp + 2;
-        int (*D)[LDSW] = Dl[w & 1];
-        if (mine) {
-            // D[16a + 4g + v][16b + lr] = acc[a][b][v]
+        compute(1);
+        sstore(R0, 0);
+        gload(R0, min(i + 4, last));
+        __syncthreads();
+    }
+
+    // recombination, one group of 7 key tiles (16 key rows) at a time:
+    // D[16 tl + 4 g + v][16 b' + lr] = acc[a][b][v] for the group's tiles
+    int(*D)[S::DC] = reinterpret_cast<int(*)[S::DC]>(lds);
+    const int lr = lane & 15, g = lane >> 4;
+    const double *wt = tab + TAB_W256;
+    const int ngrp = (rtA + GT - 1) / GT;
+    for (int gi = 0; gi < ngrp; gi++) {
+        if (gi) __syncthreads();
 #pragma unroll
-            for (int a = 0; a < 4; a++)
+        for (int a = 0; a < TM; a++) {
+            const int tl = wm + 4 * a - GT * gi;
+            if (tl < 0 || tl >= GT || 16 * tl >= S::DR) continue;
 #pragma unroll
-                for (int b = 0; b < NT; b++)
+            for (int b = 0; b < TN; b++) {
+                const int bl = wn + WN * b;
+                if (bl >= GT) continue;
 #pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        const int col = 16 * b + lr;
-                        if (col < NROWS) D[16 * a + 4 * g + v][col] = acc[a][b][v];
-                    }
+                for (int v = 0; v < 4; v++) D[16 * tl + 4 * g + v][16 * bl + lr] = acc[a][b][v];
+            }
         }
         __syncthreads();
-        if (mine) {
-            for (int o = lane; o < MJ * NK * NSPW; o += 64) {
-                const int jl = o / (NK * NSPW), kl = o % (NK * NSPW);
-                const int j = j0 + jl, k = k0 + kl;
-                if (j >= kappa || k >= r) continue;
-                // diagonals E_t = sum_{u+v=t} D[u][v]: at most 7 int32 terms,
-                // exact in fp64 and below 2^34 < q in magnitude
-                double E[13];
+        for (int o = tid; o < 256; o += S::NTHR) {
+            const int jl = o >> 4, kl = o & 15;
+            const int j = 16 * gi + jl, k = 16 * nb + kl;
+            if (j >= kappa || k >= r) continue;
+            // diagonals E_t = sum_{u+v=t} D[7jl+u][7kl+v]: at most 7 int32
+            // terms, below 2^34 < q in magnitude and exact in fp64
+            long long E[13];
 #pragma unroll
-                for (int t = 0; t < 13; t++) E[t] = 0.0;
+            for (int t = 0; t < 13; t++) E[t] = 0;
 #pragma unroll
-                for (int u = 0; u < 7; u++)
+            for (int u = 0; u < 7; u++)
 #pragma unroll
-                    for (int v = 0; v < 7; v++) E[u + v] += (double)D[jl * 7 + u][kl * 7 + v];
-                double x = 0.0;
+                for (int v = 0; v < 7; v++) E[u + v] += D[jl * 7 + u][kl * 7 + v];
+            double x = 0.0;
 #pragma unroll
-                for (int t = 0; t < 13; t++) {
-                    x += mul_lazy(E[t], wt[t], M);
-                    if ((t % 7) == 6) x = reduce_lazy(x, M);
-                }
-                partial[(((size_t)z * 64 + s) * kappa + j) * r + k] = reduce(x, M);
+            for (int t = 0; t < 13; t++) {
+                x += mul_lazy((double)E[t], wt[t], M);
+                if ((t % 7) == 6) x = reduce_lazy(x, M);
             }
+            partial[(((size_t)z * 64 + s) * kappa + j) * r + k] = reduce(x, M);
         }
     }
 }
 
+template <int TM, int TN, int WN>
+int occupancy() {
+    static int occ = 0;
+    if (occ == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void *>(&k_commit_mfma<TM, TN, WN>), 256 * WN, 0) !=
+                hipSuccess ||
+            n < 1)
+            n = 1;
+        occ = n;
+    }
+    return occ;
+}
+
+template <int TM, int TN, int WN>
+int run(Context *ctx, const int8_t *AB, size_t kappa, size_t ktA, const int8_t *WB, size_t r,
+        size_t ktW, size_t ktend, uint64_t *out) {
+    Mod M{ctx->qd, ctx->qinv};
+    const size_t nbx = (r + 15) / 16;
+    // K chunks: at most 1024 tiles (2^16 columns, exact int32 sums), an even
+    // number of tiles; pick the split that minimises (block waves x chunk
+    // length + a fixed per-block cost) for this variant's occupancy
+    const size_t slots = (size_t)ctx->ncu * occupancy<TM, TN, WN>();
+    size_t best_ks = 0, best_kc = 0;
+    double best = 0;
+    for (size_t ks = (ktend + 1023) / 1024; ks <= ktend / 2; ks *= 2) {
+        size_t kc = (ktend + ks - 1) / ks;
+        kc += kc & 1;
+        const size_t ksn = (ktend + kc - 1) / kc;
+        const size_t nblk = nbx * 64 * ksn;
+        const double cost = (double)((nblk + slots - 1) / slots) * (double)(kc + 8);
+        if (best_ks == 0 || cost < best) {
+            best = cost;
+            best_ks = ksn;
+            best_kc = kc;
+        }
+    }
+    if (best_ks == 0) {  // ktend == 2
+        best_ks = 1;
+        best_kc = ktend;
+    }
+    double *partial = nullptr;
+    SALSAA_TRY(scratch_get(ctx, 7, best_ks * 64 * kappa * r * sizeof(double), (void **)&partial));
+    const size_t nblk = nbx * 64 * best_ks;
+    SALSAA_CHECK(nblk < (1u << 31), "commit grid too large");
+    hipLaunchKernelGGL((k_commit_mfma<TM, TN, WN>), dim3((unsigned)nblk), dim3(256 * WN), 0,
+                       ctx->stream, AB, (int)kappa, (int)ktA, WB, (int)r, (int)ktW, (int)best_kc,
+                       (int)ktend, (int)nbx, partial, ctx->d_tables, M);
+    SALSAA_HIP(hipGetLastError());
+    return launch_commit_finalize(ctx, partial, best_ks, kappa, r, out);
+}
+
+template <int TM>
+int run_n(Context *ctx, const int8_t *AB, size_t kappa, size_t ktA, const int8_t *WB, size_t r,
+          size_t ktW, size_t ktend, uint64_t *out) {
+    const size_t rtB = (7 * r + 15) / 16;
+    if (rtB <= 2) return run<TM, 1, 2>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
+    if (rtB <= 4) return run<TM, 2, 2>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
+    return run<TM, 4, 2>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
+}
+
 }  // namespace
 
-int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t strideA,
-                       const int8_t *WB, size_t r, size_t m, size_t strideW, uint64_t *out) {
-    Mod M{ctx->qd, ctx->qinv};
-    // planes are zero from m up to the next multiple of 128
-    const size_t kend = (m + 2 * KSTEP - 1) / (2 * KSTEP) * (2 * KSTEP);
-    SALSAA_CHECK(kend <= strideW && kend <= strideA, "plane rows shorter than the witness");
-    // N layout: one strip per wave when r <= 8, else two strips per wave and
-    // up to 2 N-wave groups per block (8 waves; more would spill the
-    // 112 accumulator registers of a two-strip wave)
-    const size_t nstrips = (r + NK - 1) / NK;
-    const int nspw = nstrips == 1 ? 1 : 2;
-    const size_t ngroups = (nstrips + nspw - 1) / nspw;
-    const int nwn = ngroups >= 2 ? 2 : 1;
-    const size_t nbx = (ngroups + nwn - 1) / nwn;
-    const size_t mbx = (kappa + 4 * MJ - 1) / (4 * MJ);
-    // K chunks of at most 2^16 (exact int32 accumulation), at least 1024 when
-    // that still leaves >= ~1024 blocks
-    size_t ksplit = (kend + 65535) / 65536;
-    while (nbx * mbx * 64 * ksplit < 1024 && kend / (ksplit * 2) >= 1024) ksplit *= 2;
-    size_t kchunk = (kend + ksplit - 1) / ksplit;
-    kchunk = (kchunk + 2 * KSTEP - 1) / (2 * KSTEP) * (2 * KSTEP);
-    ksplit = (kend + kchunk - 1) / kchunk;
-    double *partial = nullptr;
-    SALSAA_TRY(scratch_get(ctx, 7, ksplit * 64 * kappa * r * sizeof(double), (void **)&partial));
-    const int8_t *zrow = nullptr;
-    SALSAA_TRY(zero_row(ctx, kend + 2 * KSTEP, &zrow));
-    const size_t nblk = nbx * mbx * 64 * ksplit;
-    SALSAA_CHECK(nblk < (1u << 31), "commit grid too large");
-#define SALSAA_MFMA_LAUNCH(NSPW, NWN)                                                              \
-    hipLaunchKernelGGL((k_commit_mfma<NSPW, NWN>), dim3((unsigned)nblk), dim3(4 * NWN * 64), 0,      \
-                       ctx->stream, AB, (int)kappa, strideA, WB, (int)r, strideW, kchunk, kend,    \
-                       (int)nbx, (int)mbx, zrow, partial, ctx->d_tables, M)
-    if (nspw == 1) SALSAA_MFMA_LAUNCH(1, 1);
-    else if (nwn == 1) SALSAA_MFMA_LAUNCH(2, 1);
-    else SALSAA_MFMA_LAUNCH(2, 2);
-#undef SALSAA_MFMA_LAUNCH
-    SALSAA_HIP(hipGetLastError());
-    return launch_commit_finalize(ctx, partial, ksplit, kappa, r, out);
+int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t ktA,
+                       const int8_t *WB, size_t r, size_t m, size_t ktW, uint64_t *out) {
+    const size_t ktend = plane_ktiles(m);
+    SALSAA_CHECK(kappa >= 1 && kappa <= kMfmaMaxKappa, "MFMA commit needs kappa <= 36");
+    SALSAA_CHECK(ktend == ktW && ktend <= ktA, "plane K-tiles do not cover the witness");
+    SALSAA_CHECK(ktA < (1u << 24) && r < (1u << 20), "commit too large");
+    const size_t rtA = (7 * kappa + 15) / 16;
+    if (rtA <= 4) return run_n<1>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
+    if (rtA <= 8) return run_n<2>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
+    return run_n<4>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
 }
 
 }  // namespace salsaa

@@ -182,9 +182,21 @@ __global__ __launch_bounds__(NT) void k_ntt_full(uint64_t *__restrict__ data, si
     }
 }
 
+// Slot-major staging of a block's 64 x 128 transformed values: slot s,
+// element e sits at wpos(s, e), i.e. row s with each run of 32 elements
+// rotated by r(s, e / 32) = 8 ((s / 8) % 4) + 2 (s % 8) + (e / 32) % 2.  The
+// rotation spreads the three access patterns over the LDS banks: the NTT's
+// writes (8 consecutive e by slots 8 apart), the plane packing's reads
+// (8 consecutive e per thread, 4 slots x 16 runs of 8 per wave) and the
+// slot-major store's reads (64 consecutive e of one slot).
+__device__ __forceinline__ int wpos(int s, int e) {
+    const int r = 8 * ((s >> 3) & 3) + 2 * (s & 7) + ((e >> 5) & 1);
+    return s * EL + (e & ~31) + ((e + r) & 31);
+}
+
 // Full forward NTT from phase-A inputs v (positions t + 8k); uses `xch` (a
-// 128 x SROW region) for the exchange and leaves the 64 slots transposed in
-// out[s * EL + e] (64 x 128 u64; may alias xch).
+// 128 x SROW region) for the exchange and leaves the 64 slots in
+// out[wpos(s, e)] (64 x 128 u64; may alias xch).
 __device__ __forceinline__ void full_fwd_to_work(double (&v)[8], uint64_t *xch, uint64_t *out,
                                                  int e, int t, bool live,
                                                  const double *__restrict__ tab, Mod M) {
@@ -203,7 +215,7 @@ __device__ __forceinline__ void full_fwd_to_work(double (&v)[8], uint64_t *xch, 
     __syncthreads();
     if (live) {
 #pragma unroll
-        for (int x = 0; x < 8; x++) out[(8 * t + x) * EL + e] = dbits(v[x]);
+        for (int x = 0; x < 8; x++) out[wpos(8 * t + x, e)] = dbits(v[x]);
     }
     __syncthreads();
 }
@@ -214,7 +226,7 @@ __device__ __forceinline__ void store_slotmajor(double *__restrict__ dst, const 
                                                 int nel, int tid) {
     for (int c = tid; c < 64 * EL; c += NT) {
         const int s = c >> 7, e = c & (EL - 1);
-        if (e < nel) dst[((size_t)s * rows + row) * m + i0 + e] = bitsd(work[c]);
+        if (e < nel) dst[((size_t)s * rows + row) * m + i0 + e] = bitsd(work[wpos(s, e)]);
     }
 }
 
@@ -223,33 +235,54 @@ __device__ __forceinline__ void store_slotmajor(double *__restrict__ dst, const 
 // b_u of y give d_u = b_u - 128, whose int8 bit pattern is b_u XOR 0x80.
 constexpr uint64_t DIGIT_BIAS = 0x0080808080808080ULL;
 
-// byte-plane store for the int8 MFMA commit: value (s, row, i) contributes
-// digit u to plane row (s, row * 7 + u), column i (row length `stride`,
-// a multiple of 64 covering m rounded up to 128).  Each thread packs 4
-// consecutive columns into one 32-bit store; columns past nel are written as
-// zero, so the plane is zero from m up to the next multiple of 128.
+// Fragment-tiled digit planes for the int8 MFMA commit.  A matrix of R value
+// rows has 7R digit rows (row r*7 + u holds digit u), grouped in tiles of 16;
+// columns are grouped in K-tiles of 64.  Fragment (s, kt, rt) is the 1 KiB
+// block at ((s * KT + kt) * RT + rt) * 1024, RT = ceil(7R / 16), KT = m
+// rounded up to 128, over 64: the row tiles of one K step are consecutive, so
+// a commit block streams one contiguous run per operand.  A fragment is
+// lane-ordered for
+// v_mfma_i32_16x16x64_i8: the 16 bytes at 16 l hold row l % 16, columns
+// 16 (l / 16) .. 16 (l / 16) + 15 of the tile, so lane l of a wave loads its
+// operand with one 16-byte load and the wave reads the KiB contiguously.
+// Columns in [m, 64 KT) are written as zero; the padding rows of the last row
+// tile are never written and only feed products that are discarded.
+// Each thread packs one digit of 8 columns into one 8-byte store (a pair of
+// threads fills one lane's 16 bytes).
 __device__ __forceinline__ void store_planes(int8_t *__restrict__ dst, const uint64_t *work,
-                                             size_t rows, size_t row, size_t stride, size_t i0,
+                                             size_t rows, size_t row, size_t ktiles, size_t i0,
                                              int nel, int tid) {
-    for (int c = tid; c < 64 * (EL / 4); c += NT) {
-        const int s = c >> 5, e4 = (c & 31) * 4;
-        uint32_t lo[4], hi[4];
+    const size_t rt_n = (rows * 7 + 15) / 16;
+    for (int c = tid; c < 64 * (EL / 8); c += NT) {
+        const int s = c >> 4, c8 = c & 15;
+        uint32_t lo[8], hi[8];
 #pragma unroll
-        for (int q4 = 0; q4 < 4; q4++) {
-            const int e = e4 + q4;
+        for (int q = 0; q < 8; q++) {
+            const int e = 8 * c8 + q;
             uint64_t y = 0;  // zero padding: every digit 0
-            if (e < nel) y = (d2u(bitsd(work[s * EL + e])) + DIGIT_BIAS) ^ 0x8080808080808080ULL;
-            lo[q4] = (uint32_t)y;
-            hi[q4] = (uint32_t)(y >> 32);
+            if (e < nel) y = (d2u(bitsd(work[wpos(s, e)])) + DIGIT_BIAS) ^ 0x8080808080808080ULL;
+            lo[q] = (uint32_t)y;
+            hi[q] = (uint32_t)(y >> 32);
         }
-        int8_t *base = dst + ((size_t)s * rows * 7 + row * 7) * stride + i0 + e4;
+        const size_t kt = i0 / 64 + (c8 >> 3);
+        // columns 8 (c8 % 8) .. +7 of the K-tile: lane group (c8 % 8) / 2, half c8 % 2
+        const int lane_off = ((c8 & 7) >> 1) * 256 + (c8 & 1) * 8;
+        int8_t *fbase = dst + ((size_t)s * ktiles + kt) * rt_n * 1024 + lane_off;
 #pragma unroll
         for (int u = 0; u < 7; u++) {
-            const int sh = 8 * (u & 3);
+            const unsigned dr = (unsigned)row * 7 + u;
             const uint32_t *src = u < 4 ? lo : hi;
-            const uint32_t packed = ((src[0] >> sh) & 0xFF) | (((src[1] >> sh) & 0xFF) << 8) |
-                                    (((src[2] >> sh) & 0xFF) << 16) | (((src[3] >> sh) & 0xFF) << 24);
-            *reinterpret_cast<uint32_t *>(base + (size_t)u * stride) = packed;
+            const int sh = 8 * (u & 3);
+            uint32_t p[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                uint32_t x = 0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) x |= ((src[4 * h + q] >> sh) & 0xFFu) << (8 * q);
+                p[h] = x;
+            }
+            *reinterpret_cast<uint2 *>(fbase + (size_t)(dr / 16) * 1024 + (dr % 16) * 16) =
+                make_uint2(p[0], p[1]);
         }
     }
 }
@@ -258,9 +291,10 @@ __device__ __forceinline__ void store_planes(int8_t *__restrict__ dst, const uin
 // (row % row_lo) * stride_lo + (row / row_lo) * stride_hi (u64 units), which
 // covers plain matrices and the halves of split (split.rs:21-28).
 // PLANES = false: exact doubles dst[(s * rows + row) * m + i];
-// PLANES = true: int8 digit planes (store_planes) with row length `pstride`.
+// PLANES = true: fragment-tiled int8 digit planes (store_planes) with
+// `pstride` K-tiles of 64 columns.
 template <bool PLANES>
-__global__ __launch_bounds__(NT) void k_ntt_full_sm(const uint64_t *__restrict__ src, size_t rows,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_ntt_full_sm(const uint64_t *__restrict__ src, size_t rows,
                                                     size_t m, size_t row_lo, size_t stride_lo,
                                                     size_t stride_hi, void *__restrict__ dst,
                                                     size_t pstride, const double *__restrict__ tab,
@@ -281,7 +315,7 @@ __global__ __launch_bounds__(NT) void k_ntt_full_sm(const uint64_t *__restrict__
     }
     full_fwd_to_work(v, lds, lds, e, t, live, tab, M);
     if (PLANES)
-        store_planes((int8_t *)dst, lds, rows, row, pstride, i0, nel, tid);
+        store_planes((int8_t *)dst, lds, rows, row, pstride, i0, nel, tid);  // pstride = K-tiles
     else
         store_slotmajor((double *)dst, lds, rows, row, m, i0, nel, tid);
 }
@@ -308,7 +342,6 @@ __global__ __launch_bounds__(NT) void k_decomp_sm(const uint64_t *__restrict__ W
                                                   void *__restrict__ dst, size_t pstride,
                                                   uint64_t q, const double *__restrict__ tab,
                                                   Mod M) {
-    __shared__ uint64_t raw[EL * SROW];
     __shared__ uint64_t work[EL * SROW];
     const size_t rho = blockIdx.y;
     const size_t i0 = (size_t)blockIdx.x * EL;
@@ -317,29 +350,36 @@ __global__ __launch_bounds__(NT) void k_decomp_sm(const uint64_t *__restrict__ W
     const uint64_t radix = *d_radix;
     const unsigned lg = (unsigned)__ffsll((long long)radix) - 1u;
     const uint64_t mask = radix - 1;
-    load_block<false>(raw, W + (rho * m + i0) * kD, nel, tid);
+    // coalesced load through LDS; each thread keeps the 8 coefficients it
+    // transforms (positions t + 8k) in registers for all limbs, so one LDS
+    // buffer serves the whole block and two blocks fit on a CU
+    load_block<false>(work, W + (rho * m + i0) * kD, nel, tid);
     __syncthreads();
     const bool live = e < nel;
+    uint64_t raw[8];
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) raw[k] = work[e * SROW + t + 8 * k];
+    }
+    __syncthreads();  // work becomes the NTT's exchange buffer
     const size_t rows = r * parts;
     for (int tl = 0; tl < (int)parts; tl++) {
         const size_t orow = rho * parts + tl;
-        // limb witness row, Coefficient form
-        ulonglong2 *nw = reinterpret_cast<ulonglong2 *>(new_W + (orow * m + i0) * kD);
-        for (int c = tid; c < nel * 32; c += NT) {
-            const int ee = c >> 5, k = (c & 31) * 2;
-            ulonglong2 o;
-            o.x = limb_of(raw[ee * SROW + k], q, lg, mask, tl);
-            o.y = limb_of(raw[ee * SROW + k + 1], q, lg, mask, tl);
-            nw[c] = o;
-        }
+        // limb witness row, Coefficient form: the 8 threads of an element
+        // write its 64 coefficients as 8 runs of 64 contiguous bytes
         double v[8];
         if (live) {
+            uint64_t *nw = new_W + (orow * m + i0 + e) * kD + t;
 #pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = u2d(limb_of(raw[e * SROW + t + 8 * k], q, lg, mask, tl));
+            for (int k = 0; k < 8; k++) {
+                const uint64_t lv = limb_of(raw[k], q, lg, mask, tl);
+                nw[8 * k] = lv;
+                v[k] = u2d(lv);
+            }
         }
         full_fwd_to_work(v, work, work, e, t, live, tab, M);
         if (PLANES)
-            store_planes((int8_t *)dst, work, rows, orow, pstride, i0, nel, tid);
+            store_planes((int8_t *)dst, work, rows, orow, pstride, i0, nel, tid);  // pstride = K-tiles
         else
             store_slotmajor((double *)dst, work, rows, orow, m, i0, nel, tid);
         __syncthreads();  // work is rewritten by the next limb
@@ -389,8 +429,7 @@ int launch_ntt_full_to_planes(Context *ctx, const uint64_t *src, size_t rows, si
                               size_t pstride) {
     if (rows == 0 || m == 0) return SALSAA_OK;
     SALSAA_CHECK(rows < 65536, "too many rows for one launch");
-    SALSAA_CHECK(pstride % 64 == 0 && pstride >= (m + EL - 1) / EL * EL,
-                 "plane stride must be a multiple of 64 covering m rounded up to 128");
+    SALSAA_CHECK(pstride == (m + EL - 1) / EL * 2, "K-tiles must be m rounded up to 128, over 64");
     Mod M{ctx->qd, ctx->qinv};
     dim3 grid(ceil_div(m, EL), (unsigned)rows);
     hipLaunchKernelGGL(k_ntt_full_sm<true>, grid, dim3(NT), 0, ctx->stream, src, rows, m, row_lo,
@@ -406,8 +445,7 @@ int launch_decomp_to_slotmajor(Context *ctx, const uint64_t *W, size_t r, size_t
     SALSAA_CHECK(r < 65536, "too many rows for one launch");
     Mod M{ctx->qd, ctx->qinv};
     if (planes) {
-        SALSAA_CHECK(pstride % 64 == 0 && pstride >= (m + EL - 1) / EL * EL,
-                     "plane stride must be a multiple of 64 covering m rounded up to 128");
+        SALSAA_CHECK(pstride == (m + EL - 1) / EL * 2, "K-tiles must be m rounded up to 128, over 64");
         dim3 grid(ceil_div(m, EL), (unsigned)r);
         hipLaunchKernelGGL(k_decomp_sm<true>, grid, dim3(NT), 0, ctx->stream, W, r, m, parts,
                            d_radix, new_W, dst, pstride, ctx->q, ctx->d_tables, M);
