@@ -171,6 +171,11 @@ int salsaa_ring_mul(salsaa_ctx *ctx, const uint64_t *d_a, const uint64_t *d_b,
  * layer of length m; commits with a shorter witness use the prefix layer. */
 int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t m,
                       salsaa_key **out);
+/* A view of the first kappa rows and first m ring elements of a key, sharing
+ * its storage (no copy): the counterpart of passing the prefix slice
+ * &key[..kappa] of the Vec<PowerSeries> to parallel_dot_series_matrix
+ * (arithmetic.rs:459-476).  The view must be destroyed before the key. */
+int salsaa_key_view(const salsaa_key *key, size_t kappa, size_t m, salsaa_key **out);
 int salsaa_key_destroy(salsaa_key *key);
 
 /* parallel_dot_series_matrix (arithmetic.rs:459-505): commit a witness
@@ -239,6 +244,64 @@ int salsaa_verify_decomp(salsaa_ctx *ctx, const uint64_t *d_rhs_dec, const uint6
 int salsaa_parallel_dot_matrix_matrix(salsaa_ctx *ctx, const uint64_t *d_W, size_t r_in,
                                       size_t m, const uint64_t *d_C, size_t r_out,
                                       uint64_t *d_out);
+
+/* ------------------------------------------------------------------ batches
+ * Many independent prover steps over one resident key.  Instance i has one
+ * witness row of m[i] ring elements (a power of two, at most the key's m) and
+ * commits with the key's first kappa[i] rows.  One step of an instance is
+ * what a caller of the per-instance operators does:
+ *   to_incomplete_ntt_representation and back on the witness
+ *     (cyclotomic_ring.rs:483-556; the witness is restored),
+ *   parallel_dot_series_matrix (arithmetic.rs:459-505) -> commitment,
+ *   norm_check_1/2/3 with their verifier checks (fast_norm_check.rs:282-462)
+ *     over the witness's m*d coefficients with the instance's own challenges,
+ *     `reps` repetitions (protocol.rs:222-259).
+ * The batched step runs every instance in a few grouped launches (see
+ * salsaa_batch_step); salsaa_batch_step_plain runs the same work as a loop of
+ * the per-instance calls above, for comparison.  Both write bit-identical
+ * results into separate output sets. */
+typedef struct salsaa_batch salsaa_batch;
+
+/* Device slots of one instance (layouts as for salsaa_norm_check with one
+ * row; nrounds = log2(m*d)). */
+typedef struct {
+    uint64_t *witness;     /* [m][d] Coefficient: filled by the caller          */
+    uint64_t *challenges;  /* [reps][nrounds][2]: filled by the caller          */
+    uint64_t *commitment;  /* [kappa_rows][d] IncompleteNTT; rows < kappa are
+                              the instance's commitment (kappa_rows = the
+                              batch's largest kappa; the plain step writes
+                              only rows < kappa)                               */
+    uint64_t *claims;      /* [1]                                               */
+    uint64_t *msgs;        /* [reps][nrounds][3][2]                             */
+    uint64_t *finals;      /* [reps][2]                                         */
+    uint64_t *evals;       /* [reps][2]                                         */
+    uint64_t *rhs;         /* [reps][2][d]                                      */
+    int32_t *ok;           /* [reps]                                            */
+    size_t m, kappa, nrounds;
+} salsaa_slot_t;
+
+/* Plan a batch: allocates the witness/challenge arenas and both output sets
+ * on the device, sorts the instances by length and builds the grouped
+ * launches.  tune = 1 times candidate commit K chunks and norm check
+ * first-pass widths on the device and keeps the fastest. */
+int salsaa_batch_create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m,
+                        const size_t *kappa, size_t reps, int fq2, uint64_t nr, int tune,
+                        salsaa_batch **out);
+int salsaa_batch_destroy(salsaa_batch *b);
+/* plain = 0: the batched step's outputs; 1: the plain per-call step's. */
+int salsaa_batch_slots(salsaa_batch *b, size_t i, int plain, salsaa_slot_t *out);
+/* One batched step: NTT forward and inverse over every witness (one launch
+ * each), every witness into MFMA digit planes (one launch), ONE commit and
+ * finalize for all instances, one norm check per length class.  use_graph = 1
+ * captures the step into a hipGraph on the first call and replays it. */
+int salsaa_batch_step(salsaa_batch *b, int use_graph);
+/* The same work as a native loop of per-instance calls (the plain path). */
+int salsaa_batch_step_plain(salsaa_batch *b);
+/* What the autotuner chose: commit K chunk (tiles of 64 ring elements),
+ * norm check first-pass columns per thread (0 = automatic), and the timed
+ * costs of the chosen commit and norm checks in ms. */
+int salsaa_batch_tuning(const salsaa_batch *b, size_t *kchunk, int *cpt_first, float *commit_ms,
+                        float *norm_check_ms);
 
 #ifdef __cplusplus
 }

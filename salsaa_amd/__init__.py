@@ -29,6 +29,14 @@ _sz = ctypes.c_size_t
 _vp = ctypes.c_void_p
 _int = ctypes.c_int
 _PP = ctypes.POINTER(ctypes.c_void_p)
+_PSZ = ctypes.POINTER(ctypes.c_size_t)
+
+
+class SlotT(ctypes.Structure):
+    """salsaa_slot_t: device slots of one batch instance."""
+    _fields_ = [(n, ctypes.c_void_p) for n in ("witness", "challenges", "commitment", "claims",
+                                                 "msgs", "finals", "evals", "rhs", "ok")] + \
+               [(n, ctypes.c_size_t) for n in ("m", "kappa", "nrounds")]
 
 
 class SalsaaError(RuntimeError):
@@ -97,6 +105,15 @@ def _load():
         "salsaa_compose": (_int, [_vp, _vp, _sz, _sz, _sz, _vp, _vp]),
         "salsaa_verify_decomp": (_int, [_vp, _vp, _vp, _sz, _sz, _sz, _vp, _int, _vp]),
         "salsaa_parallel_dot_matrix_matrix": (_int, [_vp, _vp, _sz, _sz, _vp, _sz, _vp]),
+        "salsaa_key_view": (_int, [_vp, _sz, _sz, _PP]),
+        "salsaa_batch_create": (_int, [_vp, _vp, _sz, _PSZ, _PSZ, _sz, _int, _u64, _int, _PP]),
+        "salsaa_batch_destroy": (_int, [_vp]),
+        "salsaa_batch_slots": (_int, [_vp, _sz, _int, ctypes.POINTER(SlotT)]),
+        "salsaa_batch_step": (_int, [_vp, _int]),
+        "salsaa_batch_step_plain": (_int, [_vp]),
+        "salsaa_batch_tuning": (_int, [_vp, _PSZ, ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_float),
+                                       ctypes.POINTER(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -118,16 +135,27 @@ def _check(status):
 
 
 class DeviceArray:
-    """A device buffer of uint64 (or int32/uint32) owned by a Context."""
+    """A device buffer of uint64 (or int32/uint32) owned by a Context, or a
+    non-owning view of device memory owned elsewhere (DeviceArray.view)."""
 
     def __init__(self, ctx, shape, dtype=np.uint64):
         self.ctx = ctx
         self.shape = tuple(int(s) for s in shape)
         self.dtype = np.dtype(dtype)
         self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.owner = True
         p = ctypes.c_void_p()
         _check(lib.salsaa_dev_alloc(ctx.h, max(self.nbytes, 1), ctypes.byref(p)))
         self.ptr = p.value
+
+    @classmethod
+    def view(cls, ctx, ptr, shape, dtype=np.uint64):
+        a = cls.__new__(cls)
+        a.ctx, a.ptr, a.owner = ctx, ptr, False
+        a.shape = tuple(int(s) for s in shape)
+        a.dtype = np.dtype(dtype)
+        a.nbytes = int(np.prod(a.shape)) * a.dtype.itemsize
+        return a
 
     def upload(self, arr):
         arr = np.ascontiguousarray(arr, dtype=self.dtype)
@@ -141,9 +169,9 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr is not None:
+        if self.ptr is not None and self.owner:
             lib.salsaa_dev_free(self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -160,6 +188,16 @@ class Key:
         p = ctypes.c_void_p()
         _check(lib.salsaa_key_create(ctx.h, A_dev.ptr, kappa, m, ctypes.byref(p)))
         self.h = p.value
+
+    def view(self, kappa, m):
+        """The key's first kappa rows and m ring elements (shares storage;
+        close the view before the key)."""
+        v = Key.__new__(Key)
+        v.ctx, v.kappa, v.m = self.ctx, kappa, m
+        p = ctypes.c_void_p()
+        _check(lib.salsaa_key_view(self.h, kappa, m, ctypes.byref(p)))
+        v.h = p.value
+        return v
 
     def close(self):
         if self.h:
@@ -326,6 +364,68 @@ class Context:
         return out
 
     fold = parallel_dot_matrix_matrix
+
+
+class Batch:
+    """Many independent prover steps (NTT forward + inverse, commit, norm
+    check) over one resident key, in grouped launches (salsaa_batch_*)."""
+
+    def __init__(self, ctx, key, m, kappa, reps=2, fq2=True, nr=NR, tune=True):
+        self.ctx, self.key = ctx, key
+        self.n = len(m)
+        self.m, self.kappa = [int(x) for x in m], [int(x) for x in kappa]
+        self.reps, self.fq2 = reps, bool(fq2)
+        self.krows = max(self.kappa)
+        M = (ctypes.c_size_t * self.n)(*self.m)
+        K = (ctypes.c_size_t * self.n)(*self.kappa)
+        p = ctypes.c_void_p()
+        _check(lib.salsaa_batch_create(ctx.h, key.h, self.n, M, K, reps, int(fq2), nr, int(tune),
+                                       ctypes.byref(p)))
+        self.h = p.value
+
+    def slot(self, i, plain=False):
+        """Device views of instance i's witness, challenges and outputs
+        (output shapes as norm_check's with one row)."""
+        st = SlotT()
+        _check(lib.salsaa_batch_slots(self.h, i, int(plain), ctypes.byref(st)))
+        c, m, n, r = self.ctx, st.m, st.nrounds, self.reps
+        V = DeviceArray.view
+        return dict(
+            witness=V(c, st.witness, (m, D)),
+            challenges=V(c, st.challenges, (r, n, 2)),
+            commitment=V(c, st.commitment, (self.krows, 1, D)),
+            claims=V(c, st.claims, (1,)),
+            msgs=V(c, st.msgs, (r, n, 1, 3, 2)),
+            finals=V(c, st.finals, (r, 1, 2)),
+            evals=V(c, st.evals, (r, 1, 2)),
+            rhs=V(c, st.rhs, (r, 1, 2, D)),
+            ok=V(c, st.ok, (r, 1), np.int32),
+            m=m, kappa=st.kappa, nrounds=n)
+
+    def step(self, graph=True):
+        _check(lib.salsaa_batch_step(self.h, int(graph)))
+
+    def step_plain(self):
+        _check(lib.salsaa_batch_step_plain(self.h))
+
+    def tuning(self):
+        kc, cpt = ctypes.c_size_t(), ctypes.c_int()
+        tc, tn = ctypes.c_float(), ctypes.c_float()
+        _check(lib.salsaa_batch_tuning(self.h, ctypes.byref(kc), ctypes.byref(cpt),
+                                       ctypes.byref(tc), ctypes.byref(tn)))
+        return dict(kchunk=kc.value, cpt_first=cpt.value, commit_ms=tc.value,
+                    norm_check_ms=tn.value)
+
+    def close(self):
+        if self.h:
+            lib.salsaa_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def header_symbols():

@@ -5,14 +5,7 @@
 #include <cstring>
 #include <new>
 
-#include "common.hpp"
-
-struct salsaa_ctx {
-    salsaa::Context c;
-};
-struct salsaa_key {
-    salsaa::Key k;
-};
+#include "handles.hpp"
 
 namespace salsaa {
 
@@ -168,15 +161,6 @@ int zero_row(Context *ctx, size_t bytes, const int8_t **out) {
 }  // namespace salsaa
 
 using namespace salsaa;
-
-#define CTX_OK(ctx)                                                  \
-    do {                                                             \
-        if (!(ctx)) {                                                \
-            set_error("null context");                               \
-            return SALSAA_ERR_ARG;                                   \
-        }                                                            \
-        SALSAA_HIP(hipSetDevice((ctx)->c.device));                   \
-    } while (0)
 
 extern "C" {
 
@@ -355,9 +339,12 @@ int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t
     k->k.ctx = &ctx->c;
     k->k.kappa = kappa;
     k->k.m = m;
+    k->k.a_rows = kappa;
+    k->k.a_m = m;
     int s = SALSAA_OK;
     if (ctx->c.mfma_commit && kappa <= kMfmaMaxKappa) {
-        k->k.pstride = plane_ktiles(m);
+        k->k.a_kt = (kappa * 7 + 15) / 16;
+        k->k.a_ktiles = plane_ktiles(m);
         hipError_t e = hipMalloc(&k->k.d_AB, plane_bytes(kappa, m));
         if (e != hipSuccess) {
             delete k;
@@ -365,7 +352,7 @@ int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t
             return SALSAA_ERR_NOMEM;
         }
         s = launch_ntt_full_to_planes(&ctx->c, d_A, kappa, m, kappa, m * kD, 0, k->k.d_AB,
-                                      k->k.pstride);
+                                      k->k.a_ktiles);
     } else {
         hipError_t e = hipMalloc(&k->k.d_A, (size_t)kD * kappa * m * sizeof(double));
         if (e != hipSuccess) {
@@ -385,11 +372,27 @@ int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t
     return SALSAA_OK;
 }
 
+int salsaa_key_view(const salsaa_key *key, size_t kappa, size_t m, salsaa_key **out) {
+    SALSAA_CHECK(key != nullptr && out != nullptr, "null key or output pointer");
+    SALSAA_CHECK(kappa >= 1 && kappa <= key->k.kappa, "view rows must be 1..kappa of the key");
+    SALSAA_CHECK(m >= 1 && m <= key->k.m, "view length must be 1..m of the key");
+    salsaa_key *v = new (std::nothrow) salsaa_key();
+    if (!v) return SALSAA_ERR_NOMEM;
+    v->k = key->k;
+    v->k.kappa = kappa;
+    v->k.m = m;
+    v->k.owner = false;
+    *out = v;
+    return SALSAA_OK;
+}
+
 int salsaa_key_destroy(salsaa_key *key) {
     if (!key) return SALSAA_OK;
-    (void)hipStreamSynchronize(key->k.ctx->stream);
-    if (key->k.d_A) (void)hipFree(key->k.d_A);
-    if (key->k.d_AB) (void)hipFree(key->k.d_AB);
+    if (key->k.owner) {
+        (void)hipStreamSynchronize(key->k.ctx->stream);
+        if (key->k.d_A) (void)hipFree(key->k.d_A);
+        if (key->k.d_AB) (void)hipFree(key->k.d_AB);
+    }
     delete key;
     return SALSAA_OK;
 }
@@ -407,14 +410,15 @@ static int commit_rows(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d
         SALSAA_TRY(scratch_get(&ctx->c, 6, plane_bytes(rows, m), (void **)&wb));
         SALSAA_TRY(launch_ntt_full_to_planes(&ctx->c, d_W, rows, m, row_lo, stride_lo, stride_hi,
                                              wb, ps));
-        return launch_commit_mfma(&ctx->c, key->k.d_AB, key->k.kappa, key->k.pstride, wb, rows, m,
-                                  ps, d_out);
+        return launch_commit_mfma(&ctx->c, key->k.d_AB, key->k.kappa, key->k.a_kt,
+                                  key->k.a_ktiles, wb, rows, m, ps, d_out);
     }
     double *w_sm = nullptr;
     SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * rows * m * sizeof(double), (void **)&w_sm));
     SALSAA_TRY(launch_ntt_full_to_slotmajor(&ctx->c, d_W, rows, m, row_lo, stride_lo, stride_hi,
                                             w_sm));
-    return launch_commit_mac(&ctx->c, key->k.d_A, key->k.kappa, key->k.m, w_sm, rows, m, d_out);
+    return launch_commit_mac(&ctx->c, key->k.d_A, key->k.kappa, key->k.a_rows, key->k.a_m, w_sm,
+                             rows, m, d_out);
 }
 
 int salsaa_parallel_dot_series_matrix(salsaa_ctx *ctx, const salsaa_key *key,
@@ -463,13 +467,14 @@ int salsaa_decomp_ell(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_
         SALSAA_TRY(scratch_get(&ctx->c, 6, plane_bytes(rows, m), (void **)&wb));
         SALSAA_TRY(launch_decomp_to_slotmajor(&ctx->c, d_W, r, m, parts, d_radix, d_new_W, wb, 1,
                                               ps));
-        return launch_commit_mfma(&ctx->c, key->k.d_AB, key->k.kappa, key->k.pstride, wb, rows, m,
-                                  ps, d_rhs);
+        return launch_commit_mfma(&ctx->c, key->k.d_AB, key->k.kappa, key->k.a_kt,
+                                  key->k.a_ktiles, wb, rows, m, ps, d_rhs);
     }
     double *w_sm = nullptr;
     SALSAA_TRY(scratch_get(&ctx->c, 6, (size_t)kD * rows * m * sizeof(double), (void **)&w_sm));
     SALSAA_TRY(launch_decomp_to_slotmajor(&ctx->c, d_W, r, m, parts, d_radix, d_new_W, w_sm, 0, 0));
-    return launch_commit_mac(&ctx->c, key->k.d_A, key->k.kappa, key->k.m, w_sm, rows, m, d_rhs);
+    return launch_commit_mac(&ctx->c, key->k.d_A, key->k.kappa, key->k.a_rows, key->k.a_m, w_sm,
+                             rows, m, d_rhs);
 }
 
 int salsaa_compose(salsaa_ctx *ctx, const uint64_t *d_in, size_t rows, size_t n, size_t parts,

@@ -1,0 +1,552 @@
+// batch.hip -- many independent prover steps (NTT forward + inverse, commit,
+// norm check) in a handful of grouped launches, optionally replayed from a
+// captured hipGraph.
+//
+// Instances are sorted by witness length (descending) once, at plan time.
+//   * NTT: the witnesses sit back to back in one arena, so the forward and
+//     inverse transforms are one launch each over every ring element.
+//   * Commit: every instance is one witness row committed against the same
+//     resident key, so consecutive instances form row blocks of 16 (one MFMA
+//     N block) and the whole batch is ONE commit: each block of the launch
+//     streams one (slot, K chunk) of the key once for up to 16 instances, and
+//     the blocks that share that key chunk run side by side on one XCD.  A
+//     plain per-instance commit reads its key prefix once per instance.
+//   * Norm check: instances of equal length are consecutive, so each length
+//     class is one multi-row norm check with per-row challenges.
+// The commit's K chunk and the norm check's first-pass width are chosen by
+// timing candidates on the device when the plan is made (the autotuner).
+#include <algorithm>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "handles.hpp"
+
+namespace salsaa {
+namespace {
+
+__global__ void k_batch_zero(uint64_t *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+struct Outputs {
+    uint64_t *com = nullptr, *claims = nullptr, *msgs = nullptr, *fin = nullptr, *ev = nullptr,
+             *rhs = nullptr;
+    int32_t *ok = nullptr;
+};
+
+template <typename T>
+int dmalloc(T **p, size_t count) {
+    void *q = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("batch allocation failed: ") + hipGetErrorString(e));
+        return SALSAA_ERR_NOMEM;
+    }
+    *p = static_cast<T *>(q);
+    return SALSAA_OK;
+}
+
+template <typename T>
+int upload(Context *ctx, T *dst, const std::vector<T> &src) {
+    if (src.empty()) return SALSAA_OK;
+    SALSAA_HIP(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice,
+                              ctx->stream));
+    SALSAA_HIP(hipStreamSynchronize(ctx->stream));
+    return SALSAA_OK;
+}
+
+}  // namespace
+}  // namespace salsaa
+
+using namespace salsaa;
+
+struct salsaa_batch {
+    salsaa_ctx *ctx = nullptr;
+    const salsaa_key *key = nullptr;
+    size_t n = 0, reps = 0;
+    int fq2 = 0;
+    uint64_t nr = 0;
+    size_t krows = 0;  // key rows every commit is computed with (the largest kappa)
+    std::vector<size_t> m, kappa;          // per instance, caller's order
+    std::vector<size_t> order, pos;        // sorted position -> instance, instance -> position
+    std::vector<size_t> off_w, off_c, off_msg, nrounds;  // per sorted position
+    size_t tot_w = 0, tot_c = 0, tot_msg = 0;
+    uint64_t *d_wit = nullptr, *d_cs = nullptr;
+    Outputs out[2];  // [0] batched step, [1] plain per-call step
+    // witness -> planes
+    std::vector<PlaneJob> pjobs;
+    size_t pnblk = 0;
+    PlaneJob *d_pjobs = nullptr;
+    unsigned long long *d_off = nullptr, *d_m = nullptr;
+    int8_t *d_planes = nullptr;
+    // commit
+    size_t njobs = 0, max_tiles = 0, kchunk = 0;
+    std::vector<int> job_ktiles, job_rows;
+    std::vector<size_t> job_planes;  // byte offset of each job's planes
+    double *d_partial = nullptr;
+    size_t partial_cap = 0;  // doubles
+    CommitJob *d_cjobs = nullptr;
+    uint2 *d_table = nullptr;
+    size_t table_cap = 0, cnblk = 0;
+    FinJob *d_fjobs = nullptr;
+    size_t fnblk = 0;
+    // norm check
+    struct Group {
+        size_t first, rows, L;
+    };
+    std::vector<Group> groups;
+    NcScratch nc;
+    int cpt_first = 0;
+    float t_commit = 0.f, t_nc = 0.f;
+    // plain path
+    std::vector<salsaa_key *> views;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int alloc_outputs(salsaa_batch *b, Outputs &o) {
+    SALSAA_TRY(dmalloc(&o.com, b->n * b->krows * kD));
+    SALSAA_TRY(dmalloc(&o.claims, b->n));
+    SALSAA_TRY(dmalloc(&o.msgs, b->tot_msg));
+    SALSAA_TRY(dmalloc(&o.fin, b->n * b->reps * 2));
+    SALSAA_TRY(dmalloc(&o.ev, b->n * b->reps * 2));
+    SALSAA_TRY(dmalloc(&o.rhs, b->n * b->reps * 2 * kD));
+    SALSAA_TRY(dmalloc(&o.ok, b->n * b->reps));
+    return SALSAA_OK;
+}
+
+void free_outputs(Outputs &o) {
+    (void)hipFree(o.com);
+    (void)hipFree(o.claims);
+    (void)hipFree(o.msgs);
+    (void)hipFree(o.fin);
+    (void)hipFree(o.ev);
+    (void)hipFree(o.rhs);
+    (void)hipFree(o.ok);
+    o = Outputs();
+}
+
+// Commit job tables for K chunk kc: block order (z, s, job) with the jobs
+// active at chunk z (a prefix, lengths being sorted) fastest.
+int build_commit_tables(salsaa_batch *b, size_t kc) {
+    Context *ctx = &b->ctx->c;
+    std::vector<CommitJob> cj(b->njobs);
+    std::vector<FinJob> fj(b->njobs);
+    std::vector<int> ks(b->njobs);
+    size_t poff = 0, fblk = 0;
+    int maxks = 0;
+    for (size_t j = 0; j < b->njobs; j++) {
+        ks[j] = (int)((b->job_ktiles[j] + kc - 1) / kc);
+        maxks = std::max(maxks, ks[j]);
+        CommitJob &c = cj[j];
+        c.WB = b->d_planes + b->job_planes[j];
+        c.ktiles = b->job_ktiles[j];
+        c.rows = b->job_rows[j];
+        c.rtiles = (7 * c.rows + 15) / 16;
+        c.ntile = c.rtiles;
+        c.partial = b->d_partial + poff;
+        FinJob &f = fj[j];
+        f.partial = c.partial;
+        f.out = nullptr;  // set per output set
+        f.osj = kD;
+        f.osk = (long long)(b->krows * kD);
+        f.ks = ks[j];
+        f.rows = c.rows;
+        f.blk0 = (int)fblk;
+        poff += (size_t)ks[j] * 64 * b->krows * c.rows;
+        fblk += b->krows * (size_t)c.rows;
+    }
+    SALSAA_CHECK(poff <= b->partial_cap, "commit partial buffer too small");
+    std::vector<uint2> table;
+    for (int z = 0; z < maxks; z++) {
+        unsigned active = 0;
+        while (active < b->njobs && ks[active] > z) active++;
+        for (unsigned s = 0; s < 64; s++)
+            for (unsigned j = 0; j < active; j++) table.push_back(make_uint2(j, ((unsigned)z << 6) | s));
+    }
+    SALSAA_CHECK(table.size() <= b->table_cap, "commit table too small");
+    SALSAA_TRY(upload(ctx, b->d_cjobs, cj));
+    SALSAA_TRY(upload(ctx, b->d_table, table));
+    // finalize jobs for both output sets: [0, njobs) batched, [njobs, 2 njobs) unused
+    for (size_t j = 0; j < b->njobs; j++) fj[j].out = b->out[0].com + (16 * j) * b->krows * kD;
+    SALSAA_TRY(upload(ctx, b->d_fjobs, fj));
+    b->cnblk = table.size();
+    b->fnblk = fblk;
+    b->kchunk = kc;
+    return SALSAA_OK;
+}
+
+int run_planes(salsaa_batch *b) {
+    return launch_ntt_planes_batch(&b->ctx->c, b->d_wit, b->d_off, b->d_m, b->d_pjobs,
+                                   b->pjobs.size(), b->pnblk);
+}
+
+int run_commit(salsaa_batch *b) {
+    const Key &k = b->key->k;
+    SALSAA_TRY(launch_commit_mfma_batch(&b->ctx->c, k.d_AB, b->krows, k.a_kt, k.a_ktiles,
+                                        b->d_cjobs, b->d_table, b->cnblk, b->kchunk,
+                                        b->max_tiles));
+    return launch_commit_finalize_batch(&b->ctx->c, b->d_fjobs, b->njobs, b->fnblk, b->krows);
+}
+
+int run_norm_checks(salsaa_batch *b, int cpt) {
+    Context *ctx = &b->ctx->c;
+    const Outputs &o = b->out[0];
+    for (const auto &g : b->groups) {
+        const size_t p = g.first, nr = b->nrounds[p];
+        NcOut no;
+        no.claims = o.claims + p;
+        no.claim_row = 1;
+        no.msgs = o.msgs + b->off_msg[p];
+        no.msg_row = b->reps * nr * 6;
+        no.msg_rep = nr * 6;
+        no.msg_t = 6;
+        no.finals = o.fin + p * b->reps * 2;
+        no.evals = o.ev + p * b->reps * 2;
+        no.fe_row = b->reps * 2;
+        no.fe_rep = 2;
+        no.rhs = o.rhs + p * b->reps * 2 * kD;
+        no.rhs_row = b->reps * 2 * kD;
+        no.rhs_rep = 2 * kD;
+        no.ok = o.ok + p * b->reps;
+        no.ok_row = b->reps;
+        no.ok_rep = 1;
+        SALSAA_TRY(launch_norm_check_ex(ctx, b->d_wit + b->off_w[p] * kD, g.rows, g.L,
+                                        b->d_cs + b->off_c[p], b->reps * nr * 2, b->reps, b->fq2,
+                                        b->nr, no, cpt, &b->nc));
+    }
+    return SALSAA_OK;
+}
+
+int run_step(salsaa_batch *b) {
+    Context *ctx = &b->ctx->c;
+    SALSAA_TRY(launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 0));
+    SALSAA_TRY(launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 1));
+    SALSAA_TRY(run_planes(b));
+    SALSAA_TRY(run_commit(b));
+    return run_norm_checks(b, b->cpt_first);
+}
+
+// median-free device timing of fn over `reps` runs (minimum), in ms
+template <typename F>
+int time_ms(salsaa_batch *b, F fn, int runs, float *ms_out) {
+    Context *ctx = &b->ctx->c;
+    hipEvent_t e0, e1;
+    SALSAA_HIP(hipEventCreate(&e0));
+    SALSAA_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int st = SALSAA_OK;
+    for (int r = 0; r < runs && st == SALSAA_OK; r++) {
+        (void)hipEventRecord(e0, ctx->stream);
+        st = fn();
+        (void)hipEventRecord(e1, ctx->stream);
+        if (st == SALSAA_OK && hipEventSynchronize(e1) == hipSuccess) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            best = std::min(best, ms);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_out = best;
+    return st;
+}
+
+int autotune(salsaa_batch *b) {
+    size_t max_kt = 0;
+    for (int kt : b->job_ktiles) max_kt = std::max(max_kt, (size_t)kt);
+    size_t best_kc = 0;
+    float best = 0.f;
+    for (size_t kc : {16, 32, 64, 128, 256}) {
+        if (kc > 2 * max_kt && best_kc) break;
+        SALSAA_TRY(build_commit_tables(b, kc));
+        float ms = 0.f;
+        SALSAA_TRY(time_ms(b, [&] { return run_commit(b); }, 2, &ms));
+        if (!best_kc || ms < best) {
+            best = ms;
+            best_kc = kc;
+        }
+    }
+    SALSAA_TRY(build_commit_tables(b, best_kc));
+    b->t_commit = best;
+    int best_cpt = 0;
+    float best_nc = 0.f;
+    for (int cpt : {1, 2, 4, 8}) {
+        float ms = 0.f;
+        SALSAA_TRY(time_ms(b, [&] { return run_norm_checks(b, cpt); }, 2, &ms));
+        if (!best_cpt || ms < best_nc) {
+            best_nc = ms;
+            best_cpt = cpt;
+        }
+    }
+    b->cpt_first = best_cpt;
+    b->t_nc = best_nc;
+    return SALSAA_OK;
+}
+
+void destroy(salsaa_batch *b) {
+    if (!b) return;
+    if (b->ctx) (void)hipStreamSynchronize(b->ctx->c.stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto *v : b->views) salsaa_key_destroy(v);
+    free_outputs(b->out[0]);
+    free_outputs(b->out[1]);
+    (void)hipFree(b->d_wit);
+    (void)hipFree(b->d_cs);
+    (void)hipFree(b->d_pjobs);
+    (void)hipFree(b->d_off);
+    (void)hipFree(b->d_m);
+    (void)hipFree(b->d_planes);
+    (void)hipFree(b->d_partial);
+    (void)hipFree(b->d_cjobs);
+    (void)hipFree(b->d_table);
+    (void)hipFree(b->d_fjobs);
+    (void)hipFree(b->nc.macc);
+    (void)hipFree(b->nc.bufA);
+    (void)hipFree(b->nc.bufB);
+    delete b;
+}
+
+int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, const size_t *kappa,
+           size_t reps, int fq2, uint64_t nr, int tune, salsaa_batch *b) {
+    Context *c = &ctx->c;
+    SALSAA_CHECK(key != nullptr && key->k.ctx == c, "key missing or from another context");
+    SALSAA_CHECK(key->k.d_AB != nullptr, "batched commit needs a key with MFMA planes");
+    SALSAA_CHECK(n >= 1 && m != nullptr && kappa != nullptr, "batch needs n >= 1 instances");
+    SALSAA_CHECK(reps >= 1 && reps < 65536, "repetitions out of range");
+    SALSAA_CHECK(nr < c->q, "non-residue must be reduced mod q");
+    b->ctx = ctx;
+    b->key = key;
+    b->n = n;
+    b->reps = reps;
+    b->fq2 = fq2 ? 1 : 0;
+    b->nr = nr;
+    b->m.assign(m, m + n);
+    b->kappa.assign(kappa, kappa + n);
+    for (size_t i = 0; i < n; i++) {
+        SALSAA_CHECK(m[i] >= 1 && (m[i] & (m[i] - 1)) == 0, "witness lengths must be powers of two");
+        SALSAA_CHECK(m[i] <= key->k.m, "witness longer than the key");
+        SALSAA_CHECK(kappa[i] >= 1 && kappa[i] <= key->k.kappa, "kappa must be 1..key rows");
+        b->krows = std::max(b->krows, kappa[i]);
+    }
+    // sort by length, longest first (stable)
+    b->order.resize(n);
+    std::iota(b->order.begin(), b->order.end(), 0);
+    std::stable_sort(b->order.begin(), b->order.end(),
+                     [&](size_t x, size_t y) { return b->m[x] > b->m[y]; });
+    b->pos.resize(n);
+    b->off_w.resize(n);
+    b->off_c.resize(n);
+    b->off_msg.resize(n);
+    b->nrounds.resize(n);
+    for (size_t p = 0; p < n; p++) {
+        const size_t i = b->order[p];
+        b->pos[i] = p;
+        b->nrounds[p] = (size_t)__builtin_ctzll(b->m[i] * kD);
+        b->off_w[p] = b->tot_w;
+        b->off_c[p] = b->tot_c;
+        b->off_msg[p] = b->tot_msg;
+        b->tot_w += b->m[i];
+        b->tot_c += reps * b->nrounds[p] * 2;
+        b->tot_msg += reps * b->nrounds[p] * 6;
+    }
+    SALSAA_TRY(dmalloc(&b->d_wit, b->tot_w * kD));
+    SALSAA_TRY(dmalloc(&b->d_cs, b->tot_c));
+    hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div(b->tot_w * kD, 256)), dim3(256), 0, c->stream,
+                       b->d_wit, b->tot_w * kD);
+    hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div(b->tot_c, 256)), dim3(256), 0, c->stream,
+                       b->d_cs, b->tot_c);
+    SALSAA_HIP(hipGetLastError());
+    SALSAA_TRY(alloc_outputs(b, b->out[0]));
+
+    // row blocks of 16 instances: planes and commit jobs
+    b->njobs = (n + 15) / 16;
+    size_t planes_bytes = 0;
+    std::vector<unsigned long long> hoff(n), hm(n);
+    for (size_t p = 0; p < n; p++) {
+        hoff[p] = b->off_w[p];
+        hm[p] = b->m[b->order[p]];
+    }
+    for (size_t j = 0; j < b->njobs; j++) {
+        PlaneJob pj;
+        pj.first = (int)(16 * j);
+        pj.rows = (int)std::min<size_t>(16, n - 16 * j);
+        pj.ktiles = (int)plane_ktiles(hm[16 * j]);
+        pj.blk0 = (int)b->pnblk;
+        b->job_planes.push_back(planes_bytes);
+        pj.dst = nullptr;
+        b->pjobs.push_back(pj);
+        b->job_ktiles.push_back(pj.ktiles);
+        b->job_rows.push_back(pj.rows);
+        const size_t rt = (7 * (size_t)pj.rows + 15) / 16;
+        b->max_tiles = std::max(b->max_tiles, rt);
+        planes_bytes += (size_t)64 * pj.ktiles * rt * 1024;
+        b->pnblk += (size_t)pj.rows * pj.ktiles / 2;
+        SALSAA_CHECK((size_t)pj.ktiles <= key->k.a_ktiles, "witness longer than the key planes");
+    }
+    SALSAA_CHECK(b->pnblk < (1u << 31), "batch too large");
+    SALSAA_TRY(dmalloc(&b->d_planes, planes_bytes));
+    for (size_t j = 0; j < b->njobs; j++) b->pjobs[j].dst = b->d_planes + b->job_planes[j];
+    SALSAA_TRY(dmalloc(&b->d_pjobs, b->pjobs.size()));
+    SALSAA_TRY(dmalloc(&b->d_off, n));
+    SALSAA_TRY(dmalloc(&b->d_m, n));
+    SALSAA_TRY(upload(c, b->d_pjobs, b->pjobs));
+    SALSAA_TRY(upload(c, b->d_off, hoff));
+    SALSAA_TRY(upload(c, b->d_m, hm));
+    // commit buffers sized for the smallest K chunk tried (16 tiles)
+    size_t pcap = 0, tcap = 0;
+    for (size_t j = 0; j < b->njobs; j++) {
+        const size_t ks = (b->job_ktiles[j] + 15) / 16;
+        pcap += ks * 64 * b->krows * b->job_rows[j];
+        tcap += ks * 64;
+    }
+    b->partial_cap = pcap;
+    b->table_cap = tcap;
+    SALSAA_TRY(dmalloc(&b->d_partial, pcap));
+    SALSAA_TRY(dmalloc(&b->d_table, tcap));
+    SALSAA_TRY(dmalloc(&b->d_cjobs, b->njobs));
+    SALSAA_TRY(dmalloc(&b->d_fjobs, b->njobs));
+    // norm check classes of equal length
+    size_t need_m = 0, need_a = 0, need_b = 0;
+    for (size_t p = 0; p < n;) {
+        size_t q = p;
+        while (q < n && hm[q] == hm[p]) q++;
+        salsaa_batch::Group g{p, q - p, hm[p] * kD};
+        size_t a, bb, mm;
+        nc_scratch_need(g.rows, g.L, reps, &mm, &a, &bb);
+        need_m = std::max(need_m, mm);
+        need_a = std::max(need_a, a);
+        need_b = std::max(need_b, bb);
+        b->groups.push_back(g);
+        p = q;
+    }
+    SALSAA_TRY(dmalloc(&b->nc.macc, need_m / sizeof(unsigned long long) + 1));
+    SALSAA_TRY(dmalloc(&b->nc.bufA, need_a / sizeof(double) + 1));
+    SALSAA_TRY(dmalloc(&b->nc.bufB, need_b / sizeof(double) + 1));
+    b->nc.macc_bytes = need_m;
+    b->nc.bufA_bytes = need_a;
+    b->nc.bufB_bytes = need_b;
+    // per-instance key views for the plain per-call path
+    for (size_t i = 0; i < n; i++) {
+        salsaa_key *v = nullptr;
+        SALSAA_TRY(salsaa_key_view(key, kappa[i], m[i], &v));
+        b->views.push_back(v);
+    }
+    if (tune) return autotune(b);
+    b->cpt_first = 0;
+    return build_commit_tables(b, 64);
+}
+
+}  // namespace
+
+extern "C" {
+
+int salsaa_batch_create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m,
+                        const size_t *kappa, size_t reps, int fq2, uint64_t nr, int tune,
+                        salsaa_batch **out) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(out != nullptr, "null output pointer");
+    salsaa_batch *b = new (std::nothrow) salsaa_batch();
+    if (!b) return SALSAA_ERR_NOMEM;
+    const int st = create(ctx, key, n, m, kappa, reps, fq2, nr, tune, b);
+    if (st != SALSAA_OK) {
+        destroy(b);
+        return st;
+    }
+    *out = b;
+    return SALSAA_OK;
+}
+
+int salsaa_batch_destroy(salsaa_batch *b) {
+    if (!b) return SALSAA_OK;
+    (void)hipSetDevice(b->ctx->c.device);
+    destroy(b);
+    return SALSAA_OK;
+}
+
+int salsaa_batch_slots(salsaa_batch *b, size_t i, int plain, salsaa_slot_t *s) {
+    SALSAA_CHECK(b != nullptr && s != nullptr, "null batch or output");
+    SALSAA_CHECK(i < b->n, "instance index out of range");
+    CTX_OK(b->ctx);
+    if (plain && !b->out[1].com) SALSAA_TRY(alloc_outputs(b, b->out[1]));
+    const Outputs &o = b->out[plain ? 1 : 0];
+    const size_t p = b->pos[i];
+    s->witness = b->d_wit + b->off_w[p] * kD;
+    s->challenges = b->d_cs + b->off_c[p];
+    s->commitment = o.com + p * b->krows * kD;
+    s->claims = o.claims + p;
+    s->msgs = o.msgs + b->off_msg[p];
+    s->finals = o.fin + p * b->reps * 2;
+    s->evals = o.ev + p * b->reps * 2;
+    s->rhs = o.rhs + p * b->reps * 2 * kD;
+    s->ok = o.ok + p * b->reps;
+    s->m = b->m[i];
+    s->kappa = b->kappa[i];
+    s->nrounds = b->nrounds[p];
+    return SALSAA_OK;
+}
+
+int salsaa_batch_step(salsaa_batch *b, int use_graph) {
+    SALSAA_CHECK(b != nullptr, "null batch");
+    CTX_OK(b->ctx);
+    Context *ctx = &b->ctx->c;
+    if (!use_graph) return run_step(b);
+    if (!b->exec) {
+        // capture; every buffer the step touches is owned by the plan, so the
+        // graph stays valid whatever else runs on the context between replays
+        SALSAA_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int st = run_step(b);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+        if (st != SALSAA_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        SALSAA_HIP(e);
+        b->graph = graph;
+        SALSAA_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    SALSAA_HIP(hipGraphLaunch(b->exec, ctx->stream));
+    return SALSAA_OK;
+}
+
+int salsaa_batch_step_plain(salsaa_batch *b) {
+    SALSAA_CHECK(b != nullptr, "null batch");
+    CTX_OK(b->ctx);
+    if (!b->out[1].com) SALSAA_TRY(alloc_outputs(b, b->out[1]));
+    salsaa_ctx *ctx = b->ctx;
+    const Outputs &o = b->out[1];
+    for (size_t i = 0; i < b->n; i++) {
+        const size_t p = b->pos[i], nr = b->nrounds[p];
+        uint64_t *w = b->d_wit + b->off_w[p] * kD;
+        SALSAA_TRY(salsaa_convert(ctx, w, b->m[i], SALSAA_REP_COEFF, SALSAA_REP_INCOMPLETE));
+        SALSAA_TRY(salsaa_convert(ctx, w, b->m[i], SALSAA_REP_INCOMPLETE, SALSAA_REP_COEFF));
+        SALSAA_TRY(salsaa_parallel_dot_series_matrix(ctx, b->views[i], w, 1, b->m[i],
+                                                     o.com + p * b->krows * kD));
+        SALSAA_TRY(salsaa_norm_check(ctx, w, 1, b->m[i] * kD, b->d_cs + b->off_c[p], b->reps,
+                                     b->fq2, b->nr, o.claims + p, o.msgs + b->off_msg[p],
+                                     o.fin + p * b->reps * 2, o.ev + p * b->reps * 2,
+                                     o.rhs + p * b->reps * 2 * kD, o.ok + p * b->reps));
+        (void)nr;
+    }
+    return SALSAA_OK;
+}
+
+int salsaa_batch_tuning(const salsaa_batch *b, size_t *kchunk, int *cpt_first, float *commit_ms,
+                        float *norm_check_ms) {
+    SALSAA_CHECK(b != nullptr, "null batch");
+    if (kchunk) *kchunk = b->kchunk;
+    if (cpt_first) *cpt_first = b->cpt_first;
+    if (commit_ms) *commit_ms = b->t_commit;
+    if (norm_check_ms) *norm_check_ms = b->t_nc;
+    return SALSAA_OK;
+}
+
+}  // extern "C"

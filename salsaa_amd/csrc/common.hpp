@@ -83,10 +83,20 @@ struct Context {
 
 struct Key {
     Context *ctx = nullptr;
-    size_t kappa = 0, m = 0;
-    double *d_A = nullptr;   // slot-major full-NTT [64][kappa][m] as exact fp64 (VALU path)
-    int8_t *d_AB = nullptr;  // fragment-tiled digit planes (MFMA path, kappa <= kMfmaMaxKappa)
-    size_t pstride = 0;      // K-tiles of the planes (plane_ktiles(m))
+    size_t kappa = 0, m = 0;  // rows and length this key commits with
+    double *d_A = nullptr;    // slot-major full-NTT [64][a_rows][a_m] as exact fp64 (VALU path)
+    int8_t *d_AB = nullptr;   // fragment-tiled digit planes (MFMA path, kappa <= kMfmaMaxKappa)
+    size_t a_rows = 0, a_m = 0;     // shape of the stored matrix (a view commits with a prefix)
+    size_t a_kt = 0, a_ktiles = 0;  // planes: row tiles per K step, K tiles (plane_ktiles(a_m))
+    bool owner = true;              // false for a view sharing another key's storage
+};
+
+// One job of a batched MFMA commit: at most 16 witness rows with their own
+// fragment-tiled planes (rtiles row tiles per K step, ktiles K tiles).
+struct CommitJob {
+    const int8_t *WB = nullptr;
+    double *partial = nullptr;  // [ceil(ktiles / kchunk)][64][kappa][rows]
+    int ktiles = 0, rtiles = 0, ntile = 0, rows = 0;
 };
 
 // The MFMA commit keeps all of a key's digit rows in one block (16 row tiles).
@@ -213,19 +223,71 @@ int launch_decomp_to_slotmajor(Context *ctx, const uint64_t *W, size_t r, size_t
 int launch_ntt_full_to_planes(Context *ctx, const uint64_t *src, size_t rows, size_t m,
                               size_t row_lo, size_t stride_lo, size_t stride_hi, int8_t *dst,
                               size_t pstride);
-int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t strideA,
-                       const int8_t *WB, size_t r, size_t m, size_t strideW, uint64_t *out);
+// One job of the batched witness -> planes transform: witness rows
+// first .. first + rows - 1 (instance order) into planes [64][ktiles][rtiles]
+// KiB fragments, rtiles = ceil(7 rows / 16); rows * ktiles / 2 blocks from blk0.
+struct PlaneJob {
+    int8_t *dst = nullptr;
+    int first = 0, rows = 0, ktiles = 0, blk0 = 0;
+};
+int launch_ntt_planes_batch(Context *ctx, const uint64_t *arena, const unsigned long long *inst_off,
+                            const unsigned long long *inst_m, const PlaneJob *jobs, size_t njobs,
+                            size_t nblk);
+int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
+                       size_t a_ktiles, const int8_t *WB, size_t r, size_t m, size_t ktW,
+                       uint64_t *out);
+// table[b] = (job, z * 64 + s) for each of nblk logical blocks; every job's
+// partial sums are left for launch_commit_finalize_batch
+int launch_commit_mfma_batch(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
+                             size_t a_ktiles, const CommitJob *jobs, const uint2 *table,
+                             size_t nblk, size_t kchunk, size_t max_tiles);
+int commit_batch_occupancy(size_t kappa, size_t max_tiles);
 int launch_commit_finalize(Context *ctx, const double *partial, size_t ksplit, size_t kappa,
                            size_t r, uint64_t *out);
 int launch_radix(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
                  uint64_t radix, uint64_t *d_radix);
-int launch_commit_mac(Context *ctx, const double *A_sm, size_t kappa, size_t mA,
+int launch_commit_mac(Context *ctx, const double *A_sm, size_t kappa, size_t a_rows, size_t mA,
                       const double *W_sm, size_t r, size_t m, uint64_t *out);
+// One finalize job: partial [ks][64][kappa][rows]; (j, k) -> out + j osj + k osk
+struct FinJob {
+    const double *partial = nullptr;
+    uint64_t *out = nullptr;
+    long long osj = 0, osk = 0;
+    int ks = 0, rows = 0, blk0 = 0;
+};
+int launch_commit_finalize_batch(Context *ctx, const FinJob *jobs, size_t njobs, size_t nblk,
+                                 size_t kappa);
 int launch_sample_uniform(Context *ctx, uint64_t *out, size_t n, uint64_t seed, uint64_t stream);
 int launch_sample_bounded(Context *ctx, uint64_t *out, size_t n, uint64_t bound, uint64_t seed,
                           uint64_t stream);
 int launch_challenges(Context *ctx, uint64_t *cs, size_t reps, size_t nrounds, int fq2,
                       uint64_t seed);
+// Transcript outputs of the norm check and their element strides: claims[row],
+// msgs[rep][t][row][6], finals/evals[rep][row][2], rhs[rep][row][2][d],
+// ok[rep][row] in the plain layout (nc_out_plain); a batch lays each row's
+// transcript out contiguously instead.  Null pointers (except ok) are skipped.
+struct NcOut {
+    uint64_t *claims = nullptr, *msgs = nullptr, *finals = nullptr, *evals = nullptr,
+             *rhs = nullptr;
+    int32_t *ok = nullptr;
+    size_t claim_row = 1, msg_rep = 0, msg_t = 0, msg_row = 6, fe_rep = 0, fe_row = 2,
+           rhs_rep = 0, rhs_row = 2 * kD, ok_rep = 0, ok_row = 1;
+};
+NcOut nc_out_plain(size_t rows, size_t reps, size_t nrounds, uint64_t *claims, uint64_t *msgs,
+                   uint64_t *finals, uint64_t *evals, uint64_t *rhs, int32_t *ok);
+// Caller-owned norm check scratch (otherwise the context's scratch is used)
+struct NcScratch {
+    unsigned long long *macc = nullptr;
+    double *bufA = nullptr, *bufB = nullptr;
+    size_t macc_bytes = 0, bufA_bytes = 0, bufB_bytes = 0;
+};
+void nc_scratch_need(size_t rows, size_t L, size_t reps, size_t *macc_bytes, size_t *bufA_bytes,
+                     size_t *bufB_bytes);
+// cs_row: u64 between consecutive rows' challenge blocks (0: shared);
+// cpt_first: columns per thread of the first pass (0: automatic)
+int launch_norm_check_ex(Context *ctx, const uint64_t *w, size_t rows, size_t L,
+                         const uint64_t *cs, size_t cs_row, size_t reps, int fq2, uint64_t nr,
+                         const NcOut &o, int cpt_first, const NcScratch *ext);
 int launch_norm_check(Context *ctx, const uint64_t *w, size_t rows, size_t L,
                       const uint64_t *cs, size_t reps, int fq2, uint64_t nr, uint64_t *claims,
                       uint64_t *msgs, uint64_t *finals, uint64_t *evals, uint64_t *rhs,

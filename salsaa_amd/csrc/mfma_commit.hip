@@ -48,31 +48,23 @@ struct CommitShape {
     static constexpr int LDS_BYTES = 2 * STAGE > DR * DC * 4 ? 2 * STAGE : DR * DC * 4;
 };
 
+// One block's work: slot s, K tiles [ktb, ktb + nsteps), all key rows
+// j < kappa and the nbv witness tiles at WB (rows k < rows of the block).
+// Key fragment (s, kt, t) sits at AB + s a_slot + (kt a_kt + t) KiB, witness
+// fragment (s, kt, t) at WB + s w_slot + (kt w_kt + t) KiB.  Writes the
+// recombined residues to part[j * prow + k].
 template <int TM, int TN, int WN>
-__global__ __launch_bounds__(256 * WN) void k_commit_mfma(
-    const int8_t *AB, int kappa, int ktA, const int8_t *WB, int r,
-    int ktW, int kchunk, int ktend, int nbx, double *__restrict__ partial,
-    const double *__restrict__ tab, Mod M) {
+__device__ __forceinline__ void commit_body(char *lds, const int8_t *AB, size_t a_slot, int a_kt,
+                                            int kappa, const int8_t *WB, size_t w_slot, int w_kt,
+                                            int nbv, int rows, int s, int ktb, int nsteps,
+                                            double *__restrict__ part, int prow,
+                                            const double *__restrict__ tab, Mod M) {
     using S = CommitShape<TM, TN, WN>;
-    __shared__ __align__(16) char lds[S::LDS_BYTES];
-    // XCD-aware block order: the 8 XCDs take consecutive runs of logical
-    // blocks, so the blocks sharing one slot's key chunk share an L2
-    const unsigned nblk = gridDim.x;
-    const unsigned pid = blockIdx.x;
-    const unsigned lid = (nblk % 8 == 0) ? (pid % 8) * (nblk / 8) + pid / 8 : pid;
-    const int nb = (int)(lid % (unsigned)nbx);
-    const unsigned rest = lid / (unsigned)nbx;
-    const int s = (int)(rest % 64u);
-    const int z = (int)(rest / 64u);
-
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = w & 3, wn = w >> 2;
-    const int rtA = (7 * kappa + 15) / 16, rtB = (7 * r + 15) / 16;
-    const int nbv = min(GT, rtB - GT * nb);  // witness tiles of this block
+    const int rtA = (7 * kappa + 15) / 16;
     const int nchunk = (rtA + nbv) * 64;
-    const int ktb = z * kchunk;
-    const int nsteps = min(ktend, ktb + kchunk) - ktb;  // even
 
     // staging: chunk c = tid + NTHR i is 16 bytes of fragment c / 64 (key
     // tiles first, then the block's witness tiles); one wave moves one KiB.
@@ -87,13 +79,13 @@ __global__ __launch_bounds__(256 * WN) void k_commit_mfma(
         const int c = c0 < nchunk ? c0 : (c0 & 63);
         const int t = c >> 6, off = (c & 63) * 16;
         if (t < rtA) {
-            gp[i] = AB + (((size_t)s * ktA + ktb) * rtA + t) * FRAG + off;
-            gs[i] = rtA * FRAG;
+            gp[i] = AB + s * a_slot + ((size_t)ktb * a_kt + t) * FRAG + off;
+            gs[i] = a_kt * FRAG;
             lo[i] = t * FRAG + off;
         } else {
             const int tb = t - rtA;
-            gp[i] = WB + (((size_t)s * ktW + ktb) * rtB + GT * nb + tb) * FRAG + off;
-            gs[i] = rtB * FRAG;
+            gp[i] = WB + s * w_slot + ((size_t)ktb * w_kt + tb) * FRAG + off;
+            gs[i] = w_kt * FRAG;
             lo[i] = (S::SA + tb) * FRAG + off;
         }
     }
@@ -181,8 +173,8 @@ __global__ __launch_bounds__(256 * WN) void k_commit_mfma(
         __syncthreads();
         for (int o = tid; o < 256; o += S::NTHR) {
             const int jl = o >> 4, kl = o & 15;
-            const int j = 16 * gi + jl, k = 16 * nb + kl;
-            if (j >= kappa || k >= r) continue;
+            const int j = 16 * gi + jl, k = kl;
+            if (j >= kappa || k >= rows) continue;
             // diagonals E_t = sum_{u+v=t} D[7jl+u][7kl+v]: at most 7 int32
             // terms, below 2^34 < q in magnitude and exact in fp64
             long long E[13];
@@ -198,20 +190,66 @@ __global__ __launch_bounds__(256 * WN) void k_commit_mfma(
                 x += mul_lazy((double)E[t], wt[t], M);
                 if ((t % 7) == 6) x = reduce_lazy(x, M);
             }
-            partial[(((size_t)z * 64 + s) * kappa + j) * r + k] = reduce(x, M);
+            part[(size_t)j * prow + k] = reduce(x, M);
         }
     }
 }
 
+// XCD-aware block order: the 8 XCDs take consecutive runs of logical
+// blocks, so blocks that share one slot's key chunk share an L2
+__device__ __forceinline__ unsigned logical_block() {
+    const unsigned nblk = gridDim.x, pid = blockIdx.x;
+    return (nblk % 8 == 0) ? (pid % 8) * (nblk / 8) + pid / 8 : pid;
+}
+
+// One commit: W planes with rtB = ceil(7 r / 16) tiles per K step; block
+// (nb, s, z) owns witness rows 16 nb .. 16 nb + 15 and K chunk z.
 template <int TM, int TN, int WN>
+__global__ __launch_bounds__(256 * WN) void k_commit_mfma(
+    const int8_t *AB, int kappa, int a_kt, int a_ktiles, const int8_t *WB, int r, int ktW,
+    int kchunk, int ktend, int nbx, double *__restrict__ partial, const double *__restrict__ tab,
+    Mod M) {
+    __shared__ __align__(16) char lds[CommitShape<TM, TN, WN>::LDS_BYTES];
+    const unsigned lid = logical_block();
+    const int nb = (int)(lid % (unsigned)nbx);
+    const unsigned rest = lid / (unsigned)nbx;
+    const int s = (int)(rest % 64u);
+    const int z = (int)(rest / 64u);
+    const int rtB = (7 * r + 15) / 16;
+    const int ktb = z * kchunk;
+    commit_body<TM, TN, WN>(lds, AB, (size_t)a_ktiles * a_kt * FRAG, a_kt, kappa,
+                            WB + (size_t)GT * nb * FRAG, (size_t)ktW * rtB * FRAG, rtB,
+                            min(GT, rtB - GT * nb), min(16, r - 16 * nb), s, ktb,
+                            min(ktend, ktb + kchunk) - ktb,
+                            partial + ((size_t)z * 64 + s) * kappa * r + 16 * nb, r, tab, M);
+}
+
+// A batch of commits against one key: job j is a block of at most 16 witness
+// rows with its own planes and K extent; table[b] = (job, z * 64 + s) in
+// logical block order.
+template <int TM, int TN, int WN>
+__global__ __launch_bounds__(256 * WN) void k_commit_mfma_batch(
+    const int8_t *AB, int kappa, int a_kt, int a_ktiles, const CommitJob *__restrict__ jobs,
+    const uint2 *__restrict__ table, int kchunk, const double *__restrict__ tab, Mod M) {
+    __shared__ __align__(16) char lds[CommitShape<TM, TN, WN>::LDS_BYTES];
+    const uint2 e = table[logical_block()];
+    const CommitJob jb = jobs[e.x];
+    const int s = (int)(e.y & 63u), z = (int)(e.y >> 6);
+    const int ktb = z * kchunk;
+    commit_body<TM, TN, WN>(lds, AB, (size_t)a_ktiles * a_kt * FRAG, a_kt, kappa, jb.WB,
+                            (size_t)jb.ktiles * jb.rtiles * FRAG, jb.rtiles, jb.ntile, jb.rows, s,
+                            ktb, min(jb.ktiles, ktb + kchunk) - ktb,
+                            jb.partial + ((size_t)z * 64 + s) * kappa * jb.rows, jb.rows, tab, M);
+}
+
+template <int TM, int TN, int WN, bool BATCH>
 int occupancy() {
     static int occ = 0;
     if (occ == 0) {
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &n, reinterpret_cast<const void *>(&k_commit_mfma<TM, TN, WN>), 256 * WN, 0) !=
-                hipSuccess ||
-            n < 1)
+        const void *fn = BATCH ? reinterpret_cast<const void *>(&k_commit_mfma_batch<TM, TN, WN>)
+                               : reinterpret_cast<const void *>(&k_commit_mfma<TM, TN, WN>);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 256 * WN, 0) != hipSuccess || n < 1)
             n = 1;
         occ = n;
     }
@@ -219,14 +257,14 @@ int occupancy() {
 }
 
 template <int TM, int TN, int WN>
-int run(Context *ctx, const int8_t *AB, size_t kappa, size_t ktA, const int8_t *WB, size_t r,
-        size_t ktW, size_t ktend, uint64_t *out) {
+int run(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt, size_t a_ktiles,
+        const int8_t *WB, size_t r, size_t ktW, size_t ktend, uint64_t *out) {
     Mod M{ctx->qd, ctx->qinv};
     const size_t nbx = (r + 15) / 16;
     // K chunks: at most 1024 tiles (2^16 columns, exact int32 sums), an even
     // number of tiles; pick the split that minimises (block waves x chunk
     // length + a fixed per-block cost) for this variant's occupancy
-    const size_t slots = (size_t)ctx->ncu * occupancy<TM, TN, WN>();
+    const size_t slots = (size_t)ctx->ncu * occupancy<TM, TN, WN, false>();
     size_t best_ks = 0, best_kc = 0;
     double best = 0;
     for (size_t ks = (ktend + 1023) / 1024; ks <= ktend / 2; ks *= 2) {
@@ -241,42 +279,95 @@ int run(Context *ctx, const int8_t *AB, size_t kappa, size_t ktA, const int8_t *
             best_kc = kc;
         }
     }
-    if (best_ks == 0) {  // ktend == 2
-        best_ks = 1;
-        best_kc = ktend;
-    }
     double *partial = nullptr;
     SALSAA_TRY(scratch_get(ctx, 7, best_ks * 64 * kappa * r * sizeof(double), (void **)&partial));
     const size_t nblk = nbx * 64 * best_ks;
     SALSAA_CHECK(nblk < (1u << 31), "commit grid too large");
     hipLaunchKernelGGL((k_commit_mfma<TM, TN, WN>), dim3((unsigned)nblk), dim3(256 * WN), 0,
-                       ctx->stream, AB, (int)kappa, (int)ktA, WB, (int)r, (int)ktW, (int)best_kc,
-                       (int)ktend, (int)nbx, partial, ctx->d_tables, M);
+                       ctx->stream, AB, (int)kappa, (int)a_kt, (int)a_ktiles, WB, (int)r, (int)ktW,
+                       (int)best_kc, (int)ktend, (int)nbx, partial, ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());
     return launch_commit_finalize(ctx, partial, best_ks, kappa, r, out);
 }
 
 template <int TM>
-int run_n(Context *ctx, const int8_t *AB, size_t kappa, size_t ktA, const int8_t *WB, size_t r,
-          size_t ktW, size_t ktend, uint64_t *out) {
+int run_n(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt, size_t a_ktiles,
+          const int8_t *WB, size_t r, size_t ktW, size_t ktend, uint64_t *out) {
     const size_t rtB = (7 * r + 15) / 16;
-    if (rtB <= 2) return run<TM, 1, 2>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
-    if (rtB <= 4) return run<TM, 2, 2>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
-    return run<TM, 4, 2>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
+    if (rtB <= 2) return run<TM, 1, 2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
+    if (rtB <= 4) return run<TM, 2, 2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
+    return run<TM, 4, 2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
+}
+
+template <int TM, int TN, int WN>
+int run_batch(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt, size_t a_ktiles,
+              const CommitJob *jobs, const uint2 *table, size_t nblk, size_t kchunk) {
+    Mod M{ctx->qd, ctx->qinv};
+    hipLaunchKernelGGL((k_commit_mfma_batch<TM, TN, WN>), dim3((unsigned)nblk), dim3(256 * WN), 0,
+                       ctx->stream, AB, (int)kappa, (int)a_kt, (int)a_ktiles, jobs, table,
+                       (int)kchunk, ctx->d_tables, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+template <int TM>
+int run_batch_n(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt, size_t a_ktiles,
+                const CommitJob *jobs, const uint2 *table, size_t nblk, size_t kchunk,
+                size_t max_tiles) {
+    if (max_tiles <= 2)
+        return run_batch<TM, 1, 2>(ctx, AB, kappa, a_kt, a_ktiles, jobs, table, nblk, kchunk);
+    if (max_tiles <= 4)
+        return run_batch<TM, 2, 2>(ctx, AB, kappa, a_kt, a_ktiles, jobs, table, nblk, kchunk);
+    return run_batch<TM, 4, 2>(ctx, AB, kappa, a_kt, a_ktiles, jobs, table, nblk, kchunk);
 }
 
 }  // namespace
 
-int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t ktA,
-                       const int8_t *WB, size_t r, size_t m, size_t ktW, uint64_t *out) {
+int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
+                       size_t a_ktiles, const int8_t *WB, size_t r, size_t m, size_t ktW,
+                       uint64_t *out) {
     const size_t ktend = plane_ktiles(m);
     SALSAA_CHECK(kappa >= 1 && kappa <= kMfmaMaxKappa, "MFMA commit needs kappa <= 36");
-    SALSAA_CHECK(ktend == ktW && ktend <= ktA, "plane K-tiles do not cover the witness");
-    SALSAA_CHECK(ktA < (1u << 24) && r < (1u << 20), "commit too large");
+    SALSAA_CHECK(ktend == ktW && ktend <= a_ktiles, "plane K-tiles do not cover the witness");
+    SALSAA_CHECK(a_ktiles < (1u << 24) && r < (1u << 20), "commit too large");
     const size_t rtA = (7 * kappa + 15) / 16;
-    if (rtA <= 4) return run_n<1>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
-    if (rtA <= 8) return run_n<2>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
-    return run_n<4>(ctx, AB, kappa, ktA, WB, r, ktW, ktend, out);
+    SALSAA_CHECK(rtA <= a_kt, "key planes hold fewer rows than kappa");
+    if (rtA <= 4) return run_n<1>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
+    if (rtA <= 8) return run_n<2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
+    return run_n<4>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
+}
+
+int launch_commit_mfma_batch(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
+                             size_t a_ktiles, const CommitJob *jobs, const uint2 *table,
+                             size_t nblk, size_t kchunk, size_t max_tiles) {
+    SALSAA_CHECK(kappa >= 1 && kappa <= kMfmaMaxKappa, "MFMA commit needs kappa <= 36");
+    SALSAA_CHECK(max_tiles >= 1 && max_tiles <= 7, "a batch job holds at most 16 witness rows");
+    SALSAA_CHECK(kchunk >= 2 && kchunk <= 1024 && kchunk % 2 == 0, "K chunk must be even, <= 1024");
+    SALSAA_CHECK(nblk >= 1 && nblk < (1u << 31), "commit grid size out of range");
+    const size_t rtA = (7 * kappa + 15) / 16;
+    SALSAA_CHECK(rtA <= a_kt, "key planes hold fewer rows than kappa");
+    if (rtA <= 4)
+        return run_batch_n<1>(ctx, AB, kappa, a_kt, a_ktiles, jobs, table, nblk, kchunk, max_tiles);
+    if (rtA <= 8)
+        return run_batch_n<2>(ctx, AB, kappa, a_kt, a_ktiles, jobs, table, nblk, kchunk, max_tiles);
+    return run_batch_n<4>(ctx, AB, kappa, a_kt, a_ktiles, jobs, table, nblk, kchunk, max_tiles);
+}
+
+int commit_batch_occupancy(size_t kappa, size_t max_tiles) {
+    const size_t rtA = (7 * kappa + 15) / 16;
+    const int tm = rtA <= 4 ? 1 : rtA <= 8 ? 2 : 4;
+    const int tn = max_tiles <= 2 ? 1 : max_tiles <= 4 ? 2 : 4;
+#define SALSAA_OCC(TM)                                                                          \
+    if (tm == TM) {                                                                              \
+        if (tn == 1) return occupancy<TM, 1, 2, true>();                                         \
+        if (tn == 2) return occupancy<TM, 2, 2, true>();                                         \
+        return occupancy<TM, 4, 2, true>();                                                      \
+    }
+    SALSAA_OCC(1)
+    SALSAA_OCC(2)
+    SALSAA_OCC(4)
+#undef SALSAA_OCC
+    return 1;
 }
 
 }  // namespace salsaa

@@ -293,18 +293,15 @@ __device__ __forceinline__ void store_planes(int8_t *__restrict__ dst, const uin
 // PLANES = false: exact doubles dst[(s * rows + row) * m + i];
 // PLANES = true: fragment-tiled int8 digit planes (store_planes) with
 // `pstride` K-tiles of 64 columns.
+// Full NTT of the (up to) 128 ring elements at base + i0 * d of one row into
+// the row's slot-major doubles or digit planes.
 template <bool PLANES>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_ntt_full_sm(const uint64_t *__restrict__ src, size_t rows,
-                                                    size_t m, size_t row_lo, size_t stride_lo,
-                                                    size_t stride_hi, void *__restrict__ dst,
-                                                    size_t pstride, const double *__restrict__ tab,
-                                                    Mod M) {
-    __shared__ uint64_t lds[EL * SROW];
-    const size_t row = blockIdx.y;
-    const size_t i0 = (size_t)blockIdx.x * EL;
+__device__ __forceinline__ void full_sm_block(uint64_t *lds, const uint64_t *base, size_t m,
+                                              size_t rows, size_t row, size_t i0, void *dst,
+                                              size_t pstride, const double *__restrict__ tab,
+                                              Mod M) {
     const int nel = i0 < m ? (int)min((size_t)EL, m - i0) : 0;
     const int tid = threadIdx.x, e = tid >> 3, t = tid & 7;
-    const uint64_t *base = src + (row % row_lo) * stride_lo + (row / row_lo) * stride_hi;
     load_block<false>(lds, base + i0 * kD, nel, tid);
     __syncthreads();
     double v[8];
@@ -318,6 +315,41 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
         store_planes((int8_t *)dst, lds, rows, row, pstride, i0, nel, tid);  // pstride = K-tiles
     else
         store_slotmajor((double *)dst, lds, rows, row, m, i0, nel, tid);
+}
+
+template <bool PLANES>
+__global__ __launch_bounds__(NT) void k_ntt_full_sm(const uint64_t *__restrict__ src, size_t rows,
+                                                    size_t m, size_t row_lo, size_t stride_lo,
+                                                    size_t stride_hi, void *__restrict__ dst,
+                                                    size_t pstride, const double *__restrict__ tab,
+                                                    Mod M) {
+    __shared__ uint64_t lds[EL * SROW];
+    const size_t row = blockIdx.y;
+    const uint64_t *base = src + (row % row_lo) * stride_lo + (row / row_lo) * stride_hi;
+    full_sm_block<PLANES>(lds, base, m, rows, row, (size_t)blockIdx.x * EL, dst, pstride, tab, M);
+}
+
+// Batched: job j turns witness rows first .. first + rows - 1 (each its own
+// length, zero beyond it) into one planes matrix of `ktiles` K tiles; block
+// b of the job handles row b / (ktiles / 2), columns 128 (b % (ktiles / 2)).
+__global__ __launch_bounds__(NT) void k_ntt_planes_batch(
+    const uint64_t *__restrict__ arena, const unsigned long long *__restrict__ inst_off,
+    const unsigned long long *__restrict__ inst_m, const PlaneJob *__restrict__ jobs, int njobs,
+    const double *__restrict__ tab, Mod M) {
+    __shared__ uint64_t lds[EL * SROW];
+    const int b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].blk0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const PlaneJob jb = jobs[lo];
+    const int per_row = jb.ktiles / 2, local = b - jb.blk0;
+    const int row = local / per_row, cb = local % per_row;
+    const int inst = jb.first + row;
+    full_sm_block<true>(lds, arena + inst_off[inst] * kD, inst_m[inst], jb.rows, row,
+                        (size_t)cb * EL, jb.dst, jb.ktiles, tab, M);
 }
 
 // signed digit t of x (arithmetic.rs:1496-1570): x is split into pos (2x <= q)
@@ -434,6 +466,17 @@ int launch_ntt_full_to_planes(Context *ctx, const uint64_t *src, size_t rows, si
     dim3 grid(ceil_div(m, EL), (unsigned)rows);
     hipLaunchKernelGGL(k_ntt_full_sm<true>, grid, dim3(NT), 0, ctx->stream, src, rows, m, row_lo,
                        stride_lo, stride_hi, (void *)dst, pstride, ctx->d_tables, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_ntt_planes_batch(Context *ctx, const uint64_t *arena, const unsigned long long *inst_off,
+                            const unsigned long long *inst_m, const PlaneJob *jobs, size_t njobs,
+                            size_t nblk) {
+    SALSAA_CHECK(njobs >= 1 && nblk >= 1 && nblk < (1u << 31), "plane grid size out of range");
+    Mod M{ctx->qd, ctx->qinv};
+    hipLaunchKernelGGL(k_ntt_planes_batch, dim3((unsigned)nblk), dim3(NT), 0, ctx->stream, arena,
+                       inst_off, inst_m, jobs, (int)njobs, ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }

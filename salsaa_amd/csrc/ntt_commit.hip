@@ -27,7 +27,7 @@ constexpr int CM_JT = 8, CM_KT = 8;  // per-wave register tile (kappa x r)
 // combined by a butterfly reduce-scatter that leaves output (a, b) = lane in
 // lane a * 8 + b.  partial[z][s][j][k] in [0, q).
 __global__ __launch_bounds__(CM_THREADS) void k_commit_mac(
-    const double *__restrict__ A_sm, int kappa, size_t mA, const double *__restrict__ W_sm,
+    const double *__restrict__ A_sm, int kappa, int a_rows, size_t mA, const double *__restrict__ W_sm,
     int r, size_t m, size_t chunk, double *__restrict__ partial, Mod M) {
     const int jtiles = (kappa + CM_JT - 1) / CM_JT, ktiles = (r + CM_KT - 1) / CM_KT;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -38,7 +38,7 @@ __global__ __launch_bounds__(CM_THREADS) void k_commit_mac(
     const int s = blockIdx.y;
     const size_t ib = (size_t)blockIdx.z * chunk;
     const size_t ie = min(m, ib + chunk);
-    const double *Ab = A_sm + ((size_t)s * kappa) * mA;
+    const double *Ab = A_sm + ((size_t)s * a_rows) * mA;
     const double *Wb = W_sm + ((size_t)s * r) * m;
 
     double acc[CM_JT * CM_KT];
@@ -86,16 +86,20 @@ __global__ __launch_bounds__(CM_THREADS) void k_commit_mac(
 // One block of 64 threads per output ring element (j, k): thread s sums the
 // K-split partials of slot s; thread c then forms IncompleteNTT coefficient c
 // as sum_s fin[s][c] * v[s], fin being the linear map IncNTT o INTT_64.
-__global__ __launch_bounds__(64) void k_commit_finalize(const double *__restrict__ partial,
-                                                        int ksplit, int kappa, int r,
-                                                        uint64_t *__restrict__ out,
-                                                        const double *__restrict__ tab, Mod M) {
+// 64 threads per (j, k): thread s sums slot s's partials over the K chunks,
+// then thread c forms coefficient c of the IncompleteNTT result through the
+// precomputed slots -> IncompleteNTT map.  partial_jk points at (z = 0, s = 0)
+// of this (j, k); consecutive slots are slot_stride apart, chunks
+// chunk_stride apart.
+__device__ __forceinline__ void finalize_one(const double *__restrict__ partial_jk, int ksplit,
+                                             size_t slot_stride, size_t chunk_stride,
+                                             uint64_t *__restrict__ out,
+                                             const double *__restrict__ tab, Mod M) {
     __shared__ double vs[64];
-    const int jk = blockIdx.x;
     const int t = threadIdx.x;
-    const size_t stride = (size_t)64 * kappa * r;
     double x = 0.0;
-    for (int z = 0; z < ksplit; z++) x = addmod(x, partial[(size_t)z * stride + (size_t)t * kappa * r + jk], M);
+    for (int z = 0; z < ksplit; z++)
+        x = addmod(x, partial_jk[(size_t)z * chunk_stride + (size_t)t * slot_stride], M);
     vs[t] = x;
     __syncthreads();
     const double *fin = tab + TAB_FIN;
@@ -105,12 +109,40 @@ __global__ __launch_bounds__(64) void k_commit_finalize(const double *__restrict
         acc += mul_lazy(fin[s * 64 + t], vs[s], M);
         if ((s & 3) == 3) acc = reduce_lazy(acc, M);
     }
-    out[(size_t)jk * kD + t] = d2u(reduce(acc, M));
+    out[t] = d2u(reduce(acc, M));
+}
+
+__global__ __launch_bounds__(64) void k_commit_finalize(const double *__restrict__ partial,
+                                                        int ksplit, int kappa, int r,
+                                                        uint64_t *__restrict__ out,
+                                                        const double *__restrict__ tab, Mod M) {
+    const int jk = blockIdx.x;
+    const size_t plane = (size_t)kappa * r;
+    finalize_one(partial + jk, ksplit, plane, 64 * plane, out + (size_t)jk * kD, tab, M);
+}
+
+// batched: job of block b found by binary search over jobs[].blk0
+__global__ __launch_bounds__(64) void k_commit_finalize_batch(const FinJob *__restrict__ jobs,
+                                                              int njobs, int kappa,
+                                                              const double *__restrict__ tab,
+                                                              Mod M) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].blk0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const FinJob jb = jobs[lo];
+    const int jk = b - jb.blk0, j = jk / jb.rows, k = jk % jb.rows;
+    const size_t plane = (size_t)kappa * jb.rows;
+    finalize_one(jb.partial + jk, jb.ks, plane, 64 * plane, jb.out + j * jb.osj + k * jb.osk,
+                 tab, M);
 }
 
 }  // namespace
 
-int launch_commit_mac(Context *ctx, const double *A_sm, size_t kappa, size_t mA,
+int launch_commit_mac(Context *ctx, const double *A_sm, size_t kappa, size_t a_rows, size_t mA,
                       const double *W_sm, size_t r, size_t m, uint64_t *out) {
     Mod M{ctx->qd, ctx->qinv};
     const size_t tiles = ((kappa + CM_JT - 1) / CM_JT) * ((r + CM_KT - 1) / CM_KT);
@@ -124,10 +156,21 @@ int launch_commit_mac(Context *ctx, const double *A_sm, size_t kappa, size_t mA,
     double *partial = nullptr;
     SALSAA_TRY(scratch_get(ctx, 7, ksplit * 64 * kappa * r * sizeof(double), (void **)&partial));
     dim3 grid((unsigned)groups, 64, (unsigned)ksplit);
-    hipLaunchKernelGGL(k_commit_mac, grid, dim3(CM_THREADS), 0, ctx->stream, A_sm, (int)kappa, mA,
+    hipLaunchKernelGGL(k_commit_mac, grid, dim3(CM_THREADS), 0, ctx->stream, A_sm, (int)kappa,
+                       (int)a_rows, mA,
                        W_sm, (int)r, m, chunk, partial, M);
     SALSAA_HIP(hipGetLastError());
     return launch_commit_finalize(ctx, partial, ksplit, kappa, r, out);
+}
+
+int launch_commit_finalize_batch(Context *ctx, const FinJob *jobs, size_t njobs, size_t nblk,
+                                 size_t kappa) {
+    SALSAA_CHECK(njobs >= 1 && nblk >= 1 && nblk < (1u << 31), "finalize grid size out of range");
+    Mod M{ctx->qd, ctx->qinv};
+    hipLaunchKernelGGL(k_commit_finalize_batch, dim3((unsigned)nblk), dim3(64), 0, ctx->stream,
+                       jobs, (int)njobs, (int)kappa, ctx->d_tables, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
 }
 
 int launch_commit_finalize(Context *ctx, const double *partial, size_t ksplit, size_t kappa,

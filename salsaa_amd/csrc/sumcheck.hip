@@ -74,6 +74,8 @@ __device__ __forceinline__ void x_sq_mac(F2 &acc, F2 x, double nr, Mod M) {
     acc.b = reduce_lazy(acc.b + im, M);
 }
 
+// challenges of one row: cs[rep][t][2] (rows either share one block or each
+// row has its own, cs_row u64 apart)
 template <bool FQ2>
 __device__ __forceinline__ F2 load_chal(const uint64_t *cs, int rep, int nrounds, int t) {
     const uint64_t *p = cs + ((size_t)rep * nrounds + t) * 2;
@@ -123,11 +125,12 @@ __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) 
 template <int REPS, bool FQ2, bool SNR>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
     const uint64_t *__restrict__ w, size_t L, int rows, int rep0, int reps_total,
-    int add_claim, const uint64_t *__restrict__ cs, int nrounds, double nr, int cpt,
-    double *__restrict__ out, unsigned long long *__restrict__ macc,
+    int add_claim, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr,
+    int cpt, double *__restrict__ out, unsigned long long *__restrict__ macc,
     unsigned long long *__restrict__ cacc, Mod M) {
     __shared__ double red[8 * 64];
     const int row = blockIdx.y;
+    const uint64_t *cs = cs0 + row * cs_row;
     const size_t S = L >> 4;
     F2 c[REPS][4];
 #pragma unroll
@@ -228,11 +231,12 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
 // F2 input in [reps][rows][len] (pairs of doubles), rounds t0..t0+3.
 template <bool FQ2, bool SNR>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
-    const double *__restrict__ in, size_t len, int rows, int t0, const uint64_t *__restrict__ cs,
-    int nrounds, double nr, int cpt, double *__restrict__ out,
+    const double *__restrict__ in, size_t len, int rows, int t0, const uint64_t *__restrict__ cs0,
+    size_t cs_row, int nrounds, double nr, int cpt, double *__restrict__ out,
     unsigned long long *__restrict__ macc, Mod M) {
     __shared__ double red[8 * 64];
     const int row = blockIdx.y, rep = blockIdx.z;
+    const uint64_t *cs = cs0 + row * cs_row;
     const size_t S = len >> 4;
     F2 c[4];
 #pragma unroll
@@ -292,12 +296,9 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
 template <bool FQ2, bool FIRST, bool SNR>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
     const uint64_t *__restrict__ w, const double *__restrict__ in, size_t len, int rows, int t0,
-    const uint64_t *__restrict__ cs, int nrounds, double nr,
+    const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr,
     const unsigned long long *__restrict__ macc, const unsigned long long *__restrict__ cacc,
-    uint64_t *__restrict__ claims_out, uint64_t *__restrict__ msgs_out,
-    uint64_t *__restrict__ finals, uint64_t *__restrict__ evals, uint64_t *__restrict__ rhs_out,
-    int32_t *__restrict__ ok_out, int want_rhs, uint64_t q, const double *__restrict__ tab,
-    Mod M) {
+    NcOut o, int want_rhs, uint64_t q, const double *__restrict__ tab, Mod M) {
     __shared__ F2 cur[SC_FINAL_MAX];
     __shared__ F2 V[kD];
     __shared__ F2 base[kD];
@@ -308,6 +309,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
     __shared__ F2 chal[48];
     const int row = blockIdx.x, rep = blockIdx.y;
     const int tid = threadIdx.x;
+    const uint64_t *cs = cs0 + row * cs_row;
     // this repetition's challenges, loaded once (the round loop and the
     // verifier replay below would otherwise wait on one load per round)
     for (int tt = tid; tt < nrounds; tt += SC_THREADS) chal[tt] = load_chal<FQ2>(cs, rep, nrounds, tt);
@@ -407,9 +409,9 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
 #pragma unroll
             for (int i = 0; i < 32; i++) v[i] = rc[comp][2 * i + p];
             ntt_fwd_regs<32>(v, tab + TAB_FWD32, M);
-            uint64_t *o = rhs_out + (((size_t)rep * rows + row) * 2 + comp) * kD + 32 * p;
+            uint64_t *dst = o.rhs + rep * o.rhs_rep + row * o.rhs_row + comp * kD + 32 * p;
 #pragma unroll
-            for (int i = 0; i < 32; i++) o[i] = d2u(v[i]);
+            for (int i = 0; i < 32; i++) dst[i] = d2u(v[i]);
         }
     }
 
@@ -423,16 +425,16 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
     }
     if (tid == SC_THREADS - 1) claim_red = FIRST ? claim_local : u64_mod(cacc[row], M);
     __syncthreads();
-    if (msgs_out) {
+    if (o.msgs) {
         for (int x = tid; x < nrounds * 6; x += SC_THREADS) {
             const int tt = x / 6, k = x % 6;
             const double f = tt < t0 ? msg_pre[tt][k] : msg_local[tt - t0][k];
-            msgs_out[(((size_t)rep * nrounds + tt) * rows + row) * 6 + k] = d2u(f);
+            o.msgs[rep * o.msg_rep + tt * o.msg_t + row * o.msg_row + k] = d2u(f);
         }
     }
     if (tid == 0) {
         const double claim0 = claim_red;
-        if (rep == 0 && claims_out) claims_out[row] = d2u(claim0);
+        if (rep == 0 && o.claims) o.claims[row * o.claim_row] = d2u(claim0);
         F2 cl2 = {claim0, 0.0};
         int good = 1;
         for (int tt = 0; tt < nrounds; tt++) {
@@ -455,16 +457,16 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
             const F2 ct2 = x_mul<true, SNR>(ct, ct, nr, M);
             if (ct2.a != cl2.a || ct2.b != cl2.b) good = 0;
         }
-        const size_t o = (size_t)rep * rows + row;
-        if (finals) {
-            finals[o * 2] = d2u(cl2.a);
-            finals[o * 2 + 1] = d2u(cl2.b);
+        const size_t fe = rep * o.fe_rep + row * o.fe_row;
+        if (o.finals) {
+            o.finals[fe] = d2u(cl2.a);
+            o.finals[fe + 1] = d2u(cl2.b);
         }
-        if (evals) {
-            evals[o * 2] = d2u(ev.a);
-            evals[o * 2 + 1] = d2u(ev.b);
+        if (o.evals) {
+            o.evals[fe] = d2u(ev.a);
+            o.evals[fe + 1] = d2u(ev.b);
         }
-        ok_out[o] = good;
+        o.ok[rep * o.ok_rep + row * o.ok_row] = good;
     }
 }
 
@@ -474,49 +476,60 @@ __global__ void k_zero_u64(unsigned long long *p, size_t n) {
 }
 
 template <bool FQ2, bool SNR>
-int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L,
-                    const uint64_t *cs, size_t reps, double nr, uint64_t *claims,
-                    uint64_t *msgs, uint64_t *finals, uint64_t *evals, uint64_t *rhs,
-                    int32_t *ok) {
+int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L, const uint64_t *cs,
+                    size_t cs_row, size_t reps, double nr, const NcOut &o, int cpt_first,
+                    const NcScratch *ext) {
     Mod M{ctx->qd, ctx->qinv};
     const int n = __builtin_ctzll(L);
-    const int want_rhs = rhs != nullptr;
+    const int want_rhs = o.rhs != nullptr;
+    size_t need_m, need_a, need_b;
+    nc_scratch_need(rows, L, reps, &need_m, &need_a, &need_b);
     unsigned long long *macc = nullptr, *cacc = nullptr;
     const size_t nmacc = reps * (size_t)n * rows * 6;
-    SALSAA_TRY(scratch_get(ctx, 0, (nmacc + rows) * sizeof(unsigned long long), (void **)&macc));
+    if (ext) {
+        SALSAA_CHECK(ext->macc_bytes >= need_m && ext->bufA_bytes >= need_a &&
+                         ext->bufB_bytes >= need_b,
+                     "norm check scratch too small");
+        macc = ext->macc;
+    } else {
+        SALSAA_TRY(scratch_get(ctx, 0, need_m, (void **)&macc));
+    }
     cacc = macc + nmacc;
     hipLaunchKernelGGL(k_zero_u64, dim3(ceil_div(nmacc + rows, 256)), dim3(256), 0, ctx->stream,
                        macc, nmacc + rows);
     SALSAA_HIP(hipGetLastError());
     if (L <= (size_t)SC_FINAL_MAX) {
         hipLaunchKernelGGL((k_sc_final<FQ2, true, SNR>), dim3((unsigned)rows, (unsigned)reps),
-                           dim3(SC_THREADS), 0, ctx->stream, w, nullptr, L, (int)rows, 0, cs, n, nr,
-                           macc, cacc, claims, msgs, finals, evals, rhs, ok, want_rhs, ctx->q,
-                           ctx->d_tables, M);
+                           dim3(SC_THREADS), 0, ctx->stream, w, nullptr, L, (int)rows, 0, cs, cs_row,
+                           n, nr, macc, cacc, o, want_rhs, ctx->q, ctx->d_tables, M);
         SALSAA_HIP(hipGetLastError());
         return SALSAA_OK;
     }
     // ping-pong buffers of folded witnesses (pairs of doubles)
     const size_t len1 = L >> 4;
     double *bufA = nullptr, *bufB = nullptr;
-    SALSAA_TRY(scratch_get(ctx, 1, reps * rows * len1 * 2 * sizeof(double), (void **)&bufA));
-    SALSAA_TRY(scratch_get(ctx, 2, reps * rows * (len1 >> 4 ? len1 >> 4 : 1) * 2 * sizeof(double),
-                           (void **)&bufB));
+    if (ext) {
+        bufA = ext->bufA;
+        bufB = ext->bufB;
+    } else {
+        SALSAA_TRY(scratch_get(ctx, 1, need_a, (void **)&bufA));
+        SALSAA_TRY(scratch_get(ctx, 2, need_b, (void **)&bufB));
+    }
     {
         const size_t S = len1;
-        const int cpt = S >= (size_t)SC_THREADS * 64 ? 4 : 1;
+        const int cpt = cpt_first > 0 ? cpt_first : (S >= (size_t)SC_THREADS * 64 ? 4 : 1);
         const unsigned bx = ceil_div(S, (size_t)SC_THREADS * cpt);
         for (size_t r0 = 0; r0 < reps; r0 += 2) {
             const int nrep = (reps - r0) >= 2 ? 2 : 1;
             dim3 grid(bx, (unsigned)rows);
             if (nrep == 2)
                 hipLaunchKernelGGL((k_sc_first<2, FQ2, SNR>), grid, dim3(SC_THREADS), 0, ctx->stream, w,
-                                   L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, n, nr, cpt, bufA,
-                                   macc, cacc, M);
+                                   L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, cs_row, n, nr, cpt,
+                                   bufA, macc, cacc, M);
             else
                 hipLaunchKernelGGL((k_sc_first<1, FQ2, SNR>), grid, dim3(SC_THREADS), 0, ctx->stream, w,
-                                   L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, n, nr, cpt, bufA,
-                                   macc, cacc, M);
+                                   L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, cs_row, n, nr, cpt,
+                                   bufA, macc, cacc, M);
             SALSAA_HIP(hipGetLastError());
         }
     }
@@ -528,7 +541,7 @@ int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L,
         const int cpt = S >= (size_t)SC_THREADS * 64 ? 4 : 1;
         dim3 grid(ceil_div(S, (size_t)SC_THREADS * cpt), (unsigned)rows, (unsigned)reps);
         hipLaunchKernelGGL((k_sc_pass<FQ2, SNR>), grid, dim3(SC_THREADS), 0, ctx->stream, src, len,
-                           (int)rows, t0, cs, n, nr, cpt, dst, macc, M);
+                           (int)rows, t0, cs, cs_row, n, nr, cpt, dst, macc, M);
         SALSAA_HIP(hipGetLastError());
         len = S;
         t0 += 4;
@@ -537,35 +550,75 @@ int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L,
         dst = tmp;
     }
     hipLaunchKernelGGL((k_sc_final<FQ2, false, SNR>), dim3((unsigned)rows, (unsigned)reps),
-                       dim3(SC_THREADS), 0, ctx->stream, nullptr, src, len, (int)rows, t0, cs, n, nr,
-                       macc, cacc, claims, msgs, finals, evals, rhs, ok, want_rhs, ctx->q,
-                       ctx->d_tables, M);
+                       dim3(SC_THREADS), 0, ctx->stream, nullptr, src, len, (int)rows, t0, cs, cs_row,
+                       n, nr, macc, cacc, o, want_rhs, ctx->q, ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }
 
 }  // namespace
 
-int launch_norm_check(Context *ctx, const uint64_t *w, size_t rows, size_t L,
-                      const uint64_t *cs, size_t reps, int fq2, uint64_t nr, uint64_t *claims,
-                      uint64_t *msgs, uint64_t *finals, uint64_t *evals, uint64_t *rhs,
-                      int32_t *ok) {
+void nc_scratch_need(size_t rows, size_t L, size_t reps, size_t *macc_bytes, size_t *bufA_bytes,
+                     size_t *bufB_bytes) {
+    const size_t n = (L >= 2) ? (size_t)__builtin_ctzll(L) : 0;
+    const size_t len1 = L >> 4;
+    *macc_bytes = (reps * n * rows * 6 + rows) * sizeof(unsigned long long);
+    *bufA_bytes = L > (size_t)SC_FINAL_MAX ? reps * rows * len1 * 2 * sizeof(double) : 0;
+    *bufB_bytes =
+        L > (size_t)SC_FINAL_MAX ? reps * rows * (len1 >> 4 ? len1 >> 4 : 1) * 2 * sizeof(double) : 0;
+}
+
+NcOut nc_out_plain(size_t rows, size_t reps, size_t nrounds, uint64_t *claims, uint64_t *msgs,
+                   uint64_t *finals, uint64_t *evals, uint64_t *rhs, int32_t *ok) {
+    NcOut o;
+    o.claims = claims;
+    o.msgs = msgs;
+    o.finals = finals;
+    o.evals = evals;
+    o.rhs = rhs;
+    o.ok = ok;
+    o.claim_row = 1;
+    o.msg_rep = nrounds * rows * 6;
+    o.msg_t = rows * 6;
+    o.msg_row = 6;
+    o.fe_rep = rows * 2;
+    o.fe_row = 2;
+    o.rhs_rep = rows * 2 * kD;
+    o.rhs_row = 2 * kD;
+    o.ok_rep = rows;
+    o.ok_row = 1;
+    (void)reps;
+    return o;
+}
+
+int launch_norm_check_ex(Context *ctx, const uint64_t *w, size_t rows, size_t L,
+                         const uint64_t *cs, size_t cs_row, size_t reps, int fq2, uint64_t nr,
+                         const NcOut &o, int cpt_first, const NcScratch *ext) {
     SALSAA_CHECK(rows > 0 && reps > 0, "norm check needs rows > 0 and reps > 0");
     SALSAA_CHECK(L >= 2 && (L & (L - 1)) == 0, "norm check length must be a power of two >= 2");
     SALSAA_CHECK(rows < 65536 && reps < 65536, "too many rows or repetitions");
     SALSAA_CHECK(__builtin_ctzll(L) <= 48, "norm check length too large");
-    SALSAA_CHECK(rhs == nullptr || L >= (size_t)kD, "norm_check_3 needs L >= d");
+    SALSAA_CHECK(o.rhs == nullptr || L >= (size_t)kD, "norm_check_3 needs L >= d");
     SALSAA_CHECK(nr < ctx->q, "non-residue must be reduced mod q");
-    SALSAA_CHECK(ok != nullptr, "ok output is required");
-    // the final block keeps at most 32 rounds of messages locally
+    SALSAA_CHECK(o.ok != nullptr, "ok output is required");
+    SALSAA_CHECK(cpt_first >= 0 && cpt_first <= 64, "columns per thread out of range");
     if (fq2 && nr < 8)
-        return norm_check_impl<true, true>(ctx, w, rows, L, cs, reps, (double)nr, claims, msgs,
-                                           finals, evals, rhs, ok);
+        return norm_check_impl<true, true>(ctx, w, rows, L, cs, cs_row, reps, (double)nr, o,
+                                           cpt_first, ext);
     if (fq2)
-        return norm_check_impl<true, false>(ctx, w, rows, L, cs, reps, (double)nr, claims, msgs,
-                                            finals, evals, rhs, ok);
-    return norm_check_impl<false, true>(ctx, w, rows, L, cs, reps, (double)nr, claims, msgs,
-                                        finals, evals, rhs, ok);
+        return norm_check_impl<true, false>(ctx, w, rows, L, cs, cs_row, reps, (double)nr, o,
+                                            cpt_first, ext);
+    return norm_check_impl<false, true>(ctx, w, rows, L, cs, cs_row, reps, (double)nr, o,
+                                        cpt_first, ext);
+}
+
+int launch_norm_check(Context *ctx, const uint64_t *w, size_t rows, size_t L,
+                      const uint64_t *cs, size_t reps, int fq2, uint64_t nr, uint64_t *claims,
+                      uint64_t *msgs, uint64_t *finals, uint64_t *evals, uint64_t *rhs,
+                      int32_t *ok) {
+    const size_t nrounds = (L >= 2 && (L & (L - 1)) == 0) ? (size_t)__builtin_ctzll(L) : 0;
+    const NcOut o = nc_out_plain(rows, reps, nrounds, claims, msgs, finals, evals, rhs, ok);
+    return launch_norm_check_ex(ctx, w, rows, L, cs, 0, reps, fq2, nr, o, 0, nullptr);
 }
 
 }  // namespace salsaa

@@ -1,0 +1,116 @@
+"""Batched prover steps (grouped launches, hipGraph replay) against the plain
+per-call path and the oracle: every output of every instance bit-exact."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import NR, Q
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def sa():
+    import salsaa_amd
+    return salsaa_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(sa):
+    c = sa.Context(Q, 64)
+    yield c
+    c.close()
+
+
+OUTS = ("claims", "msgs", "finals", "evals", "rhs", "ok")
+
+
+def fill(sa, ctx, b, seed0):
+    for i in range(b.n):
+        s = b.slot(i)
+        w = s["witness"]
+        sa._check(sa.lib.salsaa_sample_bounded(ctx.h, w.ptr, w.shape[0] * 64, 16, seed0 + i,
+                                               sa.STREAM_W))
+        cs = s["challenges"]
+        sa._check(sa.lib.salsaa_sample_challenges(ctx.h, cs.ptr, b.reps, s["nrounds"],
+                                                  int(b.fq2), seed0 + 7919 * (i + 1)))
+
+
+def snapshot(b, plain):
+    res = []
+    for i in range(b.n):
+        s = b.slot(i, plain)
+        d = {k: s[k].download() for k in OUTS}
+        d["commitment"] = s["commitment"].download()[: s["kappa"]]
+        res.append(d)
+    return res
+
+
+def same(a, b):
+    for i, (x, y) in enumerate(zip(a, b)):
+        for k in x:
+            assert np.array_equal(x[k], y[k]), (i, k)
+
+
+@pytest.mark.parametrize("fq2,reps", [(1, 2), (0, 1)])
+def test_batch_matches_plain_and_oracle(sa, ctx, orc, fq2, reps):
+    say("case: batch fq2=%d reps=%d" % (fq2, reps))
+    rng = np.random.default_rng(5 + fq2)
+    n = 37
+    m = [int(1 << rng.integers(0, 10)) for _ in range(n)]
+    kappa = [int(rng.choice([8, 16, 32])) for _ in range(n)]
+    kmax, mmax = 32, 512
+    A_h = orc.sample_uniform(3, sa.STREAM_A, (kmax, mmax, 64), Q)
+    key = ctx.key(ctx.to_device(A_h))
+    b = sa.Batch(ctx, key, m, kappa, reps=reps, fq2=fq2, tune=True)
+    say("  tuning", b.tuning())
+    fill(sa, ctx, b, 100)
+    W0 = [b.slot(i)["witness"].download() for i in range(n)]
+    b.step_plain()
+    plain = snapshot(b, True)
+    b.step(graph=False)
+    eager = snapshot(b, False)
+    same(eager, plain)
+    for i in range(n):
+        assert np.array_equal(b.slot(i)["witness"].download(), W0[i])
+        assert plain[i]["ok"].all()
+    # oracle on a few instances of different lengths
+    picks = sorted(set([int(np.argmax(m)), int(np.argmin(m)), 0, n - 1]))
+    for i in picks:
+        s = b.slot(i)
+        Wi = W0[i].reshape(1, m[i], 64)
+        want_c = orc.commit(np.ascontiguousarray(A_h[: kappa[i], : m[i]]), Wi, Q, threads=8)
+        assert np.array_equal(eager[i]["commitment"], want_c), i
+        cs = s["challenges"].download()
+        want = orc.sumcheck(W0[i].reshape(1, m[i] * 64), cs, Q, fq2=bool(fq2), nr=NR, threads=8)
+        for k in ("claims", "msgs", "finals", "evals"):
+            assert np.array_equal(eager[i][k], want[k]), (i, k)
+    # graph replays, with eager work and synchronisation between them
+    for rep in range(3):
+        b.step(graph=True)
+        ctx.sync()
+        same(snapshot(b, False), plain)
+        b.step_plain()
+        ctx.sync()
+    b.close()
+    key.close()
+
+
+def test_batch_rejects_bad_plans(sa, ctx, orc):
+    say("case: batch argument checks")
+    A = ctx.uniform((8, 64, 64), 1, sa.STREAM_A)
+    key = ctx.key(A)
+    with pytest.raises(sa.SalsaaError):
+        sa.Batch(ctx, key, [3], [8], tune=False)      # not a power of two
+    with pytest.raises(sa.SalsaaError):
+        sa.Batch(ctx, key, [128], [8], tune=False)    # longer than the key
+    with pytest.raises(sa.SalsaaError):
+        sa.Batch(ctx, key, [64], [9], tune=False)     # more rows than the key
+    key.close()
