@@ -12,10 +12,11 @@
 // exact.
 //
 // Operands are fragment-tiled (store_planes in ntt8.hip): the 16 digit rows x
-// 64 columns one MFMA reads are one contiguous KiB, lane-ordered, so a wave
-// moves a whole fragment with one 16-byte load per lane, and the row tiles of
-// one K step are consecutive, so a block's loads form one sequential stream
-// per operand.  A block owns one
+// 64 columns one MFMA reads are one contiguous row-major KiB, so a wave moves
+// a whole fragment with one 16-byte load per lane, and the row tiles of one K
+// step are consecutive, so a block's loads form one sequential stream per
+// operand.  In LDS a fragment is kept in MFMA lane order (lane 16 g + r holds
+// row r, columns 16 g .. 16 g + 15), rotated as lds_slot() describes.  A block owns one
 // slot, one K chunk, all of the key's digit rows (at most 16 row tiles) and
 // 16 witness rows (7 column tiles).  Per K step its threads stage the step's
 // fragments from HBM through registers into LDS two steps ahead (double
@@ -34,6 +35,12 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr int FRAG = 1024;  // bytes of one 16 x 64 int8 fragment
 constexpr int GT = 7;       // tiles per recombination group (16 rows of values)
+
+// 16-byte slot in LDS of (row r, column group g) of a fragment: lane order
+// 16 g + r with the rows of group g rotated by 4 g, so the staging stores of
+// 16 consecutive memory chunks (4 rows x 4 groups) and the operand reads of
+// 16 consecutive lanes (16 rows of one group) each hit 16 distinct bank groups.
+__device__ __forceinline__ int lds_slot(int r, int g) { return 16 * g + ((r + 4 * g) & 15); }
 
 // TM: row tiles per wave (wave wm owns key tiles wm + 4a, a < TM);
 // WN x TN: witness tiles (wave wn owns tiles wn + WN b of the block's 7).
@@ -78,15 +85,16 @@ __device__ __forceinline__ void commit_body(char *lds, const int8_t *AB, size_t 
         const int c0 = tid + S::NTHR * i;
         const int c = c0 < nchunk ? c0 : (c0 & 63);
         const int t = c >> 6, off = (c & 63) * 16;
+        const int lslot = lds_slot((c & 63) >> 2, c & 3) * 16;  // memory chunk 4 r + g
         if (t < rtA) {
             gp[i] = AB + s * a_slot + ((size_t)ktb * a_kt + t) * FRAG + off;
             gs[i] = a_kt * FRAG;
-            lo[i] = t * FRAG + off;
+            lo[i] = t * FRAG + lslot;
         } else {
             const int tb = t - rtA;
             gp[i] = WB + s * w_slot + ((size_t)ktb * w_kt + tb) * FRAG + off;
             gs[i] = w_kt * FRAG;
-            lo[i] = (S::SA + tb) * FRAG + off;
+            lo[i] = (S::SA + tb) * FRAG + lslot;
         }
     }
     v4i R0[S::CPT], R1[S::CPT];
@@ -115,8 +123,9 @@ __device__ __forceinline__ void commit_body(char *lds, const int8_t *AB, size_t 
         for (int b = 0; b < TN; b++) acc[a][b] = (v4i){0, 0, 0, 0};
     // fragment slots past the block's tiles hold stale bytes; they only feed
     // rows or columns of the product that are never recombined
+    const int lane_slot = lds_slot(lane & 15, lane >> 4) * 16;
     auto compute = [&](int buf) {
-        const char *base = lds + buf * S::STAGE + lane * 16;
+        const char *base = lds + buf * S::STAGE + lane_slot;
         v4i fb[TN];
 #pragma unroll
         for (int b = 0; b < TN; b++)

@@ -239,16 +239,14 @@ constexpr uint64_t DIGIT_BIAS = 0x0080808080808080ULL;
 // rows has 7R digit rows (row r*7 + u holds digit u), grouped in tiles of 16;
 // columns are grouped in K-tiles of 64.  Fragment (s, kt, rt) is the 1 KiB
 // block at ((s * KT + kt) * RT + rt) * 1024, RT = ceil(7R / 16), KT = m
-// rounded up to 128, over 64: the row tiles of one K step are consecutive, so
-// a commit block streams one contiguous run per operand.  A fragment is
-// lane-ordered for
-// v_mfma_i32_16x16x64_i8: the 16 bytes at 16 l hold row l % 16, columns
-// 16 (l / 16) .. 16 (l / 16) + 15 of the tile, so lane l of a wave loads its
-// operand with one 16-byte load and the wave reads the KiB contiguously.
+// rounded up to 128, over 64, stored row-major (digit row dr % 16 at
+// (dr % 16) * 64).  So one (slot, K-tile) step of the whole matrix is a
+// row-major [16 RT][64] byte block: the row tiles a commit block needs for
+// one K step are consecutive, and the 7 digit rows of one value row are 448
+// contiguous bytes, which this writer fills with coalesced 8-byte stores.
 // Columns in [m, 64 KT) are written as zero; the padding rows of the last row
 // tile are never written and only feed products that are discarded.
-// Each thread packs one digit of 8 columns into one 8-byte store (a pair of
-// threads fills one lane's 16 bytes).
+// Each thread packs one digit of 8 columns into one 8-byte store.
 __device__ __forceinline__ void store_planes(int8_t *__restrict__ dst, const uint64_t *work,
                                              size_t rows, size_t row, size_t ktiles, size_t i0,
                                              int nel, int tid) {
@@ -265,12 +263,9 @@ __device__ __forceinline__ void store_planes(int8_t *__restrict__ dst, const uin
             hi[q] = (uint32_t)(y >> 32);
         }
         const size_t kt = i0 / 64 + (c8 >> 3);
-        // columns 8 (c8 % 8) .. +7 of the K-tile: lane group (c8 % 8) / 2, half c8 % 2
-        const int lane_off = ((c8 & 7) >> 1) * 256 + (c8 & 1) * 8;
-        int8_t *fbase = dst + ((size_t)s * ktiles + kt) * rt_n * 1024 + lane_off;
+        int8_t *fbase = dst + ((size_t)s * ktiles + kt) * rt_n * 1024 + row * 7 * 64 + (c8 & 7) * 8;
 #pragma unroll
         for (int u = 0; u < 7; u++) {
-            const unsigned dr = (unsigned)row * 7 + u;
             const uint32_t *src = u < 4 ? lo : hi;
             const int sh = 8 * (u & 3);
             uint32_t p[2];
@@ -281,8 +276,7 @@ __device__ __forceinline__ void store_planes(int8_t *__restrict__ dst, const uin
                 for (int q = 0; q < 4; q++) x |= ((src[4 * h + q] >> sh) & 0xFFu) << (8 * q);
                 p[h] = x;
             }
-            *reinterpret_cast<uint2 *>(fbase + (size_t)(dr / 16) * 1024 + (dr % 16) * 16) =
-                make_uint2(p[0], p[1]);
+            *reinterpret_cast<uint2 *>(fbase + u * 64) = make_uint2(p[0], p[1]);
         }
     }
 }
