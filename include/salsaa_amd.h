@@ -297,11 +297,61 @@ int salsaa_batch_slots(salsaa_batch *b, size_t i, int plain, salsaa_slot_t *out)
 int salsaa_batch_step(salsaa_batch *b, int use_graph);
 /* The same work as a native loop of per-instance calls (the plain path). */
 int salsaa_batch_step_plain(salsaa_batch *b);
+/* One phase of the batched step, eagerly (for per-phase timing): 0 NTT
+ * forward, 1 NTT inverse, 2 witnesses -> digit planes, 3 commit + finalize,
+ * 4 norm checks.  Phases 0..4 in order equal one salsaa_batch_step. */
+int salsaa_batch_phase(salsaa_batch *b, int phase);
 /* What the autotuner chose: commit K chunk (tiles of 64 ring elements),
  * norm check first-pass columns per thread (0 = automatic), and the timed
  * costs of the chosen commit and norm checks in ms. */
 int salsaa_batch_tuning(const salsaa_batch *b, size_t *kchunk, int *cpt_first, float *commit_ms,
                         float *norm_check_ms);
+
+/* ------------------------------------------------------------------ serving
+ * A continuous-batching service of commit + norm-check jobs against one
+ * resident key (no reference counterpart: it schedules the per-job calls
+ * parallel_dot_series_matrix, arithmetic.rs:459-505, and norm_check_1/2/3,
+ * fast_norm_check.rs:282-462, for a stream of requests).  A scheduler thread
+ * admits jobs in arrival order while their witness fits the HBM arena, forms
+ * a batch from every admitted job whenever one of `inflight` batch slots is
+ * free (jobs arriving meanwhile join the next batch, and two slots overlap on
+ * the device), and copies each finished job's output record to the host.
+ * max_batch_jobs = 1 with inflight = 1 serves every job alone: the plain
+ * per-call path. */
+typedef struct salsaa_server salsaa_server;
+typedef struct {
+    size_t arena_elems;     /* HBM arena for admitted witnesses (ring elements) */
+    size_t max_batch_jobs;  /* jobs per batch (1..4096)                         */
+    size_t max_batch_elems; /* ring elements per batch (>= the largest job)     */
+    int inflight;           /* batches on the device at once: 1 or 2            */
+    size_t reps;            /* norm check repetitions                           */
+    int fq2;                /* 1: F_{q^2} challenges                            */
+    uint64_t nr;            /* non-residue of F_{q^2}                           */
+    size_t kchunk;          /* commit K chunk in tiles of 64 ring elements      */
+} salsaa_server_opts;
+
+/* Jobs commit with the key's first kappa rows. */
+int salsaa_server_create(salsaa_ctx *ctx, const salsaa_key *key, size_t kappa,
+                         const salsaa_server_opts *opts, salsaa_server **out);
+/* Queue a job with a witness of m ring elements (power of two): copied from
+ * h_witness ([m][d] Coefficient, host memory that stays valid until the job
+ * is done; pinned memory copies asynchronously) or, if h_witness is NULL,
+ * sampled on the device in [-bound, bound] from `seed` (stream 0xB).  The
+ * job's challenges are sampled from `seed` (as salsaa_sample_challenges). */
+int salsaa_server_submit(salsaa_server *s, size_t m, const uint64_t *h_witness, uint64_t seed,
+                         uint64_t bound, uint64_t *id);
+/* 1 when job `id` is done (its output record copied to h_record, times =
+ * submit, admit, launch, done in seconds of a steady clock, batch_info =
+ * batch number and jobs in it), 0 while pending, negative on error. */
+int salsaa_server_poll(salsaa_server *s, uint64_t id, uint64_t *h_record, double *times,
+                       uint64_t *batch_info);
+/* Block until every submitted job is done. */
+int salsaa_server_drain(salsaa_server *s);
+/* Output record layout (u64 offsets): [0] commitment [kappa][d] IncompleteNTT,
+ * [1] claim, [2] msgs [reps][nrounds][3][2], [3] finals [reps][2], [4] evals
+ * [reps][2], [5] rhs [reps][2][d], [6] ok as int32 [reps], [7] record length. */
+int salsaa_server_layout(const salsaa_server *s, size_t *offsets);
+int salsaa_server_destroy(salsaa_server *s);
 
 #ifdef __cplusplus
 }

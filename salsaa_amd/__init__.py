@@ -32,6 +32,14 @@ _PP = ctypes.POINTER(ctypes.c_void_p)
 _PSZ = ctypes.POINTER(ctypes.c_size_t)
 
 
+class ServerOpts(ctypes.Structure):
+    """salsaa_server_opts."""
+    _fields_ = [("arena_elems", ctypes.c_size_t), ("max_batch_jobs", ctypes.c_size_t),
+                ("max_batch_elems", ctypes.c_size_t), ("inflight", ctypes.c_int),
+                ("reps", ctypes.c_size_t), ("fq2", ctypes.c_int), ("nr", ctypes.c_uint64),
+                ("kchunk", ctypes.c_size_t)]
+
+
 class SlotT(ctypes.Structure):
     """salsaa_slot_t: device slots of one batch instance."""
     _fields_ = [(n, ctypes.c_void_p) for n in ("witness", "challenges", "commitment", "claims",
@@ -111,6 +119,14 @@ def _load():
         "salsaa_batch_slots": (_int, [_vp, _sz, _int, ctypes.POINTER(SlotT)]),
         "salsaa_batch_step": (_int, [_vp, _int]),
         "salsaa_batch_step_plain": (_int, [_vp]),
+        "salsaa_batch_phase": (_int, [_vp, _int]),
+        "salsaa_server_create": (_int, [_vp, _vp, _sz, ctypes.POINTER(ServerOpts), _PP]),
+        "salsaa_server_submit": (_int, [_vp, _sz, _vp, _u64, _u64, ctypes.POINTER(ctypes.c_uint64)]),
+        "salsaa_server_poll": (_int, [_vp, _u64, _vp, ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_uint64)]),
+        "salsaa_server_drain": (_int, [_vp]),
+        "salsaa_server_layout": (_int, [_vp, _PSZ]),
+        "salsaa_server_destroy": (_int, [_vp]),
         "salsaa_batch_tuning": (_int, [_vp, _PSZ, ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_float),
                                        ctypes.POINTER(ctypes.c_float)]),
@@ -408,6 +424,18 @@ class Batch:
     def step_plain(self):
         _check(lib.salsaa_batch_step_plain(self.h))
 
+    PHASES = ("ntt_fwd", "ntt_inv", "planes", "commit", "norm_check")
+
+    def phase(self, k):
+        """Run phase k of the batched step eagerly (0..4, see PHASES)."""
+        _check(lib.salsaa_batch_phase(self.h, int(k)))
+
+    def arena(self):
+        """The witness arena: every instance's witness back to back (sorted
+        by length), as one [total][d] view."""
+        p = min((self.slot(i)["witness"] for i in range(self.n)), key=lambda a: a.ptr)
+        return DeviceArray.view(self.ctx, p.ptr, (sum(self.m), D))
+
     def tuning(self):
         kc, cpt = ctypes.c_size_t(), ctypes.c_int()
         tc, tn = ctypes.c_float(), ctypes.c_float()
@@ -419,6 +447,74 @@ class Batch:
     def close(self):
         if self.h:
             lib.salsaa_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching commit + norm-check service (salsaa_server_*).
+    max_batch_jobs=1, inflight=1 serves each job alone (the plain path)."""
+
+    def __init__(self, ctx, key, kappa, arena_elems, max_batch_jobs, max_batch_elems,
+                 inflight=2, reps=2, fq2=True, nr=NR, kchunk=64):
+        self.ctx, self.key, self.kappa, self.reps = ctx, key, kappa, reps
+        o = ServerOpts(arena_elems, max_batch_jobs, max_batch_elems, inflight, reps, int(fq2), nr,
+                       kchunk)
+        p = ctypes.c_void_p()
+        _check(lib.salsaa_server_create(ctx.h, key.h, kappa, ctypes.byref(o), ctypes.byref(p)))
+        self.h = p.value
+        offs = (ctypes.c_size_t * 8)()
+        _check(lib.salsaa_server_layout(self.h, offs))
+        self.off = list(offs)
+        self._keep = {}
+
+    def submit(self, m, seed, bound=16, witness=None):
+        """Queue a job; witness (host [m][d] uint64) or None (sampled from seed)."""
+        jid = ctypes.c_uint64()
+        ptr = None
+        if witness is not None:
+            witness = np.ascontiguousarray(witness, dtype=np.uint64)
+            assert witness.shape == (m, D)
+            ptr = witness.ctypes.data
+        _check(lib.salsaa_server_submit(self.h, m, ptr, seed, bound, ctypes.byref(jid)))
+        if witness is not None:
+            self._keep[jid.value] = witness
+        return jid.value
+
+    def poll(self, jid, m):
+        """None while pending; else the job's outputs, times and batch."""
+        rec = np.empty(self.off[7], np.uint64)
+        t = (ctypes.c_double * 4)()
+        bi = (ctypes.c_uint64 * 2)()
+        st = lib.salsaa_server_poll(self.h, jid, rec.ctypes.data, t, bi)
+        if st < 0:
+            _check(st)
+        if st == 0:
+            return None
+        self._keep.pop(jid, None)
+        n = (m * D).bit_length() - 1
+        r, o = self.reps, self.off
+        return dict(
+            commitment=rec[o[0]: o[0] + self.kappa * D].reshape(self.kappa, 1, D),
+            claims=rec[o[1]: o[1] + 1],
+            msgs=rec[o[2]: o[2] + r * n * 6].reshape(r, n, 1, 3, 2),
+            finals=rec[o[3]: o[3] + 2 * r].reshape(r, 1, 2),
+            evals=rec[o[4]: o[4] + 2 * r].reshape(r, 1, 2),
+            rhs=rec[o[5]: o[5] + 2 * D * r].reshape(r, 1, 2, D),
+            ok=rec[o[6]:].view(np.int32)[:r].reshape(r, 1),
+            times=list(t), batch=int(bi[0]), batch_jobs=int(bi[1]))
+
+    def drain(self):
+        _check(lib.salsaa_server_drain(self.h))
+
+    def close(self):
+        if self.h:
+            lib.salsaa_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

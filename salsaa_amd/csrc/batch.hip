@@ -217,7 +217,7 @@ int run_norm_checks(salsaa_batch *b, int cpt) {
         no.ok = o.ok + p * b->reps;
         no.ok_row = b->reps;
         no.ok_rep = 1;
-        SALSAA_TRY(launch_norm_check_ex(ctx, b->d_wit + b->off_w[p] * kD, g.rows, g.L,
+        SALSAA_TRY(launch_norm_check_ex(ctx, b->d_wit + b->off_w[p] * kD, nullptr, g.rows, g.L,
                                         b->d_cs + b->off_c[p], b->reps * nr * 2, b->reps, b->fq2,
                                         b->nr, no, cpt, &b->nc));
     }
@@ -515,6 +515,20 @@ int salsaa_batch_step(salsaa_batch *b, int use_graph) {
     }
     SALSAA_HIP(hipGraphLaunch(b->exec, ctx->stream));
     return SALSAA_OK;
+}
+
+int salsaa_batch_phase(salsaa_batch *b, int phase) {
+    SALSAA_CHECK(b != nullptr, "null batch");
+    CTX_OK(b->ctx);
+    Context *ctx = &b->ctx->c;
+    switch (phase) {
+    case 0: return launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 0);
+    case 1: return launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 1);
+    case 2: return run_planes(b);
+    case 3: return run_commit(b);
+    case 4: return run_norm_checks(b, b->cpt_first);
+    default: SALSAA_CHECK(false, "phase must be 0..4");
+    }
 }
 
 int salsaa_batch_step_plain(salsaa_batch *b) {

@@ -283,11 +283,13 @@ struct NcScratch {
 };
 void nc_scratch_need(size_t rows, size_t L, size_t reps, size_t *macc_bytes, size_t *bufA_bytes,
                      size_t *bufB_bytes);
+// wrow: per-row offsets (u64) of the rows from w (null: row r at w + r L);
 // cs_row: u64 between consecutive rows' challenge blocks (0: shared);
 // cpt_first: columns per thread of the first pass (0: automatic)
-int launch_norm_check_ex(Context *ctx, const uint64_t *w, size_t rows, size_t L,
-                         const uint64_t *cs, size_t cs_row, size_t reps, int fq2, uint64_t nr,
-                         const NcOut &o, int cpt_first, const NcScratch *ext);
+int launch_norm_check_ex(Context *ctx, const uint64_t *w, const unsigned long long *wrow,
+                         size_t rows, size_t L, const uint64_t *cs, size_t cs_row, size_t reps,
+                         int fq2, uint64_t nr, const NcOut &o, int cpt_first,
+                         const NcScratch *ext);
 int launch_norm_check(Context *ctx, const uint64_t *w, size_t rows, size_t L,
                       const uint64_t *cs, size_t reps, int fq2, uint64_t nr, uint64_t *claims,
                       uint64_t *msgs, uint64_t *finals, uint64_t *evals, uint64_t *rhs,

@@ -124,7 +124,8 @@ __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) 
 // Round 0's messages and the norm_check_1 claim do not depend on challenges.
 template <int REPS, bool FQ2, bool SNR>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
-    const uint64_t *__restrict__ w, size_t L, int rows, int rep0, int reps_total,
+    const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
+    int rows, int rep0, int reps_total,
     int add_claim, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr,
     int cpt, double *__restrict__ out, unsigned long long *__restrict__ macc,
     unsigned long long *__restrict__ cacc, Mod M) {
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
 #pragma unroll
             for (int f = 0; f < 3; f++) acc[r][t][f] = {0.0, 0.0};
 
-    const uint64_t *wr = w + (size_t)row * L;
+    const uint64_t *wr = w + (wrow ? wrow[row] : (size_t)row * L);
     for (int cc = 0; cc < cpt; cc++) {
         const size_t x = ((size_t)blockIdx.x * cpt + cc) * SC_THREADS + threadIdx.x;
         if (x >= S) break;
@@ -295,7 +296,8 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
 // One block per (row, rep): remaining rounds t0.. in LDS, norm_check_3, verifier.
 template <bool FQ2, bool FIRST, bool SNR>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
-    const uint64_t *__restrict__ w, const double *__restrict__ in, size_t len, int rows, int t0,
+    const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow,
+    const double *__restrict__ in, size_t len, int rows, int t0,
     const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr,
     const unsigned long long *__restrict__ macc, const unsigned long long *__restrict__ cacc,
     NcOut o, int want_rhs, uint64_t q, const double *__restrict__ tab, Mod M) {
@@ -317,7 +319,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
 
     for (size_t x = tid; x < len; x += SC_THREADS) {
         if (FIRST) {
-            cur[x].a = u2d(w[(size_t)row * len + x]);
+            cur[x].a = u2d(w[(wrow ? wrow[row] : (size_t)row * len) + x]);
             cur[x].b = 0.0;
         } else {
             const double *p = in + (((size_t)rep * rows + row) * len + x) * 2;
@@ -476,9 +478,9 @@ __global__ void k_zero_u64(unsigned long long *p, size_t n) {
 }
 
 template <bool FQ2, bool SNR>
-int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L, const uint64_t *cs,
-                    size_t cs_row, size_t reps, double nr, const NcOut &o, int cpt_first,
-                    const NcScratch *ext) {
+int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *wrow, size_t rows,
+                    size_t L, const uint64_t *cs, size_t cs_row, size_t reps, double nr,
+                    const NcOut &o, int cpt_first, const NcScratch *ext) {
     Mod M{ctx->qd, ctx->qinv};
     const int n = __builtin_ctzll(L);
     const int want_rhs = o.rhs != nullptr;
@@ -500,7 +502,7 @@ int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L, cons
     SALSAA_HIP(hipGetLastError());
     if (L <= (size_t)SC_FINAL_MAX) {
         hipLaunchKernelGGL((k_sc_final<FQ2, true, SNR>), dim3((unsigned)rows, (unsigned)reps),
-                           dim3(SC_THREADS), 0, ctx->stream, w, nullptr, L, (int)rows, 0, cs, cs_row,
+                           dim3(SC_THREADS), 0, ctx->stream, w, wrow, nullptr, L, (int)rows, 0, cs, cs_row,
                            n, nr, macc, cacc, o, want_rhs, ctx->q, ctx->d_tables, M);
         SALSAA_HIP(hipGetLastError());
         return SALSAA_OK;
@@ -524,11 +526,11 @@ int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L, cons
             dim3 grid(bx, (unsigned)rows);
             if (nrep == 2)
                 hipLaunchKernelGGL((k_sc_first<2, FQ2, SNR>), grid, dim3(SC_THREADS), 0, ctx->stream, w,
-                                   L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, cs_row, n, nr, cpt,
+                                   wrow, L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, cs_row, n, nr, cpt,
                                    bufA, macc, cacc, M);
             else
                 hipLaunchKernelGGL((k_sc_first<1, FQ2, SNR>), grid, dim3(SC_THREADS), 0, ctx->stream, w,
-                                   L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, cs_row, n, nr, cpt,
+                                   wrow, L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, cs_row, n, nr, cpt,
                                    bufA, macc, cacc, M);
             SALSAA_HIP(hipGetLastError());
         }
@@ -550,7 +552,8 @@ int norm_check_impl(Context *ctx, const uint64_t *w, size_t rows, size_t L, cons
         dst = tmp;
     }
     hipLaunchKernelGGL((k_sc_final<FQ2, false, SNR>), dim3((unsigned)rows, (unsigned)reps),
-                       dim3(SC_THREADS), 0, ctx->stream, nullptr, src, len, (int)rows, t0, cs, cs_row,
+                       dim3(SC_THREADS), 0, ctx->stream, nullptr, nullptr, src, len, (int)rows, t0, cs,
+                       cs_row,
                        n, nr, macc, cacc, o, want_rhs, ctx->q, ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
@@ -591,9 +594,10 @@ NcOut nc_out_plain(size_t rows, size_t reps, size_t nrounds, uint64_t *claims, u
     return o;
 }
 
-int launch_norm_check_ex(Context *ctx, const uint64_t *w, size_t rows, size_t L,
-                         const uint64_t *cs, size_t cs_row, size_t reps, int fq2, uint64_t nr,
-                         const NcOut &o, int cpt_first, const NcScratch *ext) {
+int launch_norm_check_ex(Context *ctx, const uint64_t *w, const unsigned long long *wrow,
+                         size_t rows, size_t L, const uint64_t *cs, size_t cs_row, size_t reps,
+                         int fq2, uint64_t nr, const NcOut &o, int cpt_first,
+                         const NcScratch *ext) {
     SALSAA_CHECK(rows > 0 && reps > 0, "norm check needs rows > 0 and reps > 0");
     SALSAA_CHECK(L >= 2 && (L & (L - 1)) == 0, "norm check length must be a power of two >= 2");
     SALSAA_CHECK(rows < 65536 && reps < 65536, "too many rows or repetitions");
@@ -603,12 +607,12 @@ int launch_norm_check_ex(Context *ctx, const uint64_t *w, size_t rows, size_t L,
     SALSAA_CHECK(o.ok != nullptr, "ok output is required");
     SALSAA_CHECK(cpt_first >= 0 && cpt_first <= 64, "columns per thread out of range");
     if (fq2 && nr < 8)
-        return norm_check_impl<true, true>(ctx, w, rows, L, cs, cs_row, reps, (double)nr, o,
+        return norm_check_impl<true, true>(ctx, w, wrow, rows, L, cs, cs_row, reps, (double)nr, o,
                                            cpt_first, ext);
     if (fq2)
-        return norm_check_impl<true, false>(ctx, w, rows, L, cs, cs_row, reps, (double)nr, o,
+        return norm_check_impl<true, false>(ctx, w, wrow, rows, L, cs, cs_row, reps, (double)nr, o,
                                             cpt_first, ext);
-    return norm_check_impl<false, true>(ctx, w, rows, L, cs, cs_row, reps, (double)nr, o,
+    return norm_check_impl<false, true>(ctx, w, wrow, rows, L, cs, cs_row, reps, (double)nr, o,
                                         cpt_first, ext);
 }
 
@@ -618,7 +622,7 @@ int launch_norm_check(Context *ctx, const uint64_t *w, size_t rows, size_t L,
                       int32_t *ok) {
     const size_t nrounds = (L >= 2 && (L & (L - 1)) == 0) ? (size_t)__builtin_ctzll(L) : 0;
     const NcOut o = nc_out_plain(rows, reps, nrounds, claims, msgs, finals, evals, rhs, ok);
-    return launch_norm_check_ex(ctx, w, rows, L, cs, 0, reps, fq2, nr, o, 0, nullptr);
+    return launch_norm_check_ex(ctx, w, nullptr, rows, L, cs, 0, reps, fq2, nr, o, 0, nullptr);
 }
 
 }  // namespace salsaa
