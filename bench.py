@@ -1,19 +1,28 @@
-"""Prover-step benchmark on one MI355X.
+"""Prover-step benchmark on one MI355X (ring Z_q[X]/(X^64 + 1),
+q = 1125899906840833).  Prints one JSON line.
 
-A prover step (plain per-call GPU path, 1125899906840833-modulus ring of
-degree 64), at the full-size single-instance workload:
-  1. forward + inverse IncompleteNTT of 2^20 ring elements (uniform mod q)
-  2. commit A . W, A in R_q^{32 x 16384} uniform (resident key), W 8 rows of
-     16384 ring elements with coefficients uniform in [-16, 16]
-  3. norm check of the 8 x 2^20 flattened witness: norm_check_1 claims, two
-     repetitions of the 20-round sumcheck with F_{q^2} challenges, verifier
-     replay, norm_check_3 rhs
-  4. decomp_ell: base-2^8 decomposition of W into 6 limbs (48 rows) and the
-     commitment of the limbs, then verify_decomp: the inner product of the
-     gadget vector (1, 2^8, ..., 2^40) with the limb commitments must equal
-     the commitment (decomp.rs:95-112)
-All inputs are generated on the device from seeds before timing starts; every
-output stays in HBM.  Prints one JSON line.
+Headline (the metric's "batched vs plain per-call GPU"): 512 independent
+prover-step instances, each
+  * forward + inverse IncompleteNTT of its witness (m ring elements),
+  * commit of the witness (one row) under the first kappa rows of one
+    resident key A in R_q^{32 x 2^18} (uniform),
+  * norm check of the witness's m*64 coefficients: norm_check_1 claim, two
+    repetitions of the sumcheck with the instance's own F_{q^2} challenges,
+    verifier replay, norm_check_3 rhs,
+with m = 2^k, k uniform over the integers 10..18, kappa uniform over
+{8, 16, 32}, witness coefficients uniform in [-16, 16].  `value` is instance
+steps per second of the batched step (grouped launches replayed from a
+hipGraph); the same instances run through the plain per-call path (a native
+loop of the per-instance operator calls) for the comparison, and every output
+of every instance is compared between the two.
+
+Also reported: the served stream ("served": open-loop Poisson arrivals of
+commit + norm-check jobs through the continuous-batching server and through
+the same server one job at a time, jobs/s and p50/p99 latency) and the
+single-instance full-size step (NTT of 2^20 ring elements; commit A 32 x 16384
+. W 8 x 16384; norm check 8 x 2^20; decomp_ell base 2^8 x 6 with
+verify_decomp).  All inputs are generated on the device from seeds before
+timing starts; the batched and single-instance outputs stay in HBM.
 """
 import argparse
 import json
@@ -114,6 +123,91 @@ def traffic_bytes():
         return None
 
 
+N_INST, A_ROWS, A_M = 512, 32, 1 << 18
+
+
+def cfg3_instances(n=N_INST, seed=3):
+    rng = np.random.default_rng(seed)
+    m = [int(1 << rng.integers(10, 19)) for _ in range(n)]
+    kappa = [int(rng.choice([8, 16, 32])) for _ in range(n)]
+    return m, kappa
+
+
+class Batched:
+    """The 512-instance batched workload and its plain per-call twin."""
+
+    def __init__(self, sa, ctx, n=N_INST):
+        self.sa, self.ctx = sa, ctx
+        self.m, self.kappa = cfg3_instances(n)
+        A = ctx.uniform((A_ROWS, A_M, 64), 0, sa.STREAM_A)
+        self.key = ctx.key(A)
+        ctx.sync()
+        A.free()
+        self.b = sa.Batch(ctx, self.key, self.m, self.kappa, reps=REPS, fq2=True, tune=True)
+        for i in range(n):
+            sl = self.b.slot(i)
+            w, cs = sl["witness"], sl["challenges"]
+            sa._check(sa.lib.salsaa_sample_bounded(ctx.h, w.ptr, w.shape[0] * 64, BOUND,
+                                                   1000 + i, sa.STREAM_W))
+            sa._check(sa.lib.salsaa_sample_challenges(ctx.h, cs.ptr, REPS, sl["nrounds"], 1,
+                                                      5000 + i))
+        ctx.sync()
+
+    def outputs(self, plain):
+        res = []
+        for i in range(self.b.n):
+            sl = self.b.slot(i, plain)
+            d = [sl[k].download() for k in ("claims", "msgs", "finals", "evals", "rhs", "ok")]
+            d.append(sl["commitment"].download()[: sl["kappa"]])
+            res.append(d)
+        return res
+
+    def bit_exact(self):
+        a, b = self.outputs(False), self.outputs(True)
+        same = all(np.array_equal(x, y) for u, v in zip(a, b) for x, y in zip(u, v))
+        ok = all(u[5].all() for u in a)
+        return same and ok
+
+
+def cpu_baseline_batched(m, kappa, budget_s=20.0):
+    """The oracle (one thread) on a sample of the batched workload's
+    instances, extrapolated to all of them: NTT and norm check scale with the
+    witness length, the commit (per-product conversion, as the reference) with
+    kappa * length."""
+    from oracle import oracle as orc
+    q = 1125899906840833
+    sample = [(8192, 8), (8192, 16), (8192, 32)]
+    t_ntt = t_com = t_nc = 0.0
+    for j, (mm, kk) in enumerate(sample):
+        w = orc.sample_bounded(1000 + j, 0xB, (1, mm, 64), BOUND, q)
+        A = orc.sample_uniform(0, 0xA, (kk, mm, 64), q)
+        cs = orc.sample_uniform(5000 + j, 0x5C000, (REPS, (mm * 64).bit_length() - 1, 2), q)
+        t0 = time.perf_counter()
+        orc.from_incomplete(orc.to_incomplete(w[0], q), q)
+        t1 = time.perf_counter()
+        orc.commit(A, w, q, threads=1)
+        t2 = time.perf_counter()
+        orc.sumcheck(w.reshape(1, mm * 64), cs, q, fq2=True, nr=5, threads=1)
+        t3 = time.perf_counter()
+        t_ntt += (t1 - t0) / mm
+        t_com += (t2 - t1) / (mm * kk)
+        t_nc += (t3 - t2) / mm
+    k = len(sample)
+    sum_m = float(sum(m))
+    sum_km = float(sum(a * b for a, b in zip(m, kappa)))
+    total = (t_ntt / k) * sum_m + (t_com / k) * sum_km + (t_nc / k) * sum_m
+    return {
+        "value": len(m) / total,
+        "unit": "prover-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": ("oracle, 1 thread: NTT fwd+inv, commit (per-product conversion) and F_{q^2} "
+                   "norm check of three m=8192 instances (kappa 8, 16, 32), per-element costs "
+                   "extrapolated to the 512 instances (sum m = %d, sum kappa*m = %d): %.0f s "
+                   "for the whole batch" % (sum_m, sum_km, total)),
+    }
+
+
 def cpu_baseline(budget_s=20.0):
     """The oracle (single thread) on a bounded sample of the same step, scaled
     to one full step: each phase runs on a fraction of its work and the time is
@@ -159,12 +253,121 @@ def cpu_baseline(budget_s=20.0):
     }
 
 
+S_KAPPA, S_MMAX = 16, 1 << 16
+
+
+def served(sa, ctx, n_jobs=3000, seed=7):
+    """Open-loop Poisson stream of commit + norm-check jobs (m = 2^k ring
+    elements, k uniform in 6..16, i.e. 2^12..2^22 coefficients; kappa 16;
+    coefficients in [-16, 16]; challenges from per-job seeds) against a
+    resident key A 16 x 2^16, through the continuous-batching server and
+    through the same server with one job per batch (plain per-call)."""
+    A = ctx.uniform((S_KAPPA, S_MMAX, 64), 1, sa.STREAM_A)
+    key = ctx.key(A)
+    ctx.sync()
+    A.free()
+    rng = np.random.default_rng(seed)
+    ms = [int(1 << rng.integers(6, 17)) for _ in range(n_jobs)]
+
+    def run(batch_jobs, inflight, lam, n):
+        srv = sa.Server(ctx, key, S_KAPPA, arena_elems=1 << 23, max_batch_jobs=batch_jobs,
+                        max_batch_elems=1 << 21, inflight=inflight, reps=REPS, fq2=True)
+        for i in range(8):
+            srv.submit(ms[i], 10 ** 6 + i)
+        srv.drain()
+        arr = np.cumsum(rng.exponential(1.0 / lam, n)) if lam else np.zeros(n)
+        ids = []
+        t0 = time.perf_counter()
+        for i in range(n):
+            while time.perf_counter() - t0 < arr[i]:
+                pass
+            ids.append(srv.submit(ms[i], 1000 + i))
+        srv.drain()
+        res = [srv.poll(j, m) for j, m in zip(ids, ms[:n])]
+        srv.close()
+        ts = np.array([r["times"] for r in res])
+        lat = (ts[:, 3] - ts[:, 0]) * 1e3
+        span = ts[:, 3].max() - ts[:, 0].min()
+        return {"offered_jobs_per_s": round(lam, 1) if lam else "all at t=0", "jobs": n,
+                "jobs_per_s": round(n / span, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "mean_batch_jobs": round(float(np.mean([r["batch_jobs"] for r in res])), 1)}
+
+    cap_b = run(256, 2, 0, n_jobs)
+    cap_p = run(1, 1, 0, n_jobs // 5)
+    lam_hi = 0.6 * cap_b["jobs_per_s"]
+    lam_lo = 0.5 * cap_p["jobs_per_s"]
+    out = {
+        "workload": "open-loop Poisson commit+norm-check jobs: m = 2^k, k uniform in 6..16 "
+                    "(2^12..2^22 coefficients), kappa 16, coefficients in [-16,16], F_q^2 "
+                    "challenges from per-job seeds, 2 reps; resident key A 16 x 2^16; "
+                    "admission by an HBM arena of 2^23 ring elements",
+        "capacity": {"batched": cap_b, "plain": cap_p,
+                     "ratio": round(cap_b["jobs_per_s"] / cap_p["jobs_per_s"], 2)},
+        "at_0.6x_batched_capacity": {"batched": run(256, 2, lam_hi, n_jobs),
+                                     "plain": run(1, 1, lam_hi, n_jobs // 5)},
+        "at_0.5x_plain_capacity": {"batched": run(256, 2, lam_lo, n_jobs // 5),
+                                   "plain": run(1, 1, lam_lo, n_jobs // 5)},
+    }
+    key.close()
+    return out
+
+
+def timed(ctx, fn, k):
+    evs = [ctx.event() for _ in range(k + 1)]
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.record(evs[0])
+    for i in range(k):
+        fn()
+        ctx.record(evs[i + 1])
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    per = [ctx.elapsed_ms(evs[i], evs[i + 1]) for i in range(k)]
+    return per, ctx.elapsed_ms(evs[0], evs[-1]), wall
+
+
+def single_instance(sa, ctx, steps):
+    """The full-size single-instance plain step (per-phase and whole)."""
+    step = Step(sa, ctx)
+    for _ in range(3):
+        step.run()
+    ctx.sync()
+    for _ in range(3):
+        step.run(timed_phases=True)
+        step.collect_phases()
+    per, total_ms, _ = timed(ctx, step.run, steps)
+    alg = algorithmic(step)
+    phases = {}
+    for k, v in step.phase_ms.items():
+        v = float(np.mean(v))
+        a = alg[k]
+        e = {"ms": round(v, 4), "GB/s": round(a["bytes"] / (v * 1e-3) / 1e9, 1)}
+        if "modmac" in a:
+            e["GmodMAC/s"] = round(a["modmac"] / (v * 1e-3) / 1e9, 1)
+        phases[k] = e
+    out = {"workload": "single instance, plain per-call GPU path: NTT 2^20 ring elements; "
+                       "commit A 32x16384 . W 8x16384 in [-16,16]; norm check 8x2^20, F_q^2 "
+                       "challenges, 2 reps; decomp_ell base 2^8 x 6 + verify_decomp",
+           "value": round(1000.0 * steps / total_ms, 3), "unit": "prover-steps/s",
+           "ms_per_step": round(total_ms / steps, 4),
+           "p50_ms": round(float(np.percentile(per, 50)), 4),
+           "p99_ms": round(float(np.percentile(per, 99)), 4),
+           "bit_exact_self_check": step.check(), "phases": phases}
+    step.key.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-instance step")
+    ap.add_argument("--no-served", action="store_true", help="skip the served stream")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -184,79 +387,87 @@ def main():
         ctx.sync()
         print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 5}), flush=True)
         return
-    step = Step(sa, ctx)
-    for _ in range(args.warmup):
-        step.run()
-    ctx.sync()
-    if not step.check():
-        print("prover step self-check failed", file=sys.stderr)
-        sys.exit(1)
 
-    # per-phase timing pass (events between phases), not the timed region
+    bw = Batched(sa, ctx)
+    b = bw.b
+    n = b.n
+    # warm-up (the first graph call captures the step)
+    for _ in range(max(args.warmup, 1)):
+        b.step(graph=True)
+    b.step_plain()
+    ctx.sync()
+    # per-phase timing of the batched step (eager, events between phases)
+    ev = [ctx.event() for _ in range(len(b.PHASES) + 1)]
+    phase_ms = {k: [] for k in b.PHASES}
     for _ in range(3):
-        step.run(timed_phases=True)
-        step.collect_phases()
-    ctx.sync()
+        ctx.record(ev[0])
+        for k in range(len(b.PHASES)):
+            b.phase(k)
+            ctx.record(ev[k + 1])
+        ctx.sync()
+        for k, name in enumerate(b.PHASES):
+            phase_ms[name].append(ctx.elapsed_ms(ev[k], ev[k + 1]))
+    # timed region: K batched steps (graph replays)
+    per, total_ms, wall = timed(ctx, lambda: b.step(graph=True), args.steps)
+    # the plain per-call path on the same instances
+    pper, ptotal_ms, _ = timed(ctx, b.step_plain, args.plain_steps)
+    exact = bw.bit_exact()
 
-    # timed region: K whole steps, events at each step boundary
-    evs = [ctx.event() for _ in range(args.steps + 1)]
-    ctx.sync()
-    t0 = time.perf_counter()
-    ctx.record(evs[0])
-    for i in range(args.steps):
-        step.run()
-        ctx.record(evs[i + 1])
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    per_step = [ctx.elapsed_ms(evs[i], evs[i + 1]) for i in range(args.steps)]
-    total_ms = ctx.elapsed_ms(evs[0], evs[-1])
-    ok = step.check()
-
-    alg = algorithmic(step)
-    phase_avg = {k: float(np.mean(v)) for k, v in step.phase_ms.items()}
-    dom = max(phase_avg, key=phase_avg.get)
-    ntt_ms = phase_avg["ntt_fwd"]
-    ntt_gbs = alg["ntt_fwd"]["bytes"] / (ntt_ms * 1e-3) / 1e9
+    sum_m = sum(bw.m)
+    ntt_bytes = sum_m * 64 * 8 * 2
+    ntt_ms = float(np.mean(phase_ms["ntt_fwd"]))
+    ntt_gbs = ntt_bytes / (ntt_ms * 1e-3) / 1e9
     peak = 8000.0
-    roofline = {"bound": "hbm", "kernel": "k_ntt_inc<false> (IncompleteNTT forward)",
+    roofline = {"bound": "hbm", "kernel": "k_ntt_inc<false> (IncompleteNTT forward, whole arena)",
                 "achieved": round(ntt_gbs, 1), "peak": peak, "unit": "GB/s",
                 "frac": round(ntt_gbs / peak, 4), "traffic": traffic_bytes(),
-                "algorithmic_bytes": alg["ntt_fwd"]["bytes"], "dominant_phase": dom}
-    phases = {}
-    for k, v in phase_avg.items():
-        a = alg[k]
-        e = {"ms": round(v, 4), "GB/s": round(a["bytes"] / (v * 1e-3) / 1e9, 1)}
-        if "modmac" in a:
-            e["GmodMAC/s"] = round(a["modmac"] / (v * 1e-3) / 1e9, 1)
-        phases[k] = e
-
+                "algorithmic_bytes": ntt_bytes,
+                "per_launch_ms": round(ntt_ms, 4)}
+    value = 1000.0 * n * args.steps / total_ms
+    plain_value = 1000.0 * n * args.plain_steps / ptotal_ms
     out = {
         "metric": "prover-steps/s + p50/p99 ms, batched-served vs plain per-call GPU, bit-exact",
-        "value": round(1000.0 * args.steps / total_ms, 3),
+        "value": round(value, 2),
         "unit": "prover-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(total_ms / args.steps, 4),
-        "p50_ms": round(float(np.percentile(per_step, 50)), 4),
-        "p99_ms": round(float(np.percentile(per_step, 99)), 4),
+        "p50_ms": round(float(np.percentile(per, 50)), 4),
+        "p99_ms": round(float(np.percentile(per, 99)), 4),
         "wall_s": round(wall, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64 mod q (exact fp64 modmul)",
+        "dtype": "u64 mod q (exact fp64 modmul, int8-digit MFMA commit)",
         "data": "synthetic, seeded on device",
-        "bit_exact_self_check": ok,
-        "config": {"workload": "plain per-call GPU prover step, single instance: NTT 2^20 ring "
-                               "elements; commit A 32x16384, W 8x16384 in [-16,16]; norm check "
-                               "8x2^20, F_q^2 challenges, 2 reps; decomp_ell base 2^8 x 6",
-                   "d": 64, "q": sa.Q, "path": "plain"},
-        "phases": phases,
+        "config": {"workload": "batched: 512 independent prover steps per step (NTT fwd+inv, "
+                               "commit, F_q^2 norm check x2 reps) over one resident key "
+                               "A 32 x 2^18; m = 2^k, k uniform in 10..18; kappa uniform in "
+                               "{8,16,32}; one witness row per instance in [-16,16]",
+                   "d": 64, "q": sa.Q, "instances": n, "sum_m": sum_m, "path": "batched+graph"},
+        "plain": {"value": round(plain_value, 2), "unit": "prover-steps/s",
+                  "ms_per_step": round(ptotal_ms / args.plain_steps, 4),
+                  "steps": args.plain_steps,
+                  "path": "native loop of per-instance operator calls"},
+        "speedup_vs_plain": round(value / plain_value, 3),
+        "bit_exact_vs_plain": exact,
+        "tuning": b.tuning(),
+        "phases_ms": {k: round(float(np.mean(v)), 4) for k, v in phase_ms.items()},
         "roofline": roofline,
     }
+    b.close()
+    bw.key.close()
+    if not args.no_served:
+        out["served"] = served(sa, ctx)
+    if not args.no_single:
+        out["single_instance"] = single_instance(sa, ctx, args.steps)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline_batched(bw.m, bw.kappa)
+        out["cpu_baseline_single_instance"] = cpu_baseline()
     print(json.dumps(out), flush=True)
+    if not exact:
+        sys.exit(1)
 
 
 if __name__ == "__main__":
