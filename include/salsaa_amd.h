@@ -307,6 +307,38 @@ int salsaa_batch_phase(salsaa_batch *b, int phase);
 int salsaa_batch_tuning(const salsaa_batch *b, size_t *kchunk, int *cpt_first, float *commit_ms,
                         float *norm_check_ms);
 
+/* ------------------------------------------------------------------ folding
+ * The folding step of n accumulated instances (the protocol's folding round,
+ * protocol.rs:351-451): instance i's witness W_i [r][m][d] (Coefficient) is
+ *   decomposed into r*parts limb rows (decompose_matrix_by_chunks,
+ *     arithmetic.rs:1572-1593; radix 0 = the reference's auto radix, chosen
+ *     per instance),
+ *   folded W'_i[j] = sum_k C_i[j][k] * limb_k with its challenge matrix C_i
+ *     [r_out][r*parts] (fold, fold.rs:22-27 -> parallel_dot_matrix_matrix,
+ *     arithmetic.rs:429-457),
+ * then the folded witnesses, stacked as rows R = i*r_out + j, are committed
+ * (parallel_dot_series_matrix, arithmetic.rs:459-505) and norm-checked row by
+ * row, instance i with its own challenges cs [n][reps][log2(m d)][2]
+ * (fast_norm_check.rs:282-462).  Decomposition and fold are one fused pass:
+ * the limbs never reach HBM.  Requires r <= 8, r*parts <= 64, r_out <= 8 and
+ * m a power of two, at least 128. */
+typedef struct {
+    uint64_t *folded;       /* [n*r_out][m][d] IncompleteNTT (the fold output)  */
+    uint64_t *folded_coeff; /* [n*r_out][m][d] the same in Coefficient form     */
+    uint64_t *rhs;          /* [kappa][n*r_out][d] IncompleteNTT commitment     */
+    uint64_t *radix;        /* [n] radix used per instance                      */
+    uint64_t *claims;       /* [n*r_out]            norm check outputs as for   */
+    uint64_t *msgs;         /* [reps][nrounds][n*r_out][3][2]  salsaa_norm_check */
+    uint64_t *finals;       /* [reps][n*r_out][2]                               */
+    uint64_t *evals;        /* [reps][n*r_out][2]                               */
+    uint64_t *nc_rhs;       /* [reps][n*r_out][2][d] or NULL                    */
+    int32_t *ok;            /* [reps][n*r_out]                                  */
+} salsaa_fold_outputs;
+int salsaa_fold_step(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const uint64_t *d_W,
+                     size_t r, size_t m, size_t parts, uint64_t radix, const uint64_t *d_C,
+                     size_t r_out, const uint64_t *d_cs, size_t reps, int fq2, uint64_t nr,
+                     const salsaa_fold_outputs *out);
+
 /* ------------------------------------------------------------------ serving
  * A continuous-batching service of commit + norm-check jobs against one
  * resident key (no reference counterpart: it schedules the per-job calls

@@ -32,6 +32,12 @@ _PP = ctypes.POINTER(ctypes.c_void_p)
 _PSZ = ctypes.POINTER(ctypes.c_size_t)
 
 
+class FoldOut(ctypes.Structure):
+    """salsaa_fold_outputs."""
+    _fields_ = [(n, ctypes.c_void_p) for n in ("folded", "folded_coeff", "rhs", "radix", "claims",
+                                                 "msgs", "finals", "evals", "nc_rhs", "ok")]
+
+
 class ServerOpts(ctypes.Structure):
     """salsaa_server_opts."""
     _fields_ = [("arena_elems", ctypes.c_size_t), ("max_batch_jobs", ctypes.c_size_t),
@@ -120,6 +126,8 @@ def _load():
         "salsaa_batch_step": (_int, [_vp, _int]),
         "salsaa_batch_step_plain": (_int, [_vp]),
         "salsaa_batch_phase": (_int, [_vp, _int]),
+        "salsaa_fold_step": (_int, [_vp, _vp, _sz, _vp, _sz, _sz, _sz, _u64, _vp, _sz, _vp, _sz,
+                                     _int, _u64, ctypes.POINTER(FoldOut)]),
         "salsaa_server_create": (_int, [_vp, _vp, _sz, ctypes.POINTER(ServerOpts), _PP]),
         "salsaa_server_submit": (_int, [_vp, _sz, _vp, _u64, _u64, ctypes.POINTER(ctypes.c_uint64)]),
         "salsaa_server_poll": (_int, [_vp, _u64, _vp, ctypes.POINTER(ctypes.c_double),
@@ -380,6 +388,28 @@ class Context:
         return out
 
     fold = parallel_dot_matrix_matrix
+
+    def fold_step(self, key, W, n, parts, C, cs, fq2=True, radix=0, nr=NR):
+        """Folding step of n instances: W [n*r][m][d], C [n][r_out][r*parts][d]
+        (ternary challenges), cs [n][reps][log2(m d)][2].  Returns the fold
+        output (IncompleteNTT), its Coefficient form, the commitment of the
+        stacked folded witnesses and the norm check outputs of every row."""
+        rows_in, m, d = W.shape
+        r = rows_in // n
+        _, r_out, rk, _ = C.shape
+        assert rk == r * parts and C.shape[0] == n
+        reps, nrounds = cs.shape[1], cs.shape[2]
+        rows = n * r_out
+        o = dict(folded=self.empty((rows, m, d)), folded_coeff=self.empty((rows, m, d)),
+                 rhs=self.empty((key.kappa, rows, d)), radix=self.empty((n,)),
+                 claims=self.empty((rows,)), msgs=self.empty((reps, nrounds, rows, 3, 2)),
+                 finals=self.empty((reps, rows, 2)), evals=self.empty((reps, rows, 2)),
+                 nc_rhs=self.empty((reps, rows, 2, d)), ok=self.empty((reps, rows), np.int32))
+        fo = FoldOut(*[o[k].ptr for k in ("folded", "folded_coeff", "rhs", "radix", "claims",
+                                           "msgs", "finals", "evals", "nc_rhs", "ok")])
+        _check(lib.salsaa_fold_step(self.h, key.h, n, W.ptr, r, m, parts, radix, C.ptr, r_out,
+                                    cs.ptr, reps, int(fq2), nr, ctypes.byref(fo)))
+        return o
 
 
 class Batch:

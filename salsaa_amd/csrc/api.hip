@@ -90,6 +90,20 @@ void build_tables(uint64_t q, std::vector<double> &tab) {
     uint64_t psi32 = h_min_root(64, q);
     for (int j = 0; j < 32; j++) tab[TAB_OMEGA + j] = (double)h_powmod(psi32, 2 * h_brv(j, 5) + 1, q);
     (void)r32;
+    // full slots from the IncompleteNTT halves: slot s evaluates at
+    // zeta_s = psi_128^(2 brv(s) + 1); a(zeta) = e(zeta^2) + zeta o(zeta^2)
+    // and zeta^2 is one of the 32-point slot roots omega_j
+    {
+        const uint64_t psi128 = h_min_root(128, q);
+        for (int s = 0; s < 64; s++) {
+            const uint64_t z = h_powmod(psi128, 2 * h_brv(s, 6) + 1, q), z2 = h_mulmod(z, z, q);
+            int zj = -1;
+            for (int j = 0; j < 32; j++)
+                if ((uint64_t)tab[TAB_OMEGA + j] == z2) zj = j;
+            tab[TAB_ZETA + s] = (double)z;
+            tab[TAB_ZJ + s] = (double)zj;  // always found: zeta^2 is a root of Y^32 + 1
+        }
+    }
     // commit output map: a vector of 64 full-NTT slot values to the
     // IncompleteNTT form of the ring element it represents (linear in the slots)
     for (int t = 0; t <= 12; t++) tab[TAB_W256 + t] = (double)h_powmod(2, 8 * (uint64_t)t, q);

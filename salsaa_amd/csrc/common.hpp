@@ -57,7 +57,12 @@ enum TableOffset : int {
     TAB_FIN = 256,    // [64][64] transposed map full-NTT slots -> IncompleteNTT:
                       // fin[s * 64 + c] = (IncNTT(INTT_64(e_s)))[c]
     TAB_W256 = 256 + 64 * 64,  // [13] 2^(8t) mod q, digit weights of the int8 commit
-    TAB_SIZE = 256 + 64 * 64 + 16,
+    // full-NTT slot s from the IncompleteNTT halves (E, O):
+    // a(zeta_s) = E[zj_s] + zeta_s O[zj_s], zeta_s = psi_128^(2 brv_6(s) + 1),
+    // omega_{zj_s} = zeta_s^2
+    TAB_ZETA = 256 + 64 * 64 + 16,  // [64] zeta_s
+    TAB_ZJ = TAB_ZETA + 64,         // [64] zj_s (as a double)
+    TAB_SIZE = TAB_ZJ + 64,
 };
 
 // ---------------------------------------------------------------- context
@@ -75,7 +80,7 @@ struct Context {
     hipStream_t stream = nullptr;
     double *d_tables = nullptr;     // TAB_SIZE doubles
     std::vector<double> h_tables;   // host copy
-    ScratchBuf scratch[8];          // grown on demand, owned by the context
+    ScratchBuf scratch[10];         // grown on demand, owned by the context
     bool mfma_commit = true;        // int8 MFMA commit (false: fp64 VALU commit)
     ScratchBuf zeros;               // zero-filled, grown on demand
     std::mutex lock;                // one host thread drives a context at a time

@@ -20,8 +20,6 @@ constexpr int EL = 128;      // ring elements per block
 constexpr int NT = 1024;     // threads per block (8 per element)
 constexpr int SROW = 72;     // LDS row stride in u64 (64 + 8 pad)
 
-__device__ __forceinline__ uint64_t dbits(double v) { return (uint64_t)__double_as_longlong(v); }
-__device__ __forceinline__ double bitsd(uint64_t b) { return __longlong_as_double((long long)b); }
 
 // LDS row layouts (row stride SROW u64): "plain" keeps entry k at k; "skew"
 // puts entry k at k + k/8, which makes the 8-consecutive-entry accesses of

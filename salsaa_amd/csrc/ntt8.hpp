@@ -16,6 +16,10 @@
 
 namespace salsaa {
 
+// raw bit moves between doubles and LDS words (no conversion)
+__device__ __forceinline__ uint64_t dbits(double v) { return (uint64_t)__double_as_longlong(v); }
+__device__ __forceinline__ double bitsd(uint64_t b) { return __longlong_as_double((long long)b); }
+
 // forward butterfly X, Y <- X + wY, X - wY
 __device__ __forceinline__ void bf_fwd(double &X, double &Y, double w, Mod M) {
     const double T = mulmod(Y, w, M);

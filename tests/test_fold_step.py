@@ -1,0 +1,75 @@
+"""The fused folding step (decompose + fold in one pass, then commit and norm
+check) against the same steps as separate calls and against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import NR, Q
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def sa():
+    import salsaa_amd
+    return salsaa_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(sa):
+    c = sa.Context(Q, 64)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("n,r,m,parts,r_out,bound,kappa", [(3, 8, 128, 4, 8, 1 << 12, 16),
+                                                          (2, 3, 256, 5, 5, 1000, 9)])
+def test_fold_step_vs_separate_calls_and_oracle(sa, ctx, orc, n, r, m, parts, r_out, bound,
+                                                kappa):
+    say("case: fold step n=%d r=%d m=%d parts=%d r_out=%d" % (n, r, m, parts, r_out))
+    reps, nrounds = 2, (m * 64).bit_length() - 1
+    W_h = orc.sample_bounded(21, sa.STREAM_W, (n * r, m, 64), bound, Q)
+    C_h = orc.sample_bounded(22, sa.STREAM_FOLD, (n, r_out, r * parts, 64), 1, Q)
+    A_h = orc.sample_uniform(23, sa.STREAM_A, (kappa, m, 64), Q)
+    cs_h = np.stack([np.stack([orc.sample_uniform(30 + i, 0x5C000 + rep, (nrounds * 2,), Q)
+                               .reshape(nrounds, 2) for rep in range(reps)]) for i in range(n)])
+    key = ctx.key(ctx.to_device(A_h))
+    W, C, cs = ctx.to_device(W_h), ctx.to_device(C_h), ctx.to_device(cs_h)
+    got = ctx.fold_step(key, W, n, parts, C, cs)
+    g = {k: v.download() for k, v in got.items()}
+    assert g["ok"].all()
+    rows = n * r_out
+    # separate calls: decompose, fold, Coefficient form, commit, norm checks
+    coeff = np.empty((rows, m, 64), np.uint64)
+    for i in range(n):
+        Wi = ctx.to_device(W_h[i * r:(i + 1) * r])
+        limbs, d_radix = ctx.decompose(Wi, parts)
+        assert int(d_radix.download()[0]) == int(g["radix"][i])
+        fo = ctx.fold(limbs, ctx.to_device(C_h[i]))
+        sl = slice(i * r_out, (i + 1) * r_out)
+        assert np.array_equal(fo.download(), g["folded"][sl]), i
+        ctx.convert(fo, sa.REP_INCOMPLETE, sa.REP_COEFF)
+        coeff[sl] = fo.download()
+        nc = ctx.norm_check(fo, ctx.to_device(cs_h[i]), 1)
+        assert np.array_equal(nc["claims"].download(), g["claims"][sl])
+        assert np.array_equal(nc["msgs"].download(), g["msgs"][:, :, sl])
+        assert np.array_equal(nc["finals"].download(), g["finals"][:, sl])
+        assert np.array_equal(nc["evals"].download(), g["evals"][:, sl])
+        assert np.array_equal(nc["rhs"].download(), g["nc_rhs"][:, sl])
+    assert np.array_equal(coeff, g["folded_coeff"])
+    com = ctx.parallel_dot_series_matrix(key, ctx.to_device(coeff)).download()
+    assert np.array_equal(com, g["rhs"])
+    # oracle: decomposition + fold of instance 0, commitment of its rows
+    limbs, radix = orc.decompose(W_h[:r], parts, Q)
+    assert radix == int(g["radix"][0])
+    want_fold = orc.fold(limbs, C_h[0], Q, threads=8)
+    assert np.array_equal(want_fold, g["folded"][:r_out])
+    want_com = orc.commit(A_h, coeff[:r_out], Q, threads=8)
+    assert np.array_equal(want_com, g["rhs"][:, :r_out])
+    key.close()

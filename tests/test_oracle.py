@@ -443,3 +443,29 @@ def test_prng_definition(orc):
             assert orc.prng_draw(seed, stream, idx) == want
     u = orc.sample_uniform(5, 9, (10,), Q).tolist()
     assert u == [(orc.prng_draw(5, 9, i) * Q) >> 64 for i in range(10)]
+
+
+def test_full_slots_from_incomplete_halves(orc):
+    """a(zeta_s) = E[j] + zeta_s O[j] with omega_j = zeta_s^2: the relation the
+    fused fold uses to get full-NTT slots (the commit's domain) from the
+    IncompleteNTT form, checked on random elements."""
+    # roots straight from the forward transforms of X and Y
+    x = np.zeros((1, 64), np.uint64)
+    x[0, 1] = 1
+    zeta = orc.to_full(x, Q)[0]              # zeta_s = NTT_64(X)[s]
+    y = np.zeros((1, 64), np.uint64)
+    y[0, 2] = 1                                # X^2 = Y in the even half
+    omega = orc.to_incomplete(y, Q)[0][:32]    # omega_j = NTT_32(Y)[j]
+    zj = []
+    for s in range(64):
+        z2 = int(zeta[s]) * int(zeta[s]) % Q
+        js = [j for j in range(32) if int(omega[j]) == z2]
+        assert len(js) == 1
+        zj.append(js[0])
+    a = orc.sample_uniform(3, 1, (20, 64), Q)
+    full, inc = orc.to_full(a, Q), orc.to_incomplete(a, Q)
+    for e in range(20):
+        for s in range(64):
+            j = zj[s]
+            want = (int(inc[e, j]) + int(zeta[s]) * int(inc[e, 32 + j])) % Q
+            assert int(full[e, s]) == want
