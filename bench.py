@@ -18,8 +18,9 @@ of every instance is compared between the two.
 
 Also reported: the served stream ("served": open-loop Poisson arrivals of
 commit + norm-check jobs through the continuous-batching server and through
-the same server one job at a time, jobs/s and p50/p99 latency) and the
-single-instance full-size step (NTT of 2^20 ring elements; commit A 32 x 16384
+the same server one job at a time, jobs/s and p50/p99 latency), the folding
+step ("folding": 8 instances decomposed, folded, re-committed and
+norm-checked, fused vs separate calls) and the single-instance full-size step (NTT of 2^20 ring elements; commit A 32 x 16384
 . W 8 x 16384; norm check 8 x 2^20; decomp_ell base 2^8 x 6 with
 verify_decomp).  All inputs are generated on the device from seeds before
 timing starts; the batched and single-instance outputs stay in HBM.
@@ -314,6 +315,77 @@ def served(sa, ctx, n_jobs=3000, seed=7):
     return out
 
 
+F_N, F_R, F_M, F_PARTS, F_ROUT, F_KAPPA, F_BOUND = 8, 8, 4096, 4, 8, 16, 1 << 12
+
+
+def folding(sa, ctx, steps):
+    """The folding step: 8 instances, W 8 x 4096 in [-2^12, 2^12], decomposed
+    base 2^4 (auto radix) into 4 limbs, folded by ternary C (8 x 32) per
+    instance, the folded witnesses committed under A 16 x 4096 and
+    norm-checked (2^18 coefficients per row, F_q^2, 2 reps).  Fused path vs
+    the same steps as separate per-instance calls; outputs compared."""
+    n, r, m, parts, r_out = F_N, F_R, F_M, F_PARTS, F_ROUT
+    rows, nrounds = n * r_out, (m * 64).bit_length() - 1
+    A = ctx.uniform((F_KAPPA, m, 64), 2, sa.STREAM_A)
+    key = ctx.key(A)
+    A.free()
+    W = ctx.bounded((n * r, m, 64), F_BOUND, 2, sa.STREAM_W)
+    C = ctx.bounded((n, r_out, r * parts, 64), 1, 2, sa.STREAM_FOLD)
+    cs = ctx.empty((n, REPS, nrounds, 2))
+    for i in range(n):
+        sa._check(sa.lib.salsaa_sample_challenges(ctx.h, cs.ptr + i * REPS * nrounds * 16, REPS,
+                                                  nrounds, 1, 40 + i))
+    outs = ctx.fold_step(key, W, n, parts, C, cs)
+    V = sa.DeviceArray.view
+    # the separate-call path, buffers allocated once
+    limbs = ctx.empty((r * parts, m, 64))
+    rad = ctx.empty((1,))
+    p_fold = ctx.empty((rows, m, 64))
+    p_coeff = ctx.empty((rows, m, 64))
+    p_com = ctx.empty((F_KAPPA, rows, 64))
+    p_nc = [None] * n
+    wsz, csz = r * m * 64 * 8, REPS * nrounds * 16
+
+    def plain():
+        for i in range(n):
+            Wi = V(ctx, W.ptr + i * wsz, (r, m, 64))
+            ctx.decompose(Wi, parts, out=limbs, d_radix=rad)
+            fi = V(ctx, p_fold.ptr + i * r_out * m * 512, (r_out, m, 64))
+            ctx.fold(limbs, V(ctx, C.ptr + i * r_out * r * parts * 512, (r_out, r * parts, 64)),
+                     out=fi)
+            ci = V(ctx, p_coeff.ptr + i * r_out * m * 512, (r_out, m, 64))
+            ctx.copy(ci, fi)
+            ctx.convert(ci, sa.REP_INCOMPLETE, sa.REP_COEFF)
+            p_nc[i] = ctx.norm_check(ci, V(ctx, cs.ptr + i * csz, (REPS, nrounds, 2)), 1,
+                                     outs=p_nc[i])
+        ctx.parallel_dot_series_matrix(key, p_coeff, out=p_com)
+
+    fused = lambda: ctx.fold_step(key, W, n, parts, C, cs, outs=outs)  # noqa: E731
+    for _ in range(2):
+        fused()
+        plain()
+    ctx.sync()
+    fper, ftotal, _ = timed(ctx, fused, steps)
+    pper, ptotal, _ = timed(ctx, plain, max(steps // 4, 2))
+    exact = (np.array_equal(p_fold.download(), outs["folded"].download()) and
+             np.array_equal(p_coeff.download(), outs["folded_coeff"].download()) and
+             np.array_equal(p_com.download(), outs["rhs"].download()) and
+             all(np.array_equal(p_nc[i]["msgs"].download(),
+                                outs["msgs"].download()[:, :, i * r_out:(i + 1) * r_out])
+                 for i in range(n)) and bool(outs["ok"].download().all()))
+    key.close()
+    fms, pms = ftotal / steps, ptotal / max(steps // 4, 2)
+    return {"workload": "folding step: 8 instances, W 8x4096 in [-2^12,2^12], base 2^4 x 4 "
+                        "limbs (auto radix), ternary C 8x32 per instance, commit under A 16x4096, "
+                        "norm check 64 rows x 2^18, F_q^2, 2 reps",
+            "fused": {"value": round(1000.0 / fms, 2), "unit": "folding steps/s",
+                      "ms_per_step": round(fms, 4),
+                      "p99_ms": round(float(np.percentile(fper, 99)), 4)},
+            "separate_calls": {"value": round(1000.0 / pms, 2), "unit": "folding steps/s",
+                               "ms_per_step": round(pms, 4)},
+            "speedup": round(pms / fms, 3), "bit_exact_vs_separate_calls": exact}
+
+
 def timed(ctx, fn, k):
     evs = [ctx.event() for _ in range(k + 1)]
     ctx.sync()
@@ -368,6 +440,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the single-instance step")
     ap.add_argument("--no-served", action="store_true", help="skip the served stream")
+    ap.add_argument("--no-folding", action="store_true", help="skip the folding step")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -460,6 +533,8 @@ def main():
     bw.key.close()
     if not args.no_served:
         out["served"] = served(sa, ctx)
+    if not args.no_folding:
+        out["folding"] = folding(sa, ctx, args.steps)
     if not args.no_single:
         out["single_instance"] = single_instance(sa, ctx, args.steps)
     if not args.no_cpu_baseline:

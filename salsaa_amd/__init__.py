@@ -389,7 +389,7 @@ class Context:
 
     fold = parallel_dot_matrix_matrix
 
-    def fold_step(self, key, W, n, parts, C, cs, fq2=True, radix=0, nr=NR):
+    def fold_step(self, key, W, n, parts, C, cs, fq2=True, radix=0, nr=NR, outs=None):
         """Folding step of n instances: W [n*r][m][d], C [n][r_out][r*parts][d]
         (ternary challenges), cs [n][reps][log2(m d)][2].  Returns the fold
         output (IncompleteNTT), its Coefficient form, the commitment of the
@@ -400,7 +400,7 @@ class Context:
         assert rk == r * parts and C.shape[0] == n
         reps, nrounds = cs.shape[1], cs.shape[2]
         rows = n * r_out
-        o = dict(folded=self.empty((rows, m, d)), folded_coeff=self.empty((rows, m, d)),
+        o = outs or dict(folded=self.empty((rows, m, d)), folded_coeff=self.empty((rows, m, d)),
                  rhs=self.empty((key.kappa, rows, d)), radix=self.empty((n,)),
                  claims=self.empty((rows,)), msgs=self.empty((reps, nrounds, rows, 3, 2)),
                  finals=self.empty((reps, rows, 2)), evals=self.empty((reps, rows, 2)),
