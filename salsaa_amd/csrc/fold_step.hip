@@ -52,6 +52,7 @@ __global__ __launch_bounds__(FNT) void k_decomp_fold(
     int8_t *__restrict__ planes, size_t ktiles, size_t rt_n, uint64_t q,
     const double *__restrict__ tab, Mod M) {
     __shared__ uint64_t lds[128 * FROW];
+    __shared__ double cl[8 * 8 * kD];  // this limb's challenges C[jo][rho * parts + tl]
     const int inst = blockIdx.y;
     const size_t i0 = (size_t)blockIdx.x * FCOLS;
     const int tid = threadIdx.x;
@@ -100,18 +101,23 @@ __global__ __launch_bounds__(FNT) void k_decomp_fold(
 #pragma unroll
             for (int x = 0; x < 8; x++) lds[e * FROW + 36 * p + sk(8 * t + x)] = dbits(v[x]);
         }
+        // this limb's challenge slice into LDS: cl[(jo * 8 + rho) * d + s]
+        for (int c = tid; c < r_out * r * kD; c += FNT) {
+            const int s = c & (kD - 1), jr = c >> 6, jo = jr / r, rho = jr % r;
+            cl[(jo * 8 + rho) * kD + s] =
+                u2d(Cn[(((size_t)inst * r_out + jo) * rk + rho * parts + tl) * kD + s]);
+        }
         __syncthreads();
         // fold: out[j] += C[j][k] * limb_k in slot mj, k = rho * parts + tl
         for (int rho = 0; rho < r; rho++) {
             const uint64_t *row = lds + (rho * FCOLS + mi) * FROW;
             const double le = bitsd(row[sk(mj)]), lo = bitsd(row[36 + sk(mj)]);
-            const int k = rho * parts + tl;
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const int jo = 4 * jh + u;
                 if (jo >= r_out) continue;
-                const uint64_t *ce = Cn + (((size_t)inst * r_out + jo) * rk + k) * kD;
-                const double c0 = u2d(ce[mj]), c1 = u2d(ce[kH + mj]);
+                const double *ce = cl + (jo * 8 + rho) * kD;
+                const double c0 = ce[mj], c1 = ce[kH + mj];
                 ae[u] += mul_lazy(le, c0, M) + mul_lazy(mulmod(lo, c1, M), om, M);
                 ao[u] += mul_lazy(lo, c0, M) + mul_lazy(le, c1, M);
                 ae[u] = reduce_lazy(ae[u], M);

@@ -35,8 +35,11 @@ __global__ void k_challenges(uint64_t *__restrict__ cs, size_t reps, size_t nrou
 }
 
 // ---- decomposition (arithmetic.rs:1461-1593)
-__global__ void k_bitor(const uint64_t *__restrict__ W, size_t n, uint64_t q,
-                        unsigned long long *__restrict__ acc) {
+// OR of |x| (pos / neg split) over W; one atomic per block (a grid of a few
+// hundred blocks, so the single accumulator sees little contention)
+__global__ __launch_bounds__(256) void k_bitor(const uint64_t *__restrict__ W, size_t n,
+                                               uint64_t q, unsigned long long *__restrict__ acc) {
+    __shared__ uint64_t part[4];
     uint64_t a = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
@@ -44,7 +47,12 @@ __global__ void k_bitor(const uint64_t *__restrict__ W, size_t n, uint64_t q,
         a |= (2 * x > q) ? q - x : x;
     }
     for (int off = 32; off >= 1; off >>= 1) a |= __shfl_xor(a, off, 64);
-    if ((threadIdx.x & 63) == 0 && a) atomicOr(acc, (unsigned long long)a);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint64_t b = part[0] | part[1] | part[2] | part[3];
+        if (b) atomicOr(acc, (unsigned long long)b);
+    }
 }
 
 __global__ void k_zero1(unsigned long long *p) { *p = 0ULL; }
@@ -231,8 +239,8 @@ int launch_radix(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t par
     if (radix == 0) {
         // the OR accumulator starts at zero (kernel write, safe under capture)
         hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, acc);
-        hipLaunchKernelGGL(k_bitor, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, W, n, ctx->q,
-                           acc);
+        const unsigned g = (unsigned)std::min<size_t>(512, (n + 4095) / 4096);
+        hipLaunchKernelGGL(k_bitor, dim3(g ? g : 1), dim3(256), 0, ctx->stream, W, n, ctx->q, acc);
     }
     hipLaunchKernelGGL(k_set_radix, dim3(1), dim3(1), 0, ctx->stream, acc, radix, parts, d_radix);
     SALSAA_HIP(hipGetLastError());
