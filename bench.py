@@ -113,10 +113,11 @@ def algorithmic(step):
 
 
 def traffic_bytes():
-    """HBM bytes per launch of the NTT forward kernel, measured by rocprofv3
-    FETCH_SIZE and WRITE_SIZE (each scaled by a plain-copy calibration) and
-    committed in profiles/ntt_inc_fwd_cfg2_traffic.json."""
-    path = os.path.join(ROOT, "profiles", "ntt_inc_fwd_cfg2_traffic.json")
+    """HBM bytes per launch of the NTT forward kernel over the batched
+    workload's witness arena, measured by rocprofv3 FETCH_SIZE and WRITE_SIZE
+    (each scaled by a plain-copy calibration) and committed in
+    profiles/ntt_inc_fwd_batched_traffic.json."""
+    path = os.path.join(ROOT, "profiles", "ntt_inc_fwd_batched_traffic.json")
     try:
         with open(path) as f:
             return int(json.load(f)["hbm_bytes_per_launch"])
@@ -441,6 +442,9 @@ def main():
     ap.add_argument("--no-single", action="store_true", help="skip the single-instance step")
     ap.add_argument("--no-served", action="store_true", help="skip the served stream")
     ap.add_argument("--no-folding", action="store_true", help="skip the folding step")
+    ap.add_argument("--eager", action="store_true",
+                    help="run the batched steps eagerly instead of replaying the hipGraph "
+                         "(for counter collection)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -465,8 +469,9 @@ def main():
     b = bw.b
     n = b.n
     # warm-up (the first graph call captures the step)
+    graph = not args.eager
     for _ in range(max(args.warmup, 1)):
-        b.step(graph=True)
+        b.step(graph=graph)
     b.step_plain()
     ctx.sync()
     # per-phase timing of the batched step (eager, events between phases)
@@ -481,7 +486,7 @@ def main():
         for k, name in enumerate(b.PHASES):
             phase_ms[name].append(ctx.elapsed_ms(ev[k], ev[k + 1]))
     # timed region: K batched steps (graph replays)
-    per, total_ms, wall = timed(ctx, lambda: b.step(graph=True), args.steps)
+    per, total_ms, wall = timed(ctx, lambda: b.step(graph=graph), args.steps)
     # the plain per-call path on the same instances
     pper, ptotal_ms, _ = timed(ctx, b.step_plain, args.plain_steps)
     exact = bw.bit_exact()
@@ -518,7 +523,8 @@ def main():
                                "commit, F_q^2 norm check x2 reps) over one resident key "
                                "A 32 x 2^18; m = 2^k, k uniform in 10..18; kappa uniform in "
                                "{8,16,32}; one witness row per instance in [-16,16]",
-                   "d": 64, "q": sa.Q, "instances": n, "sum_m": sum_m, "path": "batched+graph"},
+                   "d": 64, "q": sa.Q, "instances": n, "sum_m": sum_m,
+                   "path": "batched+graph" if graph else "batched (eager)"},
         "plain": {"value": round(plain_value, 2), "unit": "prover-steps/s",
                   "ms_per_step": round(ptotal_ms / args.plain_steps, 4),
                   "steps": args.plain_steps,
