@@ -10,6 +10,8 @@
 //   k_decomp_sm            decompose_matrix_by_chunks fused with the limbs'
 //                          full NTT: writes the limb witness (Coefficient) and
 //                          the limbs' slot-major NTT in one pass over W
+#include <algorithm>
+
 #include "ntt8.hpp"
 
 namespace salsaa {
@@ -59,65 +61,150 @@ __device__ __forceinline__ void store_block(uint64_t *__restrict__ dst, const ui
     }
 }
 
+// Tile staging for the persistent NTT: thread tid moves chunks tid + NT i
+// (i < 4) of a tile of EL ring elements, held in registers so the next tile's
+// loads are in flight while this one is computed.  Branch-free, so the loads
+// stay in flight: chunks past the tile's nel elements re-read chunk 0 and land
+// in LDS rows of elements that are neither transformed nor stored.
+__device__ __forceinline__ ulonglong2 tile_chunk(const uint64_t *__restrict__ src, int c, int nel) {
+    return reinterpret_cast<const ulonglong2 *>(src)[c < nel * 32 ? c : 0];
+}
+
+template <bool SKEW>
+__device__ __forceinline__ void tile_put1(uint64_t *lds, ulonglong2 v, int c) {
+    const int e = c >> 5, k = (c & 31) * 2;
+    if (SKEW) {
+        lds[e * SROW + skew(k)] = v.x;
+        lds[e * SROW + skew(k + 1)] = v.y;
+    } else {
+        *reinterpret_cast<ulonglong2 *>(&lds[e * SROW + k]) = v;
+    }
+}
+
 // IncompleteNTT: thread h of element e works on half p = h >> 2 as t = h & 3.
 // Half p, position i lives at 36 p + skew(i) in the skewed half layout.
-template <bool INV>
-__global__ __launch_bounds__(NT) void k_ntt_inc(uint64_t *__restrict__ data, size_t count,
-                                                const double *__restrict__ tab, Mod M) {
-    __shared__ uint64_t lds[EL * SROW];
-    const size_t e0 = (size_t)blockIdx.x * EL;
-    const int nel = (int)min((size_t)EL, count - e0);
-    const int tid = threadIdx.x, e = tid >> 3, h = tid & 7, p = h >> 2, t = h & 3;
-    const bool live = e < nel;
-    uint64_t *row = lds + e * SROW;
-    double v[8];
-    if (!INV) {
-        load_block<false>(lds, data + e0 * kD, nel, tid);
-        __syncthreads();
-        if (live) {
-#pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = u2d(row[8 * k + 2 * t + p]);  // coeff 2(t+4k)+p
-            ntt32_fwd_a(v, tab + TAB_FWD32, M);
-        }
-        __syncthreads();
-        if (live) {
-#pragma unroll
-            for (int k = 0; k < 8; k++) row[36 * p + skew(t + 4 * k)] = dbits(v[k]);
-        }
-        __syncthreads();
-        if (live) {
-#pragma unroll
-            for (int x = 0; x < 8; x++) v[x] = bitsd(row[36 * p + skew(8 * t + x)]);
-            ntt32_fwd_b(v, tab + TAB_FWD32, t, M);
-            // output entry 32 p + 8 t + x, skewed: same slots this thread read
-#pragma unroll
-            for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = d2u(v[x]);
-        }
-        __syncthreads();
-        store_block<true>(data + e0 * kD, lds, nel, tid);
+// Persistent blocks: a block transforms tiles blockIdx.x, blockIdx.x + G, ...
+// and loads tile k + G into registers as soon as tile k is in LDS, so HBM
+// reads continue while the block computes and stores.
+template <bool SKEW>
+__device__ __forceinline__ ulonglong2 tile_get1(const uint64_t *lds, int c) {
+    const int e = c >> 5, k = (c & 31) * 2;
+    ulonglong2 v;
+    if (SKEW) {
+        v.x = lds[e * SROW + skew(k)];
+        v.y = lds[e * SROW + skew(k + 1)];
     } else {
-        load_block<true>(lds, data + e0 * kD, nel, tid);
-        __syncthreads();
-        if (live) {
+        v = *reinterpret_cast<const ulonglong2 *>(&lds[e * SROW + k]);
+    }
+    return v;
+}
+
+template <bool INV>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_ntt_inc(
+    uint64_t *__restrict__ data, size_t count, const double *__restrict__ tab, Mod M) {
+    __shared__ uint64_t lds[EL * SROW];
+    // twiddles in LDS: a twiddle read from global memory would have to wait
+    // for the in-order vector-memory counter, i.e. for the prefetch as well
+    __shared__ double twl[33];
+    const size_t ntiles = (count + EL - 1) / EL;
+    size_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    const int tid = threadIdx.x, e = tid >> 3, h = tid & 7, p = h >> 2, t = h & 3;
+    if (tid < 32) twl[tid] = tab[(INV ? TAB_INV32 : TAB_FWD32) + tid];
+    if (tid == 32) twl[32] = tab[TAB_NINV32];
+    uint64_t *row = lds + e * SROW;
+    // the first tile into LDS, the next one's loads in flight
+    int cur = (int)min((size_t)EL, count - tile * EL);
+    const uint64_t *src = data + tile * EL * kD;
+    ulonglong2 r0 = tile_chunk(src, tid, cur), r1 = tile_chunk(src, tid + NT, cur),
+               r2 = tile_chunk(src, tid + 2 * NT, cur), r3 = tile_chunk(src, tid + 3 * NT, cur);
+    // INV reads the IncompleteNTT layout (skewed); the forward reads plain rows
+    tile_put1<INV>(lds, r0, tid);
+    tile_put1<INV>(lds, r1, tid + NT);
+    tile_put1<INV>(lds, r2, tid + 2 * NT);
+    tile_put1<INV>(lds, r3, tid + 3 * NT);
+    __syncthreads();
+    // Every load and store below is unconditional, so the wait before the
+    // next tile's LDS write only covers its own loads: past the last tile the
+    // prefetch re-reads element 0, and a store of a chunk past nel rewrites
+    // chunk 0 of the tile with its own value.
+    size_t next = tile + gridDim.x;
+    int nxt = next < ntiles ? (int)min((size_t)EL, count - next * EL) : 1;
+    src = next < ntiles ? data + next * EL * kD : data;
+    r0 = tile_chunk(src, tid, nxt);
+    r1 = tile_chunk(src, tid + NT, nxt);
+    r2 = tile_chunk(src, tid + 2 * NT, nxt);
+    r3 = tile_chunk(src, tid + 3 * NT, nxt);
+    for (;;) {
+        const bool live = e < cur;
+        double v[8];
+        if (!INV) {
+            if (live) {
 #pragma unroll
-            for (int x = 0; x < 8; x++) v[x] = u2d(row[36 * p + skew(8 * t + x)]);
-            ntt32_inv_b(v, tab + TAB_INV32, t, M);
+                for (int k = 0; k < 8; k++) v[k] = u2d(row[8 * k + 2 * t + p]);  // coeff 2(t+4k)+p
+                ntt32_fwd_a(v, twl, M);
+            }
+            __syncthreads();
+            if (live) {
 #pragma unroll
-            for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
+                for (int k = 0; k < 8; k++) row[36 * p + skew(t + 4 * k)] = dbits(v[k]);
+            }
+            __syncthreads();
+            if (live) {
+#pragma unroll
+                for (int x = 0; x < 8; x++) v[x] = bitsd(row[36 * p + skew(8 * t + x)]);
+                ntt32_fwd_b(v, twl, t, M);
+                // output entry 32 p + 8 t + x, skewed: same slots this thread read
+#pragma unroll
+                for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = d2u(v[x]);
+            }
+        } else {
+            if (live) {
+#pragma unroll
+                for (int x = 0; x < 8; x++) v[x] = u2d(row[36 * p + skew(8 * t + x)]);
+                ntt32_inv_b(v, twl, t, M);
+#pragma unroll
+                for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
+            }
+            __syncthreads();
+            if (live) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
+                ntt32_inv_a(v, twl, twl[32], M);
+            }
+            __syncthreads();
+            if (live) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) row[8 * k + 2 * t + p] = d2u(v[k]);
+            }
         }
         __syncthreads();
-        if (live) {
+        // store the tile (forward output skewed, inverse output plain)
+        {
+            ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(data + tile * EL * kD);
 #pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
-            ntt32_inv_a(v, tab + TAB_INV32, tab[TAB_NINV32], M);
+            for (int i = 0; i < 4; i++) {
+                int c = tid + NT * i;
+                c = c < cur * 32 ? c : 0;
+                dst[c] = tile_get1<!INV>(lds, c);
+            }
         }
+        if (next >= ntiles) break;
+        __syncthreads();  // the stores read LDS that the next tile overwrites
+        tile_put1<INV>(lds, r0, tid);
+        tile_put1<INV>(lds, r1, tid + NT);
+        tile_put1<INV>(lds, r2, tid + 2 * NT);
+        tile_put1<INV>(lds, r3, tid + 3 * NT);
         __syncthreads();
-        if (live) {
-#pragma unroll
-            for (int k = 0; k < 8; k++) row[8 * k + 2 * t + p] = d2u(v[k]);
-        }
-        __syncthreads();
-        store_block<false>(data + e0 * kD, lds, nel, tid);
+        tile = next;
+        cur = nxt;
+        next = tile + gridDim.x;
+        nxt = next < ntiles ? (int)min((size_t)EL, count - next * EL) : 1;
+        src = next < ntiles ? data + next * EL * kD : data;
+        r0 = tile_chunk(src, tid, nxt);
+        r1 = tile_chunk(src, tid + NT, nxt);
+        r2 = tile_chunk(src, tid + 2 * NT, nxt);
+        r3 = tile_chunk(src, tid + 3 * NT, nxt);
     }
 }
 
@@ -412,16 +499,35 @@ __global__ __launch_bounds__(NT) void k_decomp_sm(const uint64_t *__restrict__ W
 
 }  // namespace
 
+// resident blocks per CU of a kernel (occupancy API, cached)
+template <bool INV>
+int ntt_inc_occupancy() {
+    static int occ = 0;
+    if (occ == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void *>(&k_ntt_inc<INV>), NT, 0) != hipSuccess ||
+            n < 1)
+            n = 1;
+        occ = n;
+    }
+    return occ;
+}
+
 int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse) {
     if (count == 0) return SALSAA_OK;
     Mod M{ctx->qd, ctx->qinv};
     const unsigned grid = ceil_div(count, EL);
     if (mode == SALSAA_REP_INCOMPLETE) {
+        // persistent: every resident block slot once
+        const size_t slots = (size_t)ctx->ncu * (inverse ? ntt_inc_occupancy<true>()
+                                                         : ntt_inc_occupancy<false>());
+        const unsigned g = (unsigned)std::min<size_t>(grid, slots);
         if (inverse)
-            hipLaunchKernelGGL(k_ntt_inc<true>, dim3(grid), dim3(NT), 0, ctx->stream, data, count,
+            hipLaunchKernelGGL(k_ntt_inc<true>, dim3(g), dim3(NT), 0, ctx->stream, data, count,
                                ctx->d_tables, M);
         else
-            hipLaunchKernelGGL(k_ntt_inc<false>, dim3(grid), dim3(NT), 0, ctx->stream, data, count,
+            hipLaunchKernelGGL(k_ntt_inc<false>, dim3(g), dim3(NT), 0, ctx->stream, data, count,
                                ctx->d_tables, M);
     } else {
         if (inverse)
