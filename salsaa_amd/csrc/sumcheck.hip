@@ -49,17 +49,30 @@ __device__ __forceinline__ F2 x_sub(F2 x, F2 y, Mod M) {
     if (!FQ2) return {submod(x.a, y.a, M), 0.0};
     return f2_sub(x, y, M);
 }
-// acc += x * y, lazily (|acc| stays below q/2 + 1 after each step)
+// acc += x * y, lazily (|acc| stays below q/2 + 1 after each step).  F_{q^2}
+// products use three multiplications: (a + b)(c + d) - ac - bd = ad + bc.  With
+// canonical inputs the sums stay below 2q, so (a + b)(c + d) < 4q^2 < 2^102:
+// mul_lazy's quotient estimate is then off by at most 1.5 and its result,
+// still exact and congruent, lies in (-2q, 2q); ad + bc = p3 - p1 - p2 stays
+// within (-4q, 4q), well inside reduce's 2^53 range.
 template <bool FQ2, bool SNR>
 __device__ __forceinline__ void x_mac(F2 &acc, F2 x, F2 y, double nr, Mod M) {
     if (!FQ2) {
         acc.a = reduce_lazy(acc.a + mul_lazy(x.a, y.a, M), M);
         return;
     }
-    const double re = mul_lazy(x.a, y.a, M) + nr_times<SNR>(nr, mul_lazy(x.b, y.b, M), M);
-    const double im = mul_lazy(x.a, y.b, M) + mul_lazy(x.b, y.a, M);
-    acc.a = reduce_lazy(acc.a + re, M);
-    acc.b = reduce_lazy(acc.b + im, M);
+    const double p1 = mul_lazy(x.a, y.a, M), p2 = mul_lazy(x.b, y.b, M);
+    const double p3 = mul_lazy(x.a + x.b, y.a + y.b, M);
+    acc.a = reduce_lazy(acc.a + p1 + nr_times<SNR>(nr, p2, M), M);
+    acc.b = reduce_lazy(acc.b + (p3 - p1 - p2), M);
+}
+// x * y, canonical (three multiplications as in x_mac)
+template <bool FQ2, bool SNR>
+__device__ __forceinline__ F2 x_mul3(F2 x, F2 y, double nr, Mod M) {
+    if (!FQ2) return {mulmod(x.a, y.a, M), 0.0};
+    const double p1 = mul_lazy(x.a, y.a, M), p2 = mul_lazy(x.b, y.b, M);
+    const double p3 = mul_lazy(x.a + x.b, y.a + y.b, M);
+    return {reduce(p1 + nr_times<SNR>(nr, p2, M), M), reduce(p3 - p1 - p2, M)};
 }
 // acc += x^2 = (a^2 + nr b^2) + 2ab Y
 template <bool FQ2, bool SNR>
@@ -139,14 +152,16 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
 #pragma unroll
         for (int t = 0; t < 4; t++) c[r][t] = load_chal<FQ2>(cs, rep0 + r, nrounds, t);
 
-    double claim = 0.0, m0[3] = {0.0, 0.0, 0.0};
-    F2 acc[REPS][3][3];
+    // f2 = sum (w1 - w0)^2 is not accumulated: it follows from the round
+    // identity 2 f0 + 2 f1 + f2 = claim once the totals are known
+    double claim = 0.0, m0[2] = {0.0, 0.0};
+    F2 acc[REPS][3][2];
 #pragma unroll
     for (int r = 0; r < REPS; r++)
 #pragma unroll
         for (int t = 0; t < 3; t++)
 #pragma unroll
-            for (int f = 0; f < 3; f++) acc[r][t][f] = {0.0, 0.0};
+            for (int f = 0; f < 2; f++) acc[r][t][f] = {0.0, 0.0};
 
     const uint64_t *wr = w + (wrow ? wrow[row] : (size_t)row * L);
     for (int cc = 0; cc < cpt; cc++) {
@@ -166,10 +181,9 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
             dl[y] = submod(v[y + 8], v[y], M);
             m0[0] += mul_lazy(v[y], v[y], M);
             m0[1] += mul_lazy(v[y], dl[y], M);
-            m0[2] += mul_lazy(dl[y], dl[y], M);
             if ((y & 3) == 3) {
 #pragma unroll
-                for (int f = 0; f < 3; f++) m0[f] = reduce_lazy(m0[f], M);
+                for (int f = 0; f < 2; f++) m0[f] = reduce_lazy(m0[f], M);
             }
         }
 #pragma unroll
@@ -189,8 +203,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
                     F2 d = x_sub<FQ2>(cur[y + half], w0, M);
                     x_sq_mac<FQ2, SNR>(acc[r][t - 1][0], w0, nr, M);
                     x_mac<FQ2, SNR>(acc[r][t - 1][1], w0, d, nr, M);
-                    x_sq_mac<FQ2, SNR>(acc[r][t - 1][2], d, nr, M);
-                    cur[y] = x_add<FQ2>(w0, x_mul<FQ2, SNR>(c[r][t], d, nr, M), M);
+                    cur[y] = x_add<FQ2>(w0, x_mul3<FQ2, SNR>(c[r][t], d, nr, M), M);
                 }
             }
             double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
@@ -198,33 +211,33 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
             o[1] = cur[0].b;
         }
     }
-    // reductions: claim, round-0 (3), then per rep 3 rounds x 3 msgs x 2
-    double vals[4 + REPS * 18];
+    // reductions: claim, round-0 f0 f1, then per rep 3 rounds x (f0, f1) x 2
+    double vals[3 + REPS * 12];
     vals[0] = reduce(claim, M);
 #pragma unroll
-    for (int f = 0; f < 3; f++) vals[1 + f] = reduce(m0[f], M);
+    for (int f = 0; f < 2; f++) vals[1 + f] = reduce(m0[f], M);
 #pragma unroll
     for (int r = 0; r < REPS; r++)
 #pragma unroll
         for (int t = 0; t < 3; t++)
 #pragma unroll
-            for (int f = 0; f < 3; f++) {
-                vals[4 + (r * 3 + t) * 6 + f * 2] = reduce(acc[r][t][f].a, M);
-                vals[4 + (r * 3 + t) * 6 + f * 2 + 1] = reduce(acc[r][t][f].b, M);
+            for (int f = 0; f < 2; f++) {
+                vals[3 + (r * 3 + t) * 4 + f * 2] = reduce(acc[r][t][f].a, M);
+                vals[3 + (r * 3 + t) * 4 + f * 2 + 1] = reduce(acc[r][t][f].b, M);
             }
-    block_sum<4 + REPS * 18>(vals, red, M);
+    block_sum<3 + REPS * 12>(vals, red, M);
     const int tid = threadIdx.x;
     if (tid == 0 && add_claim) atomic_add_res(&cacc[row], red[0]);
-    if (tid < REPS * 3) {
-        // round 0 messages (F_q) for every repetition
-        const int r = tid / 3, f = tid % 3;
+    if (tid < REPS * 2) {
+        // round 0 messages f0, f1 (F_q) for every repetition
+        const int r = tid / 2, f = tid % 2;
         atomic_add_res(&macc[macc_idx(rep0 + r, 0, row, nrounds, rows) + f * 2], red[1 + f]);
     }
-    if (tid < REPS * 18) {
-        const int r = tid / 18, rem = tid % 18, t = rem / 6, k = rem % 6;
+    if (tid < REPS * 12) {
+        const int r = tid / 12, rem = tid % 12, t = rem / 4, k = rem % 4;
         if (FQ2 || (k & 1) == 0)
             atomic_add_res(&macc[macc_idx(rep0 + r, t + 1, row, nrounds, rows) + k],
-                           red[4 + tid]);
+                           red[3 + tid]);
     }
 }
 
@@ -242,11 +255,11 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
     F2 c[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) c[t] = load_chal<FQ2>(cs, rep, nrounds, t0 + t);
-    F2 acc[4][3];
+    F2 acc[4][2];  // (f0, f1) per round; f2 follows from the round identity
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int f = 0; f < 3; f++) acc[t][f] = {0.0, 0.0};
+        for (int f = 0; f < 2; f++) acc[t][f] = {0.0, 0.0};
     const double *ib = in + ((size_t)rep * rows + row) * len * 2;
     for (int cc = 0; cc < cpt; cc++) {
         const size_t x = ((size_t)blockIdx.x * cpt + cc) * SC_THREADS + threadIdx.x;
@@ -267,26 +280,25 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
                 F2 d = x_sub<FQ2>(cur[y + half], w0, M);
                 x_sq_mac<FQ2, SNR>(acc[t][0], w0, nr, M);
                 x_mac<FQ2, SNR>(acc[t][1], w0, d, nr, M);
-                x_sq_mac<FQ2, SNR>(acc[t][2], d, nr, M);
-                cur[y] = x_add<FQ2>(w0, x_mul<FQ2, SNR>(c[t], d, nr, M), M);
+                cur[y] = x_add<FQ2>(w0, x_mul3<FQ2, SNR>(c[t], d, nr, M), M);
             }
         }
         double *o = out + (((size_t)rep * rows + row) * S + x) * 2;
         o[0] = cur[0].a;
         o[1] = cur[0].b;
     }
-    double vals[24];
+    double vals[16];
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int f = 0; f < 3; f++) {
-            vals[t * 6 + f * 2] = reduce(acc[t][f].a, M);
-            vals[t * 6 + f * 2 + 1] = reduce(acc[t][f].b, M);
+        for (int f = 0; f < 2; f++) {
+            vals[t * 4 + f * 2] = reduce(acc[t][f].a, M);
+            vals[t * 4 + f * 2 + 1] = reduce(acc[t][f].b, M);
         }
-    block_sum<24>(vals, red, M);
+    block_sum<16>(vals, red, M);
     const int tid = threadIdx.x;
-    if (tid < 24) {
-        const int t = tid / 6, k = tid % 6;
+    if (tid < 16) {
+        const int t = tid / 4, k = tid % 4;
         if (FQ2 || (k & 1) == 0)
             atomic_add_res(&macc[macc_idx(rep, t0 + t, row, nrounds, rows) + k], red[tid]);
     }
@@ -347,19 +359,17 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
         }
         const size_t half = cl >> 1;
         const F2 c = chal[t];
-        F2 a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, a2 = {0.0, 0.0};
+        F2 a0 = {0.0, 0.0}, a1 = {0.0, 0.0};
         for (size_t x = tid; x < half; x += SC_THREADS) {
             F2 w0 = cur[x];
             F2 d = x_sub<FQ2>(cur[x + half], w0, M);
             x_sq_mac<FQ2, SNR>(a0, w0, nr, M);
             x_mac<FQ2, SNR>(a1, w0, d, nr, M);
-            x_sq_mac<FQ2, SNR>(a2, d, nr, M);
-            cur[x] = x_add<FQ2>(w0, x_mul<FQ2, SNR>(c, d, nr, M), M);
+            cur[x] = x_add<FQ2>(w0, x_mul3<FQ2, SNR>(c, d, nr, M), M);
         }
-        double vals[6] = {reduce(a0.a, M), reduce(a0.b, M), reduce(a1.a, M),
-                          reduce(a1.b, M), reduce(a2.a, M), reduce(a2.b, M)};
-        block_sum<6>(vals, red, M);  // synchronises: folded cur is complete
-        if (tid < 6) msg_local[t - t0][tid] = red[tid];
+        double vals[4] = {reduce(a0.a, M), reduce(a0.b, M), reduce(a1.a, M), reduce(a1.b, M)};
+        block_sum<4>(vals, red, M);  // synchronises: folded cur is complete
+        if (tid < 4) msg_local[t - t0][tid] = red[tid];
         cl = half;
         t++;
     }
@@ -427,13 +437,12 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
     }
     if (tid == SC_THREADS - 1) claim_red = FIRST ? claim_local : u64_mod(cacc[row], M);
     __syncthreads();
-    if (o.msgs) {
-        for (int x = tid; x < nrounds * 6; x += SC_THREADS) {
-            const int tt = x / 6, k = x % 6;
-            const double f = tt < t0 ? msg_pre[tt][k] : msg_local[tt - t0][k];
-            o.msgs[rep * o.msg_rep + tt * o.msg_t + row * o.msg_row + k] = d2u(f);
-        }
-    }
+    // Replay (thread 0).  Each round's f2 is the one value that satisfies the
+    // round identity 2 f0 + 2 f1 + f2 = claim (f2 = sum (w1 - w0)^2 equals it
+    // exactly, so the messages are those of a direct evaluation); the
+    // accumulated f0 and f1 of every round are then checked at the end, where
+    // the folded evaluation must square to the final claim.
+    __shared__ double f2s[48][2];
     if (tid == 0) {
         const double claim0 = claim_red;
         if (rep == 0 && o.claims) o.claims[row * o.claim_row] = d2u(claim0);
@@ -441,10 +450,11 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
         int good = 1;
         for (int tt = 0; tt < nrounds; tt++) {
             const double *f = tt < t0 ? msg_pre[tt] : msg_local[tt - t0];
-            const F2 f0 = {f[0], f[1]}, f1 = {f[2], f[3]}, f2 = {f[4], f[5]};
-            const F2 lhs = x_add<true>(x_add<true>(x_add<true>(f0, f0, M), x_add<true>(f1, f1, M), M),
-                                       f2, M);
-            if (lhs.a != cl2.a || lhs.b != cl2.b) good = 0;
+            const F2 f0 = {f[0], f[1]}, f1 = {f[2], f[3]};
+            const F2 f2 = x_sub<true>(cl2, x_add<true>(x_add<true>(f0, f0, M),
+                                                         x_add<true>(f1, f1, M), M), M);
+            f2s[tt][0] = f2.a;
+            f2s[tt][1] = f2.b;
             const F2 c = chal[tt];
             const F2 c2 = x_mul<true, SNR>(c, c, nr, M);
             cl2 = x_add<true>(x_add<true>(f0, x_mul<true, SNR>(x_add<true>(f1, f1, M), c, nr, M), M),
@@ -469,6 +479,14 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
             o.evals[fe + 1] = d2u(ev.b);
         }
         o.ok[rep * o.ok_rep + row * o.ok_row] = good;
+    }
+    __syncthreads();
+    if (o.msgs) {
+        for (int x = tid; x < nrounds * 6; x += SC_THREADS) {
+            const int tt = x / 6, k = x % 6;
+            const double f = k >= 4 ? f2s[tt][k - 4] : (tt < t0 ? msg_pre[tt][k] : msg_local[tt - t0][k]);
+            o.msgs[rep * o.msg_rep + tt * o.msg_t + row * o.msg_row + k] = d2u(f);
+        }
     }
 }
 
