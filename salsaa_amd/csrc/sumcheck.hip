@@ -22,6 +22,10 @@ namespace salsaa {
 namespace {
 
 constexpr int SC_THREADS = 256;
+// occupancy cap of the streaming passes (empty: the compiler's choice)
+#ifndef SC_WPE
+#define SC_WPE
+#endif
 constexpr int SC_FINAL_MAX = 1024;
 
 // ---- field helpers: F_q (FQ2 = false, imaginary parts constant zero) or
@@ -87,12 +91,72 @@ __device__ __forceinline__ void x_sq_mac(F2 &acc, F2 x, double nr, Mod M) {
     acc.b = reduce_lazy(acc.b + im, M);
 }
 
+// ---- streaming passes: balanced lazy residues.  The passes keep every
+// witness value and challenge as a balanced residue |x| <= q/2 + 2 (what
+// reduce_lazy returns) and never canonicalise inside a level.  For |a b| <= P,
+// mul_lazy returns P' = a b - qt q exactly with |P'| <= (1/2 + 1.5 * 2^-53 P/q) q,
+// so for the operand bounds below (|w0|, |c| <= h = q/2 + 2; |d|, |w0.a + w0.b|,
+// |c.a + c.b| <= 2h; |d.a + d.b| <= 4h) every product stays within 0.875 q,
+// a group of four products within 3.5 q, and each reduce_lazy input below
+// within 6.8 q < 2^53.
+__device__ __forceinline__ double balance(double x, Mod M) { return x > 0.5 * M.q ? x - M.q : x; }
+__device__ __forceinline__ double canon(double x, Mod M) { return x < 0.0 ? x + M.q : x; }
+
+// One round over NP pairs (w0, w1) = (cur[y], cur[y + NP]): adds
+// f0 = sum w0^2 and f1 = sum w0 (w1 - w0) into the lazy accumulators and folds
+// cur[y] <- w0 + c (w1 - w0).  csum = c.a + c.b.  F_{q^2} products take three
+// multiplications: x y = (xa ya + nr xb yb) + ((xa + xb)(ya + yb) - xa ya - xb yb) Y,
+// x^2 = (xa^2 + nr xb^2) + 2 xa xb Y.  Sums are reduced per group of four pairs.
+template <bool FQ2, bool SNR, int NP>
+__device__ __forceinline__ void sc_level(F2 *cur, F2 c, double csum, F2 &f0, F2 &f1,
+                                         double nr, Mod M) {
+#pragma unroll
+    for (int g = 0; g < NP; g += 4) {
+        double A = 0.0, B = 0.0, C = 0.0, K1 = 0.0, K2 = 0.0, K3 = 0.0;
+#pragma unroll
+        for (int y = g; y < (NP < g + 4 ? NP : g + 4); y++) {
+            const F2 w0 = cur[y], w1 = cur[y + NP];
+            const double da = w1.a - w0.a;
+            A += mul_lazy(w0.a, w0.a, M);
+            K1 += mul_lazy(w0.a, da, M);
+            const double m1 = mul_lazy(c.a, da, M);
+            if (FQ2) {
+                const double db = w1.b - w0.b, sd = da + db;
+                B += mul_lazy(w0.b, w0.b, M);
+                C += mul_lazy(w0.a, w0.b, M);
+                K2 += mul_lazy(w0.b, db, M);
+                K3 += mul_lazy(w0.a + w0.b, sd, M);
+                const double m2 = mul_lazy(c.b, db, M), m3 = mul_lazy(csum, sd, M);
+                cur[y].a = reduce_lazy(w0.a + m1 + nr_times<SNR>(nr, m2, M), M);
+                cur[y].b = reduce_lazy(w0.b + (m3 - m1 - m2), M);
+            } else {
+                cur[y].a = reduce_lazy(w0.a + m1, M);
+            }
+        }
+        if (FQ2) {
+            f0.a = reduce_lazy(f0.a + A + nr_times<SNR>(nr, reduce_lazy(B, M), M), M);
+            f0.b = reduce_lazy(f0.b + 2.0 * C, M);
+            f1.a = reduce_lazy(f1.a + K1 + nr_times<SNR>(nr, reduce_lazy(K2, M), M), M);
+            f1.b = reduce_lazy(f1.b + reduce_lazy(K3, M) - K1 - K2, M);
+        } else {
+            f0.a = reduce_lazy(f0.a + A, M);
+            f1.a = reduce_lazy(f1.a + K1, M);
+        }
+    }
+}
+
 // challenges of one row: cs[rep][t][2] (rows either share one block or each
 // row has its own, cs_row u64 apart)
 template <bool FQ2>
 __device__ __forceinline__ F2 load_chal(const uint64_t *cs, int rep, int nrounds, int t) {
     const uint64_t *p = cs + ((size_t)rep * nrounds + t) * 2;
     return {u2d(p[0]), FQ2 ? u2d(p[1]) : 0.0};
+}
+
+template <bool FQ2>
+__device__ __forceinline__ F2 load_chal_bal(const uint64_t *cs, int rep, int nrounds, int t, Mod M) {
+    const F2 c = load_chal<FQ2>(cs, rep, nrounds, t);
+    return {balance(c.a, M), FQ2 ? balance(c.b, M) : 0.0};
 }
 
 __device__ __forceinline__ size_t macc_idx(int rep, int t, int row, int nrounds, int rows) {
@@ -136,7 +200,7 @@ __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) 
 // F_q input w [rows][L], rounds 0..3 for REPS repetitions (reps rep0..).
 // Round 0's messages and the norm_check_1 claim do not depend on challenges.
 template <int REPS, bool FQ2, bool SNR>
-__global__ __launch_bounds__(SC_THREADS) void k_sc_first(
+__global__ __launch_bounds__(SC_THREADS) SC_WPE void k_sc_first(
     const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
     int rows, int rep0, int reps_total,
     int add_claim, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr,
@@ -147,14 +211,19 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
     const uint64_t *cs = cs0 + row * cs_row;
     const size_t S = L >> 4;
     F2 c[REPS][4];
+    double csum[REPS][4];
 #pragma unroll
     for (int r = 0; r < REPS; r++)
 #pragma unroll
-        for (int t = 0; t < 4; t++) c[r][t] = load_chal<FQ2>(cs, rep0 + r, nrounds, t);
+        for (int t = 0; t < 4; t++) {
+            c[r][t] = load_chal_bal<FQ2>(cs, rep0 + r, nrounds, t, M);
+            csum[r][t] = c[r][t].a + c[r][t].b;
+        }
 
-    // f2 = sum (w1 - w0)^2 is not accumulated: it follows from the round
-    // identity 2 f0 + 2 f1 + f2 = claim once the totals are known
-    double claim = 0.0, m0[2] = {0.0, 0.0};
+    // Round 0 needs f0 = sum v0^2 and f1 = sum v0 (v1 - v0) = sum v0 v1 - f0;
+    // the claim is sum v0^2 + sum v1^2.  f2 = sum (w1 - w0)^2 is never
+    // accumulated: it follows from the round identity 2 f0 + 2 f1 + f2 = claim.
+    double s0 = 0.0, s1 = 0.0, sx = 0.0;
     F2 acc[REPS][3][2];
 #pragma unroll
     for (int r = 0; r < REPS; r++)
@@ -170,20 +239,17 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
         double v[16];
 #pragma unroll
         for (int y = 0; y < 16; y++) v[y] = u2d(wr[(size_t)y * S + x]);
-#pragma unroll
-        for (int y = 0; y < 16; y++) {
-            claim += mul_lazy(v[y], v[y], M);
-            if ((y & 3) == 3) claim = reduce_lazy(claim, M);
-        }
         double dl[8];
 #pragma unroll
         for (int y = 0; y < 8; y++) {
-            dl[y] = submod(v[y + 8], v[y], M);
-            m0[0] += mul_lazy(v[y], v[y], M);
-            m0[1] += mul_lazy(v[y], dl[y], M);
+            dl[y] = v[y + 8] - v[y];  // |dl| < q
+            s0 += mul_lazy(v[y], v[y], M);
+            s1 += mul_lazy(v[y + 8], v[y + 8], M);
+            sx += mul_lazy(v[y], v[y + 8], M);
             if ((y & 3) == 3) {
-#pragma unroll
-                for (int f = 0; f < 2; f++) m0[f] = reduce_lazy(m0[f], M);
+                s0 = reduce_lazy(s0, M);
+                s1 = reduce_lazy(s1, M);
+                sx = reduce_lazy(sx, M);
             }
         }
 #pragma unroll
@@ -191,21 +257,13 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
             F2 cur[8];
 #pragma unroll
             for (int y = 0; y < 8; y++) {
-                cur[y].a = addmod(v[y], mulmod(c[r][0].a, dl[y], M), M);
-                cur[y].b = FQ2 ? mulmod(c[r][0].b, dl[y], M) : 0.0;
+                cur[y].a = reduce_lazy(v[y] + mul_lazy(c[r][0].a, dl[y], M), M);
+                cur[y].b = FQ2 ? reduce_lazy(mul_lazy(c[r][0].b, dl[y], M), M) : 0.0;
             }
-#pragma unroll
-            for (int t = 1; t < 4; t++) {
-                const int half = 8 >> t;
-#pragma unroll
-                for (int y = 0; y < half; y++) {
-                    F2 w0 = cur[y];
-                    F2 d = x_sub<FQ2>(cur[y + half], w0, M);
-                    x_sq_mac<FQ2, SNR>(acc[r][t - 1][0], w0, nr, M);
-                    x_mac<FQ2, SNR>(acc[r][t - 1][1], w0, d, nr, M);
-                    cur[y] = x_add<FQ2>(w0, x_mul3<FQ2, SNR>(c[r][t], d, nr, M), M);
-                }
-            }
+            sc_level<FQ2, SNR, 4>(cur, c[r][1], csum[r][1], acc[r][0][0], acc[r][0][1], nr, M);
+            sc_level<FQ2, SNR, 2>(cur, c[r][2], csum[r][2], acc[r][1][0], acc[r][1][1], nr, M);
+            sc_level<FQ2, SNR, 1>(cur, c[r][3], csum[r][3], acc[r][2][0], acc[r][2][1], nr, M);
+            // balanced residues; the next pass reads them as such
             double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
             o[0] = cur[0].a;
             o[1] = cur[0].b;
@@ -213,9 +271,9 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
     }
     // reductions: claim, round-0 f0 f1, then per rep 3 rounds x (f0, f1) x 2
     double vals[3 + REPS * 12];
-    vals[0] = reduce(claim, M);
-#pragma unroll
-    for (int f = 0; f < 2; f++) vals[1 + f] = reduce(m0[f], M);
+    vals[0] = reduce(s0 + s1, M);
+    vals[1] = reduce(s0, M);
+    vals[2] = reduce(sx - s0, M);
 #pragma unroll
     for (int r = 0; r < REPS; r++)
 #pragma unroll
@@ -244,7 +302,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_first(
 // ---------------------------------------------------------------- later passes
 // F2 input in [reps][rows][len] (pairs of doubles), rounds t0..t0+3.
 template <bool FQ2, bool SNR>
-__global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
+__global__ __launch_bounds__(SC_THREADS) SC_WPE void k_sc_pass(
     const double *__restrict__ in, size_t len, int rows, int t0, const uint64_t *__restrict__ cs0,
     size_t cs_row, int nrounds, double nr, int cpt, double *__restrict__ out,
     unsigned long long *__restrict__ macc, Mod M) {
@@ -253,8 +311,12 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
     const uint64_t *cs = cs0 + row * cs_row;
     const size_t S = len >> 4;
     F2 c[4];
+    double csum[4];
 #pragma unroll
-    for (int t = 0; t < 4; t++) c[t] = load_chal<FQ2>(cs, rep, nrounds, t0 + t);
+    for (int t = 0; t < 4; t++) {
+        c[t] = load_chal_bal<FQ2>(cs, rep, nrounds, t0 + t, M);
+        csum[t] = c[t].a + c[t].b;
+    }
     F2 acc[4][2];  // (f0, f1) per round; f2 follows from the round identity
 #pragma unroll
     for (int t = 0; t < 4; t++)
@@ -271,18 +333,11 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
             cur[y].a = p[0];
             cur[y].b = FQ2 ? p[1] : 0.0;
         }
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int half = 8 >> t;
-#pragma unroll
-            for (int y = 0; y < half; y++) {
-                F2 w0 = cur[y];
-                F2 d = x_sub<FQ2>(cur[y + half], w0, M);
-                x_sq_mac<FQ2, SNR>(acc[t][0], w0, nr, M);
-                x_mac<FQ2, SNR>(acc[t][1], w0, d, nr, M);
-                cur[y] = x_add<FQ2>(w0, x_mul3<FQ2, SNR>(c[t], d, nr, M), M);
-            }
-        }
+        // input and output are balanced residues (written by the previous pass)
+        sc_level<FQ2, SNR, 8>(cur, c[0], csum[0], acc[0][0], acc[0][1], nr, M);
+        sc_level<FQ2, SNR, 4>(cur, c[1], csum[1], acc[1][0], acc[1][1], nr, M);
+        sc_level<FQ2, SNR, 2>(cur, c[2], csum[2], acc[2][0], acc[2][1], nr, M);
+        sc_level<FQ2, SNR, 1>(cur, c[3], csum[3], acc[3][0], acc[3][1], nr, M);
         double *o = out + (((size_t)rep * rows + row) * S + x) * 2;
         o[0] = cur[0].a;
         o[1] = cur[0].b;
@@ -334,9 +389,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
             cur[x].a = u2d(w[(wrow ? wrow[row] : (size_t)row * len) + x]);
             cur[x].b = 0.0;
         } else {
+            // the streaming passes leave balanced residues
             const double *p = in + (((size_t)rep * rows + row) * len + x) * 2;
-            cur[x].a = p[0];
-            cur[x].b = FQ2 ? p[1] : 0.0;
+            cur[x].a = canon(p[0], M);
+            cur[x].b = FQ2 ? canon(p[1], M) : 0.0;
         }
     }
     if (FIRST) {
