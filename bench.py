@@ -113,11 +113,11 @@ def algorithmic(step):
 
 
 def traffic_bytes():
-    """HBM bytes per launch of the NTT forward kernel over the batched
-    workload's witness arena, measured by rocprofv3 FETCH_SIZE and WRITE_SIZE
-    (each scaled by a plain-copy calibration) and committed in
-    profiles/ntt_inc_fwd_batched_traffic.json."""
-    path = os.path.join(ROOT, "profiles", "ntt_inc_fwd_batched_traffic.json")
+    """HBM bytes per launch of the fused forward NTT + planes kernel over the
+    batched workload's witness arena, measured by rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (each scaled by a plain-copy calibration) and committed in
+    profiles/ntt_fwd_planes_batched_traffic.json."""
+    path = os.path.join(ROOT, "profiles", "ntt_fwd_planes_batched_traffic.json")
     try:
         with open(path) as f:
             return int(json.load(f)["hbm_bytes_per_launch"])
@@ -474,17 +474,25 @@ def main():
         b.step(graph=graph)
     b.step_plain()
     ctx.sync()
-    # per-phase timing of the batched step (eager, events between phases)
-    ev = [ctx.event() for _ in range(len(b.PHASES) + 1)]
+    # per-phase timing of the batched step (eager, events between phases),
+    # then the unfused forward NTT and planes for comparison (the inverse
+    # after the plain forward restores the Coefficient witnesses)
+    names = list(b.PHASES) + ["ntt_fwd", "ntt_inv", "planes"]
+    ev = [ctx.event() for _ in range(len(names) + 1)]
     phase_ms = {k: [] for k in b.PHASES}
+    unfused_ms = {k: [] for k in ("ntt_fwd", "planes")}
     for _ in range(3):
         ctx.record(ev[0])
-        for k in range(len(b.PHASES)):
-            b.phase(k)
+        for k, name in enumerate(names):
+            b.phase(name)
             ctx.record(ev[k + 1])
         ctx.sync()
-        for k, name in enumerate(b.PHASES):
-            phase_ms[name].append(ctx.elapsed_ms(ev[k], ev[k + 1]))
+        for k, name in enumerate(names):
+            ms = ctx.elapsed_ms(ev[k], ev[k + 1])
+            if k < len(b.PHASES):
+                phase_ms[name].append(ms)
+            elif name in unfused_ms:
+                unfused_ms[name].append(ms)
     # timed region: K batched steps (graph replays)
     per, total_ms, wall = timed(ctx, lambda: b.step(graph=graph), args.steps)
     # the plain per-call path on the same instances
@@ -492,11 +500,15 @@ def main():
     exact = bw.bit_exact()
 
     sum_m = sum(bw.m)
-    ntt_bytes = sum_m * 64 * 8 * 2
-    ntt_ms = float(np.mean(phase_ms["ntt_fwd"]))
+    # fused forward NTT + planes: the arena read and written once (8-byte
+    # residues) and 64 slots x 7 digit bytes written per ring element
+    ntt_bytes = sum_m * 64 * 8 * 2 + sum_m * 64 * 7
+    ntt_ms = float(np.mean(phase_ms["ntt_fwd_planes"]))
     ntt_gbs = ntt_bytes / (ntt_ms * 1e-3) / 1e9
     peak = 8000.0
-    roofline = {"bound": "hbm", "kernel": "k_ntt_inc<false> (IncompleteNTT forward, whole arena)",
+    roofline = {"bound": "hbm",
+                "kernel": "k_ntt_fwd_planes (IncompleteNTT forward + commit digit planes, "
+                          "whole arena)",
                 "achieved": round(ntt_gbs, 1), "peak": peak, "unit": "GB/s",
                 "frac": round(ntt_gbs / peak, 4), "traffic": traffic_bytes(),
                 "algorithmic_bytes": ntt_bytes,
@@ -533,6 +545,7 @@ def main():
         "bit_exact_vs_plain": exact,
         "tuning": b.tuning(),
         "phases_ms": {k: round(float(np.mean(v)), 4) for k, v in phase_ms.items()},
+        "unfused_ms": {k: round(float(np.mean(v)), 4) for k, v in unfused_ms.items()},
         "roofline": roofline,
     }
     b.close()

@@ -454,17 +454,25 @@ class Batch:
     def step_plain(self):
         _check(lib.salsaa_batch_step_plain(self.h))
 
-    PHASES = ("ntt_fwd", "ntt_inv", "planes", "commit", "norm_check")
+    # the batched step's phases in order (forward NTT fused with the commit's
+    # digit planes, inverse NTT, commit, norm checks), and two more that are
+    # not part of it: the unfused planes (from the Coefficient witnesses) and
+    # the unfused forward NTT
+    PHASES = ("ntt_fwd_planes", "ntt_inv", "commit", "norm_check")
+    _PHASE_CODE = {"ntt_fwd_planes": 0, "ntt_inv": 1, "planes": 2, "commit": 3,
+                   "norm_check": 4, "ntt_fwd": 5}
 
-    def phase(self, k):
-        """Run phase k of the batched step eagerly (0..4, see PHASES)."""
-        _check(lib.salsaa_batch_phase(self.h, int(k)))
+    def phase(self, name):
+        """Run one phase eagerly by name (PHASES, "planes" or "ntt_fwd")."""
+        _check(lib.salsaa_batch_phase(self.h, self._PHASE_CODE[name]))
 
     def arena(self):
         """The witness arena: every instance's witness back to back (sorted
-        by length), as one [total][d] view."""
+        by length), each padded with zeros to whole tiles of 128 ring
+        elements, as one [total][d] view."""
         p = min((self.slot(i)["witness"] for i in range(self.n)), key=lambda a: a.ptr)
-        return DeviceArray.view(self.ctx, p.ptr, (sum(self.m), D))
+        total = sum((m + 127) // 128 * 128 for m in self.m)
+        return DeviceArray.view(self.ctx, p.ptr, (total, D))
 
     def tuning(self):
         kc, cpt = ctypes.c_size_t(), ctypes.c_int()

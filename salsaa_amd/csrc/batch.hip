@@ -3,8 +3,11 @@
 // captured hipGraph.
 //
 // Instances are sorted by witness length (descending) once, at plan time.
-//   * NTT: the witnesses sit back to back in one arena, so the forward and
-//     inverse transforms are one launch each over every ring element.
+//   * NTT: the witnesses sit back to back in one arena, each in whole tiles of
+//     128 ring elements (zero beyond its length), so the forward and inverse
+//     transforms are one launch each over every ring element.  The forward
+//     launch also writes the commit's digit planes: every tile belongs to one
+//     instance, and a table gives each tile's place in the planes.
 //   * Commit: every instance is one witness row committed against the same
 //     resident key, so consecutive instances form row blocks of 16 (one MFMA
 //     N block) and the whole batch is ONE commit: each block of the launch
@@ -81,7 +84,10 @@ struct salsaa_batch {
     size_t pnblk = 0;
     PlaneJob *d_pjobs = nullptr;
     unsigned long long *d_off = nullptr, *d_m = nullptr;
+    unsigned long long *d_wrow = nullptr;  // witness start of each instance (u64)
+    PlaneTile *d_ptiles = nullptr;         // per arena tile: where its planes go
     int8_t *d_planes = nullptr;
+    size_t planes_bytes = 0;
     // commit
     size_t njobs = 0, max_tiles = 0, kchunk = 0;
     std::vector<int> job_ktiles, job_rows;
@@ -217,18 +223,22 @@ int run_norm_checks(salsaa_batch *b, int cpt) {
         no.ok = o.ok + p * b->reps;
         no.ok_row = b->reps;
         no.ok_rep = 1;
-        SALSAA_TRY(launch_norm_check_ex(ctx, b->d_wit + b->off_w[p] * kD, nullptr, g.rows, g.L,
+        SALSAA_TRY(launch_norm_check_ex(ctx, b->d_wit, b->d_wrow + p, g.rows, g.L,
                                         b->d_cs + b->off_c[p], b->reps * nr * 2, b->reps, b->fq2,
                                         b->nr, no, cpt, &b->nc));
     }
     return SALSAA_OK;
 }
 
+int run_fwd_planes(salsaa_batch *b) {
+    return launch_ntt_fwd_planes(&b->ctx->c, b->d_wit, b->tot_w, b->d_planes, b->d_ptiles);
+}
+
+// forward NTT with the commit planes, inverse NTT, commit, norm checks
 int run_step(salsaa_batch *b) {
     Context *ctx = &b->ctx->c;
-    SALSAA_TRY(launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 0));
+    SALSAA_TRY(run_fwd_planes(b));
     SALSAA_TRY(launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 1));
-    SALSAA_TRY(run_planes(b));
     SALSAA_TRY(run_commit(b));
     return run_norm_checks(b, b->cpt_first);
 }
@@ -304,6 +314,8 @@ void destroy(salsaa_batch *b) {
     (void)hipFree(b->d_off);
     (void)hipFree(b->d_m);
     (void)hipFree(b->d_planes);
+    (void)hipFree(b->d_ptiles);
+    (void)hipFree(b->d_wrow);
     (void)hipFree(b->d_partial);
     (void)hipFree(b->d_cjobs);
     (void)hipFree(b->d_table);
@@ -353,7 +365,7 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
         b->off_w[p] = b->tot_w;
         b->off_c[p] = b->tot_c;
         b->off_msg[p] = b->tot_msg;
-        b->tot_w += b->m[i];
+        b->tot_w += (b->m[i] + 127) / 128 * 128;  // whole tiles
         b->tot_c += reps * b->nrounds[p] * 2;
         b->tot_msg += reps * b->nrounds[p] * 6;
     }
@@ -393,7 +405,34 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
     }
     SALSAA_CHECK(b->pnblk < (1u << 31), "batch too large");
     SALSAA_TRY(dmalloc(&b->d_planes, planes_bytes));
+    b->planes_bytes = planes_bytes;
+    // Columns of a shorter instance past its own tiles, up to its row
+    // block's length, are never written by a step: zero them once here.
+    hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div(planes_bytes / 8, 256)), dim3(256), 0,
+                       c->stream, reinterpret_cast<uint64_t *>(b->d_planes), planes_bytes / 8);
+    SALSAA_HIP(hipGetLastError());
     for (size_t j = 0; j < b->njobs; j++) b->pjobs[j].dst = b->d_planes + b->job_planes[j];
+    // each arena tile's place in the planes (see store_planes): value row
+    // p % 16 of row block p / 16, K tiles from its offset in the instance
+    std::vector<PlaneTile> pt(b->tot_w / 128);
+    std::vector<unsigned long long> hwrow(n);
+    for (size_t p = 0; p < n; p++) {
+        const size_t j = p / 16, row = p % 16, rt = (7 * (size_t)b->job_rows[j] + 15) / 16;
+        const size_t kstride = rt * 1024, sstride = (size_t)b->job_ktiles[j] * kstride;
+        SALSAA_CHECK(sstride < (1ull << 32), "row block too large for the plane tile table");
+        hwrow[p] = b->off_w[p] * kD;
+        const size_t tiles = (hm[p] + 127) / 128;
+        for (size_t k = 0; k < tiles; k++) {
+            PlaneTile &d = pt[b->off_w[p] / 128 + k];
+            d.base = b->job_planes[j] + 2 * k * kstride + row * 7 * 64;
+            d.sstride = (unsigned)sstride;
+            d.kstride = (unsigned)kstride;
+        }
+    }
+    SALSAA_TRY(dmalloc(&b->d_ptiles, pt.size()));
+    SALSAA_TRY(dmalloc(&b->d_wrow, n));
+    SALSAA_TRY(upload(c, b->d_ptiles, pt));
+    SALSAA_TRY(upload(c, b->d_wrow, hwrow));
     SALSAA_TRY(dmalloc(&b->d_pjobs, b->pjobs.size()));
     SALSAA_TRY(dmalloc(&b->d_off, n));
     SALSAA_TRY(dmalloc(&b->d_m, n));
@@ -522,12 +561,13 @@ int salsaa_batch_phase(salsaa_batch *b, int phase) {
     CTX_OK(b->ctx);
     Context *ctx = &b->ctx->c;
     switch (phase) {
-    case 0: return launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 0);
+    case 0: return run_fwd_planes(b);
     case 1: return launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 1);
     case 2: return run_planes(b);
     case 3: return run_commit(b);
     case 4: return run_norm_checks(b, b->cpt_first);
-    default: SALSAA_CHECK(false, "phase must be 0..4");
+    case 5: return launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 0);
+    default: SALSAA_CHECK(false, "phase must be 0..5");
     }
 }
 

@@ -61,6 +61,87 @@ __device__ __forceinline__ void store_block(uint64_t *__restrict__ dst, const ui
     }
 }
 
+// Slot-major staging of a block's 64 x 128 transformed values: slot s,
+// element e sits at wpos(s, e), i.e. row s with each run of 32 elements
+// rotated by r(s, e / 32) = 8 ((s / 8) % 4) + 2 (s % 8) + (e / 32) % 2.  The
+// rotation spreads the three access patterns over the LDS banks: the NTT's
+// writes (8 consecutive e by slots 8 apart), the plane packing's reads
+// (8 consecutive e per thread, 4 slots x 16 runs of 8 per wave) and the
+// slot-major store's reads (64 consecutive e of one slot).
+__device__ __forceinline__ int wpos(int s, int e) {
+    const int r = 8 * ((s >> 3) & 3) + 2 * (s & 7) + ((e >> 5) & 1);
+    return s * EL + (e & ~31) + ((e + r) & 31);
+}
+
+// Balanced base-256 digits of a residue x < 2^50: x = sum_u d_u 256^u with
+// d_u in [-128, 127] (7 digits).  With y = x + sum_u 128 * 256^u the bytes
+// b_u of y give d_u = b_u - 128, whose int8 bit pattern is b_u XOR 0x80.
+constexpr uint64_t DIGIT_BIAS = 0x0080808080808080ULL;
+
+// Fragment-tiled digit planes for the int8 MFMA commit.  A matrix of R value
+// rows has 7R digit rows (row r*7 + u holds digit u), grouped in tiles of 16;
+// columns are grouped in K-tiles of 64.  Fragment (s, kt, rt) is the 1 KiB
+// block at ((s * KT + kt) * RT + rt) * 1024, RT = ceil(7R / 16), KT = m
+// rounded up to 128, over 64, stored row-major (digit row dr % 16 at
+// (dr % 16) * 64).  So one (slot, K-tile) step of the whole matrix is a
+// row-major [16 RT][64] byte block: the row tiles a commit block needs for
+// one K step are consecutive, and the 7 digit rows of one value row are 448
+// contiguous bytes, which this writer fills with coalesced 8-byte stores.
+// Columns in [m, 64 KT) are written as zero; the padding rows of the last row
+// tile are never written and only feed products that are discarded.
+// Each thread packs one digit of 8 columns into one 8-byte store.
+// `base` is the fragment byte of (slot 0, K tile i0 / 64, value row row), i.e.
+// dst + ((i0 / 64) RT) 1024 + 448 row; slots are sstride = KT RT 1024 bytes
+// apart and K tiles kstride = RT 1024.
+__device__ __forceinline__ void store_planes_one(int8_t *__restrict__ base, size_t sstride,
+                                                 size_t kstride, const uint64_t *work, int nel,
+                                                 int c) {
+    {
+        const int s = c >> 4, c8 = c & 15;
+        // two halves of 4 columns each, so that only 4 values are live at once
+        uint32_t p[7][2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            uint32_t lo[4], hi[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int e = 8 * c8 + 4 * h + q;
+                uint64_t y = 0;  // zero padding: every digit 0
+                if (e < nel) y = (d2u(bitsd(work[wpos(s, e)])) + DIGIT_BIAS) ^ 0x8080808080808080ULL;
+                lo[q] = (uint32_t)y;
+                hi[q] = (uint32_t)(y >> 32);
+            }
+#pragma unroll
+            for (int u = 0; u < 7; u++) {
+                const uint32_t *src = u < 4 ? lo : hi;
+                const int sh = 8 * (u & 3);
+                uint32_t x = 0;
+#pragma unroll
+                for (int q = 0; q < 4; q++) x |= ((src[q] >> sh) & 0xFFu) << (8 * q);
+                p[u][h] = x;
+            }
+        }
+        int8_t *fbase = base + s * sstride + (c8 >> 3) * kstride + (c8 & 7) * 8;
+#pragma unroll
+        for (int u = 0; u < 7; u++)
+            *reinterpret_cast<uint2 *>(fbase + u * 64) = make_uint2(p[u][0], p[u][1]);
+    }
+}
+__device__ __forceinline__ void store_planes(int8_t *__restrict__ base, size_t sstride,
+                                             size_t kstride, const uint64_t *work, int nel,
+                                             int tid) {
+    for (int c = tid; c < 64 * (EL / 8); c += NT)
+        store_planes_one(base, sstride, kstride, work, nel, c);
+}
+
+__device__ __forceinline__ size_t plane_kstride(size_t rows) { return (rows * 7 + 15) / 16 * 1024; }
+__device__ __forceinline__ size_t plane_sstride(size_t rows, size_t ktiles) {
+    return ktiles * plane_kstride(rows);
+}
+__device__ __forceinline__ int8_t *plane_base(int8_t *dst, size_t rows, size_t row, size_t i0) {
+    return dst + (i0 / 64) * plane_kstride(rows) + row * 7 * 64;
+}
+
 // Tile staging for the persistent NTT: thread tid moves chunks tid + NT i
 // (i < 4) of a tile of EL ring elements, held in registers so the next tile's
 // loads are in flight while this one is computed.  Branch-free, so the loads
@@ -208,6 +289,101 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     }
 }
 
+// Forward IncompleteNTT of whole tiles fused with the full-NTT digit planes
+// of every tile (persistent, pipelined as k_ntt_inc).  Once a tile's
+// IncompleteNTT is in LDS it is stored, and its 64 full-NTT slots are formed
+// from the halves into a second LDS buffer in the wpos layout: slots s and
+// s + 1 (s even) evaluate at zeta_s and -zeta_s (brv6(s + 1) = brv6(s) + 32),
+// so with zj = zj_s they are E[zj] +- zeta_s O[zj], and thread (e, h) forms
+// slots 8 h .. 8 h + 7 of element e.  The next tile then moves into the first
+// buffer while the planes are written (store_planes) where ptiles[tile] says.
+// 138 KiB of LDS: one block per CU, so up to 128 registers.
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ntt_fwd_planes(
+    uint64_t *__restrict__ data, size_t ntiles, const double *__restrict__ tab, Mod M,
+    int8_t *__restrict__ planes, const PlaneTile *__restrict__ ptiles) {
+    __shared__ uint64_t lds[EL * SROW];
+    __shared__ uint64_t work[64 * EL];
+    __shared__ double twl[32];
+    __shared__ double zt[32];
+    __shared__ int zo[32];
+    size_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    const int tid = threadIdx.x, e = tid >> 3, h = tid & 7, p = h >> 2, t = h & 3;
+    if (tid < 32) twl[tid] = tab[TAB_FWD32 + tid];
+    if (tid >= 64 && tid < 96) {
+        zt[tid - 64] = tab[TAB_ZETA + 2 * (tid - 64)];
+        zo[tid - 64] = skew((int)tab[TAB_ZJ + 2 * (tid - 64)]);
+    }
+    uint64_t *row = lds + e * SROW;
+    const uint64_t *src = data + tile * EL * kD;
+    ulonglong2 r0 = tile_chunk(src, tid, EL), r1 = tile_chunk(src, tid + NT, EL),
+               r2 = tile_chunk(src, tid + 2 * NT, EL), r3 = tile_chunk(src, tid + 3 * NT, EL);
+    PlaneTile dsc = ptiles[tile];
+    tile_put1<false>(lds, r0, tid);
+    tile_put1<false>(lds, r1, tid + NT);
+    tile_put1<false>(lds, r2, tid + 2 * NT);
+    tile_put1<false>(lds, r3, tid + 3 * NT);
+    __syncthreads();
+    // Loads and stores are unconditional (past the last tile the prefetch
+    // re-reads tile 0), so each wait covers only the loads it needs.
+    size_t next = tile + gridDim.x;
+    src = next < ntiles ? data + next * EL * kD : data;
+    r0 = tile_chunk(src, tid, EL);
+    r1 = tile_chunk(src, tid + NT, EL);
+    r2 = tile_chunk(src, tid + 2 * NT, EL);
+    r3 = tile_chunk(src, tid + 3 * NT, EL);
+    PlaneTile ndsc = ptiles[next < ntiles ? next : 0];
+    for (;;) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = u2d(row[8 * k + 2 * t + p]);  // coeff 2(t+4k)+p
+        ntt32_fwd_a(v, twl, M);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 8; k++) row[36 * p + skew(t + 4 * k)] = dbits(v[k]);
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < 8; x++) v[x] = bitsd(row[36 * p + skew(8 * t + x)]);
+        ntt32_fwd_b(v, twl, t, M);
+#pragma unroll
+        for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = d2u(v[x]);
+        __syncthreads();
+        {
+            ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(data + tile * EL * kD);
+#pragma unroll
+            for (int i = 0; i < 4; i++) dst[tid + NT * i] = tile_get1<true>(lds, tid + NT * i);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int o = zo[4 * h + u];
+            const double E = u2d(row[o]), O = u2d(row[36 + o]);
+            const double z = mulmod(zt[4 * h + u], O, M);
+            work[wpos(8 * h + 2 * u, e)] = dbits(addmod(E, z, M));
+            work[wpos(8 * h + 2 * u + 1, e)] = dbits(submod(E, z, M));
+        }
+        __syncthreads();  // the tile rows are free and the slots complete
+        const bool more = next < ntiles;
+        if (more) {
+            tile_put1<false>(lds, r0, tid);
+            tile_put1<false>(lds, r1, tid + NT);
+            tile_put1<false>(lds, r2, tid + 2 * NT);
+            tile_put1<false>(lds, r3, tid + 3 * NT);
+        }
+        store_planes_one(planes + dsc.base, dsc.sstride, dsc.kstride, work, EL, tid);
+        if (!more) break;
+        __syncthreads();  // the next tile is in LDS; the slot buffer has been read
+        tile = next;
+        dsc = ndsc;
+        next = tile + gridDim.x;
+        src = next < ntiles ? data + next * EL * kD : data;
+        r0 = tile_chunk(src, tid, EL);
+        r1 = tile_chunk(src, tid + NT, EL);
+        r2 = tile_chunk(src, tid + 2 * NT, EL);
+        r3 = tile_chunk(src, tid + 3 * NT, EL);
+        ndsc = ptiles[next < ntiles ? next : 0];
+    }
+}
+
 template <bool INV>
 __global__ __launch_bounds__(NT) void k_ntt_full(uint64_t *__restrict__ data, size_t count,
                                                  const double *__restrict__ tab, Mod M) {
@@ -267,18 +443,6 @@ __global__ __launch_bounds__(NT) void k_ntt_full(uint64_t *__restrict__ data, si
     }
 }
 
-// Slot-major staging of a block's 64 x 128 transformed values: slot s,
-// element e sits at wpos(s, e), i.e. row s with each run of 32 elements
-// rotated by r(s, e / 32) = 8 ((s / 8) % 4) + 2 (s % 8) + (e / 32) % 2.  The
-// rotation spreads the three access patterns over the LDS banks: the NTT's
-// writes (8 consecutive e by slots 8 apart), the plane packing's reads
-// (8 consecutive e per thread, 4 slots x 16 runs of 8 per wave) and the
-// slot-major store's reads (64 consecutive e of one slot).
-__device__ __forceinline__ int wpos(int s, int e) {
-    const int r = 8 * ((s >> 3) & 3) + 2 * (s & 7) + ((e >> 5) & 1);
-    return s * EL + (e & ~31) + ((e + r) & 31);
-}
-
 // Full forward NTT from phase-A inputs v (positions t + 8k); uses `xch` (a
 // 128 x SROW region) for the exchange and leaves the 64 slots in
 // out[wpos(s, e)] (64 x 128 u64; may alias xch).
@@ -315,57 +479,6 @@ __device__ __forceinline__ void store_slotmajor(double *__restrict__ dst, const 
     }
 }
 
-// Balanced base-256 digits of a residue x < 2^50: x = sum_u d_u 256^u with
-// d_u in [-128, 127] (7 digits).  With y = x + sum_u 128 * 256^u the bytes
-// b_u of y give d_u = b_u - 128, whose int8 bit pattern is b_u XOR 0x80.
-constexpr uint64_t DIGIT_BIAS = 0x0080808080808080ULL;
-
-// Fragment-tiled digit planes for the int8 MFMA commit.  A matrix of R value
-// rows has 7R digit rows (row r*7 + u holds digit u), grouped in tiles of 16;
-// columns are grouped in K-tiles of 64.  Fragment (s, kt, rt) is the 1 KiB
-// block at ((s * KT + kt) * RT + rt) * 1024, RT = ceil(7R / 16), KT = m
-// rounded up to 128, over 64, stored row-major (digit row dr % 16 at
-// (dr % 16) * 64).  So one (slot, K-tile) step of the whole matrix is a
-// row-major [16 RT][64] byte block: the row tiles a commit block needs for
-// one K step are consecutive, and the 7 digit rows of one value row are 448
-// contiguous bytes, which this writer fills with coalesced 8-byte stores.
-// Columns in [m, 64 KT) are written as zero; the padding rows of the last row
-// tile are never written and only feed products that are discarded.
-// Each thread packs one digit of 8 columns into one 8-byte store.
-__device__ __forceinline__ void store_planes(int8_t *__restrict__ dst, const uint64_t *work,
-                                             size_t rows, size_t row, size_t ktiles, size_t i0,
-                                             int nel, int tid) {
-    const size_t rt_n = (rows * 7 + 15) / 16;
-    for (int c = tid; c < 64 * (EL / 8); c += NT) {
-        const int s = c >> 4, c8 = c & 15;
-        uint32_t lo[8], hi[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int e = 8 * c8 + q;
-            uint64_t y = 0;  // zero padding: every digit 0
-            if (e < nel) y = (d2u(bitsd(work[wpos(s, e)])) + DIGIT_BIAS) ^ 0x8080808080808080ULL;
-            lo[q] = (uint32_t)y;
-            hi[q] = (uint32_t)(y >> 32);
-        }
-        const size_t kt = i0 / 64 + (c8 >> 3);
-        int8_t *fbase = dst + ((size_t)s * ktiles + kt) * rt_n * 1024 + row * 7 * 64 + (c8 & 7) * 8;
-#pragma unroll
-        for (int u = 0; u < 7; u++) {
-            const uint32_t *src = u < 4 ? lo : hi;
-            const int sh = 8 * (u & 3);
-            uint32_t p[2];
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                uint32_t x = 0;
-#pragma unroll
-                for (int q = 0; q < 4; q++) x |= ((src[4 * h + q] >> sh) & 0xFFu) << (8 * q);
-                p[h] = x;
-            }
-            *reinterpret_cast<uint2 *>(fbase + u * 64) = make_uint2(p[0], p[1]);
-        }
-    }
-}
-
 // grid (ceil(m / 128), rows).  Source row `row` starts at
 // (row % row_lo) * stride_lo + (row / row_lo) * stride_hi (u64 units), which
 // covers plain matrices and the halves of split (split.rs:21-28).
@@ -391,7 +504,8 @@ __device__ __forceinline__ void full_sm_block(uint64_t *lds, const uint64_t *bas
     }
     full_fwd_to_work(v, lds, lds, e, t, live, tab, M);
     if (PLANES)
-        store_planes((int8_t *)dst, lds, rows, row, pstride, i0, nel, tid);  // pstride = K-tiles
+        store_planes(plane_base((int8_t *)dst, rows, row, i0), plane_sstride(rows, pstride),
+                     plane_kstride(rows), lds, nel, tid);  // pstride = K-tiles
     else
         store_slotmajor((double *)dst, lds, rows, row, m, i0, nel, tid);
 }
@@ -490,7 +604,8 @@ __global__ __launch_bounds__(NT) void k_decomp_sm(const uint64_t *__restrict__ W
         }
         full_fwd_to_work(v, work, work, e, t, live, tab, M);
         if (PLANES)
-            store_planes((int8_t *)dst, work, rows, orow, pstride, i0, nel, tid);  // pstride = K-tiles
+            store_planes(plane_base((int8_t *)dst, rows, orow, i0),
+                         plane_sstride(rows, pstride), plane_kstride(rows), work, nel, tid);
         else
             store_slotmajor((double *)dst, work, rows, orow, m, i0, nel, tid);
         __syncthreads();  // work is rewritten by the next limb
@@ -537,6 +652,26 @@ int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse
             hipLaunchKernelGGL(k_ntt_full<false>, dim3(grid), dim3(NT), 0, ctx->stream, data, count,
                                ctx->d_tables, M);
     }
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *planes,
+                          const PlaneTile *tiles) {
+    if (count == 0) return SALSAA_OK;
+    SALSAA_CHECK(count % EL == 0, "fused planes need whole 128-element tiles");
+    Mod M{ctx->qd, ctx->qinv};
+    static int occ = 0;
+    if (occ == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void *>(&k_ntt_fwd_planes), NT, 0) != hipSuccess || n < 1)
+            n = 1;
+        occ = n;
+    }
+    const unsigned g = (unsigned)std::min<size_t>(count / EL, (size_t)ctx->ncu * occ);
+    hipLaunchKernelGGL(k_ntt_fwd_planes, dim3(g), dim3(NT), 0, ctx->stream, data, count / EL,
+                       ctx->d_tables, M, planes, tiles);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }
