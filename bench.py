@@ -477,10 +477,10 @@ def main():
     # per-phase timing of the batched step (eager, events between phases),
     # then the unfused forward NTT and planes for comparison (the inverse
     # after the plain forward restores the Coefficient witnesses)
-    names = list(b.PHASES) + ["ntt_fwd", "ntt_inv", "planes"]
+    names = list(b.PHASES) + ["ntt_fwd_planes", "ntt_inv", "ntt_fwd", "ntt_inv", "planes"]
     ev = [ctx.event() for _ in range(len(names) + 1)]
     phase_ms = {k: [] for k in b.PHASES}
-    unfused_ms = {k: [] for k in ("ntt_fwd", "planes")}
+    unfused_ms = {k: [] for k in b.UNFUSED}
     for _ in range(3):
         ctx.record(ev[0])
         for k, name in enumerate(names):
@@ -500,15 +500,15 @@ def main():
     exact = bw.bit_exact()
 
     sum_m = sum(bw.m)
-    # fused forward NTT + planes: the arena read and written once (8-byte
+    # fused NTT round trip + planes: the arena read and written once (8-byte
     # residues) and 64 slots x 7 digit bytes written per ring element
     ntt_bytes = sum_m * 64 * 8 * 2 + sum_m * 64 * 7
-    ntt_ms = float(np.mean(phase_ms["ntt_fwd_planes"]))
+    ntt_ms = float(np.mean(phase_ms["ntt_roundtrip_planes"]))
     ntt_gbs = ntt_bytes / (ntt_ms * 1e-3) / 1e9
     peak = 8000.0
     roofline = {"bound": "hbm",
-                "kernel": "k_ntt_fwd_planes (IncompleteNTT forward + commit digit planes, "
-                          "whole arena)",
+                "kernel": "k_ntt_fwd_planes<128, true> (IncompleteNTT forward, commit digit "
+                          "planes, IncompleteNTT inverse; whole arena)",
                 "achieved": round(ntt_gbs, 1), "peak": peak, "unit": "GB/s",
                 "frac": round(ntt_gbs / peak, 4), "traffic": traffic_bytes(),
                 "algorithmic_bytes": ntt_bytes,

@@ -454,22 +454,23 @@ class Batch:
     def step_plain(self):
         _check(lib.salsaa_batch_step_plain(self.h))
 
-    # the batched step's phases in order (forward NTT fused with the commit's
-    # digit planes, inverse NTT, commit, norm checks), and two more that are
-    # not part of it: the unfused planes (from the Coefficient witnesses) and
-    # the unfused forward NTT
-    PHASES = ("ntt_fwd_planes", "ntt_inv", "commit", "norm_check")
-    _PHASE_CODE = {"ntt_fwd_planes": 0, "ntt_inv": 1, "planes": 2, "commit": 3,
-                   "norm_check": 4, "ntt_fwd": 5}
+    # the batched step's phases in order (forward NTT, the commit's digit
+    # planes and inverse NTT in one pass; commit; norm checks), and the
+    # unfused alternatives of the first, which are not part of the step
+    PHASES = ("ntt_roundtrip_planes", "commit", "norm_check")
+    UNFUSED = ("ntt_fwd", "ntt_fwd_planes", "ntt_inv", "planes")
+    _PHASE_CODE = {"ntt_roundtrip_planes": 0, "ntt_inv": 1, "planes": 2, "commit": 3,
+                   "norm_check": 4, "ntt_fwd": 5, "ntt_fwd_planes": 6}
 
     def phase(self, name):
-        """Run one phase eagerly by name (PHASES, "planes" or "ntt_fwd")."""
+        """Run one phase eagerly by name (PHASES or UNFUSED)."""
         _check(lib.salsaa_batch_phase(self.h, self._PHASE_CODE[name]))
 
     def arena(self):
         """The witness arena: every instance's witness back to back (sorted
-        by length), each padded with zeros to whole tiles of 128 ring
-        elements, as one [total][d] view."""
+        by length), each padded with zeros to whole tiles of ring
+        elements, as one [total][d] view (tiles of 128, the fused forward
+        NTT + planes kernel's)."""
         p = min((self.slot(i)["witness"] for i in range(self.n)), key=lambda a: a.ptr)
         total = sum((m + 127) // 128 * 128 for m in self.m)
         return DeviceArray.view(self.ctx, p.ptr, (total, D))

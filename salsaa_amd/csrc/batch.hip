@@ -4,10 +4,10 @@
 //
 // Instances are sorted by witness length (descending) once, at plan time.
 //   * NTT: the witnesses sit back to back in one arena, each in whole tiles of
-//     128 ring elements (zero beyond its length), so the forward and inverse
-//     transforms are one launch each over every ring element.  The forward
-//     launch also writes the commit's digit planes: every tile belongs to one
-//     instance, and a table gives each tile's place in the planes.
+//     kPlaneTile ring elements (zero beyond its length), so the forward and inverse
+//     transforms are one pass over every ring element, which also writes the
+//     commit's digit planes: every tile belongs to one instance, and a table
+//     gives each tile's place in the planes.
 //   * Commit: every instance is one witness row committed against the same
 //     resident key, so consecutive instances form row blocks of 16 (one MFMA
 //     N block) and the whole batch is ONE commit: each block of the launch
@@ -230,15 +230,15 @@ int run_norm_checks(salsaa_batch *b, int cpt) {
     return SALSAA_OK;
 }
 
-int run_fwd_planes(salsaa_batch *b) {
-    return launch_ntt_fwd_planes(&b->ctx->c, b->d_wit, b->tot_w, b->d_planes, b->d_ptiles);
+int run_fwd_planes(salsaa_batch *b, int roundtrip) {
+    return launch_ntt_fwd_planes(&b->ctx->c, b->d_wit, b->tot_w, b->d_planes, b->d_ptiles,
+                                 roundtrip);
 }
 
-// forward NTT with the commit planes, inverse NTT, commit, norm checks
+// one pass: forward NTT, the commit's digit planes and the inverse NTT; then
+// the commit and the norm checks
 int run_step(salsaa_batch *b) {
-    Context *ctx = &b->ctx->c;
-    SALSAA_TRY(run_fwd_planes(b));
-    SALSAA_TRY(launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 1));
+    SALSAA_TRY(run_fwd_planes(b, 1));
     SALSAA_TRY(run_commit(b));
     return run_norm_checks(b, b->cpt_first);
 }
@@ -365,7 +365,7 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
         b->off_w[p] = b->tot_w;
         b->off_c[p] = b->tot_c;
         b->off_msg[p] = b->tot_msg;
-        b->tot_w += (b->m[i] + 127) / 128 * 128;  // whole tiles
+        b->tot_w += (b->m[i] + kPlaneTile - 1) / kPlaneTile * kPlaneTile;  // whole tiles
         b->tot_c += reps * b->nrounds[p] * 2;
         b->tot_msg += reps * b->nrounds[p] * 6;
     }
@@ -414,17 +414,17 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
     for (size_t j = 0; j < b->njobs; j++) b->pjobs[j].dst = b->d_planes + b->job_planes[j];
     // each arena tile's place in the planes (see store_planes): value row
     // p % 16 of row block p / 16, K tiles from its offset in the instance
-    std::vector<PlaneTile> pt(b->tot_w / 128);
+    std::vector<PlaneTile> pt(b->tot_w / kPlaneTile);
     std::vector<unsigned long long> hwrow(n);
     for (size_t p = 0; p < n; p++) {
         const size_t j = p / 16, row = p % 16, rt = (7 * (size_t)b->job_rows[j] + 15) / 16;
         const size_t kstride = rt * 1024, sstride = (size_t)b->job_ktiles[j] * kstride;
         SALSAA_CHECK(sstride < (1ull << 32), "row block too large for the plane tile table");
         hwrow[p] = b->off_w[p] * kD;
-        const size_t tiles = (hm[p] + 127) / 128;
+        const size_t tiles = (hm[p] + kPlaneTile - 1) / kPlaneTile;
         for (size_t k = 0; k < tiles; k++) {
-            PlaneTile &d = pt[b->off_w[p] / 128 + k];
-            d.base = b->job_planes[j] + 2 * k * kstride + row * 7 * 64;
+            PlaneTile &d = pt[b->off_w[p] / kPlaneTile + k];
+            d.base = b->job_planes[j] + k * (kPlaneTile / 64) * kstride + row * 7 * 64;
             d.sstride = (unsigned)sstride;
             d.kstride = (unsigned)kstride;
         }
@@ -561,13 +561,14 @@ int salsaa_batch_phase(salsaa_batch *b, int phase) {
     CTX_OK(b->ctx);
     Context *ctx = &b->ctx->c;
     switch (phase) {
-    case 0: return run_fwd_planes(b);
+    case 0: return run_fwd_planes(b, 1);
     case 1: return launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 1);
     case 2: return run_planes(b);
     case 3: return run_commit(b);
     case 4: return run_norm_checks(b, b->cpt_first);
     case 5: return launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 0);
-    default: SALSAA_CHECK(false, "phase must be 0..5");
+    case 6: return run_fwd_planes(b, 0);
+    default: SALSAA_CHECK(false, "phase must be 0..6");
     }
 }
 

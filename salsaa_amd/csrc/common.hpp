@@ -238,18 +238,27 @@ struct PlaneJob {
 int launch_ntt_planes_batch(Context *ctx, const uint64_t *arena, const unsigned long long *inst_off,
                             const unsigned long long *inst_m, const PlaneJob *jobs, size_t njobs,
                             size_t nblk);
-// Where the digit planes of one 128-element tile of a witness arena go: the
-// fragment byte of (slot 0, the tile's first K tile, its value row), and the
-// byte distances between slots and between consecutive K tiles.
+// Ring elements per tile of the fused forward NTT + planes kernel (a
+// multiple of 64: whole K tiles of the planes).  128 (one block of 1024
+// threads per CU) measured faster than 64 (two blocks of 512).
+#ifndef SALSAA_PLANE_TILE
+#define SALSAA_PLANE_TILE 128
+#endif
+constexpr int kPlaneTile = SALSAA_PLANE_TILE;
+// Where the digit planes of one kPlaneTile-element tile of a witness arena
+// go: the fragment byte of (slot 0, the tile's first K tile, its value row),
+// and the byte distances between slots and between consecutive K tiles.
 struct PlaneTile {
     unsigned long long base = 0;
     unsigned sstride = 0, kstride = 0;
 };
-// Forward IncompleteNTT of `count` (a multiple of 128) arena elements in
+// Forward IncompleteNTT of `count` (a multiple of kPlaneTile) arena elements in
 // place, fused with the full-NTT digit planes of every tile (tiles[i] for
-// elements 128 i ..), written under `planes`.
+// elements kPlaneTile i ..), written under `planes`.  roundtrip: the inverse
+// IncompleteNTT follows in the same pass, so the arena ends in Coefficient
+// form again (the forward + inverse NTT pair of a prover step).
 int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *planes,
-                          const PlaneTile *tiles);
+                          const PlaneTile *tiles, int roundtrip);
 int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
                        size_t a_ktiles, const int8_t *WB, size_t r, size_t m, size_t ktW,
                        uint64_t *out);

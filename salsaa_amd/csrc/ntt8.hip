@@ -68,9 +68,10 @@ __device__ __forceinline__ void store_block(uint64_t *__restrict__ dst, const ui
 // writes (8 consecutive e by slots 8 apart), the plane packing's reads
 // (8 consecutive e per thread, 4 slots x 16 runs of 8 per wave) and the
 // slot-major store's reads (64 consecutive e of one slot).
+template <int L = EL>
 __device__ __forceinline__ int wpos(int s, int e) {
     const int r = 8 * ((s >> 3) & 3) + 2 * (s & 7) + ((e >> 5) & 1);
-    return s * EL + (e & ~31) + ((e + r) & 31);
+    return s * L + (e & ~31) + ((e + r) & 31);
 }
 
 // Balanced base-256 digits of a residue x < 2^50: x = sum_u d_u 256^u with
@@ -93,11 +94,13 @@ constexpr uint64_t DIGIT_BIAS = 0x0080808080808080ULL;
 // `base` is the fragment byte of (slot 0, K tile i0 / 64, value row row), i.e.
 // dst + ((i0 / 64) RT) 1024 + 448 row; slots are sstride = KT RT 1024 bytes
 // apart and K tiles kstride = RT 1024.
+// Item c of L / 8 runs of 8 columns per slot (L = the tile's columns).
+template <int L = EL>
 __device__ __forceinline__ void store_planes_one(int8_t *__restrict__ base, size_t sstride,
                                                  size_t kstride, const uint64_t *work, int nel,
                                                  int c) {
     {
-        const int s = c >> 4, c8 = c & 15;
+        const int s = c / (L / 8), c8 = c % (L / 8);
         // two halves of 4 columns each, so that only 4 values are live at once
         uint32_t p[7][2];
 #pragma unroll
@@ -107,7 +110,7 @@ __device__ __forceinline__ void store_planes_one(int8_t *__restrict__ base, size
             for (int q = 0; q < 4; q++) {
                 const int e = 8 * c8 + 4 * h + q;
                 uint64_t y = 0;  // zero padding: every digit 0
-                if (e < nel) y = (d2u(bitsd(work[wpos(s, e)])) + DIGIT_BIAS) ^ 0x8080808080808080ULL;
+                if (e < nel) y = (d2u(bitsd(work[wpos<L>(s, e)])) + DIGIT_BIAS) ^ 0x8080808080808080ULL;
                 lo[q] = (uint32_t)y;
                 hi[q] = (uint32_t)(y >> 32);
             }
@@ -121,10 +124,14 @@ __device__ __forceinline__ void store_planes_one(int8_t *__restrict__ base, size
                 p[u][h] = x;
             }
         }
+        // nontemporal: the planes are read back by the commit long after,
+        // from HBM, so they need not take up cache on the way out
         int8_t *fbase = base + s * sstride + (c8 >> 3) * kstride + (c8 & 7) * 8;
 #pragma unroll
         for (int u = 0; u < 7; u++)
-            *reinterpret_cast<uint2 *>(fbase + u * 64) = make_uint2(p[u][0], p[u][1]);
+            __builtin_nontemporal_store(
+                (unsigned long long)p[u][0] | ((unsigned long long)p[u][1] << 32),
+                reinterpret_cast<unsigned long long *>(fbase + u * 64));
     }
 }
 __device__ __forceinline__ void store_planes(int8_t *__restrict__ base, size_t sstride,
@@ -297,41 +304,53 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 // so with zj = zj_s they are E[zj] +- zeta_s O[zj], and thread (e, h) forms
 // slots 8 h .. 8 h + 7 of element e.  The next tile then moves into the first
 // buffer while the planes are written (store_planes) where ptiles[tile] says.
-// 138 KiB of LDS: one block per CU, so up to 128 registers.
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ntt_fwd_planes(
+// RT (the round trip): the tile's inverse IncompleteNTT follows in the same
+// pass, computed from the forward outputs (each thread starts the inverse
+// from the values its forward stage left in registers), and the Coefficient
+// tile is stored instead of the IncompleteNTT one, which then never goes to
+// HBM.
+// Tiles of TEL ring elements (8 threads each).  TEL = 64: 69 KiB of LDS, two
+// blocks of 512 threads per CU; TEL = 128: 138 KiB, one block of 1024.
+// Either way four waves per SIMD, so up to 128 registers.
+template <int TEL, bool RT>
+__global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ntt_fwd_planes(
     uint64_t *__restrict__ data, size_t ntiles, const double *__restrict__ tab, Mod M,
     int8_t *__restrict__ planes, const PlaneTile *__restrict__ ptiles) {
-    __shared__ uint64_t lds[EL * SROW];
-    __shared__ uint64_t work[64 * EL];
+    constexpr int TNT = 8 * TEL;
+    __shared__ uint64_t lds[TEL * SROW];
+    __shared__ uint64_t work[64 * TEL];
     __shared__ double twl[32];
+    __shared__ double twi[RT ? 33 : 1];
     __shared__ double zt[32];
     __shared__ int zo[32];
     size_t tile = blockIdx.x;
     if (tile >= ntiles) return;
     const int tid = threadIdx.x, e = tid >> 3, h = tid & 7, p = h >> 2, t = h & 3;
     if (tid < 32) twl[tid] = tab[TAB_FWD32 + tid];
+    if (RT && tid >= 128 && tid < 160) twi[tid - 128] = tab[TAB_INV32 + tid - 128];
+    if (RT && tid == 160) twi[32] = tab[TAB_NINV32];
     if (tid >= 64 && tid < 96) {
         zt[tid - 64] = tab[TAB_ZETA + 2 * (tid - 64)];
         zo[tid - 64] = skew((int)tab[TAB_ZJ + 2 * (tid - 64)]);
     }
     uint64_t *row = lds + e * SROW;
-    const uint64_t *src = data + tile * EL * kD;
-    ulonglong2 r0 = tile_chunk(src, tid, EL), r1 = tile_chunk(src, tid + NT, EL),
-               r2 = tile_chunk(src, tid + 2 * NT, EL), r3 = tile_chunk(src, tid + 3 * NT, EL);
+    const uint64_t *src = data + tile * TEL * kD;
+    ulonglong2 r0 = tile_chunk(src, tid, TEL), r1 = tile_chunk(src, tid + TNT, TEL),
+               r2 = tile_chunk(src, tid + 2 * TNT, TEL), r3 = tile_chunk(src, tid + 3 * TNT, TEL);
     PlaneTile dsc = ptiles[tile];
     tile_put1<false>(lds, r0, tid);
-    tile_put1<false>(lds, r1, tid + NT);
-    tile_put1<false>(lds, r2, tid + 2 * NT);
-    tile_put1<false>(lds, r3, tid + 3 * NT);
+    tile_put1<false>(lds, r1, tid + TNT);
+    tile_put1<false>(lds, r2, tid + 2 * TNT);
+    tile_put1<false>(lds, r3, tid + 3 * TNT);
     __syncthreads();
     // Loads and stores are unconditional (past the last tile the prefetch
     // re-reads tile 0), so each wait covers only the loads it needs.
     size_t next = tile + gridDim.x;
-    src = next < ntiles ? data + next * EL * kD : data;
-    r0 = tile_chunk(src, tid, EL);
-    r1 = tile_chunk(src, tid + NT, EL);
-    r2 = tile_chunk(src, tid + 2 * NT, EL);
-    r3 = tile_chunk(src, tid + 3 * NT, EL);
+    src = next < ntiles ? data + next * TEL * kD : data;
+    r0 = tile_chunk(src, tid, TEL);
+    r1 = tile_chunk(src, tid + TNT, TEL);
+    r2 = tile_chunk(src, tid + 2 * TNT, TEL);
+    r3 = tile_chunk(src, tid + 3 * TNT, TEL);
     PlaneTile ndsc = ptiles[next < ntiles ? next : 0];
     for (;;) {
         double v[8];
@@ -348,38 +367,61 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
 #pragma unroll
         for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = d2u(v[x]);
         __syncthreads();
-        {
-            ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(data + tile * EL * kD);
+        ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(data + tile * TEL * kD);
+        if (!RT) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) dst[tid + NT * i] = tile_get1<true>(lds, tid + NT * i);
+            for (int i = 0; i < 4; i++) dst[tid + TNT * i] = tile_get1<true>(lds, tid + TNT * i);
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int o = zo[4 * h + u];
             const double E = u2d(row[o]), O = u2d(row[36 + o]);
             const double z = mulmod(zt[4 * h + u], O, M);
-            work[wpos(8 * h + 2 * u, e)] = dbits(addmod(E, z, M));
-            work[wpos(8 * h + 2 * u + 1, e)] = dbits(submod(E, z, M));
+            work[wpos<TEL>(8 * h + 2 * u, e)] = dbits(addmod(E, z, M));
+            work[wpos<TEL>(8 * h + 2 * u + 1, e)] = dbits(submod(E, z, M));
         }
+        if (RT) ntt32_inv_b(v, twi, t, M);  // v: this thread's forward outputs 32p + 8t + x
         __syncthreads();  // the tile rows are free and the slots complete
         const bool more = next < ntiles;
-        if (more) {
+        if (RT) {
+            store_planes_one<TEL>(planes + dsc.base, dsc.sstride, dsc.kstride, work, TEL, tid);
+#pragma unroll
+            for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
+            ntt32_inv_a(v, twi, twi[32], M);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 8; k++) row[8 * k + 2 * t + p] = d2u(v[k]);
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 4; i++) dst[tid + TNT * i] = tile_get1<false>(lds, tid + TNT * i);
+            if (!more) break;
+            __syncthreads();  // the tile store has read the rows
             tile_put1<false>(lds, r0, tid);
-            tile_put1<false>(lds, r1, tid + NT);
-            tile_put1<false>(lds, r2, tid + 2 * NT);
-            tile_put1<false>(lds, r3, tid + 3 * NT);
+            tile_put1<false>(lds, r1, tid + TNT);
+            tile_put1<false>(lds, r2, tid + 2 * TNT);
+            tile_put1<false>(lds, r3, tid + 3 * TNT);
+        } else {
+            if (more) {
+                tile_put1<false>(lds, r0, tid);
+                tile_put1<false>(lds, r1, tid + TNT);
+                tile_put1<false>(lds, r2, tid + 2 * TNT);
+                tile_put1<false>(lds, r3, tid + 3 * TNT);
+            }
+            store_planes_one<TEL>(planes + dsc.base, dsc.sstride, dsc.kstride, work, TEL, tid);
+            if (!more) break;
         }
-        store_planes_one(planes + dsc.base, dsc.sstride, dsc.kstride, work, EL, tid);
-        if (!more) break;
         __syncthreads();  // the next tile is in LDS; the slot buffer has been read
         tile = next;
         dsc = ndsc;
         next = tile + gridDim.x;
-        src = next < ntiles ? data + next * EL * kD : data;
-        r0 = tile_chunk(src, tid, EL);
-        r1 = tile_chunk(src, tid + NT, EL);
-        r2 = tile_chunk(src, tid + 2 * NT, EL);
-        r3 = tile_chunk(src, tid + 3 * NT, EL);
+        src = next < ntiles ? data + next * TEL * kD : data;
+        r0 = tile_chunk(src, tid, TEL);
+        r1 = tile_chunk(src, tid + TNT, TEL);
+        r2 = tile_chunk(src, tid + 2 * TNT, TEL);
+        r3 = tile_chunk(src, tid + 3 * TNT, TEL);
         ndsc = ptiles[next < ntiles ? next : 0];
     }
 }
@@ -656,22 +698,35 @@ int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse
     return SALSAA_OK;
 }
 
-int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *planes,
-                          const PlaneTile *tiles) {
-    if (count == 0) return SALSAA_OK;
-    SALSAA_CHECK(count % EL == 0, "fused planes need whole 128-element tiles");
-    Mod M{ctx->qd, ctx->qinv};
+template <bool RT>
+int fwd_planes_occupancy() {
     static int occ = 0;
     if (occ == 0) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &n, reinterpret_cast<const void *>(&k_ntt_fwd_planes), NT, 0) != hipSuccess || n < 1)
+                &n, reinterpret_cast<const void *>(&k_ntt_fwd_planes<kPlaneTile, RT>),
+                8 * kPlaneTile, 0) != hipSuccess || n < 1)
             n = 1;
         occ = n;
     }
-    const unsigned g = (unsigned)std::min<size_t>(count / EL, (size_t)ctx->ncu * occ);
-    hipLaunchKernelGGL(k_ntt_fwd_planes, dim3(g), dim3(NT), 0, ctx->stream, data, count / EL,
-                       ctx->d_tables, M, planes, tiles);
+    return occ;
+}
+
+int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *planes,
+                          const PlaneTile *tiles, int roundtrip) {
+    if (count == 0) return SALSAA_OK;
+    SALSAA_CHECK(count % kPlaneTile == 0, "fused planes need whole tiles");
+    Mod M{ctx->qd, ctx->qinv};
+    const size_t nt = count / kPlaneTile;
+    if (roundtrip) {
+        const unsigned g = (unsigned)std::min<size_t>(nt, (size_t)ctx->ncu * fwd_planes_occupancy<true>());
+        hipLaunchKernelGGL((k_ntt_fwd_planes<kPlaneTile, true>), dim3(g), dim3(8 * kPlaneTile), 0,
+                           ctx->stream, data, nt, ctx->d_tables, M, planes, tiles);
+    } else {
+        const unsigned g = (unsigned)std::min<size_t>(nt, (size_t)ctx->ncu * fwd_planes_occupancy<false>());
+        hipLaunchKernelGGL((k_ntt_fwd_planes<kPlaneTile, false>), dim3(g), dim3(8 * kPlaneTile), 0,
+                           ctx->stream, data, nt, ctx->d_tables, M, planes, tiles);
+    }
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }
