@@ -200,6 +200,9 @@ __device__ __forceinline__ double reduce(double x, Mod M) {
 __device__ __forceinline__ double reduce_lazy(double x, Mod M) {
     return __fma_rn(-__builtin_rint(x * M.qinv), M.q, x);
 }
+// canonical x in [0, q) -> balanced (-q/2, q/2]; balanced (|x| < q) -> canonical
+__device__ __forceinline__ double balance(double x, Mod M) { return x > 0.5 * M.q ? x - M.q : x; }
+__device__ __forceinline__ double canon(double x, Mod M) { return x < 0.0 ? x + M.q : x; }
 
 // F_{q^2} = F_q[Y]/(Y^2 - nr), element a + bY
 struct F2 {

@@ -198,8 +198,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     size_t tile = blockIdx.x;
     if (tile >= ntiles) return;
     const int tid = threadIdx.x, e = tid >> 3, h = tid & 7, p = h >> 2, t = h & 3;
-    if (tid < 32) twl[tid] = tab[(INV ? TAB_INV32 : TAB_FWD32) + tid];
-    if (tid == 32) twl[32] = tab[TAB_NINV32];
+    // balanced twiddles for the lazy butterflies
+    if (tid < 32) twl[tid] = balance(tab[(INV ? TAB_INV32 : TAB_FWD32) + tid], M);
+    if (tid == 32) twl[32] = balance(tab[TAB_NINV32], M);
     uint64_t *row = lds + e * SROW;
     // the first tile into LDS, the next one's loads in flight
     int cur = (int)min((size_t)EL, count - tile * EL);
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             if (live) {
 #pragma unroll
                 for (int k = 0; k < 8; k++) v[k] = u2d(row[8 * k + 2 * t + p]);  // coeff 2(t+4k)+p
-                ntt32_fwd_a(v, twl, M);
+                ntt32_fwd_a_lazy(v, twl, M);
             }
             __syncthreads();
             if (live) {
@@ -241,7 +242,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             if (live) {
 #pragma unroll
                 for (int x = 0; x < 8; x++) v[x] = bitsd(row[36 * p + skew(8 * t + x)]);
-                ntt32_fwd_b(v, twl, t, M);
+                ntt32_fwd_b_lazy(v, twl, t, M);
                 // output entry 32 p + 8 t + x, skewed: same slots this thread read
 #pragma unroll
                 for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = d2u(v[x]);
@@ -250,7 +251,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             if (live) {
 #pragma unroll
                 for (int x = 0; x < 8; x++) v[x] = u2d(row[36 * p + skew(8 * t + x)]);
-                ntt32_inv_b(v, twl, t, M);
+                ntt32_inv_b_lazy(v, twl, t, M);
 #pragma unroll
                 for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
             }
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
             if (live) {
 #pragma unroll
                 for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
-                ntt32_inv_a(v, twl, twl[32], M);
+                ntt32_inv_a_lazy(v, twl, twl[32], M);
             }
             __syncthreads();
             if (live) {
@@ -326,9 +327,10 @@ __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4)))
     size_t tile = blockIdx.x;
     if (tile >= ntiles) return;
     const int tid = threadIdx.x, e = tid >> 3, h = tid & 7, p = h >> 2, t = h & 3;
-    if (tid < 32) twl[tid] = tab[TAB_FWD32 + tid];
-    if (RT && tid >= 128 && tid < 160) twi[tid - 128] = tab[TAB_INV32 + tid - 128];
-    if (RT && tid == 160) twi[32] = tab[TAB_NINV32];
+    // balanced twiddles for the lazy butterflies
+    if (tid < 32) twl[tid] = balance(tab[TAB_FWD32 + tid], M);
+    if (RT && tid >= 128 && tid < 160) twi[tid - 128] = balance(tab[TAB_INV32 + tid - 128], M);
+    if (RT && tid == 160) twi[32] = balance(tab[TAB_NINV32], M);
     if (tid >= 64 && tid < 96) {
         zt[tid - 64] = tab[TAB_ZETA + 2 * (tid - 64)];
         zo[tid - 64] = skew((int)tab[TAB_ZJ + 2 * (tid - 64)]);
@@ -356,14 +358,14 @@ __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4)))
         double v[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = u2d(row[8 * k + 2 * t + p]);  // coeff 2(t+4k)+p
-        ntt32_fwd_a(v, twl, M);
+        ntt32_fwd_a_lazy(v, twl, M);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < 8; k++) row[36 * p + skew(t + 4 * k)] = dbits(v[k]);
         __syncthreads();
 #pragma unroll
         for (int x = 0; x < 8; x++) v[x] = bitsd(row[36 * p + skew(8 * t + x)]);
-        ntt32_fwd_b(v, twl, t, M);
+        ntt32_fwd_b_lazy(v, twl, t, M);
 #pragma unroll
         for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = d2u(v[x]);
         __syncthreads();
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4)))
             work[wpos<TEL>(8 * h + 2 * u, e)] = dbits(addmod(E, z, M));
             work[wpos<TEL>(8 * h + 2 * u + 1, e)] = dbits(submod(E, z, M));
         }
-        if (RT) ntt32_inv_b(v, twi, t, M);  // v: this thread's forward outputs 32p + 8t + x
+        if (RT) ntt32_inv_b_lazy(v, twi, t, M);  // v: this thread's forward outputs 32p + 8t + x
         __syncthreads();  // the tile rows are free and the slots complete
         const bool more = next < ntiles;
         if (RT) {
@@ -390,7 +392,7 @@ __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4)))
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
-            ntt32_inv_a(v, twi, twi[32], M);
+            ntt32_inv_a_lazy(v, twi, twi[32], M);
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < 8; k++) row[8 * k + 2 * t + p] = d2u(v[k]);

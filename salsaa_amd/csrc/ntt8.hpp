@@ -79,6 +79,80 @@ __device__ __forceinline__ void ntt32_inv_a(double (&v)[8], const double *__rest
     }
 }
 
+// ---------------------------------------------------------------- n = 32, lazy
+// The same transforms without canonical corrections between stages.  The
+// values stay exact integers congruent to the canonical ones; twiddles are
+// balanced (|w| <= q/2).  mul_lazy(a, b) returns a b - k q exactly with
+// |.| <= q/2 + 3 * 2^-53 |a b| (the rounding of a b, of 1/q and of the
+// quotient estimate), so with |w| <= q/2 a forward stage maps a bound B to
+// B + q/2 + 0.1875 B (in units where q ~ 2^50): from canonical inputs the
+// three stages of fwd_a stay below 3.5 q and the two of fwd_b below 6.0 q,
+// inside 8 q < 2^53 where every sum and the final reduction are exact.
+__device__ __forceinline__ void bf_fwd_lazy(double &X, double &Y, double w, Mod M) {
+    const double T = mul_lazy(Y, w, M);
+    const double x = X;
+    X = x + T;
+    Y = x - T;
+}
+__device__ __forceinline__ void bf_inv_lazy(double &X, double &Y, double w, Mod M) {
+    const double x = X, y = Y;
+    X = x + y;
+    Y = mul_lazy(x - y, w, M);
+}
+// canonical in, |v| < 3.5 q out
+__device__ __forceinline__ void ntt32_fwd_a_lazy(double (&v)[8], const double *__restrict__ tw,
+                                                 Mod M) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) bf_fwd_lazy(v[k], v[k + 4], tw[1], M);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if ((k & 2) == 0) bf_fwd_lazy(v[k], v[k + 2], tw[2 + (k >> 2)], M);
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) bf_fwd_lazy(v[k], v[k + 1], tw[4 + (k >> 1)], M);
+}
+// |v| < 3.5 q in, canonical out
+__device__ __forceinline__ void ntt32_fwd_b_lazy(double (&v)[8], const double *__restrict__ tw,
+                                                 int t, Mod M) {
+    const double w8a = tw[8 + 2 * t], w8b = tw[8 + 2 * t + 1];
+#pragma unroll
+    for (int x = 0; x < 8; x++)
+        if ((x & 2) == 0) bf_fwd_lazy(v[x], v[x + 2], (x & 4) ? w8b : w8a, M);
+#pragma unroll
+    for (int x = 0; x < 8; x += 2) bf_fwd_lazy(v[x], v[x + 1], tw[16 + 4 * t + (x >> 1)], M);
+#pragma unroll
+    for (int x = 0; x < 8; x++) v[x] = reduce(v[x], M);
+}
+// Inverse: X + Y doubles the bound per stage, (X - Y) w maps 2B to
+// q/2 + 0.375 B.  inv_b: canonical in, |v| <= 4 q out; inv_a reduces its
+// input to |v| <= q/2 + 2 first, runs two stages (|v| <= 2 q) and the
+// last with n^{-1} (|x + y| <= 4 q), then reduces to canonical.
+__device__ __forceinline__ void ntt32_inv_b_lazy(double (&v)[8], const double *__restrict__ twi,
+                                                 int t, Mod M) {
+#pragma unroll
+    for (int x = 0; x < 8; x += 2) bf_inv_lazy(v[x], v[x + 1], twi[16 + 4 * t + (x >> 1)], M);
+    const double w8a = twi[8 + 2 * t], w8b = twi[8 + 2 * t + 1];
+#pragma unroll
+    for (int x = 0; x < 8; x++)
+        if ((x & 2) == 0) bf_inv_lazy(v[x], v[x + 2], (x & 4) ? w8b : w8a, M);
+}
+__device__ __forceinline__ void ntt32_inv_a_lazy(double (&v)[8], const double *__restrict__ twi,
+                                                 double ninv, Mod M) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = reduce_lazy(v[k], M);
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) bf_inv_lazy(v[k], v[k + 1], twi[4 + (k >> 1)], M);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        if ((k & 2) == 0) bf_inv_lazy(v[k], v[k + 2], twi[2 + (k >> 2)], M);
+    const double wl = twi[0];  // twi[1] * n^{-1}
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double x = v[k], y = v[k + 4];
+        v[k] = reduce(mul_lazy(x + y, ninv, M), M);
+        v[k + 4] = reduce(mul_lazy(x - y, wl, M), M);
+    }
+}
+
 // ---------------------------------------------------------------- n = 64
 // Thread t (0..7); phase A positions t + 8k, phase B 8t + x.
 __device__ __forceinline__ void ntt64_fwd_a(double (&v)[8], const double *__restrict__ tw, Mod M) {

@@ -99,8 +99,6 @@ __device__ __forceinline__ void x_sq_mac(F2 &acc, F2 x, double nr, Mod M) {
 // |c.a + c.b| <= 2h; |d.a + d.b| <= 4h) every product stays within 0.875 q,
 // a group of four products within 3.5 q, and each reduce_lazy input below
 // within 6.8 q < 2^53.
-__device__ __forceinline__ double balance(double x, Mod M) { return x > 0.5 * M.q ? x - M.q : x; }
-__device__ __forceinline__ double canon(double x, Mod M) { return x < 0.0 ? x + M.q : x; }
 
 // One round over NP pairs (w0, w1) = (cur[y], cur[y + NP]): adds
 // f0 = sum w0^2 and f1 = sum w0 (w1 - w0) into the lazy accumulators and folds
