@@ -549,7 +549,8 @@ int create(salsaa_server *s) {
     size_t nc_m, nc_a, nc_b;
     nc_scratch_need(1, E * kD, o.reps, &nc_m, &nc_a, &nc_b);
     nc_b += o.reps * J * 16 * 2 * sizeof(double);
-    nc_m = (o.reps * kMaxRounds * J * 6 + J) * sizeof(unsigned long long);
+    // messages of every round, the claims and the rows' Gram sums (16 x 16)
+    nc_m = (o.reps * kMaxRounds * J * 6 + J + J * 256) * sizeof(unsigned long long);
     s->nslots = o.inflight;
     for (int si = 0; si < s->nslots; si++) {
         Slot &sl = s->slots[si];

@@ -9,12 +9,16 @@
 // fast_norm_check.rs:375-385), so one HBM pass can run SC_T = 4 rounds: a
 // thread owns the 16 entries {y * S + x : y < 16} of a column x (S = len/16),
 // which are exactly the entries rounds t..t+3 pair up.  Each pass writes the
-// 1/16-length folded witness; per-block message partials are added into u64
+// 1/16-length folded witness.  The first pass (over the F_q witness) gets
+// its messages from the rows' Gram matrices on the matrix cores (see the
+// Gram pass below); the later passes add per-block message partials into u64
 // accumulators with integer atomics (exact, order independent).  Once the
 // length is at most 1024 a single block per (repetition, row) runs the
 // remaining rounds in LDS, captures the length-d folded vector V for
 // norm_check_3, forms rhs = conj(tensor(coefficient challenges)) * V, and
 // replays the verifier over all rounds.
+#include <algorithm>
+
 #include "ntt_device.hpp"
 
 namespace salsaa {
@@ -22,10 +26,6 @@ namespace salsaa {
 namespace {
 
 constexpr int SC_THREADS = 256;
-// occupancy cap of the streaming passes (empty: the compiler's choice)
-#ifndef SC_WPE
-#define SC_WPE
-#endif
 constexpr int SC_FINAL_MAX = 1024;
 
 // ---- field helpers: F_q (FQ2 = false, imaginary parts constant zero) or
@@ -194,113 +194,240 @@ __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) 
     atomicAdd(p, (unsigned long long)d2u(v));
 }
 
-// ---------------------------------------------------------------- first pass
-// F_q input w [rows][L], rounds 0..3 for REPS repetitions (reps rep0..).
-// Round 0's messages and the norm_check_1 claim do not depend on challenges.
-template <int REPS, bool FQ2, bool SNR>
-__global__ __launch_bounds__(SC_THREADS) SC_WPE void k_sc_first(
+// ---------------------------------------------------------------- Gram pass
+// Rounds 0..3 need, per row and repetition, only quadratic forms of the
+// row's 16 x 16 Gram matrix over F_q,
+//     G[y][y'] = sum_x v[y S + x] v[y' S + x]     (S = L / 16):
+// with A_0 = I and A_{t+1}[z] = A_t[z] + c_t (A_t[z + h] - A_t[z]) (h = 8 >> t),
+// round t's witness is w_t[z S + x] = sum_y A_t[z][y] v[y S + x], so
+//     f0 = sum_{z<h} A_t[z]^T G A_t[z],  f1 = sum_{z<h} (A_t[z + h] - A_t[z])^T G A_t[z]
+// (G is symmetric), the claim is tr G, and the 1/16-length witness after
+// round 3 is sum_y A_4[0][y] v[y S + x].  k_sc_gram streams the witness
+// once: every thread folds one column x for the repetitions on VALU (16
+// products per component), and the block's columns go through LDS into the
+// int8 matrix cores for G.  With the 7 signed base-256 digits d_i of each
+// residue (as in the commit), G = sum_k 2^(8k) E_k, E_k = sum_{i+j=k} D_i D_j^T,
+// and each E_k accumulates exactly in int32 (|d| <= 128, at most 7 digit
+// pairs, at most 512 columns per wave).  k_sc_gram_msgs then evaluates the
+// forms for each (row, repetition).
+typedef int v4i __attribute__((ext_vector_type(4)));
+constexpr int GR_X = SC_THREADS;  // columns per iteration
+constexpr int GR_LD = GR_X + 2;   // LDS row stride (u64): fragment reads hit distinct banks
+constexpr int GR_MAXREP = 4;      // repetitions folded per launch
+constexpr uint64_t GR_BIAS = 0x0080808080808080ULL;
+
+// orders one wave's LDS writes before its later reads (and reads before
+// later writes) without waiting for the rest of the block
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// signed digit bytes i of four residues (y_q = (x_q + bias) ^ 0x80..80)
+__device__ __forceinline__ int gr_pack(const uint32_t (&lo)[4], const uint32_t (&hi)[4], int i) {
+    const uint32_t *src = i < 4 ? lo : hi;
+    const int sh = 8 * (i & 3);
+    uint32_t x = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) x |= ((src[q] >> sh) & 0xFFu) << (8 * q);
+    return (int)x;
+}
+
+template <bool FQ2, bool SNR, bool GRAM>
+__global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
     const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
-    int rows, int rep0, int reps_total,
-    int add_claim, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr,
-    int cpt, double *__restrict__ out, unsigned long long *__restrict__ macc,
-    unsigned long long *__restrict__ cacc, Mod M) {
-    __shared__ double red[8 * 64];
-    const int row = blockIdx.y;
+    int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
+    double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc,
+    const double *__restrict__ tab, Mod M) {
+    __shared__ uint64_t T[16 * GR_LD];
+    __shared__ F2 coef[GR_MAXREP][16];
+    const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t *cs = cs0 + row * cs_row;
     const size_t S = L >> 4;
-    F2 c[REPS][4];
-    double csum[REPS][4];
-#pragma unroll
-    for (int r = 0; r < REPS; r++)
-#pragma unroll
+    // fold coefficients A_4[0][y] = prod_s (bit 3 - s of y ? c_s : 1 - c_s), balanced
+    if (tid < 16 * nrep) {
+        const int r = tid >> 4, y = tid & 15;
+        F2 a = {1.0, 0.0};
         for (int t = 0; t < 4; t++) {
-            c[r][t] = load_chal_bal<FQ2>(cs, rep0 + r, nrounds, t, M);
-            csum[r][t] = c[r][t].a + c[r][t].b;
+            const F2 c = load_chal<FQ2>(cs, rep0 + r, nrounds, t);
+            a = x_mul<FQ2, SNR>(a, ((y >> (3 - t)) & 1) ? c : x_sub<FQ2>(F2{1.0, 0.0}, c, M), nr, M);
         }
-
-    // Round 0 needs f0 = sum v0^2 and f1 = sum v0 (v1 - v0) = sum v0 v1 - f0;
-    // the claim is sum v0^2 + sum v1^2.  f2 = sum (w1 - w0)^2 is never
-    // accumulated: it follows from the round identity 2 f0 + 2 f1 + f2 = claim.
-    double s0 = 0.0, s1 = 0.0, sx = 0.0;
-    F2 acc[REPS][3][2];
+        coef[r][y] = {balance(a.a, M), balance(a.b, M)};
+    }
+    __syncthreads();
+    v4i acc[13];
 #pragma unroll
-    for (int r = 0; r < REPS; r++)
-#pragma unroll
-        for (int t = 0; t < 3; t++)
-#pragma unroll
-            for (int f = 0; f < 2; f++) acc[r][t][f] = {0.0, 0.0};
-
+    for (int k = 0; k < 13; k++) acc[k] = (v4i){0, 0, 0, 0};
     const uint64_t *wr = w + (wrow ? wrow[row] : (size_t)row * L);
-    for (int cc = 0; cc < cpt; cc++) {
-        const size_t x = ((size_t)blockIdx.x * cpt + cc) * SC_THREADS + threadIdx.x;
-        if (x >= S) break;
-        double v[16];
+    // Column x = xb + tid; the next iteration's 16 loads are issued before
+    // this one is folded (unconditionally: a dead column re-reads column
+    // S - 1 and counts as zero).
+    size_t xb = (size_t)blockIdx.x * cpt * GR_X;
+    if (xb >= S) return;  // whole block past the end (uniform)
+    double v[16];
+    {
+        const size_t xc = min(xb + tid, S - 1);
 #pragma unroll
-        for (int y = 0; y < 16; y++) v[y] = u2d(wr[(size_t)y * S + x]);
-        double dl[8];
+        for (int y = 0; y < 16; y++) v[y] = u2d(wr[(size_t)y * S + xc]);
+    }
+    for (int cc = 0;; cc++) {
+        const size_t x = xb + tid;
+        const bool live = x < S;
+        const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform
+        double nv[16];
+        {
+            const size_t xc = min(more ? x + GR_X : x, S - 1);
 #pragma unroll
-        for (int y = 0; y < 8; y++) {
-            dl[y] = v[y + 8] - v[y];  // |dl| < q
-            s0 += mul_lazy(v[y], v[y], M);
-            s1 += mul_lazy(v[y + 8], v[y + 8], M);
-            sx += mul_lazy(v[y], v[y + 8], M);
-            if ((y & 3) == 3) {
-                s0 = reduce_lazy(s0, M);
-                s1 = reduce_lazy(s1, M);
-                sx = reduce_lazy(sx, M);
+            for (int y = 0; y < 16; y++) nv[y] = u2d(wr[(size_t)y * S + xc]);
+        }
+        if (live) {
+            // products |coef v| <= 0.69 q (balanced coef, canonical v): eight
+            // of them stay below 5.6 q, then two lazy reductions
+            for (int r = 0; r < nrep; r++) {
+                double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+#pragma unroll
+                for (int y = 0; y < 8; y++) {
+                    const F2 c0 = coef[r][y], c1 = coef[r][y + 8];
+                    a0 += mul_lazy(c0.a, v[y], M);
+                    a1 += mul_lazy(c1.a, v[y + 8], M);
+                    if (FQ2) {
+                        b0 += mul_lazy(c0.b, v[y], M);
+                        b1 += mul_lazy(c1.b, v[y + 8], M);
+                    }
+                }
+                double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
+                o[0] = reduce_lazy(reduce_lazy(a0, M) + a1, M);
+                o[1] = FQ2 ? reduce_lazy(reduce_lazy(b0, M) + b1, M) : 0.0;
             }
         }
+        if (GRAM) {
+            // Each wave multiplies the 64 columns its own lanes hold (lane
+            // 16 g + r takes digit row (i, r), columns 16 g .. 16 g + 15), so
+            // only the wave has to be in step with itself around T.
 #pragma unroll
-        for (int r = 0; r < REPS; r++) {
-            F2 cur[8];
+            for (int y = 0; y < 16; y++) T[y * GR_LD + tid] = live ? d2u(v[y]) : 0;
+            wave_sync();
+            const int r = lane & 15, g = lane >> 4;
+            const uint64_t *src = T + r * GR_LD + 64 * wave + 16 * g;
+            v4i f[7];
 #pragma unroll
-            for (int y = 0; y < 8; y++) {
-                cur[y].a = reduce_lazy(v[y] + mul_lazy(c[r][0].a, dl[y], M), M);
-                cur[y].b = FQ2 ? reduce_lazy(mul_lazy(c[r][0].b, dl[y], M), M) : 0.0;
+            for (int k4 = 0; k4 < 4; k4++) {
+                uint32_t lo[4], hi[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint64_t yv = (src[4 * k4 + q] + GR_BIAS) ^ 0x8080808080808080ULL;
+                    lo[q] = (uint32_t)yv;
+                    hi[q] = (uint32_t)(yv >> 32);
+                }
+#pragma unroll
+                for (int i = 0; i < 7; i++) f[i][k4] = gr_pack(lo, hi, i);
             }
-            sc_level<FQ2, SNR, 4>(cur, c[r][1], csum[r][1], acc[r][0][0], acc[r][0][1], nr, M);
-            sc_level<FQ2, SNR, 2>(cur, c[r][2], csum[r][2], acc[r][1][0], acc[r][1][1], nr, M);
-            sc_level<FQ2, SNR, 1>(cur, c[r][3], csum[r][3], acc[r][2][0], acc[r][2][1], nr, M);
-            // balanced residues; the next pass reads them as such
-            double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
-            o[0] = cur[0].a;
-            o[1] = cur[0].b;
+            wave_sync();  // T is rewritten by the next iteration
+#pragma unroll
+            for (int i = 0; i < 7; i++)
+#pragma unroll
+                for (int j = 0; j < 7; j++)
+                    acc[i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f[i], f[j], acc[i + j], 0, 0, 0);
         }
+        if (!more) break;
+        xb += GR_X;
+#pragma unroll
+        for (int y = 0; y < 16; y++) v[y] = nv[y];
     }
-    // reductions: claim, round-0 f0 f1, then per rep 3 rounds x (f0, f1) x 2
-    double vals[3 + REPS * 12];
-    vals[0] = reduce(s0 + s1, M);
-    vals[1] = reduce(s0, M);
-    vals[2] = reduce(sx - s0, M);
+    if (GRAM) {
+        // lane (r, g) holds E_k[4 g + e][r]; G entry = sum_k 2^(8k) E_k mod q
+        // (every wave reaches this point: the loop above exits uniformly)
+        __shared__ double gpart[4][256];
+        const int r = lane & 15, g = lane >> 4;
+        const double *wt = tab + TAB_W256;
 #pragma unroll
-    for (int r = 0; r < REPS; r++)
+        for (int e = 0; e < 4; e++) {
+            double xg = 0.0;
 #pragma unroll
-        for (int t = 0; t < 3; t++)
-#pragma unroll
-            for (int f = 0; f < 2; f++) {
-                vals[3 + (r * 3 + t) * 4 + f * 2] = reduce(acc[r][t][f].a, M);
-                vals[3 + (r * 3 + t) * 4 + f * 2 + 1] = reduce(acc[r][t][f].b, M);
+            for (int k = 0; k < 13; k++) {
+                xg += mul_lazy((double)acc[k][e], wt[k], M);
+                if ((k % 7) == 6) xg = reduce_lazy(xg, M);
             }
-    block_sum<3 + REPS * 12>(vals, red, M);
-    const int tid = threadIdx.x;
-    if (tid == 0 && add_claim) atomic_add_res(&cacc[row], red[0]);
-    if (tid < REPS * 2) {
-        // round 0 messages f0, f1 (F_q) for every repetition
-        const int r = tid / 2, f = tid % 2;
-        atomic_add_res(&macc[macc_idx(rep0 + r, 0, row, nrounds, rows) + f * 2], red[1 + f]);
+            gpart[wave][(4 * g + e) * 16 + r] = reduce(xg, M);
+        }
+        __syncthreads();
+        double tot = gpart[0][tid];
+#pragma unroll
+        for (int wv = 1; wv < 4; wv++) tot = addmod(tot, gpart[wv][tid], M);
+        atomicAdd(&gacc[(size_t)row * 256 + tid], (unsigned long long)d2u(tot));
     }
-    if (tid < REPS * 12) {
-        const int r = tid / 12, rem = tid % 12, t = rem / 4, k = rem % 4;
-        if (FQ2 || (k & 1) == 0)
-            atomic_add_res(&macc[macc_idx(rep0 + r, t + 1, row, nrounds, rows) + k],
-                           red[3 + tid]);
+}
+
+// Messages of rounds 0..3 (and the claim) from G, one block per (row, rep):
+// thread (z, y) (z < 8, y < 16) forms column y of G A[z] and its share of
+// both forms; the shares are summed across the block.
+template <bool FQ2, bool SNR>
+__global__ __launch_bounds__(128) void k_sc_gram_msgs(const unsigned long long *__restrict__ gacc,
+                                                      int rows, const uint64_t *__restrict__ cs0,
+                                                      size_t cs_row, int nrounds, double nr,
+                                                      unsigned long long *__restrict__ macc,
+                                                      unsigned long long *__restrict__ cacc, Mod M) {
+    __shared__ double G[16][17];
+    __shared__ F2 A[16][17];
+    __shared__ double red[2][4];
+    const int row = blockIdx.x, rep = blockIdx.y, tid = threadIdx.x;
+    const int z = tid >> 4, y = tid & 15, lane = tid & 63, wave = tid >> 6;
+    const uint64_t *cs = cs0 + row * cs_row;
+    for (int i = tid; i < 256; i += 128) {
+        G[i >> 4][i & 15] = u64_mod(gacc[(size_t)row * 256 + i], M);
+        A[i >> 4][i & 15] = {(i >> 4) == (i & 15) ? 1.0 : 0.0, 0.0};
+    }
+    __syncthreads();
+    if (rep == 0 && tid == 0) {
+        double tr = 0.0;
+        for (int k = 0; k < 16; k++) tr = addmod(tr, G[k][k], M);
+        cacc[row] = (unsigned long long)d2u(tr);
+    }
+    for (int t = 0; t < 4; t++) {
+        const int h = 8 >> t;
+        double vals[4] = {0.0, 0.0, 0.0, 0.0};
+        if (z < h) {
+            F2 u = {0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const F2 a = A[z][k];
+                u.a = addmod(u.a, mulmod(G[y][k], a.a, M), M);
+                if (FQ2) u.b = addmod(u.b, mulmod(G[y][k], a.b, M), M);
+            }
+            const F2 az = A[z][y];
+            const F2 f0 = x_mul<FQ2, SNR>(az, u, nr, M);
+            const F2 f1 = x_mul<FQ2, SNR>(x_sub<FQ2>(A[z + h][y], az, M), u, nr, M);
+            vals[0] = f0.a;
+            vals[1] = f0.b;
+            vals[2] = f1.a;
+            vals[3] = f1.b;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double v = vals[k];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) v = addmod(v, __shfl_xor(v, off, 64), M);
+            if (lane == 0) red[wave][k] = v;
+        }
+        __syncthreads();  // also: every G A[z] above has read A
+        if (tid < 4)
+            macc[macc_idx(rep, t, row, nrounds, rows) + tid] =
+                (unsigned long long)d2u(addmod(red[0][tid], red[1][tid], M));
+        // fold the vectors with this repetition's challenge
+        if (z < h) {
+            const F2 c = load_chal<FQ2>(cs, rep, nrounds, t);
+            const F2 a0 = A[z][y];
+            A[z][y] = x_add<FQ2>(a0, x_mul<FQ2, SNR>(c, x_sub<FQ2>(A[z + h][y], a0, M), nr, M), M);
+        }
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------- later passes
 // F2 input in [reps][rows][len] (pairs of doubles), rounds t0..t0+3.
 template <bool FQ2, bool SNR>
-__global__ __launch_bounds__(SC_THREADS) SC_WPE void k_sc_pass(
+__global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
     const double *__restrict__ in, size_t len, int rows, int t0, const uint64_t *__restrict__ cs0,
     size_t cs_row, int nrounds, double nr, int cpt, double *__restrict__ out,
     unsigned long long *__restrict__ macc, Mod M) {
@@ -568,9 +695,10 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
     } else {
         SALSAA_TRY(scratch_get(ctx, 0, need_m, (void **)&macc));
     }
-    cacc = macc + nmacc;
-    hipLaunchKernelGGL(k_zero_u64, dim3(ceil_div(nmacc + rows, 256)), dim3(256), 0, ctx->stream,
-                       macc, nmacc + rows);
+    cacc = macc + nmacc;  // then the rows' Gram sums (rows x 256)
+    const size_t nzero = nmacc + rows + (L > (size_t)SC_FINAL_MAX ? rows * 256 : 0);
+    hipLaunchKernelGGL(k_zero_u64, dim3(ceil_div(nzero, 256)), dim3(256), 0, ctx->stream, macc,
+                       nzero);
     SALSAA_HIP(hipGetLastError());
     if (L <= (size_t)SC_FINAL_MAX) {
         hipLaunchKernelGGL((k_sc_final<FQ2, true, SNR>), dim3((unsigned)rows, (unsigned)reps),
@@ -590,22 +718,30 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         SALSAA_TRY(scratch_get(ctx, 2, need_b, (void **)&bufB));
     }
     {
+        // one streaming pass: the folds after round 3 and the Gram matrix,
+        // then rounds 0..3's messages from it (at most 8 columns of 256 per
+        // block keeps every wave's int32 digit sums exact)
         const size_t S = len1;
-        const int cpt = cpt_first > 0 ? cpt_first : (S >= (size_t)SC_THREADS * 64 ? 4 : 1);
-        const unsigned bx = ceil_div(S, (size_t)SC_THREADS * cpt);
-        for (size_t r0 = 0; r0 < reps; r0 += 2) {
-            const int nrep = (reps - r0) >= 2 ? 2 : 1;
-            dim3 grid(bx, (unsigned)rows);
-            if (nrep == 2)
-                hipLaunchKernelGGL((k_sc_first<2, FQ2, SNR>), grid, dim3(SC_THREADS), 0, ctx->stream, w,
-                                   wrow, L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, cs_row, n, nr, cpt,
-                                   bufA, macc, cacc, M);
+        const int cpt = cpt_first > 0 ? std::min(cpt_first, 8)
+                                      : (S >= (size_t)GR_X * 64 ? 4 : 1);
+        const unsigned bx = ceil_div(S, (size_t)GR_X * cpt);
+        unsigned long long *gacc = cacc + rows;
+        dim3 grid(bx, (unsigned)rows);
+        for (size_t r0 = 0; r0 < reps; r0 += GR_MAXREP) {
+            const int nrep = (int)std::min<size_t>(GR_MAXREP, reps - r0);
+            if (r0 == 0)
+                hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0, ctx->stream,
+                                   w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr, cpt, bufA,
+                                   gacc, ctx->d_tables, M);
             else
-                hipLaunchKernelGGL((k_sc_first<1, FQ2, SNR>), grid, dim3(SC_THREADS), 0, ctx->stream, w,
-                                   wrow, L, (int)rows, (int)r0, (int)reps, r0 == 0, cs, cs_row, n, nr, cpt,
-                                   bufA, macc, cacc, M);
+                hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0, ctx->stream,
+                                   w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr, cpt, bufA,
+                                   gacc, ctx->d_tables, M);
             SALSAA_HIP(hipGetLastError());
         }
+        hipLaunchKernelGGL((k_sc_gram_msgs<FQ2, SNR>), dim3((unsigned)rows, (unsigned)reps), dim3(128),
+                           0, ctx->stream, gacc, (int)rows, cs, cs_row, n, nr, macc, cacc, M);
+        SALSAA_HIP(hipGetLastError());
     }
     size_t len = len1;
     int t0 = 4;
@@ -637,7 +773,8 @@ void nc_scratch_need(size_t rows, size_t L, size_t reps, size_t *macc_bytes, siz
                      size_t *bufB_bytes) {
     const size_t n = (L >= 2) ? (size_t)__builtin_ctzll(L) : 0;
     const size_t len1 = L >> 4;
-    *macc_bytes = (reps * n * rows * 6 + rows) * sizeof(unsigned long long);
+    *macc_bytes = (reps * n * rows * 6 + rows + (L > (size_t)SC_FINAL_MAX ? rows * 256 : 0)) *
+                  sizeof(unsigned long long);
     *bufA_bytes = L > (size_t)SC_FINAL_MAX ? reps * rows * len1 * 2 * sizeof(double) : 0;
     *bufB_bytes =
         L > (size_t)SC_FINAL_MAX ? reps * rows * (len1 >> 4 ? len1 >> 4 : 1) * 2 * sizeof(double) : 0;
