@@ -214,7 +214,7 @@ __device__ __forceinline__ unsigned logical_block() {
 // One commit: W planes with rtB = ceil(7 r / 16) tiles per K step; block
 // (nb, s, z) owns witness rows 16 nb .. 16 nb + 15 and K chunk z.
 template <int TM, int TN, int WN>
-__global__ __launch_bounds__(256 * WN) void k_commit_mfma(
+__global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) void k_commit_mfma(
     const int8_t *AB, int kappa, int a_kt, int a_ktiles, const int8_t *WB, int r, int ktW,
     int kchunk, int ktend, int nbx, double *__restrict__ partial, const double *__restrict__ tab,
     Mod M) {
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256 * WN) void k_commit_mfma(
 // rows with its own planes and K extent; table[b] = (job, z * 64 + s) in
 // logical block order.
 template <int TM, int TN, int WN>
-__global__ __launch_bounds__(256 * WN) void k_commit_mfma_batch(
+__global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) void k_commit_mfma_batch(
     const int8_t *AB, int kappa, int a_kt, int a_ktiles, const CommitJob *__restrict__ jobs,
     const uint2 *__restrict__ table, int kchunk, const double *__restrict__ tab, Mod M) {
     __shared__ __align__(16) char lds[CommitShape<TM, TN, WN>::LDS_BYTES];
