@@ -167,6 +167,27 @@ def test_norm_check_vs_oracle(ctx, orc, fq2, rows, L, reps):
             assert not rhs[rep, :, 1].any()
 
 
+@pytest.mark.parametrize("reps,nr", [(6, NR), (2, 20)])
+def test_norm_check_reps_and_nonresidue(ctx, orc, reps, nr):
+    # more repetitions than one launch of the first pass folds (the rest run
+    # fold-only), and a non-residue of 8 or more (the general F_{q^2} product)
+    assert pow(nr, (Q - 1) // 2, Q) == Q - 1
+    rows, L = 2, 1 << 13
+    say("case: norm_check reps=%d nr=%d rows=%d L=%d" % (reps, nr, rows, L))
+    n = L.bit_length() - 1
+    w = rand_ring(orc, (rows, L), 21, 0xB)
+    cs = ctx.challenges(reps, n, 1, 78)
+    outs = ctx.norm_check(ctx.to_device(w), cs, 1, nr=nr)
+    want = orc.sumcheck(w, cs.download(), Q, fq2=True, nr=nr, threads=8)
+    for k in ("claims", "msgs", "finals", "evals"):
+        assert np.array_equal(outs[k].download(), want[k]), k
+    assert outs["ok"].download().all() and want["ok"].all()
+    rhs, csd = outs["rhs"].download(), cs.download()
+    W3 = w.reshape(rows, L // 64, 64)
+    for rep in range(reps):
+        assert np.array_equal(rhs[rep], orc.norm_check_3_fq2(W3, csd[rep], Q, nr))
+
+
 def test_norm_check_small_lengths(ctx, orc):
     for L in (2, 4, 32):
         say("case: norm_check L=%d (no rhs)" % L)
