@@ -15,7 +15,10 @@
 //     the blocks that share that key chunk run side by side on one XCD.  A
 //     plain per-instance commit reads its key prefix once per instance.
 //   * Norm check: instances of equal length are consecutive, so each length
-//     class is one multi-row norm check with per-row challenges.
+//     class is one multi-row norm check with per-row challenges.  The classes
+//     run on kNcStreams streams of their own (forked from and joined back into
+//     the context's stream with events, which the hipGraph capture follows),
+//     so the short launches at the end of one class overlap the others.
 // The commit's K chunk and the norm check's first-pass width are chosen by
 // timing candidates on the device when the plan is made (the autotuner).
 #include <algorithm>
@@ -27,6 +30,8 @@
 
 namespace salsaa {
 namespace {
+
+constexpr int kNcStreams = 3;
 
 __global__ void k_batch_zero(uint64_t *p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -104,7 +109,10 @@ struct salsaa_batch {
         size_t first, rows, L;
     };
     std::vector<Group> groups;
-    NcScratch nc;
+    std::vector<int> group_stream;       // which norm-check stream runs each class
+    Context nc_ctx[kNcStreams];          // the context's tables on streams of their own
+    NcScratch nc[kNcStreams];
+    hipEvent_t ev_fork = nullptr, ev_done[kNcStreams] = {};
     int cpt_first = 0;
     float t_commit = 0.f, t_nc = 0.f;
     // plain path
@@ -201,10 +209,17 @@ int run_commit(salsaa_batch *b) {
     return launch_commit_finalize_batch(&b->ctx->c, b->d_fjobs, b->njobs, b->fnblk, b->krows);
 }
 
-int run_norm_checks(salsaa_batch *b, int cpt) {
+// Enqueues the norm checks on their streams after the work already on the
+// context's stream; norm_checks_join makes the context's stream wait for them.
+int norm_checks_fork(salsaa_batch *b, int cpt) {
     Context *ctx = &b->ctx->c;
     const Outputs &o = b->out[0];
-    for (const auto &g : b->groups) {
+    SALSAA_HIP(hipEventRecord(b->ev_fork, ctx->stream));
+    for (int k = 0; k < kNcStreams; k++)
+        SALSAA_HIP(hipStreamWaitEvent(b->nc_ctx[k].stream, b->ev_fork, 0));
+    for (size_t gi = 0; gi < b->groups.size(); gi++) {
+        const auto &g = b->groups[gi];
+        const int k = b->group_stream[gi];
         const size_t p = g.first, nr = b->nrounds[p];
         NcOut no;
         no.claims = o.claims + p;
@@ -223,11 +238,24 @@ int run_norm_checks(salsaa_batch *b, int cpt) {
         no.ok = o.ok + p * b->reps;
         no.ok_row = b->reps;
         no.ok_rep = 1;
-        SALSAA_TRY(launch_norm_check_ex(ctx, b->d_wit, b->d_wrow + p, g.rows, g.L,
+        SALSAA_TRY(launch_norm_check_ex(&b->nc_ctx[k], b->d_wit, b->d_wrow + p, g.rows, g.L,
                                         b->d_cs + b->off_c[p], b->reps * nr * 2, b->reps, b->fq2,
-                                        b->nr, no, cpt, &b->nc));
+                                        b->nr, no, cpt, &b->nc[k]));
     }
+    for (int k = 0; k < kNcStreams; k++)
+        SALSAA_HIP(hipEventRecord(b->ev_done[k], b->nc_ctx[k].stream));
     return SALSAA_OK;
+}
+
+int norm_checks_join(salsaa_batch *b) {
+    for (int k = 0; k < kNcStreams; k++)
+        SALSAA_HIP(hipStreamWaitEvent(b->ctx->c.stream, b->ev_done[k], 0));
+    return SALSAA_OK;
+}
+
+int run_norm_checks(salsaa_batch *b, int cpt) {
+    SALSAA_TRY(norm_checks_fork(b, cpt));
+    return norm_checks_join(b);
 }
 
 int run_fwd_planes(salsaa_batch *b, int roundtrip) {
@@ -236,11 +264,13 @@ int run_fwd_planes(salsaa_batch *b, int roundtrip) {
 }
 
 // one pass: forward NTT, the commit's digit planes and the inverse NTT; then
-// the commit and the norm checks
+// the commit on the context's stream while the norm checks (which need only
+// the Coefficient witnesses) run beside it on theirs
 int run_step(salsaa_batch *b) {
     SALSAA_TRY(run_fwd_planes(b, 1));
+    SALSAA_TRY(norm_checks_fork(b, b->cpt_first));
     SALSAA_TRY(run_commit(b));
-    return run_norm_checks(b, b->cpt_first);
+    return norm_checks_join(b);
 }
 
 // median-free device timing of fn over `reps` runs (minimum), in ms
@@ -320,9 +350,14 @@ void destroy(salsaa_batch *b) {
     (void)hipFree(b->d_cjobs);
     (void)hipFree(b->d_table);
     (void)hipFree(b->d_fjobs);
-    (void)hipFree(b->nc.macc);
-    (void)hipFree(b->nc.bufA);
-    (void)hipFree(b->nc.bufB);
+    for (int k = 0; k < kNcStreams; k++) {
+        (void)hipFree(b->nc[k].macc);
+        (void)hipFree(b->nc[k].bufA);
+        (void)hipFree(b->nc[k].bufB);
+        if (b->nc_ctx[k].stream) (void)hipStreamDestroy(b->nc_ctx[k].stream);
+        if (b->ev_done[k]) (void)hipEventDestroy(b->ev_done[k]);
+    }
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     delete b;
 }
 
@@ -453,25 +488,39 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
     SALSAA_TRY(dmalloc(&b->d_cjobs, b->njobs));
     SALSAA_TRY(dmalloc(&b->d_fjobs, b->njobs));
     // norm check classes of equal length
-    size_t need_m = 0, need_a = 0, need_b = 0;
+    // classes to streams, largest first onto the least loaded stream; each
+    // stream's scratch fits the largest class it runs
+    size_t need_m[kNcStreams] = {}, need_a[kNcStreams] = {}, need_b[kNcStreams] = {};
+    double load[kNcStreams] = {};
     for (size_t p = 0; p < n;) {
         size_t q = p;
         while (q < n && hm[q] == hm[p]) q++;
         salsaa_batch::Group g{p, q - p, hm[p] * kD};
+        int k = 0;
+        for (int j = 1; j < kNcStreams; j++)
+            if (load[j] < load[k]) k = j;
+        load[k] += (double)g.rows * (double)g.L + 1e6;  // a fixed cost per class
         size_t a, bb, mm;
         nc_scratch_need(g.rows, g.L, reps, &mm, &a, &bb);
-        need_m = std::max(need_m, mm);
-        need_a = std::max(need_a, a);
-        need_b = std::max(need_b, bb);
+        need_m[k] = std::max(need_m[k], mm);
+        need_a[k] = std::max(need_a[k], a);
+        need_b[k] = std::max(need_b[k], bb);
         b->groups.push_back(g);
+        b->group_stream.push_back(k);
         p = q;
     }
-    SALSAA_TRY(dmalloc(&b->nc.macc, need_m / sizeof(unsigned long long) + 1));
-    SALSAA_TRY(dmalloc(&b->nc.bufA, need_a / sizeof(double) + 1));
-    SALSAA_TRY(dmalloc(&b->nc.bufB, need_b / sizeof(double) + 1));
-    b->nc.macc_bytes = need_m;
-    b->nc.bufA_bytes = need_a;
-    b->nc.bufB_bytes = need_b;
+    SALSAA_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    for (int k = 0; k < kNcStreams; k++) {
+        share_context(*c, b->nc_ctx[k]);
+        SALSAA_HIP(hipStreamCreateWithFlags(&b->nc_ctx[k].stream, hipStreamNonBlocking));
+        SALSAA_HIP(hipEventCreateWithFlags(&b->ev_done[k], hipEventDisableTiming));
+        SALSAA_TRY(dmalloc(&b->nc[k].macc, need_m[k] / sizeof(unsigned long long) + 1));
+        SALSAA_TRY(dmalloc(&b->nc[k].bufA, need_a[k] / sizeof(double) + 1));
+        SALSAA_TRY(dmalloc(&b->nc[k].bufB, need_b[k] / sizeof(double) + 1));
+        b->nc[k].macc_bytes = need_m[k];
+        b->nc[k].bufA_bytes = need_a[k];
+        b->nc[k].bufB_bytes = need_b[k];
+    }
     // per-instance key views for the plain per-call path
     for (size_t i = 0; i < n; i++) {
         salsaa_key *v = nullptr;

@@ -86,6 +86,19 @@ struct Context {
     std::mutex lock;                // one host thread drives a context at a time
 };
 
+// A context on another stream over the same device tables (its scratch and
+// stream are its own).
+inline void share_context(const Context &from, Context &to) {
+    to.device = from.device;
+    to.ncu = from.ncu;
+    to.q = from.q;
+    to.d = from.d;
+    to.qd = from.qd;
+    to.qinv = from.qinv;
+    to.d_tables = from.d_tables;
+    to.mfma_commit = from.mfma_commit;
+}
+
 struct Key {
     Context *ctx = nullptr;
     size_t kappa = 0, m = 0;  // rows and length this key commits with

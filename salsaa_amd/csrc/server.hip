@@ -493,16 +493,6 @@ void destroy(salsaa_server *s) {
     delete s;
 }
 
-void share_context(const Context &from, Context &to) {
-    to.device = from.device;
-    to.ncu = from.ncu;
-    to.q = from.q;
-    to.d = from.d;
-    to.qd = from.qd;
-    to.qinv = from.qinv;
-    to.d_tables = from.d_tables;
-    to.mfma_commit = from.mfma_commit;
-}
 
 template <typename T>
 int dalloc(T **p, size_t count) {
