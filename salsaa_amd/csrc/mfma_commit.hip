@@ -15,8 +15,12 @@
 // 64 columns one MFMA reads are one contiguous row-major KiB, so a wave moves
 // a whole fragment with one 16-byte load per lane, and the row tiles of one K
 // step are consecutive, so a block's loads form one sequential stream per
-// operand.  In LDS a fragment is kept in MFMA lane order (lane 16 g + r holds
-// row r, columns 16 g .. 16 g + 15), rotated as lds_slot() describes.  A block owns one
+// operand.  In LDS a fragment is kept in MFMA lane order: slot 16 g + r holds
+// row r, columns 16 g .. 16 g + 15, and the staging lane that writes slot l
+// is the one that loaded it (the row/column transposition happens in the
+// global load's lane-to-address map, within the fragment's 1 KiB), so both
+// the staging writes and the operand reads are lane-contiguous, the pattern
+// gfx950's LDS serves without bank conflicts.  A block owns one
 // slot, one K chunk, all of the key's digit rows (at most 16 row tiles) and
 // 16 witness rows (7 column tiles).  Per K step its threads stage the step's
 // fragments from HBM through registers into LDS two steps ahead (double
@@ -36,11 +40,8 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int FRAG = 1024;  // bytes of one 16 x 64 int8 fragment
 constexpr int GT = 7;       // tiles per recombination group (16 rows of values)
 
-// 16-byte slot in LDS of (row r, column group g) of a fragment: lane order
-// 16 g + r with the rows of group g rotated by 4 g, so the staging stores of
-// 16 consecutive memory chunks (4 rows x 4 groups) and the operand reads of
-// 16 consecutive lanes (16 rows of one group) each hit 16 distinct bank groups.
-__device__ __forceinline__ int lds_slot(int r, int g) { return 16 * g + ((r + 4 * g) & 15); }
+// Memory chunk (16 bytes, row-major 4 r + g) that fills LDS slot l = 16 g + r.
+__device__ __forceinline__ int slot_chunk(int l) { return 4 * (l & 15) + (l >> 4); }
 
 // TM: row tiles per wave (wave wm owns key tiles wm + 4a, a < TM);
 // WN x TN: witness tiles (wave wn owns tiles wn + WN b of the block's 7).
@@ -73,8 +74,8 @@ __device__ __forceinline__ void commit_body(char *lds, const int8_t *AB, size_t 
     const int rtA = (7 * kappa + 15) / 16;
     const int nchunk = (rtA + nbv) * 64;
 
-    // staging: chunk c = tid + NTHR i is 16 bytes of fragment c / 64 (key
-    // tiles first, then the block's witness tiles); one wave moves one KiB.
+    // staging: lane c = tid + NTHR i moves LDS slot c % 64 of fragment c / 64
+    // (key tiles first, then the block's witness tiles); one wave moves one KiB.
     // Chunks past the block's tiles repeat the same-lane chunk of key tile 0
     // (the same bytes to the same LDS address), so loads and stores carry no
     // branches and the wait counts can keep two steps in flight.
@@ -84,8 +85,8 @@ __device__ __forceinline__ void commit_body(char *lds, const int8_t *AB, size_t 
     for (int i = 0; i < S::CPT; i++) {
         const int c0 = tid + S::NTHR * i;
         const int c = c0 < nchunk ? c0 : (c0 & 63);
-        const int t = c >> 6, off = (c & 63) * 16;
-        const int lslot = lds_slot((c & 63) >> 2, c & 3) * 16;  // memory chunk 4 r + g
+        const int t = c >> 6, off = slot_chunk(c & 63) * 16;
+        const int lslot = (c & 63) * 16;
         if (t < rtA) {
             gp[i] = AB + s * a_slot + ((size_t)ktb * a_kt + t) * FRAG + off;
             gs[i] = a_kt * FRAG;
@@ -123,7 +124,7 @@ __device__ __forceinline__ void commit_body(char *lds, const int8_t *AB, size_t 
         for (int b = 0; b < TN; b++) acc[a][b] = (v4i){0, 0, 0, 0};
     // fragment slots past the block's tiles hold stale bytes; they only feed
     // rows or columns of the product that are never recombined
-    const int lane_slot = lds_slot(lane & 15, lane >> 4) * 16;
+    const int lane_slot = lane * 16;
     auto compute = [&](int buf) {
         const char *base = lds + buf * S::STAGE + lane_slot;
         v4i fb[TN];
