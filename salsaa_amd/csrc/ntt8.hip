@@ -305,15 +305,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 // so with zj = zj_s they are E[zj] +- zeta_s O[zj], and thread (e, h) forms
 // slots 8 h .. 8 h + 7 of element e.  The next tile then moves into the first
 // buffer while the planes are written (store_planes) where ptiles[tile] says.
-// RT (the round trip): the tile's inverse IncompleteNTT follows in the same
-// pass, computed from the forward outputs (each thread starts the inverse
-// from the values its forward stage left in registers), and the Coefficient
-// tile is stored instead of the IncompleteNTT one, which then never goes to
-// HBM.
 // Tiles of TEL ring elements (8 threads each).  TEL = 64: 69 KiB of LDS, two
 // blocks of 512 threads per CU; TEL = 128: 138 KiB, one block of 1024.
 // Either way four waves per SIMD, so up to 128 registers.
-template <int TEL, bool RT>
+template <int TEL>
 __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ntt_fwd_planes(
     uint64_t *__restrict__ data, size_t ntiles, const double *__restrict__ tab, Mod M,
     int8_t *__restrict__ planes, const PlaneTile *__restrict__ ptiles) {
@@ -321,7 +316,6 @@ __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4)))
     __shared__ uint64_t lds[TEL * SROW];
     __shared__ uint64_t work[64 * TEL];
     __shared__ double twl[32];
-    __shared__ double twi[RT ? 33 : 1];
     __shared__ double zt[32];
     __shared__ int zo[32];
     size_t tile = blockIdx.x;
@@ -329,8 +323,6 @@ __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4)))
     const int tid = threadIdx.x, e = tid >> 3, h = tid & 7, p = h >> 2, t = h & 3;
     // balanced twiddles for the lazy butterflies
     if (tid < 32) twl[tid] = balance(tab[TAB_FWD32 + tid], M);
-    if (RT && tid >= 128 && tid < 160) twi[tid - 128] = balance(tab[TAB_INV32 + tid - 128], M);
-    if (RT && tid == 160) twi[32] = balance(tab[TAB_NINV32], M);
     if (tid >= 64 && tid < 96) {
         zt[tid - 64] = tab[TAB_ZETA + 2 * (tid - 64)];
         zo[tid - 64] = skew((int)tab[TAB_ZJ + 2 * (tid - 64)]);
@@ -369,8 +361,8 @@ __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4)))
 #pragma unroll
         for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = d2u(v[x]);
         __syncthreads();
-        ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(data + tile * TEL * kD);
-        if (!RT) {
+        {
+            ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(data + tile * TEL * kD);
 #pragma unroll
             for (int i = 0; i < 4; i++) dst[tid + TNT * i] = tile_get1<true>(lds, tid + TNT * i);
         }
@@ -382,39 +374,16 @@ __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4)))
             work[wpos<TEL>(8 * h + 2 * u, e)] = dbits(addmod(E, z, M));
             work[wpos<TEL>(8 * h + 2 * u + 1, e)] = dbits(submod(E, z, M));
         }
-        if (RT) ntt32_inv_b_lazy(v, twi, t, M);  // v: this thread's forward outputs 32p + 8t + x
         __syncthreads();  // the tile rows are free and the slots complete
         const bool more = next < ntiles;
-        if (RT) {
-            store_planes_one<TEL>(planes + dsc.base, dsc.sstride, dsc.kstride, work, TEL, tid);
-#pragma unroll
-            for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
-            ntt32_inv_a_lazy(v, twi, twi[32], M);
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < 8; k++) row[8 * k + 2 * t + p] = d2u(v[k]);
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < 4; i++) dst[tid + TNT * i] = tile_get1<false>(lds, tid + TNT * i);
-            if (!more) break;
-            __syncthreads();  // the tile store has read the rows
+        if (more) {
             tile_put1<false>(lds, r0, tid);
             tile_put1<false>(lds, r1, tid + TNT);
             tile_put1<false>(lds, r2, tid + 2 * TNT);
             tile_put1<false>(lds, r3, tid + 3 * TNT);
-        } else {
-            if (more) {
-                tile_put1<false>(lds, r0, tid);
-                tile_put1<false>(lds, r1, tid + TNT);
-                tile_put1<false>(lds, r2, tid + 2 * TNT);
-                tile_put1<false>(lds, r3, tid + 3 * TNT);
-            }
-            store_planes_one<TEL>(planes + dsc.base, dsc.sstride, dsc.kstride, work, TEL, tid);
-            if (!more) break;
         }
+        store_planes_one<TEL>(planes + dsc.base, dsc.sstride, dsc.kstride, work, TEL, tid);
+        if (!more) break;
         __syncthreads();  // the next tile is in LDS; the slot buffer has been read
         tile = next;
         dsc = ndsc;
@@ -425,6 +394,130 @@ __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4)))
         r2 = tile_chunk(src, tid + 2 * TNT, TEL);
         r3 = tile_chunk(src, tid + 3 * TNT, TEL);
         ndsc = ptiles[next < ntiles ? next : 0];
+    }
+}
+
+// The batched step's NTT pass with the transforms per wave: wave w of a
+// block owns ring elements 8 w .. 8 w + 7 of the block's 128-element tile
+// (4 KiB contiguous) with its own LDS rows and its own register prefetch of
+// its next slice, so the transforms need only wave-level ordering and the 16
+// waves of a CU overlap them.  Per tile: forward IncompleteNTT; the 64
+// full-NTT slots of each element (slots s, s + 1 at +-zeta_s from E[zj],
+// O[zj]) into the block's slot staging (wpos layout); two block barriers
+// around it, after which every thread packs one slot x 8 columns into digit
+// planes (store_planes: a wave's stores cover 4 slots, so each touches few
+// pages); then, per wave again, the inverse IncompleteNTT from the forward
+// outputs still in registers and the Coefficient slice stored.
+static_assert(kPlaneTile == EL, "the round trip works on 128-element tiles");
+constexpr int WEL = 8;                // ring elements per wave
+constexpr int WROWS = WEL * SROW;     // u64 of a wave's rows
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ntt_rt_planes(
+    uint64_t *__restrict__ data, size_t ntiles, const double *__restrict__ tab, Mod M,
+    int8_t *__restrict__ planes, const PlaneTile *__restrict__ ptiles) {
+    __shared__ uint64_t rows_all[(NT / 64) * WROWS];
+    __shared__ uint64_t work[64 * EL];
+    __shared__ double twl[32], twi[33], zt[32];
+    __shared__ int zo[32];
+    size_t tile = blockIdx.x;
+    if (tile >= ntiles) return;  // uniform in the block
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int ew = lane >> 3, h = lane & 7, p = h >> 2, t = h & 3;
+    // balanced twiddles for the lazy butterflies; zeta_s and the skewed
+    // position of E[zj_s] for even s
+    if (tid < 32) twl[tid] = balance(tab[TAB_FWD32 + tid], M);
+    if (tid >= 64 && tid < 96) twi[tid - 64] = balance(tab[TAB_INV32 + tid - 64], M);
+    if (tid == 96) twi[32] = balance(tab[TAB_NINV32], M);
+    if (tid >= 128 && tid < 160) {
+        zt[tid - 128] = tab[TAB_ZETA + 2 * (tid - 128)];
+        zo[tid - 128] = skew((int)tab[TAB_ZJ + 2 * (tid - 128)]);
+    }
+    __syncthreads();
+    uint64_t *rows = rows_all + w * WROWS;
+    uint64_t *row = rows + ew * SROW;
+    const int e = WEL * w + ew;  // element of the block's tile
+    // chunk c (16 B) of the slice: element c / 32, entries 2 (c % 32) ..
+    auto slice = [&](size_t tl) { return data + (tl * EL + WEL * w) * kD; };
+    const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(slice(tile));
+    ulonglong2 r0 = src[lane], r1 = src[lane + 64], r2 = src[lane + 128], r3 = src[lane + 192];
+    PlaneTile dsc = ptiles[tile];
+    auto put = [&]() {
+        *reinterpret_cast<ulonglong2 *>(&rows[(lane >> 5) * SROW + (lane & 31) * 2]) = r0;
+        *reinterpret_cast<ulonglong2 *>(&rows[(2 + (lane >> 5)) * SROW + (lane & 31) * 2]) = r1;
+        *reinterpret_cast<ulonglong2 *>(&rows[(4 + (lane >> 5)) * SROW + (lane & 31) * 2]) = r2;
+        *reinterpret_cast<ulonglong2 *>(&rows[(6 + (lane >> 5)) * SROW + (lane & 31) * 2]) = r3;
+    };
+    put();
+    wave_sync();
+    // Loads and stores are unconditional (the last prefetch re-reads the
+    // current slice), so each wait covers only the loads it needs.
+    size_t next = tile + gridDim.x;
+    src = reinterpret_cast<const ulonglong2 *>(slice(next < ntiles ? next : tile));
+    r0 = src[lane];
+    r1 = src[lane + 64];
+    r2 = src[lane + 128];
+    r3 = src[lane + 192];
+    PlaneTile ndsc = ptiles[next < ntiles ? next : tile];
+    for (;;) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = u2d(row[8 * k + 2 * t + p]);  // coeff 2(t+4k)+p
+        ntt32_fwd_a_lazy(v, twl, M);
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < 8; k++) row[36 * p + skew(t + 4 * k)] = dbits(v[k]);
+        wave_sync();
+#pragma unroll
+        for (int x = 0; x < 8; x++) v[x] = bitsd(row[36 * p + skew(8 * t + x)]);
+        ntt32_fwd_b_lazy(v, twl, t, M);
+#pragma unroll
+        for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = d2u(v[x]);
+        wave_sync();
+        // slots 8 h .. 8 h + 7 of element e
+        double a[8];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int o = zo[4 * h + u];
+            const double E = u2d(row[o]), O = u2d(row[36 + o]);
+            const double z = mulmod(zt[4 * h + u], O, M);
+            a[2 * u] = addmod(E, z, M);
+            a[2 * u + 1] = submod(E, z, M);
+        }
+        ntt32_inv_b_lazy(v, twi, t, M);  // v: this lane's forward outputs 32p + 8t + x
+        __syncthreads();  // the previous tile's planes have been read out of work
+#pragma unroll
+        for (int x = 0; x < 8; x++) work[wpos(8 * h + x, e)] = dbits(a[x]);
+        __syncthreads();
+        store_planes_one<EL>(planes + dsc.base, dsc.sstride, dsc.kstride, work, EL, tid);
+#pragma unroll
+        for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
+        ntt32_inv_a_lazy(v, twi, twi[32], M);
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < 8; k++) row[8 * k + 2 * t + p] = d2u(v[k]);
+        wave_sync();
+        {
+            ulonglong2 *dst = reinterpret_cast<ulonglong2 *>(slice(tile));
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                dst[lane + 64 * i] =
+                    *reinterpret_cast<const ulonglong2 *>(&rows[(2 * i + (lane >> 5)) * SROW + (lane & 31) * 2]);
+        }
+        if (next >= ntiles) break;
+        wave_sync();  // the store has read the rows
+        put();
+        wave_sync();
+        tile = next;
+        dsc = ndsc;
+        next = tile + gridDim.x;
+        src = reinterpret_cast<const ulonglong2 *>(slice(next < ntiles ? next : tile));
+        r0 = src[lane];
+        r1 = src[lane + 64];
+        r2 = src[lane + 128];
+        r3 = src[lane + 192];
+        ndsc = ptiles[next < ntiles ? next : tile];
     }
 }
 
@@ -700,35 +793,30 @@ int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse
     return SALSAA_OK;
 }
 
-template <bool RT>
-int fwd_planes_occupancy() {
-    static int occ = 0;
-    if (occ == 0) {
-        int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &n, reinterpret_cast<const void *>(&k_ntt_fwd_planes<kPlaneTile, RT>),
-                8 * kPlaneTile, 0) != hipSuccess || n < 1)
-            n = 1;
-        occ = n;
-    }
-    return occ;
-}
-
 int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *planes,
                           const PlaneTile *tiles, int roundtrip) {
     if (count == 0) return SALSAA_OK;
     SALSAA_CHECK(count % kPlaneTile == 0, "fused planes need whole tiles");
     Mod M{ctx->qd, ctx->qinv};
     const size_t nt = count / kPlaneTile;
-    if (roundtrip) {
-        const unsigned g = (unsigned)std::min<size_t>(nt, (size_t)ctx->ncu * fwd_planes_occupancy<true>());
-        hipLaunchKernelGGL((k_ntt_fwd_planes<kPlaneTile, true>), dim3(g), dim3(8 * kPlaneTile), 0,
-                           ctx->stream, data, nt, ctx->d_tables, M, planes, tiles);
-    } else {
-        const unsigned g = (unsigned)std::min<size_t>(nt, (size_t)ctx->ncu * fwd_planes_occupancy<false>());
-        hipLaunchKernelGGL((k_ntt_fwd_planes<kPlaneTile, false>), dim3(g), dim3(8 * kPlaneTile), 0,
-                           ctx->stream, data, nt, ctx->d_tables, M, planes, tiles);
+    static int occ[2] = {0, 0};
+    const void *fn = roundtrip ? reinterpret_cast<const void *>(&k_ntt_rt_planes)
+                               : reinterpret_cast<const void *>(&k_ntt_fwd_planes<kPlaneTile>);
+    const int threads = roundtrip ? NT : 8 * kPlaneTile;
+    int &o = occ[roundtrip ? 1 : 0];
+    if (o == 0) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, 0) != hipSuccess || n < 1)
+            n = 1;
+        o = n;
     }
+    const unsigned g = (unsigned)std::min<size_t>(nt, (size_t)ctx->ncu * o);
+    if (roundtrip)
+        hipLaunchKernelGGL(k_ntt_rt_planes, dim3(g), dim3(NT), 0, ctx->stream, data, nt,
+                           ctx->d_tables, M, planes, tiles);
+    else
+        hipLaunchKernelGGL((k_ntt_fwd_planes<kPlaneTile>), dim3(g), dim3(8 * kPlaneTile), 0,
+                           ctx->stream, data, nt, ctx->d_tables, M, planes, tiles);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }
