@@ -216,14 +216,6 @@ constexpr int GR_LD = GR_X + 2;   // LDS row stride (u64): fragment reads hit di
 constexpr int GR_MAXREP = 4;      // repetitions folded per launch
 constexpr uint64_t GR_BIAS = 0x0080808080808080ULL;
 
-// orders one wave's LDS writes before its later reads (and reads before
-// later writes) without waiting for the rest of the block
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // signed digit bytes i of four residues (y_q = (x_q + bias) ^ 0x80..80)
 __device__ __forceinline__ int gr_pack(const uint32_t (&lo)[4], const uint32_t (&hi)[4], int i) {
     const uint32_t *src = i < 4 ? lo : hi;
@@ -260,9 +252,11 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
 #pragma unroll
     for (int k = 0; k < 13; k++) acc[k] = (v4i){0, 0, 0, 0};
     const uint64_t *wr = w + (wrow ? wrow[row] : (size_t)row * L);
-    // Column x = xb + tid; the next iteration's 16 loads are issued before
-    // this one is folded (unconditionally: a dead column re-reads column
-    // S - 1 and counts as zero).
+    // Column x = xb + tid.  Each iteration moves its column into the wave's
+    // columns of T and reuses the same registers for the next iteration's 16
+    // loads (unconditional: a dead column re-reads column S - 1 and counts as
+    // zero), which stay in flight while the fold and the matrix products
+    // read this one back from T.
     size_t xb = (size_t)blockIdx.x * cpt * GR_X;
     if (xb >= S) return;  // whole block past the end (uniform)
     double v[16];
@@ -275,12 +269,14 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
         const size_t x = xb + tid;
         const bool live = x < S;
         const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform
-        double nv[16];
+#pragma unroll
+        for (int y = 0; y < 16; y++) T[y * GR_LD + tid] = live ? d2u(v[y]) : 0;
         {
             const size_t xc = min(more ? x + GR_X : x, S - 1);
 #pragma unroll
-            for (int y = 0; y < 16; y++) nv[y] = u2d(wr[(size_t)y * S + xc]);
+            for (int y = 0; y < 16; y++) v[y] = u2d(wr[(size_t)y * S + xc]);
         }
+        wave_sync();
         if (live) {
             // products |coef v| <= 0.69 q (balanced coef, canonical v): eight
             // of them stay below 5.6 q, then two lazy reductions
@@ -289,11 +285,12 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
 #pragma unroll
                 for (int y = 0; y < 8; y++) {
                     const F2 c0 = coef[r][y], c1 = coef[r][y + 8];
-                    a0 += mul_lazy(c0.a, v[y], M);
-                    a1 += mul_lazy(c1.a, v[y + 8], M);
+                    const double v0 = u2d(T[y * GR_LD + tid]), v1 = u2d(T[(y + 8) * GR_LD + tid]);
+                    a0 += mul_lazy(c0.a, v0, M);
+                    a1 += mul_lazy(c1.a, v1, M);
                     if (FQ2) {
-                        b0 += mul_lazy(c0.b, v[y], M);
-                        b1 += mul_lazy(c1.b, v[y + 8], M);
+                        b0 += mul_lazy(c0.b, v0, M);
+                        b1 += mul_lazy(c1.b, v1, M);
                     }
                 }
                 double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
@@ -305,9 +302,6 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
             // Each wave multiplies the 64 columns its own lanes hold (lane
             // 16 g + r takes digit row (i, r), columns 16 g .. 16 g + 15), so
             // only the wave has to be in step with itself around T.
-#pragma unroll
-            for (int y = 0; y < 16; y++) T[y * GR_LD + tid] = live ? d2u(v[y]) : 0;
-            wave_sync();
             const int r = lane & 15, g = lane >> 4;
             const uint64_t *src = T + r * GR_LD + 64 * wave + 16 * g;
             v4i f[7];
@@ -329,11 +323,11 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
 #pragma unroll
                 for (int j = 0; j < 7; j++)
                     acc[i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f[i], f[j], acc[i + j], 0, 0, 0);
+        } else {
+            wave_sync();  // T is rewritten by the next iteration
         }
         if (!more) break;
         xb += GR_X;
-#pragma unroll
-        for (int y = 0; y < 16; y++) v[y] = nv[y];
     }
     if (GRAM) {
         // lane (r, g) holds E_k[4 g + e][r]; G entry = sum_k 2^(8k) E_k mod q
