@@ -114,3 +114,49 @@ def test_batch_rejects_bad_plans(sa, ctx, orc):
     with pytest.raises(sa.SalsaaError):
         sa.Batch(ctx, key, [64], [9], tune=False)     # more rows than the key
     key.close()
+
+
+def test_batch_unfused_phases_agree(sa, ctx, orc):
+    # The unfused alternatives of the step's NTT pass (forward alone, forward
+    # with planes, inverse, planes from the Coefficient witnesses), run through
+    # the phase API, give the fused pass's commitments and restore the witnesses.
+    say("case: batch unfused phases")
+    rng = np.random.default_rng(11)
+    n = 21
+    m = [int(1 << rng.integers(0, 10)) for _ in range(n)]
+    kappa = [int(rng.choice([8, 16, 32])) for _ in range(n)]
+    A_h = orc.sample_uniform(4, sa.STREAM_A, (32, 512, 64), Q)
+    key = ctx.key(ctx.to_device(A_h))
+    b = sa.Batch(ctx, key, m, kappa, reps=1, fq2=1, tune=False)
+    fill(sa, ctx, b, 300)
+    W0 = [b.slot(i)["witness"].download() for i in range(n)]
+
+    def commitments():
+        return [b.slot(i)["commitment"].download()[: kappa[i]] for i in range(n)]
+
+    def witnesses_restored():
+        for i in range(n):
+            assert np.array_equal(b.slot(i)["witness"].download(), W0[i]), i
+
+    b.step(graph=False)
+    want = commitments()
+    for seq in (("ntt_fwd_planes", "ntt_inv", "commit"),
+                ("ntt_fwd", "ntt_inv", "planes", "commit"),
+                ("planes", "commit")):
+        say("  phases", seq)
+        for i in range(n):  # clear the commitments between runs
+            c = b.slot(i)["commitment"]
+            c.upload(np.zeros(c.shape, np.uint64))
+        for name in seq:
+            b.phase(name)
+        ctx.sync()
+        witnesses_restored()
+        got = commitments()
+        for i in range(n):
+            assert np.array_equal(got[i], want[i]), (seq, i)
+    for i in (0, n - 1):
+        Wi = W0[i].reshape(1, m[i], 64)
+        assert np.array_equal(want[i], orc.commit(np.ascontiguousarray(A_h[: kappa[i], : m[i]]),
+                                                  Wi, Q, threads=8)), i
+    b.close()
+    key.close()
