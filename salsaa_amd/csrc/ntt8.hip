@@ -74,6 +74,17 @@ __device__ __forceinline__ int wpos(int s, int e) {
     return s * L + (e & ~31) + ((e + r) & 31);
 }
 
+// The same staging for the per-wave NTT pass (128-element rows), as an XOR
+// swizzle of each run of 32 elements.  gfx950 serves an 8-byte LDS access
+// conflict-free when each half of the wave hits 32 distinct u64 modulo 32
+// (measured); this layout does that for both of the pass's patterns: slot
+// writes (8 elements of a wave x slots 8 h + x, h < 8) and the packers'
+// reads (2 slots x 16 runs of 8 elements per half wave).
+__device__ __forceinline__ int wpos_w(int s, int e) {
+    const int f = ((e >> 5) & 3) ^ (4 * ((s ^ (s >> 5)) & 1)) ^ (8 * ((s >> 3) & 3));
+    return s * EL + (e & ~31) + ((e & 31) ^ f);
+}
+
 // Balanced base-256 digits of a residue x < 2^50: x = sum_u d_u 256^u with
 // d_u in [-128, 127] (7 digits).  With y = x + sum_u 128 * 256^u the bytes
 // b_u of y give d_u = b_u - 128, whose int8 bit pattern is b_u XOR 0x80.
@@ -94,8 +105,9 @@ constexpr uint64_t DIGIT_BIAS = 0x0080808080808080ULL;
 // `base` is the fragment byte of (slot 0, K tile i0 / 64, value row row), i.e.
 // dst + ((i0 / 64) RT) 1024 + 448 row; slots are sstride = KT RT 1024 bytes
 // apart and K tiles kstride = RT 1024.
-// Item c of L / 8 runs of 8 columns per slot (L = the tile's columns).
-template <int L = EL>
+// Item c of L / 8 runs of 8 columns per slot (L = the tile's columns); WL:
+// the staging is in the wpos_w layout.
+template <int L = EL, bool WL = false>
 __device__ __forceinline__ void store_planes_one(int8_t *__restrict__ base, size_t sstride,
                                                  size_t kstride, const uint64_t *work, int nel,
                                                  int c) {
@@ -110,7 +122,9 @@ __device__ __forceinline__ void store_planes_one(int8_t *__restrict__ base, size
             for (int q = 0; q < 4; q++) {
                 const int e = 8 * c8 + 4 * h + q;
                 uint64_t y = 0;  // zero padding: every digit 0
-                if (e < nel) y = (d2u(bitsd(work[wpos<L>(s, e)])) + DIGIT_BIAS) ^ 0x8080808080808080ULL;
+                if (e < nel)
+                    y = (d2u(bitsd(work[WL ? wpos_w(s, e) : wpos<L>(s, e)])) + DIGIT_BIAS) ^
+                        0x8080808080808080ULL;
                 lo[q] = (uint32_t)y;
                 hi[q] = (uint32_t)(y >> 32);
             }
@@ -485,9 +499,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         ntt32_inv_b_lazy(v, twi, t, M);  // v: this lane's forward outputs 32p + 8t + x
         __syncthreads();  // the previous tile's planes have been read out of work
 #pragma unroll
-        for (int x = 0; x < 8; x++) work[wpos(8 * h + x, e)] = dbits(a[x]);
+        for (int x = 0; x < 8; x++) work[wpos_w(8 * h + x, e)] = dbits(a[x]);
         __syncthreads();
-        store_planes_one<EL>(planes + dsc.base, dsc.sstride, dsc.kstride, work, EL, tid);
+        store_planes_one<EL, true>(planes + dsc.base, dsc.sstride, dsc.kstride, work, EL, tid);
 #pragma unroll
         for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
         wave_sync();
