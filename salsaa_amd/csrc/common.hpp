@@ -213,6 +213,14 @@ __device__ __forceinline__ double reduce(double x, Mod M) {
 __device__ __forceinline__ double reduce_lazy(double x, Mod M) {
     return __fma_rn(-__builtin_rint(x * M.qinv), M.q, x);
 }
+// orders one wave's LDS writes before its later reads (and reads before
+// later writes) without waiting for the rest of the block
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // canonical x in [0, q) -> balanced (-q/2, q/2]; balanced (|x| < q) -> canonical
 __device__ __forceinline__ double balance(double x, Mod M) { return x > 0.5 * M.q ? x - M.q : x; }
 __device__ __forceinline__ double canon(double x, Mod M) { return x < 0.0 ? x + M.q : x; }
