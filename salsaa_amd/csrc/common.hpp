@@ -221,6 +221,29 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Byte i (i < 7) of four 64-bit words y_0 .. y_3 given as lo/hi dwords:
+// d[i] = (byte i of y_0) | (byte i of y_1) << 8 | ... , a 4 x 4 byte
+// transpose per dword half in two v_perm_b32 stages (byte k of
+// perm(a, b, sel) is byte sel_k of the 8-byte value a:b, b the low half).
+__device__ __forceinline__ void bytes_transpose4(const uint32_t (&lo)[4], const uint32_t (&hi)[4],
+                                                 uint32_t (&d)[7]) {
+    const uint32_t a01 = __builtin_amdgcn_perm(lo[1], lo[0], 0x05010400u);
+    const uint32_t b01 = __builtin_amdgcn_perm(lo[1], lo[0], 0x07030602u);
+    const uint32_t a23 = __builtin_amdgcn_perm(lo[3], lo[2], 0x05010400u);
+    const uint32_t b23 = __builtin_amdgcn_perm(lo[3], lo[2], 0x07030602u);
+    d[0] = __builtin_amdgcn_perm(a23, a01, 0x05040100u);
+    d[1] = __builtin_amdgcn_perm(a23, a01, 0x07060302u);
+    d[2] = __builtin_amdgcn_perm(b23, b01, 0x05040100u);
+    d[3] = __builtin_amdgcn_perm(b23, b01, 0x07060302u);
+    const uint32_t c01 = __builtin_amdgcn_perm(hi[1], hi[0], 0x05010400u);
+    const uint32_t e01 = __builtin_amdgcn_perm(hi[1], hi[0], 0x07030602u);
+    const uint32_t c23 = __builtin_amdgcn_perm(hi[3], hi[2], 0x05010400u);
+    const uint32_t e23 = __builtin_amdgcn_perm(hi[3], hi[2], 0x07030602u);
+    d[4] = __builtin_amdgcn_perm(c23, c01, 0x05040100u);
+    d[5] = __builtin_amdgcn_perm(c23, c01, 0x07060302u);
+    d[6] = __builtin_amdgcn_perm(e23, e01, 0x05040100u);
+}
+
 // canonical x in [0, q) -> balanced (-q/2, q/2]; balanced (|x| < q) -> canonical
 __device__ __forceinline__ double balance(double x, Mod M) { return x > 0.5 * M.q ? x - M.q : x; }
 __device__ __forceinline__ double canon(double x, Mod M) { return x < 0.0 ? x + M.q : x; }

@@ -164,20 +164,16 @@ __global__ __launch_bounds__(FNT) void k_decomp_fold(
         }
         const size_t R = (size_t)inst * r_out + jo;
         int8_t *base = planes + (((size_t)s * ktiles + i0 / 64) * rt_n) * 1024 + R * 7 * 64 + (i0 % 64);
+        uint32_t pk[4][7];
 #pragma unroll
-        for (int u = 0; u < 7; u++) {
-            const uint32_t *src = u < 4 ? lo : hi;
-            const int sh = 8 * (u & 3);
-            uint32_t pk[4];
-#pragma unroll
-            for (int w4 = 0; w4 < 4; w4++) {
-                uint32_t x = 0;
-#pragma unroll
-                for (int b = 0; b < 4; b++) x |= ((src[4 * w4 + b] >> sh) & 0xFFu) << (8 * b);
-                pk[w4] = x;
-            }
-            *reinterpret_cast<uint4 *>(base + u * 64) = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+        for (int w4 = 0; w4 < 4; w4++) {
+            const uint32_t l4[4] = {lo[4 * w4], lo[4 * w4 + 1], lo[4 * w4 + 2], lo[4 * w4 + 3]};
+            const uint32_t h4[4] = {hi[4 * w4], hi[4 * w4 + 1], hi[4 * w4 + 2], hi[4 * w4 + 3]};
+            bytes_transpose4(l4, h4, pk[w4]);
         }
+#pragma unroll
+        for (int u = 0; u < 7; u++)
+            *reinterpret_cast<uint4 *>(base + u * 64) = make_uint4(pk[0][u], pk[1][u], pk[2][u], pk[3][u]);
     }
     __syncthreads();
     // (c) Coefficient form: inverse IncompleteNTT of the folded rows in place

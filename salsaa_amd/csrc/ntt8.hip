@@ -128,15 +128,10 @@ __device__ __forceinline__ void store_planes_one(int8_t *__restrict__ base, size
                 lo[q] = (uint32_t)y;
                 hi[q] = (uint32_t)(y >> 32);
             }
+            uint32_t dg[7];
+            bytes_transpose4(lo, hi, dg);
 #pragma unroll
-            for (int u = 0; u < 7; u++) {
-                const uint32_t *src = u < 4 ? lo : hi;
-                const int sh = 8 * (u & 3);
-                uint32_t x = 0;
-#pragma unroll
-                for (int q = 0; q < 4; q++) x |= ((src[q] >> sh) & 0xFFu) << (8 * q);
-                p[u][h] = x;
-            }
+            for (int u = 0; u < 7; u++) p[u][h] = dg[u];
         }
         // nontemporal: the planes are read back by the commit long after,
         // from HBM, so they need not take up cache on the way out

@@ -216,16 +216,6 @@ constexpr int GR_LD = GR_X + 2;   // LDS row stride (u64): fragment reads hit di
 constexpr int GR_MAXREP = 4;      // repetitions folded per launch
 constexpr uint64_t GR_BIAS = 0x0080808080808080ULL;
 
-// signed digit bytes i of four residues (y_q = (x_q + bias) ^ 0x80..80)
-__device__ __forceinline__ int gr_pack(const uint32_t (&lo)[4], const uint32_t (&hi)[4], int i) {
-    const uint32_t *src = i < 4 ? lo : hi;
-    const int sh = 8 * (i & 3);
-    uint32_t x = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) x |= ((src[q] >> sh) & 0xFFu) << (8 * q);
-    return (int)x;
-}
-
 template <bool FQ2, bool SNR, bool GRAM>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
     const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
@@ -314,8 +304,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
                     lo[q] = (uint32_t)yv;
                     hi[q] = (uint32_t)(yv >> 32);
                 }
+                uint32_t dg[7];
+                bytes_transpose4(lo, hi, dg);
 #pragma unroll
-                for (int i = 0; i < 7; i++) f[i][k4] = gr_pack(lo, hi, i);
+                for (int i = 0; i < 7; i++) f[i][k4] = (int)dg[i];
             }
             wave_sync();  // T is rewritten by the next iteration
 #pragma unroll
