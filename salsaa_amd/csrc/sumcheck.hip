@@ -210,11 +210,29 @@ __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) 
 // and each E_k accumulates exactly in int32 (|d| <= 128, at most 7 digit
 // pairs, at most 512 columns per wave).  k_sc_gram_msgs then evaluates the
 // forms for each (row, repetition).
+//
+// Short witnesses (the norm check's purpose) take a cheaper path per wave and
+// iteration: when every residue of the wave's 64 columns is b in balanced form
+// with |b| < 2^14, t = b + 2^14 < 2^15 goes through LDS as a u32.  The fold is
+// then exact in fp64 without reductions: with c = c_h 2^25 + c_l (|c_h|,
+// |c_l| <= 2^24), sum_y c_h t_y and sum_y c_l t_y stay below 2^44, and
+// sum_y c b_y = 2^25 sum c_h t + sum c_l t - 2^14 sum c.  The Gram matrix
+// needs only b's two balanced base-256 digits (E_0, E_1, E_2: 4 products).
 typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int GR_X = SC_THREADS;  // columns per iteration
 constexpr int GR_LD = GR_X + 2;   // LDS row stride (u64): fragment reads hit distinct banks
 constexpr int GR_MAXREP = 4;      // repetitions folded per launch
 constexpr uint64_t GR_BIAS = 0x0080808080808080ULL;
+constexpr uint32_t GR_SMALL = 1u << 14;  // short path: balanced |b| < 2^14, t = b + 2^14
+
+// 2^25 h + l + k for the short path's exact sums (|h|, |l| < 2^44, k balanced),
+// reduced to a balanced residue: 2^25 h is exact and below 2^69, its remainder
+// fma(-qt, q, .) is an exact integer below q
+__device__ __forceinline__ double gr_small_sum(double h, double l, double k, Mod M) {
+    const double hi = h * 33554432.0;
+    const double r = __fma_rn(-__builtin_rint(hi * M.qinv), M.q, hi);
+    return reduce_lazy(r + l + k, M);
+}
 
 template <bool FQ2, bool SNR, bool GRAM>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
@@ -222,8 +240,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
     int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
     double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc,
     const double *__restrict__ tab, Mod M) {
-    __shared__ uint64_t T[16 * GR_LD];
+    __shared__ __align__(16) uint64_t T[16 * GR_LD];
     __shared__ F2 coef[GR_MAXREP][16];
+    __shared__ __align__(16) double csplit[GR_MAXREP][16][4];  // (c_h, c_l) of both components
+    __shared__ double coff[GR_MAXREP][2];                     // -2^14 sum_y c, balanced
     const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t *cs = cs0 + row * cs_row;
     const size_t S = L >> 4;
@@ -235,7 +255,20 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
             const F2 c = load_chal<FQ2>(cs, rep0 + r, nrounds, t);
             a = x_mul<FQ2, SNR>(a, ((y >> (3 - t)) & 1) ? c : x_sub<FQ2>(F2{1.0, 0.0}, c, M), nr, M);
         }
-        coef[r][y] = {balance(a.a, M), balance(a.b, M)};
+        const double ca = balance(a.a, M), cb = balance(a.b, M);
+        coef[r][y] = {ca, cb};
+        const double ha = __builtin_rint(ca * 0x1p-25), hb = __builtin_rint(cb * 0x1p-25);
+        csplit[r][y][0] = ha;
+        csplit[r][y][1] = ca - ha * 0x1p25;
+        csplit[r][y][2] = hb;
+        csplit[r][y][3] = cb - hb * 0x1p25;
+    }
+    __syncthreads();
+    if (tid < 2 * nrep) {
+        const int r = tid >> 1, comp = tid & 1;
+        double sum = 0.0;  // 16 balanced terms: below 8 q
+        for (int y = 0; y < 16; y++) sum += comp ? coef[r][y].b : coef[r][y].a;
+        coff[r][comp] = reduce_lazy(mul_lazy(reduce_lazy(sum, M), -(double)GR_SMALL, M), M);
     }
     __syncthreads();
     v4i acc[13];
@@ -249,74 +282,139 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
     // read this one back from T.
     size_t xb = (size_t)blockIdx.x * cpt * GR_X;
     if (xb >= S) return;  // whole block past the end (uniform)
-    double v[16];
+    const uint64_t qi = (uint64_t)M.q, half = qi >> 1;
+    const uint64_t off_pos = GR_SMALL, off_neg = GR_SMALL - qi;  // t = b + 2^14 (mod 2^64)
+    // the short path's u32 view of the wave's own columns of T: row y at
+    // T32 + 2 GR_LD y (the first half of the row's 64 u64 slots)
+    uint32_t *T32 = reinterpret_cast<uint32_t *>(T) + 128 * wave;
+    uint64_t v[16];
     {
         const size_t xc = min(xb + tid, S - 1);
 #pragma unroll
-        for (int y = 0; y < 16; y++) v[y] = u2d(wr[(size_t)y * S + xc]);
+        for (int y = 0; y < 16; y++) v[y] = wr[(size_t)y * S + xc];
     }
     for (int cc = 0;; cc++) {
         const size_t x = xb + tid;
         const bool live = x < S;
         const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform
+        uint32_t bad = 0;
 #pragma unroll
-        for (int y = 0; y < 16; y++) T[y * GR_LD + tid] = live ? d2u(v[y]) : 0;
+        for (int y = 0; y < 16; y++) {
+            const uint64_t t = v[y] + (v[y] > half ? off_neg : off_pos);
+            bad |= (uint32_t)(t >> 32) | ((uint32_t)t & ~(2 * GR_SMALL - 1));
+        }
+        const bool small = __builtin_amdgcn_ballot_w64(live && bad != 0) == 0;  // uniform
+        if (small) {
+#pragma unroll
+            for (int y = 0; y < 16; y++)
+                T32[2 * GR_LD * y + lane] =
+                    live ? (uint32_t)(v[y] + (v[y] > half ? off_neg : off_pos)) : GR_SMALL;
+        } else {
+#pragma unroll
+            for (int y = 0; y < 16; y++) T[y * GR_LD + tid] = live ? v[y] : 0;
+        }
         {
             const size_t xc = min(more ? x + GR_X : x, S - 1);
 #pragma unroll
-            for (int y = 0; y < 16; y++) v[y] = u2d(wr[(size_t)y * S + xc]);
+            for (int y = 0; y < 16; y++) v[y] = wr[(size_t)y * S + xc];
         }
         wave_sync();
-        if (live) {
-            // products |coef v| <= 0.69 q (balanced coef, canonical v): eight
-            // of them stay below 5.6 q, then two lazy reductions
-            for (int r = 0; r < nrep; r++) {
-                double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+        // digit fragments for the matrix cores: lane 16 g + r takes digit row
+        // (i, r) of the wave's columns 16 g .. 16 g + 15, so only the wave has
+        // to be in step with itself around T.  The short path fills digits 0
+        // and 1; both paths then run the same 49 products (kept unconditional:
+        // accumulators that only one branch updates cost copies instead).
+        v4i f[7];
+        const int fr = lane & 15, fg = lane >> 4;
+        if (small) {
+            if (live) {
+                uint32_t tt[16];
 #pragma unroll
-                for (int y = 0; y < 8; y++) {
-                    const F2 c0 = coef[r][y], c1 = coef[r][y + 8];
-                    const double v0 = u2d(T[y * GR_LD + tid]), v1 = u2d(T[(y + 8) * GR_LD + tid]);
-                    a0 += mul_lazy(c0.a, v0, M);
-                    a1 += mul_lazy(c1.a, v1, M);
-                    if (FQ2) {
-                        b0 += mul_lazy(c0.b, v0, M);
-                        b1 += mul_lazy(c1.b, v1, M);
+                for (int y = 0; y < 16; y++) tt[y] = T32[2 * GR_LD * y + lane];
+                for (int r = 0; r < nrep; r++) {
+                    double ah = 0.0, al = 0.0, bh = 0.0, bl = 0.0;
+#pragma unroll
+                    for (int y = 0; y < 16; y++) {
+                        const double tv = (double)tt[y];
+                        const double2 ca = *reinterpret_cast<const double2 *>(&csplit[r][y][0]);
+                        ah = __fma_rn(ca.x, tv, ah);
+                        al = __fma_rn(ca.y, tv, al);
+                        if (FQ2) {
+                            const double2 cb = *reinterpret_cast<const double2 *>(&csplit[r][y][2]);
+                            bh = __fma_rn(cb.x, tv, bh);
+                            bl = __fma_rn(cb.y, tv, bl);
+                        }
                     }
+                    double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
+                    o[0] = gr_small_sum(ah, al, coff[r][0], M);
+                    o[1] = FQ2 ? gr_small_sum(bh, bl, coff[r][1], M) : 0.0;
                 }
-                double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
-                o[0] = reduce_lazy(reduce_lazy(a0, M) + a1, M);
-                o[1] = FQ2 ? reduce_lazy(reduce_lazy(b0, M) + b1, M) : 0.0;
+            }
+            if (GRAM) {
+                // digits of b = t - 2^14: (b + 0x80) ^ 0x80 holds d0, d1 in bytes 0, 1
+                const uint4 *src = reinterpret_cast<const uint4 *>(T32 + 2 * GR_LD * fr + 16 * fg);
+#pragma unroll
+                for (int k4 = 0; k4 < 4; k4++) {
+                    const uint4 u = src[k4];
+                    const uint32_t y0 = (u.x - (GR_SMALL - 0x80u)) ^ 0x80u;
+                    const uint32_t y1 = (u.y - (GR_SMALL - 0x80u)) ^ 0x80u;
+                    const uint32_t y2 = (u.z - (GR_SMALL - 0x80u)) ^ 0x80u;
+                    const uint32_t y3 = (u.w - (GR_SMALL - 0x80u)) ^ 0x80u;
+                    const uint32_t p01 = __builtin_amdgcn_perm(y1, y0, 0x05010400u);
+                    const uint32_t p23 = __builtin_amdgcn_perm(y3, y2, 0x05010400u);
+                    f[0][k4] = (int)__builtin_amdgcn_perm(p23, p01, 0x05040100u);
+                    f[1][k4] = (int)__builtin_amdgcn_perm(p23, p01, 0x07060302u);
+                }
+#pragma unroll
+                for (int i = 2; i < 7; i++) f[i] = (v4i){0, 0, 0, 0};
+            }
+        } else {
+            if (live) {
+                // products |coef v| <= 0.69 q (balanced coef, canonical v): eight
+                // of them stay below 5.6 q, then two lazy reductions
+                for (int r = 0; r < nrep; r++) {
+                    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+#pragma unroll
+                    for (int y = 0; y < 8; y++) {
+                        const F2 c0 = coef[r][y], c1 = coef[r][y + 8];
+                        const double v0 = u2d(T[y * GR_LD + tid]), v1 = u2d(T[(y + 8) * GR_LD + tid]);
+                        a0 += mul_lazy(c0.a, v0, M);
+                        a1 += mul_lazy(c1.a, v1, M);
+                        if (FQ2) {
+                            b0 += mul_lazy(c0.b, v0, M);
+                            b1 += mul_lazy(c1.b, v1, M);
+                        }
+                    }
+                    double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
+                    o[0] = reduce_lazy(reduce_lazy(a0, M) + a1, M);
+                    o[1] = FQ2 ? reduce_lazy(reduce_lazy(b0, M) + b1, M) : 0.0;
+                }
+            }
+            if (GRAM) {
+                const uint64_t *src = T + fr * GR_LD + 64 * wave + 16 * fg;
+#pragma unroll
+                for (int k4 = 0; k4 < 4; k4++) {
+                    uint32_t lo[4], hi[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const uint64_t yv = (src[4 * k4 + q] + GR_BIAS) ^ 0x8080808080808080ULL;
+                        lo[q] = (uint32_t)yv;
+                        hi[q] = (uint32_t)(yv >> 32);
+                    }
+                    uint32_t dg[7];
+                    bytes_transpose4(lo, hi, dg);
+#pragma unroll
+                    for (int i = 0; i < 7; i++) f[i][k4] = (int)dg[i];
+                }
             }
         }
+        wave_sync();  // T is rewritten by the next iteration
         if (GRAM) {
-            // Each wave multiplies the 64 columns its own lanes hold (lane
-            // 16 g + r takes digit row (i, r), columns 16 g .. 16 g + 15), so
-            // only the wave has to be in step with itself around T.
-            const int r = lane & 15, g = lane >> 4;
-            const uint64_t *src = T + r * GR_LD + 64 * wave + 16 * g;
-            v4i f[7];
-#pragma unroll
-            for (int k4 = 0; k4 < 4; k4++) {
-                uint32_t lo[4], hi[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint64_t yv = (src[4 * k4 + q] + GR_BIAS) ^ 0x8080808080808080ULL;
-                    lo[q] = (uint32_t)yv;
-                    hi[q] = (uint32_t)(yv >> 32);
-                }
-                uint32_t dg[7];
-                bytes_transpose4(lo, hi, dg);
-#pragma unroll
-                for (int i = 0; i < 7; i++) f[i][k4] = (int)dg[i];
-            }
-            wave_sync();  // T is rewritten by the next iteration
 #pragma unroll
             for (int i = 0; i < 7; i++)
 #pragma unroll
                 for (int j = 0; j < 7; j++)
                     acc[i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f[i], f[j], acc[i + j], 0, 0, 0);
-        } else {
-            wave_sync();  // T is rewritten by the next iteration
         }
         if (!more) break;
         xb += GR_X;

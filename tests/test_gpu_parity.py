@@ -188,6 +188,47 @@ def test_norm_check_reps_and_nonresidue(ctx, orc, reps, nr):
         assert np.array_equal(rhs[rep], orc.norm_check_3_fq2(W3, csd[rep], Q, nr))
 
 
+@pytest.mark.parametrize("fq2,L", [(1, 1 << 16), (0, 1 << 16), (1, 2048)])
+def test_norm_check_mixed_magnitudes(ctx, orc, fq2, L):
+    # The first pass takes a short path for a wave (64 columns) whose residues
+    # are all below 2^14 in balanced form.  Rows here mix waves of short
+    # values, of the extreme short values 2^14 - 1 and -2^14, of short values
+    # with one residue just past the bound (2^14 or -2^14 - 1), and of uniform
+    # residues; at L = 2048 half of the block's lanes are past the row's end.
+    rows, reps = 2, 2
+    say("case: norm_check mixed magnitudes fq2=%d L=%d" % (fq2, L))
+    n = L.bit_length() - 1
+    S = L // 16
+    w = rand_ring(orc, (rows, L), 31, 0xB, 16)
+    uni = rand_ring(orc, (rows, L), 32, 0xB)
+    W, U = w.reshape(rows, 16, S), uni.reshape(rows, 16, S)
+    pick = np.random.default_rng(5).random((rows, 16, S)) < 0.5
+    for k in range(S // 64):
+        c = slice(64 * k, 64 * k + 64)
+        kind = k % 5 if S > 128 else (0, 4)[k]
+        if kind == 1:
+            W[:, :, c] = np.where(pick[:, :, c], (1 << 14) - 1, Q - (1 << 14))
+        elif kind == 2:
+            W[:, 3, 64 * k + 17] = 1 << 14
+        elif kind == 3:
+            W[:, 15, 64 * k + 63] = Q - (1 << 14) - 1
+        elif kind == 4:
+            W[:, :, c] = U[:, :, c]
+    cs = ctx.challenges(reps, n, fq2, 79)
+    outs = ctx.norm_check(ctx.to_device(w), cs, fq2)
+    want = orc.sumcheck(w, cs.download(), Q, fq2=bool(fq2), nr=NR, threads=8)
+    for k in ("claims", "msgs", "finals", "evals"):
+        assert np.array_equal(outs[k].download(), want[k]), k
+    assert outs["ok"].download().all() and want["ok"].all()
+    rhs, csd = outs["rhs"].download(), cs.download()
+    W3 = w.reshape(rows, L // 64, 64)
+    for rep in range(reps):
+        if fq2:
+            assert np.array_equal(rhs[rep], orc.norm_check_3_fq2(W3, csd[rep], Q, NR))
+        else:
+            assert np.array_equal(rhs[rep, :, 0], orc.norm_check_3(W3, csd[rep, :, 0], Q))
+
+
 def test_norm_check_small_lengths(ctx, orc):
     for L in (2, 4, 32):
         say("case: norm_check L=%d (no rhs)" % L)
