@@ -298,11 +298,15 @@ int salsaa_batch_step(salsaa_batch *b, int use_graph);
 /* The same work as a native loop of per-instance calls (the plain path). */
 int salsaa_batch_step_plain(salsaa_batch *b);
 /* One phase, eagerly (for per-phase timing): 0 NTT forward, the commit's
- * digit planes and NTT inverse in one pass, 3 commit + finalize, 4 norm
- * checks; phases 0, 3, 4 in order equal one salsaa_batch_step.  The unfused
- * alternatives of phase 0, not part of the step: 5 NTT forward, 6 NTT
- * forward with the digit planes, 1 NTT inverse, 2 digit planes from the
- * Coefficient witnesses. */
+ * digit planes, NTT inverse and the witnesses' short copy (each residue b
+ * with |b| < 2^14 as two bytes, per tile of 128 ring elements a flag for
+ * any longer one) in one pass, 3 commit + finalize, 7 norm checks whose
+ * first pass reads the short copy where its tiles are short (valid after
+ * phase 0 over the current witnesses); phases 0, 3, 7 in order equal one
+ * salsaa_batch_step.  4: the norm checks from the witnesses alone.  The
+ * unfused alternatives of phase 0, not part of the step: 5 NTT forward,
+ * 6 NTT forward with the digit planes, 1 NTT inverse, 2 digit planes from
+ * the Coefficient witnesses. */
 int salsaa_batch_phase(salsaa_batch *b, int phase);
 /* What the autotuner chose: commit K chunk (tiles of 64 ring elements),
  * norm check first-pass columns per thread (0 = automatic), and the timed

@@ -455,12 +455,15 @@ class Batch:
         _check(lib.salsaa_batch_step_plain(self.h))
 
     # the batched step's phases in order (forward NTT, the commit's digit
-    # planes and inverse NTT in one pass; commit; norm checks), and the
-    # unfused alternatives of the first, which are not part of the step
+    # planes, inverse NTT and the witnesses' short copy in one pass; commit;
+    # norm checks from the short copy, valid after the first phase over the
+    # current witnesses), and the alternatives, which are not part of the
+    # step: the norm checks from the witnesses alone and the unfused passes
+    # of the first phase
     PHASES = ("ntt_roundtrip_planes", "commit", "norm_check")
-    UNFUSED = ("ntt_fwd", "ntt_fwd_planes", "ntt_inv", "planes")
+    UNFUSED = ("ntt_fwd", "ntt_fwd_planes", "ntt_inv", "planes", "norm_check_witness")
     _PHASE_CODE = {"ntt_roundtrip_planes": 0, "ntt_inv": 1, "planes": 2, "commit": 3,
-                   "norm_check": 4, "ntt_fwd": 5, "ntt_fwd_planes": 6}
+                   "norm_check_witness": 4, "ntt_fwd": 5, "ntt_fwd_planes": 6, "norm_check": 7}
 
     def phase(self, name):
         """Run one phase eagerly by name (PHASES or UNFUSED)."""

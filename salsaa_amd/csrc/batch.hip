@@ -37,6 +37,10 @@ __global__ void k_batch_zero(uint64_t *p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
+__global__ void k_batch_ones(uint32_t *p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 1;
+}
 
 struct Outputs {
     uint64_t *com = nullptr, *claims = nullptr, *msgs = nullptr, *fin = nullptr, *ev = nullptr,
@@ -93,6 +97,9 @@ struct salsaa_batch {
     PlaneTile *d_ptiles = nullptr;         // per arena tile: where its planes go
     int8_t *d_planes = nullptr;
     size_t planes_bytes = 0;
+    // the arena's short copy, written by every round trip and read by the
+    // norm checks of the same step (its tile flags start at "not short")
+    ShortCopy sc;
     // commit
     size_t njobs = 0, max_tiles = 0, kchunk = 0;
     std::vector<int> job_ktiles, job_rows;
@@ -211,7 +218,7 @@ int run_commit(salsaa_batch *b) {
 
 // Enqueues the norm checks on their streams after the work already on the
 // context's stream; norm_checks_join makes the context's stream wait for them.
-int norm_checks_fork(salsaa_batch *b, int cpt) {
+int norm_checks_fork(salsaa_batch *b, int cpt, const ShortCopy *sc) {
     Context *ctx = &b->ctx->c;
     const Outputs &o = b->out[0];
     SALSAA_HIP(hipEventRecord(b->ev_fork, ctx->stream));
@@ -240,7 +247,7 @@ int norm_checks_fork(salsaa_batch *b, int cpt) {
         no.ok_rep = 1;
         SALSAA_TRY(launch_norm_check_ex(&b->nc_ctx[k], b->d_wit, b->d_wrow + p, g.rows, g.L,
                                         b->d_cs + b->off_c[p], b->reps * nr * 2, b->reps, b->fq2,
-                                        b->nr, no, cpt, &b->nc[k]));
+                                        b->nr, no, cpt, &b->nc[k], sc));
     }
     for (int k = 0; k < kNcStreams; k++)
         SALSAA_HIP(hipEventRecord(b->ev_done[k], b->nc_ctx[k].stream));
@@ -253,22 +260,23 @@ int norm_checks_join(salsaa_batch *b) {
     return SALSAA_OK;
 }
 
-int run_norm_checks(salsaa_batch *b, int cpt) {
-    SALSAA_TRY(norm_checks_fork(b, cpt));
+int run_norm_checks(salsaa_batch *b, int cpt, const ShortCopy *sc) {
+    SALSAA_TRY(norm_checks_fork(b, cpt, sc));
     return norm_checks_join(b);
 }
 
 int run_fwd_planes(salsaa_batch *b, int roundtrip) {
     return launch_ntt_fwd_planes(&b->ctx->c, b->d_wit, b->tot_w, b->d_planes, b->d_ptiles,
-                                 roundtrip);
+                                 roundtrip, roundtrip ? &b->sc : nullptr);
 }
 
-// one pass: forward NTT, the commit's digit planes and the inverse NTT; then
-// the commit on the context's stream while the norm checks (which need only
-// the Coefficient witnesses) run beside it on theirs
+// one pass: forward NTT, the commit's digit planes, the inverse NTT and the
+// short copy; then the commit on the context's stream while the norm checks
+// (which need only the Coefficient witnesses, or their short copy) run beside
+// it on theirs
 int run_step(salsaa_batch *b) {
     SALSAA_TRY(run_fwd_planes(b, 1));
-    SALSAA_TRY(norm_checks_fork(b, b->cpt_first));
+    SALSAA_TRY(norm_checks_fork(b, b->cpt_first, &b->sc));
     SALSAA_TRY(run_commit(b));
     return norm_checks_join(b);
 }
@@ -315,11 +323,14 @@ int autotune(salsaa_batch *b) {
     }
     SALSAA_TRY(build_commit_tables(b, best_kc));
     b->t_commit = best;
+    // the norm checks as the step runs them: after a round trip (of the
+    // still empty, short witnesses), from the short copy
+    SALSAA_TRY(run_fwd_planes(b, 1));
     int best_cpt = 0;
     float best_nc = 0.f;
     for (int cpt : {1, 2, 4, 8}) {
         float ms = 0.f;
-        SALSAA_TRY(time_ms(b, [&] { return run_norm_checks(b, cpt); }, 2, &ms));
+        SALSAA_TRY(time_ms(b, [&] { return run_norm_checks(b, cpt, &b->sc); }, 2, &ms));
         if (!best_cpt || ms < best_nc) {
             best_nc = ms;
             best_cpt = cpt;
@@ -340,6 +351,8 @@ void destroy(salsaa_batch *b) {
     free_outputs(b->out[1]);
     (void)hipFree(b->d_wit);
     (void)hipFree(b->d_cs);
+    (void)hipFree(b->sc.t);
+    (void)hipFree(b->sc.flag);
     (void)hipFree(b->d_pjobs);
     (void)hipFree(b->d_off);
     (void)hipFree(b->d_m);
@@ -406,8 +419,12 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
     }
     SALSAA_TRY(dmalloc(&b->d_wit, b->tot_w * kD));
     SALSAA_TRY(dmalloc(&b->d_cs, b->tot_c));
+    SALSAA_TRY(dmalloc(&b->sc.t, b->tot_w * kD));
+    SALSAA_TRY(dmalloc(&b->sc.flag, b->tot_w / kPlaneTile));
     hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div(b->tot_w * kD, 256)), dim3(256), 0, c->stream,
                        b->d_wit, b->tot_w * kD);
+    hipLaunchKernelGGL(k_batch_ones, dim3(ceil_div(b->tot_w / kPlaneTile, 256)), dim3(256), 0,
+                       c->stream, b->sc.flag, b->tot_w / kPlaneTile);
     hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div(b->tot_c, 256)), dim3(256), 0, c->stream,
                        b->d_cs, b->tot_c);
     SALSAA_HIP(hipGetLastError());
@@ -614,10 +631,11 @@ int salsaa_batch_phase(salsaa_batch *b, int phase) {
     case 1: return launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 1);
     case 2: return run_planes(b);
     case 3: return run_commit(b);
-    case 4: return run_norm_checks(b, b->cpt_first);
+    case 4: return run_norm_checks(b, b->cpt_first, nullptr);
     case 5: return launch_ntt(ctx, b->d_wit, b->tot_w, SALSAA_REP_INCOMPLETE, 0);
     case 6: return run_fwd_planes(b, 0);
-    default: SALSAA_CHECK(false, "phase must be 0..6");
+    case 7: return run_norm_checks(b, b->cpt_first, &b->sc);
+    default: SALSAA_CHECK(false, "phase must be 0..7");
     }
 }
 

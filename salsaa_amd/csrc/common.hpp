@@ -299,13 +299,33 @@ struct PlaneTile {
     unsigned long long base = 0;
     unsigned sstride = 0, kstride = 0;
 };
+// The short copy of a witness arena, for the norm check's first pass: the
+// norm check proves a witness short, and a residue x whose balanced form b has
+// |b| < 2^14 is kept as the u16 t = b + 2^14 (t[i] for arena coefficient i).
+// flag[k] is nonzero when some coefficient of tile k (kShortTile coefficients,
+// one kPlaneTile-element tile of the arena) is not short; such a tile's t
+// entries are meaningless.
+constexpr uint32_t kShortOff = 1u << 14;
+constexpr size_t kShortTile = (size_t)kPlaneTile * kD;
+struct ShortCopy {
+    uint16_t *t = nullptr;
+    uint32_t *flag = nullptr;
+};
+// t = b + 2^14 (mod 2^64) for canonical x; x is short iff short_bad(t) == 0
+__device__ __forceinline__ uint64_t short_t(uint64_t x, uint64_t half, uint64_t off_neg) {
+    return x + (x > half ? off_neg : (uint64_t)kShortOff);
+}
+__device__ __forceinline__ uint32_t short_bad(uint64_t t) {
+    return (uint32_t)(t >> 32) | ((uint32_t)t & ~(2 * kShortOff - 1));
+}
 // Forward IncompleteNTT of `count` (a multiple of kPlaneTile) arena elements in
 // place, fused with the full-NTT digit planes of every tile (tiles[i] for
 // elements kPlaneTile i ..), written under `planes`.  roundtrip: the inverse
 // IncompleteNTT follows in the same pass, so the arena ends in Coefficient
-// form again (the forward + inverse NTT pair of a prover step).
+// form again (the forward + inverse NTT pair of a prover step); with `sc`
+// the pass also writes the arena's short copy.
 int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *planes,
-                          const PlaneTile *tiles, int roundtrip);
+                          const PlaneTile *tiles, int roundtrip, const ShortCopy *sc = nullptr);
 int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
                        size_t a_ktiles, const int8_t *WB, size_t r, size_t m, size_t ktW,
                        uint64_t *out);
@@ -358,11 +378,13 @@ void nc_scratch_need(size_t rows, size_t L, size_t reps, size_t *macc_bytes, siz
                      size_t *bufB_bytes);
 // wrow: per-row offsets (u64) of the rows from w (null: row r at w + r L);
 // cs_row: u64 between consecutive rows' challenge blocks (0: shared);
-// cpt_first: columns per thread of the first pass (0: automatic)
+// cpt_first: columns per thread of the first pass (0: automatic);
+// sc: the short copy of the arena starting at w (the first pass reads it
+// where every tile it covers is flagged short)
 int launch_norm_check_ex(Context *ctx, const uint64_t *w, const unsigned long long *wrow,
                          size_t rows, size_t L, const uint64_t *cs, size_t cs_row, size_t reps,
                          int fq2, uint64_t nr, const NcOut &o, int cpt_first,
-                         const NcScratch *ext);
+                         const NcScratch *ext, const ShortCopy *sc = nullptr);
 int launch_norm_check(Context *ctx, const uint64_t *w, size_t rows, size_t L,
                       const uint64_t *cs, size_t reps, int fq2, uint64_t nr, uint64_t *claims,
                       uint64_t *msgs, uint64_t *finals, uint64_t *evals, uint64_t *rhs,

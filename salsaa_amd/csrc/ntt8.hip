@@ -420,13 +420,16 @@ __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4)))
 static_assert(kPlaneTile == EL, "the round trip works on 128-element tiles");
 constexpr int WEL = 8;                // ring elements per wave
 constexpr int WROWS = WEL * SROW;     // u64 of a wave's rows
+template <bool SHORT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ntt_rt_planes(
     uint64_t *__restrict__ data, size_t ntiles, const double *__restrict__ tab, Mod M,
-    int8_t *__restrict__ planes, const PlaneTile *__restrict__ ptiles) {
+    int8_t *__restrict__ planes, const PlaneTile *__restrict__ ptiles, uint16_t *__restrict__ sht,
+    uint32_t *__restrict__ shf) {
     __shared__ uint64_t rows_all[(NT / 64) * WROWS];
     __shared__ uint64_t work[64 * EL];
     __shared__ double twl[32], twi[33], zt[32];
     __shared__ int zo[32];
+    __shared__ uint32_t wbad[NT / 64];  // per wave: a coefficient of its slice is not short
     size_t tile = blockIdx.x;
     if (tile >= ntiles) return;  // uniform in the block
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -449,13 +452,31 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     const ulonglong2 *src = reinterpret_cast<const ulonglong2 *>(slice(tile));
     ulonglong2 r0 = src[lane], r1 = src[lane + 64], r2 = src[lane + 128], r3 = src[lane + 192];
     PlaneTile dsc = ptiles[tile];
-    auto put = [&]() {
+    const uint64_t qi = (uint64_t)M.q, half = qi >> 1, off_neg = (uint64_t)kShortOff - qi;
+    // the slice of tile tl into the wave's rows; with a short copy, also its
+    // u16 entries (chunk c holds coefficients 2c, 2c + 1: one u32 store) and
+    // the wave's flag, which thread 0 folds into the tile's after the next
+    // block barrier
+    auto put = [&](size_t tl) {
         *reinterpret_cast<ulonglong2 *>(&rows[(lane >> 5) * SROW + (lane & 31) * 2]) = r0;
         *reinterpret_cast<ulonglong2 *>(&rows[(2 + (lane >> 5)) * SROW + (lane & 31) * 2]) = r1;
         *reinterpret_cast<ulonglong2 *>(&rows[(4 + (lane >> 5)) * SROW + (lane & 31) * 2]) = r2;
         *reinterpret_cast<ulonglong2 *>(&rows[(6 + (lane >> 5)) * SROW + (lane & 31) * 2]) = r3;
+        if (SHORT) {
+            const ulonglong2 rr[4] = {r0, r1, r2, r3};
+            uint32_t *dst = reinterpret_cast<uint32_t *>(sht + (tl * EL + WEL * w) * kD) + lane;
+            uint32_t bad = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint64_t ta = short_t(rr[i].x, half, off_neg), tb = short_t(rr[i].y, half, off_neg);
+                bad |= short_bad(ta) | short_bad(tb);
+                dst[64 * i] = ((uint32_t)ta & 0xFFFFu) | ((uint32_t)tb << 16);
+            }
+            const bool any = __builtin_amdgcn_ballot_w64(bad != 0) != 0;
+            if (lane == 0) wbad[w] = any;
+        }
     };
-    put();
+    put(tile);
     wave_sync();
     // Loads and stores are unconditional (the last prefetch re-reads the
     // current slice), so each wait covers only the loads it needs.
@@ -493,6 +514,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         }
         ntt32_inv_b_lazy(v, twi, t, M);  // v: this lane's forward outputs 32p + 8t + x
         __syncthreads();  // the previous tile's planes have been read out of work
+        if (SHORT && tid == 0) {
+            uint32_t f = 0;
+#pragma unroll
+            for (int k = 0; k < NT / 64; k++) f |= wbad[k];
+            shf[tile] = f;
+        }
 #pragma unroll
         for (int x = 0; x < 8; x++) work[wpos_w(8 * h + x, e)] = dbits(a[x]);
         __syncthreads();
@@ -516,7 +543,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         }
         if (next >= ntiles) break;
         wave_sync();  // the store has read the rows
-        put();
+        put(next);
         wave_sync();
         tile = next;
         dsc = ndsc;
@@ -803,16 +830,20 @@ int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse
 }
 
 int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *planes,
-                          const PlaneTile *tiles, int roundtrip) {
+                          const PlaneTile *tiles, int roundtrip, const ShortCopy *sc) {
     if (count == 0) return SALSAA_OK;
     SALSAA_CHECK(count % kPlaneTile == 0, "fused planes need whole tiles");
+    SALSAA_CHECK(sc == nullptr || (roundtrip && sc->t && sc->flag),
+                 "the short copy comes with the round trip");
     Mod M{ctx->qd, ctx->qinv};
     const size_t nt = count / kPlaneTile;
     static int occ[2] = {0, 0};
-    const void *fn = roundtrip ? reinterpret_cast<const void *>(&k_ntt_rt_planes)
-                               : reinterpret_cast<const void *>(&k_ntt_fwd_planes<kPlaneTile>);
+    static int occ_short = 0;
+    const void *fn = !roundtrip ? reinterpret_cast<const void *>(&k_ntt_fwd_planes<kPlaneTile>)
+                     : sc       ? reinterpret_cast<const void *>(&k_ntt_rt_planes<true>)
+                                : reinterpret_cast<const void *>(&k_ntt_rt_planes<false>);
     const int threads = roundtrip ? NT : 8 * kPlaneTile;
-    int &o = occ[roundtrip ? 1 : 0];
+    int &o = sc ? occ_short : occ[roundtrip ? 1 : 0];
     if (o == 0) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, 0) != hipSuccess || n < 1)
@@ -820,9 +851,12 @@ int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *pl
         o = n;
     }
     const unsigned g = (unsigned)std::min<size_t>(nt, (size_t)ctx->ncu * o);
-    if (roundtrip)
-        hipLaunchKernelGGL(k_ntt_rt_planes, dim3(g), dim3(NT), 0, ctx->stream, data, nt,
-                           ctx->d_tables, M, planes, tiles);
+    if (roundtrip && sc)
+        hipLaunchKernelGGL(k_ntt_rt_planes<true>, dim3(g), dim3(NT), 0, ctx->stream, data, nt,
+                           ctx->d_tables, M, planes, tiles, sc->t, sc->flag);
+    else if (roundtrip)
+        hipLaunchKernelGGL(k_ntt_rt_planes<false>, dim3(g), dim3(NT), 0, ctx->stream, data, nt,
+                           ctx->d_tables, M, planes, tiles, nullptr, nullptr);
     else
         hipLaunchKernelGGL((k_ntt_fwd_planes<kPlaneTile>), dim3(g), dim3(8 * kPlaneTile), 0,
                            ctx->stream, data, nt, ctx->d_tables, M, planes, tiles);

@@ -211,19 +211,23 @@ __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) 
 // pairs, at most 512 columns per wave).  k_sc_gram_msgs then evaluates the
 // forms for each (row, repetition).
 //
-// Short witnesses (the norm check's purpose) take a cheaper path per wave and
-// iteration: when every residue of the wave's 64 columns is b in balanced form
-// with |b| < 2^14, t = b + 2^14 < 2^15 goes through LDS as a u32.  The fold is
-// then exact in fp64 without reductions: with c = c_h 2^25 + c_l (|c_h|,
-// |c_l| <= 2^24), sum_y c_h t_y and sum_y c_l t_y stay below 2^44, and
+// Short witnesses (the norm check's purpose) take a cheaper path: when every
+// residue of a wave's 64 columns is b in balanced form with |b| < 2^14,
+// t = b + 2^14 < 2^15 goes through LDS as a u32.  The fold is then exact in
+// fp64 without reductions: with c = c_h 2^25 + c_l (|c_h|, |c_l| <= 2^24),
+// sum_y c_h t_y and sum_y c_l t_y stay below 2^44, and
 // sum_y c b_y = 2^25 sum c_h t + sum c_l t - 2^14 sum c.  The Gram matrix
 // needs only b's two balanced base-256 digits (E_0, E_1, E_2: 4 products).
+// k_sc_gram takes this path per wave and iteration after checking the
+// residues; with the round trip's short copy of a batch arena (ShortCopy),
+// k_sc_gram_short reads the copy's t directly for every block whose tiles
+// are all flagged short, and k_sc_gram skips those blocks.
 typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int GR_X = SC_THREADS;  // columns per iteration
 constexpr int GR_LD = GR_X + 2;   // LDS row stride (u64): fragment reads hit distinct banks
 constexpr int GR_MAXREP = 4;      // repetitions folded per launch
 constexpr uint64_t GR_BIAS = 0x0080808080808080ULL;
-constexpr uint32_t GR_SMALL = 1u << 14;  // short path: balanced |b| < 2^14, t = b + 2^14
+constexpr uint32_t GR_SMALL = kShortOff;  // short path: balanced |b| < 2^14, t = b + 2^14
 
 // 2^25 h + l + k for the short path's exact sums (|h|, |l| < 2^44, k balanced),
 // reduced to a balanced residue: 2^25 h is exact and below 2^69, its remainder
@@ -234,20 +238,17 @@ __device__ __forceinline__ double gr_small_sum(double h, double l, double k, Mod
     return reduce_lazy(r + l + k, M);
 }
 
-template <bool FQ2, bool SNR, bool GRAM>
-__global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
-    const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
-    int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
-    double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc,
-    const double *__restrict__ tab, Mod M) {
-    __shared__ __align__(16) uint64_t T[16 * GR_LD];
-    __shared__ F2 coef[GR_MAXREP][16];
-    __shared__ __align__(16) double csplit[GR_MAXREP][16][4];  // (c_h, c_l) of both components
-    __shared__ double coff[GR_MAXREP][2];                     // -2^14 sum_y c, balanced
-    const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint64_t *cs = cs0 + row * cs_row;
-    const size_t S = L >> 4;
-    // fold coefficients A_4[0][y] = prod_s (bit 3 - s of y ? c_s : 1 - c_s), balanced
+// A block's fold coefficients for repetitions rep0 .. rep0 + nrep - 1:
+// A_4[0][y] = prod_s (bit 3 - s of y ? c_s : 1 - c_s), balanced; their
+// halves for the short path (spare repetition slots zero) and -2^14 sum_y c.
+struct GrCoef {
+    F2 coef[GR_MAXREP][16];
+    __align__(32) double csplit[GR_MAXREP][16][4];  // (c_h, c_l) of both components
+    double coff[GR_MAXREP][2];
+};
+template <bool FQ2, bool SNR>
+__device__ __forceinline__ void gr_coefs(GrCoef &C, const uint64_t *cs, int rep0, int nrep,
+                                         int nrounds, double nr, Mod M, int tid) {
     if (tid < 16 * nrep) {
         const int r = tid >> 4, y = tid & 15;
         F2 a = {1.0, 0.0};
@@ -256,37 +257,174 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
             a = x_mul<FQ2, SNR>(a, ((y >> (3 - t)) & 1) ? c : x_sub<FQ2>(F2{1.0, 0.0}, c, M), nr, M);
         }
         const double ca = balance(a.a, M), cb = balance(a.b, M);
-        coef[r][y] = {ca, cb};
+        C.coef[r][y] = {ca, cb};
         const double ha = __builtin_rint(ca * 0x1p-25), hb = __builtin_rint(cb * 0x1p-25);
-        csplit[r][y][0] = ha;
-        csplit[r][y][1] = ca - ha * 0x1p25;
-        csplit[r][y][2] = hb;
-        csplit[r][y][3] = cb - hb * 0x1p25;
+        C.csplit[r][y][0] = ha;
+        C.csplit[r][y][1] = ca - ha * 0x1p25;
+        C.csplit[r][y][2] = hb;
+        C.csplit[r][y][3] = cb - hb * 0x1p25;
+    } else if (tid < 16 * GR_MAXREP) {
+        C.csplit[tid >> 4][tid & 15][0] = C.csplit[tid >> 4][tid & 15][1] = 0.0;
+        C.csplit[tid >> 4][tid & 15][2] = C.csplit[tid >> 4][tid & 15][3] = 0.0;
     }
     __syncthreads();
     if (tid < 2 * nrep) {
         const int r = tid >> 1, comp = tid & 1;
         double sum = 0.0;  // 16 balanced terms: below 8 q
-        for (int y = 0; y < 16; y++) sum += comp ? coef[r][y].b : coef[r][y].a;
-        coff[r][comp] = reduce_lazy(mul_lazy(reduce_lazy(sum, M), -(double)GR_SMALL, M), M);
+        for (int y = 0; y < 16; y++) sum += comp ? C.coef[r][y].b : C.coef[r][y].a;
+        C.coff[r][comp] = reduce_lazy(mul_lazy(reduce_lazy(sum, M), -(double)GR_SMALL, M), M);
     }
     __syncthreads();
+}
+
+// Whether every tile of the short copy that the block's columns
+// [xb, xb + cpt GR_X) of the 16 row segments meet is flagged short (the
+// range, at most 2048 < kShortTile columns, meets at most two per segment).
+__device__ __forceinline__ bool gr_compact(const uint32_t *__restrict__ shf, size_t base, size_t S,
+                                           size_t xb, int cpt) {
+    const size_t xe = min(xb + (size_t)cpt * GR_X, S) - 1;
+    uint32_t any = 0;
+    for (int y = 0; y < 16; y++)
+        any |= shf[(base + (size_t)y * S + xb) / kShortTile] |
+               shf[(base + (size_t)y * S + xe) / kShortTile];
+    return any == 0;
+}
+
+// The short fold of column x from its 16 entries t (PAIR: two repetitions per
+// sweep, so each y issues its coefficient reads together; a lone last
+// repetition reads a zeroed spare slot and discards those sums).
+template <bool FQ2, bool PAIR>
+__device__ __forceinline__ void gr_short_fold(const GrCoef &C, const uint32_t (&tt)[16], int nrep,
+                                              int rep0, int rows, int row, size_t S, size_t x,
+                                              double *__restrict__ out, Mod M) {
+    if (PAIR) {
+        for (int r = 0; r < nrep; r += 2) {
+            double h0 = 0.0, l0 = 0.0, h1 = 0.0, l1 = 0.0;  // repetition r: a, b
+            double h2 = 0.0, l2 = 0.0, h3 = 0.0, l3 = 0.0;  // repetition r + 1
+#pragma unroll
+            for (int y = 0; y < 16; y++) {
+                const double tv = (double)tt[y];
+                const double4 c0 = *reinterpret_cast<const double4 *>(&C.csplit[r][y][0]);
+                const double4 c1 = *reinterpret_cast<const double4 *>(&C.csplit[r + 1][y][0]);
+                h0 = __fma_rn(c0.x, tv, h0);
+                l0 = __fma_rn(c0.y, tv, l0);
+                h2 = __fma_rn(c1.x, tv, h2);
+                l2 = __fma_rn(c1.y, tv, l2);
+                if (FQ2) {
+                    h1 = __fma_rn(c0.z, tv, h1);
+                    l1 = __fma_rn(c0.w, tv, l1);
+                    h3 = __fma_rn(c1.z, tv, h3);
+                    l3 = __fma_rn(c1.w, tv, l3);
+                }
+            }
+            double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
+            o[0] = gr_small_sum(h0, l0, C.coff[r][0], M);
+            o[1] = FQ2 ? gr_small_sum(h1, l1, C.coff[r][1], M) : 0.0;
+            if (r + 1 < nrep) {
+                o += (size_t)rows * S * 2;
+                o[0] = gr_small_sum(h2, l2, C.coff[r + 1][0], M);
+                o[1] = FQ2 ? gr_small_sum(h3, l3, C.coff[r + 1][1], M) : 0.0;
+            }
+        }
+    } else {
+        for (int r = 0; r < nrep; r++) {
+            double ah = 0.0, al = 0.0, bh = 0.0, bl = 0.0;
+#pragma unroll
+            for (int y = 0; y < 16; y++) {
+                const double tv = (double)tt[y];
+                const double2 ca = *reinterpret_cast<const double2 *>(&C.csplit[r][y][0]);
+                ah = __fma_rn(ca.x, tv, ah);
+                al = __fma_rn(ca.y, tv, al);
+                if (FQ2) {
+                    const double2 cb = *reinterpret_cast<const double2 *>(&C.csplit[r][y][2]);
+                    bh = __fma_rn(cb.x, tv, bh);
+                    bl = __fma_rn(cb.y, tv, bl);
+                }
+            }
+            double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
+            o[0] = gr_small_sum(ah, al, C.coff[r][0], M);
+            o[1] = FQ2 ? gr_small_sum(bh, bl, C.coff[r][1], M) : 0.0;
+        }
+    }
+}
+
+// Fragments of digits 0 and 1 of b = t - 2^14 for the matrix cores from the
+// wave's u32 rows (row y at T32 + ld y): lane 16 g + r takes row r, columns
+// 16 g .. 16 g + 15 ((b + 0x80) ^ 0x80 holds d0, d1 in bytes 0, 1).
+__device__ __forceinline__ void gr_short_frags(const uint32_t *T32, int ld, int fr, int fg, v4i &f0,
+                                               v4i &f1) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(T32 + ld * fr + 16 * fg);
+#pragma unroll
+    for (int k4 = 0; k4 < 4; k4++) {
+        const uint4 u = src[k4];
+        const uint32_t y0 = (u.x - (GR_SMALL - 0x80u)) ^ 0x80u;
+        const uint32_t y1 = (u.y - (GR_SMALL - 0x80u)) ^ 0x80u;
+        const uint32_t y2 = (u.z - (GR_SMALL - 0x80u)) ^ 0x80u;
+        const uint32_t y3 = (u.w - (GR_SMALL - 0x80u)) ^ 0x80u;
+        const uint32_t p01 = __builtin_amdgcn_perm(y1, y0, 0x05010400u);
+        const uint32_t p23 = __builtin_amdgcn_perm(y3, y2, 0x05010400u);
+        f0[k4] = (int)__builtin_amdgcn_perm(p23, p01, 0x05040100u);
+        f1[k4] = (int)__builtin_amdgcn_perm(p23, p01, 0x07060302u);
+    }
+}
+
+// The block's Gram sums into gacc[row]: lane (r, g) of wave `wave` holds
+// E_k[4 g + e][r]; G entry = sum_k 2^(8k) E_k mod q, summed over the waves
+// through gpart[4][256] (every wave reaches this point).
+template <int K>
+__device__ __forceinline__ void gr_finish(const v4i (&acc)[K], double *gpart, int wave, int lane,
+                                          int tid, const double *__restrict__ tab,
+                                          unsigned long long *__restrict__ gacc, int row, Mod M) {
+    const int r = lane & 15, g = lane >> 4;
+    const double *wt = tab + TAB_W256;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        double xg = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            xg += mul_lazy((double)acc[k][e], wt[k], M);
+            if ((k % 7) == 6) xg = reduce_lazy(xg, M);
+        }
+        gpart[wave * 256 + (4 * g + e) * 16 + r] = reduce(xg, M);
+    }
+    __syncthreads();
+    double tot = gpart[tid];
+#pragma unroll
+    for (int wv = 1; wv < 4; wv++) tot = addmod(tot, gpart[wv * 256 + tid], M);
+    atomicAdd(&gacc[(size_t)row * 256 + tid], (unsigned long long)d2u(tot));
+}
+
+template <bool FQ2, bool SNR, bool GRAM>
+__global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_sc_gram(
+    const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
+    int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
+    double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc,
+    const double *__restrict__ tab, Mod M, const uint32_t *__restrict__ shf) {
+    __shared__ __align__(16) uint64_t T[16 * GR_LD];
+    __shared__ GrCoef C;
+    __shared__ double gpart[4 * 256];
+    const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t S = L >> 4;
+    const size_t base = wrow ? wrow[row] : (size_t)row * L;
+    size_t xb = (size_t)blockIdx.x * cpt * GR_X;
+    if (xb >= S) return;                                     // whole block past the end
+    if (shf != nullptr && gr_compact(shf, base, S, xb, cpt)) return;  // k_sc_gram_short's
+    gr_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
+    const uint64_t *wr = w + base;
+    // the short path's u32 view of the wave's own columns of T: row y at
+    // T32 + 2 GR_LD y (the first half of the row's 64 u64 slots)
+    uint32_t *T32 = reinterpret_cast<uint32_t *>(T) + 128 * wave;
+    const int fr = lane & 15, fg = lane >> 4;
+    const uint64_t qi = (uint64_t)M.q, half = qi >> 1;
+    const uint64_t off_neg = (uint64_t)GR_SMALL - qi;
     v4i acc[13];
 #pragma unroll
     for (int k = 0; k < 13; k++) acc[k] = (v4i){0, 0, 0, 0};
-    const uint64_t *wr = w + (wrow ? wrow[row] : (size_t)row * L);
     // Column x = xb + tid.  Each iteration moves its column into the wave's
     // columns of T and reuses the same registers for the next iteration's 16
     // loads (unconditional: a dead column re-reads column S - 1 and counts as
     // zero), which stay in flight while the fold and the matrix products
     // read this one back from T.
-    size_t xb = (size_t)blockIdx.x * cpt * GR_X;
-    if (xb >= S) return;  // whole block past the end (uniform)
-    const uint64_t qi = (uint64_t)M.q, half = qi >> 1;
-    const uint64_t off_pos = GR_SMALL, off_neg = GR_SMALL - qi;  // t = b + 2^14 (mod 2^64)
-    // the short path's u32 view of the wave's own columns of T: row y at
-    // T32 + 2 GR_LD y (the first half of the row's 64 u64 slots)
-    uint32_t *T32 = reinterpret_cast<uint32_t *>(T) + 128 * wave;
     uint64_t v[16];
     {
         const size_t xc = min(xb + tid, S - 1);
@@ -299,16 +437,12 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
         const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform
         uint32_t bad = 0;
 #pragma unroll
-        for (int y = 0; y < 16; y++) {
-            const uint64_t t = v[y] + (v[y] > half ? off_neg : off_pos);
-            bad |= (uint32_t)(t >> 32) | ((uint32_t)t & ~(2 * GR_SMALL - 1));
-        }
+        for (int y = 0; y < 16; y++) bad |= short_bad(short_t(v[y], half, off_neg));
         const bool small = __builtin_amdgcn_ballot_w64(live && bad != 0) == 0;  // uniform
         if (small) {
 #pragma unroll
             for (int y = 0; y < 16; y++)
-                T32[2 * GR_LD * y + lane] =
-                    live ? (uint32_t)(v[y] + (v[y] > half ? off_neg : off_pos)) : GR_SMALL;
+                T32[2 * GR_LD * y + lane] = live ? (uint32_t)short_t(v[y], half, off_neg) : GR_SMALL;
         } else {
 #pragma unroll
             for (int y = 0; y < 16; y++) T[y * GR_LD + tid] = live ? v[y] : 0;
@@ -319,52 +453,18 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
             for (int y = 0; y < 16; y++) v[y] = wr[(size_t)y * S + xc];
         }
         wave_sync();
-        // digit fragments for the matrix cores: lane 16 g + r takes digit row
-        // (i, r) of the wave's columns 16 g .. 16 g + 15, so only the wave has
-        // to be in step with itself around T.  The short path fills digits 0
-        // and 1; both paths then run the same 49 products (kept unconditional:
-        // accumulators that only one branch updates cost copies instead).
+        // both paths then run the same 49 products (kept unconditional:
+        // accumulators that only one branch updates cost copies instead)
         v4i f[7];
-        const int fr = lane & 15, fg = lane >> 4;
         if (small) {
             if (live) {
                 uint32_t tt[16];
 #pragma unroll
                 for (int y = 0; y < 16; y++) tt[y] = T32[2 * GR_LD * y + lane];
-                for (int r = 0; r < nrep; r++) {
-                    double ah = 0.0, al = 0.0, bh = 0.0, bl = 0.0;
-#pragma unroll
-                    for (int y = 0; y < 16; y++) {
-                        const double tv = (double)tt[y];
-                        const double2 ca = *reinterpret_cast<const double2 *>(&csplit[r][y][0]);
-                        ah = __fma_rn(ca.x, tv, ah);
-                        al = __fma_rn(ca.y, tv, al);
-                        if (FQ2) {
-                            const double2 cb = *reinterpret_cast<const double2 *>(&csplit[r][y][2]);
-                            bh = __fma_rn(cb.x, tv, bh);
-                            bl = __fma_rn(cb.y, tv, bl);
-                        }
-                    }
-                    double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
-                    o[0] = gr_small_sum(ah, al, coff[r][0], M);
-                    o[1] = FQ2 ? gr_small_sum(bh, bl, coff[r][1], M) : 0.0;
-                }
+                gr_short_fold<FQ2, false>(C, tt, nrep, rep0, rows, row, S, x, out, M);
             }
             if (GRAM) {
-                // digits of b = t - 2^14: (b + 0x80) ^ 0x80 holds d0, d1 in bytes 0, 1
-                const uint4 *src = reinterpret_cast<const uint4 *>(T32 + 2 * GR_LD * fr + 16 * fg);
-#pragma unroll
-                for (int k4 = 0; k4 < 4; k4++) {
-                    const uint4 u = src[k4];
-                    const uint32_t y0 = (u.x - (GR_SMALL - 0x80u)) ^ 0x80u;
-                    const uint32_t y1 = (u.y - (GR_SMALL - 0x80u)) ^ 0x80u;
-                    const uint32_t y2 = (u.z - (GR_SMALL - 0x80u)) ^ 0x80u;
-                    const uint32_t y3 = (u.w - (GR_SMALL - 0x80u)) ^ 0x80u;
-                    const uint32_t p01 = __builtin_amdgcn_perm(y1, y0, 0x05010400u);
-                    const uint32_t p23 = __builtin_amdgcn_perm(y3, y2, 0x05010400u);
-                    f[0][k4] = (int)__builtin_amdgcn_perm(p23, p01, 0x05040100u);
-                    f[1][k4] = (int)__builtin_amdgcn_perm(p23, p01, 0x07060302u);
-                }
+                gr_short_frags(T32, 2 * GR_LD, fr, fg, f[0], f[1]);
 #pragma unroll
                 for (int i = 2; i < 7; i++) f[i] = (v4i){0, 0, 0, 0};
             }
@@ -376,7 +476,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
                     double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
 #pragma unroll
                     for (int y = 0; y < 8; y++) {
-                        const F2 c0 = coef[r][y], c1 = coef[r][y + 8];
+                        const F2 c0 = C.coef[r][y], c1 = C.coef[r][y + 8];
                         const double v0 = u2d(T[y * GR_LD + tid]), v1 = u2d(T[(y + 8) * GR_LD + tid]);
                         a0 += mul_lazy(c0.a, v0, M);
                         a1 += mul_lazy(c1.a, v1, M);
@@ -391,6 +491,8 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
                 }
             }
             if (GRAM) {
+                // lane 16 g + r takes digit row (i, r) of the wave's columns
+                // 16 g .. 16 g + 15: only the wave has to be in step around T
                 const uint64_t *src = T + fr * GR_LD + 64 * wave + 16 * fg;
 #pragma unroll
                 for (int k4 = 0; k4 < 4; k4++) {
@@ -419,27 +521,70 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram(
         if (!more) break;
         xb += GR_X;
     }
-    if (GRAM) {
-        // lane (r, g) holds E_k[4 g + e][r]; G entry = sum_k 2^(8k) E_k mod q
-        // (every wave reaches this point: the loop above exits uniformly)
-        __shared__ double gpart[4][256];
-        const int r = lane & 15, g = lane >> 4;
-        const double *wt = tab + TAB_W256;
+    if (GRAM) gr_finish<13>(acc, gpart, wave, lane, tid, tab, gacc, row, M);
+}
+
+// The short copy's blocks: the same columns and outputs as k_sc_gram, read
+// as t from the copy (four waves, each staging its 64 columns of 16 rows).
+constexpr int GS_LD = GR_X + 4;  // u32 row stride of the staging rows
+template <bool FQ2, bool SNR, bool GRAM>
+__global__ __launch_bounds__(SC_THREADS) void k_sc_gram_short(
+    const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
+    const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr, int cpt,
+    double *__restrict__ out, unsigned long long *__restrict__ gacc, const double *__restrict__ tab,
+    Mod M, const uint16_t *__restrict__ sht, const uint32_t *__restrict__ shf) {
+    __shared__ __align__(16) uint32_t T32all[16 * GS_LD];  // then the Gram partials
+    __shared__ GrCoef C;
+    const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t S = L >> 4;
+    const size_t base = wrow ? wrow[row] : (size_t)row * L;
+    size_t xb = (size_t)blockIdx.x * cpt * GR_X;
+    if (xb >= S || !gr_compact(shf, base, S, xb, cpt)) return;  // uniform
+    gr_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
+    uint32_t *T32 = T32all + 64 * wave;
+    const int fr = lane & 15, fg = lane >> 4;
+    const uint16_t *tr = sht + base;
+    v4i acc[3] = {(v4i){0, 0, 0, 0}, (v4i){0, 0, 0, 0}, (v4i){0, 0, 0, 0}};
+    // row segment y at the uniform tr + y S, the lane's column as a 32-bit
+    // offset (one address register for all 16 loads)
+    auto fetch = [&](uint32_t(&t)[16], size_t xc) {
+        const uint32_t xo = (uint32_t)min(xc, S - 1);
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            double xg = 0.0;
+        for (int y = 0; y < 16; y++) t[y] = (tr + (size_t)y * S)[xo];
+    };
+    uint32_t tv[16];
+    fetch(tv, xb + tid);
+    for (int cc = 0;; cc++) {
+        const size_t x = xb + tid;
+        const bool live = x < S;
+        const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform
 #pragma unroll
-            for (int k = 0; k < 13; k++) {
-                xg += mul_lazy((double)acc[k][e], wt[k], M);
-                if ((k % 7) == 6) xg = reduce_lazy(xg, M);
-            }
-            gpart[wave][(4 * g + e) * 16 + r] = reduce(xg, M);
+        for (int y = 0; y < 16; y++) T32[GS_LD * y + lane] = live ? tv[y] : GR_SMALL;
+        fetch(tv, more ? x + GR_X : x);
+        wave_sync();
+        if (live) {
+            uint32_t tt[16];
+#pragma unroll
+            for (int y = 0; y < 16; y++) tt[y] = T32[GS_LD * y + lane];
+            gr_short_fold<FQ2, true>(C, tt, nrep, rep0, rows, row, S, x, out, M);
         }
-        __syncthreads();
-        double tot = gpart[0][tid];
-#pragma unroll
-        for (int wv = 1; wv < 4; wv++) tot = addmod(tot, gpart[wv][tid], M);
-        atomicAdd(&gacc[(size_t)row * 256 + tid], (unsigned long long)d2u(tot));
+        if (GRAM) {
+            v4i f0, f1;
+            gr_short_frags(T32, GS_LD, fr, fg, f0, f1);
+            wave_sync();  // T32 is rewritten by the next iteration
+            acc[0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f0, f0, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f0, f1, acc[1], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f1, f0, acc[1], 0, 0, 0);
+            acc[2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f1, f1, acc[2], 0, 0, 0);
+        } else {
+            wave_sync();
+        }
+        if (!more) break;
+        xb += GR_X;
+    }
+    if (GRAM) {
+        __syncthreads();  // the partials reuse the staging rows
+        gr_finish<3>(acc, reinterpret_cast<double *>(T32all), wave, lane, tid, tab, gacc, row, M);
     }
 }
 
@@ -763,7 +908,7 @@ __global__ void k_zero_u64(unsigned long long *p, size_t n) {
 template <bool FQ2, bool SNR>
 int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *wrow, size_t rows,
                     size_t L, const uint64_t *cs, size_t cs_row, size_t reps, double nr,
-                    const NcOut &o, int cpt_first, const NcScratch *ext) {
+                    const NcOut &o, int cpt_first, const NcScratch *ext, const ShortCopy *sc) {
     Mod M{ctx->qd, ctx->qinv};
     const int n = __builtin_ctzll(L);
     const int want_rhs = o.rhs != nullptr;
@@ -813,14 +958,24 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         dim3 grid(bx, (unsigned)rows);
         for (size_t r0 = 0; r0 < reps; r0 += GR_MAXREP) {
             const int nrep = (int)std::min<size_t>(GR_MAXREP, reps - r0);
+            // with a short copy, its short blocks first, then the others
+            const uint32_t *shf = sc ? sc->flag : nullptr;
+            if (sc && r0 == 0)
+                hipLaunchKernelGGL((k_sc_gram_short<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
+                                   ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf);
+            else if (sc)
+                hipLaunchKernelGGL((k_sc_gram_short<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0,
+                                   ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf);
             if (r0 == 0)
                 hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0, ctx->stream,
                                    w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr, cpt, bufA,
-                                   gacc, ctx->d_tables, M);
+                                   gacc, ctx->d_tables, M, shf);
             else
                 hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0, ctx->stream,
                                    w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr, cpt, bufA,
-                                   gacc, ctx->d_tables, M);
+                                   gacc, ctx->d_tables, M, shf);
             SALSAA_HIP(hipGetLastError());
         }
         hipLaunchKernelGGL((k_sc_gram_msgs<FQ2, SNR>), dim3((unsigned)rows, (unsigned)reps), dim3(128),
@@ -890,8 +1045,9 @@ NcOut nc_out_plain(size_t rows, size_t reps, size_t nrounds, uint64_t *claims, u
 int launch_norm_check_ex(Context *ctx, const uint64_t *w, const unsigned long long *wrow,
                          size_t rows, size_t L, const uint64_t *cs, size_t cs_row, size_t reps,
                          int fq2, uint64_t nr, const NcOut &o, int cpt_first,
-                         const NcScratch *ext) {
+                         const NcScratch *ext, const ShortCopy *sc) {
     SALSAA_CHECK(rows > 0 && reps > 0, "norm check needs rows > 0 and reps > 0");
+    SALSAA_CHECK(sc == nullptr || (sc->t && sc->flag), "incomplete short copy");
     SALSAA_CHECK(L >= 2 && (L & (L - 1)) == 0, "norm check length must be a power of two >= 2");
     SALSAA_CHECK(rows < 65536 && reps < 65536, "too many rows or repetitions");
     SALSAA_CHECK(__builtin_ctzll(L) <= 48, "norm check length too large");
@@ -901,12 +1057,12 @@ int launch_norm_check_ex(Context *ctx, const uint64_t *w, const unsigned long lo
     SALSAA_CHECK(cpt_first >= 0 && cpt_first <= 64, "columns per thread out of range");
     if (fq2 && nr < 8)
         return norm_check_impl<true, true>(ctx, w, wrow, rows, L, cs, cs_row, reps, (double)nr, o,
-                                           cpt_first, ext);
+                                           cpt_first, ext, sc);
     if (fq2)
         return norm_check_impl<true, false>(ctx, w, wrow, rows, L, cs, cs_row, reps, (double)nr, o,
-                                            cpt_first, ext);
+                                            cpt_first, ext, sc);
     return norm_check_impl<false, true>(ctx, w, wrow, rows, L, cs, cs_row, reps, (double)nr, o,
-                                        cpt_first, ext);
+                                        cpt_first, ext, sc);
 }
 
 int launch_norm_check(Context *ctx, const uint64_t *w, size_t rows, size_t L,

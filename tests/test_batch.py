@@ -160,3 +160,70 @@ def test_batch_unfused_phases_agree(sa, ctx, orc):
                                                   Wi, Q, threads=8)), i
     b.close()
     key.close()
+
+
+def test_batch_short_copy_mixed(sa, ctx, orc):
+    # The step's norm checks read the round trip's two-byte copy of the
+    # witnesses wherever a tile (128 ring elements) holds only residues below
+    # 2^14 in balanced form.  Here one instance is uniform, two have a single
+    # residue just past the bound (2^14 and -2^14 - 1), one holds only the
+    # extreme short values, and the rest are short: first-pass blocks of one
+    # row take either path.  The step (eager, graph, and phase by phase) must
+    # equal the plain per-call path and the oracle.
+    say("case: batch short copy, mixed magnitudes")
+    m = [4096, 4096, 4096, 512, 256, 2048, 1024, 64]
+    kappa = [8, 16, 32, 8, 16, 8, 32, 8]
+    n = len(m)
+    A_h = orc.sample_uniform(6, sa.STREAM_A, (32, 4096, 64), Q)
+    key = ctx.key(ctx.to_device(A_h))
+    b = sa.Batch(ctx, key, m, kappa, reps=2, fq2=1, tune=False)
+    fill(sa, ctx, b, 500)
+    W0 = [b.slot(i)["witness"].download() for i in range(n)]
+
+    def put(i, flat_idx, value):
+        W0[i].reshape(-1)[flat_idx] = value
+
+    put(0, 3 * (4 * m[0]) + 100, 1 << 14)            # row segment 3, column 100
+    put(3, 7 * (4 * m[3]) + 4 * m[3] - 1, Q - (1 << 14) - 1)
+    W0[1] = orc.sample_uniform(501, sa.STREAM_W, W0[1].shape, Q)
+    pick = np.random.default_rng(9).random(W0[2].shape) < 0.5
+    W0[2] = np.where(pick, (1 << 14) - 1, Q - (1 << 14)).astype(np.uint64)
+    for i in range(n):
+        b.slot(i)["witness"].upload(W0[i])
+    b.step_plain()
+    plain = snapshot(b, True)
+    b.step(graph=False)
+    same(snapshot(b, False), plain)
+    for i in range(n):
+        s = b.slot(i)
+        assert np.array_equal(s["witness"].download(), W0[i]), i
+        Wi = W0[i].reshape(1, m[i], 64)
+        want_c = orc.commit(np.ascontiguousarray(A_h[: kappa[i], : m[i]]), Wi, Q, threads=8)
+        assert np.array_equal(plain[i]["commitment"], want_c), i
+        want = orc.sumcheck(W0[i].reshape(1, m[i] * 64), s["challenges"].download(), Q,
+                            fq2=True, nr=NR, threads=8)
+        for k in ("claims", "msgs", "finals", "evals"):
+            assert np.array_equal(plain[i][k], want[k]), (i, k)
+        assert plain[i]["ok"].all()
+    for rep in range(2):
+        b.step(graph=True)
+        ctx.sync()
+        same(snapshot(b, False), plain)
+        b.step_plain()
+        ctx.sync()
+
+    def clear():
+        for i in range(n):
+            for k in ("msgs", "claims"):
+                t = b.slot(i)[k]
+                t.upload(np.zeros(t.shape, t.download().dtype))
+
+    for seq in (("ntt_roundtrip_planes", "commit", "norm_check"), ("norm_check_witness",)):
+        say("  phases", seq)
+        clear()
+        for name in seq:
+            b.phase(name)
+        ctx.sync()
+        same(snapshot(b, False), plain)
+    b.close()
+    key.close()
