@@ -399,16 +399,20 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
     int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
     double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc,
-    const double *__restrict__ tab, Mod M, const uint32_t *__restrict__ shf) {
+    const double *__restrict__ tab, Mod M, const uint32_t *__restrict__ shf, int sbn) {
     __shared__ __align__(16) uint64_t T[16 * GR_LD];
     __shared__ GrCoef C;
     __shared__ double gpart[4 * 256];
     const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t S = L >> 4;
     const size_t base = wrow ? wrow[row] : (size_t)row * L;
-    size_t xb = (size_t)blockIdx.x * cpt * GR_X;
-    if (xb >= S) return;                                     // whole block past the end
-    if (shf != nullptr && gr_compact(shf, base, S, xb, cpt)) return;  // k_sc_gram_short's
+    // sub-blocks sbn x .. sbn x + sbn - 1 of cpt GR_X columns each; those
+    // that lie in short tiles are k_sc_gram_short's (the set is uniform)
+    const size_t span = (size_t)cpt * GR_X, sb0 = (size_t)blockIdx.x * sbn;
+    uint32_t todo = 0;
+    for (int k = 0; k < sbn && (sb0 + k) * span < S; k++)
+        if (shf == nullptr || !gr_compact(shf, base, S, (sb0 + k) * span, cpt)) todo |= 1u << k;
+    if (todo == 0) return;
     gr_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
     const uint64_t *wr = w + base;
     // the short path's u32 view of the wave's own columns of T: row y at
@@ -420,106 +424,113 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     v4i acc[13];
 #pragma unroll
     for (int k = 0; k < 13; k++) acc[k] = (v4i){0, 0, 0, 0};
-    // Column x = xb + tid.  Each iteration moves its column into the wave's
-    // columns of T and reuses the same registers for the next iteration's 16
-    // loads (unconditional: a dead column re-reads column S - 1 and counts as
-    // zero), which stay in flight while the fold and the matrix products
-    // read this one back from T.
-    uint64_t v[16];
-    {
-        const size_t xc = min(xb + tid, S - 1);
-#pragma unroll
-        for (int y = 0; y < 16; y++) v[y] = wr[(size_t)y * S + xc];
-    }
-    for (int cc = 0;; cc++) {
-        const size_t x = xb + tid;
-        const bool live = x < S;
-        const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform
-        uint32_t bad = 0;
-#pragma unroll
-        for (int y = 0; y < 16; y++) bad |= short_bad(short_t(v[y], half, off_neg));
-        const bool small = __builtin_amdgcn_ballot_w64(live && bad != 0) == 0;  // uniform
-        if (small) {
-#pragma unroll
-            for (int y = 0; y < 16; y++)
-                T32[2 * GR_LD * y + lane] = live ? (uint32_t)short_t(v[y], half, off_neg) : GR_SMALL;
-        } else {
-#pragma unroll
-            for (int y = 0; y < 16; y++) T[y * GR_LD + tid] = live ? v[y] : 0;
-        }
+    for (int k = 0; k < sbn; k++) {
+        if (!((todo >> k) & 1)) continue;
+        size_t xb = (sb0 + k) * span;
+        // Column x = xb + tid.  Each iteration moves its column into the wave's
+        // columns of T and reuses the same registers for the next iteration's 16
+        // loads (unconditional: a dead column re-reads column S - 1 and counts as
+        // zero), which stay in flight while the fold and the matrix products
+        // read this one back from T.
+        uint64_t v[16];
         {
-            const size_t xc = min(more ? x + GR_X : x, S - 1);
+            const size_t xc = min(xb + tid, S - 1);
 #pragma unroll
             for (int y = 0; y < 16; y++) v[y] = wr[(size_t)y * S + xc];
         }
-        wave_sync();
-        // both paths then run the same 49 products (kept unconditional:
-        // accumulators that only one branch updates cost copies instead)
-        v4i f[7];
-        if (small) {
-            if (live) {
-                uint32_t tt[16];
+        for (int cc = 0;; cc++) {
+            const size_t x = xb + tid;
+            const bool live = x < S;
+            const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform
+            uint32_t bad = 0;
 #pragma unroll
-                for (int y = 0; y < 16; y++) tt[y] = T32[2 * GR_LD * y + lane];
-                gr_short_fold<FQ2, false>(C, tt, nrep, rep0, rows, row, S, x, out, M);
+            for (int y = 0; y < 16; y++) bad |= short_bad(short_t(v[y], half, off_neg));
+            const bool small = __builtin_amdgcn_ballot_w64(live && bad != 0) == 0;  // uniform
+            if (small) {
+#pragma unroll
+                for (int y = 0; y < 16; y++)
+                    T32[2 * GR_LD * y + lane] =
+                        live ? (uint32_t)short_t(v[y], half, off_neg) : GR_SMALL;
+            } else {
+#pragma unroll
+                for (int y = 0; y < 16; y++) T[y * GR_LD + tid] = live ? v[y] : 0;
             }
-            if (GRAM) {
-                gr_short_frags(T32, 2 * GR_LD, fr, fg, f[0], f[1]);
+            {
+                const size_t xc = min(more ? x + GR_X : x, S - 1);
 #pragma unroll
-                for (int i = 2; i < 7; i++) f[i] = (v4i){0, 0, 0, 0};
+                for (int y = 0; y < 16; y++) v[y] = wr[(size_t)y * S + xc];
             }
-        } else {
-            if (live) {
-                // products |coef v| <= 0.69 q (balanced coef, canonical v): eight
-                // of them stay below 5.6 q, then two lazy reductions
-                for (int r = 0; r < nrep; r++) {
-                    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+            wave_sync();
+            // both paths then run the same 49 products (kept unconditional:
+            // accumulators that only one branch updates cost copies instead)
+            v4i f[7];
+            if (small) {
+                if (live) {
+                    uint32_t tt[16];
 #pragma unroll
-                    for (int y = 0; y < 8; y++) {
-                        const F2 c0 = C.coef[r][y], c1 = C.coef[r][y + 8];
-                        const double v0 = u2d(T[y * GR_LD + tid]), v1 = u2d(T[(y + 8) * GR_LD + tid]);
-                        a0 += mul_lazy(c0.a, v0, M);
-                        a1 += mul_lazy(c1.a, v1, M);
-                        if (FQ2) {
-                            b0 += mul_lazy(c0.b, v0, M);
-                            b1 += mul_lazy(c1.b, v1, M);
+                    for (int y = 0; y < 16; y++) tt[y] = T32[2 * GR_LD * y + lane];
+                    gr_short_fold<FQ2, false>(C, tt, nrep, rep0, rows, row, S, x, out, M);
+                }
+                if (GRAM) {
+                    gr_short_frags(T32, 2 * GR_LD, fr, fg, f[0], f[1]);
+#pragma unroll
+                    for (int i = 2; i < 7; i++) f[i] = (v4i){0, 0, 0, 0};
+                }
+            } else {
+                if (live) {
+                    // products |coef v| <= 0.69 q (balanced coef, canonical v): eight
+                    // of them stay below 5.6 q, then two lazy reductions
+                    for (int r = 0; r < nrep; r++) {
+                        double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+#pragma unroll
+                        for (int y = 0; y < 8; y++) {
+                            const F2 c0 = C.coef[r][y], c1 = C.coef[r][y + 8];
+                            const double v0 = u2d(T[y * GR_LD + tid]);
+                            const double v1 = u2d(T[(y + 8) * GR_LD + tid]);
+                            a0 += mul_lazy(c0.a, v0, M);
+                            a1 += mul_lazy(c1.a, v1, M);
+                            if (FQ2) {
+                                b0 += mul_lazy(c0.b, v0, M);
+                                b1 += mul_lazy(c1.b, v1, M);
+                            }
                         }
+                        double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
+                        o[0] = reduce_lazy(reduce_lazy(a0, M) + a1, M);
+                        o[1] = FQ2 ? reduce_lazy(reduce_lazy(b0, M) + b1, M) : 0.0;
                     }
-                    double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
-                    o[0] = reduce_lazy(reduce_lazy(a0, M) + a1, M);
-                    o[1] = FQ2 ? reduce_lazy(reduce_lazy(b0, M) + b1, M) : 0.0;
+                }
+                if (GRAM) {
+                    // lane 16 g + r takes digit row (i, r) of the wave's columns
+                    // 16 g .. 16 g + 15: only the wave has to be in step around T
+                    const uint64_t *src = T + fr * GR_LD + 64 * wave + 16 * fg;
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; k4++) {
+                        uint32_t lo[4], hi[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const uint64_t yv = (src[4 * k4 + q] + GR_BIAS) ^ 0x8080808080808080ULL;
+                            lo[q] = (uint32_t)yv;
+                            hi[q] = (uint32_t)(yv >> 32);
+                        }
+                        uint32_t dg[7];
+                        bytes_transpose4(lo, hi, dg);
+#pragma unroll
+                        for (int i = 0; i < 7; i++) f[i][k4] = (int)dg[i];
+                    }
                 }
             }
+            wave_sync();  // T is rewritten by the next iteration
             if (GRAM) {
-                // lane 16 g + r takes digit row (i, r) of the wave's columns
-                // 16 g .. 16 g + 15: only the wave has to be in step around T
-                const uint64_t *src = T + fr * GR_LD + 64 * wave + 16 * fg;
 #pragma unroll
-                for (int k4 = 0; k4 < 4; k4++) {
-                    uint32_t lo[4], hi[4];
+                for (int i = 0; i < 7; i++)
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const uint64_t yv = (src[4 * k4 + q] + GR_BIAS) ^ 0x8080808080808080ULL;
-                        lo[q] = (uint32_t)yv;
-                        hi[q] = (uint32_t)(yv >> 32);
-                    }
-                    uint32_t dg[7];
-                    bytes_transpose4(lo, hi, dg);
-#pragma unroll
-                    for (int i = 0; i < 7; i++) f[i][k4] = (int)dg[i];
-                }
+                    for (int j = 0; j < 7; j++)
+                        acc[i + j] =
+                            __builtin_amdgcn_mfma_i32_16x16x64_i8(f[i], f[j], acc[i + j], 0, 0, 0);
             }
+            if (!more) break;
+            xb += GR_X;
         }
-        wave_sync();  // T is rewritten by the next iteration
-        if (GRAM) {
-#pragma unroll
-            for (int i = 0; i < 7; i++)
-#pragma unroll
-                for (int j = 0; j < 7; j++)
-                    acc[i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f[i], f[j], acc[i + j], 0, 0, 0);
-        }
-        if (!more) break;
-        xb += GR_X;
     }
     if (GRAM) gr_finish<13>(acc, gpart, wave, lane, tid, tab, gacc, row, M);
 }
@@ -532,14 +543,18 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_short(
     const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
     const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr, int cpt,
     double *__restrict__ out, unsigned long long *__restrict__ gacc, const double *__restrict__ tab,
-    Mod M, const uint16_t *__restrict__ sht, const uint32_t *__restrict__ shf) {
+    Mod M, const uint16_t *__restrict__ sht, const uint32_t *__restrict__ shf, int sbn) {
     __shared__ __align__(16) uint32_t T32all[16 * GS_LD];  // then the Gram partials
     __shared__ GrCoef C;
     const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t S = L >> 4;
     const size_t base = wrow ? wrow[row] : (size_t)row * L;
-    size_t xb = (size_t)blockIdx.x * cpt * GR_X;
-    if (xb >= S || !gr_compact(shf, base, S, xb, cpt)) return;  // uniform
+    // the sub-blocks (as in k_sc_gram) that lie in short tiles (uniform)
+    const size_t span = (size_t)cpt * GR_X, sb0 = (size_t)blockIdx.x * sbn;
+    uint32_t todo = 0;
+    for (int k = 0; k < sbn && (sb0 + k) * span < S; k++)
+        if (gr_compact(shf, base, S, (sb0 + k) * span, cpt)) todo |= 1u << k;
+    if (todo == 0) return;
     gr_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
     uint32_t *T32 = T32all + 64 * wave;
     const int fr = lane & 15, fg = lane >> 4;
@@ -552,35 +567,39 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_short(
 #pragma unroll
         for (int y = 0; y < 16; y++) t[y] = (tr + (size_t)y * S)[xo];
     };
-    uint32_t tv[16];
-    fetch(tv, xb + tid);
-    for (int cc = 0;; cc++) {
-        const size_t x = xb + tid;
-        const bool live = x < S;
-        const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform
+    for (int k = 0; k < sbn; k++) {
+        if (!((todo >> k) & 1)) continue;
+        size_t xb = (sb0 + k) * span;
+        uint32_t tv[16];
+        fetch(tv, xb + tid);
+        for (int cc = 0;; cc++) {
+            const size_t x = xb + tid;
+            const bool live = x < S;
+            const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform
 #pragma unroll
-        for (int y = 0; y < 16; y++) T32[GS_LD * y + lane] = live ? tv[y] : GR_SMALL;
-        fetch(tv, more ? x + GR_X : x);
-        wave_sync();
-        if (live) {
-            uint32_t tt[16];
-#pragma unroll
-            for (int y = 0; y < 16; y++) tt[y] = T32[GS_LD * y + lane];
-            gr_short_fold<FQ2, true>(C, tt, nrep, rep0, rows, row, S, x, out, M);
-        }
-        if (GRAM) {
-            v4i f0, f1;
-            gr_short_frags(T32, GS_LD, fr, fg, f0, f1);
-            wave_sync();  // T32 is rewritten by the next iteration
-            acc[0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f0, f0, acc[0], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f0, f1, acc[1], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f1, f0, acc[1], 0, 0, 0);
-            acc[2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f1, f1, acc[2], 0, 0, 0);
-        } else {
+            for (int y = 0; y < 16; y++) T32[GS_LD * y + lane] = live ? tv[y] : GR_SMALL;
+            fetch(tv, more ? x + GR_X : x);
             wave_sync();
+            if (live) {
+                uint32_t tt[16];
+#pragma unroll
+                for (int y = 0; y < 16; y++) tt[y] = T32[GS_LD * y + lane];
+                gr_short_fold<FQ2, true>(C, tt, nrep, rep0, rows, row, S, x, out, M);
+            }
+            if (GRAM) {
+                v4i f0, f1;
+                gr_short_frags(T32, GS_LD, fr, fg, f0, f1);
+                wave_sync();  // T32 is rewritten by the next iteration
+                acc[0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f0, f0, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f0, f1, acc[1], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f1, f0, acc[1], 0, 0, 0);
+                acc[2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f1, f1, acc[2], 0, 0, 0);
+            } else {
+                wave_sync();
+            }
+            if (!more) break;
+            xb += GR_X;
         }
-        if (!more) break;
-        xb += GR_X;
     }
     if (GRAM) {
         __syncthreads();  // the partials reuse the staging rows
@@ -948,12 +967,20 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
     }
     {
         // one streaming pass: the folds after round 3 and the Gram matrix,
-        // then rounds 0..3's messages from it (at most 8 columns of 256 per
-        // block keeps every wave's int32 digit sums exact)
+        // then rounds 0..3's messages from it
         const size_t S = len1;
         const int cpt = cpt_first > 0 ? std::min(cpt_first, 8)
                                       : (S >= (size_t)GR_X * 64 ? 4 : 1);
-        const unsigned bx = ceil_div(S, (size_t)GR_X * cpt);
+        const size_t nsub = ceil_div(S, (size_t)GR_X * cpt);
+        // With a short copy, a block takes up to 8 sub-blocks of cpt GR_X
+        // columns (each kernel skips the other's): the fewer, longer blocks
+        // amortise the skipped blocks and the per-block setup.  A wave then
+        // sums at most 8 x 8 x 64 columns: its int32 digit sums stay exact.
+        int sbn = 1;
+        if (sc)
+            for (sbn = 8; sbn > 1 && ceil_div(nsub, (size_t)sbn) * rows < (size_t)ctx->ncu * 4;)
+                sbn >>= 1;
+        const unsigned bx = (unsigned)ceil_div(nsub, (size_t)sbn);
         unsigned long long *gacc = cacc + rows;
         dim3 grid(bx, (unsigned)rows);
         for (size_t r0 = 0; r0 < reps; r0 += GR_MAXREP) {
@@ -963,19 +990,19 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
             if (sc && r0 == 0)
                 hipLaunchKernelGGL((k_sc_gram_short<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf);
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn);
             else if (sc)
                 hipLaunchKernelGGL((k_sc_gram_short<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf);
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn);
             if (r0 == 0)
                 hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0, ctx->stream,
                                    w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr, cpt, bufA,
-                                   gacc, ctx->d_tables, M, shf);
+                                   gacc, ctx->d_tables, M, shf, sbn);
             else
                 hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0, ctx->stream,
                                    w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr, cpt, bufA,
-                                   gacc, ctx->d_tables, M, shf);
+                                   gacc, ctx->d_tables, M, shf, sbn);
             SALSAA_HIP(hipGetLastError());
         }
         hipLaunchKernelGGL((k_sc_gram_msgs<FQ2, SNR>), dim3((unsigned)rows, (unsigned)reps), dim3(128),
