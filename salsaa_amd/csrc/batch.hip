@@ -101,16 +101,25 @@ struct salsaa_batch {
     // norm checks of the same step (its tile flags start at "not short")
     ShortCopy sc;
     // commit
-    size_t njobs = 0, max_tiles = 0, kchunk = 0;
+    // Jobs of 16 instances in commit order: instances grouped by the key row
+    // tiles their kappa needs (at most 16, 8 or 4: one kernel shape each), in
+    // length order within a group; cpos[p] is the commit position of arena
+    // position p, and commitments are stored in commit order.
+    struct CommitClass {
+        size_t kappa = 0, job0 = 0, njobs = 0, max_tiles = 0, t0 = 0, nblk = 0, fnblk = 0;
+    };
+    std::vector<CommitClass> cclass;
+    std::vector<size_t> cpos;
+    size_t njobs = 0, kchunk = 0;
     std::vector<int> job_ktiles, job_rows;
     std::vector<size_t> job_planes;  // byte offset of each job's planes
+    std::vector<size_t> job_cfirst;  // commit position of each job's first row
     double *d_partial = nullptr;
     size_t partial_cap = 0;  // doubles
     CommitJob *d_cjobs = nullptr;
     uint2 *d_table = nullptr;
-    size_t table_cap = 0, cnblk = 0;
+    size_t table_cap = 0;
     FinJob *d_fjobs = nullptr;
-    size_t fnblk = 0;
     // norm check
     struct Group {
         size_t first, rows, L;
@@ -153,52 +162,56 @@ void free_outputs(Outputs &o) {
     o = Outputs();
 }
 
-// Commit job tables for K chunk kc: block order (z, s, job) with the jobs
-// active at chunk z (a prefix, lengths being sorted) fastest.
+// Commit job tables for K chunk kc, per class: block order (z, s, job) with
+// the jobs active at chunk z (a prefix, lengths being sorted) fastest.
 int build_commit_tables(salsaa_batch *b, size_t kc) {
     Context *ctx = &b->ctx->c;
     std::vector<CommitJob> cj(b->njobs);
     std::vector<FinJob> fj(b->njobs);
-    std::vector<int> ks(b->njobs);
-    size_t poff = 0, fblk = 0;
-    int maxks = 0;
-    for (size_t j = 0; j < b->njobs; j++) {
-        ks[j] = (int)((b->job_ktiles[j] + kc - 1) / kc);
-        maxks = std::max(maxks, ks[j]);
-        CommitJob &c = cj[j];
-        c.WB = b->d_planes + b->job_planes[j];
-        c.ktiles = b->job_ktiles[j];
-        c.rows = b->job_rows[j];
-        c.rtiles = (7 * c.rows + 15) / 16;
-        c.ntile = c.rtiles;
-        c.partial = b->d_partial + poff;
-        FinJob &f = fj[j];
-        f.partial = c.partial;
-        f.out = nullptr;  // set per output set
-        f.osj = kD;
-        f.osk = (long long)(b->krows * kD);
-        f.ks = ks[j];
-        f.rows = c.rows;
-        f.blk0 = (int)fblk;
-        poff += (size_t)ks[j] * 64 * b->krows * c.rows;
-        fblk += b->krows * (size_t)c.rows;
+    std::vector<uint2> table;
+    size_t poff = 0;
+    for (auto &cc : b->cclass) {
+        std::vector<int> ks(cc.njobs);
+        int maxks = 0;
+        size_t fblk = 0;
+        for (size_t jj = 0; jj < cc.njobs; jj++) {
+            const size_t j = cc.job0 + jj;
+            ks[jj] = (int)((b->job_ktiles[j] + kc - 1) / kc);
+            maxks = std::max(maxks, ks[jj]);
+            CommitJob &c = cj[j];
+            c.WB = b->d_planes + b->job_planes[j];
+            c.ktiles = b->job_ktiles[j];
+            c.rows = b->job_rows[j];
+            c.rtiles = (7 * c.rows + 15) / 16;
+            c.ntile = c.rtiles;
+            c.partial = b->d_partial + poff;
+            FinJob &f = fj[j];
+            f.partial = c.partial;
+            f.out = b->out[0].com + b->job_cfirst[j] * b->krows * kD;
+            f.osj = kD;
+            f.osk = (long long)(b->krows * kD);
+            f.ks = ks[jj];
+            f.rows = c.rows;
+            f.blk0 = (int)fblk;
+            poff += (size_t)ks[jj] * 64 * cc.kappa * c.rows;
+            fblk += cc.kappa * (size_t)c.rows;
+        }
+        cc.t0 = table.size();
+        for (int z = 0; z < maxks; z++) {
+            unsigned active = 0;
+            while (active < cc.njobs && ks[active] > z) active++;
+            for (unsigned s = 0; s < 64; s++)
+                for (unsigned j = 0; j < active; j++)
+                    table.push_back(make_uint2(j, ((unsigned)z << 6) | s));
+        }
+        cc.nblk = table.size() - cc.t0;
+        cc.fnblk = fblk;
     }
     SALSAA_CHECK(poff <= b->partial_cap, "commit partial buffer too small");
-    std::vector<uint2> table;
-    for (int z = 0; z < maxks; z++) {
-        unsigned active = 0;
-        while (active < b->njobs && ks[active] > z) active++;
-        for (unsigned s = 0; s < 64; s++)
-            for (unsigned j = 0; j < active; j++) table.push_back(make_uint2(j, ((unsigned)z << 6) | s));
-    }
     SALSAA_CHECK(table.size() <= b->table_cap, "commit table too small");
     SALSAA_TRY(upload(ctx, b->d_cjobs, cj));
     SALSAA_TRY(upload(ctx, b->d_table, table));
-    // finalize jobs for both output sets: [0, njobs) batched, [njobs, 2 njobs) unused
-    for (size_t j = 0; j < b->njobs; j++) fj[j].out = b->out[0].com + (16 * j) * b->krows * kD;
     SALSAA_TRY(upload(ctx, b->d_fjobs, fj));
-    b->cnblk = table.size();
-    b->fnblk = fblk;
     b->kchunk = kc;
     return SALSAA_OK;
 }
@@ -208,12 +221,17 @@ int run_planes(salsaa_batch *b) {
                                    b->pjobs.size(), b->pnblk);
 }
 
+// one commit launch per class (its kappa and kernel shape), then one finalize each
 int run_commit(salsaa_batch *b) {
     const Key &k = b->key->k;
-    SALSAA_TRY(launch_commit_mfma_batch(&b->ctx->c, k.d_AB, b->krows, k.a_kt, k.a_ktiles,
-                                        b->d_cjobs, b->d_table, b->cnblk, b->kchunk,
-                                        b->max_tiles));
-    return launch_commit_finalize_batch(&b->ctx->c, b->d_fjobs, b->njobs, b->fnblk, b->krows);
+    for (const auto &cc : b->cclass)
+        SALSAA_TRY(launch_commit_mfma_batch(&b->ctx->c, k.d_AB, cc.kappa, k.a_kt, k.a_ktiles,
+                                            b->d_cjobs + cc.job0, b->d_table + cc.t0, cc.nblk,
+                                            b->kchunk, cc.max_tiles));
+    for (const auto &cc : b->cclass)
+        SALSAA_TRY(launch_commit_finalize_batch(&b->ctx->c, b->d_fjobs + cc.job0, cc.njobs,
+                                                cc.fnblk, cc.kappa));
+    return SALSAA_OK;
 }
 
 // Enqueues the norm checks on their streams after the work already on the
@@ -430,30 +448,55 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
     SALSAA_HIP(hipGetLastError());
     SALSAA_TRY(alloc_outputs(b, b->out[0]));
 
-    // row blocks of 16 instances: planes and commit jobs
-    b->njobs = (n + 15) / 16;
+    // commit jobs: per class of key row tiles (16, 8, 4), in length order,
+    // 16 instances each; planes per job, and the planes batch reads the
+    // arena rows in commit order
+    std::vector<unsigned long long> hm(n);  // length per arena position
+    for (size_t p = 0; p < n; p++) hm[p] = b->m[b->order[p]];
+    auto key_tiles = [](size_t kap) { return (7 * kap + 15) / 16; };
+    std::vector<size_t> corder;  // commit order -> arena position
+    b->cpos.resize(n);
     size_t planes_bytes = 0;
-    std::vector<unsigned long long> hoff(n), hm(n);
-    for (size_t p = 0; p < n; p++) {
-        hoff[p] = b->off_w[p];
-        hm[p] = b->m[b->order[p]];
+    for (size_t lim : {(size_t)16, (size_t)8, (size_t)4}) {
+        salsaa_batch::CommitClass cc;
+        const size_t c0 = corder.size();
+        for (size_t p = 0; p < n; p++) {
+            const size_t rt = key_tiles(b->kappa[b->order[p]]);
+            if (rt <= lim && (lim == 4 || rt > lim / 2)) {
+                b->cpos[p] = corder.size();
+                corder.push_back(p);
+                cc.kappa = std::max(cc.kappa, b->kappa[b->order[p]]);
+            }
+        }
+        const size_t cnt = corder.size() - c0;
+        if (cnt == 0) continue;
+        cc.job0 = b->njobs;
+        cc.njobs = (cnt + 15) / 16;
+        for (size_t jj = 0; jj < cc.njobs; jj++) {
+            PlaneJob pj;
+            pj.first = (int)(c0 + 16 * jj);
+            pj.rows = (int)std::min<size_t>(16, cnt - 16 * jj);
+            pj.ktiles = (int)plane_ktiles(hm[corder[pj.first]]);
+            pj.blk0 = (int)b->pnblk;
+            b->job_planes.push_back(planes_bytes);
+            b->job_cfirst.push_back((size_t)pj.first);
+            pj.dst = nullptr;
+            b->pjobs.push_back(pj);
+            b->job_ktiles.push_back(pj.ktiles);
+            b->job_rows.push_back(pj.rows);
+            const size_t rt = (7 * (size_t)pj.rows + 15) / 16;
+            cc.max_tiles = std::max(cc.max_tiles, rt);
+            planes_bytes += (size_t)64 * pj.ktiles * rt * 1024;
+            b->pnblk += (size_t)pj.rows * pj.ktiles / 2;
+            SALSAA_CHECK((size_t)pj.ktiles <= key->k.a_ktiles, "witness longer than the key planes");
+        }
+        b->njobs += cc.njobs;
+        b->cclass.push_back(cc);
     }
-    for (size_t j = 0; j < b->njobs; j++) {
-        PlaneJob pj;
-        pj.first = (int)(16 * j);
-        pj.rows = (int)std::min<size_t>(16, n - 16 * j);
-        pj.ktiles = (int)plane_ktiles(hm[16 * j]);
-        pj.blk0 = (int)b->pnblk;
-        b->job_planes.push_back(planes_bytes);
-        pj.dst = nullptr;
-        b->pjobs.push_back(pj);
-        b->job_ktiles.push_back(pj.ktiles);
-        b->job_rows.push_back(pj.rows);
-        const size_t rt = (7 * (size_t)pj.rows + 15) / 16;
-        b->max_tiles = std::max(b->max_tiles, rt);
-        planes_bytes += (size_t)64 * pj.ktiles * rt * 1024;
-        b->pnblk += (size_t)pj.rows * pj.ktiles / 2;
-        SALSAA_CHECK((size_t)pj.ktiles <= key->k.a_ktiles, "witness longer than the key planes");
+    std::vector<unsigned long long> hoff(n), hmc(n);  // in commit order
+    for (size_t c2 = 0; c2 < n; c2++) {
+        hoff[c2] = b->off_w[corder[c2]];
+        hmc[c2] = hm[corder[c2]];
     }
     SALSAA_CHECK(b->pnblk < (1u << 31), "batch too large");
     SALSAA_TRY(dmalloc(&b->d_planes, planes_bytes));
@@ -465,11 +508,15 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
     SALSAA_HIP(hipGetLastError());
     for (size_t j = 0; j < b->njobs; j++) b->pjobs[j].dst = b->d_planes + b->job_planes[j];
     // each arena tile's place in the planes (see store_planes): value row
-    // p % 16 of row block p / 16, K tiles from its offset in the instance
+    // of its instance in its job, K tiles from its offset in the instance
+    std::vector<size_t> job_of(n);
+    for (size_t j = 0; j < b->njobs; j++)
+        for (int r = 0; r < b->job_rows[j]; r++) job_of[b->job_cfirst[j] + r] = j;
     std::vector<PlaneTile> pt(b->tot_w / kPlaneTile);
     std::vector<unsigned long long> hwrow(n);
     for (size_t p = 0; p < n; p++) {
-        const size_t j = p / 16, row = p % 16, rt = (7 * (size_t)b->job_rows[j] + 15) / 16;
+        const size_t j = job_of[b->cpos[p]], row = b->cpos[p] - b->job_cfirst[j];
+        const size_t rt = (7 * (size_t)b->job_rows[j] + 15) / 16;
         const size_t kstride = rt * 1024, sstride = (size_t)b->job_ktiles[j] * kstride;
         SALSAA_CHECK(sstride < (1ull << 32), "row block too large for the plane tile table");
         hwrow[p] = b->off_w[p] * kD;
@@ -490,14 +537,15 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
     SALSAA_TRY(dmalloc(&b->d_m, n));
     SALSAA_TRY(upload(c, b->d_pjobs, b->pjobs));
     SALSAA_TRY(upload(c, b->d_off, hoff));
-    SALSAA_TRY(upload(c, b->d_m, hm));
+    SALSAA_TRY(upload(c, b->d_m, hmc));
     // commit buffers sized for the smallest K chunk tried (16 tiles)
     size_t pcap = 0, tcap = 0;
-    for (size_t j = 0; j < b->njobs; j++) {
-        const size_t ks = (b->job_ktiles[j] + 15) / 16;
-        pcap += ks * 64 * b->krows * b->job_rows[j];
-        tcap += ks * 64;
-    }
+    for (const auto &cc : b->cclass)
+        for (size_t j = cc.job0; j < cc.job0 + cc.njobs; j++) {
+            const size_t ks = (b->job_ktiles[j] + 15) / 16;
+            pcap += ks * 64 * cc.kappa * b->job_rows[j];
+            tcap += ks * 64;
+        }
     b->partial_cap = pcap;
     b->table_cap = tcap;
     SALSAA_TRY(dmalloc(&b->d_partial, pcap));
@@ -585,7 +633,7 @@ int salsaa_batch_slots(salsaa_batch *b, size_t i, int plain, salsaa_slot_t *s) {
     const size_t p = b->pos[i];
     s->witness = b->d_wit + b->off_w[p] * kD;
     s->challenges = b->d_cs + b->off_c[p];
-    s->commitment = o.com + p * b->krows * kD;
+    s->commitment = o.com + b->cpos[p] * b->krows * kD;
     s->claims = o.claims + p;
     s->msgs = o.msgs + b->off_msg[p];
     s->finals = o.fin + p * b->reps * 2;
@@ -651,7 +699,7 @@ int salsaa_batch_step_plain(salsaa_batch *b) {
         SALSAA_TRY(salsaa_convert(ctx, w, b->m[i], SALSAA_REP_COEFF, SALSAA_REP_INCOMPLETE));
         SALSAA_TRY(salsaa_convert(ctx, w, b->m[i], SALSAA_REP_INCOMPLETE, SALSAA_REP_COEFF));
         SALSAA_TRY(salsaa_parallel_dot_series_matrix(ctx, b->views[i], w, 1, b->m[i],
-                                                     o.com + p * b->krows * kD));
+                                                     o.com + b->cpos[p] * b->krows * kD));
         SALSAA_TRY(salsaa_norm_check(ctx, w, 1, b->m[i] * kD, b->d_cs + b->off_c[p], b->reps,
                                      b->fq2, b->nr, o.claims + p, o.msgs + b->off_msg[p],
                                      o.fin + p * b->reps * 2, o.ev + p * b->reps * 2,
