@@ -185,6 +185,24 @@ int salsaa_parallel_dot_series_matrix(salsaa_ctx *ctx, const salsaa_key *key,
                                       const uint64_t *d_W, size_t r, size_t m,
                                       uint64_t *d_out);
 
+/* complete_power_series (arithmetic.rs:158-176) with kronecker_product
+ * (arithmetic.rs:845-864): the largest expanded layer of a power series with
+ * base layer d_base [m0][d] and length-2 tensors d_tensors [n][2][d]
+ * (IncompletePowerSeries order: tensors[0] is the most significant factor),
+ *   d_out[((i_0 2 + i_1) 2 + ... + i_{n-1}) m0 + j] = t_0[i_0] ... t_{n-1}[i_{n-1}] base[j],
+ * [m0 << n][d].  Inputs and output in Coefficient form (the reference's
+ * products carry IncompleteNTT tags; the ring elements are the same). */
+int salsaa_complete_power_series(salsaa_ctx *ctx, const uint64_t *d_base, size_t m0,
+                                 const uint64_t *d_tensors, size_t n, uint64_t *d_out);
+
+/* parallel_dot_single_series_matrix (arithmetic.rs:507-536): out[j] =
+ * sum_i layer[i] * W[j][i] for the expanded layer of length m (the one the
+ * reference finds by length); d_layer [m][d], d_W [r][m][d] Coefficient,
+ * d_out [r][d] IncompleteNTT.  The commit with the layer as its only key row
+ * (a key is made and freed per call). */
+int salsaa_parallel_dot_single_series_matrix(salsaa_ctx *ctx, const uint64_t *d_layer, size_t m,
+                                             const uint64_t *d_W, size_t r, uint64_t *d_out);
+
 /* split (split.rs:45-76): the rows [L_0..L_{r-1}, R_0..R_{r-1}] of the halves
  * of W [r][m][d] (a view, not copied) are committed under the length-m/2
  * prefix: d_out [kappa][2r][d] IncompleteNTT. */

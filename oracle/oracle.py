@@ -37,6 +37,8 @@ def _load():
         "or_min_primitive_root": (u64, [u64, u64]),
         "or_get_roots": (None, [sz, u64, P]),
         "or_get_inv_roots": (None, [sz, u64, P]),
+        "or_complete_power_series": (None, [P, sz, P, sz, sz, u64, P]),
+        "or_dot_single_series": (None, [P, P, P, sz, sz, sz, u64]),
         "or_ntt_forward": (None, [P, sz, u64]),
         "or_ntt_inverse": (None, [P, sz, u64]),
         "or_ntt_forward_direct": (None, [P, P, sz, u64]),
@@ -180,6 +182,31 @@ def commit(A, W, q, threads=0):
     assert W.shape == (r, m, d)
     out = np.zeros((kappa, r, d), np.uint64)
     lib.or_commit(_p(A), _p(W), _p(out), kappa, r, m, d, q, threads)
+    return out
+
+
+def complete_power_series(base, tensors, q):
+    """complete_power_series (arithmetic.rs:158-176): base [m0][d], tensors
+    [n][2][d] (IncompletePowerSeries order), Coefficient -> the largest
+    expanded layer [m0 << n][d], Coefficient."""
+    base, tensors = _u(base), _u(tensors)
+    m0, d = base.shape
+    n = tensors.shape[0]
+    assert tensors.shape == (n, 2, d)
+    out = np.zeros((m0 << n, d), np.uint64)
+    lib.or_complete_power_series(_p(base), m0, _p(tensors), n, d, q, _p(out))
+    return out
+
+
+def dot_single_series(layer, W, q):
+    """parallel_dot_single_series_matrix (arithmetic.rs:507-536): layer [m][d],
+    W [r][m][d] Coefficient -> [r][d] IncompleteNTT."""
+    layer, W = _u(layer), _u(W)
+    m, d = layer.shape
+    r = W.shape[0]
+    assert W.shape == (r, m, d)
+    out = np.zeros((r, d), np.uint64)
+    lib.or_dot_single_series(_p(layer), _p(W), _p(out), r, m, d, q)
     return out
 
 

@@ -368,6 +368,62 @@ void or_commit(const uint64_t *A, const uint64_t *W, uint64_t *out, size_t kappa
 }
 
 /* ------------------------------------------------------------------------ */
+/* power series                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* complete_power_series (arithmetic.rs:158-176) from kronecker_product
+ * (arithmetic.rs:845-864: out[i |b| + j] = a[i] * b[j]), applying the tensors
+ * last first as the reference does.  With length-2 tensors t_0 .. t_{n-1}
+ * (IncompletePowerSeries order) the largest expanded layer is
+ *   out[((i_0 2 + i_1) 2 + ... + i_{n-1}) m0 + j] = t_0[i_0] ... t_{n-1}[i_{n-1}] base[j],
+ * the first tensor the most significant index bit.  Ring products are the
+ * schoolbook product mod X^d + 1 (cyclotomic_ring.rs:816-839), everything in
+ * Coefficient form.  base [m0][d], tensors [n][2][d], out [m0 << n][d]. */
+void or_complete_power_series(const uint64_t *base, size_t m0, const uint64_t *tensors, size_t n,
+                              size_t d, uint64_t q, uint64_t *out) {
+    const size_t full = m0 << n;
+    uint64_t *cur = (uint64_t *)malloc(full * d * sizeof(uint64_t));
+    uint64_t *nxt = (uint64_t *)malloc(full * d * sizeof(uint64_t));
+    memcpy(cur, base, m0 * d * sizeof(uint64_t));
+    size_t len = m0;
+    for (size_t k = n; k-- > 0;) {
+        const uint64_t *t = tensors + k * 2 * d;
+        for (size_t a = 0; a < 2; a++)
+            for (size_t j = 0; j < len; j++)
+                or_ring_mul_naive(t + a * d, cur + j * d, nxt + (a * len + j) * d, d, q);
+        uint64_t *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+        len *= 2;
+    }
+    memcpy(out, cur, full * d * sizeof(uint64_t));
+    free(cur);
+    free(nxt);
+}
+
+/* parallel_dot_single_series_matrix (arithmetic.rs:507-536): out[j] =
+ * sum_i layer[i] * W[j][i] with the expanded layer of length m.  The
+ * reference's products leave IncompleteNTT form (cyclotomic_ring.rs:193-198)
+ * and the sum adopts it; here the products are schoolbook ones summed in
+ * Coefficient form and converted once (the conversion is linear), a route
+ * independent of or_commit.  layer [m][d], W [r][m][d] Coefficient,
+ * out [r][d] IncompleteNTT. */
+void or_dot_single_series(const uint64_t *layer, const uint64_t *W, uint64_t *out, size_t r,
+                          size_t m, size_t d, uint64_t q) {
+    uint64_t *prod = (uint64_t *)malloc(d * sizeof(uint64_t));
+    for (size_t j = 0; j < r; j++) {
+        uint64_t *acc = out + j * d;
+        memset(acc, 0, d * sizeof(uint64_t));
+        for (size_t i = 0; i < m; i++) {
+            or_ring_mul_naive(layer + i * d, W + (j * m + i) * d, prod, d, q);
+            for (size_t c = 0; c < d; c++) acc[c] = or_addmod(acc[c], prod[c], q);
+        }
+        or_to_incomplete(acc, d, q);
+    }
+    free(prod);
+}
+
+/* ------------------------------------------------------------------------ */
 /* norm check                                                                */
 /* ------------------------------------------------------------------------ */
 

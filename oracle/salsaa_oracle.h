@@ -35,6 +35,12 @@
  *   or_decompose/or_compose               : decomp.rs:78-92 invariant, digit
  *                                           definition brute force
  *   or_fold                               : fold.rs:36-55 homomorphism
+ *   or_complete_power_series              : kronecker_product example of
+ *                                           arithmetic.rs:866-872 (constants),
+ *                                           all-ones tensors repeat the base,
+ *                                           brute-force index formula
+ *   or_dot_single_series                  : equals or_commit with kappa = 1
+ *                                           (slot products vs schoolbook)
  *   or_get_inv_roots                      : parity unpinned (no reference test
  *                                           reads HEXL's inverse table layout)
  */
@@ -88,6 +94,14 @@ void or_conjugate(const uint64_t *a, uint64_t *out, size_t d, uint64_t q);
  * IncompleteNTT.  Follows the reference's per-product conversion. */
 void or_commit(const uint64_t *A, const uint64_t *W, uint64_t *out, size_t kappa,
                size_t r, size_t m, size_t d, uint64_t q, int threads);
+/* complete_power_series / kronecker_product (arithmetic.rs:158-176, 845-864),
+ * Coefficient form: base [m0][d], tensors [n][2][d] -> out [m0 << n][d] */
+void or_complete_power_series(const uint64_t *base, size_t m0, const uint64_t *tensors, size_t n,
+                              size_t d, uint64_t q, uint64_t *out);
+/* parallel_dot_single_series_matrix (arithmetic.rs:507-536): layer [m][d],
+ * W [r][m][d] Coefficient -> out [r][d] IncompleteNTT */
+void or_dot_single_series(const uint64_t *layer, const uint64_t *W, uint64_t *out, size_t r,
+                          size_t m, size_t d, uint64_t q);
 
 /* ---- norm check (fast_norm_check.rs) ---- */
 void or_tensor(const uint64_t *cs, size_t n, uint64_t q, uint64_t *out);

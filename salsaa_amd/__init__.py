@@ -108,6 +108,8 @@ def _load():
         "salsaa_sample_challenges": (_int, [_vp, _vp, _sz, _sz, _int, _u64]),
         "salsaa_convert": (_int, [_vp, _vp, _sz, _int, _int]),
         "salsaa_ring_mul": (_int, [_vp, _vp, _vp, _vp, _sz]),
+        "salsaa_complete_power_series": (_int, [_vp, _vp, _sz, _vp, _sz, _vp]),
+        "salsaa_parallel_dot_single_series_matrix": (_int, [_vp, _vp, _sz, _vp, _sz, _vp]),
         "salsaa_key_create": (_int, [_vp, _vp, _sz, _sz, _PP]),
         "salsaa_key_destroy": (_int, [_vp]),
         "salsaa_parallel_dot_series_matrix": (_int, [_vp, _vp, _vp, _sz, _sz, _vp]),
@@ -304,6 +306,28 @@ class Context:
     def ring_mul(self, a, b):
         out = self.empty(a.shape)
         _check(lib.salsaa_ring_mul(self.h, a.ptr, b.ptr, out.ptr, int(np.prod(a.shape[:-1]))))
+        return out
+
+    def complete_power_series(self, base, tensors):
+        """The largest layer of the power series with base [m0][d] and length-2
+        tensors [n][2][d] (complete_power_series, arithmetic.rs:158-176),
+        Coefficient form: [m0 << n][d]."""
+        m0, d = base.shape
+        n = tensors.shape[0]
+        assert d == self.d and tensors.shape == (n, 2, d)
+        out = self.empty((m0 << n, d))
+        _check(lib.salsaa_complete_power_series(self.h, base.ptr, m0, tensors.ptr, n, out.ptr))
+        return out
+
+    def parallel_dot_single_series_matrix(self, layer, W):
+        """out[j] = sum_i layer[i] W[j][i] (arithmetic.rs:507-536): layer [m][d],
+        W [r][m][d] Coefficient -> [r][d] IncompleteNTT."""
+        m, d = layer.shape
+        r = W.shape[0]
+        assert d == self.d and W.shape == (r, m, d)
+        out = self.empty((r, d))
+        _check(lib.salsaa_parallel_dot_single_series_matrix(self.h, layer.ptr, m, W.ptr, r,
+                                                            out.ptr))
         return out
 
     def key(self, A_dev):

@@ -441,6 +441,36 @@ int salsaa_parallel_dot_series_matrix(salsaa_ctx *ctx, const salsaa_key *key,
     return commit_rows(ctx, key, d_W, r, m, r, m * kD, 0, d_out);
 }
 
+int salsaa_complete_power_series(salsaa_ctx *ctx, const uint64_t *d_base, size_t m0,
+                                 const uint64_t *d_tensors, size_t n, uint64_t *d_out) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(m0 >= 1 && n <= 40 && ((m0 << n) >> n) == m0, "power series size out of range");
+    // copies of the base and the tensors in NTT form, where each product is pointwise
+    uint64_t *tmp = nullptr;
+    const size_t nin = m0 + 2 * n;
+    SALSAA_TRY(scratch_get(&ctx->c, 9, nin * kD * sizeof(uint64_t), (void **)&tmp));
+    SALSAA_HIP(hipMemcpyAsync(tmp, d_base, m0 * kD * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                              ctx->c.stream));
+    if (n)
+        SALSAA_HIP(hipMemcpyAsync(tmp + m0 * kD, d_tensors, 2 * n * kD * sizeof(uint64_t),
+                                  hipMemcpyDeviceToDevice, ctx->c.stream));
+    SALSAA_TRY(launch_ntt(&ctx->c, tmp, nin, SALSAA_REP_FULL, 0));
+    SALSAA_TRY(launch_series_expand(&ctx->c, tmp, m0, tmp + m0 * kD, n, d_out));
+    return launch_ntt(&ctx->c, d_out, m0 << n, SALSAA_REP_FULL, 1);
+}
+
+int salsaa_parallel_dot_single_series_matrix(salsaa_ctx *ctx, const uint64_t *d_layer, size_t m,
+                                             const uint64_t *d_W, size_t r, uint64_t *d_out) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(m >= 1 && r >= 1, "inner product needs m, r >= 1");
+    // the commit with the layer as the only key row
+    salsaa_key *key = nullptr;
+    SALSAA_TRY(salsaa_key_create(ctx, d_layer, 1, m, &key));
+    const int st = salsaa_parallel_dot_series_matrix(ctx, key, d_W, r, m, d_out);
+    const int st2 = salsaa_key_destroy(key);
+    return st != SALSAA_OK ? st : st2;
+}
+
 int salsaa_split(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_W, size_t r, size_t m,
                  uint64_t *d_out) {
     CTX_OK(ctx);

@@ -266,6 +266,10 @@ inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b
 
 // entry points implemented in the kernel files
 int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse);
+// the largest layer of a power series from its base and length-2 tensors,
+// all in NTT form (see k_series_expand)
+int launch_series_expand(Context *ctx, const uint64_t *base, size_t m0, const uint64_t *tens,
+                         size_t n, uint64_t *out);
 int launch_ntt_full_to_slotmajor(Context *ctx, const uint64_t *src, size_t rows, size_t m,
                                  size_t row_lo, size_t stride_lo, size_t stride_hi,
                                  double *dst);

@@ -317,4 +317,40 @@ int launch_fold(Context *ctx, const uint64_t *W, size_t r_in, size_t m, const ui
     return SALSAA_OK;
 }
 
+// ---------------------------------------------------------------- power series
+// The largest expanded layer of complete_power_series (arithmetic.rs:158-176)
+// in the NTT domain, where every ring product is pointwise:
+//   out[((i_0 2 + i_1) 2 + ... + i_{n-1}) m0 + j][s] = T_0[i_0][s] ... T_{n-1}[i_{n-1}][s] B[j][s]
+// (kronecker_product, arithmetic.rs:845-864, applied last tensor first, so
+// the first tensor is the most significant index bit).  One thread per output
+// residue; base [m0][d], tensors [n][2][d], out [m0 << n][d], all NTT form.
+namespace {
+__global__ void k_series_expand(const uint64_t *__restrict__ base, size_t m0,
+                                const uint64_t *__restrict__ tens, int n,
+                                uint64_t *__restrict__ out, size_t total, Mod M) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const size_t idx = e / kD, s = e % kD;
+        const size_t bits = idx / m0, j = idx % m0;
+        double v = u2d(base[j * kD + s]);
+        for (int k = 0; k < n; k++) {
+            const size_t bit = (bits >> (n - 1 - k)) & 1;
+            v = mulmod(v, u2d(tens[((size_t)k * 2 + bit) * kD + s]), M);
+        }
+        out[e] = d2u(v);
+    }
+}
+}  // namespace
+
+int launch_series_expand(Context *ctx, const uint64_t *base, size_t m0, const uint64_t *tens,
+                         size_t n, uint64_t *out) {
+    Mod M{ctx->qd, ctx->qinv};
+    const size_t total = (m0 << n) * kD;
+    const unsigned g = (unsigned)std::min<size_t>(ceil_div(total, (size_t)256), (size_t)ctx->ncu * 32);
+    hipLaunchKernelGGL(k_series_expand, dim3(g), dim3(256), 0, ctx->stream, base, m0, tens, (int)n,
+                       out, total, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
 }  // namespace salsaa

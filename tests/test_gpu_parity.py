@@ -127,6 +127,38 @@ def test_split_vs_oracle(ctx, orc):
     key.close()
 
 
+# ------------------------------------------------------------------ power series
+@pytest.mark.parametrize("m0,n", [(1, 1), (5, 2), (16, 4)])
+def test_complete_power_series_vs_oracle(ctx, orc, m0, n):
+    say("case: complete_power_series m0=%d n=%d" % (m0, n))
+    base = rand_ring(orc, (m0, 64), 41 + n, 0xC)
+    tensors = rand_ring(orc, (n, 2, 64), 42 + n, 0xD)
+    got = ctx.complete_power_series(ctx.to_device(base), ctx.to_device(tensors)).download()
+    assert np.array_equal(got, orc.complete_power_series(base, tensors, Q))
+
+
+@pytest.mark.parametrize("m,r", [(1, 1), (48, 3), (200, 5)])
+def test_dot_single_series_vs_oracle(ctx, orc, m, r):
+    say("case: parallel_dot_single_series_matrix m=%d r=%d" % (m, r))
+    layer = rand_ring(orc, (m, 64), 43, 0xC)
+    W = rand_ring(orc, (r, m, 64), 44, 0xC, 16)
+    got = ctx.parallel_dot_single_series_matrix(ctx.to_device(layer), ctx.to_device(W))
+    assert np.array_equal(got.download(), orc.dot_single_series(layer, W, Q))
+
+
+def test_power_series_then_dot_vs_oracle(ctx, orc):
+    # the batching step of the projection (project.rs:174-193): a layer
+    # expanded from a base and tensors, then its inner product with the rows
+    say("case: expanded layer, then inner product")
+    base = rand_ring(orc, (8, 64), 45, 0xC)
+    tensors = rand_ring(orc, (3, 2, 64), 46, 0xD)
+    W = rand_ring(orc, (2, 64, 64), 47, 0xC)
+    lay = ctx.complete_power_series(ctx.to_device(base), ctx.to_device(tensors))
+    got = ctx.parallel_dot_single_series_matrix(lay, ctx.to_device(W)).download()
+    want = orc.dot_single_series(orc.complete_power_series(base, tensors, Q), W, Q)
+    assert np.array_equal(got, want)
+
+
 # ------------------------------------------------------------------ norm check
 def _oracle_cs(reps, nrounds, fq2, seed):
     from oracle import oracle as orc

@@ -415,6 +415,52 @@ def test_fold_homomorphism(orc):
             assert orc.from_incomplete(lhs[j, jo], Q).tolist() == acc
 
 
+# ---------------------------------------------------------------- power series
+def _const(v, d=D):
+    a = np.zeros(d, np.uint64)
+    a[0] = v
+    return a
+
+
+def test_kronecker_order_kat(orc):
+    # arithmetic.rs:866-872: kronecker_product([1, 2, 3], [4, 5]) =
+    # [4, 5, 8, 10, 12, 15].  With the 2-vectors [1, 2] and [2, 3] as the one
+    # tensor over the base [4, 5], the layers are its first and last four entries.
+    base = np.stack([_const(4), _const(5)])
+    for t, want in (([1, 2], [4, 5, 8, 10]), ([2, 3], [8, 10, 12, 15])):
+        lay = orc.complete_power_series(base, np.stack([[_const(t[0]), _const(t[1])]]), Q)
+        assert lay[:, 0].tolist() == want and not lay[:, 1:].any()
+
+
+def test_complete_power_series_brute_force(orc):
+    # the index formula (the first tensor the most significant bit) with
+    # random ring elements against schoolbook products; all-ones tensors (the
+    # projection's layers, project.rs:177-186) repeat the base
+    rng = random.Random(23)
+    d, m0, n = 8, 3, 3
+    base = np.array(rnd(rng, m0 * d, Q), np.uint64).reshape(m0, d)
+    tensors = np.array(rnd(rng, n * 2 * d, Q), np.uint64).reshape(n, 2, d)
+    lay = orc.complete_power_series(base, tensors, Q)
+    for idx in range(m0 << n):
+        bits, j = idx // m0, idx % m0
+        acc = base[j].tolist()
+        for k in range(n):
+            acc = pyref.negacyclic_mul(acc, tensors[k, (bits >> (n - 1 - k)) & 1].tolist(), Q)
+        assert lay[idx].tolist() == acc, idx
+    ones = np.stack([np.stack([_const(1, d), _const(1, d)])] * n)
+    assert np.array_equal(orc.complete_power_series(base, ones, Q), np.tile(base, (1 << n, 1)))
+
+
+def test_dot_single_series_is_commit_kappa1(orc):
+    # arithmetic.rs:507-536 is the commit (arithmetic.rs:459-505) with the layer
+    # as the only key row; the two oracle routes share no product code
+    rng = random.Random(29)
+    d, r, m = 8, 3, 5
+    layer = np.array(rnd(rng, m * d, Q), np.uint64).reshape(m, d)
+    W = np.array(rnd(rng, r * m * d, Q), np.uint64).reshape(r, m, d)
+    assert np.array_equal(orc.dot_single_series(layer, W, Q), orc.commit(layer[None], W, Q)[0])
+
+
 # ---------------------------------------------------------------- sampling
 def test_samplers_support(orc):
     u = orc.sample_uniform(0, 1, (4096,), Q)
