@@ -203,6 +203,17 @@ int salsaa_complete_power_series(salsaa_ctx *ctx, const uint64_t *d_base, size_t
 int salsaa_parallel_dot_single_series_matrix(salsaa_ctx *ctx, const uint64_t *d_layer, size_t m,
                                              const uint64_t *d_W, size_t r, uint64_t *d_out);
 
+/* project (project.rs:52-98) up to its closing commit: with cw = r * 256,
+ * chunk j < m / cw of row i of W [r][m][d] gives the 256 ring elements
+ *   out[(i (m / cw) + j) 256 + k] = sum_l C[k][l] W[i][j cw + l] mod q,
+ * C[k][l] = 0 where d_is_zero[k][l], else +1 or -1 by d_sign[k][l] (the
+ * Challenge of challenge_for_project, project.rs:19-39, as bytes 0/1,
+ * [256][cw] each).  d_out [r (m / cw) 256][d], in W's representation; the
+ * reference then commits it (parallel_dot_series_matrix).  Needs cw <= m
+ * (the reference panics) and cw (q - 1) < 2^64 (its u64 sums wrap past it). */
+int salsaa_project(salsaa_ctx *ctx, const uint64_t *d_W, size_t r, size_t m,
+                   const uint8_t *d_is_zero, const uint8_t *d_sign, uint64_t *d_out);
+
 /* split (split.rs:45-76): the rows [L_0..L_{r-1}, R_0..R_{r-1}] of the halves
  * of W [r][m][d] (a view, not copied) are committed under the length-m/2
  * prefix: d_out [kappa][2r][d] IncompleteNTT. */

@@ -39,6 +39,7 @@ def _load():
         "or_get_inv_roots": (None, [sz, u64, P]),
         "or_complete_power_series": (None, [P, sz, P, sz, sz, u64, P]),
         "or_dot_single_series": (None, [P, P, P, sz, sz, sz, u64]),
+        "or_project": (None, [P, sz, sz, sz, ctypes.c_void_p, ctypes.c_void_p, u64, P]),
         "or_ntt_forward": (None, [P, sz, u64]),
         "or_ntt_inverse": (None, [P, sz, u64]),
         "or_ntt_forward_direct": (None, [P, P, sz, u64]),
@@ -207,6 +208,20 @@ def dot_single_series(layer, W, q):
     assert W.shape == (r, m, d)
     out = np.zeros((r, d), np.uint64)
     lib.or_dot_single_series(_p(layer), _p(W), _p(out), r, m, d, q)
+    return out
+
+
+def project(W, is_zero, sign, q):
+    """project (project.rs:52-98) before its commit: W [r][m][d], is_zero and
+    sign [256][r*256] (0/1) -> [r*(m // (r*256))*256][d]."""
+    W = _u(W)
+    r, m, d = W.shape
+    cw = r * 256
+    z = np.ascontiguousarray(is_zero, np.uint8)
+    sg = np.ascontiguousarray(sign, np.uint8)
+    assert z.shape == (256, cw) and sg.shape == (256, cw)
+    out = np.zeros((r * (m // cw) * 256, d), np.uint64)
+    lib.or_project(_p(W), r, m, d, z.ctypes.data, sg.ctypes.data, q, _p(out))
     return out
 
 

@@ -424,6 +424,41 @@ void or_dot_single_series(const uint64_t *layer, const uint64_t *W, uint64_t *ou
 }
 
 /* ------------------------------------------------------------------------ */
+/* projection                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* project (project.rs:52-98) without its closing commit: with cw = r * 256,
+ * row i's chunks j < m / cw (row-major, i outer) each give 256 ring elements
+ *   out[(i (m / cw) + j) 256 + k] = pos - neg mod q,
+ * pos / neg the u64 sums (add_no_reduction, arithmetic.rs:19-44) of the chunk's
+ * elements W[i][j cw + l] with is_zero[k][l] = 0 and sign[k][l] = 1 / 0, each
+ * reduced mod q (eltwise_reduce_mod) before the subtraction.  is_zero, sign
+ * [256][cw] bytes; W [r][m][d]; out [r (m / cw) 256][d], the data taken as
+ * they are (every representation is linear). */
+void or_project(const uint64_t *W, size_t r, size_t m, size_t d, const uint8_t *is_zero,
+                const uint8_t *sign, uint64_t q, uint64_t *out) {
+    const size_t H = 256, cw = r * H, nch = m / cw;
+    uint64_t *pos = (uint64_t *)malloc(d * sizeof(uint64_t));
+    uint64_t *neg = (uint64_t *)malloc(d * sizeof(uint64_t));
+    for (size_t i = 0; i < r; i++)
+        for (size_t j = 0; j < nch; j++)
+            for (size_t k = 0; k < H; k++) {
+                memset(pos, 0, d * sizeof(uint64_t));
+                memset(neg, 0, d * sizeof(uint64_t));
+                for (size_t l = 0; l < cw; l++) {
+                    if (is_zero[k * cw + l]) continue;
+                    const uint64_t *x = W + (i * m + j * cw + l) * d;
+                    uint64_t *acc = sign[k * cw + l] ? pos : neg;
+                    for (size_t c = 0; c < d; c++) acc[c] += x[c];
+                }
+                uint64_t *o = out + ((i * nch + j) * H + k) * d;
+                for (size_t c = 0; c < d; c++) o[c] = or_submod(pos[c] % q, neg[c] % q, q);
+            }
+    free(pos);
+    free(neg);
+}
+
+/* ------------------------------------------------------------------------ */
 /* norm check                                                                */
 /* ------------------------------------------------------------------------ */
 

@@ -41,6 +41,11 @@
  *                                           brute-force index formula
  *   or_dot_single_series                  : equals or_commit with kappa = 1
  *                                           (slot products vs schoolbook)
+ *   or_project                            : signed matrix product mod q on
+ *                                           small inputs, linearity, and the
+ *                                           batching identity of project.rs:
+ *                                           154-190 (a transposed challenge
+ *                                           fails it)
  *   or_get_inv_roots                      : parity unpinned (no reference test
  *                                           reads HEXL's inverse table layout)
  */
@@ -102,6 +107,10 @@ void or_complete_power_series(const uint64_t *base, size_t m0, const uint64_t *t
  * W [r][m][d] Coefficient -> out [r][d] IncompleteNTT */
 void or_dot_single_series(const uint64_t *layer, const uint64_t *W, uint64_t *out, size_t r,
                           size_t m, size_t d, uint64_t q);
+/* project (project.rs:52-98) before its commit: W [r][m][d], is_zero and sign
+ * [256][r 256] bytes -> out [r (m / (r 256)) 256][d] */
+void or_project(const uint64_t *W, size_t r, size_t m, size_t d, const uint8_t *is_zero,
+                const uint8_t *sign, uint64_t q, uint64_t *out);
 
 /* ---- norm check (fast_norm_check.rs) ---- */
 void or_tensor(const uint64_t *cs, size_t n, uint64_t q, uint64_t *out);

@@ -110,6 +110,7 @@ def _load():
         "salsaa_ring_mul": (_int, [_vp, _vp, _vp, _vp, _sz]),
         "salsaa_complete_power_series": (_int, [_vp, _vp, _sz, _vp, _sz, _vp]),
         "salsaa_parallel_dot_single_series_matrix": (_int, [_vp, _vp, _sz, _vp, _sz, _vp]),
+        "salsaa_project": (_int, [_vp, _vp, _sz, _sz, _vp, _vp, _vp]),
         "salsaa_key_create": (_int, [_vp, _vp, _sz, _sz, _PP]),
         "salsaa_key_destroy": (_int, [_vp]),
         "salsaa_parallel_dot_series_matrix": (_int, [_vp, _vp, _vp, _sz, _sz, _vp]),
@@ -328,6 +329,15 @@ class Context:
         out = self.empty((r, d))
         _check(lib.salsaa_parallel_dot_single_series_matrix(self.h, layer.ptr, m, W.ptr, r,
                                                             out.ptr))
+        return out
+
+    def project(self, W, is_zero, sign):
+        """project (project.rs:52-98) before its commit: W [r][m][d], is_zero and
+        sign device bytes [256][r*256] -> [r*(m // (r*256))*256][d]."""
+        r, m, d = W.shape
+        assert d == self.d and is_zero.shape == (256, r * 256) and sign.shape == is_zero.shape
+        out = self.empty((r * (m // (r * 256)) * 256, d))
+        _check(lib.salsaa_project(self.h, W.ptr, r, m, is_zero.ptr, sign.ptr, out.ptr))
         return out
 
     def key(self, A_dev):

@@ -471,6 +471,12 @@ int salsaa_parallel_dot_single_series_matrix(salsaa_ctx *ctx, const uint64_t *d_
     return st != SALSAA_OK ? st : st2;
 }
 
+int salsaa_project(salsaa_ctx *ctx, const uint64_t *d_W, size_t r, size_t m,
+                   const uint8_t *d_is_zero, const uint8_t *d_sign, uint64_t *d_out) {
+    CTX_OK(ctx);
+    return launch_project(&ctx->c, d_W, r, m, d_is_zero, d_sign, d_out);
+}
+
 int salsaa_split(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_W, size_t r, size_t m,
                  uint64_t *d_out) {
     CTX_OK(ctx);

@@ -270,6 +270,9 @@ int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse
 // all in NTT form (see k_series_expand)
 int launch_series_expand(Context *ctx, const uint64_t *base, size_t m0, const uint64_t *tens,
                          size_t n, uint64_t *out);
+// project (project.rs:52-98) before its commit, see project.hip
+int launch_project(Context *ctx, const uint64_t *W, size_t r, size_t m, const uint8_t *is_zero,
+                   const uint8_t *sign, uint64_t *out);
 int launch_ntt_full_to_slotmajor(Context *ctx, const uint64_t *src, size_t rows, size_t m,
                                  size_t row_lo, size_t stride_lo, size_t stride_hi,
                                  double *dst);

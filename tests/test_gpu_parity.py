@@ -159,6 +159,37 @@ def test_power_series_then_dot_vs_oracle(ctx, orc):
     assert np.array_equal(got, want)
 
 
+# ------------------------------------------------------------------ projection
+@pytest.mark.parametrize("r,m", [(1, 256), (2, 1536), (4, 2048)])
+def test_project_vs_oracle(ctx, orc, r, m):
+    # m = 1536 at r = 2: the last 512 elements of each row are outside every chunk
+    say("case: project r=%d m=%d" % (r, m))
+    rng = np.random.default_rng(7 * r + m)
+    W = rand_ring(orc, (r, m, 64), 48, 0xE)
+    z = rng.integers(0, 2, (256, r * 256), dtype=np.uint8)
+    sg = rng.integers(0, 2, (256, r * 256), dtype=np.uint8)
+    got = ctx.project(ctx.to_device(W), ctx.to_device(z, np.uint8), ctx.to_device(sg, np.uint8))
+    assert np.array_equal(got.download(), orc.project(W, z, sg, Q))
+
+
+def test_project_extremes_and_checks(sa, ctx, orc):
+    # the largest part sums (every residue q - 1, every sign +1: cw (q - 1) in
+    # u64), all zero, all -1; and the reference's chunk-width precondition
+    say("case: project extremes")
+    r, m = 4, 1024
+    W = np.full((r, m, 64), Q - 1, np.uint64)
+    dW = ctx.to_device(W)
+    for zv, sv in ((0, 1), (1, 1), (0, 0)):
+        z = np.full((256, r * 256), zv, np.uint8)
+        sg = np.full((256, r * 256), sv, np.uint8)
+        got = ctx.project(dW, ctx.to_device(z, np.uint8), ctx.to_device(sg, np.uint8)).download()
+        assert np.array_equal(got, orc.project(W, z, sg, Q)), (zv, sv)
+    z = np.zeros((256, 2 * 256), np.uint8)
+    with pytest.raises(sa.SalsaaError):
+        ctx.project(ctx.to_device(np.zeros((2, 256, 64), np.uint64)),
+                    ctx.to_device(z, np.uint8), ctx.to_device(z, np.uint8))
+
+
 # ------------------------------------------------------------------ norm check
 def _oracle_cs(reps, nrounds, fq2, seed):
     from oracle import oracle as orc

@@ -461,6 +461,69 @@ def test_dot_single_series_is_commit_kappa1(orc):
     assert np.array_equal(orc.dot_single_series(layer, W, Q), orc.commit(layer[None], W, Q)[0])
 
 
+# ---------------------------------------------------------------- projection
+def _proj_inputs(seed, r, m, d):
+    rng = np.random.default_rng(seed)
+    W = rng.integers(0, Q, (r, m, d), dtype=np.uint64)
+    z = rng.integers(0, 2, (256, r * 256), dtype=np.uint8)
+    sg = rng.integers(0, 2, (256, r * 256), dtype=np.uint8)
+    return W, z, sg
+
+
+def test_project_signed_product(orc):
+    # project.rs:52-98: chunk j of row i gives 256 elements, the signed sums
+    # over the chunk with C[k][l] = 0 (is_zero), +1 or -1 (sign): a plain
+    # signed matrix product in int64 (|sum| < 2^58) reduced mod q
+    d, r, m = 8, 2, 1536  # cw = 512: three chunks per row, the last 512 columns dropped
+    W, z, sg = _proj_inputs(3, r, m, d)
+    got = orc.project(W, z, sg, Q)
+    C = np.where(z == 1, 0, np.where(sg == 1, 1, -1)).astype(np.int64)
+    cw, nch = r * 256, m // (r * 256)
+    assert got.shape == (r * nch * 256, d)
+    for i in range(r):
+        for j in range(nch):
+            chunk = W[i, j * cw:(j + 1) * cw].astype(np.int64)
+            want = np.mod(C @ chunk, Q).astype(np.uint64)
+            assert np.array_equal(got[(i * nch + j) * 256:(i * nch + j + 1) * 256], want), (i, j)
+
+
+def test_project_linear(orc):
+    d, r, m = 8, 1, 512
+    W1, z, sg = _proj_inputs(4, r, m, d)
+    W2, _, _ = _proj_inputs(5, r, m, d)
+    W12 = ((W1.astype(object) + W2.astype(object)) % Q).astype(np.uint64)
+    lhs = orc.project(W12, z, sg, Q)
+    rhs = (orc.project(W1, z, sg, Q).astype(object) + orc.project(W2, z, sg, Q).astype(object)) % Q
+    assert np.array_equal(lhs, rhs.astype(np.uint64))
+
+
+def test_project_batching_identity(orc):
+    # project.rs:154-190 (batch_projections): with c_j = sum_i C[i][j] c0^i,
+    # <(1, c0, .., c0^255), projected chunk> = <(c_j), chunk> in R_q; the
+    # reference asserts it ("Inner product mismatch") with c0 = 1 and random
+    # batching challenges; here c0 is a random ring element
+    d, r, m = 8, 1, 256
+    W, z, sg = _proj_inputs(6, r, m, d)
+    P = orc.project(W, z, sg, Q)
+    rng = random.Random(31)
+    c0 = rnd(rng, d, Q)
+    pw = [[1] + [0] * (d - 1)]
+    for _ in range(255):
+        pw.append(pyref.negacyclic_mul(pw[-1], c0, Q))
+    C = np.where(z == 1, 0, np.where(sg == 1, 1, -1))
+    lhs = [0] * d
+    for k in range(256):
+        lhs = [(a + b) % Q for a, b in zip(lhs, pyref.negacyclic_mul(pw[k], P[k].tolist(), Q))]
+    rhs = [0] * d
+    for l in range(256):
+        cj = [0] * d
+        for k in range(256):
+            if C[k, l]:
+                cj = [(a + int(C[k, l]) * b) % Q for a, b in zip(cj, pw[k])]
+        rhs = [(a + b) % Q for a, b in zip(rhs, pyref.negacyclic_mul(cj, W[0, l].tolist(), Q))]
+    assert lhs == rhs
+
+
 # ---------------------------------------------------------------- sampling
 def test_samplers_support(orc):
     u = orc.sample_uniform(0, 1, (4096,), Q)
