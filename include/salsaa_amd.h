@@ -214,6 +214,35 @@ int salsaa_parallel_dot_single_series_matrix(salsaa_ctx *ctx, const uint64_t *d_
 int salsaa_project(salsaa_ctx *ctx, const uint64_t *d_W, size_t r, size_t m,
                    const uint8_t *d_is_zero, const uint8_t *d_sign, uint64_t *d_out);
 
+/* batch_projections (project.rs:131-203) for the projection d_P [m][d] of
+ * W [r][m][d] under the challenge (d_is_zero, d_sign [256][256 r]) with the
+ * batching challenge c0 (d_c0 [d], the middle element of
+ * batching_challenge_for_project, project.rs:104-116): with
+ * c_l = sum_k C[k][l] c0^k (l < 256 r),
+ *   d_witness_lhs   [m][d]  the layer c_{x mod 256 r}          (may be NULL)
+ *   d_projected_lhs [m][d]  the layer c0^{x mod 256}            (may be NULL)
+ *   d_witness_rhs   [r][d]  sum_x c_{x mod 256 r} W[j][x]      IncompleteNTT
+ *   d_projection_rhs  [d]   sum_x c0^{x mod 256} P[x]          IncompleteNTT
+ *   d_ok            [1]     1 if sum_j witness_rhs[j] = projection_rhs (the
+ *                           assertion of project.rs:198), else 0 (may be NULL)
+ * Inputs and layers in Coefficient form.  Needs r and m / (256 r) powers of
+ * two (the reference finds no matching layer otherwise). */
+int salsaa_batch_projections(salsaa_ctx *ctx, const uint64_t *d_P, const uint64_t *d_W, size_t r,
+                             size_t m, const uint8_t *d_is_zero, const uint8_t *d_sign,
+                             const uint64_t *d_c0, uint64_t *d_projected_lhs,
+                             uint64_t *d_witness_lhs, uint64_t *d_projection_rhs,
+                             uint64_t *d_witness_rhs, int32_t *d_ok);
+
+/* send_cross_terms_before_join (project.rs:229-245): l0 = sum_x
+ * projected_lhs[x] W[0][x] and l1 = sum_x witness_lhs[x] P[x] for the layers
+ * [m][d] of salsaa_batch_projections, the folded witness W (row 0 is read)
+ * and the projection P [m][d]; d_l0, d_l1 [d] IncompleteNTT.  (The reference
+ * forms the products of every row and keeps entry 0.) */
+int salsaa_send_cross_terms_before_join(salsaa_ctx *ctx, const uint64_t *d_projected_lhs,
+                                        const uint64_t *d_witness_lhs, size_t m,
+                                        const uint64_t *d_W, const uint64_t *d_P, uint64_t *d_l0,
+                                        uint64_t *d_l1);
+
 /* split (split.rs:45-76): the rows [L_0..L_{r-1}, R_0..R_{r-1}] of the halves
  * of W [r][m][d] (a view, not copied) are committed under the length-m/2
  * prefix: d_out [kappa][2r][d] IncompleteNTT. */

@@ -225,6 +225,62 @@ def project(W, is_zero, sign, q):
     return out
 
 
+def consecutive_powers(c, width, q):
+    """consecutive_powers (project.rs:118-129): [1, c, c^2, ...] [width][d]
+    Coefficient, each power the schoolbook product of the last with c."""
+    c = _u(c)
+    out = np.zeros((width, c.size), np.uint64)
+    out[0, 0] = 1
+    for k in range(1, width):
+        out[k] = ring_mul_naive(out[k - 1], c, q)
+    return out
+
+
+def _signed_col_sums(is_zero, sign, pw, q):
+    """c_l = sum_k C[k][l] pw[k] mod q with C from (is_zero, sign) [H][w]:
+    the positive and negative parts summed apart in u64 (H q < 2^64)."""
+    keep = np.asarray(is_zero) == 0
+    pos = (keep & (np.asarray(sign) != 0)).astype(np.uint64).T
+    neg = (keep & (np.asarray(sign) == 0)).astype(np.uint64).T
+    a, b = (pos @ pw) % np.uint64(q), (neg @ pw) % np.uint64(q)
+    return np.where(a >= b, a - b, a + np.uint64(q) - b).astype(np.uint64)
+
+
+def batch_projections(P, W, is_zero, sign, c0, q):
+    """batch_projections (project.rs:131-203), step by step as the reference
+    takes them: c_l = sum_k C[k][l] c0^k (:154-166), both layers by
+    complete_power_series with [1, 1] tensors (:168-181), both right-hand
+    sides by dot_single_series (:183-188) and the assertion (:190-198).
+    P [m][d], W [r][m][d], c0 [d] Coefficient.  Returns (projected_lhs,
+    witness_lhs, projection_rhs [d], witness_rhs [r][d], ok)."""
+    P, W = _u(P), _u(W)
+    r, m, d = W.shape
+    cw = r * 256
+    pw = consecutive_powers(c0, 256, q)
+    cj = _signed_col_sums(is_zero, sign, pw, q)
+    one = np.zeros(d, np.uint64)
+    one[0] = 1
+    n_w = int(m // cw).bit_length() - 1
+    n_p = n_w + r.bit_length() - 1
+    wlhs = complete_power_series(cj, np.tile(one, (n_w, 2, 1)), q)
+    plhs = complete_power_series(pw, np.tile(one, (n_p, 2, 1)), q)
+    wrhs = dot_single_series(wlhs, W, q)
+    prhs = dot_single_series(plhs, P[None], q)[0]
+    total = np.zeros(d, np.uint64)
+    for x in wrhs:
+        total = (total + x) % np.uint64(q)
+    return plhs, wlhs, prhs, wrhs, bool(np.array_equal(total, prhs))
+
+
+def send_cross_terms_before_join(projected_lhs, witness_lhs, W, P, q):
+    """send_cross_terms_before_join (project.rs:229-245): entry 0 of
+    dot_single_series(projected_lhs, W) and of dot_single_series(witness_lhs,
+    [P]), each [d] IncompleteNTT."""
+    l0 = dot_single_series(projected_lhs, W, q)[0]
+    l1 = dot_single_series(witness_lhs, _u(P)[None], q)[0]
+    return l0, l1
+
+
 def tensor(cs, q):
     cs = _u(cs)
     out = np.zeros(1 << cs.size, np.uint64)

@@ -111,6 +111,9 @@ def _load():
         "salsaa_complete_power_series": (_int, [_vp, _vp, _sz, _vp, _sz, _vp]),
         "salsaa_parallel_dot_single_series_matrix": (_int, [_vp, _vp, _sz, _vp, _sz, _vp]),
         "salsaa_project": (_int, [_vp, _vp, _sz, _sz, _vp, _vp, _vp]),
+        "salsaa_batch_projections": (_int, [_vp, _vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp,
+                                            _vp, _vp, _vp]),
+        "salsaa_send_cross_terms_before_join": (_int, [_vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
         "salsaa_key_create": (_int, [_vp, _vp, _sz, _sz, _PP]),
         "salsaa_key_destroy": (_int, [_vp]),
         "salsaa_parallel_dot_series_matrix": (_int, [_vp, _vp, _vp, _sz, _sz, _vp]),
@@ -339,6 +342,35 @@ class Context:
         out = self.empty((r * (m // (r * 256)) * 256, d))
         _check(lib.salsaa_project(self.h, W.ptr, r, m, is_zero.ptr, sign.ptr, out.ptr))
         return out
+
+    def batch_projections(self, P, W, is_zero, sign, c0, layers=True):
+        """batch_projections (project.rs:131-203): P [m][d] = project(W) for
+        W [r][m][d], the challenge bytes [256][r*256], c0 [d] (Coefficient) ->
+        (projected_lhs [m][d] or None, witness_lhs [m][d] or None,
+        projection_rhs [d], witness_rhs [r][d], ok [1] int32)."""
+        r, m, d = W.shape
+        assert d == self.d and P.shape == (m, d) and c0.shape == (d,)
+        assert is_zero.shape == (256, r * 256) and sign.shape == is_zero.shape
+        plhs = self.empty((m, d)) if layers else None
+        wlhs = self.empty((m, d)) if layers else None
+        prhs, wrhs = self.empty((d,)), self.empty((r, d))
+        ok = self.empty((1,), np.int32)
+        _check(lib.salsaa_batch_projections(self.h, P.ptr, W.ptr, r, m, is_zero.ptr, sign.ptr,
+                                            c0.ptr, plhs.ptr if layers else None,
+                                            wlhs.ptr if layers else None, prhs.ptr, wrhs.ptr,
+                                            ok.ptr))
+        return plhs, wlhs, prhs, wrhs, ok
+
+    def send_cross_terms_before_join(self, projected_lhs, witness_lhs, W, P):
+        """send_cross_terms_before_join (project.rs:229-245): (l0, l1), each [d]
+        IncompleteNTT, from the layers [m][d], the folded witness W [r][m][d]
+        (row 0 used) and the projection P [m][d]."""
+        m, d = projected_lhs.shape
+        assert witness_lhs.shape == (m, d) and W.shape[1:] == (m, d) and P.shape == (m, d)
+        l0, l1 = self.empty((d,)), self.empty((d,))
+        _check(lib.salsaa_send_cross_terms_before_join(self.h, projected_lhs.ptr, witness_lhs.ptr,
+                                                       m, W.ptr, P.ptr, l0.ptr, l1.ptr))
+        return l0, l1
 
     def key(self, A_dev):
         kappa, m, d = A_dev.shape

@@ -477,6 +477,57 @@ int salsaa_project(salsaa_ctx *ctx, const uint64_t *d_W, size_t r, size_t m,
     return launch_project(&ctx->c, d_W, r, m, d_is_zero, d_sign, d_out);
 }
 
+static bool is_pow2(size_t x) { return x && !(x & (x - 1)); }
+
+int salsaa_batch_projections(salsaa_ctx *ctx, const uint64_t *d_P, const uint64_t *d_W, size_t r,
+                             size_t m, const uint8_t *d_is_zero, const uint8_t *d_sign,
+                             const uint64_t *d_c0, uint64_t *d_projected_lhs,
+                             uint64_t *d_witness_lhs, uint64_t *d_projection_rhs,
+                             uint64_t *d_witness_rhs, int32_t *d_ok) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(d_projection_rhs && d_witness_rhs, "null output pointer");
+    const size_t cw = r * 256;
+    // the reference's series have layers of lengths cw 2^t and 256 2^t only,
+    // so its inner products find no layer for other shapes
+    SALSAA_CHECK(is_pow2(r) && cw <= m && m % cw == 0 && is_pow2(m / cw) && cw < 65536,
+                 "batch_projections needs r and m / (256 r) powers of two (project.rs:168-186)");
+    Context *c = &ctx->c;
+    // scratch: c0 [1], powers [256], c_l [cw], column sums of W [r][cw] and of P [256]
+    uint64_t *buf = nullptr;
+    const size_t nbuf = 1 + 256 + cw + r * cw + 256;
+    SALSAA_TRY(scratch_get(c, 9, nbuf * kD * sizeof(uint64_t), (void **)&buf));
+    uint64_t *c0 = buf, *pw = c0 + kD, *cj = pw + 256 * kD, *wsum = cj + cw * kD,
+             *psum = wsum + r * cw * kD;
+    // consecutive powers of c0 (project.rs:118-129), slot by slot in NTT form
+    SALSAA_HIP(hipMemcpyAsync(c0, d_c0, kD * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
+    SALSAA_TRY(launch_ntt(c, c0, 1, SALSAA_REP_FULL, 0));
+    SALSAA_TRY(launch_powers(c, c0, pw));
+    SALSAA_TRY(launch_ntt(c, pw, 256, SALSAA_REP_FULL, 1));
+    // c_l = sum_k C[k][l] c0^k (project.rs:154-166)
+    SALSAA_TRY(launch_project_transposed(c, pw, cw, d_is_zero, d_sign, cj));
+    // Both layers repeat their base (all tensors are [1, 1]), so each inner
+    // product is the base's against the sum of the matrix's chunks.
+    SALSAA_TRY(launch_chunk_sums(c, d_W, r, m, cw, wsum));
+    SALSAA_TRY(launch_chunk_sums(c, d_P, 1, m, 256, psum));
+    SALSAA_TRY(salsaa_parallel_dot_single_series_matrix(ctx, cj, cw, wsum, r, d_witness_rhs));
+    SALSAA_TRY(salsaa_parallel_dot_single_series_matrix(ctx, pw, 256, psum, 1, d_projection_rhs));
+    if (d_witness_lhs) SALSAA_TRY(launch_repeat(c, cj, cw, m, d_witness_lhs));
+    if (d_projected_lhs) SALSAA_TRY(launch_repeat(c, pw, 256, m, d_projected_lhs));
+    if (d_ok) SALSAA_TRY(launch_sum_equals(c, d_witness_rhs, r, d_projection_rhs, d_ok));
+    return SALSAA_OK;
+}
+
+int salsaa_send_cross_terms_before_join(salsaa_ctx *ctx, const uint64_t *d_projected_lhs,
+                                        const uint64_t *d_witness_lhs, size_t m,
+                                        const uint64_t *d_W, const uint64_t *d_P, uint64_t *d_l0,
+                                        uint64_t *d_l1) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(d_l0 && d_l1, "null output pointer");
+    // the reference takes entry 0 of each inner product: row 0 of each matrix
+    SALSAA_TRY(salsaa_parallel_dot_single_series_matrix(ctx, d_projected_lhs, m, d_W, 1, d_l0));
+    return salsaa_parallel_dot_single_series_matrix(ctx, d_witness_lhs, m, d_P, 1, d_l1);
+}
+
 int salsaa_split(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_W, size_t r, size_t m,
                  uint64_t *d_out) {
     CTX_OK(ctx);

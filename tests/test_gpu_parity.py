@@ -190,6 +190,71 @@ def test_project_extremes_and_checks(sa, ctx, orc):
                     ctx.to_device(z, np.uint8), ctx.to_device(z, np.uint8))
 
 
+@pytest.mark.parametrize("r,m,unit", [(1, 256, False), (2, 2048, True), (4, 4096, False)])
+def test_batch_projections_vs_oracle(ctx, orc, r, m, unit):
+    # batch_projections (project.rs:131-203) and the cross terms before the
+    # join (:229-245) on the projection of W; c0 = 1 is the reference's own
+    # batching challenge, a random c0 the general case
+    say("case: batch_projections r=%d m=%d unit=%d" % (r, m, unit))
+    rng = np.random.default_rng(100 + r)
+    W = rng.integers(0, Q, (r, m, 64), dtype=np.uint64)
+    z = rng.integers(0, 2, (256, r * 256), dtype=np.uint8)
+    sg = rng.integers(0, 2, (256, r * 256), dtype=np.uint8)
+    c0 = np.zeros(64, np.uint64)
+    if unit:
+        c0[0] = 1
+    else:
+        c0 = rng.integers(0, Q, 64, dtype=np.uint64)
+    dW, dz, dsg = ctx.to_device(W), ctx.to_device(z, np.uint8), ctx.to_device(sg, np.uint8)
+    dP = ctx.project(dW, dz, dsg)
+    P = dP.download()
+    assert np.array_equal(P, orc.project(W, z, sg, Q))
+    plhs, wlhs, prhs, wrhs, ok = ctx.batch_projections(dP, dW, dz, dsg, ctx.to_device(c0))
+    w_plhs, w_wlhs, w_prhs, w_wrhs, w_ok = orc.batch_projections(P, W, z, sg, c0, Q)
+    assert w_ok and ok.download()[0] == 1
+    assert np.array_equal(plhs.download(), w_plhs)
+    assert np.array_equal(wlhs.download(), w_wlhs)
+    assert np.array_equal(prhs.download(), w_prhs)
+    assert np.array_equal(wrhs.download(), w_wrhs)
+    # the right-hand sides alone (layers not written) are the same
+    _, _, prhs2, wrhs2, ok2 = ctx.batch_projections(dP, dW, dz, dsg, ctx.to_device(c0),
+                                                    layers=False)
+    assert np.array_equal(prhs2.download(), w_prhs) and np.array_equal(wrhs2.download(), w_wrhs)
+    assert ok2.download()[0] == 1
+    # cross terms against a second witness of the same shape (the folded one)
+    Wf = rng.integers(0, Q, (2, m, 64), dtype=np.uint64)
+    l0, l1 = ctx.send_cross_terms_before_join(plhs, wlhs, ctx.to_device(Wf), dP)
+    w_l0, w_l1 = orc.send_cross_terms_before_join(w_plhs, w_wlhs, Wf, P, Q)
+    assert np.array_equal(l0.download(), w_l0) and np.array_equal(l1.download(), w_l1)
+
+
+def test_batch_projections_mismatch_and_checks(sa, ctx, orc):
+    # a projection that is not the witness's fails the assertion (ok = 0, the
+    # outputs still the oracle's); shapes with no matching layer are refused
+    say("case: batch_projections mismatch")
+    r, m = 2, 1024
+    rng = np.random.default_rng(7)
+    W = rng.integers(0, Q, (r, m, 64), dtype=np.uint64)
+    z = rng.integers(0, 2, (256, r * 256), dtype=np.uint8)
+    sg = rng.integers(0, 2, (256, r * 256), dtype=np.uint8)
+    c0 = rng.integers(0, Q, 64, dtype=np.uint64)
+    P = orc.project(W, z, sg, Q)
+    P[17, 3] = (P[17, 3] + 1) % Q
+    dz, dsg = ctx.to_device(z, np.uint8), ctx.to_device(sg, np.uint8)
+    _, _, prhs, wrhs, ok = ctx.batch_projections(ctx.to_device(P), ctx.to_device(W), dz, dsg,
+                                                 ctx.to_device(c0), layers=False)
+    w = orc.batch_projections(P, W, z, sg, c0, Q)
+    assert not w[4] and ok.download()[0] == 0
+    assert np.array_equal(prhs.download(), w[2]) and np.array_equal(wrhs.download(), w[3])
+    for rr, mm in ((3, 768 * 2), (2, 512 * 3)):
+        zz = np.zeros((256, rr * 256), np.uint8)
+        with pytest.raises(sa.SalsaaError):
+            ctx.batch_projections(ctx.to_device(np.zeros((mm, 64), np.uint64)),
+                                  ctx.to_device(np.zeros((rr, mm, 64), np.uint64)),
+                                  ctx.to_device(zz, np.uint8), ctx.to_device(zz, np.uint8),
+                                  ctx.to_device(c0), layers=False)
+
+
 # ------------------------------------------------------------------ norm check
 def _oracle_cs(reps, nrounds, fq2, seed):
     from oracle import oracle as orc

@@ -524,6 +524,76 @@ def test_project_batching_identity(orc):
     assert lhs == rhs
 
 
+def test_consecutive_powers_monomial(orc):
+    # project.rs:118-129 with c = X: X^k = (-1)^(k // d) X^(k mod d) in R_q
+    d = 8
+    X = np.zeros(d, np.uint64)
+    X[1] = 1
+    pw = orc.consecutive_powers(X, 256, Q)
+    for k in range(256):
+        want = np.zeros(d, np.uint64)
+        want[k % d] = 1 if (k // d) % 2 == 0 else Q - 1
+        assert np.array_equal(pw[k], want), k
+
+
+def test_batch_projections_constant_challenge(orc):
+    # batching_challenge_for_project (project.rs:104-116) draws c0 = 1: every
+    # power is 1, so c_l is the constant column sum of C and both layers are
+    # constants; the assertion holds
+    d, r, m = 8, 2, 1024
+    W, z, sg = _proj_inputs(7, r, m, d)
+    P = orc.project(W, z, sg, Q)
+    one = np.zeros(d, np.uint64)
+    one[0] = 1
+    plhs, wlhs, prhs, wrhs, ok = orc.batch_projections(P, W, z, sg, one, Q)
+    assert ok
+    C = np.where(z == 1, 0, np.where(sg == 1, 1, -1)).astype(np.int64)
+    col = np.mod(C.sum(axis=0), Q)
+    assert plhs.shape == (m, d) and wlhs.shape == (m, d)
+    assert (plhs[:, 0] == 1).all() and (plhs[:, 1:] == 0).all()
+    assert np.array_equal(wlhs[:, 0], np.tile(col, m // (r * 256)).astype(np.uint64))
+    assert (wlhs[:, 1:] == 0).all()
+
+
+def test_batch_projections_identity_and_plants(orc):
+    # random c0: the assertion of project.rs:198 holds; the projection rhs is
+    # the direct sum_k c0^(k mod 256) P[k]; a changed projection residue and
+    # a transposed challenge (both planted here, in the oracle's inputs) fail
+    d, r, m = 8, 1, 512
+    W, z, sg = _proj_inputs(8, r, m, d)
+    P = orc.project(W, z, sg, Q)
+    rng = random.Random(41)
+    c0 = np.array(rnd(rng, d, Q), np.uint64)
+    plhs, wlhs, prhs, wrhs, ok = orc.batch_projections(P, W, z, sg, c0, Q)
+    assert ok and wrhs.shape == (r, d)
+    pw = [[1] + [0] * (d - 1)]
+    for _ in range(255):
+        pw.append(pyref.negacyclic_mul(pw[-1], c0.tolist(), Q))
+    direct = [0] * d
+    for x in range(m):
+        direct = [(a + b) % Q for a, b in zip(direct, pyref.negacyclic_mul(pw[x % 256], P[x].tolist(), Q))]
+    assert np.array_equal(prhs, orc.to_incomplete(np.array(direct, np.uint64), Q))
+    P2 = P.copy()
+    P2[3, 5] = (P2[3, 5] + 1) % Q
+    assert not orc.batch_projections(P2, W, z, sg, c0, Q)[4]
+    assert not orc.batch_projections(P, W, z.T.copy(), sg.T.copy(), c0, Q)[4]
+
+
+def test_cross_terms_are_row0_products(orc):
+    # send_cross_terms_before_join (project.rs:229-245) keeps entry 0 of each
+    # inner product: rows past the first change nothing
+    d, r, m = 8, 2, 512
+    W, z, sg = _proj_inputs(9, r, m, d)
+    P = orc.project(W, z, sg, Q)
+    rng = random.Random(43)
+    c0 = np.array(rnd(rng, d, Q), np.uint64)
+    plhs, wlhs, _, _, _ = orc.batch_projections(P, W, z, sg, c0, Q)
+    l0, l1 = orc.send_cross_terms_before_join(plhs, wlhs, W, P, Q)
+    l0b, l1b = orc.send_cross_terms_before_join(plhs, wlhs, W[:1], P, Q)
+    assert np.array_equal(l0, l0b) and np.array_equal(l1, l1b)
+    assert np.array_equal(l0, orc.dot_single_series(plhs, W[:1], Q)[0])
+
+
 # ---------------------------------------------------------------- sampling
 def test_samplers_support(orc):
     u = orc.sample_uniform(0, 1, (4096,), Q)
