@@ -20,9 +20,11 @@ Also reported: the served stream ("served": open-loop Poisson arrivals of
 commit + norm-check jobs through the continuous-batching server and through
 the same server one job at a time, jobs/s and p50/p99 latency), the folding
 step ("folding": 8 instances decomposed, folded, re-committed and
-norm-checked, fused vs separate calls) and the single-instance full-size step (NTT of 2^20 ring elements; commit A 32 x 16384
-. W 8 x 16384; norm check 8 x 2^20; decomp_ell base 2^8 x 6 with
-verify_decomp).  All inputs are generated on the device from seeds before
+norm-checked, fused vs separate calls), the single-instance full-size step
+(NTT of 2^20 ring elements; commit A 32 x 16384 . W 8 x 16384; norm check
+8 x 2^20; decomp_ell base 2^8 x 6 with verify_decomp) and the projection
+("projection": project, batch_projections and the cross terms of an
+8 x 2^19 witness).  All inputs are generated on the device from seeds before
 timing starts; the batched and single-instance outputs stay in HBM.
 """
 import argparse
@@ -387,6 +389,102 @@ def folding(sa, ctx, steps):
             "speedup": round(pms / fms, 3), "bit_exact_vs_separate_calls": exact}
 
 
+PJ_R, PJ_M = 8, 1 << 19  # the largest projection of the published C2 run (C2.log:122-129)
+
+
+def projection(sa, ctx, steps, cpu_budget_s=8.0):
+    """Projection, batching and cross terms of an 8 x 2^19 witness (cw = 2048,
+    2048 chunks): project (signed sums over 256 x 2048 ternary challenges),
+    batch_projections with c0 = 1 (the reference's batching challenge) and
+    both layers written, and send_cross_terms_before_join.  project's kernel
+    is priced against HBM (W read once, the projection written once) and the
+    int8 matrix cores (2 * 256 * 448 int8 MACs per witness element); four
+    chunks are checked against an int64 matrix product, the batching
+    identity on the device (ok)."""
+    r, m = PJ_R, PJ_M
+    cw = r * 256
+    W = ctx.uniform((r, m, 64), 5, sa.STREAM_W)
+    rng = np.random.default_rng(11)
+    z = rng.integers(0, 2, (256, cw), dtype=np.uint8)
+    sg = rng.integers(0, 2, (256, cw), dtype=np.uint8)
+    dz, dsg = ctx.to_device(z, np.uint8), ctx.to_device(sg, np.uint8)
+    c0 = np.zeros(64, np.uint64)
+    c0[0] = 1
+    dc0 = ctx.to_device(c0)
+    P = ctx.project(W, dz, dsg)
+    outs = ctx.batch_projections(P, W, dz, dsg, dc0)
+    plhs, wlhs = outs[0], outs[1]
+    V = sa.DeviceArray.view
+    W0 = V(ctx, W.ptr, (1, m, 64))
+
+    def proj():
+        sa._check(sa.lib.salsaa_project(ctx.h, W.ptr, r, m, dz.ptr, dsg.ptr, P.ptr))
+
+    def batch():
+        sa._check(sa.lib.salsaa_batch_projections(ctx.h, P.ptr, W.ptr, r, m, dz.ptr, dsg.ptr,
+                                                  dc0.ptr, plhs.ptr, wlhs.ptr, outs[2].ptr,
+                                                  outs[3].ptr, outs[4].ptr))
+
+    l01 = [None, None]
+
+    def cross():
+        l01[0], l01[1] = ctx.send_cross_terms_before_join(plhs, wlhs, W0, P)
+
+    for _ in range(2):
+        proj()
+        batch()
+        cross()
+    ctx.sync()
+    pper, ptot, _ = timed(ctx, proj, steps)
+    bper, btot, _ = timed(ctx, batch, max(steps // 2, 2))
+    cper, ctot, _ = timed(ctx, cross, max(steps // 2, 2))
+    pms, bms, cms = ptot / steps, btot / max(steps // 2, 2), ctot / max(steps // 2, 2)
+    # four chunks against an int64 product (|sum| < 2048 q < 2^61)
+    C = np.where(z == 1, 0, np.where(sg == 1, 1, -1)).astype(np.int64)
+    got = P.download()
+    nch = m // cw
+    exact = bool(outs[4].download()[0] == 1)
+    for idx in (0, 1, nch + 3, r * nch - 1):
+        i, j = divmod(idx, nch)
+        chunk = V(ctx, W.ptr + (i * m + j * cw) * 512, (cw, 64)).download().astype(np.int64)
+        want = np.mod(C @ chunk, sa.Q).astype(np.uint64)
+        exact = exact and np.array_equal(got[idx * 256:(idx + 1) * 256], want)
+    alg_bytes = (r * m + m) * 512
+    int8_ops = 2.0 * 256 * 448 * r * m
+    cpu = None
+    if cpu_budget_s > 0:
+        from oracle import oracle as orc  # the checker's restatement, timed as the CPU baseline
+        host = V(ctx, W.ptr, (1, cw * 4, 64)).download()
+        t0, n_done = time.perf_counter(), 0
+        while time.perf_counter() - t0 < cpu_budget_s:
+            orc.project(host, z[:, :256], sg[:, :256], sa.Q)  # r = 1: 4 * 256 chunks of width 256
+            n_done += 1
+        el = time.perf_counter() - t0
+        # one run: 4 cw = 8192 elements x 256 outputs; the workload: r m x 256
+        per_elem = el / (n_done * cw * 4)
+        cpu = {"value": round(1.0 / (per_elem * r * m), 4), "unit": "projections/s",
+               "cores": 1, "kind": "port",
+               "sample": "oracle or_project on 8192 witness elements x 256 outputs, %d runs in "
+                         "%.1f s, scaled to 8 x 2^19 elements" % (n_done, el)}
+    P.free()
+    for a in (plhs, wlhs, outs[2], outs[3], outs[4], W):
+        a.free()
+    return {"workload": "projection of W 8 x 2^19 (cw = 2048, 2048 chunks x 256 outputs), "
+                        "batch_projections (c0 = 1, layers written), cross terms; the "
+                        "published C2 run: project 4.67 s, batch projections 434 ms "
+                        "(C2.log:125-129)",
+            "project": {"ms": round(pms, 4), "value": round(1000.0 / pms, 2),
+                        "unit": "projections/s",
+                        "p99_ms": round(float(np.percentile(pper, 99)), 4),
+                        "GB/s": round(alg_bytes / (pms * 1e-3) / 1e9, 1),
+                        "hbm_frac": round(alg_bytes / (pms * 1e-3) / 8e12, 3),
+                        "int8_TOP/s": round(int8_ops / (pms * 1e-3) / 1e12, 1),
+                        "mfma_frac": round(int8_ops / (pms * 1e-3) / 5.0e15, 3),
+                        "algorithmic_bytes": alg_bytes},
+            "batch_projections_ms": round(bms, 4), "cross_terms_ms": round(cms, 4),
+            "bit_exact_sampled": exact, "cpu_baseline_project": cpu}
+
+
 def timed(ctx, fn, k):
     evs = [ctx.event() for _ in range(k + 1)]
     ctx.sync()
@@ -442,6 +540,8 @@ def main():
     ap.add_argument("--no-single", action="store_true", help="skip the single-instance step")
     ap.add_argument("--no-served", action="store_true", help="skip the served stream")
     ap.add_argument("--no-folding", action="store_true", help="skip the folding step")
+    ap.add_argument("--no-projection", action="store_true",
+                    help="skip the projection / batching / cross terms")
     ap.add_argument("--eager", action="store_true",
                     help="run the batched steps eagerly instead of replaying the hipGraph "
                          "(for counter collection)")
@@ -568,6 +668,9 @@ def main():
         out["served"] = served(sa, ctx)
     if not args.no_folding:
         out["folding"] = folding(sa, ctx, args.steps)
+    if not args.no_projection:
+        out["projection"] = projection(sa, ctx, args.steps,
+                                       0.0 if args.no_cpu_baseline else 8.0)
     if not args.no_single:
         out["single_instance"] = single_instance(sa, ctx, args.steps)
     if not args.no_cpu_baseline:

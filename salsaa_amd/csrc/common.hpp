@@ -80,8 +80,9 @@ struct Context {
     hipStream_t stream = nullptr;
     double *d_tables = nullptr;     // TAB_SIZE doubles
     std::vector<double> h_tables;   // host copy
-    ScratchBuf scratch[10];         // grown on demand, owned by the context
+    ScratchBuf scratch[11];         // grown on demand, owned by the context
     bool mfma_commit = true;        // int8 MFMA commit (false: fp64 VALU commit)
+    bool mfma_project = true;       // int8 MFMA projection (false: VALU signed sums)
     ScratchBuf zeros;               // zero-filled, grown on demand
     std::mutex lock;                // one host thread drives a context at a time
 };
@@ -97,6 +98,7 @@ inline void share_context(const Context &from, Context &to) {
     to.qinv = from.qinv;
     to.d_tables = from.d_tables;
     to.mfma_commit = from.mfma_commit;
+    to.mfma_project = from.mfma_project;
 }
 
 struct Key {

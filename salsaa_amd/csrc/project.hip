@@ -85,6 +85,185 @@ __global__ __launch_bounds__(PJ_THREADS) void k_project(const uint64_t *__restri
     }
 }
 
+// ---- the projection on the int8 matrix cores
+//
+// Per chunk the projection is one integer matrix product: C (256 x cw, entries
+// -1, 0, 1) times the chunk's coefficients (cw x 64).  A coefficient x, shifted
+// by h = (q - 1) / 2 (|x - h| <= h < 2^49), is written as 7 signed base-256
+// digits: with u = x - h + 0x80..80 (seven 0x80 bytes, so 0 <= u < 2^56), byte
+// t of u is d_t + 128, i.e. byte t of u ^ 0x80..80 is d_t as an int8.  Then
+//     out[k][c] = sum_t 2^(8t) (C . D_t)[k][c] + h sum_l C[k][l]  mod q,
+// every C . D_t is exact in int32 (|sum| <= 128 cw < 2^23), and the last term
+// is one constant per output (k_proj_bias).  The digit
+// matrices are the B operands of v_mfma_i32_16x16x64_i8: tile t holds digit t
+// of 16 coefficients, so the 7 tiles of one output sit in the same lane and
+// register of 7 accumulators and are recombined without a transpose.
+//
+// Block: one chunk, 16 coefficients (blockIdx.y), all 256 outputs; wave w
+// owns outputs 64 w .. 64 w + 63 (4 row tiles) x 7 digit tiles = 28
+// accumulators.  The chunk goes through in stages of 256 elements (4 K steps
+// of 64): thread (ks, g, c) loads coefficient c of elements 16 g .. 16 g + 15
+// of K step ks, turns them into digits and writes the 7 digit tiles' LDS slot
+// 16 g + c (MFMA lane order: column c, K 16 g ..), double buffered, one block
+// barrier per stage.  The signs come pre-tiled (k_proj_cfrag) from L2.
+constexpr int PM_THREADS = 256;
+constexpr int PM_STAGE = 4 * 7 * 1024;  // LDS bytes of one stage's digit tiles
+
+typedef int v4i_pm __attribute__((ext_vector_type(4)));
+
+// Cf[((ks 16 + rt) 64 + L) 16 + b] = C[16 rt + L % 16][64 ks + 16 (L / 16) + b]
+// as an int8 (the A fragments of K step ks, row tile rt, in lane order).
+__global__ void k_proj_cfrag(const uint8_t *__restrict__ zero, const uint8_t *__restrict__ sign,
+                             size_t cw, uint8_t *__restrict__ Cf) {
+    const size_t nslot = (size_t)PJ_H * cw / 16;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nslot;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const size_t ks = e / 1024;
+        const int rt = (int)((e / 64) % 16), L = (int)(e % 64);
+        const size_t off = (size_t)(16 * rt + L % 16) * cw + 64 * ks + 16 * (L / 16);
+        uint32_t w[4];
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint8_t z = zero[off + 4 * h + b], sg = sign[off + 4 * h + b];
+                v |= (uint32_t)(z ? 0u : (sg ? 1u : 0xFFu)) << (8 * b);
+            }
+            w[h] = v;
+        }
+        *reinterpret_cast<uint4 *>(Cf + e * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+// bias[k] = h sum_l C[k][l] mod q with h = (q - 1) / 2: the projection of the
+// constant h, added back to the projection of the shifted coefficients x - h.
+__global__ __launch_bounds__(256) void k_proj_bias(const uint8_t *__restrict__ zero,
+                                                   const uint8_t *__restrict__ sign, size_t cw,
+                                                   uint64_t *__restrict__ bias, Mod M) {
+    __shared__ int part[4];
+    const int k = blockIdx.x, tid = threadIdx.x;
+    int s = 0;
+    for (size_t l = tid; l < cw; l += 256) {
+        const size_t o = (size_t)k * cw + l;
+        s += zero[o] ? 0 : (sign[o] ? 1 : -1);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((tid & 63) == 0) part[tid >> 6] = s;
+    __syncthreads();
+    if (tid == 0) {
+        const int t = part[0] + part[1] + part[2] + part[3];  // |t| <= cw
+        const double h = (double)(((uint64_t)M.q - 1) / 2);
+        bias[k] = d2u(mulmod(h, canon((double)t, M), M));
+    }
+}
+
+__global__ __launch_bounds__(PM_THREADS, 2) void k_project_mfma(
+    const uint64_t *__restrict__ W, size_t m, size_t cw, size_t nch,
+    const uint8_t *__restrict__ Cf, const uint64_t *__restrict__ bias, uint64_t *__restrict__ out,
+    const double *__restrict__ tab, Mod M) {
+    __shared__ __align__(16) char lds[2 * PM_STAGE];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const size_t idx = blockIdx.x, i = idx / nch, j = idx % nch;
+    const int cg = blockIdx.y;
+    // staging role: K step sks, element group sg, coefficient sc
+    const int sks = tid >> 6, sg = (tid >> 4) & 3, sc = tid & 15;
+    const uint64_t *src = W + (i * m + j * cw + 64 * sks + 16 * sg) * kD + 16 * cg + sc;
+    // u = x - h + 0x80..80 (seven 0x80 bytes): 0 <= u < 2^56 for x in [0, q)
+    const uint64_t ushift = 0x0080808080808080ULL - (((uint64_t)M.q - 1) >> 1);
+    const int nst = (int)(cw / 256);
+    uint64_t R[16];
+    auto gload = [&](int st) {
+        const uint64_t *p = src + (size_t)st * 256 * kD;
+#pragma unroll
+        for (int e = 0; e < 16; e++) R[e] = p[e * kD];
+    };
+    auto stage = [&](int buf) {
+        uint32_t d[4][7];
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            uint32_t lo[4], hi[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint64_t u = R[4 * h + b] + ushift;
+                lo[b] = (uint32_t)u;
+                hi[b] = (uint32_t)(u >> 32);
+            }
+            bytes_transpose4(lo, hi, d[h]);
+#pragma unroll
+            for (int t = 0; t < 7; t++) d[h][t] ^= 0x80808080u;  // d_t + 128 -> d_t as int8
+        }
+        char *base = lds + buf * PM_STAGE + sks * 7 * 1024 + (16 * sg + sc) * 16;
+#pragma unroll
+        for (int t = 0; t < 7; t++)
+            *reinterpret_cast<v4i_pm *>(base + t * 1024) = (v4i_pm){(int)d[0][t], (int)d[1][t],
+                                                                    (int)d[2][t], (int)d[3][t]};
+    };
+    v4i_pm acc[4][7];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int t = 0; t < 7; t++) acc[a][t] = (v4i_pm){0, 0, 0, 0};
+    const uint8_t *cf = Cf + (size_t)(4 * w) * 1024 + lane * 16;
+    const int nk = 4 * nst;
+    v4i_pm fa[4], fn[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) fa[a] = *reinterpret_cast<const v4i_pm *>(cf + a * 1024);
+    gload(0);
+    for (int st = 0; st < nst; st++) {
+        stage(st & 1);
+        if (st + 1 < nst) gload(st + 1);
+        __syncthreads();
+        const char *bb = lds + (st & 1) * PM_STAGE + lane * 16;
+#pragma unroll 2
+        for (int ks = 0; ks < 4; ks++) {
+            // the next K step's sign fragments (the last one repeats itself)
+            const int kn = min(4 * st + ks + 1, nk - 1);
+            const uint8_t *ca = cf + (size_t)kn * 16 * 1024;
+            // issued here, ahead of this step's products (the barriers keep the
+            // compiler from sinking them to the end of the step)
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int a = 0; a < 4; a++) fn[a] = *reinterpret_cast<const v4i_pm *>(ca + a * 1024);
+            __builtin_amdgcn_sched_barrier(0);
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int t = 0; t < 7; t++) {
+                const v4i_pm fb = *reinterpret_cast<const v4i_pm *>(bb + (ks * 7 + t) * 1024);
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+                    acc[a][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[a], fb, acc[a][t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++) fa[a] = fn[a];
+        }
+    }
+    // recombination: lane (g, lr) holds rows 16 (4 w + a) + 4 g + v, column lr.
+    // sum_t 2^(8t) acc_t = lo + 2^32 hi with lo = acc_0 + 2^8 acc_1 + 2^16 acc_2
+    // + 2^24 acc_3 and hi = acc_4 + 2^8 acc_5 + 2^16 acc_6, both exact in fp64
+    // (|acc_t| <= 128 cw < 2^23: |lo| < 2^48, |hi| < 2^40), and one modular
+    // product for 2^32 hi.
+    const double w32 = tab[TAB_W256 + 4];
+    const int lr = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            double d[7];
+#pragma unroll
+            for (int t = 0; t < 7; t++) d[t] = (double)acc[a][t][v];
+            const double lo =
+                __fma_rn(d[3], 16777216.0, __fma_rn(d[2], 65536.0, __fma_rn(d[1], 256.0, d[0])));
+            const double hi = __fma_rn(d[6], 65536.0, __fma_rn(d[5], 256.0, d[4]));
+            const int k = 16 * (4 * w + a) + 4 * g + v;
+            const double x = lo + mul_lazy(hi, w32, M) + u2d(bias[k]);  // |x| < 2^48 + 2q
+            out[(idx * PJ_H + k) * kD + 16 * cg + lr] = d2u(reduce(x, M));
+        }
+}
+
 // out[i][l] = sum_j src[i][j cw + l] mod q (j < nch) for rows i < rows of
 // length m, per coefficient (u64 sums, reduced every 8192 terms).
 __global__ void k_chunk_sums(const uint64_t *__restrict__ src, size_t rows, size_t m, size_t cw,
@@ -152,8 +331,24 @@ int launch_project(Context *ctx, const uint64_t *W, size_t r, size_t m, const ui
     const size_t nch = m / cw;
     SALSAA_CHECK(r * nch < (1u << 31), "projection too large");
     Mod M{ctx->qd, ctx->qinv};
-    hipLaunchKernelGGL(k_project<false>, dim3((unsigned)(r * nch), PJ_H / PJ_KB), dim3(PJ_THREADS),
-                       0, ctx->stream, W, m, cw, nch, is_zero, sign, out, M);
+    if (ctx->mfma_project) {
+        // scratch: the tiled signs [256 cw] bytes, then the biases [256]
+        uint8_t *Cf = nullptr;
+        SALSAA_TRY(scratch_get(ctx, 10, (size_t)PJ_H * cw + PJ_H * sizeof(uint64_t), (void **)&Cf));
+        uint64_t *bias = reinterpret_cast<uint64_t *>(Cf + (size_t)PJ_H * cw);
+        const size_t nslot = (size_t)PJ_H * cw / 16;
+        hipLaunchKernelGGL(k_proj_cfrag, dim3(ceil_div(nslot, (size_t)256)), dim3(256), 0,
+                           ctx->stream, is_zero, sign, cw, Cf);
+        SALSAA_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_proj_bias, dim3(PJ_H), dim3(256), 0, ctx->stream, is_zero, sign, cw,
+                           bias, M);
+        SALSAA_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_project_mfma, dim3((unsigned)(r * nch), kD / 16), dim3(PM_THREADS), 0,
+                           ctx->stream, W, m, cw, nch, Cf, bias, out, ctx->d_tables, M);
+    } else {
+        hipLaunchKernelGGL(k_project<false>, dim3((unsigned)(r * nch), PJ_H / PJ_KB),
+                           dim3(PJ_THREADS), 0, ctx->stream, W, m, cw, nch, is_zero, sign, out, M);
+    }
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }

@@ -160,7 +160,7 @@ def test_power_series_then_dot_vs_oracle(ctx, orc):
 
 
 # ------------------------------------------------------------------ projection
-@pytest.mark.parametrize("r,m", [(1, 256), (2, 1536), (4, 2048)])
+@pytest.mark.parametrize("r,m", [(1, 256), (2, 1536), (3, 1536), (4, 2048), (5, 2560)])
 def test_project_vs_oracle(ctx, orc, r, m):
     # m = 1536 at r = 2: the last 512 elements of each row are outside every chunk
     say("case: project r=%d m=%d" % (r, m))
@@ -184,6 +184,16 @@ def test_project_extremes_and_checks(sa, ctx, orc):
         sg = np.full((256, r * 256), sv, np.uint8)
         got = ctx.project(dW, ctx.to_device(z, np.uint8), ctx.to_device(sg, np.uint8)).download()
         assert np.array_equal(got, orc.project(W, z, sg, Q)), (zv, sv)
+    # the balanced extremes (q - 1) / 2 and (q + 1) / 2 (the largest digits),
+    # 0 and 1, with every sign +1 and with random signs
+    W2 = np.resize(np.array([(Q - 1) // 2, (Q + 1) // 2, 0, 1, Q - 1], np.uint64), W.shape)
+    rng = np.random.default_rng(5)
+    for z, sg in ((np.zeros((256, r * 256), np.uint8), np.ones((256, r * 256), np.uint8)),
+                  (rng.integers(0, 2, (256, r * 256), dtype=np.uint8),
+                   rng.integers(0, 2, (256, r * 256), dtype=np.uint8))):
+        got = ctx.project(ctx.to_device(W2), ctx.to_device(z, np.uint8),
+                          ctx.to_device(sg, np.uint8)).download()
+        assert np.array_equal(got, orc.project(W2, z, sg, Q))
     z = np.zeros((256, 2 * 256), np.uint8)
     with pytest.raises(sa.SalsaaError):
         ctx.project(ctx.to_device(np.zeros((2, 256, 64), np.uint64)),
