@@ -24,7 +24,8 @@ norm-checked, fused vs separate calls), the single-instance full-size step
 (NTT of 2^20 ring elements; commit A 32 x 16384 . W 8 x 16384; norm check
 8 x 2^20; decomp_ell base 2^8 x 6 with verify_decomp) and the projection
 ("projection": project, batch_projections and the cross terms of an
-8 x 2^19 witness).  All inputs are generated on the device from seeds before
+8 x 2^19 witness) and the power-series CRS key ("crs": 13 x 2^21, generated
+on the device vs stored rows).  All inputs are generated on the device from seeds before
 timing starts; the batched and single-instance outputs stay in HBM.
 """
 import argparse
@@ -485,6 +486,55 @@ def projection(sa, ctx, steps, cpu_budget_s=8.0):
             "bit_exact_sampled": exact, "cpu_baseline_project": cpu}
 
 
+CRS_KAPPA, CRS_M = 13, 1 << 21  # the F2 run of the published results (results.md:19)
+
+
+def crs(sa, ctx, steps):
+    """The power-series CRS key (compute_commitment_keys, crs.rs:43-72) of 13
+    module elements with rows of 2^21 ring elements, generated on the device
+    straight into the commit's digit planes, against the stored-key path (the
+    Coefficient rows, 13.6 GB, then salsaa_key_create).  Both keys commit one
+    witness row of 2^21 elements to the same result."""
+    mod = ctx.uniform((CRS_KAPPA, 64), 9, sa.STREAM_A)
+    W = ctx.bounded((1, CRS_M, 64), 16, 9, sa.STREAM_W)
+    keys = []
+
+    def gen():
+        keys.append(ctx.crs_key(mod, CRS_M))
+        while len(keys) > 1:
+            keys.pop(0).close()
+
+    gen()
+    ctx.sync()
+    gper, gtot, _ = timed(ctx, gen, max(steps // 4, 2))
+    rows = ctx.compute_commitment_keys(mod, CRS_M)
+    skeys = []
+
+    def stored():
+        skeys.append(ctx.key(rows))
+        while len(skeys) > 1:
+            skeys.pop(0).close()
+
+    stored()
+    ctx.sync()
+    sper, stot, _ = timed(ctx, stored, max(steps // 4, 2))
+    a = ctx.parallel_dot_series_matrix(keys[-1], W).download()
+    b = ctx.parallel_dot_series_matrix(skeys[-1], W).download()
+    n = max(steps // 4, 2)
+    gms, sms = gtot / n, stot / n
+    plane_bytes = 64 * ((7 * CRS_KAPPA + 15) // 16) * 1024 * (CRS_M // 64)
+    for k in keys + skeys:
+        k.close()
+    for x in (rows, W, mod):
+        x.free()
+    return {"workload": "CRS key, 13 module elements x 2^21 powers (the F2 parameters): "
+                        "generated on the device vs stored Coefficient rows + key_create",
+            "generated_ms": round(gms, 3), "stored_key_create_ms": round(sms, 3),
+            "plane_GB/s": round(plane_bytes / (gms * 1e-3) / 1e9, 1),
+            "coefficient_rows_not_stored_GB": round(CRS_KAPPA * CRS_M * 512 / 1e9, 2),
+            "bit_exact_commit": bool(np.array_equal(a, b))}
+
+
 def timed(ctx, fn, k):
     evs = [ctx.event() for _ in range(k + 1)]
     ctx.sync()
@@ -542,6 +592,7 @@ def main():
     ap.add_argument("--no-folding", action="store_true", help="skip the folding step")
     ap.add_argument("--no-projection", action="store_true",
                     help="skip the projection / batching / cross terms")
+    ap.add_argument("--no-crs", action="store_true", help="skip the CRS key generation")
     ap.add_argument("--eager", action="store_true",
                     help="run the batched steps eagerly instead of replaying the hipGraph "
                          "(for counter collection)")
@@ -671,6 +722,8 @@ def main():
     if not args.no_projection:
         out["projection"] = projection(sa, ctx, args.steps,
                                        0.0 if args.no_cpu_baseline else 8.0)
+    if not args.no_crs:
+        out["crs"] = crs(sa, ctx, args.steps)
     if not args.no_single:
         out["single_instance"] = single_instance(sa, ctx, args.steps)
     if not args.no_cpu_baseline:

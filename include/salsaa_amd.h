@@ -171,6 +171,20 @@ int salsaa_ring_mul(salsaa_ctx *ctx, const uint64_t *d_a, const uint64_t *d_b,
  * layer of length m; commits with a shorter witness use the prefix layer. */
 int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t m,
                       salsaa_key **out);
+/* compute_commitment_keys (crs.rs:43-72), the CRS of CRS::gen_crs
+ * (crs.rs:22-30) for the module elements d_module [kappa][d] (Coefficient):
+ * row j = m_j, m_j^2, ..., m_j^len, d_out [kappa][len][d] Coefficient.  The
+ * reference's PowerSeries keeps this row's prefixes of lengths len, len/2,
+ * ... (while even) as its layers and [1, m_j^c] (= [1, row[c - 1]]) as the
+ * tensor below the layer of length 2c. */
+int salsaa_compute_commitment_keys(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa,
+                                   size_t len, uint64_t *d_out);
+/* The commitment key of that CRS (kappa rows of length m), generated on the
+ * device straight into the resident layout: the same key as
+ * salsaa_key_create on salsaa_compute_commitment_keys' output, without the
+ * [kappa][m][d] matrix (the stored ck of the reference, protocol.rs:132). */
+int salsaa_key_create_crs(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t m,
+                          salsaa_key **out);
 /* A view of the first kappa rows and first m ring elements of a key, sharing
  * its storage (no copy): the counterpart of passing the prefix slice
  * &key[..kappa] of the Vec<PowerSeries> to parallel_dot_series_matrix

@@ -236,6 +236,38 @@ def consecutive_powers(c, width, q):
     return out
 
 
+def compute_commitment_keys(module, length, q):
+    """compute_commitment_keys (crs.rs:43-72): row j = m_j, m_j^2, ...,
+    m_j^length, each power the schoolbook product of the last with m_j.
+    module [kappa][d] Coefficient -> [kappa][length][d]."""
+    module = _u(module)
+    kappa, d = module.shape
+    out = np.zeros((kappa, length, d), np.uint64)
+    for j in range(kappa):
+        out[j, 0] = module[j]
+        for i in range(1, length):
+            out[j, i] = ring_mul_naive(out[j, i - 1], module[j], q)
+    return out
+
+
+def crs_power_series(row):
+    """The PowerSeries of one CRS row (crs.rs:58-69): layers row[:len],
+    row[:len/2], ... while the length is even, and below the layer of length
+    2c the tensor [1, row[c - 1]] (1 as the ring's one).  Returns (layers,
+    tensors), tensors in the order the reference pushes them."""
+    d = row.shape[1]
+    one = np.zeros(d, np.uint64)
+    one[0] = 1
+    layers, tensors = [], []
+    cur = row.shape[0]
+    while cur % 2 == 0:
+        layers.append(row[:cur])
+        cur //= 2
+        tensors.append(np.stack([one, row[cur - 1]]))
+    layers.append(row[:cur])
+    return layers, tensors
+
+
 def _signed_col_sums(is_zero, sign, pw, q):
     """c_l = sum_k C[k][l] pw[k] mod q with C from (is_zero, sign) [H][w]:
     the positive and negative parts summed apart in u64 (H q < 2^64)."""

@@ -115,6 +115,8 @@ def _load():
                                             _vp, _vp, _vp]),
         "salsaa_send_cross_terms_before_join": (_int, [_vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
         "salsaa_key_create": (_int, [_vp, _vp, _sz, _sz, _PP]),
+        "salsaa_key_create_crs": (_int, [_vp, _vp, _sz, _sz, _PP]),
+        "salsaa_compute_commitment_keys": (_int, [_vp, _vp, _sz, _sz, _vp]),
         "salsaa_key_destroy": (_int, [_vp]),
         "salsaa_parallel_dot_series_matrix": (_int, [_vp, _vp, _vp, _sz, _sz, _vp]),
         "salsaa_split": (_int, [_vp, _vp, _vp, _sz, _sz, _vp]),
@@ -218,6 +220,18 @@ class Key:
         p = ctypes.c_void_p()
         _check(lib.salsaa_key_create(ctx.h, A_dev.ptr, kappa, m, ctypes.byref(p)))
         self.h = p.value
+
+    @classmethod
+    def crs(cls, ctx, module, m):
+        """The commitment key of the power-series CRS of `module` [kappa][d]
+        (Coefficient) with rows of length m, generated on the device
+        (compute_commitment_keys, crs.rs:43-72)."""
+        k = cls.__new__(cls)
+        k.ctx, k.kappa, k.m = ctx, module.shape[0], m
+        p = ctypes.c_void_p()
+        _check(lib.salsaa_key_create_crs(ctx.h, module.ptr, k.kappa, m, ctypes.byref(p)))
+        k.h = p.value
+        return k
 
     def view(self, kappa, m):
         """The key's first kappa rows and m ring elements (shares storage;
@@ -376,6 +390,20 @@ class Context:
         kappa, m, d = A_dev.shape
         assert d == self.d
         return Key(self, A_dev, kappa, m)
+
+    def crs_key(self, module, m):
+        """Key.crs: the power-series CRS key of module [kappa][d], length m."""
+        assert module.shape[1:] == (self.d,)
+        return Key.crs(self, module, m)
+
+    def compute_commitment_keys(self, module, length):
+        """compute_commitment_keys (crs.rs:43-72): [kappa][length][d]
+        Coefficient, row j = m_j, m_j^2, ..., m_j^length."""
+        kappa, d = module.shape
+        assert d == self.d
+        out = self.empty((kappa, length, d))
+        _check(lib.salsaa_compute_commitment_keys(self.h, module.ptr, kappa, length, out.ptr))
+        return out
 
     def parallel_dot_series_matrix(self, key, W, out=None):
         r, m, d = W.shape

@@ -345,8 +345,12 @@ int salsaa_ring_mul(salsaa_ctx *ctx, const uint64_t *d_a, const uint64_t *d_b, u
     return launch_ring_mul(&ctx->c, d_a, d_b, d_out, count);
 }
 
-int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t m,
-                      salsaa_key **out) {
+}  // extern "C"
+
+// A key of kappa rows and length m whose device layout `fill` writes: the
+// MFMA commit's digit planes (kappa <= kMfmaMaxKappa) or slot-major doubles.
+template <typename Fill>
+static int key_make(salsaa_ctx *ctx, size_t kappa, size_t m, salsaa_key **out, Fill fill) {
     CTX_OK(ctx);
     SALSAA_CHECK(out != nullptr, "null output pointer");
     SALSAA_CHECK(kappa >= 1 && m >= 1, "key needs kappa, m >= 1");
@@ -367,8 +371,7 @@ int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t
             set_error(std::string("key allocation: ") + hipGetErrorString(e));
             return SALSAA_ERR_NOMEM;
         }
-        s = launch_ntt_full_to_planes(&ctx->c, d_A, kappa, m, kappa, m * kD, 0, k->k.d_AB,
-                                      k->k.a_ktiles);
+        s = fill(true, k->k);
     } else {
         hipError_t e = hipMalloc(&k->k.d_A, (size_t)kD * kappa * m * sizeof(double));
         if (e != hipSuccess) {
@@ -376,7 +379,7 @@ int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t
             set_error(std::string("key allocation: ") + hipGetErrorString(e));
             return SALSAA_ERR_NOMEM;
         }
-        s = launch_ntt_full_to_slotmajor(&ctx->c, d_A, kappa, m, kappa, m * kD, 0, k->k.d_A);
+        s = fill(false, k->k);
     }
     if (s != SALSAA_OK) {
         (void)hipFree(k->k.d_A);
@@ -386,6 +389,33 @@ int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t
     }
     *out = k;
     return SALSAA_OK;
+}
+
+extern "C" {
+
+int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t m,
+                      salsaa_key **out) {
+    return key_make(ctx, kappa, m, out, [&](bool planes, Key &k) {
+        return planes ? launch_ntt_full_to_planes(&ctx->c, d_A, kappa, m, kappa, m * kD, 0, k.d_AB,
+                                                  k.a_ktiles)
+                      : launch_ntt_full_to_slotmajor(&ctx->c, d_A, kappa, m, kappa, m * kD, 0,
+                                                     k.d_A);
+    });
+}
+
+int salsaa_key_create_crs(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t m,
+                          salsaa_key **out) {
+    return key_make(ctx, kappa, m, out, [&](bool planes, Key &k) {
+        return planes ? launch_crs(&ctx->c, d_module, kappa, m, 2, k.d_AB, k.a_ktiles)
+                      : launch_crs(&ctx->c, d_module, kappa, m, 1, k.d_A, 0);
+    });
+}
+
+int salsaa_compute_commitment_keys(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa,
+                                   size_t len, uint64_t *d_out) {
+    CTX_OK(ctx);
+    SALSAA_TRY(launch_crs(&ctx->c, d_module, kappa, len, 0, d_out, 0));
+    return launch_ntt(&ctx->c, d_out, kappa * len, SALSAA_REP_FULL, 1);
 }
 
 int salsaa_key_view(const salsaa_key *key, size_t kappa, size_t m, salsaa_key **out) {

@@ -80,7 +80,7 @@ struct Context {
     hipStream_t stream = nullptr;
     double *d_tables = nullptr;     // TAB_SIZE doubles
     std::vector<double> h_tables;   // host copy
-    ScratchBuf scratch[11];         // grown on demand, owned by the context
+    ScratchBuf scratch[12];         // grown on demand, owned by the context
     bool mfma_commit = true;        // int8 MFMA commit (false: fp64 VALU commit)
     bool mfma_project = true;       // int8 MFMA projection (false: VALU signed sums)
     ScratchBuf zeros;               // zero-filled, grown on demand
@@ -288,6 +288,11 @@ int launch_ntt_full_to_slotmajor(Context *ctx, const uint64_t *src, size_t rows,
 int launch_decomp_to_slotmajor(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
                                const uint64_t *d_radix, uint64_t *new_W, void *dst, int planes,
                                size_t pstride);
+// the power-series CRS rows (crs.rs:43-72) of `rows` module elements (Coefficient),
+// length m: mode 0 NTT-form rows [rows][m][d], 1 a key's slot-major doubles, 2 a
+// key's digit planes with pstride K tiles
+int launch_crs(Context *ctx, const uint64_t *module, size_t rows, size_t m, int mode, void *dst,
+               size_t pstride);
 int launch_ntt_full_to_planes(Context *ctx, const uint64_t *src, size_t rows, size_t m,
                               size_t row_lo, size_t stride_lo, size_t stride_hi, int8_t *dst,
                               size_t pstride);

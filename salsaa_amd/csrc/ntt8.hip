@@ -785,6 +785,78 @@ __global__ __launch_bounds__(NT) void k_decomp_sm(const uint64_t *__restrict__ W
     }
 }
 
+// ---- the power-series CRS (compute_commitment_keys, crs.rs:43-72) on the device
+//
+// Row j of the CRS is m_j, m_j^2, ..., m_j^len.  In the full NTT domain each
+// slot of a power is the slot's scalar power, so element i of row j is
+// mhat_j[s]^(i + 1) slot by slot.  tpow[j][e][s] = mhat_j[s]^e for e < 128
+// (k_crs_table); a block of the 128 elements i0 .. of row j takes
+// b[s] = mhat_j[s]^(i0 + 1) once (square and multiply) and then one product
+// per value, and writes the values where the key's layout wants them.
+
+__device__ __forceinline__ double crs_pow(double b, unsigned long long e, Mod M) {
+    double r = 1.0;
+    while (e) {
+        if (e & 1) r = mulmod(r, b, M);
+        b = mulmod(b, b, M);
+        e >>= 1;
+    }
+    return r;
+}
+
+__global__ void k_crs_table(const uint64_t *__restrict__ mhat, size_t rows,
+                            uint64_t *__restrict__ tpow, Mod M) {
+    const size_t total = rows * EL * kD;
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < total;
+         x += (size_t)gridDim.x * blockDim.x) {
+        const size_t j = x / (EL * kD);
+        const int e = (int)((x / kD) % EL), s = (int)(x % kD);
+        tpow[x] = d2u(crs_pow(u2d(mhat[j * kD + s]), (unsigned long long)e, M));
+    }
+}
+
+// grid (ceil(m / 128), rows).  MODE 0: NTT-form rows dst[(j m + i) 64 + s];
+// 1: the key's slot-major doubles; 2: the key's digit planes with `pstride`
+// K tiles (as full_sm_block writes them from a Coefficient key).
+template <int MODE>
+__global__ __launch_bounds__(NT) void k_crs_sm(const uint64_t *__restrict__ mhat,
+                                               const uint64_t *__restrict__ tpow, size_t rows,
+                                               size_t m, void *__restrict__ dst, size_t pstride,
+                                               Mod M) {
+    __shared__ uint64_t lds[EL * SROW];
+    __shared__ double bp[kD];
+    const size_t row = blockIdx.y, i0 = (size_t)blockIdx.x * EL;
+    const int nel = (int)min((size_t)EL, m - i0);
+    const int tid = threadIdx.x, e = tid >> 3, t = tid & 7;
+    if (tid < kD) bp[tid] = crs_pow(u2d(mhat[row * kD + tid]), (unsigned long long)(i0 + 1), M);
+    __syncthreads();
+    const uint64_t *tp = tpow + (row * EL + e) * kD;
+    if (MODE == 0) {
+        if (e < nel) {
+            uint64_t *o = reinterpret_cast<uint64_t *>(dst) + (row * m + i0 + e) * kD;
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                const int s = t + 8 * x;
+                o[s] = d2u(mulmod(bp[s], u2d(tp[s]), M));
+            }
+        }
+        return;
+    }
+    if (e < nel) {
+#pragma unroll
+        for (int x = 0; x < 8; x++) {
+            const int s = 8 * t + x;
+            lds[wpos(s, e)] = dbits(mulmod(bp[s], u2d(tp[s]), M));
+        }
+    }
+    __syncthreads();
+    if (MODE == 2)
+        store_planes(plane_base((int8_t *)dst, rows, row, i0), plane_sstride(rows, pstride),
+                     plane_kstride(rows), lds, nel, tid);
+    else
+        store_slotmajor((double *)dst, lds, rows, row, m, i0, nel, tid);
+}
+
 }  // namespace
 
 // resident blocks per CU of a kernel (occupancy API, cached)
@@ -918,6 +990,37 @@ int launch_decomp_to_slotmajor(Context *ctx, const uint64_t *W, size_t r, size_t
         hipLaunchKernelGGL(k_decomp_sm<false>, grid, dim3(NT), 0, ctx->stream, W, r, m, parts,
                            d_radix, new_W, dst, (size_t)0, ctx->q, ctx->d_tables, M);
     }
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_crs(Context *ctx, const uint64_t *module, size_t rows, size_t m, int mode, void *dst,
+               size_t pstride) {
+    SALSAA_CHECK(rows >= 1 && rows < 65536 && m >= 1, "CRS size out of range");
+    SALSAA_CHECK(mode >= 0 && mode <= 2, "unknown CRS layout");
+    SALSAA_CHECK(mode != 2 || pstride == (m + EL - 1) / EL * 2, "K-tiles must be m rounded up to 128, over 64");
+    Mod M{ctx->qd, ctx->qinv};
+    // scratch: the module in NTT form [rows][64], then tpow [rows][128][64]
+    uint64_t *buf = nullptr;
+    SALSAA_TRY(scratch_get(ctx, 11, (rows + rows * EL) * kD * sizeof(uint64_t), (void **)&buf));
+    uint64_t *mhat = buf, *tpow = buf + rows * kD;
+    SALSAA_HIP(hipMemcpyAsync(mhat, module, rows * kD * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    SALSAA_TRY(launch_ntt(ctx, mhat, rows, SALSAA_REP_FULL, 0));
+    const size_t nt = rows * EL * kD;
+    hipLaunchKernelGGL(k_crs_table, dim3((unsigned)std::min<size_t>(ceil_div(nt, (size_t)256), 4096)),
+                       dim3(256), 0, ctx->stream, mhat, rows, tpow, M);
+    SALSAA_HIP(hipGetLastError());
+    dim3 grid(ceil_div(m, EL), (unsigned)rows);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_crs_sm<0>, grid, dim3(NT), 0, ctx->stream, mhat, tpow, rows, m, dst,
+                           pstride, M);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_crs_sm<1>, grid, dim3(NT), 0, ctx->stream, mhat, tpow, rows, m, dst,
+                           pstride, M);
+    else
+        hipLaunchKernelGGL(k_crs_sm<2>, grid, dim3(NT), 0, ctx->stream, mhat, tpow, rows, m, dst,
+                           pstride, M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }

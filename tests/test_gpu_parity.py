@@ -159,6 +159,40 @@ def test_power_series_then_dot_vs_oracle(ctx, orc):
     assert np.array_equal(got, want)
 
 
+# ------------------------------------------------------------------ CRS
+@pytest.mark.parametrize("kappa,length", [(2, 256), (3, 300), (1, 1)])
+def test_compute_commitment_keys_vs_oracle(ctx, orc, kappa, length):
+    say("case: compute_commitment_keys kappa=%d len=%d" % (kappa, length))
+    module = rand_ring(orc, (kappa, 64), 61 + kappa, 0xC)
+    got = ctx.compute_commitment_keys(ctx.to_device(module), length).download()
+    assert np.array_equal(got, orc.compute_commitment_keys(module, length, Q))
+
+
+@pytest.mark.parametrize("kappa,m,r", [(4, 200, 3), (16, 1024, 8), (40, 64, 2)])
+def test_crs_key_commit_vs_oracle(ctx, orc, kappa, m, r):
+    # the key generated on the device (digit planes for kappa <= 36, slot-major
+    # doubles above) commits as the key made from the oracle's CRS rows, and as
+    # the oracle; prefix views too
+    say("case: CRS key kappa=%d m=%d r=%d" % (kappa, m, r))
+    module = rand_ring(orc, (kappa, 64), 70 + kappa, 0xC)
+    A = orc.compute_commitment_keys(module, m, Q)
+    W = rand_ring(orc, (r, m, 64), 71, 0xB, 16)
+    dW = ctx.to_device(W)
+    key = ctx.crs_key(ctx.to_device(module), m)
+    got = ctx.parallel_dot_series_matrix(key, dW).download()
+    assert np.array_equal(got, orc.commit(A, W, Q, threads=8))
+    ref_key = ctx.key(ctx.to_device(A))
+    assert np.array_equal(got, ctx.parallel_dot_series_matrix(ref_key, dW).download())
+    k2, m2 = max(1, kappa // 2), m // 2
+    v = key.view(k2, m2)
+    W2 = np.ascontiguousarray(W[:, :m2])
+    got2 = ctx.parallel_dot_series_matrix(v, ctx.to_device(W2)).download()
+    assert np.array_equal(got2, orc.commit(np.ascontiguousarray(A[:k2, :m2]), W2, Q, threads=8))
+    v.close()
+    ref_key.close()
+    key.close()
+
+
 # ------------------------------------------------------------------ projection
 @pytest.mark.parametrize("r,m", [(1, 256), (2, 1536), (3, 1536), (4, 2048), (5, 2560)])
 def test_project_vs_oracle(ctx, orc, r, m):

@@ -594,6 +594,36 @@ def test_cross_terms_are_row0_products(orc):
     assert np.array_equal(l0, orc.dot_single_series(plhs, W[:1], Q)[0])
 
 
+def test_crs_monomial(orc):
+    # compute_commitment_keys (crs.rs:43-72) of m = X: row[i] = X^(i+1) in R_q
+    d = 8
+    X = np.zeros((1, d), np.uint64)
+    X[0, 1] = 1
+    row = orc.compute_commitment_keys(X, 40, Q)[0]
+    for i in range(40):
+        k = i + 1
+        want = np.zeros(d, np.uint64)
+        want[k % d] = 1 if (k // d) % 2 == 0 else Q - 1
+        assert np.array_equal(row[i], want), i
+
+
+def test_crs_layers_form_a_power_series(orc):
+    # crs.rs:58-69: the layers are the row's prefixes and the tensors
+    # [1, m^c] rebuild each layer from the one below (complete_power_series,
+    # tensors in the order the reference pushes them = most significant first)
+    d = 8
+    rng = random.Random(51)
+    module = np.array([rnd(rng, d, Q), rnd(rng, d, Q)], np.uint64)
+    rows = orc.compute_commitment_keys(module, 24, Q)
+    for row in rows:
+        layers, tensors = orc.crs_power_series(row)
+        assert [len(x) for x in layers] == [24, 12, 6, 3] and len(tensors) == 3
+        full = orc.complete_power_series(layers[-1], np.stack(tensors), Q)
+        assert np.array_equal(full, row)
+        # a reversed tensor order (a plausible slip) does not rebuild it
+        assert not np.array_equal(orc.complete_power_series(layers[-1], np.stack(tensors[::-1]), Q), row)
+
+
 # ---------------------------------------------------------------- sampling
 def test_samplers_support(orc):
     u = orc.sample_uniform(0, 1, (4096,), Q)
