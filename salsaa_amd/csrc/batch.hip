@@ -128,7 +128,11 @@ struct salsaa_batch {
     std::vector<int> group_stream;       // which norm-check stream runs each class
     Context nc_ctx[kNcStreams];          // the context's tables on streams of their own
     NcScratch nc[kNcStreams];
-    hipEvent_t ev_fork = nullptr, ev_done[kNcStreams] = {};
+    hipEvent_t ev_fork = nullptr, ev_fork2 = nullptr, ev_done[kNcStreams] = {};
+    // the step's NTT pass in two launches: arena tiles [split_tile, end) hold
+    // the length classes [split_group, end), whose norm checks run while the
+    // second launch transforms tiles [0, split_tile) (split_group = 0: one launch)
+    size_t split_tile = 0, split_group = 0;
     int cpt_first = 0;
     float t_commit = 0.f, t_nc = 0.f;
     // plain path
@@ -234,15 +238,16 @@ int run_commit(salsaa_batch *b) {
     return SALSAA_OK;
 }
 
-// Enqueues the norm checks on their streams after the work already on the
-// context's stream; norm_checks_join makes the context's stream wait for them.
-int norm_checks_fork(salsaa_batch *b, int cpt, const ShortCopy *sc) {
+// Enqueues the norm checks of length classes [g0, g1) on their streams after
+// the work already on the context's stream (marked by `ev`).
+int norm_checks_fork_range(salsaa_batch *b, int cpt, const ShortCopy *sc, size_t g0, size_t g1,
+                           hipEvent_t ev) {
     Context *ctx = &b->ctx->c;
     const Outputs &o = b->out[0];
-    SALSAA_HIP(hipEventRecord(b->ev_fork, ctx->stream));
+    SALSAA_HIP(hipEventRecord(ev, ctx->stream));
     for (int k = 0; k < kNcStreams; k++)
-        SALSAA_HIP(hipStreamWaitEvent(b->nc_ctx[k].stream, b->ev_fork, 0));
-    for (size_t gi = 0; gi < b->groups.size(); gi++) {
+        SALSAA_HIP(hipStreamWaitEvent(b->nc_ctx[k].stream, ev, 0));
+    for (size_t gi = g0; gi < g1; gi++) {
         const auto &g = b->groups[gi];
         const int k = b->group_stream[gi];
         const size_t p = g.first, nr = b->nrounds[p];
@@ -267,9 +272,20 @@ int norm_checks_fork(salsaa_batch *b, int cpt, const ShortCopy *sc) {
                                         b->d_cs + b->off_c[p], b->reps * nr * 2, b->reps, b->fq2,
                                         b->nr, no, cpt, &b->nc[k], sc));
     }
+    return SALSAA_OK;
+}
+
+int norm_checks_mark_done(salsaa_batch *b) {
     for (int k = 0; k < kNcStreams; k++)
         SALSAA_HIP(hipEventRecord(b->ev_done[k], b->nc_ctx[k].stream));
     return SALSAA_OK;
+}
+
+// Enqueues every norm check on their streams after the work already on the
+// context's stream; norm_checks_join makes the context's stream wait for them.
+int norm_checks_fork(salsaa_batch *b, int cpt, const ShortCopy *sc) {
+    SALSAA_TRY(norm_checks_fork_range(b, cpt, sc, 0, b->groups.size(), b->ev_fork));
+    return norm_checks_mark_done(b);
 }
 
 int norm_checks_join(salsaa_batch *b) {
@@ -288,13 +304,33 @@ int run_fwd_planes(salsaa_batch *b, int roundtrip) {
                                  roundtrip, roundtrip ? &b->sc : nullptr);
 }
 
+// the round trip over arena tiles [t0, t1) (the pass's tables and short copy
+// are indexed from the range's first tile)
+int run_rt_range(salsaa_batch *b, size_t t0, size_t t1) {
+    ShortCopy sc{b->sc.t + t0 * kShortTile, b->sc.flag + t0};
+    return launch_ntt_fwd_planes(&b->ctx->c, b->d_wit + t0 * kShortTile,
+                                 (t1 - t0) * kPlaneTile, b->d_planes, b->d_ptiles + t0, 1, &sc);
+}
+
 // one pass: forward NTT, the commit's digit planes, the inverse NTT and the
 // short copy; then the commit on the context's stream while the norm checks
 // (which need only the Coefficient witnesses, or their short copy) run beside
-// it on theirs
+// it on theirs.  With a split, the pass first covers the longest classes,
+// whose norm checks then run beside the pass over the rest.
 int run_step(salsaa_batch *b) {
-    SALSAA_TRY(run_fwd_planes(b, 1));
-    SALSAA_TRY(norm_checks_fork(b, b->cpt_first, &b->sc));
+    if (b->split_group == 0) {
+        SALSAA_TRY(run_fwd_planes(b, 1));
+        SALSAA_TRY(norm_checks_fork(b, b->cpt_first, &b->sc));
+    } else {
+        const size_t nt = b->tot_w / kPlaneTile;
+        SALSAA_TRY(run_rt_range(b, b->split_tile, nt));
+        SALSAA_TRY(norm_checks_fork_range(b, b->cpt_first, &b->sc, b->split_group,
+                                          b->groups.size(), b->ev_fork));
+        SALSAA_TRY(run_rt_range(b, 0, b->split_tile));
+        SALSAA_TRY(norm_checks_fork_range(b, b->cpt_first, &b->sc, 0, b->split_group,
+                                          b->ev_fork2));
+        SALSAA_TRY(norm_checks_mark_done(b));
+    }
     SALSAA_TRY(run_commit(b));
     return norm_checks_join(b);
 }
@@ -389,6 +425,7 @@ void destroy(salsaa_batch *b) {
         if (b->ev_done[k]) (void)hipEventDestroy(b->ev_done[k]);
     }
     if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_fork2) (void)hipEventDestroy(b->ev_fork2);
     delete b;
 }
 
@@ -575,6 +612,26 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
         p = q;
     }
     SALSAA_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    SALSAA_HIP(hipEventCreateWithFlags(&b->ev_fork2, hipEventDisableTiming));
+    // The split: the longest classes while they hold at most 70 % of the arena
+    // (their norm checks, at about 0.4 of the pass's time per element, then fit
+    // beside the pass over the rest).  SALSAA_STEP_SPLIT=0 keeps one pass.
+    {
+        const char *env = getenv("SALSAA_STEP_SPLIT");
+        const bool want = !(env && std::string(env) == "0");
+        size_t gsplit = b->groups.size(), acc = 0;
+        while (want && gsplit > 1) {
+            const auto &g = b->groups[gsplit - 1];
+            const size_t e = g.rows * (g.L / kD);
+            if (10 * (acc + e) > 7 * b->tot_w) break;
+            acc += e;
+            gsplit--;
+        }
+        if (gsplit < b->groups.size() && gsplit > 0) {
+            b->split_group = gsplit;
+            b->split_tile = b->off_w[b->groups[gsplit].first] / kPlaneTile;
+        }
+    }
     for (int k = 0; k < kNcStreams; k++) {
         share_context(*c, b->nc_ctx[k]);
         SALSAA_HIP(hipStreamCreateWithFlags(&b->nc_ctx[k].stream, hipStreamNonBlocking));

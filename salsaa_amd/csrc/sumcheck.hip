@@ -399,21 +399,14 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
     int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
     double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc,
-    const double *__restrict__ tab, Mod M, const uint32_t *__restrict__ shf, int sbn) {
+    const double *__restrict__ tab, Mod M, const uint32_t *__restrict__ shf, int sbn, int nbx) {
     __shared__ __align__(16) uint64_t T[16 * GR_LD];
     __shared__ GrCoef C;
     __shared__ double gpart[4 * 256];
     const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t S = L >> 4;
     const size_t base = wrow ? wrow[row] : (size_t)row * L;
-    // sub-blocks sbn x .. sbn x + sbn - 1 of cpt GR_X columns each; those
-    // that lie in short tiles are k_sc_gram_short's (the set is uniform)
-    const size_t span = (size_t)cpt * GR_X, sb0 = (size_t)blockIdx.x * sbn;
-    uint32_t todo = 0;
-    for (int k = 0; k < sbn && (sb0 + k) * span < S; k++)
-        if (shf == nullptr || !gr_compact(shf, base, S, (sb0 + k) * span, cpt)) todo |= 1u << k;
-    if (todo == 0) return;
-    gr_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
+    const size_t span = (size_t)cpt * GR_X;
     const uint64_t *wr = w + base;
     // the short path's u32 view of the wave's own columns of T: row y at
     // T32 + 2 GR_LD y (the first half of the row's 64 u64 slots)
@@ -424,6 +417,21 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     v4i acc[13];
 #pragma unroll
     for (int k = 0; k < 13; k++) acc[k] = (v4i){0, 0, 0, 0};
+    // super-blocks x = blockIdx.x, + gridDim.x, .. < nbx: sub-blocks sbn x ..
+    // sbn x + sbn - 1 of cpt GR_X columns each; those that lie in short tiles
+    // are k_sc_gram_short's (the set is uniform).  The challenge tables are
+    // made on the first sub-block to do; a block with none returns.
+    bool ready = false;
+    for (size_t bxi = blockIdx.x; bxi < (size_t)nbx; bxi += gridDim.x) {
+    const size_t sb0 = bxi * sbn;
+    uint32_t todo = 0;
+    for (int k = 0; k < sbn && (sb0 + k) * span < S; k++)
+        if (shf == nullptr || !gr_compact(shf, base, S, (sb0 + k) * span, cpt)) todo |= 1u << k;
+    if (todo == 0) continue;
+    if (!ready) {
+        gr_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
+        ready = true;
+    }
     for (int k = 0; k < sbn; k++) {
         if (!((todo >> k) & 1)) continue;
         size_t xb = (sb0 + k) * span;
@@ -532,6 +540,8 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
             xb += GR_X;
         }
     }
+    }
+    if (!ready) return;
     if (GRAM) gr_finish<13>(acc, gpart, wave, lane, tid, tab, gacc, row, M);
 }
 
@@ -983,6 +993,11 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         const unsigned bx = (unsigned)ceil_div(nsub, (size_t)sbn);
         unsigned long long *gacc = cacc + rows;
         dim3 grid(bx, (unsigned)rows);
+        // With a short copy the general kernel only takes the blocks the copy
+        // cannot serve, usually none: a grid of about one block per CU walks
+        // the super-blocks instead of one heavy block each.
+        dim3 ggrid = grid;
+        if (sc) ggrid.x = (unsigned)std::min<size_t>(bx, std::max<size_t>(1, (size_t)ctx->ncu / rows));
         for (size_t r0 = 0; r0 < reps; r0 += GR_MAXREP) {
             const int nrep = (int)std::min<size_t>(GR_MAXREP, reps - r0);
             // with a short copy, its short blocks first, then the others
@@ -996,13 +1011,13 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
                                    cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn);
             if (r0 == 0)
-                hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0, ctx->stream,
-                                   w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr, cpt, bufA,
-                                   gacc, ctx->d_tables, M, shf, sbn);
+                hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, true>), ggrid, dim3(SC_THREADS), 0,
+                                   ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
+                                   nr, cpt, bufA, gacc, ctx->d_tables, M, shf, sbn, (int)bx);
             else
-                hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0, ctx->stream,
-                                   w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr, cpt, bufA,
-                                   gacc, ctx->d_tables, M, shf, sbn);
+                hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, false>), ggrid, dim3(SC_THREADS), 0,
+                                   ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
+                                   nr, cpt, bufA, gacc, ctx->d_tables, M, shf, sbn, (int)bx);
             SALSAA_HIP(hipGetLastError());
         }
         hipLaunchKernelGGL((k_sc_gram_msgs<FQ2, SNR>), dim3((unsigned)rows, (unsigned)reps), dim3(128),

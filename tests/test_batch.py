@@ -227,3 +227,34 @@ def test_batch_short_copy_mixed(sa, ctx, orc):
         same(snapshot(b, False), plain)
     b.close()
     key.close()
+
+
+def test_batch_split_pass_equals_single_pass(sa, ctx, orc):
+    # The step's NTT pass runs in two launches (the longest classes first, so
+    # their norm checks overlap the rest of the pass) unless SALSAA_STEP_SPLIT=0
+    # at batch creation: both orders give the plain path's outputs.
+    import os
+    say("case: batch split pass vs single pass")
+    m = [1024, 1024, 512, 256, 256, 128, 64, 32, 8, 1]
+    kappa = [8, 32, 16, 8, 16, 32, 8, 16, 8, 32]
+    n = len(m)
+    A_h = orc.sample_uniform(8, sa.STREAM_A, (32, 1024, 64), Q)
+    key = ctx.key(ctx.to_device(A_h))
+    outs = []
+    for env in ("1", "0"):
+        os.environ["SALSAA_STEP_SPLIT"] = env
+        try:
+            b = sa.Batch(ctx, key, m, kappa, reps=2, fq2=1, tune=False)
+        finally:
+            del os.environ["SALSAA_STEP_SPLIT"]
+        fill(sa, ctx, b, 900)
+        b.step_plain()
+        plain = snapshot(b, True)
+        for graph in (False, True, True):
+            b.step(graph=graph)
+            ctx.sync()
+            same(snapshot(b, False), plain)
+        outs.append(plain)
+        b.close()
+    same(outs[0], outs[1])
+    key.close()
