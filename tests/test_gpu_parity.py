@@ -193,6 +193,58 @@ def test_crs_key_commit_vs_oracle(ctx, orc, kappa, m, r):
     key.close()
 
 
+def test_verifier_split_identity_on_crs_key(sa, ctx, orc):
+    # verifier_split (split.rs:187-211): under a power-series key, the split
+    # halves' commitments L, R and the layer's tensor [1, m^c] (c = m / 2)
+    # recombine to the commitment of the whole witness, L + m^c R, row by row
+    say("case: verifier_split identity")
+    kappa, r, m = 3, 2, 256
+    c = m // 2
+    module = rand_ring(orc, (kappa, 64), 81, 0xC)
+    W = rand_ring(orc, (r, m, 64), 82, 0xB, 16)
+    dW = ctx.to_device(W)
+    key = ctx.crs_key(ctx.to_device(module), m)
+    full = ctx.parallel_dot_series_matrix(key, dW).download()
+    lr = ctx.split(key, dW)                       # [kappa][2r][d]: L rows, then R rows
+    R = ctx.to_device(np.ascontiguousarray(lr.download()[:, r:]))
+    ctx.convert(R, sa.REP_INCOMPLETE, sa.REP_COEFF)
+    mult = orc.compute_commitment_keys(module, c, Q)[:, c - 1]   # m_j^c = row_j[c - 1]
+    mrep = ctx.to_device(np.repeat(mult[:, None], r, axis=1))
+    prod = ctx.ring_mul(mrep, R).download()
+    L = lr.download()[:, :r]
+    assert np.array_equal((L + prod) % np.uint64(Q), full)
+    # the identity fails for the other tensor entry (a plausible slip)
+    assert not np.array_equal((L + R.download()) % np.uint64(Q), full)
+    key.close()
+
+
+def test_fold_homomorphism_on_device(sa, ctx, orc):
+    # test_fold (fold.rs:36-55) on the device: with a ternary challenge
+    # (challenge_for_fold, fold.rs:68-73), committing the folded witness gives
+    # the challenge-combined commitment (verifier_fold, fold.rs:57-66)
+    say("case: fold homomorphism")
+    kappa, r_in, r_out, m = 4, 3, 2, 128
+    A = rand_ring(orc, (kappa, m, 64), 83, 0xA)
+    W = rand_ring(orc, (r_in, m, 64), 84, 0xB)
+    key = ctx.key(ctx.to_device(A))
+    dW = ctx.to_device(W)
+    com = ctx.parallel_dot_series_matrix(key, dW)             # [kappa][r_in][d]
+    C = ctx.bounded((r_out, r_in, 64), 1, 85, 0xF)            # {-1, 0, 1} coefficients
+    folded = ctx.fold(dW, C)                                  # [r_out][m][d] IncompleteNTT
+    ctx.convert(folded, sa.REP_INCOMPLETE, sa.REP_COEFF)
+    lhs = ctx.parallel_dot_series_matrix(key, folded).download()
+    ctx.convert(com, sa.REP_INCOMPLETE, sa.REP_COEFF)
+    comh, Ch = com.download(), C.download()
+    a = np.ascontiguousarray(np.broadcast_to(Ch[None], (kappa, r_out, r_in, 64)))
+    b = np.ascontiguousarray(np.broadcast_to(comh[:, None], (kappa, r_out, r_in, 64)))
+    prod = ctx.ring_mul(ctx.to_device(a), ctx.to_device(b)).download()
+    rhs = np.zeros((kappa, r_out, 64), np.uint64)
+    for k in range(r_in):
+        rhs = (rhs + prod[:, :, k]) % np.uint64(Q)
+    assert np.array_equal(lhs, rhs)
+    key.close()
+
+
 # ------------------------------------------------------------------ projection
 @pytest.mark.parametrize("r,m", [(1, 256), (2, 1536), (3, 1536), (4, 2048), (5, 2560)])
 def test_project_vs_oracle(ctx, orc, r, m):
