@@ -185,6 +185,13 @@ int salsaa_compute_commitment_keys(salsaa_ctx *ctx, const uint64_t *d_module, si
  * [kappa][m][d] matrix (the stored ck of the reference, protocol.rs:132). */
 int salsaa_key_create_crs(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t m,
                           salsaa_key **out);
+/* The same CRS key without any stored layout (kappa <= 36): it keeps only the
+ * module (kappa ring elements), and salsaa_parallel_dot_series_matrix and
+ * salsaa_split generate the key rows inside the commit kernel (the ck of
+ * protocol.rs:132 never exists in memory).  Views work as for other keys;
+ * decomp_ell, batches, the server and the folding step need a stored key. */
+int salsaa_key_crs_implicit(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t m,
+                            salsaa_key **out);
 /* A view of the first kappa rows and first m ring elements of a key, sharing
  * its storage (no copy): the counterpart of passing the prefix slice
  * &key[..kappa] of the Vec<PowerSeries> to parallel_dot_series_matrix

@@ -116,6 +116,7 @@ def _load():
         "salsaa_send_cross_terms_before_join": (_int, [_vp, _vp, _vp, _sz, _vp, _vp, _vp, _vp]),
         "salsaa_key_create": (_int, [_vp, _vp, _sz, _sz, _PP]),
         "salsaa_key_create_crs": (_int, [_vp, _vp, _sz, _sz, _PP]),
+        "salsaa_key_crs_implicit": (_int, [_vp, _vp, _sz, _sz, _PP]),
         "salsaa_compute_commitment_keys": (_int, [_vp, _vp, _sz, _sz, _vp]),
         "salsaa_key_destroy": (_int, [_vp]),
         "salsaa_parallel_dot_series_matrix": (_int, [_vp, _vp, _vp, _sz, _sz, _vp]),
@@ -220,6 +221,17 @@ class Key:
         p = ctypes.c_void_p()
         _check(lib.salsaa_key_create(ctx.h, A_dev.ptr, kappa, m, ctypes.byref(p)))
         self.h = p.value
+
+    @classmethod
+    def crs_implicit(cls, ctx, module, m):
+        """The same CRS key stored as its module only: commits generate the
+        key rows in the kernel."""
+        k = cls.__new__(cls)
+        k.ctx, k.kappa, k.m = ctx, module.shape[0], m
+        p = ctypes.c_void_p()
+        _check(lib.salsaa_key_crs_implicit(ctx.h, module.ptr, k.kappa, m, ctypes.byref(p)))
+        k.h = p.value
+        return k
 
     @classmethod
     def crs(cls, ctx, module, m):
@@ -391,10 +403,11 @@ class Context:
         assert d == self.d
         return Key(self, A_dev, kappa, m)
 
-    def crs_key(self, module, m):
-        """Key.crs: the power-series CRS key of module [kappa][d], length m."""
+    def crs_key(self, module, m, implicit=False):
+        """Key.crs: the power-series CRS key of module [kappa][d], length m
+        (implicit: generated inside each commit, nothing stored)."""
         assert module.shape[1:] == (self.d,)
-        return Key.crs(self, module, m)
+        return Key.crs_implicit(self, module, m) if implicit else Key.crs(self, module, m)
 
     def compute_commitment_keys(self, module, length):
         """compute_commitment_keys (crs.rs:43-72): [kappa][length][d]

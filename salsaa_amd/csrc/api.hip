@@ -411,6 +411,40 @@ int salsaa_key_create_crs(salsaa_ctx *ctx, const uint64_t *d_module, size_t kapp
     });
 }
 
+int salsaa_key_crs_implicit(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t m,
+                            salsaa_key **out) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(out != nullptr, "null output pointer");
+    SALSAA_CHECK(kappa >= 1 && kappa <= kMfmaMaxKappa && m >= 1,
+                 "a generated CRS key needs 1 <= kappa <= 36 and m >= 1");
+    salsaa_key *k = new (std::nothrow) salsaa_key();
+    if (!k) return SALSAA_ERR_NOMEM;
+    k->k.ctx = &ctx->c;
+    k->k.kappa = kappa;
+    k->k.m = m;
+    k->k.a_rows = kappa;
+    k->k.a_m = m;
+    hipError_t e = hipMalloc(&k->k.d_mhat, kappa * kD * sizeof(uint64_t));
+    if (e != hipSuccess) {
+        delete k;
+        set_error(std::string("key allocation: ") + hipGetErrorString(e));
+        return SALSAA_ERR_NOMEM;
+    }
+    // the module in NTT form: each slot of a power is the slot's scalar power
+    int st = SALSAA_OK;
+    if (hipMemcpyAsync(k->k.d_mhat, d_module, kappa * kD * sizeof(uint64_t),
+                       hipMemcpyDeviceToDevice, ctx->c.stream) != hipSuccess)
+        st = SALSAA_ERR_HIP;
+    if (st == SALSAA_OK) st = launch_ntt(&ctx->c, k->k.d_mhat, kappa, SALSAA_REP_FULL, 0);
+    if (st != SALSAA_OK) {
+        (void)hipFree(k->k.d_mhat);
+        delete k;
+        return st;
+    }
+    *out = k;
+    return SALSAA_OK;
+}
+
 int salsaa_compute_commitment_keys(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa,
                                    size_t len, uint64_t *d_out) {
     CTX_OK(ctx);
@@ -438,6 +472,7 @@ int salsaa_key_destroy(salsaa_key *key) {
         (void)hipStreamSynchronize(key->k.ctx->stream);
         if (key->k.d_A) (void)hipFree(key->k.d_A);
         if (key->k.d_AB) (void)hipFree(key->k.d_AB);
+        if (key->k.d_mhat) (void)hipFree(key->k.d_mhat);
     }
     delete key;
     return SALSAA_OK;
@@ -450,6 +485,15 @@ static int commit_rows(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d
     SALSAA_CHECK(key->k.ctx == &ctx->c, "key belongs to another context");
     SALSAA_CHECK(m >= 1 && m <= key->k.m, "witness longer than the key");
     SALSAA_CHECK(rows >= 1, "commit needs at least one witness row");
+    if (key->k.d_mhat) {
+        // a generated CRS key: its rows are made inside the commit kernel
+        const size_t ps = plane_ktiles(m);
+        int8_t *wb = nullptr;
+        SALSAA_TRY(scratch_get(&ctx->c, 6, plane_bytes(rows, m), (void **)&wb));
+        SALSAA_TRY(launch_ntt_full_to_planes(&ctx->c, d_W, rows, m, row_lo, stride_lo, stride_hi,
+                                             wb, ps));
+        return launch_commit_crs(&ctx->c, key->k.d_mhat, key->k.kappa, wb, rows, m, ps, d_out);
+    }
     if (key->k.d_AB) {
         const size_t ps = plane_ktiles(m);
         int8_t *wb = nullptr;
@@ -591,6 +635,8 @@ int salsaa_decomp_ell(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_
     SALSAA_CHECK(key->k.ctx == &ctx->c, "key belongs to another context");
     SALSAA_CHECK(m >= 1 && m <= key->k.m, "witness longer than the key");
     SALSAA_CHECK(r >= 1, "decomp_ell needs at least one witness row");
+    SALSAA_CHECK(key->k.d_AB || key->k.d_A,
+                 "decomp_ell needs a stored key (a generated CRS key commits and splits only)");
     SALSAA_TRY(launch_radix(&ctx->c, d_W, r, m, parts, radix, d_radix));
     // one pass over W writes the limb witness and the limbs' slot-major NTT
     const size_t rows = r * parts;

@@ -106,6 +106,7 @@ struct Key {
     size_t kappa = 0, m = 0;  // rows and length this key commits with
     double *d_A = nullptr;    // slot-major full-NTT [64][a_rows][a_m] as exact fp64 (VALU path)
     int8_t *d_AB = nullptr;   // fragment-tiled digit planes (MFMA path, kappa <= kMfmaMaxKappa)
+    uint64_t *d_mhat = nullptr;  // generated CRS key: the module in NTT form [kappa][d], no layout
     size_t a_rows = 0, a_m = 0;     // shape of the stored matrix (a view commits with a prefix)
     size_t a_kt = 0, a_ktiles = 0;  // planes: row tiles per K step, K tiles (plane_ktiles(a_m))
     bool owner = true;              // false for a view sharing another key's storage
@@ -356,6 +357,10 @@ int launch_commit_mfma_batch(Context *ctx, const int8_t *AB, size_t kappa, size_
                              size_t a_ktiles, const CommitJob *jobs, const uint2 *table,
                              size_t nblk, size_t kchunk, size_t max_tiles);
 int commit_batch_occupancy(size_t kappa, size_t max_tiles);
+// the commit against the power-series CRS with NTT-form module mhat [kappa][d],
+// the key generated in the kernel (WB: the witness planes)
+int launch_commit_crs(Context *ctx, const uint64_t *mhat, size_t kappa, const int8_t *WB, size_t r,
+                      size_t m, size_t ktW, uint64_t *out);
 int launch_commit_finalize(Context *ctx, const double *partial, size_t ksplit, size_t kappa,
                            size_t r, uint64_t *out);
 int launch_radix(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,

@@ -300,6 +300,210 @@ int run(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt, size_t a_ktil
     return launch_commit_finalize(ctx, partial, best_ks, kappa, r, out);
 }
 
+// ---- commit against the power-series CRS generated in the kernel
+//
+// The key row j of the CRS is m_j, m_j^2, ...: in slot s of the full NTT its
+// element i is mhat_j[s]^(i + 1).  k_commit_crs is k_commit_mfma with the key's
+// fragments made in LDS instead of read: per K step (64 columns), thread
+// (j, e4) forms the four values base_j * mhat_j^e (e = 4 e4 .. 4 e4 + 3, the
+// powers e < 64 from a per-block table), splits them into digits exactly as
+// the stored planes do (byte u of (v + 0x80..80) ^ 0x80..80) and writes digit
+// u of the four into row 7 j + u of the fragment tiles; base_j then advances
+// by mhat_j^64.  The witness fragments stream from their planes one K step
+// ahead in registers; single LDS stage, two block barriers per K step.
+__device__ __forceinline__ double crs_powd(double b, unsigned long long e, Mod M) {
+    double r = 1.0;
+    while (e) {
+        if (e & 1) r = mulmod(r, b, M);
+        b = mulmod(b, b, M);
+        e >>= 1;
+    }
+    return r;
+}
+
+template <int TM, int TN, int WN>
+__global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) void k_commit_crs(
+    const uint64_t *__restrict__ mhat, int kappa, const int8_t *WB, int r, int ktW, int kchunk,
+    int ktend, int nbx, double *__restrict__ partial, const double *__restrict__ tab, Mod M) {
+    using S = CommitShape<TM, TN, WN>;
+    __shared__ __align__(16) char lds[S::LDS_BYTES];
+    __shared__ double tpow[kMfmaMaxKappa][64];
+    __shared__ double step64[kMfmaMaxKappa];
+    const unsigned lid = logical_block();
+    const int nb = (int)(lid % (unsigned)nbx);
+    const unsigned rest = lid / (unsigned)nbx;
+    const int s = (int)(rest % 64u);
+    const int z = (int)(rest / 64u);
+    const int rtA = (7 * kappa + 15) / 16;
+    const int rtB = (7 * r + 15) / 16;
+    const int nbv = min(GT, rtB - GT * nb), rows = min(16, r - 16 * nb);
+    const int ktb = z * kchunk, nsteps = min(ktend, ktb + kchunk) - ktb;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = w & 3, wn = w >> 2;
+    for (int x = tid; x < kappa * 64; x += S::NTHR)
+        tpow[x >> 6][x & 63] = crs_powd(u2d(mhat[(x >> 6) * kD + s]), (unsigned)(x & 63), M);
+    for (int x = tid; x < kappa; x += S::NTHR) step64[x] = crs_powd(u2d(mhat[x * kD + s]), 64, M);
+    // generator roles (j, e4), at most GPT per thread
+    constexpr int GPT = (int)((kMfmaMaxKappa * 16 + S::NTHR - 1) / S::NTHR);
+    double base[GPT];
+#pragma unroll
+    for (int gi = 0; gi < GPT; gi++) {
+        const int x = tid + S::NTHR * gi;
+        base[gi] = x < kappa * 16
+                       ? crs_powd(u2d(mhat[(x >> 4) * kD + s]), 64ull * (unsigned)ktb + 1, M)
+                       : 0.0;
+    }
+    // witness staging: chunk c = tid + NTHR i (i < CW) of the block's nbv
+    // tiles; LDS slot c % 64 of tile c / 64 takes memory chunk slot_chunk(.)
+    constexpr int CW = (GT * 64 + S::NTHR - 1) / S::NTHR;
+    const int8_t *wbase = WB + (size_t)GT * nb * FRAG + (size_t)s * ktW * rtB * FRAG;
+    v4i RW[CW];
+    auto wload = [&](int st) {
+#pragma unroll
+        for (int i = 0; i < CW; i++) {
+            const int c = tid + S::NTHR * i;
+            if (c < nbv * 64)
+                RW[i] = *reinterpret_cast<const v4i *>(
+                    wbase + ((size_t)(ktb + st) * rtB + (c >> 6)) * FRAG + slot_chunk(c & 63) * 16);
+        }
+    };
+    auto wstore = [&]() {
+#pragma unroll
+        for (int i = 0; i < CW; i++) {
+            const int c = tid + S::NTHR * i;
+            if (c < nbv * 64)
+                *reinterpret_cast<v4i *>(lds + (S::SA + (c >> 6)) * FRAG + (c & 63) * 16) = RW[i];
+        }
+    };
+    v4i acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++) acc[a][b] = (v4i){0, 0, 0, 0};
+    __syncthreads();  // the power tables
+    wload(0);
+    for (int st = 0; st < nsteps; st++) {
+        wstore();
+        wload(min(st + 1, nsteps - 1));
+        // the key's digit rows for this K step
+#pragma unroll
+        for (int gi = 0; gi < GPT; gi++) {
+            const int x = tid + S::NTHR * gi;
+            if (x < kappa * 16) {
+                const int j = x >> 4, e4 = x & 15;
+                uint32_t lo[4], hi[4], dg[7];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint64_t y =
+                        (d2u(mulmod(base[gi], tpow[j][4 * e4 + q], M)) + 0x0080808080808080ULL) ^
+                        0x8080808080808080ULL;
+                    lo[q] = (uint32_t)y;
+                    hi[q] = (uint32_t)(y >> 32);
+                }
+                bytes_transpose4(lo, hi, dg);
+#pragma unroll
+                for (int u = 0; u < 7; u++) {
+                    const int dr = 7 * j + u;
+                    // tile dr / 16, LDS slot 16 g + dr % 16 (g = e4 / 4), bytes 4 (e4 % 4) ..
+                    *reinterpret_cast<uint32_t *>(lds + (dr >> 4) * FRAG +
+                                                  (16 * (e4 >> 2) + (dr & 15)) * 16 + 4 * (e4 & 3)) =
+                        dg[u];
+                }
+                base[gi] = mulmod(base[gi], step64[j], M);
+            }
+        }
+        __syncthreads();
+        {
+            const char *bb = lds + lane * 16;
+            v4i fb[TN];
+#pragma unroll
+            for (int b = 0; b < TN; b++)
+                fb[b] = *reinterpret_cast<const v4i *>(bb + (S::SA + wn + WN * b) * FRAG);
+#pragma unroll
+            for (int a = 0; a < TM; a++) {
+                const v4i fa = *reinterpret_cast<const v4i *>(bb + (wm + 4 * a) * FRAG);
+#pragma unroll
+                for (int b = 0; b < TN; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa, fb[b], acc[a][b], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // recombination, as in commit_body
+    double *part = partial + ((size_t)z * 64 + s) * kappa * r + 16 * nb;
+    const int prow = r;
+    int(*D)[S::DC] = reinterpret_cast<int(*)[S::DC]>(lds);
+    const int lr = lane & 15, g = lane >> 4;
+    const double *wt = tab + TAB_W256;
+    const int ngrp = (rtA + GT - 1) / GT;
+    for (int gi = 0; gi < ngrp; gi++) {
+        if (gi) __syncthreads();
+#pragma unroll
+        for (int a = 0; a < TM; a++) {
+            const int tl = wm + 4 * a - GT * gi;
+            if (tl < 0 || tl >= GT || 16 * tl >= S::DR) continue;
+#pragma unroll
+            for (int b = 0; b < TN; b++) {
+                const int bl = wn + WN * b;
+                if (bl >= GT) continue;
+#pragma unroll
+                for (int v = 0; v < 4; v++) D[16 * tl + 4 * g + v][16 * bl + lr] = acc[a][b][v];
+            }
+        }
+        __syncthreads();
+        for (int o = tid; o < 256; o += S::NTHR) {
+            const int jl = o >> 4, kl = o & 15;
+            const int j = 16 * gi + jl, k = kl;
+            if (j >= kappa || k >= rows) continue;
+            long long E[13];
+#pragma unroll
+            for (int t = 0; t < 13; t++) E[t] = 0;
+#pragma unroll
+            for (int u = 0; u < 7; u++)
+#pragma unroll
+                for (int v = 0; v < 7; v++) E[u + v] += D[jl * 7 + u][kl * 7 + v];
+            double x = 0.0;
+#pragma unroll
+            for (int t = 0; t < 13; t++) {
+                x += mul_lazy((double)E[t], wt[t], M);
+                if ((t % 7) == 6) x = reduce_lazy(x, M);
+            }
+            part[(size_t)j * prow + k] = reduce(x, M);
+        }
+    }
+}
+
+template <int TM, int TN, int WN>
+int run_crs(Context *ctx, const uint64_t *mhat, size_t kappa, const int8_t *WB, size_t r,
+            size_t ktW, size_t ktend, uint64_t *out) {
+    Mod M{ctx->qd, ctx->qinv};
+    const size_t nbx = (r + 15) / 16;
+    // K chunks of at most 1024 tiles (exact int32 sums): enough blocks to fill
+    // the device several times over
+    size_t kc = std::min<size_t>(1024, std::max<size_t>(2, ktend));
+    while (kc > 16 && nbx * 64 * ((ktend + kc - 1) / kc) < (size_t)ctx->ncu * 8) kc /= 2;
+    const size_t ks = (ktend + kc - 1) / kc;
+    double *partial = nullptr;
+    SALSAA_TRY(scratch_get(ctx, 7, ks * 64 * kappa * r * sizeof(double), (void **)&partial));
+    const size_t nblk = nbx * 64 * ks;
+    SALSAA_CHECK(nblk < (1u << 31), "commit grid too large");
+    hipLaunchKernelGGL((k_commit_crs<TM, TN, WN>), dim3((unsigned)nblk), dim3(256 * WN), 0,
+                       ctx->stream, mhat, (int)kappa, WB, (int)r, (int)ktW, (int)kc, (int)ktend,
+                       (int)nbx, partial, ctx->d_tables, M);
+    SALSAA_HIP(hipGetLastError());
+    return launch_commit_finalize(ctx, partial, ks, kappa, r, out);
+}
+
+template <int TM>
+int run_crs_n(Context *ctx, const uint64_t *mhat, size_t kappa, const int8_t *WB, size_t r,
+              size_t ktW, size_t ktend, uint64_t *out) {
+    const size_t rtB = (7 * r + 15) / 16;
+    if (rtB <= 2) return run_crs<TM, 1, 2>(ctx, mhat, kappa, WB, r, ktW, ktend, out);
+    if (rtB <= 4) return run_crs<TM, 2, 2>(ctx, mhat, kappa, WB, r, ktW, ktend, out);
+    return run_crs<TM, 4, 2>(ctx, mhat, kappa, WB, r, ktW, ktend, out);
+}
+
 template <int TM>
 int run_n(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt, size_t a_ktiles,
           const int8_t *WB, size_t r, size_t ktW, size_t ktend, uint64_t *out) {
@@ -345,6 +549,18 @@ int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt
     if (rtA <= 4) return run_n<1>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
     if (rtA <= 8) return run_n<2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
     return run_n<4>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
+}
+
+int launch_commit_crs(Context *ctx, const uint64_t *mhat, size_t kappa, const int8_t *WB, size_t r,
+                      size_t m, size_t ktW, uint64_t *out) {
+    const size_t ktend = plane_ktiles(m);
+    SALSAA_CHECK(kappa >= 1 && kappa <= kMfmaMaxKappa, "MFMA commit needs kappa <= 36");
+    SALSAA_CHECK(ktend == ktW, "plane K-tiles do not cover the witness");
+    SALSAA_CHECK(ktend < (1u << 24) && r < (1u << 20), "commit too large");
+    const size_t rtA = (7 * kappa + 15) / 16;
+    if (rtA <= 4) return run_crs_n<1>(ctx, mhat, kappa, WB, r, ktW, ktend, out);
+    if (rtA <= 8) return run_crs_n<2>(ctx, mhat, kappa, WB, r, ktW, ktend, out);
+    return run_crs_n<4>(ctx, mhat, kappa, WB, r, ktW, ktend, out);
 }
 
 int launch_commit_mfma_batch(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,

@@ -193,6 +193,33 @@ def test_crs_key_commit_vs_oracle(ctx, orc, kappa, m, r):
     key.close()
 
 
+@pytest.mark.parametrize("kappa,m,r", [(4, 200, 3), (13, 4096, 1), (16, 1024, 8), (36, 256, 20)])
+def test_crs_key_generated_in_commit(sa, ctx, orc, kappa, m, r):
+    # the CRS key kept as its module only, the rows made inside the commit
+    # kernel: commit, split and a prefix view equal the stored CRS key's and
+    # the oracle's; decomp_ell refuses it
+    say("case: implicit CRS key kappa=%d m=%d r=%d" % (kappa, m, r))
+    module = rand_ring(orc, (kappa, 64), 90 + kappa, 0xC)
+    W = rand_ring(orc, (r, m, 64), 91, 0xB, 16)
+    dW, dmod = ctx.to_device(W), ctx.to_device(module)
+    imp = ctx.crs_key(dmod, m, implicit=True)
+    got = ctx.parallel_dot_series_matrix(imp, dW).download()
+    A = orc.compute_commitment_keys(module, m, Q)
+    assert np.array_equal(got, orc.commit(A, W, Q, threads=8))
+    stored = ctx.crs_key(dmod, m)
+    assert np.array_equal(ctx.split(imp, dW).download(), ctx.split(stored, dW).download())
+    k2, m2 = max(1, kappa // 3), max(1, m // 3)
+    v = imp.view(k2, m2)
+    W2 = np.ascontiguousarray(W[:, :m2])
+    got2 = ctx.parallel_dot_series_matrix(v, ctx.to_device(W2)).download()
+    assert np.array_equal(got2, orc.commit(np.ascontiguousarray(A[:k2, :m2]), W2, Q, threads=8))
+    v.close()
+    with pytest.raises(sa.SalsaaError):
+        ctx.decomp_ell(imp, dW, 2, 0)
+    stored.close()
+    imp.close()
+
+
 def test_verifier_split_identity_on_crs_key(sa, ctx, orc):
     # verifier_split (split.rs:187-211): under a power-series key, the split
     # halves' commitments L, R and the layer's tensor [1, m^c] (c = m / 2)
