@@ -635,17 +635,17 @@ int salsaa_decomp_ell(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_
     SALSAA_CHECK(key->k.ctx == &ctx->c, "key belongs to another context");
     SALSAA_CHECK(m >= 1 && m <= key->k.m, "witness longer than the key");
     SALSAA_CHECK(r >= 1, "decomp_ell needs at least one witness row");
-    SALSAA_CHECK(key->k.d_AB || key->k.d_A,
-                 "decomp_ell needs a stored key (a generated CRS key commits and splits only)");
     SALSAA_TRY(launch_radix(&ctx->c, d_W, r, m, parts, radix, d_radix));
     // one pass over W writes the limb witness and the limbs' slot-major NTT
     const size_t rows = r * parts;
-    if (key->k.d_AB) {
+    if (key->k.d_AB || key->k.d_mhat) {
         const size_t ps = plane_ktiles(m);
         int8_t *wb = nullptr;
         SALSAA_TRY(scratch_get(&ctx->c, 6, plane_bytes(rows, m), (void **)&wb));
         SALSAA_TRY(launch_decomp_to_slotmajor(&ctx->c, d_W, r, m, parts, d_radix, d_new_W, wb, 1,
                                               ps));
+        if (key->k.d_mhat)
+            return launch_commit_crs(&ctx->c, key->k.d_mhat, key->k.kappa, wb, rows, m, ps, d_rhs);
         return launch_commit_mfma(&ctx->c, key->k.d_AB, key->k.kappa, key->k.a_kt,
                                   key->k.a_ktiles, wb, rows, m, ps, d_rhs);
     }

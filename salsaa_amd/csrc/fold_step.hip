@@ -226,7 +226,8 @@ int salsaa_fold_step(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const uin
     CTX_OK(ctx);
     Context *c = &ctx->c;
     SALSAA_CHECK(key && o && d_W && d_C && d_cs, "null argument");
-    SALSAA_CHECK(key->k.ctx == c && key->k.d_AB, "key from another context or without planes");
+    SALSAA_CHECK(key->k.ctx == c && (key->k.d_AB || key->k.d_mhat),
+                 "key from another context or without planes");
     SALSAA_CHECK(n >= 1 && n < 65536, "instances out of range");
     SALSAA_CHECK(r >= 1 && r <= 8 && parts >= 1 && r * parts <= 64,
                  "the fused fold takes r <= 8 rows and r*parts <= 64 limb rows");
@@ -255,8 +256,11 @@ int salsaa_fold_step(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const uin
                        o->folded_coeff, planes, ktiles, rt_n, c->q, c->d_tables, M);
     SALSAA_HIP(hipGetLastError());
     // one commit of the stacked folded witnesses
-    SALSAA_TRY(launch_commit_mfma(c, key->k.d_AB, key->k.kappa, key->k.a_kt, key->k.a_ktiles,
-                                  planes, rows, m, ktiles, o->rhs));
+    if (key->k.d_mhat)
+        SALSAA_TRY(launch_commit_crs(c, key->k.d_mhat, key->k.kappa, planes, rows, m, ktiles, o->rhs));
+    else
+        SALSAA_TRY(launch_commit_mfma(c, key->k.d_AB, key->k.kappa, key->k.a_kt, key->k.a_ktiles,
+                                      planes, rows, m, ktiles, o->rhs));
     // norm checks: every folded row with its instance's challenges
     const size_t L = m * kD, nrounds = (size_t)__builtin_ctzll(L), per = reps * nrounds * 2;
     uint64_t *csx = nullptr;

@@ -73,3 +73,27 @@ def test_fold_step_vs_separate_calls_and_oracle(sa, ctx, orc, n, r, m, parts, r_
     want_com = orc.commit(A_h, coeff[:r_out], Q, threads=8)
     assert np.array_equal(want_com, g["rhs"][:, :r_out])
     key.close()
+
+
+def test_fold_step_with_generated_crs_key(sa, ctx, orc):
+    # the folding step under a CRS key kept as its module only (rows made in
+    # the commit kernel) equals the same step under the stored CRS key
+    say("case: fold step, implicit CRS key")
+    n, r, m, parts, r_out, kappa = 2, 4, 256, 3, 4, 12
+    reps, nrounds = 2, (m * 64).bit_length() - 1
+    W = ctx.to_device(orc.sample_bounded(41, sa.STREAM_W, (n * r, m, 64), 1 << 10, Q))
+    C = ctx.to_device(orc.sample_bounded(42, sa.STREAM_FOLD, (n, r_out, r * parts, 64), 1, Q))
+    cs = ctx.to_device(np.stack([np.stack([orc.sample_uniform(50 + i, 0x5C000 + rep,
+                                                              (nrounds * 2,), Q)
+                                           .reshape(nrounds, 2) for rep in range(reps)])
+                                 for i in range(n)]))
+    module = ctx.to_device(orc.sample_uniform(43, sa.STREAM_A, (kappa, 64), Q))
+    outs = []
+    for implicit in (True, False):
+        key = ctx.crs_key(module, m, implicit=implicit)
+        got = ctx.fold_step(key, W, n, parts, C, cs)
+        outs.append({k: v.download() for k, v in got.items()})
+        key.close()
+    assert outs[0]["ok"].all()
+    for k in outs[1]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k

@@ -196,8 +196,8 @@ def test_crs_key_commit_vs_oracle(ctx, orc, kappa, m, r):
 @pytest.mark.parametrize("kappa,m,r", [(4, 200, 3), (13, 4096, 1), (16, 1024, 8), (36, 256, 20)])
 def test_crs_key_generated_in_commit(sa, ctx, orc, kappa, m, r):
     # the CRS key kept as its module only, the rows made inside the commit
-    # kernel: commit, split and a prefix view equal the stored CRS key's and
-    # the oracle's; decomp_ell refuses it
+    # kernel: commit, split, decomp_ell and a prefix view equal the stored CRS
+    # key's and the oracle's
     say("case: implicit CRS key kappa=%d m=%d r=%d" % (kappa, m, r))
     module = rand_ring(orc, (kappa, 64), 90 + kappa, 0xC)
     W = rand_ring(orc, (r, m, 64), 91, 0xB, 16)
@@ -214,8 +214,10 @@ def test_crs_key_generated_in_commit(sa, ctx, orc, kappa, m, r):
     got2 = ctx.parallel_dot_series_matrix(v, ctx.to_device(W2)).download()
     assert np.array_equal(got2, orc.commit(np.ascontiguousarray(A[:k2, :m2]), W2, Q, threads=8))
     v.close()
-    with pytest.raises(sa.SalsaaError):
-        ctx.decomp_ell(imp, dW, 2, 0)
+    nw_i, rhs_i, _ = ctx.decomp_ell(imp, dW, 2, 0)
+    nw_s, rhs_s, _ = ctx.decomp_ell(stored, dW, 2, 0)
+    assert np.array_equal(nw_i.download(), nw_s.download())
+    assert np.array_equal(rhs_i.download(), rhs_s.download())
     stored.close()
     imp.close()
 
