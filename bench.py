@@ -616,7 +616,7 @@ def main():
                     help="only run 5 plain device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs the prover path on a single "
+        print("bench.py measures one GPU only: salsaa_amd runs the prover path on a single "
               "MI355X and has no multi-GPU mode (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)

@@ -643,7 +643,7 @@ def _mix64(z):
 
 
 def test_prng_definition(orc):
-    # The seeded stream is this build's definition (the reference rng is
+    # The seeded stream is this library's definition (the reference rng is
     # unseeded): SplitMix64 keyed by (seed, stream), counter = element index.
     for seed, stream in ((0, 1), (7, 3), (123456789, 42)):
         key = _mix64(seed ^ ((stream * 0xD1B54A32D192ED03) & M64))
