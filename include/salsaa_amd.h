@@ -220,8 +220,9 @@ int salsaa_complete_power_series(salsaa_ctx *ctx, const uint64_t *d_base, size_t
 /* parallel_dot_single_series_matrix (arithmetic.rs:507-536): out[j] =
  * sum_i layer[i] * W[j][i] for the expanded layer of length m (the one the
  * reference finds by length); d_layer [m][d], d_W [r][m][d] Coefficient,
- * d_out [r][d] IncompleteNTT.  The commit with the layer as its only key row
- * (a key is made and freed per call). */
+ * d_out [r][d] IncompleteNTT.  Up to 16 rows in one pass over the layer and
+ * the rows in the NTT domain; more rows through the commit with the layer as
+ * its only key row (a key is made and freed per call). */
 int salsaa_parallel_dot_single_series_matrix(salsaa_ctx *ctx, const uint64_t *d_layer, size_t m,
                                              const uint64_t *d_W, size_t r, uint64_t *d_out);
 

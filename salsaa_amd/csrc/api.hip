@@ -539,7 +539,9 @@ int salsaa_parallel_dot_single_series_matrix(salsaa_ctx *ctx, const uint64_t *d_
                                              const uint64_t *d_W, size_t r, uint64_t *d_out) {
     CTX_OK(ctx);
     SALSAA_CHECK(m >= 1 && r >= 1, "inner product needs m, r >= 1");
-    // the commit with the layer as the only key row
+    // up to 16 rows: one pass over the layer and the rows in the NTT domain
+    if (r <= 16) return launch_dot_series(&ctx->c, d_layer, m, d_W, r, d_out);
+    // more rows: the commit with the layer as the only key row
     salsaa_key *key = nullptr;
     SALSAA_TRY(salsaa_key_create(ctx, d_layer, 1, m, &key));
     const int st = salsaa_parallel_dot_series_matrix(ctx, key, d_W, r, m, d_out);

@@ -292,6 +292,10 @@ int launch_decomp_to_slotmajor(Context *ctx, const uint64_t *W, size_t r, size_t
 // the power-series CRS rows (crs.rs:43-72) of `rows` module elements (Coefficient),
 // length m: mode 0 NTT-form rows [rows][m][d], 1 a key's slot-major doubles, 2 a
 // key's digit planes with pstride K tiles
+// out[j] = sum_i layer[i] W[j][i] (IncompleteNTT) for r <= 16 rows, in one pass
+// over the layer and the rows (parallel_dot_single_series_matrix)
+int launch_dot_series(Context *ctx, const uint64_t *layer, size_t m, const uint64_t *W, size_t r,
+                      uint64_t *out);
 int launch_crs(Context *ctx, const uint64_t *module, size_t rows, size_t m, int mode, void *dst,
                size_t pstride);
 int launch_ntt_full_to_planes(Context *ctx, const uint64_t *src, size_t rows, size_t m,

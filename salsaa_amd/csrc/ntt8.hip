@@ -857,6 +857,97 @@ __global__ __launch_bounds__(NT) void k_crs_sm(const uint64_t *__restrict__ mhat
         store_slotmajor((double *)dst, lds, rows, row, m, i0, nel, tid);
 }
 
+// ---- inner products with one power-series layer (parallel_dot_single_series_matrix)
+//
+// out[j] = sum_i layer[i] W[j][i] in R_q for up to DOT_MAXR rows: per tile of
+// 128 elements the layer's 64-point NTT is taken once and kept in registers
+// (thread (e, t): slots 8t .. 8t + 7 of element e), each row's tile is
+// transformed and multiplied slot by slot, the products are summed over the
+// wave's 8 elements with lane shuffles, over the waves in LDS and over the
+// block's tiles in LDS, and each block adds its sums (canonical, so at most
+// gridDim.x q < 2^63) to acc [r][64] with one atomic per slot.
+constexpr int DOT_MAXR = 16;
+__global__ __launch_bounds__(NT) void k_dot_series(const uint64_t *__restrict__ layer,
+                                                   const uint64_t *__restrict__ W, size_t m, int r,
+                                                   unsigned long long *__restrict__ acc,
+                                                   const double *__restrict__ tab, Mod M) {
+    __shared__ uint64_t lds[EL * SROW];
+    __shared__ double wsum[NT / 64][64];
+    __shared__ double bsum[DOT_MAXR][64];
+    const int tid = threadIdx.x, e = tid >> 3, t = tid & 7, lane = tid & 63, wave = tid >> 6;
+    for (int x = tid; x < r * 64; x += NT) bsum[x >> 6][x & 63] = 0.0;
+    const size_t ntiles = (m + EL - 1) / EL;
+    for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const size_t i0 = tile * EL;
+        const int nel = (int)min((size_t)EL, m - i0);
+        const bool live = e < nel;
+        auto ntt_tile = [&](const uint64_t *src, double(&v)[8]) {
+            __syncthreads();  // the previous users of lds and wsum are done
+            load_block<false>(lds, src + i0 * kD, nel, tid);
+            __syncthreads();
+            if (live) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) v[k] = u2d(lds[e * SROW + t + 8 * k]);
+            }
+            full_fwd_to_work(v, lds, lds, e, t, live, tab, M);
+            if (!live) {
+#pragma unroll
+                for (int x = 0; x < 8; x++) v[x] = 0.0;
+            }
+        };
+        double lv[8];
+        ntt_tile(layer, lv);
+        for (int j = 0; j < r; j++) {
+            double p[8];
+            ntt_tile(W + (size_t)j * m * kD, p);
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                p[x] = mulmod(lv[x], p[x], M);
+#pragma unroll
+                for (int off = 8; off < 64; off <<= 1) p[x] = addmod(p[x], __shfl_xor(p[x], off, 64), M);
+            }
+            if (lane < 8) {
+#pragma unroll
+                for (int x = 0; x < 8; x++) wsum[wave][8 * lane + x] = p[x];
+            }
+            __syncthreads();
+            if (tid < 64) {
+                double sum = bsum[j][tid];
+#pragma unroll
+                for (int w = 0; w < NT / 64; w++) sum = addmod(sum, wsum[w][tid], M);
+                bsum[j][tid] = sum;
+            }
+        }
+    }
+    __syncthreads();
+    for (int x = tid; x < r * 64; x += NT)
+        atomicAdd(&acc[x], (unsigned long long)d2u(bsum[x >> 6][x & 63]));
+}
+
+__global__ void k_dot_zero(unsigned long long *__restrict__ acc, int n) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n) acc[x] = 0;
+}
+
+// slot sums acc[j][s] (mod q) -> out[j] in IncompleteNTT form (the commit's
+// output map); one block of 64 threads per row
+__global__ __launch_bounds__(64) void k_dot_finalize(const unsigned long long *__restrict__ acc,
+                                                     uint64_t *__restrict__ out,
+                                                     const double *__restrict__ tab, Mod M) {
+    __shared__ double vs[64];
+    const int j = blockIdx.x, tt = threadIdx.x;
+    vs[tt] = u2d(acc[j * 64 + tt] % (unsigned long long)M.q);
+    __syncthreads();
+    const double *fin = tab + TAB_FIN;
+    double a = 0.0;
+#pragma unroll 8
+    for (int s2 = 0; s2 < 64; s2++) {
+        a += mul_lazy(fin[s2 * 64 + tt], vs[s2], M);
+        if ((s2 & 3) == 3) a = reduce_lazy(a, M);
+    }
+    out[(size_t)j * kD + tt] = d2u(reduce(a, M));
+}
+
 }  // namespace
 
 // resident blocks per CU of a kernel (occupancy API, cached)
@@ -1021,6 +1112,24 @@ int launch_crs(Context *ctx, const uint64_t *module, size_t rows, size_t m, int 
     else
         hipLaunchKernelGGL(k_crs_sm<2>, grid, dim3(NT), 0, ctx->stream, mhat, tpow, rows, m, dst,
                            pstride, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+int launch_dot_series(Context *ctx, const uint64_t *layer, size_t m, const uint64_t *W, size_t r,
+                      uint64_t *out) {
+    SALSAA_CHECK(m >= 1 && r >= 1 && r <= (size_t)DOT_MAXR, "inner product size out of range");
+    Mod M{ctx->qd, ctx->qinv};
+    unsigned long long *acc = nullptr;
+    SALSAA_TRY(scratch_get(ctx, 11, r * 64 * sizeof(unsigned long long), (void **)&acc));
+    hipLaunchKernelGGL(k_dot_zero, dim3(ceil_div(r * 64, (size_t)256)), dim3(256), 0, ctx->stream,
+                       acc, (int)(r * 64));
+    const size_t ntiles = (m + EL - 1) / EL;
+    const unsigned g = (unsigned)std::min<size_t>(ntiles, (size_t)ctx->ncu * 2);
+    hipLaunchKernelGGL(k_dot_series, dim3(g), dim3(NT), 0, ctx->stream, layer, W, m, (int)r, acc,
+                       ctx->d_tables, M);
+    hipLaunchKernelGGL(k_dot_finalize, dim3((unsigned)r), dim3(64), 0, ctx->stream, acc, out,
+                       ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }

@@ -137,8 +137,9 @@ def test_complete_power_series_vs_oracle(ctx, orc, m0, n):
     assert np.array_equal(got, orc.complete_power_series(base, tensors, Q))
 
 
-@pytest.mark.parametrize("m,r", [(1, 1), (48, 3), (200, 5)])
+@pytest.mark.parametrize("m,r", [(1, 1), (48, 3), (200, 5), (3000, 16), (700, 17)])
 def test_dot_single_series_vs_oracle(ctx, orc, m, r):
+    # r <= 16: one pass in the NTT domain; r = 17: the commit with the layer as its key
     say("case: parallel_dot_single_series_matrix m=%d r=%d" % (m, r))
     layer = rand_ring(orc, (m, 64), 43, 0xC)
     W = rand_ring(orc, (r, m, 64), 44, 0xC, 16)
