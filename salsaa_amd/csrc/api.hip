@@ -570,12 +570,15 @@ int salsaa_batch_projections(salsaa_ctx *ctx, const uint64_t *d_P, const uint64_
     SALSAA_CHECK(is_pow2(r) && cw <= m && m % cw == 0 && is_pow2(m / cw) && cw < 65536,
                  "batch_projections needs r and m / (256 r) powers of two (project.rs:168-186)");
     Context *c = &ctx->c;
-    // scratch: c0 [1], powers [256], c_l [cw], column sums of W [r][cw] and of P [256]
+    // scratch: c0 [1], powers [256], c_l [cw], column sums of W [r][cw] and of
+    // P [256], and the split sums' accumulator [max(r cw, 256)]
     uint64_t *buf = nullptr;
-    const size_t nbuf = 1 + 256 + cw + r * cw + 256;
+    const size_t nacc = std::max(r * cw, (size_t)256);
+    const size_t nbuf = 1 + 256 + cw + r * cw + 256 + nacc;
     SALSAA_TRY(scratch_get(c, 9, nbuf * kD * sizeof(uint64_t), (void **)&buf));
     uint64_t *c0 = buf, *pw = c0 + kD, *cj = pw + 256 * kD, *wsum = cj + cw * kD,
              *psum = wsum + r * cw * kD;
+    unsigned long long *acc = reinterpret_cast<unsigned long long *>(psum + 256 * kD);
     // consecutive powers of c0 (project.rs:118-129), slot by slot in NTT form
     SALSAA_HIP(hipMemcpyAsync(c0, d_c0, kD * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
     SALSAA_TRY(launch_ntt(c, c0, 1, SALSAA_REP_FULL, 0));
@@ -585,8 +588,8 @@ int salsaa_batch_projections(salsaa_ctx *ctx, const uint64_t *d_P, const uint64_
     SALSAA_TRY(launch_project_transposed(c, pw, cw, d_is_zero, d_sign, cj));
     // Both layers repeat their base (all tensors are [1, 1]), so each inner
     // product is the base's against the sum of the matrix's chunks.
-    SALSAA_TRY(launch_chunk_sums(c, d_W, r, m, cw, wsum));
-    SALSAA_TRY(launch_chunk_sums(c, d_P, 1, m, 256, psum));
+    SALSAA_TRY(launch_chunk_sums(c, d_W, r, m, cw, wsum, acc));
+    SALSAA_TRY(launch_chunk_sums(c, d_P, 1, m, 256, psum, acc));
     SALSAA_TRY(salsaa_parallel_dot_single_series_matrix(ctx, cj, cw, wsum, r, d_witness_rhs));
     SALSAA_TRY(salsaa_parallel_dot_single_series_matrix(ctx, pw, 256, psum, 1, d_projection_rhs));
     if (d_witness_lhs) SALSAA_TRY(launch_repeat(c, cj, cw, m, d_witness_lhs));

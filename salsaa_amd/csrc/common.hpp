@@ -278,8 +278,9 @@ int launch_project(Context *ctx, const uint64_t *W, size_t r, size_t m, const ui
                    const uint8_t *sign, uint64_t *out);
 int launch_project_transposed(Context *ctx, const uint64_t *P, size_t cw, const uint8_t *is_zero,
                               const uint8_t *sign, uint64_t *out);
+// acc: scratch of rows * cw * d u64 for split sums (NULL: no split)
 int launch_chunk_sums(Context *ctx, const uint64_t *src, size_t rows, size_t m, size_t cw,
-                      uint64_t *out);
+                      uint64_t *out, unsigned long long *acc);
 int launch_powers(Context *ctx, const uint64_t *c_ntt, uint64_t *out);
 int launch_repeat(Context *ctx, const uint64_t *base, size_t m0, size_t n, uint64_t *out);
 int launch_sum_equals(Context *ctx, const uint64_t *a, size_t r, const uint64_t *b, int32_t *ok);

@@ -265,21 +265,42 @@ __global__ __launch_bounds__(PM_THREADS, 2) void k_project_mfma(
 }
 
 // out[i][l] = sum_j src[i][j cw + l] mod q (j < nch) for rows i < rows of
-// length m, per coefficient (u64 sums, reduced every 8192 terms).
+// length m, per coefficient (u64 sums, reduced every 8192 terms).  With few
+// outputs the chunks are split over gridDim.y: each part adds its sum (mod q)
+// to acc, and k_chunk_sums_finish reduces acc into out.
 __global__ void k_chunk_sums(const uint64_t *__restrict__ src, size_t rows, size_t m, size_t cw,
-                             size_t nch, uint64_t *__restrict__ out, uint64_t q) {
+                             size_t nch, uint64_t *__restrict__ out,
+                             unsigned long long *__restrict__ acc, uint64_t q) {
     const size_t total = rows * cw * kD;
+    const size_t per = (nch + gridDim.y - 1) / gridDim.y;
+    const size_t j0 = (size_t)blockIdx.y * per, j1 = min(nch, j0 + per);
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
          e += (size_t)gridDim.x * blockDim.x) {
         const size_t i = e / (cw * kD), lc = e % (cw * kD);
         const uint64_t *p = src + i * m * kD + lc;
-        uint64_t acc = 0;
-        for (size_t j = 0; j < nch; j++) {
-            acc += p[j * cw * kD];
-            if ((j & 8191) == 8191) acc %= q;  // below (8192 + 1) q < 2^64
+        uint64_t a = 0;
+        for (size_t j = j0; j < j1; j++) {
+            a += p[j * cw * kD];
+            if (((j - j0) & 8191) == 8191) a %= q;  // below (8192 + 1) q < 2^64
         }
-        out[e] = acc % q;
+        if (gridDim.y == 1)
+            out[e] = a % q;
+        else
+            atomicAdd(&acc[e], (unsigned long long)(a % q));  // at most gridDim.y q < 2^64
     }
+}
+
+__global__ void k_chunk_sums_finish(const unsigned long long *__restrict__ acc, size_t n,
+                                    uint64_t *__restrict__ out, uint64_t q) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        out[e] = acc[e] % q;
+}
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        p[e] = 0;
 }
 
 // out[k][s] = c[s]^k for k < 256 and the 64 slots s of one element in NTT form
@@ -364,12 +385,21 @@ int launch_project_transposed(Context *ctx, const uint64_t *P, size_t cw, const 
 }
 
 int launch_chunk_sums(Context *ctx, const uint64_t *src, size_t rows, size_t m, size_t cw,
-                      uint64_t *out) {
+                      uint64_t *out, unsigned long long *acc) {
     SALSAA_CHECK(cw >= 1 && m % cw == 0, "chunk sums need cw | m");
-    const size_t total = rows * cw * kD;
+    const size_t total = rows * cw * kD, nch = m / cw;
+    // about a million summing threads: split the chunks when the outputs are fewer
+    size_t split = 1;
+    while (acc && split < nch && split < 8192 && total * split < ((size_t)1 << 20)) split *= 2;
     const unsigned g = (unsigned)std::min<size_t>(ceil_div(total, (size_t)256), (size_t)ctx->ncu * 16);
-    hipLaunchKernelGGL(k_chunk_sums, dim3(g), dim3(256), 0, ctx->stream, src, rows, m, cw, m / cw,
-                       out, ctx->q);
+    const unsigned gz = (unsigned)std::min<size_t>(ceil_div(total, (size_t)256), 4096);
+    if (split > 1)
+        hipLaunchKernelGGL(k_fill_u64, dim3(gz), dim3(256), 0, ctx->stream, acc, total);
+    hipLaunchKernelGGL(k_chunk_sums, dim3(g, (unsigned)split), dim3(256), 0, ctx->stream, src, rows,
+                       m, cw, nch, out, acc, ctx->q);
+    if (split > 1)
+        hipLaunchKernelGGL(k_chunk_sums_finish, dim3(gz), dim3(256), 0, ctx->stream, acc, total,
+                           out, ctx->q);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }
