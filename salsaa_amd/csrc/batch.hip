@@ -129,10 +129,13 @@ struct salsaa_batch {
     Context nc_ctx[kNcStreams];          // the context's tables on streams of their own
     NcScratch nc[kNcStreams];
     hipEvent_t ev_fork = nullptr, ev_fork2 = nullptr, ev_done[kNcStreams] = {};
-    // the step's NTT pass in two launches: arena tiles [split_tile, end) hold
-    // the length classes [split_group, end), whose norm checks run while the
-    // second launch transforms tiles [0, split_tile) (split_group = 0: one launch)
+    // the step's NTT pass in two launches split at arena tile split_tile (the
+    // start of length class split_group; the arena runs longest first):
+    // split_mode 1 transforms the shorter classes (tiles from split_tile on)
+    // first, 2 the longer ones (tiles before split_tile); the first part's
+    // norm checks run beside the second launch (split_mode 0: one launch)
     size_t split_tile = 0, split_group = 0;
+    int split_mode = 0;
     int cpt_first = 0;
     float t_commit = 0.f, t_nc = 0.f;
     // plain path
@@ -315,19 +318,25 @@ int run_rt_range(salsaa_batch *b, size_t t0, size_t t1) {
 // one pass: forward NTT, the commit's digit planes, the inverse NTT and the
 // short copy; then the commit on the context's stream while the norm checks
 // (which need only the Coefficient witnesses, or their short copy) run beside
-// it on theirs.  With a split, the pass first covers the longest classes,
-// whose norm checks then run beside the pass over the rest.
+// it on theirs.  With a split, the pass first covers one part of the length
+// classes, whose norm checks then run beside the pass over the rest.
 int run_step(salsaa_batch *b) {
-    if (b->split_group == 0) {
+    const size_t nt = b->tot_w / kPlaneTile, G = b->groups.size();
+    if (b->split_mode == 0) {
         SALSAA_TRY(run_fwd_planes(b, 1));
         SALSAA_TRY(norm_checks_fork(b, b->cpt_first, &b->sc));
-    } else {
-        const size_t nt = b->tot_w / kPlaneTile;
+    } else if (b->split_mode == 1) {
         SALSAA_TRY(run_rt_range(b, b->split_tile, nt));
-        SALSAA_TRY(norm_checks_fork_range(b, b->cpt_first, &b->sc, b->split_group,
-                                          b->groups.size(), b->ev_fork));
+        SALSAA_TRY(norm_checks_fork_range(b, b->cpt_first, &b->sc, b->split_group, G, b->ev_fork));
         SALSAA_TRY(run_rt_range(b, 0, b->split_tile));
         SALSAA_TRY(norm_checks_fork_range(b, b->cpt_first, &b->sc, 0, b->split_group,
+                                          b->ev_fork2));
+        SALSAA_TRY(norm_checks_mark_done(b));
+    } else {
+        SALSAA_TRY(run_rt_range(b, 0, b->split_tile));
+        SALSAA_TRY(norm_checks_fork_range(b, b->cpt_first, &b->sc, 0, b->split_group, b->ev_fork));
+        SALSAA_TRY(run_rt_range(b, b->split_tile, nt));
+        SALSAA_TRY(norm_checks_fork_range(b, b->cpt_first, &b->sc, b->split_group, G,
                                           b->ev_fork2));
         SALSAA_TRY(norm_checks_mark_done(b));
     }
@@ -613,21 +622,35 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
     }
     SALSAA_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
     SALSAA_HIP(hipEventCreateWithFlags(&b->ev_fork2, hipEventDisableTiming));
-    // The split: the longest classes while they hold at most 70 % of the arena
-    // (their norm checks, at about 0.4 of the pass's time per element, then fit
-    // beside the pass over the rest).  SALSAA_STEP_SPLIT=0 keeps one pass.
+    // The split: the first part takes whole classes while they hold at most
+    // 70 % of the arena (their norm checks, at about 0.4 of the pass's time
+    // per element, then fit beside the pass over the rest).  Mode from
+    // SALSAA_STEP_SPLIT: 0 one pass, 1 shorter classes first, 2 longer first.
     {
         const char *env = getenv("SALSAA_STEP_SPLIT");
-        const bool want = !(env && std::string(env) == "0");
-        size_t gsplit = b->groups.size(), acc = 0;
-        while (want && gsplit > 1) {
-            const auto &g = b->groups[gsplit - 1];
-            const size_t e = g.rows * (g.L / kD);
-            if (10 * (acc + e) > 7 * b->tot_w) break;
-            acc += e;
-            gsplit--;
+        const int mode = env ? atoi(env) : 1;
+        const size_t G = b->groups.size();
+        size_t acc = 0, gsplit = 0;
+        if (mode == 1) {
+            gsplit = G;
+            while (gsplit > 1) {
+                const auto &g = b->groups[gsplit - 1];
+                const size_t e = g.rows * (g.L / kD);
+                if (10 * (acc + e) > 7 * b->tot_w) break;
+                acc += e;
+                gsplit--;
+            }
+        } else if (mode == 2) {
+            while (gsplit + 1 < G) {
+                const auto &g = b->groups[gsplit];
+                const size_t e = g.rows * (g.L / kD);
+                if (10 * (acc + e) > 7 * b->tot_w) break;
+                acc += e;
+                gsplit++;
+            }
         }
-        if (gsplit < b->groups.size() && gsplit > 0) {
+        if ((mode == 1 || mode == 2) && gsplit > 0 && gsplit < G) {
+            b->split_mode = mode;
             b->split_group = gsplit;
             b->split_tile = b->off_w[b->groups[gsplit].first] / kPlaneTile;
         }

@@ -230,9 +230,10 @@ def test_batch_short_copy_mixed(sa, ctx, orc):
 
 
 def test_batch_split_pass_equals_single_pass(sa, ctx, orc):
-    # The step's NTT pass runs in two launches (the longest classes first, so
-    # their norm checks overlap the rest of the pass) unless SALSAA_STEP_SPLIT=0
-    # at batch creation: both orders give the plain path's outputs.
+    # The step's NTT pass runs in two launches (SALSAA_STEP_SPLIT=1: the
+    # shorter classes first, 2: the longer ones first; the first part's norm
+    # checks overlap the rest of the pass) or one (0): every order gives the
+    # plain path's outputs.
     import os
     say("case: batch split pass vs single pass")
     m = [1024, 1024, 512, 256, 256, 128, 64, 32, 8, 1]
@@ -241,7 +242,7 @@ def test_batch_split_pass_equals_single_pass(sa, ctx, orc):
     A_h = orc.sample_uniform(8, sa.STREAM_A, (32, 1024, 64), Q)
     key = ctx.key(ctx.to_device(A_h))
     outs = []
-    for env in ("1", "0"):
+    for env in ("1", "2", "0"):
         os.environ["SALSAA_STEP_SPLIT"] = env
         try:
             b = sa.Batch(ctx, key, m, kappa, reps=2, fq2=1, tune=False)
@@ -257,4 +258,5 @@ def test_batch_split_pass_equals_single_pass(sa, ctx, orc):
         outs.append(plain)
         b.close()
     same(outs[0], outs[1])
+    same(outs[0], outs[2])
     key.close()
