@@ -187,10 +187,9 @@ int salsaa_key_create_crs(salsaa_ctx *ctx, const uint64_t *d_module, size_t kapp
                           salsaa_key **out);
 /* The same CRS key without any stored layout (kappa <= 36): it keeps only the
  * module (kappa ring elements), and salsaa_parallel_dot_series_matrix and
- * salsaa_split, salsaa_decomp_ell and salsaa_fold_step generate the key rows
- * inside the commit kernel (the ck of protocol.rs:132 never exists in
- * memory).  Views work as for other keys; batches and the server need a
- * stored key. */
+ * salsaa_split, salsaa_decomp_ell, salsaa_fold_step, batches and the server
+ * generate the key rows inside the commit kernels (the ck of protocol.rs:132
+ * never exists in memory).  Views work as for other keys. */
 int salsaa_key_crs_implicit(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t m,
                             salsaa_key **out);
 /* A view of the first kappa rows and first m ring elements of a key, sharing

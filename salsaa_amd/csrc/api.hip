@@ -424,6 +424,7 @@ int salsaa_key_crs_implicit(salsaa_ctx *ctx, const uint64_t *d_module, size_t ka
     k->k.m = m;
     k->k.a_rows = kappa;
     k->k.a_m = m;
+    k->k.a_ktiles = plane_ktiles(m);  // the K extent the key covers (nothing stored)
     hipError_t e = hipMalloc(&k->k.d_mhat, kappa * kD * sizeof(uint64_t));
     if (e != hipSuccess) {
         delete k;

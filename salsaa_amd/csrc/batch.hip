@@ -231,10 +231,16 @@ int run_planes(salsaa_batch *b) {
 // one commit launch per class (its kappa and kernel shape), then one finalize each
 int run_commit(salsaa_batch *b) {
     const Key &k = b->key->k;
-    for (const auto &cc : b->cclass)
-        SALSAA_TRY(launch_commit_mfma_batch(&b->ctx->c, k.d_AB, cc.kappa, k.a_kt, k.a_ktiles,
-                                            b->d_cjobs + cc.job0, b->d_table + cc.t0, cc.nblk,
-                                            b->kchunk, cc.max_tiles));
+    for (const auto &cc : b->cclass) {
+        if (k.d_mhat)  // a CRS key generated inside the commit
+            SALSAA_TRY(launch_commit_crs_batch(&b->ctx->c, k.d_mhat, cc.kappa, b->d_cjobs + cc.job0,
+                                               b->d_table + cc.t0, cc.nblk, b->kchunk,
+                                               cc.max_tiles));
+        else
+            SALSAA_TRY(launch_commit_mfma_batch(&b->ctx->c, k.d_AB, cc.kappa, k.a_kt, k.a_ktiles,
+                                                b->d_cjobs + cc.job0, b->d_table + cc.t0, cc.nblk,
+                                                b->kchunk, cc.max_tiles));
+    }
     for (const auto &cc : b->cclass)
         SALSAA_TRY(launch_commit_finalize_batch(&b->ctx->c, b->d_fjobs + cc.job0, cc.njobs,
                                                 cc.fnblk, cc.kappa));
@@ -442,7 +448,8 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
            size_t reps, int fq2, uint64_t nr, int tune, salsaa_batch *b) {
     Context *c = &ctx->c;
     SALSAA_CHECK(key != nullptr && key->k.ctx == c, "key missing or from another context");
-    SALSAA_CHECK(key->k.d_AB != nullptr, "batched commit needs a key with MFMA planes");
+    SALSAA_CHECK(key->k.d_AB != nullptr || key->k.d_mhat != nullptr,
+                 "batched commit needs a key with MFMA planes or a generated CRS key");
     SALSAA_CHECK(n >= 1 && m != nullptr && kappa != nullptr, "batch needs n >= 1 instances");
     SALSAA_CHECK(reps >= 1 && reps < 65536, "repetitions out of range");
     SALSAA_CHECK(nr < c->q, "non-residue must be reduced mod q");

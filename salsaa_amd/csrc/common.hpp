@@ -366,6 +366,8 @@ int commit_batch_occupancy(size_t kappa, size_t max_tiles);
 // the key generated in the kernel (WB: the witness planes)
 int launch_commit_crs(Context *ctx, const uint64_t *mhat, size_t kappa, const int8_t *WB, size_t r,
                       size_t m, size_t ktW, uint64_t *out);
+int launch_commit_crs_batch(Context *ctx, const uint64_t *mhat, size_t kappa, const CommitJob *jobs,
+                            const uint2 *table, size_t nblk, size_t kchunk, size_t max_tiles);
 int launch_commit_finalize(Context *ctx, const double *partial, size_t ksplit, size_t kappa,
                            size_t r, uint64_t *out);
 int launch_radix(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,

@@ -321,23 +321,19 @@ __device__ __forceinline__ double crs_powd(double b, unsigned long long e, Mod M
     return r;
 }
 
+// One block's work: slot s, K tiles [ktb, ktb + nsteps), all key rows j <
+// kappa and the nbv witness tiles (rows k < rows) whose fragment (s, kt, t)
+// sits at WB + s w_slot + (kt w_kt + t) KiB; the recombined residues go to
+// part[j * prow + k].
 template <int TM, int TN, int WN>
-__global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) void k_commit_crs(
-    const uint64_t *__restrict__ mhat, int kappa, const int8_t *WB, int r, int ktW, int kchunk,
-    int ktend, int nbx, double *__restrict__ partial, const double *__restrict__ tab, Mod M) {
+__device__ __forceinline__ void crs_body(char *lds, double (*tpow)[64], double *step64,
+                                         const uint64_t *__restrict__ mhat, int kappa,
+                                         const int8_t *WB, size_t w_slot, int w_kt, int nbv,
+                                         int rows, int s, int ktb, int nsteps,
+                                         double *__restrict__ part, int prow,
+                                         const double *__restrict__ tab, Mod M) {
     using S = CommitShape<TM, TN, WN>;
-    __shared__ __align__(16) char lds[S::LDS_BYTES];
-    __shared__ double tpow[kMfmaMaxKappa][64];
-    __shared__ double step64[kMfmaMaxKappa];
-    const unsigned lid = logical_block();
-    const int nb = (int)(lid % (unsigned)nbx);
-    const unsigned rest = lid / (unsigned)nbx;
-    const int s = (int)(rest % 64u);
-    const int z = (int)(rest / 64u);
     const int rtA = (7 * kappa + 15) / 16;
-    const int rtB = (7 * r + 15) / 16;
-    const int nbv = min(GT, rtB - GT * nb), rows = min(16, r - 16 * nb);
-    const int ktb = z * kchunk, nsteps = min(ktend, ktb + kchunk) - ktb;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = w & 3, wn = w >> 2;
@@ -357,7 +353,7 @@ __global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) v
     // witness staging: chunk c = tid + NTHR i (i < CW) of the block's nbv
     // tiles; LDS slot c % 64 of tile c / 64 takes memory chunk slot_chunk(.)
     constexpr int CW = (GT * 64 + S::NTHR - 1) / S::NTHR;
-    const int8_t *wbase = WB + (size_t)GT * nb * FRAG + (size_t)s * ktW * rtB * FRAG;
+    const int8_t *wbase = WB + (size_t)s * w_slot;
     v4i RW[CW];
     auto wload = [&](int st) {
 #pragma unroll
@@ -365,7 +361,7 @@ __global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) v
             const int c = tid + S::NTHR * i;
             if (c < nbv * 64)
                 RW[i] = *reinterpret_cast<const v4i *>(
-                    wbase + ((size_t)(ktb + st) * rtB + (c >> 6)) * FRAG + slot_chunk(c & 63) * 16);
+                    wbase + ((size_t)(ktb + st) * w_kt + (c >> 6)) * FRAG + slot_chunk(c & 63) * 16);
         }
     };
     auto wstore = [&]() {
@@ -431,8 +427,6 @@ __global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) v
         __syncthreads();
     }
     // recombination, as in commit_body
-    double *part = partial + ((size_t)z * 64 + s) * kappa * r + 16 * nb;
-    const int prow = r;
     int(*D)[S::DC] = reinterpret_cast<int(*)[S::DC]>(lds);
     const int lr = lane & 15, g = lane >> 4;
     const double *wt = tab + TAB_W256;
@@ -472,6 +466,63 @@ __global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) v
             part[(size_t)j * prow + k] = reduce(x, M);
         }
     }
+}
+
+template <int TM, int TN, int WN>
+__global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) void k_commit_crs(
+    const uint64_t *__restrict__ mhat, int kappa, const int8_t *WB, int r, int ktW, int kchunk,
+    int ktend, int nbx, double *__restrict__ partial, const double *__restrict__ tab, Mod M) {
+    __shared__ __align__(16) char lds[CommitShape<TM, TN, WN>::LDS_BYTES];
+    __shared__ double tpow[kMfmaMaxKappa][64];
+    __shared__ double step64[kMfmaMaxKappa];
+    const unsigned lid = logical_block();
+    const int nb = (int)(lid % (unsigned)nbx);
+    const unsigned rest = lid / (unsigned)nbx;
+    const int s = (int)(rest % 64u);
+    const int z = (int)(rest / 64u);
+    const int rtB = (7 * r + 15) / 16;
+    const int ktb = z * kchunk;
+    crs_body<TM, TN, WN>(lds, tpow, step64, mhat, kappa, WB + (size_t)GT * nb * FRAG,
+                         (size_t)ktW * rtB * FRAG, rtB, min(GT, rtB - GT * nb),
+                         min(16, r - 16 * nb), s, ktb, min(ktend, ktb + kchunk) - ktb,
+                         partial + ((size_t)z * 64 + s) * kappa * r + 16 * nb, r, tab, M);
+}
+
+// A batch of commits against the generated key: the job tables of
+// k_commit_mfma_batch
+template <int TM, int TN, int WN>
+__global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) void k_commit_crs_batch(
+    const uint64_t *__restrict__ mhat, int kappa, const CommitJob *__restrict__ jobs,
+    const uint2 *__restrict__ table, int kchunk, const double *__restrict__ tab, Mod M) {
+    __shared__ __align__(16) char lds[CommitShape<TM, TN, WN>::LDS_BYTES];
+    __shared__ double tpow[kMfmaMaxKappa][64];
+    __shared__ double step64[kMfmaMaxKappa];
+    const uint2 e = table[logical_block()];
+    const CommitJob jb = jobs[e.x];
+    const int s = (int)(e.y & 63u), z = (int)(e.y >> 6);
+    const int ktb = z * kchunk;
+    crs_body<TM, TN, WN>(lds, tpow, step64, mhat, kappa, jb.WB,
+                         (size_t)jb.ktiles * jb.rtiles * FRAG, jb.rtiles, jb.ntile, jb.rows, s, ktb,
+                         min(jb.ktiles, ktb + kchunk) - ktb,
+                         jb.partial + ((size_t)z * 64 + s) * kappa * jb.rows, jb.rows, tab, M);
+}
+
+template <int TM, int TN, int WN>
+int run_crs_batch(Context *ctx, const uint64_t *mhat, size_t kappa, const CommitJob *jobs,
+                  const uint2 *table, size_t nblk, size_t kchunk) {
+    Mod M{ctx->qd, ctx->qinv};
+    hipLaunchKernelGGL((k_commit_crs_batch<TM, TN, WN>), dim3((unsigned)nblk), dim3(256 * WN), 0,
+                       ctx->stream, mhat, (int)kappa, jobs, table, (int)kchunk, ctx->d_tables, M);
+    SALSAA_HIP(hipGetLastError());
+    return SALSAA_OK;
+}
+
+template <int TM>
+int run_crs_batch_n(Context *ctx, const uint64_t *mhat, size_t kappa, const CommitJob *jobs,
+                    const uint2 *table, size_t nblk, size_t kchunk, size_t max_tiles) {
+    if (max_tiles <= 2) return run_crs_batch<TM, 1, 2>(ctx, mhat, kappa, jobs, table, nblk, kchunk);
+    if (max_tiles <= 4) return run_crs_batch<TM, 2, 2>(ctx, mhat, kappa, jobs, table, nblk, kchunk);
+    return run_crs_batch<TM, 4, 2>(ctx, mhat, kappa, jobs, table, nblk, kchunk);
 }
 
 template <int TM, int TN, int WN>
@@ -561,6 +612,18 @@ int launch_commit_crs(Context *ctx, const uint64_t *mhat, size_t kappa, const in
     if (rtA <= 4) return run_crs_n<1>(ctx, mhat, kappa, WB, r, ktW, ktend, out);
     if (rtA <= 8) return run_crs_n<2>(ctx, mhat, kappa, WB, r, ktW, ktend, out);
     return run_crs_n<4>(ctx, mhat, kappa, WB, r, ktW, ktend, out);
+}
+
+int launch_commit_crs_batch(Context *ctx, const uint64_t *mhat, size_t kappa, const CommitJob *jobs,
+                            const uint2 *table, size_t nblk, size_t kchunk, size_t max_tiles) {
+    SALSAA_CHECK(kappa >= 1 && kappa <= kMfmaMaxKappa, "MFMA commit needs kappa <= 36");
+    SALSAA_CHECK(max_tiles >= 1 && max_tiles <= 7, "a batch job holds at most 16 witness rows");
+    SALSAA_CHECK(kchunk >= 2 && kchunk <= 1024 && kchunk % 2 == 0, "K chunk must be even, <= 1024");
+    SALSAA_CHECK(nblk >= 1 && nblk < (1u << 31), "commit grid size out of range");
+    const size_t rtA = (7 * kappa + 15) / 16;
+    if (rtA <= 4) return run_crs_batch_n<1>(ctx, mhat, kappa, jobs, table, nblk, kchunk, max_tiles);
+    if (rtA <= 8) return run_crs_batch_n<2>(ctx, mhat, kappa, jobs, table, nblk, kchunk, max_tiles);
+    return run_crs_batch_n<4>(ctx, mhat, kappa, jobs, table, nblk, kchunk, max_tiles);
 }
 
 int launch_commit_mfma_batch(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,

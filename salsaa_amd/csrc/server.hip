@@ -287,8 +287,13 @@ int launch_batch(salsaa_server *s, Slot &sl, Plan &p) {
     }
     SALSAA_TRY(launch_ntt_planes_batch(c, s->d_arena, sl.d_off, sl.d_m, sl.d_pjobs, pj.size(),
                                        pnblk));
-    SALSAA_TRY(launch_commit_mfma_batch(c, k.d_AB, s->kappa, k.a_kt, k.a_ktiles, sl.d_cjobs,
-                                        sl.d_table, table.size(), kc, (7 * std::min<size_t>(n, 16) + 15) / 16));
+    const size_t mt = (7 * std::min<size_t>(n, 16) + 15) / 16;
+    if (k.d_mhat)  // a CRS key generated inside the commit
+        SALSAA_TRY(launch_commit_crs_batch(c, k.d_mhat, s->kappa, sl.d_cjobs, sl.d_table,
+                                           table.size(), kc, mt));
+    else
+        SALSAA_TRY(launch_commit_mfma_batch(c, k.d_AB, s->kappa, k.a_kt, k.a_ktiles, sl.d_cjobs,
+                                            sl.d_table, table.size(), kc, mt));
     SALSAA_TRY(launch_commit_finalize_batch(c, sl.d_fjobs, fj.size(), fblk, s->kappa));
     for (size_t i = 0; i < n;) {
         size_t e = i;
@@ -582,7 +587,8 @@ int salsaa_server_create(salsaa_ctx *ctx, const salsaa_key *key, size_t kappa,
     CTX_OK(ctx);
     SALSAA_CHECK(key && opts && out, "null key, options or output");
     SALSAA_CHECK(key->k.ctx == &ctx->c, "key belongs to another context");
-    SALSAA_CHECK(key->k.d_AB != nullptr, "the server needs a key with MFMA planes");
+    SALSAA_CHECK(key->k.d_AB != nullptr || key->k.d_mhat != nullptr,
+                 "the server needs a key with MFMA planes or a generated CRS key");
     SALSAA_CHECK(kappa >= 1 && kappa <= key->k.kappa, "kappa must be 1..key rows");
     const salsaa_server_opts &o = *opts;
     SALSAA_CHECK(o.arena_elems >= 1 && o.max_batch_jobs >= 1 && o.max_batch_jobs <= 4096 &&

@@ -260,3 +260,32 @@ def test_batch_split_pass_equals_single_pass(sa, ctx, orc):
     same(outs[0], outs[1])
     same(outs[0], outs[2])
     key.close()
+
+
+def test_batch_with_generated_crs_key(sa, ctx, orc):
+    # the batched step over a CRS key kept as its module only (the key rows
+    # made in the batched commit kernel): eager step and graph replays = the
+    # plain path (per-instance views, the single commit kernel) = the oracle
+    say("case: batch, implicit CRS key")
+    m = [2048, 1024, 1024, 512, 256, 128, 64, 8]
+    kappa = [8, 32, 16, 8, 32, 16, 8, 32]
+    n = len(m)
+    module = orc.sample_uniform(13, sa.STREAM_A, (32, 64), Q)
+    key = ctx.crs_key(ctx.to_device(module), 2048, implicit=True)
+    b = sa.Batch(ctx, key, m, kappa, reps=2, fq2=1, tune=True)
+    fill(sa, ctx, b, 1300)
+    W0 = [b.slot(i)["witness"].download() for i in range(n)]
+    b.step_plain()
+    plain = snapshot(b, True)
+    for graph in (False, True, True):
+        b.step(graph=graph)
+        ctx.sync()
+        same(snapshot(b, False), plain)
+    A = orc.compute_commitment_keys(module, 2048, Q)
+    for i in (0, 1, 4, n - 1):
+        want = orc.commit(np.ascontiguousarray(A[: kappa[i], : m[i]]), W0[i].reshape(1, m[i], 64),
+                          Q, threads=8)
+        assert np.array_equal(plain[i]["commitment"], want), i
+        assert plain[i]["ok"].all()
+    b.close()
+    key.close()

@@ -105,3 +105,25 @@ def test_server_rejects_oversized_jobs(sa, ctx, keyA):
     with pytest.raises(sa.SalsaaError):
         srv.submit(48, 1)      # not a power of two
     srv.close()
+
+
+def test_server_with_generated_crs_key(sa, ctx, orc):
+    # the server over a CRS key kept as its module only (the key rows made in
+    # the batched commit kernel): mixed batches = one job at a time = oracle
+    say("case: server, implicit CRS key")
+    module = orc.sample_uniform(12, sa.STREAM_A, (KAPPA, 64), Q)
+    key = ctx.crs_key(ctx.to_device(module), MMAX, implicit=True)
+    jobs = jobs_list(orc, n=24, seed=5)
+    mixed = serve(sa, ctx, key, jobs, 64, 2, True)
+    alone = serve(sa, ctx, key, jobs, 1, 1, False)
+    for i, (x, y) in enumerate(zip(mixed, alone)):
+        for f in FIELDS:
+            assert np.array_equal(x[f], y[f]), (i, f)
+        assert x["ok"].all()
+    A = orc.compute_commitment_keys(module, MMAX, Q)
+    for i in (0, 3, len(jobs) - 1):
+        m, seed, host = jobs[i]
+        W = host if host is not None else orc.sample_bounded(seed, 0xB, (m, 64), 16, Q)
+        want_c = orc.commit(np.ascontiguousarray(A[:, :m]), W.reshape(1, m, 64), Q, threads=8)
+        assert np.array_equal(mixed[i]["commitment"], want_c), i
+    key.close()
