@@ -426,6 +426,41 @@ int salsaa_fold_step(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const uin
                      size_t r_out, const uint64_t *d_cs, size_t reps, int fq2, uint64_t nr,
                      const salsaa_fold_outputs *out);
 
+/* ------------------------------------------------------------------ protocol
+ * protocol (protocol.rs:125-483) on the device: the CRS of the module d_module
+ * [kappa][d] (an implicit key, crs.rs:23-72), the commitment of the witness
+ * d_W [r][m][d] (Coefficient, INIT_WIT_REP = r rows), then nrounds rounds of
+ *   decomp_ell + verify_decomp when parts[i] > 1       (protocol.rs:166-196)
+ *   norm_check_1/2/3 + their verifiers, 2 repetitions  (:198-259)
+ *   split + verifier_split                             (:261-291)
+ *   challenge_for_project, project, batch_projections,
+ *     verify_batching                                  (:293-348)
+ *   challenge_for_fold, fold, verifier_fold (WIT_REP 1) (:349-386)
+ *   send_cross_terms_before_join, verifier_join, witness.append (:387-410)
+ * and the final check parallel_dot_series_matrix(statement, witness) ==
+ * verifier rhs (:464-472).  folding = 1 runs the folding scheme instead (no
+ * decomposition at the round's start, no split, and after the join
+ * decomp_ell(.., 2) + verify_decomp, then stop; parts may be NULL).  The
+ * statement grows by norm_check_3's two power series, witness_lhs and
+ * projected_lhs per round, and every commitment runs against all its rows.
+ * Challenges are drawn on the device from `seed` (the reference's are
+ * unseeded).  Outputs: *status (host) = 0 when every verifier relation held,
+ * else 100 * round + check of the first that failed (check 1 verify_decomp,
+ * 2 norm check, 3 verifier_split, 4 verify_batching, 5 the folding scheme's
+ * verify_decomp, 6 the final check); shape (host) [4] = witness rows and
+ * length, statement rows R and verifier columns; d_W_out (capacity w_cap u64)
+ * the final witness [r][m][d] and d_rhs_out (capacity rhs_cap) the verifier
+ * rhs [R][r][d] IncompleteNTT, either may be NULL; times (host, may be NULL)
+ * [8] milliseconds per phase: commitment, decomp, norm check, split,
+ * projection (with batching), fold, join, final check.  Needs kappa <= 36,
+ * m and r powers of two and 256 r <= m at every projection (project.rs
+ * panics otherwise). */
+int salsaa_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t m,
+                    const uint64_t *d_W, size_t r, size_t nrounds, const size_t *parts,
+                    int folding, uint64_t seed, uint64_t *d_W_out, size_t w_cap,
+                    uint64_t *d_rhs_out, size_t rhs_cap, size_t *shape, int32_t *status,
+                    double *times);
+
 /* ------------------------------------------------------------------ serving
  * A continuous-batching service of commit + norm-check jobs against one
  * resident key (no reference counterpart: it schedules the per-job calls

@@ -137,6 +137,9 @@ def _load():
         "salsaa_batch_phase": (_int, [_vp, _int]),
         "salsaa_fold_step": (_int, [_vp, _vp, _sz, _vp, _sz, _sz, _sz, _u64, _vp, _sz, _vp, _sz,
                                      _int, _u64, ctypes.POINTER(FoldOut)]),
+        "salsaa_protocol": (_int, [_vp, _vp, _sz, _sz, _vp, _sz, _sz, _PSZ, _int, _u64, _vp, _sz,
+                                    _vp, _sz, _PSZ, ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.POINTER(ctypes.c_double)]),
         "salsaa_server_create": (_int, [_vp, _vp, _sz, ctypes.POINTER(ServerOpts), _PP]),
         "salsaa_server_submit": (_int, [_vp, _sz, _vp, _u64, _u64, ctypes.POINTER(ctypes.c_uint64)]),
         "salsaa_server_poll": (_int, [_vp, _u64, _vp, ctypes.POINTER(ctypes.c_double),
@@ -265,6 +268,11 @@ class Key:
             self.close()
         except Exception:
             pass
+
+
+#: phases of salsaa_protocol's timers, in the order of its times[] output
+PROTOCOL_PHASES = ("commit", "decomp", "norm_check", "split", "projection", "fold", "join",
+                   "final")
 
 
 class Context:
@@ -517,6 +525,41 @@ class Context:
         _check(lib.salsaa_fold_step(self.h, key.h, n, W.ptr, r, m, parts, radix, C.ptr, r_out,
                                     cs.ptr, reps, int(fq2), nr, ctypes.byref(fo)))
         return o
+
+
+    def protocol(self, module, W, nrounds, parts=None, folding=False, seed=0):
+        """protocol (protocol.rs:125-483) on the device under the CRS of module
+        [kappa][d], from the witness W [r][m][d] (Coefficient).  Returns a dict:
+        status (0 when every verifier relation held, else 100 * round +
+        check), W (final witness), rhs (verifier rhs [R][r][d]) and times (ms
+        per phase, PROTOCOL_PHASES)."""
+        kappa = module.shape[0]
+        r, m, d = W.shape
+        parts = list(parts) if parts is not None else [1] * nrounds
+        rf, mf, R = r, m, kappa
+        for i in range(nrounds):
+            if not folding and parts[i] > 1:
+                rf *= parts[i]
+            R += 2
+            if not folding:
+                rf, mf = rf * 2, mf // 2
+            R += 2
+            rf = 2
+            if folding:
+                rf *= 2
+                break
+        Wout, rhs = self.empty((rf, mf, d)), self.empty((R, rf, d))
+        pa = (ctypes.c_size_t * max(1, len(parts)))(*parts)
+        shape = (ctypes.c_size_t * 4)()
+        st = ctypes.c_int32(-1)
+        times = (ctypes.c_double * 8)()
+        _check(lib.salsaa_protocol(self.h, module.ptr, kappa, m, W.ptr, r, nrounds, pa,
+                                   int(folding), seed, Wout.ptr, Wout.nbytes // 8, rhs.ptr,
+                                   rhs.nbytes // 8,
+                                   shape, ctypes.byref(st), times))
+        assert tuple(shape) == (rf, mf, R, rf), tuple(shape)
+        return dict(status=st.value, W=Wout, rhs=rhs,
+                    times=dict(zip(PROTOCOL_PHASES, list(times))))
 
 
 class Batch:
