@@ -551,6 +551,54 @@ def crs(sa, ctx, steps):
             "bit_exact_commit": bool(np.array_equal(a, b) and np.array_equal(a, c))}
 
 
+# The protocol runs (salsaa_protocol): the reference's parameter sets at the
+# same witness size in coefficients, in ring degree 64 (the reference's sets
+# use N = 128, so m doubles).  (name, kappa, m, INIT_WIT_REP, rounds, folding,
+# published prover and commit seconds at N = 128, experiments/results.md:5-19)
+PROTOCOL_SETS = (("A0", 12, 1 << 20, 1, 8, False, 3.05, 0.0796),
+                 ("B0", 12, 1 << 22, 1, 10, False, 10.61, 0.34822),
+                 ("F1", 13, 1 << 20, 4, 1, True, 1.66, 0.35612),
+                 ("F2", 13, 1 << 22, 4, 1, True, 5.52, 1.38))
+
+
+def protocol(sa, ctx, runs=2):
+    """Whole protocol runs on the device: the commitment, every round's prover
+    operators and verifier relations, and the final check, with the
+    reference's DECOMP_PARTS = [1, 2, 2, ..] (protocol.rs:26).  Each set runs
+    once untimed, then `runs` times; wall = host time around the call
+    (allocation and the final synchronisation included), phases = device
+    time per phase summed over rounds."""
+    res = {}
+    for name, kappa, m, r, rounds, fold, pub_prover, pub_commit in PROTOCOL_SETS:
+        mod = ctx.uniform((kappa, 64), 21, sa.STREAM_A)
+        W = ctx.bounded((r, m, 64), 1, 21, sa.STREAM_W)
+        parts = [1] + [2] * (rounds - 1)
+        out = ctx.protocol(mod, W, rounds, parts, folding=fold, seed=1)
+        ok = out["status"] == 0
+        walls, phases = [], None
+        for i in range(runs):
+            out["W"].free()
+            out["rhs"].free()
+            t0 = time.perf_counter()
+            out = ctx.protocol(mod, W, rounds, parts, folding=fold, seed=2 + i)
+            walls.append((time.perf_counter() - t0) * 1e3)
+            ok = ok and out["status"] == 0
+            phases = out["times"]
+        out["W"].free()
+        out["rhs"].free()
+        for x in (mod, W):
+            x.free()
+        res[name] = {"kappa": kappa, "m": m, "d": 64, "init_wit_rep": r, "rounds": rounds,
+                     "scheme": "folding" if fold else "snark",
+                     "wall_ms": round(min(walls), 2),
+                     "phases_ms": {k: round(v, 3) for k, v in phases.items()},
+                     "device_ms": round(sum(phases.values()), 2),
+                     "all_relations_held": bool(ok),
+                     "reference_published_prover_s_at_N128": pub_prover,
+                     "reference_published_commit_s_at_N128": pub_commit}
+    return res
+
+
 def timed(ctx, fn, k):
     evs = [ctx.event() for _ in range(k + 1)]
     ctx.sync()
@@ -609,6 +657,7 @@ def main():
     ap.add_argument("--no-projection", action="store_true",
                     help="skip the projection / batching / cross terms")
     ap.add_argument("--no-crs", action="store_true", help="skip the CRS key generation")
+    ap.add_argument("--no-protocol", action="store_true", help="skip the protocol runs")
     ap.add_argument("--eager", action="store_true",
                     help="run the batched steps eagerly instead of replaying the hipGraph "
                          "(for counter collection)")
@@ -740,6 +789,8 @@ def main():
                                        0.0 if args.no_cpu_baseline else 8.0)
     if not args.no_crs:
         out["crs"] = crs(sa, ctx, args.steps)
+    if not args.no_protocol:
+        out["protocol"] = protocol(sa, ctx)
     if not args.no_single:
         out["single_instance"] = single_instance(sa, ctx, args.steps)
     if not args.no_cpu_baseline:
