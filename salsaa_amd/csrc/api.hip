@@ -232,6 +232,7 @@ int salsaa_ctx_destroy(salsaa_ctx *ctx) {
     for (auto &b : ctx->c.scratch)
         if (b.ptr) (void)hipFree(b.ptr);
     if (ctx->c.zeros.ptr) (void)hipFree(ctx->c.zeros.ptr);
+    if (ctx->c.pool) (void)hipMemPoolDestroy(ctx->c.pool);
     (void)hipFree(ctx->c.d_tables);
     (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;

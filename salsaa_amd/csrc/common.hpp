@@ -84,6 +84,8 @@ struct Context {
     bool mfma_commit = true;        // int8 MFMA commit (false: fp64 VALU commit)
     bool mfma_project = true;       // int8 MFMA projection (false: VALU signed sums)
     ScratchBuf zeros;               // zero-filled, grown on demand
+    hipMemPool_t pool = nullptr;    // stream-ordered pool of the protocol run's buffers (kept
+                                    // reserved between runs, released with the context)
     std::mutex lock;                // one host thread drives a context at a time
 };
 

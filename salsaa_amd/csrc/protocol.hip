@@ -133,19 +133,22 @@ __global__ void k_nc_series(const uint64_t *__restrict__ cs, int nrounds, Mod M,
     }
 }
 
-// Device buffers of one run, released after the stream has drained.
+// Device buffers of one run, drawn from the context's stream-ordered pool and
+// returned to it at the end (the pool keeps the memory reserved, so a repeated
+// run neither maps nor unmaps device memory on the host's path).
 struct Arena {
     hipStream_t s;
+    hipMemPool_t pool;
     std::vector<void *> p;
-    explicit Arena(hipStream_t st) : s(st) {}
+    Arena(hipStream_t st, hipMemPool_t pl) : s(st), pool(pl) {}
     ~Arena() {
+        for (void *x : p) (void)hipFreeAsync(x, s);
         (void)hipStreamSynchronize(s);
-        for (void *x : p) (void)hipFree(x);
     }
     template <class T>
     int get(size_t n, T **out) {
         void *x = nullptr;
-        if (hipMalloc(&x, (n ? n : 1) * sizeof(T)) != hipSuccess) {
+        if (hipMallocFromPoolAsync(&x, (n ? n : 1) * sizeof(T), pool, s) != hipSuccess) {
             (void)hipGetLastError();
             set_error("protocol: out of device memory");
             return SALSAA_ERR_NOMEM;
@@ -155,6 +158,21 @@ struct Arena {
         return SALSAA_OK;
     }
 };
+
+int pool_get(Context *c, hipMemPool_t *out) {
+    if (!c->pool) {
+        hipMemPoolProps props = {};
+        props.allocType = hipMemAllocationTypePinned;
+        props.handleTypes = hipMemHandleTypeNone;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = c->device;
+        SALSAA_HIP(hipMemPoolCreate(&c->pool, &props));
+        uint64_t keep = UINT64_MAX;
+        SALSAA_HIP(hipMemPoolSetAttribute(c->pool, hipMemPoolAttrReleaseThreshold, &keep));
+    }
+    *out = c->pool;
+    return SALSAA_OK;
+}
 
 // A statement row other than the CRS rows.
 struct Row {
@@ -284,7 +302,9 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
     const hipStream_t st = c->stream;
     const size_t b = kD * sizeof(uint64_t);
     const Mod M{c->qd, c->qinv};
-    Arena A(st);
+    hipMemPool_t pool = nullptr;
+    SALSAA_TRY(pool_get(c, &pool));
+    Arena A(st, pool);
     Run run{ctx, c, nullptr, d_module, kappa, {}, nullptr, nullptr, &A};
     SALSAA_TRY(salsaa_key_crs_implicit(ctx, d_module, kappa, m, &run.key));
     struct KeyGuard {
