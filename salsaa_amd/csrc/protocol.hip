@@ -25,6 +25,7 @@
 // verify_decomp, 6 the final commitment check (protocol.rs:464-472).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -209,6 +210,7 @@ struct Run {
                      "protocol: a statement row has no layer of the witness length (the "
                      "reference finds none either)");
         const size_t k = (size_t)lg2(l / w.m0);
+        if (out == nullptr) return SALSAA_OK;  // validation only
         if (w.tens == nullptr || k == 0) {
             *out = w.base;
             return SALSAA_OK;
@@ -219,14 +221,42 @@ struct Run {
         return SALSAA_OK;
     }
 
-    // parallel_dot_series_matrix over the whole statement: out [R][r][d]
+    // parallel_dot_series_matrix over the whole statement: out [R][r][d].  A
+    // periodic row needs only the witness summed over its period,
+    // sum_x base[x mod m0] W[x] = sum_j base[j] sum_t W[t m0 + j]: the sums
+    // over the longest period P are taken in one pass over W and folded to
+    // each shorter period.
     int commit(const uint64_t *W, size_t r, size_t l, uint64_t *out) {
         SALSAA_TRY(salsaa_parallel_dot_series_matrix(ctx, key, W, r, l, out));
+        size_t P = 0;
         for (size_t e = 0; e < rows.size(); e++) {
             const uint64_t *lay = nullptr;
-            SALSAA_TRY(layer(rows[e], l, &lay));
-            SALSAA_TRY(salsaa_parallel_dot_single_series_matrix(ctx, lay, l, W, r,
-                                                                out + (kappa + e) * r * kD));
+            SALSAA_TRY(layer(rows[e], l, rows[e].tens ? nullptr : &lay));
+            if (rows[e].tens == nullptr) P = std::max(P, rows[e].m0);
+        }
+        uint64_t *SP = nullptr, *Ss = nullptr;
+        unsigned long long *acc = nullptr;
+        if (P) {
+            SALSAA_TRY(A->get(r * P * kD, &SP));
+            SALSAA_TRY(A->get(r * P * kD, &Ss));
+            SALSAA_TRY(A->get(r * P * kD, &acc));
+            SALSAA_TRY(launch_chunk_sums(c, W, r, l, P, SP, acc));
+        }
+        for (size_t e = 0; e < rows.size(); e++) {
+            const Row &w = rows[e];
+            uint64_t *o = out + (kappa + e) * r * kD;
+            if (w.tens == nullptr) {
+                const uint64_t *S = SP;
+                if (w.m0 != P) {
+                    SALSAA_TRY(launch_chunk_sums(c, SP, r, P, w.m0, Ss, acc));
+                    S = Ss;
+                }
+                SALSAA_TRY(salsaa_parallel_dot_single_series_matrix(ctx, w.base, w.m0, S, r, o));
+            } else {
+                const uint64_t *lay = nullptr;
+                SALSAA_TRY(layer(w, l, &lay));
+                SALSAA_TRY(salsaa_parallel_dot_single_series_matrix(ctx, lay, l, W, r, o));
+            }
         }
         return SALSAA_OK;
     }
