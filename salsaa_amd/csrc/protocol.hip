@@ -15,7 +15,12 @@
 // The layer of length l of a series with n tensors over a base of m0 elements
 // is the product of the base with the last k = log2(l / m0) tensors
 // (complete_power_series, arithmetic.rs:158-176), and verifier_split's
-// multipliers at witness length l are tensor n - k (split.rs:187-211).
+// multipliers at witness length l are tensor n - k (split.rs:187-211).  No
+// layer is expanded: a periodic row commits through the witness summed over
+// its period, and a norm-check row's layer is a scalar table (the product of
+// its constant tensors) times its base, so it commits as the base times the
+// table-weighted sum of the witness; all rows of each kind share one pass
+// over the witness.
 //
 // The verifier state V [R][c][d] (IncompleteNTT) has one row per statement
 // row and one column per witness row.  Each relation the reference asserts
@@ -134,6 +139,79 @@ __global__ void k_nc_series(const uint64_t *__restrict__ cs, int nrounds, Mod M,
     }
 }
 
+// The scalar factor of a norm-check row's layer of length 2^k:
+// w[x] = prod_j tens[n - k + j][bit j of x, from the top][0] (the tensors are
+// constants, so the layer is w[x] times the base element).
+__global__ void k_eq_table(const uint64_t *__restrict__ tens, int n, int k, Mod M,
+                           uint64_t *__restrict__ w) {
+    const size_t len = (size_t)1 << k;
+    for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < len;
+         x += (size_t)gridDim.x * blockDim.x) {
+        double t = 1.0;
+        for (int j = 0; j < k; j++) {
+            const int bit = (int)((x >> (k - 1 - j)) & 1);
+            t = mulmod(t, u2d(tens[((size_t)(n - k + j) * 2 + bit) * kD]), M);
+        }
+        w[x] = d2u(t);
+    }
+}
+
+constexpr int EV_MAXE = 8;   // norm-check rows per pass over the witness
+constexpr int EV_BLOCKS = 4096;
+
+// acc[e][k][c] += sum over this block's x of w[e][x] W[k][x][c]: the four
+// waves of a block take every fourth element, one coefficient per lane (a
+// wave reads one ring element, 512 contiguous bytes, per step).  Partial sums
+// are reduced mod q before the atomic, and at most EV_BLOCKS blocks add to
+// an entry, so acc stays below 2^62.
+__global__ __launch_bounds__(256) void k_eval_rows(const uint64_t *__restrict__ W, size_t r,
+                                                   size_t l, const uint64_t *__restrict__ w, int E,
+                                                   Mod M, unsigned long long *__restrict__ acc) {
+    __shared__ double red[4][EV_MAXE][kD];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    for (size_t k = 0; k < r; k++) {
+        double s[EV_MAXE];
+#pragma unroll
+        for (int e = 0; e < EV_MAXE; e++) s[e] = 0.0;
+        for (size_t x = (size_t)blockIdx.x * 4 + g; x < l; x += (size_t)gridDim.x * 4) {
+            const double v = u2d(W[(k * l + x) * kD + c]);
+#pragma unroll
+            for (int e = 0; e < EV_MAXE; e++)
+                if (e < E) {
+                    s[e] += mulmod(u2d(w[e * l + x]), v, M);
+                    if (s[e] >= M.q) s[e] -= M.q;
+                }
+        }
+#pragma unroll
+        for (int e = 0; e < EV_MAXE; e++)
+            if (e < E) red[g][e][c] = s[e];
+        __syncthreads();
+        if (g == 0)
+            for (int e = 0; e < E; e++) {
+                double t = red[0][e][c];
+                for (int h = 1; h < 4; h++) {
+                    t += red[h][e][c];
+                    if (t >= M.q) t -= M.q;
+                }
+                atomicAdd(&acc[(e * r + k) * kD + c], (unsigned long long)t);
+            }
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_acc(unsigned long long *__restrict__ a, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        a[i] = 0;
+}
+
+__global__ void k_eval_finish(const unsigned long long *__restrict__ acc, size_t n, uint64_t q,
+                              uint64_t *__restrict__ out) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        out[i] = acc[i] % q;
+}
+
 // Device buffers of one run, drawn from the context's stream-ordered pool and
 // returned to it at the end (the pool keeps the memory reserved, so a repeated
 // run neither maps nor unmaps device memory on the host's path).
@@ -198,26 +276,17 @@ struct Run {
     const uint64_t *module;
     size_t kappa;
     std::vector<Row> rows;  // rows[e] for e >= kappa
-    uint64_t *tmp_layer = nullptr;
     uint64_t *one = nullptr;
     Arena *A;
 
     hipStream_t st() const { return c->stream; }
 
-    // the layer of length l of a non-CRS row
-    int layer(const Row &w, size_t l, const uint64_t **out) {
+    // a non-CRS row must have a layer of the witness length l (m0 << k for
+    // some k <= n); the reference panics otherwise
+    int check_layer(const Row &w, size_t l) {
         SALSAA_CHECK(l >= w.m0 && l % w.m0 == 0 && pow2(l / w.m0) && lg2(l / w.m0) <= (int)w.n,
                      "protocol: a statement row has no layer of the witness length (the "
                      "reference finds none either)");
-        const size_t k = (size_t)lg2(l / w.m0);
-        if (out == nullptr) return SALSAA_OK;  // validation only
-        if (w.tens == nullptr || k == 0) {
-            *out = w.base;
-            return SALSAA_OK;
-        }
-        SALSAA_TRY(salsaa_complete_power_series(ctx, w.base, w.m0, w.tens + (w.n - k) * 2 * kD, k,
-                                                tmp_layer));
-        *out = tmp_layer;
         return SALSAA_OK;
     }
 
@@ -230,8 +299,7 @@ struct Run {
         SALSAA_TRY(salsaa_parallel_dot_series_matrix(ctx, key, W, r, l, out));
         size_t P = 0;
         for (size_t e = 0; e < rows.size(); e++) {
-            const uint64_t *lay = nullptr;
-            SALSAA_TRY(layer(rows[e], l, rows[e].tens ? nullptr : &lay));
+            SALSAA_TRY(check_layer(rows[e], l));
             if (rows[e].tens == nullptr) P = std::max(P, rows[e].m0);
         }
         uint64_t *SP = nullptr, *Ss = nullptr;
@@ -252,10 +320,44 @@ struct Run {
                     S = Ss;
                 }
                 SALSAA_TRY(salsaa_parallel_dot_single_series_matrix(ctx, w.base, w.m0, S, r, o));
-            } else {
-                const uint64_t *lay = nullptr;
-                SALSAA_TRY(layer(w, l, &lay));
-                SALSAA_TRY(salsaa_parallel_dot_single_series_matrix(ctx, lay, l, W, r, o));
+            }
+        }
+        // norm-check rows: layer[x] = w_e[x] base_e, so the row's commitment is
+        // base_e times the w_e-weighted sum of the witness, for up to EV_MAXE
+        // rows per pass over W
+        std::vector<size_t> nrow;
+        for (size_t e = 0; e < rows.size(); e++)
+            if (rows[e].tens) nrow.push_back(e);
+        if (!nrow.empty()) {
+            const int k = lg2(l);
+            const Mod M{c->qd, c->qinv};
+            uint64_t *wts = nullptr, *V = nullptr, *Brep = nullptr;
+            unsigned long long *acc = nullptr;
+            SALSAA_TRY(A->get(EV_MAXE * l, &wts));
+            SALSAA_TRY(A->get(EV_MAXE * r * kD, &V));
+            SALSAA_TRY(A->get(r * kD, &Brep));
+            SALSAA_TRY(A->get(EV_MAXE * r * kD, &acc));
+            const unsigned nb = (unsigned)std::min<size_t>((l + 3) / 4, EV_BLOCKS);
+            for (size_t g0 = 0; g0 < nrow.size(); g0 += EV_MAXE) {
+                const int E = (int)std::min<size_t>(EV_MAXE, nrow.size() - g0);
+                for (int i = 0; i < E; i++) {
+                    const Row &w = rows[nrow[g0 + i]];
+                    hipLaunchKernelGGL(k_eq_table, dim3(grid1(l, 256)), dim3(256), 0, st(), w.tens,
+                                       (int)w.n, k, M, wts + (size_t)i * l);
+                }
+                hipLaunchKernelGGL(k_zero_acc, dim3(grid1(E * r * kD, 256)), dim3(256), 0, st(),
+                                   acc, (size_t)E * r * kD);
+                hipLaunchKernelGGL(k_eval_rows, dim3(nb), dim3(256), 0, st(), W, r, l, wts, E, M,
+                                   acc);
+                hipLaunchKernelGGL(k_eval_finish, dim3(grid1(E * r * kD, 256)), dim3(256), 0,
+                                   st(), acc, (size_t)E * r * kD, c->q, V);
+                SALSAA_HIP(hipGetLastError());
+                for (int i = 0; i < E; i++) {
+                    const size_t e = nrow[g0 + i];
+                    SALSAA_TRY(launch_repeat(c, rows[e].base, 1, r, Brep));
+                    SALSAA_TRY(salsaa_ring_mul(ctx, V + (size_t)i * r * kD, Brep,
+                                               out + (kappa + e) * r * kD, r));
+                }
             }
         }
         return SALSAA_OK;
@@ -335,7 +437,7 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
     hipMemPool_t pool = nullptr;
     SALSAA_TRY(pool_get(c, &pool));
     Arena A(st, pool);
-    Run run{ctx, c, nullptr, d_module, kappa, {}, nullptr, nullptr, &A};
+    Run run{ctx, c, nullptr, d_module, kappa, {}, nullptr, &A};
     SALSAA_TRY(salsaa_key_crs_implicit(ctx, d_module, kappa, m, &run.key));
     struct KeyGuard {
         salsaa_key *k;
@@ -347,7 +449,6 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
     } kg{run.key, st};
     int32_t *status = nullptr;
     SALSAA_TRY(A.get(1, &status));
-    SALSAA_TRY(A.get(m * kD, &run.tmp_layer));
     SALSAA_TRY(A.get(kD, &run.one));
     SALSAA_HIP(hipMemsetAsync(status, 0, sizeof(int32_t), st));
     SALSAA_HIP(hipMemsetAsync(run.one, 0, b, st));
