@@ -212,6 +212,26 @@ __global__ void k_eval_finish(const unsigned long long *__restrict__ acc, size_t
         out[i] = acc[i] % q;
 }
 
+// verifier_split's multipliers T [R][2][d] of up to kMultRows rows per launch:
+// kind 0 [1, 1] (all-ones tensors), 1 [1, src] (CRS row: src = m_j^(l/2)),
+// 2 the tensor [2][d] at src (norm-check row).
+constexpr int kMultRows = 64;
+struct MultSrc {
+    const uint64_t *src[kMultRows];
+    int kind[kMultRows];
+};
+
+__global__ void k_build_mult(MultSrc a, int rows, uint64_t *__restrict__ T) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < rows * 2 * kD;
+         i += gridDim.x * blockDim.x) {
+        const int e = i / (2 * kD), side = (i / kD) & 1, c = i % kD;
+        uint64_t v = c == 0 ? 1 : 0;
+        if (a.kind[e] == 1 && side == 1) v = a.src[e][c];
+        if (a.kind[e] == 2) v = a.src[e][side * kD + c];
+        T[i] = v;
+    }
+}
+
 // Device buffers of one run, drawn from the context's stream-ordered pool and
 // returned to it at the end (the pool keeps the memory reserved, so a repeated
 // run neither maps nor unmaps device memory on the host's path).
@@ -378,23 +398,31 @@ struct Run {
             SALSAA_TRY(salsaa_convert(ctx, sq, kappa, SALSAA_REP_INCOMPLETE, SALSAA_REP_COEFF));
             std::swap(pw, sq);
         }
-        for (size_t j = 0; j < kappa; j++) {
-            SALSAA_HIP(hipMemcpyAsync(T + j * 2 * kD, one, b, hipMemcpyDeviceToDevice, st()));
-            SALSAA_HIP(hipMemcpyAsync(T + (j * 2 + 1) * kD, pw + j * kD, b,
-                                      hipMemcpyDeviceToDevice, st()));
-        }
-        for (size_t e = 0; e < rows.size(); e++) {
-            const Row &w = rows[e];
-            uint64_t *t = T + (kappa + e) * 2 * kD;
-            if (w.tens == nullptr) {
-                SALSAA_HIP(hipMemcpyAsync(t, one, b, hipMemcpyDeviceToDevice, st()));
-                SALSAA_HIP(hipMemcpyAsync(t + kD, one, b, hipMemcpyDeviceToDevice, st()));
-            } else {
-                const size_t k = (size_t)lg2(l / w.m0);
-                SALSAA_CHECK(k >= 1 && k <= w.n, "protocol: split below a row's base layer");
-                SALSAA_HIP(hipMemcpyAsync(t, w.tens + (w.n - k) * 2 * kD, 2 * b,
-                                          hipMemcpyDeviceToDevice, st()));
+        // one table of sources per kMultRows rows
+        const size_t R_ = R();
+        for (size_t e0 = 0; e0 < R_; e0 += kMultRows) {
+            MultSrc a = {};
+            const int n = (int)std::min<size_t>(kMultRows, R_ - e0);
+            for (int i = 0; i < n; i++) {
+                const size_t e = e0 + i;
+                if (e < kappa) {
+                    a.kind[i] = 1;
+                    a.src[i] = pw + e * kD;
+                } else {
+                    const Row &w = rows[e - kappa];
+                    if (w.tens == nullptr) {
+                        a.kind[i] = 0;
+                    } else {
+                        const size_t k = (size_t)lg2(l / w.m0);
+                        SALSAA_CHECK(k >= 1 && k <= w.n, "protocol: split below a row's base layer");
+                        a.kind[i] = 2;
+                        a.src[i] = w.tens + (w.n - k) * 2 * kD;
+                    }
+                }
             }
+            hipLaunchKernelGGL(k_build_mult, dim3(grid1((size_t)n * 2 * kD, 256)), dim3(256), 0,
+                               st(), a, n, T + e0 * 2 * kD);
+            SALSAA_HIP(hipGetLastError());
         }
         return SALSAA_OK;
     }
