@@ -86,7 +86,7 @@ struct Context {
     ScratchBuf zeros;               // zero-filled, grown on demand
     hipMemPool_t pool = nullptr;    // stream-ordered pool of the protocol run's buffers (kept
                                     // reserved between runs, released with the context)
-    std::mutex lock;                // one host thread drives a context at a time
+    std::recursive_mutex lock;      // held by every entry point on this context
 };
 
 // A context on another stream over the same device tables (its scratch and

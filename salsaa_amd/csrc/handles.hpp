@@ -10,11 +10,13 @@ struct salsaa_key {
     salsaa::Key k;
 };
 
-#define CTX_OK(ctx)                                                  \
-    do {                                                             \
-        if (!(ctx)) {                                                \
-            ::salsaa::set_error("null context");                     \
-            return SALSAA_ERR_ARG;                                   \
-        }                                                            \
-        SALSAA_HIP(hipSetDevice((ctx)->c.device));                   \
-    } while (0)
+// Checks the context, holds its lock until the calling entry point returns
+// (recursive: entry points that call other entry points keep it), and makes
+// its device current.  Use once per function scope.
+#define CTX_OK(ctx)                                                          \
+    if (!(ctx)) {                                                            \
+        ::salsaa::set_error("null context");                                 \
+        return SALSAA_ERR_ARG;                                               \
+    }                                                                        \
+    std::lock_guard<std::recursive_mutex> salsaa_ctx_guard_((ctx)->c.lock); \
+    SALSAA_HIP(hipSetDevice((ctx)->c.device))

@@ -728,7 +728,6 @@ extern "C" int salsaa_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t
                                uint64_t *d_W_out, size_t w_cap, uint64_t *d_rhs_out,
                                size_t rhs_cap, size_t *shape, int32_t *status, double *times) {
     CTX_OK(ctx);
-    std::lock_guard<std::mutex> g(ctx->c.lock);
     return run_protocol(ctx, d_module, kappa, m, d_W, r, nrounds, parts, folding, seed, d_W_out,
                         w_cap, d_rhs_out, rhs_cap, shape, status, times);
 }
