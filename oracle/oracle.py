@@ -66,6 +66,7 @@ def _load():
         "or_fq2_mul": (None, [u64, u64, u64, u64, u64, u64, P]),
         "or_prng_draw": (u64, [u64, u64, u64]),
         "or_sample_uniform": (None, [u64, u64, sz, u64, P]),
+        "or_sample_uniform_at": (None, [u64, u64, sz, sz, u64, P]),
         "or_sample_bounded": (None, [u64, u64, sz, u64, u64, P]),
     }
     for name, (res, args) in sig.items():
@@ -414,6 +415,14 @@ def sample_uniform(seed, stream, shape, q):
     n = int(np.prod(shape))
     out = np.zeros(n, np.uint64)
     lib.or_sample_uniform(seed, stream, n, q, _p(out))
+    return out.reshape(shape)
+
+
+def sample_uniform_at(seed, stream, start, shape, q):
+    """Elements start, start + 1, .. of sample_uniform's stream."""
+    n = int(np.prod(shape))
+    out = np.zeros(n, np.uint64)
+    lib.or_sample_uniform_at(seed, stream, start, n, q, _p(out))
     return out.reshape(shape)
 
 

@@ -844,6 +844,14 @@ void or_sample_uniform(uint64_t seed, uint64_t stream, size_t n, uint64_t q,
         out[i] = (uint64_t)(((u128)or_prng_draw(seed, stream, i) * q) >> 64);
 }
 
+/* elements start .. start + n - 1 of the same uniform stream (a row range
+ * of a matrix sampled whole by or_sample_uniform) */
+void or_sample_uniform_at(uint64_t seed, uint64_t stream, size_t start, size_t n, uint64_t q,
+                          uint64_t *out) {
+    for (size_t i = 0; i < n; i++)
+        out[i] = (uint64_t)(((u128)or_prng_draw(seed, stream, start + i) * q) >> 64);
+}
+
 /* uniform over the 2*bound+1 integers [-bound, bound], mapped mod q
  * (cyclotomic_ring.rs:293-317 draws |v| < b with a random sign) */
 void or_sample_bounded(uint64_t seed, uint64_t stream, size_t n, uint64_t bound,

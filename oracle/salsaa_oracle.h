@@ -167,6 +167,8 @@ void or_fq2_mul(uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1, uint64_t nr,
 uint64_t or_prng_draw(uint64_t seed, uint64_t stream, uint64_t idx);
 void or_sample_uniform(uint64_t seed, uint64_t stream, size_t n, uint64_t q,
                        uint64_t *out);
+void or_sample_uniform_at(uint64_t seed, uint64_t stream, size_t start, size_t n, uint64_t q,
+                          uint64_t *out);
 void or_sample_bounded(uint64_t seed, uint64_t stream, size_t n, uint64_t bound,
                        uint64_t q, uint64_t *out);
 
