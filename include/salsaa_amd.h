@@ -291,6 +291,21 @@ int salsaa_norm_check(salsaa_ctx *ctx, const uint64_t *d_w, size_t rows, size_t 
                       uint64_t *d_claims, uint64_t *d_msgs, uint64_t *d_finals,
                       uint64_t *d_evals, uint64_t *d_rhs, int32_t *d_ok);
 
+/* salsaa_norm_check without any allocation.  No counterpart in the
+ * reference: this entry point is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, outputs and kernels as
+ * salsaa_norm_check; the scratch is a fixed 64 MiB device array that each GPU
+ * holds for the life of the process, so reps * rows * L must stay within
+ * about 60 M (for example 2 reps x 8 rows x 2^21 coefficients); larger calls
+ * return SALSAA_ERR_ARG.  Every call on one GPU shares that array, so calls
+ * that overlap on one GPU (from several contexts or streams) race; calls on
+ * one context are ordered by its stream. */
+int salsaa_norm_check_noalloc(salsaa_ctx *ctx, const uint64_t *d_w, size_t rows, size_t L,
+                              const uint64_t *d_cs, size_t reps, int fq2, uint64_t nr,
+                              uint64_t *d_claims, uint64_t *d_msgs, uint64_t *d_finals,
+                              uint64_t *d_evals, uint64_t *d_rhs, int32_t *d_ok);
+
 /* decompose_matrix_by_chunks (arithmetic.rs:1572-1593): W [r][m][d]
  * Coefficient -> d_out [r*parts][m][d], limb t of row rho in row rho*parts+t.
  * radix = 0 selects the reference's auto radix 2^ceil(bw/parts); the radix

@@ -123,6 +123,8 @@ def _load():
         "salsaa_split": (_int, [_vp, _vp, _vp, _sz, _sz, _vp]),
         "salsaa_norm_check": (_int, [_vp, _vp, _sz, _sz, _vp, _sz, _int, _u64, _vp, _vp, _vp,
                                      _vp, _vp, _vp]),
+        "salsaa_norm_check_noalloc": (_int, [_vp, _vp, _sz, _sz, _vp, _sz, _int, _u64, _vp, _vp,
+                                             _vp, _vp, _vp, _vp]),
         "salsaa_decompose": (_int, [_vp, _vp, _sz, _sz, _sz, _u64, _vp, _vp]),
         "salsaa_decomp_ell": (_int, [_vp, _vp, _vp, _sz, _sz, _sz, _u64, _vp, _vp, _vp]),
         "salsaa_compose": (_int, [_vp, _vp, _sz, _sz, _sz, _vp, _vp]),
@@ -457,6 +459,29 @@ class Context:
         _check(lib.salsaa_norm_check(self.h, w.ptr, rows, L, cs.ptr, reps, int(fq2), nr,
                                      outs["claims"].ptr, outs["msgs"].ptr, outs["finals"].ptr,
                                      outs["evals"].ptr, rhs, outs["ok"].ptr))
+        return outs
+
+    def norm_check_noalloc(self, w, cs, fq2, nr=NR, want_rhs=True, outs=None):
+        """norm_check through salsaa_norm_check_noalloc: the same outputs, with
+        the library's fixed device scratch instead of allocated scratch (the
+        output buffers here are still allocated by this mirror)."""
+        rows = w.shape[0]
+        L = int(np.prod(w.shape[1:]))
+        reps, nrounds, _ = cs.shape
+        if outs is None:
+            outs = dict(
+                claims=self.empty((rows,)),
+                msgs=self.empty((reps, nrounds, rows, 3, 2)),
+                finals=self.empty((reps, rows, 2)),
+                evals=self.empty((reps, rows, 2)),
+                rhs=self.empty((reps, rows, 2, self.d)) if want_rhs else None,
+                ok=self.empty((reps, rows), np.int32),
+            )
+        rhs = outs["rhs"].ptr if outs.get("rhs") is not None else None
+        _check(lib.salsaa_norm_check_noalloc(self.h, w.ptr, rows, L, cs.ptr, reps, int(fq2), nr,
+                                             outs["claims"].ptr, outs["msgs"].ptr,
+                                             outs["finals"].ptr, outs["evals"].ptr, rhs,
+                                             outs["ok"].ptr))
         return outs
 
     def decompose(self, W, parts, radix=0, out=None, d_radix=None):

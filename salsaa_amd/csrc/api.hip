@@ -7,10 +7,48 @@
 
 #include "handles.hpp"
 
+// Scratch of salsaa_norm_check_noalloc (the message accumulators and the
+// folded witnesses between passes of one norm check): a fixed array that
+// every device the library runs on holds until the process ends.  Host code
+// never passes `scratch` itself to a kernel: it uses the device address that
+// hipGetSymbolAddress gives for each device (noalloc_scratch below).
+static constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace salsaa {
 
 namespace {
 thread_local std::string tl_error;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxDevices] = {};
+
+// Resolves the device address of `scratch` on device `dev`, which must be
+// current; once per device, at context creation.
+int noalloc_scratch_init(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) {
+        tl_error = "device index out of range";
+        return SALSAA_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            tl_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return SALSAA_ERR_HIP;
+        }
+        g_noalloc_ptr[dev] = p;
+    }
+    return SALSAA_OK;
+}
+
+void *noalloc_scratch(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    return g_noalloc_ptr[dev];
+}
 
 uint64_t h_min_root(uint64_t order, uint64_t q) {
     uint64_t w = 0;
@@ -220,6 +258,13 @@ int salsaa_ctx_create(uint64_t q, uint32_t d, salsaa_ctx **out) {
         if (x.d_tables) (void)hipFree(x.d_tables);
         delete c;
         return SALSAA_ERR_HIP;
+    }
+    const int st = noalloc_scratch_init(dev);
+    if (st != SALSAA_OK) {
+        (void)hipStreamDestroy(x.stream);
+        (void)hipFree(x.d_tables);
+        delete c;
+        return st;
     }
     *out = c;
     return SALSAA_OK;
@@ -626,6 +671,45 @@ int salsaa_norm_check(salsaa_ctx *ctx, const uint64_t *d_w, size_t rows, size_t 
     CTX_OK(ctx);
     return launch_norm_check(&ctx->c, d_w, rows, L, d_cs, reps, fq2, nr, d_claims, d_msgs,
                              d_finals, d_evals, d_rhs, d_ok);
+}
+
+int salsaa_norm_check_noalloc(salsaa_ctx *ctx, const uint64_t *d_w, size_t rows, size_t L,
+                              const uint64_t *d_cs, size_t reps, int fq2, uint64_t nr,
+                              uint64_t *d_claims, uint64_t *d_msgs, uint64_t *d_finals,
+                              uint64_t *d_evals, uint64_t *d_rhs, int32_t *d_ok) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(rows > 0 && reps > 0 && rows < 65536 && reps < 65536,
+                 "norm_check_noalloc: rows and reps must be in 1..65535");
+    SALSAA_CHECK(L >= 2 && (L & (L - 1)) == 0 && __builtin_ctzll(L) <= 48,
+                 "norm_check_noalloc: length must be a power of two >= 2");
+    // the folded witnesses alone take reps * rows * L bytes
+    if ((unsigned __int128)reps * rows * L > kNoAllocScratchBytes) {
+        set_error("norm_check_noalloc: reps * rows * L = " +
+                  std::to_string((double)reps * rows * L) +
+                  " bytes of scratch exceed the fixed 64 MiB (use salsaa_norm_check)");
+        return SALSAA_ERR_ARG;
+    }
+    size_t nm = 0, na = 0, nb = 0;
+    nc_scratch_need(rows, L, reps, &nm, &na, &nb);
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    if (up(nm) + up(na) + up(nb) > kNoAllocScratchBytes) {
+        set_error("norm_check_noalloc: needs " + std::to_string(up(nm) + up(na) + up(nb)) +
+                  " bytes of scratch, more than the fixed 64 MiB (use salsaa_norm_check)");
+        return SALSAA_ERR_ARG;
+    }
+    unsigned char *base = static_cast<unsigned char *>(noalloc_scratch(ctx->c.device));
+    SALSAA_CHECK(base != nullptr, "norm_check_noalloc: scratch not set up on this device");
+    NcScratch ext;
+    ext.macc = reinterpret_cast<unsigned long long *>(base);
+    ext.macc_bytes = nm;
+    ext.bufA = reinterpret_cast<double *>(base + up(nm));
+    ext.bufA_bytes = na;
+    ext.bufB = reinterpret_cast<double *>(base + up(nm) + up(na));
+    ext.bufB_bytes = nb;
+    const NcOut o = nc_out_plain(rows, reps, (size_t)__builtin_ctzll(L), d_claims, d_msgs,
+                                 d_finals, d_evals, d_rhs, d_ok);
+    return launch_norm_check_ex(&ctx->c, d_w, nullptr, rows, L, d_cs, 0, reps, fq2, nr, o, 0,
+                                &ext);
 }
 
 int salsaa_decompose(salsaa_ctx *ctx, const uint64_t *d_W, size_t r, size_t m, size_t parts,
