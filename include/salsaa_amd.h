@@ -476,6 +476,24 @@ int salsaa_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, siz
                     uint64_t *d_rhs_out, size_t rhs_cap, size_t *shape, int32_t *status,
                     double *times);
 
+/* salsaa_protocol with options (opts may be NULL: the same as
+ * salsaa_protocol).  fault_check 1..6 injects a fault, for tests of the
+ * verifier checks: one prover value that check reads is off by one (mod q) in
+ * round fault_round (check 6, the final check, ignores the round): 1 and 5 the
+ * commitment of the decomposed witness, 2 row 0's norm_check_1 claim (needs
+ * m * d > 1024 at that round), 3 the split commitments, 4 the projection
+ * after its commitment, 6 the final witness.  The run then reports that
+ * check's code in *status.  0 runs the protocol as it is. */
+typedef struct {
+    size_t fault_round;
+    int fault_check;
+} salsaa_protocol_opts;
+int salsaa_protocol_ex(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t m,
+                       const uint64_t *d_W, size_t r, size_t nrounds, const size_t *parts,
+                       int folding, uint64_t seed, uint64_t *d_W_out, size_t w_cap,
+                       uint64_t *d_rhs_out, size_t rhs_cap, size_t *shape, int32_t *status,
+                       double *times, const salsaa_protocol_opts *opts);
+
 /* ------------------------------------------------------------------ serving
  * A continuous-batching service of commit + norm-check jobs against one
  * resident key (no reference counterpart: it schedules the per-job calls

@@ -68,6 +68,8 @@ def _load():
         "or_sample_uniform": (None, [u64, u64, sz, u64, P]),
         "or_sample_uniform_at": (None, [u64, u64, sz, sz, u64, P]),
         "or_sample_bounded": (None, [u64, u64, sz, u64, u64, P]),
+        "or_sample_bits": (None, [u64, u64, sz, ctypes.c_void_p]),
+        "or_stream_key": (u64, [u64, u64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -424,6 +426,18 @@ def sample_uniform_at(seed, stream, start, shape, q):
     out = np.zeros(n, np.uint64)
     lib.or_sample_uniform_at(seed, stream, start, n, q, _p(out))
     return out.reshape(shape)
+
+
+def sample_bits(seed, stream, shape):
+    """uint8 0/1, one bit per element of the stream"""
+    n = int(np.prod(shape))
+    out = np.zeros(n, np.uint8)
+    lib.or_sample_bits(seed, stream, n, out.ctypes.data)
+    return out.reshape(shape)
+
+
+def stream_key(seed, stream):
+    return int(lib.or_stream_key(seed, stream))
 
 
 def sample_bounded(seed, stream, shape, bound, q):

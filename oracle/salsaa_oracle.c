@@ -852,6 +852,18 @@ void or_sample_uniform_at(uint64_t seed, uint64_t stream, size_t start, size_t n
         out[i] = (uint64_t)(((u128)or_prng_draw(seed, stream, start + i) * q) >> 64);
 }
 
+/* one random bit per byte (challenge_for_project, project.rs:26-39, draws
+ * rand::random::<bool>()): the low bit of each draw */
+void or_sample_bits(uint64_t seed, uint64_t stream, size_t n, uint8_t *out) {
+    for (size_t i = 0; i < n; i++) out[i] = (uint8_t)(or_prng_draw(seed, stream, i) & 1);
+}
+
+/* the key of stream `stream` under `seed` (the per-round seed of the
+ * protocol's challenges is the key of stream 0x7000 + round) */
+uint64_t or_stream_key(uint64_t seed, uint64_t stream) {
+    return mix64(seed ^ (stream * 0xD1B54A32D192ED03ULL));
+}
+
 /* uniform over the 2*bound+1 integers [-bound, bound], mapped mod q
  * (cyclotomic_ring.rs:293-317 draws |v| < b with a random sign) */
 void or_sample_bounded(uint64_t seed, uint64_t stream, size_t n, uint64_t bound,

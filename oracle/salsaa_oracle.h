@@ -169,6 +169,8 @@ void or_sample_uniform(uint64_t seed, uint64_t stream, size_t n, uint64_t q,
                        uint64_t *out);
 void or_sample_uniform_at(uint64_t seed, uint64_t stream, size_t start, size_t n, uint64_t q,
                           uint64_t *out);
+void or_sample_bits(uint64_t seed, uint64_t stream, size_t n, uint8_t *out);
+uint64_t or_stream_key(uint64_t seed, uint64_t stream);
 void or_sample_bounded(uint64_t seed, uint64_t stream, size_t n, uint64_t bound,
                        uint64_t q, uint64_t *out);
 

@@ -46,6 +46,11 @@ class ServerOpts(ctypes.Structure):
                 ("kchunk", ctypes.c_size_t)]
 
 
+class ProtocolOpts(ctypes.Structure):
+    """salsaa_protocol_opts."""
+    _fields_ = [("fault_round", ctypes.c_size_t), ("fault_check", ctypes.c_int)]
+
+
 class SlotT(ctypes.Structure):
     """salsaa_slot_t: device slots of one batch instance."""
     _fields_ = [(n, ctypes.c_void_p) for n in ("witness", "challenges", "commitment", "claims",
@@ -142,6 +147,10 @@ def _load():
         "salsaa_protocol": (_int, [_vp, _vp, _sz, _sz, _vp, _sz, _sz, _PSZ, _int, _u64, _vp, _sz,
                                     _vp, _sz, _PSZ, ctypes.POINTER(ctypes.c_int32),
                                     ctypes.POINTER(ctypes.c_double)]),
+        "salsaa_protocol_ex": (_int, [_vp, _vp, _sz, _sz, _vp, _sz, _sz, _PSZ, _int, _u64, _vp,
+                                       _sz, _vp, _sz, _PSZ, ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ProtocolOpts)]),
         "salsaa_server_create": (_int, [_vp, _vp, _sz, ctypes.POINTER(ServerOpts), _PP]),
         "salsaa_server_submit": (_int, [_vp, _sz, _vp, _u64, _u64, ctypes.POINTER(ctypes.c_uint64)]),
         "salsaa_server_poll": (_int, [_vp, _u64, _vp, ctypes.POINTER(ctypes.c_double),
@@ -552,14 +561,10 @@ class Context:
         return o
 
 
-    def protocol(self, module, W, nrounds, parts=None, folding=False, seed=0):
-        """protocol (protocol.rs:125-483) on the device under the CRS of module
-        [kappa][d], from the witness W [r][m][d] (Coefficient).  Returns a dict:
-        status (0 when every verifier relation held, else 100 * round +
-        check), W (final witness), rhs (verifier rhs [R][r][d]) and times (ms
-        per phase, PROTOCOL_PHASES)."""
-        kappa = module.shape[0]
-        r, m, d = W.shape
+    @staticmethod
+    def protocol_shape(kappa, r, m, nrounds, parts=None, folding=False):
+        """(rows, length) of the protocol's final witness and the statement
+        rows R of its verifier rhs [R][rows][d]."""
         parts = list(parts) if parts is not None else [1] * nrounds
         rf, mf, R = r, m, kappa
         for i in range(nrounds):
@@ -573,15 +578,35 @@ class Context:
             if folding:
                 rf *= 2
                 break
-        Wout, rhs = self.empty((rf, mf, d)), self.empty((R, rf, d))
+        return rf, mf, R
+
+    def protocol(self, module, W, nrounds, parts=None, folding=False, seed=0, fault=None,
+                 outs=None):
+        """protocol (protocol.rs:125-483) on the device under the CRS of module
+        [kappa][d], from the witness W [r][m][d] (Coefficient).  Returns a dict:
+        status (0 when every verifier relation held, else 100 * round +
+        check), W (final witness), rhs (verifier rhs [R][r][d]) and times (ms
+        per phase, PROTOCOL_PHASES).  fault = (round, check) injects a fault
+        for the verifier check `check` (salsaa_protocol_opts); outs = the W
+        and rhs of an earlier call of the same shape, written again instead
+        of allocating new outputs."""
+        kappa = module.shape[0]
+        r, m, d = W.shape
+        parts = list(parts) if parts is not None else [1] * nrounds
+        rf, mf, R = self.protocol_shape(kappa, r, m, nrounds, parts, folding)
+        if outs is not None and outs["W"].shape == (rf, mf, d) and outs["rhs"].shape == (R, rf, d):
+            Wout, rhs = outs["W"], outs["rhs"]
+        else:
+            Wout, rhs = self.empty((rf, mf, d)), self.empty((R, rf, d))
         pa = (ctypes.c_size_t * max(1, len(parts)))(*parts)
         shape = (ctypes.c_size_t * 4)()
         st = ctypes.c_int32(-1)
         times = (ctypes.c_double * 8)()
-        _check(lib.salsaa_protocol(self.h, module.ptr, kappa, m, W.ptr, r, nrounds, pa,
-                                   int(folding), seed, Wout.ptr, Wout.nbytes // 8, rhs.ptr,
-                                   rhs.nbytes // 8,
-                                   shape, ctypes.byref(st), times))
+        opts = ProtocolOpts(fault[0], fault[1]) if fault else None
+        _check(lib.salsaa_protocol_ex(self.h, module.ptr, kappa, m, W.ptr, r, nrounds, pa,
+                                      int(folding), seed, Wout.ptr, Wout.nbytes // 8, rhs.ptr,
+                                      rhs.nbytes // 8, shape, ctypes.byref(st), times,
+                                      ctypes.byref(opts) if opts else None))
         assert tuple(shape) == (rf, mf, R, rf), tuple(shape)
         return dict(status=st.value, W=Wout, rhs=rhs,
                     times=dict(zip(PROTOCOL_PHASES, list(times))))

@@ -415,11 +415,15 @@ void nc_scratch_need(size_t rows, size_t L, size_t reps, size_t *macc_bytes, siz
 // cs_row: u64 between consecutive rows' challenge blocks (0: shared);
 // cpt_first: columns per thread of the first pass (0: automatic);
 // sc: the short copy of the arena starting at w (the first pass reads it
-// where every tile it covers is flagged short)
+// where every tile it covers is flagged short);
+// fault_claim: adds 1 to row 0's claim (norm_check_1) before the verifier
+// replay reads it, as a prover sending a wrong claim would (needs L > 1024);
+// for tests of the verifier
 int launch_norm_check_ex(Context *ctx, const uint64_t *w, const unsigned long long *wrow,
                          size_t rows, size_t L, const uint64_t *cs, size_t cs_row, size_t reps,
                          int fq2, uint64_t nr, const NcOut &o, int cpt_first,
-                         const NcScratch *ext, const ShortCopy *sc = nullptr);
+                         const NcScratch *ext, const ShortCopy *sc = nullptr,
+                         int fault_claim = 0);
 int launch_norm_check(Context *ctx, const uint64_t *w, size_t rows, size_t L,
                       const uint64_t *cs, size_t reps, int fq2, uint64_t nr, uint64_t *claims,
                       uint64_t *msgs, uint64_t *finals, uint64_t *evals, uint64_t *rhs,

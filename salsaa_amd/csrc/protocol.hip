@@ -49,6 +49,14 @@ inline unsigned grid1(size_t n, unsigned bs) {
     return (unsigned)(g < 65536 ? (g ? g : 1) : 65536);
 }
 
+// the fault injection of salsaa_protocol_ex: one value off by one (mod q)
+__global__ void k_fault(uint64_t *p, uint64_t q) {
+    if (threadIdx.x == 0) {
+        const uint64_t v = *p + 1;
+        *p = v >= q ? v - q : v;
+    }
+}
+
 __global__ void k_status_ok(const int32_t *__restrict__ ok, size_t n, int32_t code,
                             int32_t *status) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -427,9 +435,10 @@ struct Run {
         return SALSAA_OK;
     }
 
-    // decomp_ell against the whole statement + verify_decomp (decomp.rs:32-112)
+    // decomp_ell against the whole statement + verify_decomp (decomp.rs:32-112);
+    // corrupt: the decomposed witness's commitment off by one (fault injection)
     int decomp(uint64_t **W, size_t *r, size_t m, size_t parts, uint64_t **V, int32_t code,
-               int32_t *status) {
+               int32_t *status, bool corrupt) {
         uint64_t *W2 = nullptr, *radix = nullptr, *Vd = nullptr;
         uint32_t *cnt = nullptr;
         const size_t rp = *r * parts;
@@ -439,6 +448,7 @@ struct Run {
         SALSAA_TRY(A->get(1, &cnt));
         SALSAA_TRY(salsaa_decompose(ctx, *W, *r, m, parts, 0, W2, radix));
         SALSAA_TRY(commit(W2, rp, m, Vd));
+        if (corrupt) hipLaunchKernelGGL(k_fault, dim3(1), dim3(64), 0, st(), Vd, c->q);
         SALSAA_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), st()));
         SALSAA_TRY(salsaa_verify_decomp(ctx, Vd, *V, R(), *r, parts, radix, 0, cnt));
         hipLaunchKernelGGL(k_status_count, dim3(1), dim3(64), 0, st(), cnt, code, status);
@@ -453,8 +463,15 @@ struct Run {
 int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t m,
                  const uint64_t *d_W, size_t r, size_t nrounds, const size_t *parts, int folding,
                  uint64_t seed, uint64_t *d_W_out, size_t w_cap, uint64_t *d_rhs_out,
-                 size_t rhs_cap, size_t *shape, int32_t *status_out, double *times) {
+                 size_t rhs_cap, size_t *shape, int32_t *status_out, double *times,
+                 const salsaa_protocol_opts *opts) {
     Context *c = &ctx->c;
+    SALSAA_CHECK(!opts || (opts->fault_check >= 0 && opts->fault_check <= 6),
+                 "protocol: fault_check must be 0..6");
+    // the fault injection point of check `check` in round `round`
+    auto fault = [&](size_t round, int check) {
+        return opts && opts->fault_check == check && (check == 6 || opts->fault_round == round);
+    };
     SALSAA_CHECK(d_module && d_W && status_out && shape, "protocol: null argument");
     SALSAA_CHECK(nrounds >= 1 && (parts || folding), "protocol: no rounds");
     SALSAA_CHECK(kappa >= 1 && kappa <= kMfmaMaxKappa, "protocol: kappa out of range");
@@ -518,7 +535,7 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
         const size_t p = parts ? parts[i] : 1;
         if (!folding && p > 1) {
             SALSAA_CHECK(pow2(p), "protocol: decomposition parts must be a power of two");
-            SALSAA_TRY(run.decomp(&W, &r, m, p, &V, code(i, 1), status));
+            SALSAA_TRY(run.decomp(&W, &r, m, p, &V, code(i, 1), status, fault(i, 1)));
             SALSAA_TRY(mark(PH_DECOMP));
         }
         // norm_check_1/2/3 with their verifiers, two repetitions (:198-259)
@@ -531,8 +548,10 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
             SALSAA_TRY(A.get(2 * r * 2 * kD, &rhs));
             SALSAA_TRY(A.get(2 * r, &ok));
             SALSAA_TRY(salsaa_sample_challenges(ctx, cs, 2, nr, 0, rseed));
-            SALSAA_TRY(salsaa_norm_check(ctx, W, r, L, cs, 2, 0, 0, nullptr, nullptr, nullptr,
-                                         nullptr, rhs, ok));
+            const NcOut no = nc_out_plain(r, 2, (size_t)nr, nullptr, nullptr, nullptr, nullptr, rhs,
+                                          ok);
+            SALSAA_TRY(launch_norm_check_ex(c, W, nullptr, r, L, cs, 0, 2, 0, 0, no, 0, nullptr,
+                                            nullptr, fault(i, 2) ? 1 : 0));
             hipLaunchKernelGGL(k_status_ok, dim3(1), dim3(64), 0, st, ok, 2 * r, code(i, 2),
                                status);
             SALSAA_HIP(hipGetLastError());
@@ -572,6 +591,7 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
             SALSAA_HIP(hipMemcpy2DAsync(W3 + r * h * kD, h * b, W + h * kD, m * b, h * b, r,
                                         hipMemcpyDeviceToDevice, st));
             SALSAA_TRY(run.commit(W3, 2 * r, h, S));
+            if (fault(i, 3)) hipLaunchKernelGGL(k_fault, dim3(1), dim3(64), 0, st, S, c->q);
             SALSAA_TRY(run.multipliers(m, T));
             hipLaunchKernelGGL(k_expand_mult, dim3(grid1(R * 2 * r * kD, 256)), dim3(256), 0, st,
                                T, R, r, Mx);
@@ -613,6 +633,7 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
             SALSAA_HIP(hipGetLastError());
             SALSAA_TRY(salsaa_project(ctx, W, r, m, z, sg, P));
             SALSAA_TRY(run.commit(P, 1, m, pcom));
+            if (fault(i, 4)) hipLaunchKernelGGL(k_fault, dim3(1), dim3(64), 0, st, P, c->q);
             // batching_challenge_for_project's first element is the constant 1
             SALSAA_TRY(salsaa_batch_projections(ctx, P, W, r, m, z, sg, run.one, plhs, wlhs, prhs,
                                                 wrhs, bok));
@@ -679,7 +700,7 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
         rounds_done = i + 1;
         if (folding) {
             // the folding scheme closes with decomp_ell(.., 2) + verify_decomp
-            SALSAA_TRY(run.decomp(&W, &r, m, 2, &V, code(i, 5), status));
+            SALSAA_TRY(run.decomp(&W, &r, m, 2, &V, code(i, 5), status, fault(i, 5)));
             SALSAA_TRY(mark(PH_DECOMP));
             break;
         }
@@ -689,6 +710,7 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
     if (!folding) {
         uint64_t *F = nullptr;
         SALSAA_TRY(A.get(R * r * kD, &F));
+        if (fault(rounds_done, 6)) hipLaunchKernelGGL(k_fault, dim3(1), dim3(64), 0, st, W, c->q);
         SALSAA_TRY(run.commit(W, r, m, F));
         hipLaunchKernelGGL(k_status_equal, dim3(grid1(R * r * kD, 256)), dim3(256), 0, st, F, V,
                            R * r * kD, code(rounds_done, 6), status);
@@ -729,5 +751,16 @@ extern "C" int salsaa_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t
                                size_t rhs_cap, size_t *shape, int32_t *status, double *times) {
     CTX_OK(ctx);
     return run_protocol(ctx, d_module, kappa, m, d_W, r, nrounds, parts, folding, seed, d_W_out,
-                        w_cap, d_rhs_out, rhs_cap, shape, status, times);
+                        w_cap, d_rhs_out, rhs_cap, shape, status, times, nullptr);
+}
+
+extern "C" int salsaa_protocol_ex(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa,
+                                  size_t m, const uint64_t *d_W, size_t r, size_t nrounds,
+                                  const size_t *parts, int folding, uint64_t seed,
+                                  uint64_t *d_W_out, size_t w_cap, uint64_t *d_rhs_out,
+                                  size_t rhs_cap, size_t *shape, int32_t *status, double *times,
+                                  const salsaa_protocol_opts *opts) {
+    CTX_OK(ctx);
+    return run_protocol(ctx, d_module, kappa, m, d_W, r, nrounds, parts, folding, seed, d_W_out,
+                        w_cap, d_rhs_out, rhs_cap, shape, status, times, opts);
 }

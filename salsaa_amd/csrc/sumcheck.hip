@@ -934,10 +934,15 @@ __global__ void k_zero_u64(unsigned long long *p, size_t n) {
     if (i < n) p[i] = 0ULL;
 }
 
+__global__ void k_add_one(unsigned long long *p) {
+    if (threadIdx.x == 0) *p += 1ULL;
+}
+
 template <bool FQ2, bool SNR>
 int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *wrow, size_t rows,
                     size_t L, const uint64_t *cs, size_t cs_row, size_t reps, double nr,
-                    const NcOut &o, int cpt_first, const NcScratch *ext, const ShortCopy *sc) {
+                    const NcOut &o, int cpt_first, const NcScratch *ext, const ShortCopy *sc,
+                    int fault_claim) {
     Mod M{ctx->qd, ctx->qinv};
     const int n = __builtin_ctzll(L);
     const int want_rhs = o.rhs != nullptr;
@@ -1022,6 +1027,8 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         }
         hipLaunchKernelGGL((k_sc_gram_msgs<FQ2, SNR>), dim3((unsigned)rows, (unsigned)reps), dim3(128),
                            0, ctx->stream, gacc, (int)rows, cs, cs_row, n, nr, macc, cacc, M);
+        if (fault_claim)
+            hipLaunchKernelGGL(k_add_one, dim3(1), dim3(64), 0, ctx->stream, cacc);
         SALSAA_HIP(hipGetLastError());
     }
     size_t len = len1;
@@ -1087,7 +1094,7 @@ NcOut nc_out_plain(size_t rows, size_t reps, size_t nrounds, uint64_t *claims, u
 int launch_norm_check_ex(Context *ctx, const uint64_t *w, const unsigned long long *wrow,
                          size_t rows, size_t L, const uint64_t *cs, size_t cs_row, size_t reps,
                          int fq2, uint64_t nr, const NcOut &o, int cpt_first,
-                         const NcScratch *ext, const ShortCopy *sc) {
+                         const NcScratch *ext, const ShortCopy *sc, int fault_claim) {
     SALSAA_CHECK(rows > 0 && reps > 0, "norm check needs rows > 0 and reps > 0");
     SALSAA_CHECK(sc == nullptr || (sc->t && sc->flag), "incomplete short copy");
     SALSAA_CHECK(L >= 2 && (L & (L - 1)) == 0, "norm check length must be a power of two >= 2");
@@ -1097,14 +1104,15 @@ int launch_norm_check_ex(Context *ctx, const uint64_t *w, const unsigned long lo
     SALSAA_CHECK(nr < ctx->q, "non-residue must be reduced mod q");
     SALSAA_CHECK(o.ok != nullptr, "ok output is required");
     SALSAA_CHECK(cpt_first >= 0 && cpt_first <= 64, "columns per thread out of range");
+    SALSAA_CHECK(!fault_claim || L > (size_t)SC_FINAL_MAX, "claim fault needs L > 1024");
     if (fq2 && nr < 8)
         return norm_check_impl<true, true>(ctx, w, wrow, rows, L, cs, cs_row, reps, (double)nr, o,
-                                           cpt_first, ext, sc);
+                                           cpt_first, ext, sc, fault_claim);
     if (fq2)
         return norm_check_impl<true, false>(ctx, w, wrow, rows, L, cs, cs_row, reps, (double)nr, o,
-                                            cpt_first, ext, sc);
+                                            cpt_first, ext, sc, fault_claim);
     return norm_check_impl<false, true>(ctx, w, wrow, rows, L, cs, cs_row, reps, (double)nr, o,
-                                        cpt_first, ext, sc);
+                                        cpt_first, ext, sc, fault_claim);
 }
 
 int launch_norm_check(Context *ctx, const uint64_t *w, size_t rows, size_t L,
