@@ -121,6 +121,12 @@ const char *salsaa_version(void);
 int salsaa_ctx_create(uint64_t q, uint32_t d, salsaa_ctx **out);
 int salsaa_ctx_destroy(salsaa_ctx *ctx);
 int salsaa_ctx_sync(salsaa_ctx *ctx);
+/* Waits for the context's stream, then frees the device memory the context
+ * keeps for reuse between calls: operator scratch and the protocol's run
+ * buffers (a protocol run keeps what it used for the next run).  Later calls
+ * allocate again as needed.  Resident keys, batches and servers are not
+ * touched. */
+int salsaa_ctx_trim(salsaa_ctx *ctx);
 /* The context's stream (hipStream_t) so callers can order their own work. */
 void *salsaa_ctx_stream(salsaa_ctx *ctx);
 

@@ -98,6 +98,7 @@ def _load():
         "salsaa_ctx_create": (_int, [_u64, ctypes.c_uint32, _PP]),
         "salsaa_ctx_destroy": (_int, [_vp]),
         "salsaa_ctx_sync": (_int, [_vp]),
+        "salsaa_ctx_trim": (_int, [_vp]),
         "salsaa_ctx_stream": (_vp, [_vp]),
         "salsaa_dev_alloc": (_int, [_vp, _sz, _PP]),
         "salsaa_dev_free": (_int, [_vp, _vp]),
@@ -301,6 +302,10 @@ class Context:
 
     def sync(self):
         _check(lib.salsaa_ctx_sync(self.h))
+
+    def trim(self):
+        """Free the device memory the context keeps for reuse (salsaa_ctx_trim)."""
+        _check(lib.salsaa_ctx_trim(self.h))
 
     # ---- buffers
     def empty(self, shape, dtype=np.uint64):

@@ -277,10 +277,24 @@ int salsaa_ctx_destroy(salsaa_ctx *ctx) {
     for (auto &b : ctx->c.scratch)
         if (b.ptr) (void)hipFree(b.ptr);
     if (ctx->c.zeros.ptr) (void)hipFree(ctx->c.zeros.ptr);
-    if (ctx->c.pool) (void)hipMemPoolDestroy(ctx->c.pool);
+    for (auto &b : ctx->c.slabs) (void)hipFree(b.ptr);
     (void)hipFree(ctx->c.d_tables);
     (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
+    return SALSAA_OK;
+}
+
+int salsaa_ctx_trim(salsaa_ctx *ctx) {
+    CTX_OK(ctx);
+    SALSAA_HIP(hipStreamSynchronize(ctx->c.stream));
+    for (auto &b : ctx->c.scratch) {
+        if (b.ptr) SALSAA_HIP(hipFree(b.ptr));
+        b = ScratchBuf();
+    }
+    if (ctx->c.zeros.ptr) SALSAA_HIP(hipFree(ctx->c.zeros.ptr));
+    ctx->c.zeros = ScratchBuf();
+    for (auto &b : ctx->c.slabs) SALSAA_HIP(hipFree(b.ptr));
+    ctx->c.slabs.clear();
     return SALSAA_OK;
 }
 

@@ -84,8 +84,7 @@ struct Context {
     bool mfma_commit = true;        // int8 MFMA commit (false: fp64 VALU commit)
     bool mfma_project = true;       // int8 MFMA projection (false: VALU signed sums)
     ScratchBuf zeros;               // zero-filled, grown on demand
-    hipMemPool_t pool = nullptr;    // stream-ordered pool of the protocol run's buffers (kept
-                                    // reserved between runs, released with the context)
+    std::vector<ScratchBuf> slabs;  // the protocol run's buffers (kept for the next run)
     std::recursive_mutex lock;      // held by every entry point on this context
 };
 

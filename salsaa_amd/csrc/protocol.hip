@@ -240,46 +240,50 @@ __global__ void k_build_mult(MultSrc a, int rows, uint64_t *__restrict__ T) {
     }
 }
 
-// Device buffers of one run, drawn from the context's stream-ordered pool and
-// returned to it at the end (the pool keeps the memory reserved, so a repeated
-// run neither maps nor unmaps device memory on the host's path).
+// Device buffers of one run: bump allocation from slabs the context keeps
+// between runs, so a run like the one before it calls no allocator at all.
+// When a run ends, the slabs it did not touch are freed: the context keeps
+// what the last run used, and salsaa_ctx_trim returns that too.
+constexpr size_t kSlabMin = (size_t)256 << 20;
+
 struct Arena {
-    hipStream_t s;
-    hipMemPool_t pool;
-    std::vector<void *> p;
-    Arena(hipStream_t st, hipMemPool_t pl) : s(st), pool(pl) {}
+    Context *c;
+    std::vector<size_t> used;  // bytes taken from each slab by this run
+    size_t cur = 0;            // the slab requests are taken from
+    explicit Arena(Context *ctx) : c(ctx), used(ctx->slabs.size(), 0) {}
     ~Arena() {
-        for (void *x : p) (void)hipFreeAsync(x, s);
-        (void)hipStreamSynchronize(s);
+        // the run's kernels may still be in flight on an error path
+        (void)hipStreamSynchronize(c->stream);
+        std::vector<ScratchBuf> keep;
+        for (size_t k = 0; k < c->slabs.size(); k++) {
+            if (used[k])
+                keep.push_back(c->slabs[k]);
+            else
+                (void)hipFree(c->slabs[k].ptr);
+        }
+        c->slabs.swap(keep);
     }
     template <class T>
     int get(size_t n, T **out) {
-        void *x = nullptr;
-        if (hipMallocFromPoolAsync(&x, (n ? n : 1) * sizeof(T), pool, s) != hipSuccess) {
-            (void)hipGetLastError();
-            set_error("protocol: out of device memory");
-            return SALSAA_ERR_NOMEM;
+        const size_t b = ((n ? n : 1) * sizeof(T) + 255) / 256 * 256;
+        std::vector<ScratchBuf> &S = c->slabs;
+        while (cur < S.size() && used[cur] + b > S[cur].bytes) cur++;
+        if (cur == S.size()) {
+            ScratchBuf nb;
+            nb.bytes = std::max(b, kSlabMin);
+            if (hipMalloc(&nb.ptr, nb.bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                set_error("protocol: out of device memory");
+                return SALSAA_ERR_NOMEM;
+            }
+            S.push_back(nb);
+            used.push_back(0);
         }
-        p.push_back(x);
-        *out = static_cast<T *>(x);
+        *out = reinterpret_cast<T *>(static_cast<char *>(S[cur].ptr) + used[cur]);
+        used[cur] += b;
         return SALSAA_OK;
     }
 };
-
-int pool_get(Context *c, hipMemPool_t *out) {
-    if (!c->pool) {
-        hipMemPoolProps props = {};
-        props.allocType = hipMemAllocationTypePinned;
-        props.handleTypes = hipMemHandleTypeNone;
-        props.location.type = hipMemLocationTypeDevice;
-        props.location.id = c->device;
-        SALSAA_HIP(hipMemPoolCreate(&c->pool, &props));
-        uint64_t keep = UINT64_MAX;
-        SALSAA_HIP(hipMemPoolSetAttribute(c->pool, hipMemPoolAttrReleaseThreshold, &keep));
-    }
-    *out = c->pool;
-    return SALSAA_OK;
-}
 
 // A statement row other than the CRS rows.
 struct Row {
@@ -436,13 +440,14 @@ struct Run {
     }
 
     // decomp_ell against the whole statement + verify_decomp (decomp.rs:32-112);
-    // corrupt: the decomposed witness's commitment off by one (fault injection)
+    // corrupt: the decomposed witness's commitment off by one (fault
+    // injection); dst: where the new witness goes (null: a run buffer)
     int decomp(uint64_t **W, size_t *r, size_t m, size_t parts, uint64_t **V, int32_t code,
-               int32_t *status, bool corrupt) {
-        uint64_t *W2 = nullptr, *radix = nullptr, *Vd = nullptr;
+               int32_t *status, bool corrupt, uint64_t *dst = nullptr) {
+        uint64_t *W2 = dst, *radix = nullptr, *Vd = nullptr;
         uint32_t *cnt = nullptr;
         const size_t rp = *r * parts;
-        SALSAA_TRY(A->get(rp * m * kD, &W2));
+        if (!W2) SALSAA_TRY(A->get(rp * m * kD, &W2));
         SALSAA_TRY(A->get(1, &radix));
         SALSAA_TRY(A->get(R() * rp * kD, &Vd));
         SALSAA_TRY(A->get(1, &cnt));
@@ -479,9 +484,7 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
     const hipStream_t st = c->stream;
     const size_t b = kD * sizeof(uint64_t);
     const Mod M{c->qd, c->qinv};
-    hipMemPool_t pool = nullptr;
-    SALSAA_TRY(pool_get(c, &pool));
-    Arena A(st, pool);
+    Arena A(c);
     Run run{ctx, c, nullptr, d_module, kappa, {}, nullptr, &A};
     SALSAA_TRY(salsaa_key_crs_implicit(ctx, d_module, kappa, m, &run.key));
     struct KeyGuard {
@@ -614,12 +617,21 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
                      "protocol: the projection needs 256 r <= m (project.rs panics otherwise)");
         uint64_t *P = nullptr, *pcom = nullptr, *plhs = nullptr, *wlhs = nullptr, *prhs = nullptr,
                  *wrhs = nullptr;
+        // the joined witness [fold output; projection] (witness.append,
+        // protocol.rs:402-406) is one buffer: the projection goes to its second
+        // row and the fold to its first, so the join copies nothing; in the
+        // SNARK's last round it is the caller's output
+        uint64_t *J = nullptr;
+        if (!folding && i + 1 == nrounds && d_W_out && w_cap >= 2 * m * kD)
+            J = d_W_out;
+        else
+            SALSAA_TRY(A.get(2 * m * kD, &J));
+        P = J + m * kD;
         {
             uint8_t *z = nullptr, *sg = nullptr;
             int32_t *bok = nullptr;
             SALSAA_TRY(A.get(kHeight * width, &z));
             SALSAA_TRY(A.get(kHeight * width, &sg));
-            SALSAA_TRY(A.get(m * kD, &P));
             SALSAA_TRY(A.get(run.R() * kD, &pcom));
             SALSAA_TRY(A.get(m * kD, &plhs));
             SALSAA_TRY(A.get(m * kD, &wlhs));
@@ -649,12 +661,11 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
             SALSAA_TRY(mark(PH_PROJECT));
         }
         // fold + verifier_fold with WIT_REP = 1 (:349-386)
-        uint64_t *W4 = nullptr, *Vf = nullptr;
+        uint64_t *W4 = J, *Vf = nullptr;
         {
             const size_t R = run.R();
             uint64_t *C = nullptr, *Vc = nullptr, *Vt = nullptr;
             SALSAA_TRY(A.get(r * kD, &C));
-            SALSAA_TRY(A.get(m * kD, &W4));
             SALSAA_TRY(A.get(R * r * kD, &Vc));
             SALSAA_TRY(A.get(R * r * kD, &Vt));
             SALSAA_TRY(A.get(R * kD, &Vf));
@@ -674,11 +685,10 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
         // send_cross_terms_before_join + verifier_join, witness.append (:387-410)
         {
             const size_t R = run.R();
-            uint64_t *l0 = nullptr, *l1 = nullptr, *Vj = nullptr, *Wj = nullptr;
+            uint64_t *l0 = nullptr, *l1 = nullptr, *Vj = nullptr;
             SALSAA_TRY(A.get(kD, &l0));
             SALSAA_TRY(A.get(kD, &l1));
             SALSAA_TRY(A.get((R + 1) * 2 * kD, &Vj));
-            SALSAA_TRY(A.get(2 * m * kD, &Wj));
             SALSAA_TRY(salsaa_send_cross_terms_before_join(ctx, plhs, wlhs, m, W4, P, l0, l1));
             // rows e < R - 1: [rhs_e, commitment of the projection]; row R - 1
             // (batching): [rhs, l1]; the new row: [l0, projection_rhs]
@@ -689,18 +699,17 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
                                       st));
             SALSAA_HIP(hipMemcpyAsync(Vj + R * 2 * kD, l0, b, hipMemcpyDeviceToDevice, st));
             SALSAA_HIP(hipMemcpyAsync(Vj + (R * 2 + 1) * kD, prhs, b, hipMemcpyDeviceToDevice, st));
-            SALSAA_HIP(hipMemcpyAsync(Wj, W4, m * b, hipMemcpyDeviceToDevice, st));
-            SALSAA_HIP(hipMemcpyAsync(Wj + m * kD, P, m * b, hipMemcpyDeviceToDevice, st));
             run.rows.push_back(Row{-1, plhs, kHeight, nullptr, (size_t)lg2(m / kHeight)});
             V = Vj;
-            W = Wj;
+            W = J;
             r = 2;
             SALSAA_TRY(mark(PH_JOIN));
         }
         rounds_done = i + 1;
         if (folding) {
             // the folding scheme closes with decomp_ell(.., 2) + verify_decomp
-            SALSAA_TRY(run.decomp(&W, &r, m, 2, &V, code(i, 5), status, fault(i, 5)));
+            uint64_t *dst = (d_W_out && w_cap >= 2 * r * m * kD) ? d_W_out : nullptr;
+            SALSAA_TRY(run.decomp(&W, &r, m, 2, &V, code(i, 5), status, fault(i, 5), dst));
             SALSAA_TRY(mark(PH_DECOMP));
             break;
         }
@@ -721,7 +730,7 @@ int run_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t
     shape[1] = m;
     shape[2] = R;
     shape[3] = r;
-    if (d_W_out) {
+    if (d_W_out && W != d_W_out) {
         SALSAA_CHECK(w_cap >= r * m * kD, "protocol: witness output too small");
         SALSAA_HIP(hipMemcpyAsync(d_W_out, W, r * m * b, hipMemcpyDeviceToDevice, st));
     }

@@ -174,3 +174,19 @@ def test_many_statement_rows(sa, ctx, orc):
     assert np.array_equal(ctx.parallel_dot_series_matrix(key, Wf).download(), rhs[:kappa])
     key.close()
     assert np.array_equal(ones_row(sa, ctx, Wf), rhs[-1])
+
+
+def test_outputs_reused_and_memory_trimmed(sa, ctx, orc):
+    # a run into the outputs of the one before it (the joined witness is
+    # written in place), after the context's cached memory was given back,
+    # equals the first run
+    say("case: protocol outputs reused, context trimmed")
+    for folding, r, m, parts in ((False, 1, 1 << 12, (2,)), (True, 4, 1 << 11, None)):
+        module, W = setup(sa, ctx, orc, 5, r, m, seed=330)
+        a = ctx.protocol(module, W, 1, parts, folding=folding, seed=4)
+        Wa, Va = a["W"].download(), a["rhs"].download()
+        ctx.trim()
+        b = ctx.protocol(module, W, 1, parts, folding=folding, seed=4, outs=a)
+        assert b["W"] is a["W"] and b["status"] == a["status"] == 0
+        assert np.array_equal(b["W"].download(), Wa)
+        assert np.array_equal(b["rhs"].download(), Va)
