@@ -174,87 +174,169 @@ class Batched:
         return same and ok
 
 
-def cpu_baseline_batched(m, kappa, budget_s=20.0):
-    """The oracle (one thread) on a sample of the batched workload's
-    instances, extrapolated to all of them: NTT and norm check scale with the
-    witness length, the commit (per-product conversion, as the reference) with
-    kappa * length."""
+def host_cpu():
+    """The host's processor, its hardware threads and the threads the
+    baseline uses (the process's CPU share, capped by OMP_NUM_THREADS when the
+    environment sets one)."""
+    model = ""
+    flags = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and not model:
+                    model = line.split(":", 1)[1].strip()
+                if line.startswith("flags") and not flags:
+                    flags = line
+    except OSError:
+        pass
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    env = os.environ.get("OMP_NUM_THREADS")
+    use = min(share, int(env)) if env and env.isdigit() and int(env) > 0 else share
+    return {"model": model, "nproc": os.cpu_count(), "cpu_share": share, "threads": use,
+            "avx512ifma": "avx512ifma" in flags}
+
+
+def _median_runs(fn, runs):
+    fn()  # warm-up
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline_batched(bw, cls=1 << 14, runs=5):
+    """The CPU baseline (oracle/cpu_baseline.cpp: the reference's CPU prover
+    path, per-product conversion in the commit, the 10-pass chunked sumcheck,
+    AVX-512 IFMA lanes, all threads of the host's share) on one whole length
+    class of the batched workload: every instance with m = cls, its own
+    witness, challenges and key prefix, 1 warm-up and the median of `runs`
+    timed runs per phase.  The batch's time is the class's per phase, scaled
+    by the batch's work over the class's (NTT and norm check by the sum of m,
+    the commit by the sum of kappa * m); the class's outputs are compared with
+    the device's."""
+    from oracle import cpu_baseline as cb
     from oracle import oracle as orc
     q = 1125899906840833
-    sample = [(8192, 8), (8192, 16), (8192, 32)]
-    t_ntt = t_com = t_nc = 0.0
-    for j, (mm, kk) in enumerate(sample):
-        w = orc.sample_bounded(1000 + j, 0xB, (1, mm, 64), BOUND, q)
-        A = orc.sample_uniform(0, 0xA, (kk, mm, 64), q)
-        cs = orc.sample_uniform(5000 + j, 0x5C000, (REPS, (mm * 64).bit_length() - 1, 2), q)
-        t0 = time.perf_counter()
-        orc.from_incomplete(orc.to_incomplete(w[0], q), q)
-        t1 = time.perf_counter()
-        orc.commit(A, w, q, threads=1)
-        t2 = time.perf_counter()
-        orc.sumcheck(w.reshape(1, mm * 64), cs, q, fq2=True, nr=5, threads=1)
-        t3 = time.perf_counter()
-        t_ntt += (t1 - t0) / mm
-        t_com += (t2 - t1) / (mm * kk)
-        t_nc += (t3 - t2) / mm
-    k = len(sample)
-    sum_m = float(sum(m))
-    sum_km = float(sum(a * b for a, b in zip(m, kappa)))
-    total = (t_ntt / k) * sum_m + (t_com / k) * sum_km + (t_nc / k) * sum_m
+    host = host_cpu()
+    threads = cb.init(host["threads"], True)
+    m, kappa = bw.m, bw.kappa
+    idx = [i for i in range(len(m)) if m[i] == cls]
+    A = np.stack([orc.sample_uniform_at(0, 0xA, j * A_M * 64, (cls, 64), q) for j in range(A_ROWS)])
+    key = cb.Key(A, q)
+    del A
+    n = (cls * 64).bit_length() - 1
+    Ws = [orc.sample_bounded(1000 + i, 0xB, (1, cls, 64), BOUND, q) for i in idx]
+    css = [np.stack([orc.sample_uniform(5000 + i, 0x5C000 + rep, (2 * n,), q).reshape(n, 2)
+                     for rep in range(REPS)]) for i in idx]
+    res = {}
+
+    def ntt():
+        for W in Ws:
+            cb.ntt_roundtrip(W, q)
+
+    def com():
+        res["com"] = [cb.commit(key, W, kappa[i]) for i, W in zip(idx, Ws)]
+
+    def nc():
+        res["nc"] = [cb.norm_check(W.reshape(1, -1), cs, q, 5) for W, cs in zip(Ws, css)]
+
+    t = {}
+    spread = {}
+    for name, fn in (("ntt", ntt), ("commit", com), ("norm_check", nc)):
+        t[name], spread[name] = _median_runs(fn, runs)
+    key.close()
+    # the class's outputs against the device's batched step
+    exact = True
+    for k, i in enumerate(idx):
+        sl = bw.b.slot(i)
+        exact &= np.array_equal(res["com"][k], sl["commitment"].download()[: kappa[i]])
+        o = res["nc"][k]
+        for name in ("claims", "msgs", "finals", "evals", "rhs"):
+            exact &= np.array_equal(o[name], sl[name].download())
+    sum_m, sum_km = float(sum(m)), float(sum(a * b for a, b in zip(m, kappa)))
+    c_m = float(cls * len(idx))
+    c_km = float(sum(kappa[i] * cls for i in idx))
+    total = (t["ntt"] + t["norm_check"]) * sum_m / c_m + t["commit"] * sum_km / c_km
+    class_s = sum(t.values())
     return {
         "value": len(m) / total,
         "unit": "prover-steps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": ("oracle, 1 thread: NTT fwd+inv, commit (per-product conversion) and F_{q^2} "
-                   "norm check of three m=8192 instances (kappa 8, 16, 32), per-element costs "
-                   "extrapolated to the 512 instances (sum m = %d, sum kappa*m = %d): %.0f s "
-                   "for the whole batch" % (sum_m, sum_km, total)),
+        "variant": cb.variant(),
+        "host": host,
+        "class": {"m": cls, "instances": len(idx), "median_s": round(class_s, 4),
+                  "prover_steps_per_s": len(idx) / class_s,
+                  "phases_median_s": {k: round(v, 4) for k, v in t.items()},
+                  "phases_runs_s": {k: [round(x, 4) for x in v] for k, v in spread.items()},
+                  "bit_exact_vs_device": bool(exact)},
+        "sample": ("oracle/cpu_baseline.cpp (the reference's CPU path: per-product NTT in the "
+                   "commit, 10-pass sumcheck in 120-way chunks, norm_check_3 by power series; "
+                   "AVX-512 IFMA lanes) on %d threads: the whole m = %d class (%d instances), "
+                   "median of %d runs after a warm-up, %.2f s; scaled to the batch by each "
+                   "phase's work (sum m %d / %d, sum kappa m %d / %d): %.1f s for the 512 "
+                   "instances" % (threads, cls, len(idx), runs, class_s, sum_m, c_m, sum_km,
+                                  c_km, total)),
     }
 
 
-def cpu_baseline(budget_s=20.0):
-    """The oracle (single thread) on a bounded sample of the same step, scaled
-    to one full step: each phase runs on a fraction of its work and the time is
-    multiplied by the inverse fraction."""
+def cpu_baseline(step, runs=5):
+    """The CPU baseline on the single-instance step at full size, the same
+    inputs as the device's: NTT forward + inverse of 2^20 ring elements,
+    commit 32 x 16384 . 8 x 16384, the F_{q^2} norm check of 8 x 2^20 with
+    norm_check_3, decomp_ell base 2^8 x 6 with verify_decomp; 1 warm-up, then
+    the median of `runs` runs per phase; outputs compared with the device's."""
+    from oracle import cpu_baseline as cb
     from oracle import oracle as orc
     q = 1125899906840833
-    t = {}
-    # NTT: 65536 of the 2^20 elements, forward + inverse
-    nn = 65536
-    x = orc.sample_uniform(0, 0xC, (nn, 64), q)
-    t0 = time.perf_counter()
-    y = orc.to_incomplete(x, q)
-    orc.from_incomplete(y, q)
-    t["ntt"] = (time.perf_counter() - t0) * (NTT_COUNT / nn)
-    # commit: full kappa x r, 64 of the 16384 columns (per-product conversion)
-    ms = 128
-    A = orc.sample_uniform(0, 0xA, (KAPPA, ms, 64), q)
-    W = orc.sample_bounded(0, 0xB, (R, ms, 64), BOUND, q)
-    t0 = time.perf_counter()
-    orc.commit(A, W, q, threads=1)
-    t["commit"] = (time.perf_counter() - t0) * (M / ms)
-    # norm check: one of the 8 rows at full length, 2 repetitions, F_{q^2}
-    w = orc.sample_bounded(0, 0xB, (1, M * 64), BOUND, q)
-    cs = orc.sample_uniform(0, 0x5C000, (REPS, 20, 2), q)
-    t0 = time.perf_counter()
-    orc.sumcheck(w, cs, q, fq2=True, nr=5, threads=1)
-    t["norm_check"] = (time.perf_counter() - t0) * R
-    # decomp_ell: decomposition of the commit sample and the limb commitment
-    t0 = time.perf_counter()
-    limbs, radix = orc.decompose(W, PARTS, q, RADIX)
-    orc.commit(A, limbs, q, threads=1)
-    t["decomp_ell"] = (time.perf_counter() - t0) * (M / ms)
+    host = host_cpu()
+    threads = cb.init(host["threads"], True)
+    X = orc.sample_uniform(0, 0xC, (NTT_COUNT, 64), q)
+    A = orc.sample_uniform(0, 0xA, (KAPPA, M, 64), q)
+    key = cb.Key(A, q)
+    del A
+    W = orc.sample_bounded(0, 0xB, (R, M, 64), BOUND, q)
+    n = (M * 64).bit_length() - 1
+    cs = np.stack([orc.sample_uniform(0, 0x5C000 + rep, (2 * n,), q).reshape(n, 2)
+                   for rep in range(REPS)])
+    res = {}
+
+    def ntt():
+        cb.ntt_roundtrip(X, q)
+
+    def com():
+        res["com"] = cb.commit(key, W)
+
+    def nc():
+        res["nc"] = cb.norm_check(W.reshape(R, -1), cs, q, 5)
+
+    def dec():
+        limbs, radix = cb.decompose(W, PARTS, q, RADIX)
+        res["dec"] = cb.commit(key, limbs)
+        res["bad"] = cb.verify_decomp(res["dec"], res["com"], PARTS, radix, q)
+
+    t, spread = {}, {}
+    for name, fn in (("ntt", ntt), ("commit", com), ("norm_check", nc), ("decomp_ell", dec)):
+        t[name], spread[name] = _median_runs(fn, runs)
+    key.close()
+    exact = (np.array_equal(res["com"], step.commitment.download()) and res["bad"] == 0
+             and np.array_equal(res["dec"], step.rhs_dec.download()))
+    for name in ("claims", "msgs", "finals", "evals", "rhs"):
+        exact &= np.array_equal(res["nc"][name], step.nc[name].download())
     total = sum(t.values())
     return {
         "value": 1.0 / total,
         "unit": "prover-steps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": ("oracle, 1 thread: NTT on 65536 of 2^20 elements; commit and decomp_ell "
-                   "with all 32x8 (32x48) rows on 128 of 16384 columns; norm check on 1 of 8 "
-                   "rows; each phase scaled to the full step (%s)" %
-                   ", ".join("%s %.1fs" % kv for kv in t.items())),
+        "variant": cb.variant(),
+        "phases_median_s": {k: round(v, 4) for k, v in t.items()},
+        "phases_runs_s": {k: [round(x, 4) for x in v] for k, v in spread.items()},
+        "bit_exact_vs_device": bool(exact),
+        "sample": ("oracle/cpu_baseline.cpp on %d threads (%s): the whole config-2 step, "
+                   "median of %d runs after a warm-up" % (threads, cb.variant(), runs)),
     }
 
 
@@ -613,7 +695,11 @@ def timed(ctx, fn, k):
     return per, ctx.elapsed_ms(evs[0], evs[-1]), wall
 
 
-def single_instance(sa, ctx, steps):
+def progress(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def single_instance(sa, ctx, steps, cpu=False):
     """The full-size single-instance plain step (per-phase and whole)."""
     step = Step(sa, ctx)
     for _ in range(3):
@@ -640,6 +726,9 @@ def single_instance(sa, ctx, steps):
            "p50_ms": round(float(np.percentile(per, 50)), 4),
            "p99_ms": round(float(np.percentile(per, 99)), 4),
            "bit_exact_self_check": step.check(), "phases": phases}
+    if cpu:
+        progress("CPU baseline, config-2 step")
+        out["cpu_baseline"] = cpu_baseline(step)
     step.key.close()
     return out
 
@@ -716,6 +805,12 @@ def main():
     # the plain per-call path on the same instances
     pper, ptotal_ms, _ = timed(ctx, b.step_plain, args.plain_steps)
     exact = bw.bit_exact()
+    progress("batched step %.2f ms, plain %.2f ms" % (total_ms / args.steps,
+                                                      ptotal_ms / args.plain_steps))
+    cpu_batched = None
+    if not args.no_cpu_baseline:
+        progress("CPU baseline, one length class of the batch")
+        cpu_batched = cpu_baseline_batched(bw)
 
     sum_m = sum(bw.m)
     # fused NTT round trip + planes + short copy: the arena read and written
@@ -780,22 +875,30 @@ def main():
     }
     b.close()
     bw.key.close()
+    if cpu_batched is not None:
+        out["cpu_baseline"] = cpu_batched
     if not args.no_served:
+        progress("served stream")
         out["served"] = served(sa, ctx)
     if not args.no_folding:
+        progress("folding step")
         out["folding"] = folding(sa, ctx, args.steps)
     if not args.no_projection:
+        progress("projection")
         out["projection"] = projection(sa, ctx, args.steps,
                                        0.0 if args.no_cpu_baseline else 8.0)
     if not args.no_crs:
+        progress("CRS key")
         out["crs"] = crs(sa, ctx, args.steps)
     if not args.no_protocol:
+        progress("protocol runs")
         out["protocol"] = protocol(sa, ctx)
     if not args.no_single:
-        out["single_instance"] = single_instance(sa, ctx, args.steps)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_batched(bw.m, bw.kappa)
-        out["cpu_baseline_single_instance"] = cpu_baseline()
+        progress("single instance")
+        out["single_instance"] = single_instance(sa, ctx, args.steps,
+                                                 cpu=not args.no_cpu_baseline)
+        if "cpu_baseline" in out["single_instance"]:
+            out["cpu_baseline_single_instance"] = out["single_instance"].pop("cpu_baseline")
     print(json.dumps(out), flush=True)
     if not exact:
         sys.exit(1)
