@@ -56,7 +56,11 @@ void eltwise_mult_mod(uint64_t *result, const uint64_t *operand1, const uint64_t
 const uint64_t *get_roots(uint64_t n, uint64_t modulus);
 /* hexl_wrapper.cpp:52-54, bindings.rs:20 */
 uint64_t inv_mod(uint64_t a, uint64_t modulus);
-/* hexl_wrapper.cpp:56-59, bindings.rs:21 */
+/* hexl_wrapper.cpp:56-59, bindings.rs:21: the inverse roots psi^-brv(k) in
+ * this library's own order (inv[0], then inv[m .. 2m) for m = n/2, n/4, ..,
+ * 1, the order its inverse transform reads them).  The reference declares
+ * this symbol but never calls it and no reference test reads the table, so
+ * whether the layout equals HEXL's is not established. */
 const uint64_t *get_inv_roots(uint64_t n, uint64_t modulus);
 /* hexl_wrapper.cpp:262-268, bindings.rs:23 */
 uint64_t sum_sq(const uint64_t *operand1, uint64_t n, uint64_t modulus);

@@ -89,7 +89,8 @@ struct NttTable {
     std::vector<uint64_t> roots_shoup;  // floor(roots * 2^64 / q)
     std::vector<uint64_t> inv;          // inv[k] = roots[k]^{-1}
     std::vector<uint64_t> inv_shoup;
-    std::vector<uint64_t> inv_hexl;     // HEXL's traversal-ordered inverse table
+    std::vector<uint64_t> inv_hexl;     // inv[0], then inv[m .. 2m) for m = n/2, n/4, .., 1:
+                                        // the order the inverse transform's loops read them
     uint64_t n_inv, n_inv_shoup;
 };
 
