@@ -451,6 +451,69 @@ int salsaa_fold_step(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const uin
                      size_t r_out, const uint64_t *d_cs, size_t reps, int fq2, uint64_t nr,
                      const salsaa_fold_outputs *out);
 
+/* ------------------------------------------------------------------ transcripts
+ * The proof transcript.  The reference has none: it logs its values
+ * (protocol.rs:474-482), and its verifier draws each norm-check challenge at
+ * random, independent of the round's message (fast_norm_check.rs:375-385).
+ *
+ * Byte format (little-endian): a 96-byte header, then canonical u64 sections.
+ *   0 "SLSATRN1"  8 u32 kind  12 u32 flags  16 u64 q  24 u64 nr  32 u64 L
+ *  40 u32 rows  44 u32 reps  48 u32 nrounds  52 u32 d  56 u32 kappa  60 u32 0
+ *  64 32-byte seed
+ *   kind 1, norm check: claims[rows], msgs[reps][nrounds][rows][3][2],
+ *     challenges[reps][nrounds][2], finals[reps][rows][2], evals[reps][rows][2],
+ *     rhs[reps][rows][2][d] when flags bit 2
+ *   kind 2, prover step: commitment[kappa][rows][d], then kind 1's sections
+ *   kind 3, protocol (rows = final witness rows, reps = rounds, nrounds =
+ *     statement rows R, L = final witness length m, seed = the run's u64
+ *     seed): verifier rhs[R][rows][d] (IncompleteNTT), final witness
+ *     [rows][m][d]
+ * flags: bit 0 Fiat-Shamir challenges, bit 1 F_{q^2} (nr is 0 otherwise),
+ * bit 2 rhs present, bit 3 folding scheme (kind 3), bits 16-31 the protocol
+ * status (kind 3, as int16).
+ * Each transcript function writes into the device buffer d_out of cap bytes
+ * (stream-ordered, from device outputs) and sets *len to the transcript's
+ * size; with d_out NULL it only sets *len. */
+
+/* The norm check (norm_check_1/2/3 as salsaa_norm_check) with Fiat-Shamir
+ * challenges.  All rows share each round's challenge, from a SHA-256 chain:
+ *   h_0 = SHA-256("SALSAA-NC-FS-v1" || seed[32] || q (u64) || L (u64) ||
+ *                 rows (u32) || rep (u32) || claims[rows] (u64))
+ *   h_{t+1} = SHA-256(h_t || round t's messages of every row in row order,
+ *                     each f0.a f0.b f1.a f1.b f2.a f2.b (u64))
+ *   c_t = (mulhi64(u64 of h_{t+1}[0..8], q), fq2 ? mulhi64(u64 of
+ *         h_{t+1}[8..16], q) : 0)
+ * seed (host, 32 bytes, NULL = zeros) binds the statement, e.g. a digest of
+ * the commitment.  d_cs [reps][log2 L][2] receives the challenges; the other
+ * outputs are salsaa_norm_check's (all required but d_claims and d_rhs).
+ * f2 is summed, not derived, so each round identity is checked.  L >= d. */
+int salsaa_norm_check_fs(salsaa_ctx *ctx, const uint64_t *d_w, size_t rows, size_t L,
+                         size_t reps, int fq2, uint64_t nr, const uint8_t *seed, uint64_t *d_cs,
+                         uint64_t *d_claims, uint64_t *d_msgs, uint64_t *d_finals,
+                         uint64_t *d_evals, uint64_t *d_rhs, int32_t *d_ok);
+/* kind 1 from a norm check's outputs (fs: the challenges came from
+ * salsaa_norm_check_fs with this seed); d_rhs may be NULL */
+int salsaa_transcript_norm_check(salsaa_ctx *ctx, size_t rows, size_t L, size_t reps, int fq2,
+                                 uint64_t nr, int fs, const uint8_t *seed,
+                                 const uint64_t *d_claims, const uint64_t *d_msgs,
+                                 const uint64_t *d_cs, const uint64_t *d_finals,
+                                 const uint64_t *d_evals, const uint64_t *d_rhs, uint8_t *d_out,
+                                 size_t cap, size_t *len);
+/* kind 2: a commitment [kappa][rows][d] and the norm check of the same rows
+ * (for one batch instance: its slot's commitment, kappa rows, rows = 1) */
+int salsaa_transcript_prover_step(salsaa_ctx *ctx, size_t kappa, const uint64_t *d_commitment,
+                                  size_t rows, size_t L, size_t reps, int fq2, uint64_t nr,
+                                  int fs, const uint8_t *seed, const uint64_t *d_claims,
+                                  const uint64_t *d_msgs, const uint64_t *d_cs,
+                                  const uint64_t *d_finals, const uint64_t *d_evals,
+                                  const uint64_t *d_rhs, uint8_t *d_out, size_t cap, size_t *len);
+/* kind 3 from a salsaa_protocol run: its shape[4] output, status, verifier
+ * rhs and final witness */
+int salsaa_transcript_protocol(salsaa_ctx *ctx, size_t kappa, size_t nrounds, int folding,
+                               uint64_t seed, int32_t status, const size_t *shape,
+                               const uint64_t *d_rhs, const uint64_t *d_W, uint8_t *d_out,
+                               size_t cap, size_t *len);
+
 /* ------------------------------------------------------------------ protocol
  * protocol (protocol.rs:125-483) on the device: the CRS of the module d_module
  * [kappa][d] (an implicit key, crs.rs:23-72), the commitment of the witness

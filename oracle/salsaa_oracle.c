@@ -616,6 +616,85 @@ void or_sumcheck(const uint64_t *w, size_t rows, size_t L, const uint64_t *cs,
     }
 }
 
+/* The sumcheck with every row in step (the reference's rows share each
+ * round's challenge, fast_norm_check.rs:352-399): per repetition and round,
+ * the messages f0, f1, f2 of every row (all three summed directly), then
+ * chal(user, rep, t, msgs of the round [rows][6], c[2]) supplies the round's
+ * challenge (the Fiat-Shamir replay derives it from the messages), which
+ * goes to cs_out [reps][nrounds][2]; then every row folds.  ok is the
+ * verifier's: each round identity and the final eval^2 == claim. */
+void or_sumcheck_cb(const uint64_t *w, size_t rows, size_t L, size_t reps, int fq2, uint64_t nr,
+                    uint64_t q, uint64_t *claims, uint64_t *msgs, uint64_t *cs_out,
+                    uint64_t *finals, uint64_t *evals, int *ok, or_chal_fn chal, void *user) {
+    size_t nrounds = ilog2(L);
+    fq2_t *cur = (fq2_t *)malloc(rows * L * sizeof(fq2_t));
+    fq2_t *cl = (fq2_t *)malloc(rows * sizeof(fq2_t));
+    for (size_t row = 0; row < rows; row++) {
+        uint64_t s = 0;
+        for (size_t x = 0; x < L; x++) s = or_addmod(s, or_mulmod(w[row * L + x], w[row * L + x], q), q);
+        claims[row] = s;
+    }
+    for (size_t rep = 0; rep < reps; rep++) {
+        for (size_t row = 0; row < rows; row++) {
+            for (size_t x = 0; x < L; x++) {
+                cur[row * L + x].a = w[row * L + x];
+                cur[row * L + x].b = 0;
+            }
+            cl[row].a = claims[row];
+            cl[row].b = 0;
+            ok[rep * rows + row] = 1;
+        }
+        size_t len = L;
+        for (size_t t = 0; t < nrounds; t++) {
+            size_t half = len / 2;
+            uint64_t *mo = msgs + ((rep * nrounds + t) * rows) * 6;
+            for (size_t row = 0; row < rows; row++) {
+                fq2_t *c = cur + row * L;
+                fq2_t f0 = {0, 0}, f1 = {0, 0}, f2 = {0, 0};
+                for (size_t x = 0; x < half; x++) {
+                    fq2_t w0 = c[x], dl = f2_sub(c[x + half], c[x], q);
+                    f0 = f2_add(f0, f2_mul(w0, w0, nr, q), q);
+                    f1 = f2_add(f1, f2_mul(w0, dl, nr, q), q);
+                    f2 = f2_add(f2, f2_mul(dl, dl, nr, q), q);
+                }
+                mo[row * 6 + 0] = f0.a; mo[row * 6 + 1] = f0.b;
+                mo[row * 6 + 2] = f1.a; mo[row * 6 + 3] = f1.b;
+                mo[row * 6 + 4] = f2.a; mo[row * 6 + 5] = f2.b;
+                fq2_t lhs = f2_add(f2_add(f2_add(f0, f0, q), f2_add(f1, f1, q), q), f2, q);
+                if (!f2_eq(lhs, cl[row])) ok[rep * rows + row] = 0;
+            }
+            uint64_t cc[2] = {0, 0};
+            chal(user, rep, t, mo, cc);
+            fq2_t ch = {cc[0] % q, fq2 ? cc[1] % q : 0};
+            cs_out[(rep * nrounds + t) * 2] = ch.a;
+            cs_out[(rep * nrounds + t) * 2 + 1] = ch.b;
+            fq2_t c2 = f2_mul(ch, ch, nr, q);
+            for (size_t row = 0; row < rows; row++) {
+                fq2_t f0 = {mo[row * 6], mo[row * 6 + 1]}, f1 = {mo[row * 6 + 2], mo[row * 6 + 3]},
+                      f2 = {mo[row * 6 + 4], mo[row * 6 + 5]};
+                cl[row] = f2_add(f2_add(f0, f2_mul(f2_add(f1, f1, q), ch, nr, q), q),
+                                 f2_mul(f2, c2, nr, q), q);
+                fq2_t *c = cur + row * L;
+                for (size_t x = 0; x < half; x++) {
+                    fq2_t dl = f2_sub(c[x + half], c[x], q);
+                    c[x] = f2_add(f2_mul(dl, ch, nr, q), c[x], q);
+                }
+            }
+            len = half;
+        }
+        for (size_t row = 0; row < rows; row++) {
+            fq2_t ev = cur[row * L];
+            if (!f2_eq(f2_mul(ev, ev, nr, q), cl[row])) ok[rep * rows + row] = 0;
+            finals[(rep * rows + row) * 2] = cl[row].a;
+            finals[(rep * rows + row) * 2 + 1] = cl[row].b;
+            evals[(rep * rows + row) * 2] = ev.a;
+            evals[(rep * rows + row) * 2 + 1] = ev.b;
+        }
+    }
+    free(cur);
+    free(cl);
+}
+
 /* fast_norm_check.rs:401-441: cs = [ring challenges (log2 m) | coefficient
  * challenges (log2 d)]; base = conj(tensor(coef cs)) in IncompleteNTT;
  * ps_i = tensor(ring cs)_i * base (complete_power_series, arithmetic.rs:158-176,

@@ -132,6 +132,16 @@ void or_sumcheck(const uint64_t *w, size_t rows, size_t L, const uint64_t *cs,
                  uint64_t *msgs, uint64_t *finals, uint64_t *evals, int *ok,
                  int threads);
 
+/* The sumcheck with all rows in step and the challenge of each round from a
+ * callback given that round's messages [rows][6] (the Fiat-Shamir replay);
+ * cs_out [reps][nrounds][2] receives the challenges, the other outputs are
+ * or_sumcheck's (f2 summed directly). */
+typedef void (*or_chal_fn)(void *user, size_t rep, size_t t, const uint64_t *round_msgs,
+                           uint64_t *c_out);
+void or_sumcheck_cb(const uint64_t *w, size_t rows, size_t L, size_t reps, int fq2, uint64_t nr,
+                    uint64_t q, uint64_t *claims, uint64_t *msgs, uint64_t *cs_out,
+                    uint64_t *finals, uint64_t *evals, int *ok, or_chal_fn chal, void *user);
+
 /* norm_check_3 (fast_norm_check.rs:401-441) for F_q challenges:
  * rhs[row] = sum_i ps_i * W[row][i], IncompleteNTT form.  cs has log2(m*d)
  * entries (F_q). */

@@ -152,6 +152,16 @@ def _load():
                                        _sz, _vp, _sz, _PSZ, ctypes.POINTER(ctypes.c_int32),
                                        ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ProtocolOpts)]),
+        "salsaa_norm_check_fs": (_int, [_vp, _vp, _sz, _sz, _sz, _int, _u64, ctypes.c_char_p, _vp,
+                                        _vp, _vp, _vp, _vp, _vp, _vp]),
+        "salsaa_transcript_norm_check": (_int, [_vp, _sz, _sz, _sz, _int, _u64, _int,
+                                                ctypes.c_char_p, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                _vp, _sz, _PSZ]),
+        "salsaa_transcript_prover_step": (_int, [_vp, _sz, _vp, _sz, _sz, _sz, _int, _u64, _int,
+                                                 ctypes.c_char_p, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                 _vp, _sz, _PSZ]),
+        "salsaa_transcript_protocol": (_int, [_vp, _sz, _sz, _int, _u64, ctypes.c_int32, _PSZ,
+                                              _vp, _vp, _vp, _sz, _PSZ]),
         "salsaa_server_create": (_int, [_vp, _vp, _sz, ctypes.POINTER(ServerOpts), _PP]),
         "salsaa_server_submit": (_int, [_vp, _sz, _vp, _u64, _u64, ctypes.POINTER(ctypes.c_uint64)]),
         "salsaa_server_poll": (_int, [_vp, _u64, _vp, ctypes.POINTER(ctypes.c_double),
@@ -497,6 +507,63 @@ class Context:
                                              outs["finals"].ptr, outs["evals"].ptr, rhs,
                                              outs["ok"].ptr))
         return outs
+
+    def norm_check_fs(self, w, reps, fq2=True, nr=NR, seed=bytes(32), want_rhs=True):
+        """The norm check with Fiat-Shamir challenges (salsaa_norm_check_fs):
+        norm_check's outputs plus cs [reps][nrounds][2], the derived
+        challenges."""
+        rows = w.shape[0]
+        L = int(np.prod(w.shape[1:]))
+        n = L.bit_length() - 1
+        seed = bytes(seed)
+        assert len(seed) == 32
+        o = dict(cs=self.empty((reps, n, 2)), claims=self.empty((rows,)),
+                 msgs=self.empty((reps, n, rows, 3, 2)), finals=self.empty((reps, rows, 2)),
+                 evals=self.empty((reps, rows, 2)),
+                 rhs=self.empty((reps, rows, 2, self.d)) if want_rhs else None,
+                 ok=self.empty((reps, rows), np.int32))
+        _check(lib.salsaa_norm_check_fs(self.h, w.ptr, rows, L, reps, int(fq2), nr, seed,
+                                        o["cs"].ptr, o["claims"].ptr, o["msgs"].ptr,
+                                        o["finals"].ptr, o["evals"].ptr,
+                                        o["rhs"].ptr if want_rhs else None, o["ok"].ptr))
+        return o
+
+    def _transcript(self, call, *args):
+        n = ctypes.c_size_t()
+        _check(call(self.h, *args, None, 0, ctypes.byref(n)))
+        buf = self.empty((max(1, n.value),), np.uint8)
+        _check(call(self.h, *args, buf.ptr, n.value, ctypes.byref(n)))
+        out = buf.download()[: n.value].tobytes()
+        buf.free()
+        return out
+
+    def transcript_norm_check(self, o, L, cs, fq2=True, nr=NR, fs=False, seed=None):
+        """The kind-1 transcript bytes of a norm check's outputs o (claims,
+        msgs, finals, evals, rhs or None) with its challenges cs."""
+        reps, n, rows = o["msgs"].shape[:3]
+        rhs = o.get("rhs")
+        return self._transcript(lib.salsaa_transcript_norm_check, rows, L, reps, int(fq2), nr,
+                                int(fs), bytes(seed) if seed else None, o["claims"].ptr,
+                                o["msgs"].ptr, cs.ptr, o["finals"].ptr, o["evals"].ptr,
+                                rhs.ptr if rhs is not None else None)
+
+    def transcript_prover_step(self, commitment, kappa, o, L, cs, fq2=True, nr=NR, fs=False,
+                               seed=None):
+        """kind 2: commitment (its first kappa rows) then the norm check."""
+        reps, n, rows = o["msgs"].shape[:3]
+        rhs = o.get("rhs")
+        return self._transcript(lib.salsaa_transcript_prover_step, kappa, commitment.ptr, rows, L,
+                                reps, int(fq2), nr, int(fs), bytes(seed) if seed else None,
+                                o["claims"].ptr, o["msgs"].ptr, cs.ptr, o["finals"].ptr,
+                                o["evals"].ptr, rhs.ptr if rhs is not None else None)
+
+    def transcript_protocol(self, out, kappa, nrounds, folding, seed):
+        """kind 3 from protocol()'s result."""
+        r, m, _ = out["W"].shape
+        R = out["rhs"].shape[0]
+        shape = (ctypes.c_size_t * 4)(r, m, R, r)
+        return self._transcript(lib.salsaa_transcript_protocol, kappa, nrounds, int(folding), seed,
+                                out["status"], shape, out["rhs"].ptr, out["W"].ptr)
 
     def decompose(self, W, parts, radix=0, out=None, d_radix=None):
         r, m, d = W.shape
