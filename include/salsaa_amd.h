@@ -418,6 +418,14 @@ int salsaa_batch_phase(salsaa_batch *b, int phase);
  * costs of the chosen commit and norm checks in ms. */
 int salsaa_batch_tuning(const salsaa_batch *b, size_t *kchunk, int *cpt_first, float *commit_ms,
                         float *norm_check_ms);
+/* The NTT pass's schedule the tuner chose (xch 0: the exchange between the
+ * register phases of each 32-point half through LDS; 1: through DPP quad
+ * permutes, the wavefront-shuffle variant) and its times of the whole pass
+ * with each (milliseconds; 0 when the batch was made without tuning). */
+int salsaa_batch_tuning_ntt(const salsaa_batch *b, int *xch, float *lds_ms, float *dpp_ms);
+/* Sets the NTT pass's schedule (0 LDS, 1 DPP); the step is captured again on
+ * its next graph replay.  Waits for the context's stream. */
+int salsaa_batch_set_ntt_schedule(salsaa_batch *b, int xch);
 
 /* ------------------------------------------------------------------ folding
  * The folding step of n accumulated instances (the protocol's folding round,

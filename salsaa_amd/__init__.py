@@ -172,6 +172,10 @@ def _load():
         "salsaa_batch_tuning": (_int, [_vp, _PSZ, ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_float),
                                        ctypes.POINTER(ctypes.c_float)]),
+        "salsaa_batch_tuning_ntt": (_int, [_vp, ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_float),
+                                           ctypes.POINTER(ctypes.c_float)]),
+        "salsaa_batch_set_ntt_schedule": (_int, [_vp, _int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -755,8 +759,16 @@ class Batch:
         tc, tn = ctypes.c_float(), ctypes.c_float()
         _check(lib.salsaa_batch_tuning(self.h, ctypes.byref(kc), ctypes.byref(cpt),
                                        ctypes.byref(tc), ctypes.byref(tn)))
+        x, tl, td = ctypes.c_int(), ctypes.c_float(), ctypes.c_float()
+        _check(lib.salsaa_batch_tuning_ntt(self.h, ctypes.byref(x), ctypes.byref(tl),
+                                           ctypes.byref(td)))
         return dict(kchunk=kc.value, cpt_first=cpt.value, commit_ms=tc.value,
-                    norm_check_ms=tn.value)
+                    norm_check_ms=tn.value, ntt_schedule=("lds", "dpp")[x.value],
+                    ntt_pass_ms={"lds": tl.value, "dpp": td.value})
+
+    def set_ntt_schedule(self, name):
+        """The NTT pass's exchange: "lds" or "dpp" (quad permutes)."""
+        _check(lib.salsaa_batch_set_ntt_schedule(self.h, {"lds": 0, "dpp": 1}[name]))
 
     def close(self):
         if self.h:

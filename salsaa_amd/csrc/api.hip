@@ -248,6 +248,8 @@ int salsaa_ctx_create(uint64_t q, uint32_t d, salsaa_ctx **out) {
     x.mfma_commit = !(path && std::string(path) == "valu");
     const char *ppath = getenv("SALSAA_PROJECT");
     x.mfma_project = !(ppath && std::string(ppath) == "valu");
+    const char *xpath = getenv("SALSAA_NTT_XCH");
+    x.ntt_xch = (xpath && std::string(xpath) == "dpp") ? 1 : 0;
     build_tables(q, x.h_tables);
     if (hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&x.d_tables, TAB_SIZE * sizeof(double)) != hipSuccess ||

@@ -138,6 +138,10 @@ struct salsaa_batch {
     int split_mode = 0;
     int cpt_first = 0;
     float t_commit = 0.f, t_nc = 0.f;
+    // the NTT pass's exchange between register phases (0 LDS, 1 DPP quad
+    // permutes) and the tuner's times of the whole pass with each
+    int ntt_xch = 0;
+    float t_ntt[2] = {0.f, 0.f};
     // plain path
     std::vector<salsaa_key *> views;
     // graph
@@ -310,7 +314,7 @@ int run_norm_checks(salsaa_batch *b, int cpt, const ShortCopy *sc) {
 
 int run_fwd_planes(salsaa_batch *b, int roundtrip) {
     return launch_ntt_fwd_planes(&b->ctx->c, b->d_wit, b->tot_w, b->d_planes, b->d_ptiles,
-                                 roundtrip, roundtrip ? &b->sc : nullptr);
+                                 roundtrip, roundtrip ? &b->sc : nullptr, b->ntt_xch);
 }
 
 // the round trip over arena tiles [t0, t1) (the pass's tables and short copy
@@ -318,7 +322,8 @@ int run_fwd_planes(salsaa_batch *b, int roundtrip) {
 int run_rt_range(salsaa_batch *b, size_t t0, size_t t1) {
     ShortCopy sc{b->sc.t + t0 * kShortTile, b->sc.flag + t0};
     return launch_ntt_fwd_planes(&b->ctx->c, b->d_wit + t0 * kShortTile,
-                                 (t1 - t0) * kPlaneTile, b->d_planes, b->d_ptiles + t0, 1, &sc);
+                                 (t1 - t0) * kPlaneTile, b->d_planes, b->d_ptiles + t0, 1, &sc,
+                                 b->ntt_xch);
 }
 
 // one pass: forward NTT, the commit's digit planes, the inverse NTT and the
@@ -392,6 +397,14 @@ int autotune(salsaa_batch *b) {
     }
     SALSAA_TRY(build_commit_tables(b, best_kc));
     b->t_commit = best;
+    // the NTT pass's radix-exchange schedule: the whole round trip (the
+    // witnesses come back unchanged) with the exchange through LDS and
+    // through DPP quad permutes
+    for (int x = 0; x < 2; x++) {
+        b->ntt_xch = x;
+        SALSAA_TRY(time_ms(b, [&] { return run_fwd_planes(b, 1); }, 2, &b->t_ntt[x]));
+    }
+    b->ntt_xch = b->t_ntt[1] < b->t_ntt[0] ? 1 : 0;
     // the norm checks as the step runs them: after a round trip (of the
     // still empty, short witnesses), from the short copy
     SALSAA_TRY(run_fwd_planes(b, 1));
@@ -803,6 +816,28 @@ int salsaa_batch_tuning(const salsaa_batch *b, size_t *kchunk, int *cpt_first, f
     if (cpt_first) *cpt_first = b->cpt_first;
     if (commit_ms) *commit_ms = b->t_commit;
     if (norm_check_ms) *norm_check_ms = b->t_nc;
+    return SALSAA_OK;
+}
+
+int salsaa_batch_tuning_ntt(const salsaa_batch *b, int *xch, float *lds_ms, float *dpp_ms) {
+    SALSAA_CHECK(b != nullptr, "null batch");
+    if (xch) *xch = b->ntt_xch;
+    if (lds_ms) *lds_ms = b->t_ntt[0];
+    if (dpp_ms) *dpp_ms = b->t_ntt[1];
+    return SALSAA_OK;
+}
+
+int salsaa_batch_set_ntt_schedule(salsaa_batch *b, int xch) {
+    SALSAA_CHECK(b != nullptr, "null batch");
+    SALSAA_CHECK(xch == 0 || xch == 1, "NTT schedule must be 0 (LDS) or 1 (DPP)");
+    CTX_OK(b->ctx);
+    // the captured step launches the other schedule: capture again on next use
+    SALSAA_HIP(hipStreamSynchronize(b->ctx->c.stream));
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->ntt_xch = xch;
     return SALSAA_OK;
 }
 

@@ -83,6 +83,8 @@ struct Context {
     ScratchBuf scratch[12];         // grown on demand, owned by the context
     bool mfma_commit = true;        // int8 MFMA commit (false: fp64 VALU commit)
     bool mfma_project = true;       // int8 MFMA projection (false: VALU signed sums)
+    int ntt_xch = 0;                // IncompleteNTT exchange between register phases:
+                                    // 0 through LDS, 1 DPP quad permutes
     ScratchBuf zeros;               // zero-filled, grown on demand
     std::vector<ScratchBuf> slabs;  // the protocol run's buffers (kept for the next run)
     std::recursive_mutex lock;      // held by every entry point on this context
@@ -100,6 +102,7 @@ inline void share_context(const Context &from, Context &to) {
     to.d_tables = from.d_tables;
     to.mfma_commit = from.mfma_commit;
     to.mfma_project = from.mfma_project;
+    to.ntt_xch = from.ntt_xch;
 }
 
 struct Key {
@@ -353,7 +356,8 @@ __device__ __forceinline__ uint32_t short_bad(uint64_t t) {
 // form again (the forward + inverse NTT pair of a prover step); with `sc`
 // the pass also writes the arena's short copy.
 int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *planes,
-                          const PlaneTile *tiles, int roundtrip, const ShortCopy *sc = nullptr);
+                          const PlaneTile *tiles, int roundtrip, const ShortCopy *sc = nullptr,
+                          int xch = 0);
 int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
                        size_t a_ktiles, const int8_t *WB, size_t r, size_t m, size_t ktW,
                        uint64_t *out);

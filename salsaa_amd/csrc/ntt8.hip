@@ -196,7 +196,10 @@ __device__ __forceinline__ ulonglong2 tile_get1(const uint64_t *lds, int c) {
     return v;
 }
 
-template <bool INV>
+// DPP: the exchange between the register phases through quad permutes
+// (xch_a_to_b / xch_b_to_a) instead of LDS, which saves one barrier and 16
+// LDS accesses per thread and tile for 64 VALU moves.
+template <bool INV, bool DPP>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_ntt_inc(
     uint64_t *__restrict__ data, size_t count, const double *__restrict__ tab, Mod M) {
     __shared__ uint64_t lds[EL * SROW];
@@ -241,17 +244,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 #pragma unroll
                 for (int k = 0; k < 8; k++) v[k] = u2d(row[8 * k + 2 * t + p]);  // coeff 2(t+4k)+p
                 ntt32_fwd_a_lazy(v, twl, M);
+                if (DPP) {
+                    xch_a_to_b(v, t);
+                    ntt32_fwd_b_lazy(v, twl, t, M);
+                }
             }
             __syncthreads();
-            if (live) {
+            if (live && !DPP) {
 #pragma unroll
                 for (int k = 0; k < 8; k++) row[36 * p + skew(t + 4 * k)] = dbits(v[k]);
             }
-            __syncthreads();
+            if (!DPP) __syncthreads();
             if (live) {
+                if (!DPP) {
 #pragma unroll
-                for (int x = 0; x < 8; x++) v[x] = bitsd(row[36 * p + skew(8 * t + x)]);
-                ntt32_fwd_b_lazy(v, twl, t, M);
+                    for (int x = 0; x < 8; x++) v[x] = bitsd(row[36 * p + skew(8 * t + x)]);
+                    ntt32_fwd_b_lazy(v, twl, t, M);
+                }
                 // output entry 32 p + 8 t + x, skewed: same slots this thread read
 #pragma unroll
                 for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = d2u(v[x]);
@@ -261,14 +270,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 #pragma unroll
                 for (int x = 0; x < 8; x++) v[x] = u2d(row[36 * p + skew(8 * t + x)]);
                 ntt32_inv_b_lazy(v, twl, t, M);
+                if (DPP) {
+                    xch_b_to_a(v, t);
+                    ntt32_inv_a_lazy(v, twl, twl[32], M);
+                } else {
 #pragma unroll
-                for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
+                    for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
+                }
             }
-            __syncthreads();
-            if (live) {
+            if (!DPP) {
+                __syncthreads();
+                if (live) {
 #pragma unroll
-                for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
-                ntt32_inv_a_lazy(v, twl, twl[32], M);
+                    for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
+                    ntt32_inv_a_lazy(v, twl, twl[32], M);
+                }
             }
             __syncthreads();
             if (live) {
@@ -420,7 +436,9 @@ __global__ __launch_bounds__(8 * TEL) __attribute__((amdgpu_waves_per_eu(4, 4)))
 static_assert(kPlaneTile == EL, "the round trip works on 128-element tiles");
 constexpr int WEL = 8;                // ring elements per wave
 constexpr int WROWS = WEL * SROW;     // u64 of a wave's rows
-template <bool SHORT>
+// DPP: both exchanges between register phases (forward a -> b, inverse
+// b -> a) through quad permutes instead of the wave's LDS rows.
+template <bool SHORT, bool DPP>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ntt_rt_planes(
     uint64_t *__restrict__ data, size_t ntiles, const double *__restrict__ tab, Mod M,
     int8_t *__restrict__ planes, const PlaneTile *__restrict__ ptiles, uint16_t *__restrict__ sht,
@@ -492,13 +510,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
 #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = u2d(row[8 * k + 2 * t + p]);  // coeff 2(t+4k)+p
         ntt32_fwd_a_lazy(v, twl, M);
-        wave_sync();
+        if (DPP) {
+            xch_a_to_b(v, t);
+        } else {
+            wave_sync();
 #pragma unroll
-        for (int k = 0; k < 8; k++) row[36 * p + skew(t + 4 * k)] = dbits(v[k]);
-        wave_sync();
+            for (int k = 0; k < 8; k++) row[36 * p + skew(t + 4 * k)] = dbits(v[k]);
+            wave_sync();
 #pragma unroll
-        for (int x = 0; x < 8; x++) v[x] = bitsd(row[36 * p + skew(8 * t + x)]);
+            for (int x = 0; x < 8; x++) v[x] = bitsd(row[36 * p + skew(8 * t + x)]);
+        }
         ntt32_fwd_b_lazy(v, twl, t, M);
+        if (DPP) wave_sync();  // every lane has read its coefficients
 #pragma unroll
         for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = d2u(v[x]);
         wave_sync();
@@ -524,11 +547,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         for (int x = 0; x < 8; x++) work[wpos_w(8 * h + x, e)] = dbits(a[x]);
         __syncthreads();
         store_planes_one<EL, true>(planes + dsc.base, dsc.sstride, dsc.kstride, work, EL, tid);
+        if (DPP) {
+            xch_b_to_a(v, t);
+        } else {
 #pragma unroll
-        for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
-        wave_sync();
+            for (int x = 0; x < 8; x++) row[36 * p + skew(8 * t + x)] = dbits(v[x]);
+            wave_sync();
 #pragma unroll
-        for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
+            for (int k = 0; k < 8; k++) v[k] = bitsd(row[36 * p + skew(t + 4 * k)]);
+        }
         ntt32_inv_a_lazy(v, twi, twi[32], M);
         wave_sync();
 #pragma unroll
@@ -951,13 +978,13 @@ __global__ __launch_bounds__(64) void k_dot_finalize(const unsigned long long *_
 }  // namespace
 
 // resident blocks per CU of a kernel (occupancy API, cached)
-template <bool INV>
+template <bool INV, bool DPP>
 int ntt_inc_occupancy() {
     static int occ = 0;
     if (occ == 0) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &n, reinterpret_cast<const void *>(&k_ntt_inc<INV>), NT, 0) != hipSuccess ||
+                &n, reinterpret_cast<const void *>(&k_ntt_inc<INV, DPP>), NT, 0) != hipSuccess ||
             n < 1)
             n = 1;
         occ = n;
@@ -965,21 +992,26 @@ int ntt_inc_occupancy() {
     return occ;
 }
 
+template <bool INV, bool DPP>
+void run_ntt_inc(Context *ctx, uint64_t *data, size_t count, unsigned grid, Mod M) {
+    // persistent: every resident block slot once
+    const size_t slots = (size_t)ctx->ncu * ntt_inc_occupancy<INV, DPP>();
+    const unsigned g = (unsigned)std::min<size_t>(grid, slots);
+    hipLaunchKernelGGL((k_ntt_inc<INV, DPP>), dim3(g), dim3(NT), 0, ctx->stream, data, count,
+                       ctx->d_tables, M);
+}
+
 int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse) {
     if (count == 0) return SALSAA_OK;
     Mod M{ctx->qd, ctx->qinv};
     const unsigned grid = ceil_div(count, EL);
     if (mode == SALSAA_REP_INCOMPLETE) {
-        // persistent: every resident block slot once
-        const size_t slots = (size_t)ctx->ncu * (inverse ? ntt_inc_occupancy<true>()
-                                                         : ntt_inc_occupancy<false>());
-        const unsigned g = (unsigned)std::min<size_t>(grid, slots);
         if (inverse)
-            hipLaunchKernelGGL(k_ntt_inc<true>, dim3(g), dim3(NT), 0, ctx->stream, data, count,
-                               ctx->d_tables, M);
+            ctx->ntt_xch ? run_ntt_inc<true, true>(ctx, data, count, grid, M)
+                         : run_ntt_inc<true, false>(ctx, data, count, grid, M);
         else
-            hipLaunchKernelGGL(k_ntt_inc<false>, dim3(g), dim3(NT), 0, ctx->stream, data, count,
-                               ctx->d_tables, M);
+            ctx->ntt_xch ? run_ntt_inc<false, true>(ctx, data, count, grid, M)
+                         : run_ntt_inc<false, false>(ctx, data, count, grid, M);
     } else {
         if (inverse)
             hipLaunchKernelGGL(k_ntt_full<true>, dim3(grid), dim3(NT), 0, ctx->stream, data, count,
@@ -993,7 +1025,7 @@ int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse
 }
 
 int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *planes,
-                          const PlaneTile *tiles, int roundtrip, const ShortCopy *sc) {
+                          const PlaneTile *tiles, int roundtrip, const ShortCopy *sc, int xch) {
     if (count == 0) return SALSAA_OK;
     SALSAA_CHECK(count % kPlaneTile == 0, "fused planes need whole tiles");
     SALSAA_CHECK(sc == nullptr || (roundtrip && sc->t && sc->flag),
@@ -1001,12 +1033,15 @@ int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *pl
     Mod M{ctx->qd, ctx->qinv};
     const size_t nt = count / kPlaneTile;
     static int occ[2] = {0, 0};
-    static int occ_short = 0;
+    static int occ_rt[2][2] = {{0, 0}, {0, 0}};  // [short][xch]
+    const bool dpp = roundtrip && xch;
     const void *fn = !roundtrip ? reinterpret_cast<const void *>(&k_ntt_fwd_planes<kPlaneTile>)
-                     : sc       ? reinterpret_cast<const void *>(&k_ntt_rt_planes<true>)
-                                : reinterpret_cast<const void *>(&k_ntt_rt_planes<false>);
+                     : sc ? (dpp ? reinterpret_cast<const void *>(&k_ntt_rt_planes<true, true>)
+                                 : reinterpret_cast<const void *>(&k_ntt_rt_planes<true, false>))
+                          : (dpp ? reinterpret_cast<const void *>(&k_ntt_rt_planes<false, true>)
+                                 : reinterpret_cast<const void *>(&k_ntt_rt_planes<false, false>));
     const int threads = roundtrip ? NT : 8 * kPlaneTile;
-    int &o = sc ? occ_short : occ[roundtrip ? 1 : 0];
+    int &o = roundtrip ? occ_rt[sc ? 1 : 0][dpp ? 1 : 0] : occ[0];
     if (o == 0) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, 0) != hipSuccess || n < 1)
@@ -1014,12 +1049,18 @@ int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *pl
         o = n;
     }
     const unsigned g = (unsigned)std::min<size_t>(nt, (size_t)ctx->ncu * o);
-    if (roundtrip && sc)
-        hipLaunchKernelGGL(k_ntt_rt_planes<true>, dim3(g), dim3(NT), 0, ctx->stream, data, nt,
-                           ctx->d_tables, M, planes, tiles, sc->t, sc->flag);
+    if (roundtrip && sc && dpp)
+        hipLaunchKernelGGL((k_ntt_rt_planes<true, true>), dim3(g), dim3(NT), 0, ctx->stream, data,
+                           nt, ctx->d_tables, M, planes, tiles, sc->t, sc->flag);
+    else if (roundtrip && sc)
+        hipLaunchKernelGGL((k_ntt_rt_planes<true, false>), dim3(g), dim3(NT), 0, ctx->stream, data,
+                           nt, ctx->d_tables, M, planes, tiles, sc->t, sc->flag);
+    else if (roundtrip && dpp)
+        hipLaunchKernelGGL((k_ntt_rt_planes<false, true>), dim3(g), dim3(NT), 0, ctx->stream, data,
+                           nt, ctx->d_tables, M, planes, tiles, nullptr, nullptr);
     else if (roundtrip)
-        hipLaunchKernelGGL(k_ntt_rt_planes<false>, dim3(g), dim3(NT), 0, ctx->stream, data, nt,
-                           ctx->d_tables, M, planes, tiles, nullptr, nullptr);
+        hipLaunchKernelGGL((k_ntt_rt_planes<false, false>), dim3(g), dim3(NT), 0, ctx->stream,
+                           data, nt, ctx->d_tables, M, planes, tiles, nullptr, nullptr);
     else
         hipLaunchKernelGGL((k_ntt_fwd_planes<kPlaneTile>), dim3(g), dim3(8 * kPlaneTile), 0,
                            ctx->stream, data, nt, ctx->d_tables, M, planes, tiles);

@@ -34,6 +34,65 @@ __device__ __forceinline__ void bf_inv(double &X, double &Y, double w, Mod M) {
     Y = mulmod(submod(x, y, M), w, M);
 }
 
+// ---------------------------------------------------------------- quad exchange
+// The exchange between phase A (thread t of a 32-point half holds positions
+// t + 4k, k < 8) and phase B (positions 8t + x) without LDS.  The 4 threads
+// of a half are the 4 lanes of a quad, and the exchange is a 4 x 4 transpose
+// of value pairs across them: A -> B sends the pair (v[2j], v[2j + 1]) of
+// thread t to thread j as its entries t and t + 4.  Two butterfly stages
+// (lanes t ^ 1, then t ^ 2) of DPP quad permutes move the pairs in registers.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+// pairs (a[j], b[j]), j < 4: lane t ends with lane j's pair t in slot j
+__device__ __forceinline__ void quad_transpose(double (&a)[4], double (&b)[4], int t) {
+    constexpr int X1 = 0xB1, X2 = 0x4E;  // quad_perm [1, 0, 3, 2] and [2, 3, 0, 1]
+    double ca[4], cb[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        ca[j] = dpp_d<X1>(a[j ^ 1]);
+        cb[j] = dpp_d<X1>(b[j ^ 1]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const bool take = ((j ^ t) & 1) != 0;
+        a[j] = take ? ca[j] : a[j];
+        b[j] = take ? cb[j] : b[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        ca[j] = dpp_d<X2>(a[j ^ 2]);
+        cb[j] = dpp_d<X2>(b[j ^ 2]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const bool take = ((j ^ t) & 2) != 0;
+        a[j] = take ? ca[j] : a[j];
+        b[j] = take ? cb[j] : b[j];
+    }
+}
+// phase A layout -> phase B layout, and back
+__device__ __forceinline__ void xch_a_to_b(double (&v)[8], int t) {
+    double a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) a[j] = v[2 * j], b[j] = v[2 * j + 1];
+    quad_transpose(a, b, t);
+#pragma unroll
+    for (int s = 0; s < 4; s++) v[s] = a[s], v[s + 4] = b[s];
+}
+__device__ __forceinline__ void xch_b_to_a(double (&v)[8], int t) {
+    double a[4], b[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) a[s] = v[s], b[s] = v[s + 4];
+    quad_transpose(a, b, t);
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[2 * j] = a[j], v[2 * j + 1] = b[j];
+}
+
 // ---------------------------------------------------------------- n = 32
 // Thread t (0..3) of a half; phase A positions t + 4k (k < 8), phase B 8t + x.
 __device__ __forceinline__ void ntt32_fwd_a(double (&v)[8], const double *__restrict__ tw, Mod M) {
