@@ -695,6 +695,44 @@ def timed(ctx, fn, k):
     return per, ctx.elapsed_ms(evs[0], evs[-1]), wall
 
 
+def ntt_schedules(sa, steps):
+    """Config 2's NTT (forward + inverse IncompleteNTT of 2^20 ring elements)
+    under each radix-exchange schedule: between the register phases of each
+    32-point half through LDS, or through DPP quad permutes.  A context per
+    schedule (SALSAA_NTT_XCH at creation); HBM fraction at 1024 B per element
+    per direction."""
+    res = {}
+    old = os.environ.get("SALSAA_NTT_XCH")
+    for name in ("lds", "dpp"):
+        os.environ["SALSAA_NTT_XCH"] = name
+        c = sa.Context(sa.Q, 64)
+        x = c.uniform((NTT_COUNT, 64), 0, sa.STREAM_NTT)
+        evs = [c.event() for _ in range(3)]
+        fw, iv = [], []
+        for i in range(steps + 3):
+            c.record(evs[0])
+            c.convert(x, sa.REP_COEFF, sa.REP_INCOMPLETE)
+            c.record(evs[1])
+            c.convert(x, sa.REP_INCOMPLETE, sa.REP_COEFF)
+            c.record(evs[2])
+            c.sync()
+            if i >= 3:
+                fw.append(c.elapsed_ms(evs[0], evs[1]))
+                iv.append(c.elapsed_ms(evs[1], evs[2]))
+        by = NTT_COUNT * 1024
+        f, v = float(np.median(fw)), float(np.median(iv))
+        res[name] = {"fwd_ms": round(f, 4), "inv_ms": round(v, 4),
+                     "fwd_hbm_frac": round(by / (f * 1e-3) / 8e12, 4),
+                     "inv_hbm_frac": round(by / (v * 1e-3) / 8e12, 4)}
+        x.free()
+        c.close()
+    if old is None:
+        os.environ.pop("SALSAA_NTT_XCH", None)
+    else:
+        os.environ["SALSAA_NTT_XCH"] = old
+    return res
+
+
 def progress(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
@@ -894,6 +932,8 @@ def main():
         progress("protocol runs")
         out["protocol"] = protocol(sa, ctx)
     if not args.no_single:
+        progress("NTT schedules")
+        out["ntt_schedules"] = ntt_schedules(sa, args.steps)
         progress("single instance")
         out["single_instance"] = single_instance(sa, ctx, args.steps,
                                                  cpu=not args.no_cpu_baseline)
