@@ -220,6 +220,34 @@ __device__ __forceinline__ double reduce(double x, Mod M) {
 __device__ __forceinline__ double reduce_lazy(double x, Mod M) {
     return __fma_rn(-__builtin_rint(x * M.qinv), M.q, x);
 }
+// A double moved across lanes by a DPP control (every lane active; quad_perm
+// 0xB1 = lane ^ 1, 0x4E = lane ^ 2, 0x141 half-row mirror, 0x140 row mirror).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), lane);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+// The sum mod q of one residue in [0, q) per lane over the whole wave (every
+// lane active), canonical and the same in every lane: DPP moves within each
+// 16-lane row (lane ^ 1, lane ^ 2, the other half-row, the other 8 lanes),
+// then the four row sums by readlane.  A partial sum of 8 residues is below
+// 2^53, so the double adds are exact between the two reductions.
+__device__ __forceinline__ double wave_sum_res(double x, Mod M) {
+    x += dpp_d<0xB1>(x);
+    x += dpp_d<0x4E>(x);
+    x += dpp_d<0x141>(x);
+    x = reduce_lazy(x, M);  // |x| <= q/2 + 1
+    x += dpp_d<0x140>(x);   // |x| <= q + 2
+    return reduce(readlane_d(x, 0) + readlane_d(x, 16) + readlane_d(x, 32) + readlane_d(x, 48), M);
+}
 // orders one wave's LDS writes before its later reads (and reads before
 // later writes) without waiting for the rest of the block
 __device__ __forceinline__ void wave_sync() {

@@ -41,13 +41,6 @@ __device__ __forceinline__ void bf_inv(double &X, double &Y, double w, Mod M) {
 // of value pairs across them: A -> B sends the pair (v[2j], v[2j + 1]) of
 // thread t to thread j as its entries t and t + 4.  Two butterfly stages
 // (lanes t ^ 1, then t ^ 2) of DPP quad permutes move the pairs in registers.
-template <int CTRL>
-__device__ __forceinline__ double dpp_d(double v) {
-    const uint64_t b = (uint64_t)__double_as_longlong(v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, true);
-    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
-}
 // pairs (a[j], b[j]), j < 4: lane t ends with lane j's pair t in slot j
 __device__ __forceinline__ void quad_transpose(double (&a)[4], double (&b)[4], int t) {
     constexpr int X1 = 0xB1, X2 = 0x4E;  // quad_perm [1, 0, 3, 2] and [2, 3, 0, 1]

@@ -13,8 +13,9 @@
 // its messages from the rows' Gram matrices on the matrix cores (see the
 // Gram pass below); the later passes add per-block message partials into u64
 // accumulators with integer atomics (exact, order independent).  Once the
-// length is at most 1024 a single block per (repetition, row) runs the
-// remaining rounds in LDS, captures the length-d folded vector V for
+// length is at most 4096 a single block per (repetition, row) evaluates the
+// Gram messages, runs one more 4-round pass if the length exceeds 1024, runs
+// the remaining rounds in LDS, captures the length-d folded vector V for
 // norm_check_3, forms rhs = conj(tensor(coefficient challenges)) * V, and
 // replays the verifier over all rounds.
 #include <algorithm>
@@ -26,7 +27,8 @@ namespace salsaa {
 namespace {
 
 constexpr int SC_THREADS = 256;
-constexpr int SC_FINAL_MAX = 1024;
+constexpr int SC_FINAL_MAX = 1024;  // lengths the final block holds in LDS
+constexpr int SC_TAIL_MAX = 4096;   // lengths the final block folds once from HBM first
 
 // ---- field helpers: F_q (FQ2 = false, imaginary parts constant zero) or
 // F_{q^2}.  SNR: the non-residue is below 8, so nr * t for a lazy product
@@ -161,18 +163,19 @@ __device__ __forceinline__ size_t macc_idx(int rep, int t, int row, int nrounds,
     return (((size_t)rep * nrounds + t) * rows + row) * 6;
 }
 
-// Block-wide sum of NV values (each in [0, q)); result in red[0..NV).
+// Block-wide sum of NV values (each in [0, q)); result in red[0..NV).  Every
+// thread of the block calls it.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&x)[NV], double *red, Mod M) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int NW = SC_THREADS / 64;
     __syncthreads();
+    double ws[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) {
-        double t = x[v];
+    for (int v = 0; v < NV; v++) ws[v] = wave_sum_res(x[v], M);
+    if (lane == 0) {
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) t = addmod(t, __shfl_xor(t, off, 64), M);
-        if (lane == 0) red[NV + wave * NV + v] = t;
+        for (int v = 0; v < NV; v++) red[NV + wave * NV + v] = ws[v];
     }
     __syncthreads();
     if (threadIdx.x < NV) {
@@ -208,8 +211,9 @@ __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) 
 // int8 matrix cores for G.  With the 7 signed base-256 digits d_i of each
 // residue (as in the commit), G = sum_k 2^(8k) E_k, E_k = sum_{i+j=k} D_i D_j^T,
 // and each E_k accumulates exactly in int32 (|d| <= 128, at most 7 digit
-// pairs, at most 512 columns per wave).  k_sc_gram_msgs then evaluates the
-// forms for each (row, repetition).
+// pairs, at most 2^14 columns per wave: 7 * 2^14 * 2^14 < 2^31).  The final
+// block then evaluates the
+// forms for its (row, repetition).
 //
 // Short witnesses (the norm check's purpose) take a cheaper path: when every
 // residue of a wave's 64 columns is b in balanced form with |b| < 2^14,
@@ -225,6 +229,9 @@ __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) 
 typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int GR_X = SC_THREADS;  // columns per iteration
 constexpr int GR_LD = GR_X + 2;   // LDS row stride (u64): fragment reads hit distinct banks
+// k_sc_gram_rest's per-wave staging row stride (u32): unpadded, so that its
+// LDS stays below k_sc_gram_short's and fits beside the batched step's NTT pass
+constexpr int GW_LD = 64;
 constexpr int GR_MAXREP = 4;      // repetitions folded per launch
 constexpr uint64_t GR_BIAS = 0x0080808080808080ULL;
 constexpr uint32_t GR_SMALL = kShortOff;  // short path: balanced |b| < 2^14, t = b + 2^14
@@ -394,23 +401,75 @@ __device__ __forceinline__ void gr_finish(const v4i (&acc)[K], double *gpart, in
     atomicAdd(&gacc[(size_t)row * 256 + tid], (unsigned long long)d2u(tot));
 }
 
-template <bool FQ2, bool SNR, bool GRAM>
-__global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_sc_gram(
-    const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
-    int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
-    double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc,
-    const double *__restrict__ tab, Mod M, const uint32_t *__restrict__ shf, int sbn, int nbx) {
-    __shared__ __align__(16) uint64_t T[16 * GR_LD];
-    __shared__ GrCoef C;
-    __shared__ double gpart[4 * 256];
+// The fold of a wide column (canonical residues at col[ld y], y < 16):
+// products |coef v| <= 0.69 q (balanced coef, canonical v), eight of them stay
+// below 5.6 q, then two lazy reductions.
+template <bool FQ2>
+__device__ __forceinline__ void gr_wide_fold(const GrCoef &C, const uint64_t *col, int ld, int nrep,
+                                             int rep0, int rows, int row, size_t S, size_t x,
+                                             double *__restrict__ out, Mod M) {
+    for (int r = 0; r < nrep; r++) {
+        double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            const F2 c0 = C.coef[r][y], c1 = C.coef[r][y + 8];
+            const double v0 = u2d(col[ld * y]), v1 = u2d(col[ld * (y + 8)]);
+            a0 += mul_lazy(c0.a, v0, M);
+            a1 += mul_lazy(c1.a, v1, M);
+            if (FQ2) {
+                b0 += mul_lazy(c0.b, v0, M);
+                b1 += mul_lazy(c1.b, v1, M);
+            }
+        }
+        double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
+        o[0] = reduce_lazy(reduce_lazy(a0, M) + a1, M);
+        o[1] = FQ2 ? reduce_lazy(reduce_lazy(b0, M) + b1, M) : 0.0;
+    }
+}
+
+// The 7 signed base-256 digit fragments of 16 consecutive staged residues
+// src[0 .. 15] (one digit row per fragment, four per dword).
+__device__ __forceinline__ void gr_wide_frags(const uint64_t *src, v4i (&f)[7]) {
+#pragma unroll
+    for (int k4 = 0; k4 < 4; k4++) {
+        uint32_t lo[4], hi[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint64_t yv = (src[4 * k4 + q] + GR_BIAS) ^ 0x8080808080808080ULL;
+            lo[q] = (uint32_t)yv;
+            hi[q] = (uint32_t)(yv >> 32);
+        }
+        uint32_t dg[7];
+        bytes_transpose4(lo, hi, dg);
+#pragma unroll
+        for (int i = 0; i < 7; i++) f[i][k4] = (int)dg[i];
+    }
+}
+
+// The first pass over a row's columns, shared by k_sc_gram (NARROW = false:
+// one 16 x GR_LD u64 staging block, for calls without a short copy) and
+// k_sc_gram_rest (NARROW = true: per wave 16 rows of GW_LD u32, which hold the
+// short path's t or half of the wave's columns of wide residues, so that it
+// fits beside the batched step's NTT pass).  Tall: the staging rows, reused
+// for the Gram partials after the loop.
+template <bool FQ2, bool SNR, bool GRAM, bool NARROW>
+__device__ __forceinline__ void sc_gram_body(
+    uint32_t *Tall, GrCoef &C, const uint64_t *__restrict__ w,
+    const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
+    const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr, int cpt,
+    double *__restrict__ out, unsigned long long *__restrict__ gacc, const double *__restrict__ tab,
+    Mod M, const uint32_t *__restrict__ shf, int sbn, int nbx) {
     const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t S = L >> 4;
     const size_t base = wrow ? wrow[row] : (size_t)row * L;
     const size_t span = (size_t)cpt * GR_X;
     const uint64_t *wr = w + base;
-    // the short path's u32 view of the wave's own columns of T: row y at
-    // T32 + 2 GR_LD y (the first half of the row's 64 u64 slots)
-    uint32_t *T32 = reinterpret_cast<uint32_t *>(T) + 128 * wave;
+    uint64_t *T = reinterpret_cast<uint64_t *>(Tall);  // NARROW = false: row y at T + GR_LD y
+    // the short path's u32 rows of the wave's columns: NARROW = false, row y
+    // at T32 + 2 GR_LD y (the first half of the row's 64 u64 slots)
+    const int ld32 = NARROW ? GW_LD : 2 * GR_LD;
+    uint32_t *T32 = NARROW ? Tall + 16 * GW_LD * wave : Tall + 128 * wave;
+    uint64_t *T64 = reinterpret_cast<uint64_t *>(T32);  // NARROW: a half's u64 rows
     const int fr = lane & 15, fg = lane >> 4;
     const uint64_t qi = (uint64_t)M.q, half = qi >> 1;
     const uint64_t off_neg = (uint64_t)GR_SMALL - qi;
@@ -435,11 +494,11 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     for (int k = 0; k < sbn; k++) {
         if (!((todo >> k) & 1)) continue;
         size_t xb = (sb0 + k) * span;
-        // Column x = xb + tid.  Each iteration moves its column into the wave's
-        // columns of T and reuses the same registers for the next iteration's 16
-        // loads (unconditional: a dead column re-reads column S - 1 and counts as
-        // zero), which stay in flight while the fold and the matrix products
-        // read this one back from T.
+        // Column x = xb + tid.  Each iteration moves its column into the
+        // staging rows and reuses the same registers for the next iteration's
+        // 16 loads (unconditional: a dead column re-reads column S - 1 and
+        // counts as zero), which stay in flight while the fold and the matrix
+        // products read this one back.
         uint64_t v[16];
         {
             const size_t xc = min(xb + tid, S - 1);
@@ -450,84 +509,72 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
             const size_t x = xb + tid;
             const bool live = x < S;
             const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform
+            const size_t xn = min(more ? x + GR_X : x, S - 1);
             uint32_t bad = 0;
 #pragma unroll
             for (int y = 0; y < 16; y++) bad |= short_bad(short_t(v[y], half, off_neg));
             const bool small = __builtin_amdgcn_ballot_w64(live && bad != 0) == 0;  // uniform
-            if (small) {
-#pragma unroll
-                for (int y = 0; y < 16; y++)
-                    T32[2 * GR_LD * y + lane] =
-                        live ? (uint32_t)short_t(v[y], half, off_neg) : GR_SMALL;
-            } else {
-#pragma unroll
-                for (int y = 0; y < 16; y++) T[y * GR_LD + tid] = live ? v[y] : 0;
-            }
-            {
-                const size_t xc = min(more ? x + GR_X : x, S - 1);
-#pragma unroll
-                for (int y = 0; y < 16; y++) v[y] = wr[(size_t)y * S + xc];
-            }
-            wave_sync();
             // both paths then run the same 49 products (kept unconditional:
             // accumulators that only one branch updates cost copies instead)
             v4i f[7];
+            if (small || !NARROW) {
+                if (small) {
+#pragma unroll
+                    for (int y = 0; y < 16; y++)
+                        T32[ld32 * y + lane] = live ? (uint32_t)short_t(v[y], half, off_neg) : GR_SMALL;
+                } else {
+#pragma unroll
+                    for (int y = 0; y < 16; y++) T[y * GR_LD + tid] = live ? v[y] : 0;
+                }
+#pragma unroll
+                for (int y = 0; y < 16; y++) v[y] = wr[(size_t)y * S + xn];
+                wave_sync();
+            }
             if (small) {
                 if (live) {
                     uint32_t tt[16];
 #pragma unroll
-                    for (int y = 0; y < 16; y++) tt[y] = T32[2 * GR_LD * y + lane];
+                    for (int y = 0; y < 16; y++) tt[y] = T32[ld32 * y + lane];
                     gr_short_fold<FQ2, false>(C, tt, nrep, rep0, rows, row, S, x, out, M);
                 }
                 if (GRAM) {
-                    gr_short_frags(T32, 2 * GR_LD, fr, fg, f[0], f[1]);
+                    gr_short_frags(T32, ld32, fr, fg, f[0], f[1]);
 #pragma unroll
                     for (int i = 2; i < 7; i++) f[i] = (v4i){0, 0, 0, 0};
                 }
+            } else if (!NARROW) {
+                if (live) gr_wide_fold<FQ2>(C, T + tid, GR_LD, nrep, rep0, rows, row, S, x, out, M);
+                // lane 16 g + r takes digit row (i, r) of the wave's columns
+                // 16 g .. 16 g + 15: only the wave has to be in step around T
+                if (GRAM) gr_wide_frags(T + fr * GR_LD + 64 * wave + 16 * fg, f);
             } else {
-                if (live) {
-                    // products |coef v| <= 0.69 q (balanced coef, canonical v): eight
-                    // of them stay below 5.6 q, then two lazy reductions
-                    for (int r = 0; r < nrep; r++) {
-                        double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+                // The wave's 64 columns in two halves of 32: lanes 32 h ..
+                // 32 h + 31 stage their columns and fold them from the rows,
+                // then lane 16 g + r (g = 2 h, 2 h + 1) takes digit row (i, r)
+                // of columns 16 g .. 16 g + 15.  The next loads go out once the
+                // second half is staged.
 #pragma unroll
-                        for (int y = 0; y < 8; y++) {
-                            const F2 c0 = C.coef[r][y], c1 = C.coef[r][y + 8];
-                            const double v0 = u2d(T[y * GR_LD + tid]);
-                            const double v1 = u2d(T[(y + 8) * GR_LD + tid]);
-                            a0 += mul_lazy(c0.a, v0, M);
-                            a1 += mul_lazy(c1.a, v1, M);
-                            if (FQ2) {
-                                b0 += mul_lazy(c0.b, v0, M);
-                                b1 += mul_lazy(c1.b, v1, M);
-                            }
-                        }
-                        double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
-                        o[0] = reduce_lazy(reduce_lazy(a0, M) + a1, M);
-                        o[1] = FQ2 ? reduce_lazy(reduce_lazy(b0, M) + b1, M) : 0.0;
+                for (int hf = 0; hf < 2; hf++) {
+                    const bool mine = (lane >> 5) == hf;
+                    if (mine) {
+#pragma unroll
+                        for (int y = 0; y < 16; y++)
+                            T64[(GW_LD / 2) * y + (lane & 31)] = live ? v[y] : 0;
                     }
-                }
-                if (GRAM) {
-                    // lane 16 g + r takes digit row (i, r) of the wave's columns
-                    // 16 g .. 16 g + 15: only the wave has to be in step around T
-                    const uint64_t *src = T + fr * GR_LD + 64 * wave + 16 * fg;
+                    if (hf == 1) {
 #pragma unroll
-                    for (int k4 = 0; k4 < 4; k4++) {
-                        uint32_t lo[4], hi[4];
-#pragma unroll
-                        for (int q = 0; q < 4; q++) {
-                            const uint64_t yv = (src[4 * k4 + q] + GR_BIAS) ^ 0x8080808080808080ULL;
-                            lo[q] = (uint32_t)yv;
-                            hi[q] = (uint32_t)(yv >> 32);
-                        }
-                        uint32_t dg[7];
-                        bytes_transpose4(lo, hi, dg);
-#pragma unroll
-                        for (int i = 0; i < 7; i++) f[i][k4] = (int)dg[i];
+                        for (int y = 0; y < 16; y++) v[y] = wr[(size_t)y * S + xn];
                     }
+                    wave_sync();
+                    if (mine && live)
+                        gr_wide_fold<FQ2>(C, T64 + (lane & 31), GW_LD / 2, nrep, rep0, rows, row, S,
+                                          x, out, M);
+                    if (GRAM && (fg >> 1) == hf)
+                        gr_wide_frags(T64 + fr * (GW_LD / 2) + 16 * (fg & 1), f);
+                    if (hf == 0) wave_sync();  // the rows are restaged
                 }
             }
-            wave_sync();  // T is rewritten by the next iteration
+            wave_sync();  // the rows are rewritten by the next iteration
             if (GRAM) {
 #pragma unroll
                 for (int i = 0; i < 7; i++)
@@ -542,7 +589,40 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     }
     }
     if (!ready) return;
-    if (GRAM) gr_finish<13>(acc, gpart, wave, lane, tid, tab, gacc, row, M);
+    if (GRAM) {
+        __syncthreads();  // the partials reuse the staging rows
+        gr_finish<13>(acc, reinterpret_cast<double *>(Tall), wave, lane, tid, tab, gacc, row, M);
+    }
+}
+
+template <bool FQ2, bool SNR, bool GRAM>
+__global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_sc_gram(
+    const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
+    int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
+    double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc,
+    const double *__restrict__ tab, Mod M) {
+    __shared__ __align__(16) uint64_t T[16 * GR_LD];
+    __shared__ GrCoef C;
+    sc_gram_body<FQ2, SNR, GRAM, false>(reinterpret_cast<uint32_t *>(T), C, w, wrow, L, rows, rep0,
+                                        nrep, cs0, cs_row, nrounds, nr, cpt, out, gacc, tab, M,
+                                        nullptr, 1, (int)gridDim.x);
+}
+
+// The sub-blocks of a batch arena that k_sc_gram_short leaves (some tile not
+// short), on a grid that walks the super-blocks; it returns at once when
+// k_sc_gram_short has flagged none (need_long zero), the usual case.
+template <bool FQ2, bool SNR, bool GRAM>
+__global__ __launch_bounds__(SC_THREADS) void k_sc_gram_rest(
+    const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
+    int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
+    double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc,
+    const double *__restrict__ tab, Mod M, const uint32_t *__restrict__ shf, int sbn, int nbx,
+    const unsigned long long *__restrict__ need_long) {
+    __shared__ __align__(16) uint32_t Tall[4 * 16 * GW_LD];
+    __shared__ GrCoef C;
+    if (*need_long == 0) return;  // uniform
+    sc_gram_body<FQ2, SNR, GRAM, true>(Tall, C, w, wrow, L, rows, rep0, nrep, cs0, cs_row, nrounds,
+                                       nr, cpt, out, gacc, tab, M, shf, sbn, nbx);
 }
 
 // The short copy's blocks: the same columns and outputs as k_sc_gram, read
@@ -553,7 +633,8 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_short(
     const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
     const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr, int cpt,
     double *__restrict__ out, unsigned long long *__restrict__ gacc, const double *__restrict__ tab,
-    Mod M, const uint16_t *__restrict__ sht, const uint32_t *__restrict__ shf, int sbn) {
+    Mod M, const uint16_t *__restrict__ sht, const uint32_t *__restrict__ shf, int sbn,
+    unsigned long long *__restrict__ need_long) {
     __shared__ __align__(16) uint32_t T32all[16 * GS_LD];  // then the Gram partials
     __shared__ GrCoef C;
     const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -562,8 +643,14 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_short(
     // the sub-blocks (as in k_sc_gram) that lie in short tiles (uniform)
     const size_t span = (size_t)cpt * GR_X, sb0 = (size_t)blockIdx.x * sbn;
     uint32_t todo = 0;
-    for (int k = 0; k < sbn && (sb0 + k) * span < S; k++)
-        if (gr_compact(shf, base, S, (sb0 + k) * span, cpt)) todo |= 1u << k;
+    bool left = false;  // a sub-block for k_sc_gram
+    for (int k = 0; k < sbn && (sb0 + k) * span < S; k++) {
+        if (gr_compact(shf, base, S, (sb0 + k) * span, cpt))
+            todo |= 1u << k;
+        else
+            left = true;
+    }
+    if (left && tid == 0) *need_long = 1ULL;
     if (todo == 0) return;
     gr_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
     uint32_t *T32 = T32all + 64 * wave;
@@ -620,68 +707,6 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_short(
 // Messages of rounds 0..3 (and the claim) from G, one block per (row, rep):
 // thread (z, y) (z < 8, y < 16) forms column y of G A[z] and its share of
 // both forms; the shares are summed across the block.
-template <bool FQ2, bool SNR>
-__global__ __launch_bounds__(128) void k_sc_gram_msgs(const unsigned long long *__restrict__ gacc,
-                                                      int rows, const uint64_t *__restrict__ cs0,
-                                                      size_t cs_row, int nrounds, double nr,
-                                                      unsigned long long *__restrict__ macc,
-                                                      unsigned long long *__restrict__ cacc, Mod M) {
-    __shared__ double G[16][17];
-    __shared__ F2 A[16][17];
-    __shared__ double red[2][4];
-    const int row = blockIdx.x, rep = blockIdx.y, tid = threadIdx.x;
-    const int z = tid >> 4, y = tid & 15, lane = tid & 63, wave = tid >> 6;
-    const uint64_t *cs = cs0 + row * cs_row;
-    for (int i = tid; i < 256; i += 128) {
-        G[i >> 4][i & 15] = u64_mod(gacc[(size_t)row * 256 + i], M);
-        A[i >> 4][i & 15] = {(i >> 4) == (i & 15) ? 1.0 : 0.0, 0.0};
-    }
-    __syncthreads();
-    if (rep == 0 && tid == 0) {
-        double tr = 0.0;
-        for (int k = 0; k < 16; k++) tr = addmod(tr, G[k][k], M);
-        cacc[row] = (unsigned long long)d2u(tr);
-    }
-    for (int t = 0; t < 4; t++) {
-        const int h = 8 >> t;
-        double vals[4] = {0.0, 0.0, 0.0, 0.0};
-        if (z < h) {
-            F2 u = {0.0, 0.0};
-#pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const F2 a = A[z][k];
-                u.a = addmod(u.a, mulmod(G[y][k], a.a, M), M);
-                if (FQ2) u.b = addmod(u.b, mulmod(G[y][k], a.b, M), M);
-            }
-            const F2 az = A[z][y];
-            const F2 f0 = x_mul<FQ2, SNR>(az, u, nr, M);
-            const F2 f1 = x_mul<FQ2, SNR>(x_sub<FQ2>(A[z + h][y], az, M), u, nr, M);
-            vals[0] = f0.a;
-            vals[1] = f0.b;
-            vals[2] = f1.a;
-            vals[3] = f1.b;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            double v = vals[k];
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) v = addmod(v, __shfl_xor(v, off, 64), M);
-            if (lane == 0) red[wave][k] = v;
-        }
-        __syncthreads();  // also: every G A[z] above has read A
-        if (tid < 4)
-            macc[macc_idx(rep, t, row, nrounds, rows) + tid] =
-                (unsigned long long)d2u(addmod(red[0][tid], red[1][tid], M));
-        // fold the vectors with this repetition's challenge
-        if (z < h) {
-            const F2 c = load_chal<FQ2>(cs, rep, nrounds, t);
-            const F2 a0 = A[z][y];
-            A[z][y] = x_add<FQ2>(a0, x_mul<FQ2, SNR>(c, x_sub<FQ2>(A[z + h][y], a0, M), nr, M), M);
-        }
-        __syncthreads();
-    }
-}
-
 // ---------------------------------------------------------------- later passes
 // F2 input in [reps][rows][len] (pairs of doubles), rounds t0..t0+3.
 template <bool FQ2, bool SNR>
@@ -742,23 +767,86 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_pass(
     }
 }
 
+// Rounds 0..3's messages of (row, rep) from the row's Gram sums (graw: this
+// thread's entry, gacc[row][tid]) into
+// msg[0..3] (f0, f1), and the claim tr G into *claim (row 0's plus one with
+// claim_fault, a planted error).  The fold vectors have a closed form: after
+// t rounds (n = 16 >> t segments, h = n / 2), A_t[z][y] = [y mod n = z] e_t(y)
+// with e_t(y) = prod_{s<t} (bit 3 - s of y ? c_s : 1 - c_s), so
+//     f0(t) = sum over y = y' (mod n), y' mod n < h of e_t(y) G[y][y'] e_t(y'),
+//     f1(t) = sum over y = y' + h (mod n), y' mod n < h of the same - f0(t),
+// and every thread takes one Gram entry for all four rounds at once.
+template <bool FQ2, bool SNR>
+__device__ __forceinline__ void gram_msgs(unsigned long long graw, int row, const F2 *chal,
+                                          double (*msg)[4], double *claim, int claim_fault,
+                                          double *red, double nr, Mod M) {
+    static_assert(SC_THREADS == 256, "one thread per Gram entry");
+    __shared__ F2 E[4][16];
+    const int tid = threadIdx.x, y = tid >> 4, y2 = tid & 15;
+    const double g = u64_mod(graw, M);
+    if (tid < 64) {
+        const int t = tid >> 4, yy = tid & 15;
+        F2 e = {1.0, 0.0};
+        for (int s = 0; s < t; s++) {
+            const F2 c = chal[s];
+            e = x_mul<FQ2, SNR>(e, ((yy >> (3 - s)) & 1) ? c : x_sub<FQ2>(F2{1.0, 0.0}, c, M), nr, M);
+        }
+        E[t][yy] = e;
+    }
+    __syncthreads();
+    double vals[17];
+#pragma unroll
+    for (int k = 0; k < 17; k++) vals[k] = 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int n = 16 >> t, h = n >> 1, ry = y & (n - 1), ry2 = y2 & (n - 1);
+        if (ry2 < h && (ry == ry2 || ry == ry2 + h)) {
+            const F2 p = x_mul<FQ2, SNR>(E[t][y], E[t][y2], nr, M);
+            const int k = 4 * t + (ry == ry2 ? 0 : 2);
+            vals[k] = mulmod(p.a, g, M);
+            vals[k + 1] = FQ2 ? mulmod(p.b, g, M) : 0.0;
+        }
+    }
+    vals[16] = y == y2 ? g : 0.0;  // tr G
+    block_sum<17>(vals, red, M);
+    if (tid < 4) {
+        const double f0a = red[4 * tid], f0b = red[4 * tid + 1];
+        msg[tid][0] = f0a;
+        msg[tid][1] = f0b;
+        msg[tid][2] = submod(red[4 * tid + 2], f0a, M);
+        msg[tid][3] = submod(red[4 * tid + 3], f0b, M);
+    }
+    if (tid == 0) *claim = claim_fault && row == 0 ? addmod(red[16], 1.0, M) : red[16];
+}
+
 // ---------------------------------------------------------------- final block
-// One block per (row, rep): remaining rounds t0.. in LDS, norm_check_3, verifier.
+// One block per (row, rep) runs the rest of the norm check.  FIRST: the whole
+// row (L <= SC_FINAL_MAX) from the witness.  Otherwise: rounds 0..3 from the
+// Gram sums, rounds 4 .. t0 - 1 from the streaming passes' accumulators, then
+// the folded witness `in` of length len <= SC_TAIL_MAX: above SC_FINAL_MAX the
+// block first runs rounds t0 .. t0 + 3 as one streaming pass (a column of 16
+// entries per thread, at most 256 columns) into LDS.  The remaining rounds run
+// in LDS, V (length d) is captured for norm_check_3, and the verifier is
+// replayed over all rounds.
 template <bool FQ2, bool FIRST, bool SNR>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
     const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow,
     const double *__restrict__ in, size_t len, int rows, int t0,
     const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr,
-    const unsigned long long *__restrict__ macc, const unsigned long long *__restrict__ cacc,
-    NcOut o, int want_rhs, uint64_t q, const double *__restrict__ tab, Mod M) {
+    const unsigned long long *__restrict__ macc, const unsigned long long *__restrict__ gacc,
+    int claim_fault, NcOut o, int want_rhs, uint64_t q, const double *__restrict__ tab, Mod M) {
     __shared__ F2 cur[SC_FINAL_MAX];
     __shared__ F2 V[kD];
     __shared__ F2 base[kD];
     __shared__ double rc[2][kD];
     __shared__ double red[8 * 64];
-    __shared__ double msg_local[32][6];
-    __shared__ double claim_local;
+    __shared__ double msg[48][4];  // f0, f1 of every round (f2 follows in the replay)
+    __shared__ double claim_sh;
     __shared__ F2 chal[48];
+    __shared__ double f2s[48][2];
+    __shared__ F2 part[4][kD];  // norm_check_3's partial products
+    __shared__ double msgp[10][4][64];  // the in-LDS rounds' per-lane messages
+    __shared__ F2 sa_[48], sb_[48], clv[48];  // the replay's affine maps, claims
     const int row = blockIdx.x, rep = blockIdx.y;
     const int tid = threadIdx.x;
     const uint64_t *cs = cs0 + row * cs_row;
@@ -767,18 +855,12 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
     for (int tt = tid; tt < nrounds; tt += SC_THREADS) chal[tt] = load_chal<FQ2>(cs, rep, nrounds, tt);
     __syncthreads();
 
-    for (size_t x = tid; x < len; x += SC_THREADS) {
-        if (FIRST) {
+    int t = t0;
+    if (FIRST) {
+        for (size_t x = tid; x < len; x += SC_THREADS) {
             cur[x].a = u2d(w[(wrow ? wrow[row] : (size_t)row * len) + x]);
             cur[x].b = 0.0;
-        } else {
-            // the streaming passes leave balanced residues
-            const double *p = in + (((size_t)rep * rows + row) * len + x) * 2;
-            cur[x].a = canon(p[0], M);
-            cur[x].b = FQ2 ? canon(p[1], M) : 0.0;
         }
-    }
-    if (FIRST) {
         // norm_check_1 claim, computed locally from the whole row
         __syncthreads();
         double s[1] = {0.0};
@@ -786,71 +868,153 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
             s[0] = reduce_lazy(s[0] + mul_lazy(cur[x].a, cur[x].a, M), M);
         s[0] = reduce(s[0], M);
         block_sum<1>(s, red, M);
-        if (tid == 0) claim_local = red[0];
+        if (tid == 0) claim_sh = red[0];
+    } else {
+        const double *ib = in + ((size_t)rep * rows + row) * len * 2;
+        const bool pre = len > (size_t)SC_FINAL_MAX;
+        const size_t S = len >> 4;
+        // this thread's Gram entry, then (with the pass below) its column's 16
+        // entries: the loads stay in flight through the Gram messages
+        const unsigned long long graw = gacc[(size_t)row * 256 + tid];
+        F2 col[16];
+        if (pre && (size_t)tid < S) {
+#pragma unroll
+            for (int y = 0; y < 16; y++) {
+                const double *p = ib + ((size_t)y * S + tid) * 2;
+                col[y].a = p[0];
+                col[y].b = FQ2 ? p[1] : 0.0;
+            }
+        }
+        gram_msgs<FQ2, SNR>(graw, row, chal, msg, &claim_sh, claim_fault, red, nr, M);
+        for (int x = tid; x < (t0 - 4) * 4; x += SC_THREADS) {
+            const int tt = 4 + x / 4, k = x % 4;
+            msg[tt][k] = u64_mod(macc[macc_idx(rep, tt, row, nrounds, rows) + k], M);
+        }
+        if (pre) {
+            // rounds t0 .. t0 + 3 over the columns x < S of 16 entries
+            // (balanced residues, as in k_sc_pass)
+            F2 acc[4][2];
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[r][0] = acc[r][1] = F2{0.0, 0.0};
+            if ((size_t)tid < S) {
+                F2 c[4];
+                double csum[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const F2 cr = chal[t0 + r];
+                    c[r] = {balance(cr.a, M), FQ2 ? balance(cr.b, M) : 0.0};
+                    csum[r] = c[r].a + c[r].b;
+                }
+                sc_level<FQ2, SNR, 8>(col, c[0], csum[0], acc[0][0], acc[0][1], nr, M);
+                sc_level<FQ2, SNR, 4>(col, c[1], csum[1], acc[1][0], acc[1][1], nr, M);
+                sc_level<FQ2, SNR, 2>(col, c[2], csum[2], acc[2][0], acc[2][1], nr, M);
+                sc_level<FQ2, SNR, 1>(col, c[3], csum[3], acc[3][0], acc[3][1], nr, M);
+                cur[tid] = {canon(col[0].a, M), FQ2 ? canon(col[0].b, M) : 0.0};
+            }
+            double vals[16];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    vals[r * 4 + f * 2] = reduce(acc[r][f].a, M);
+                    vals[r * 4 + f * 2 + 1] = reduce(acc[r][f].b, M);
+                }
+            block_sum<16>(vals, red, M);  // synchronises: cur is complete
+            if (tid < 16) msg[t0 + tid / 4][tid % 4] = red[tid];
+            len = S;
+            t = t0 + 4;
+        } else {
+            // the streaming passes leave balanced residues
+            for (size_t x = tid; x < len; x += SC_THREADS) {
+                const double *p = ib + x * 2;
+                cur[x].a = canon(p[0], M);
+                cur[x].b = FQ2 ? canon(p[1], M) : 0.0;
+            }
+        }
     }
     __syncthreads();
 
-    size_t cl = len;
-    int t = t0;
-    while (cl > 1) {
-        if (cl == (size_t)kD && want_rhs) {
-            for (int x = tid; x < kD; x += SC_THREADS) V[x] = cur[x];
+    // ---- the remaining rounds: wave 0 folds cur in LDS (ordered within the
+    // wave only, messages by wave reductions); the other waves meanwhile form
+    // norm_check_3's base = conj(tensor(last log2 d challenges))
+    const int lane = tid & 63, wave = tid >> 6;
+    const int tl = t;  // the first of these rounds
+    if (wave == 0) {
+        size_t cl = len;
+        while (cl > 1) {
+            if (cl == (size_t)kD && want_rhs) V[lane] = cur[lane];
+            const size_t half = cl >> 1;
+            const F2 c = chal[t];
+            F2 a0 = {0.0, 0.0}, a1 = {0.0, 0.0};
+            for (size_t x = lane; x < half; x += 64) {
+                F2 w0 = cur[x];
+                F2 d = x_sub<FQ2>(cur[x + half], w0, M);
+                x_sq_mac<FQ2, SNR>(a0, w0, nr, M);
+                x_mac<FQ2, SNR>(a1, w0, d, nr, M);
+                cur[x] = x_add<FQ2>(w0, x_mul3<FQ2, SNR>(c, d, nr, M), M);
+            }
+            // the lanes' message parts, summed after the last round
+            msgp[t - tl][0][lane] = reduce(a0.a, M);
+            msgp[t - tl][1][lane] = reduce(a0.b, M);
+            msgp[t - tl][2][lane] = reduce(a1.a, M);
+            msgp[t - tl][3][lane] = reduce(a1.b, M);
+            wave_sync();  // the folded entries are visible to the whole wave
+            cl = half;
+            t++;
         }
-        const size_t half = cl >> 1;
-        const F2 c = chal[t];
-        F2 a0 = {0.0, 0.0}, a1 = {0.0, 0.0};
-        for (size_t x = tid; x < half; x += SC_THREADS) {
-            F2 w0 = cur[x];
-            F2 d = x_sub<FQ2>(cur[x + half], w0, M);
-            x_sq_mac<FQ2, SNR>(a0, w0, nr, M);
-            x_mac<FQ2, SNR>(a1, w0, d, nr, M);
-            cur[x] = x_add<FQ2>(w0, x_mul3<FQ2, SNR>(c, d, nr, M), M);
+    } else if (want_rhs && wave == 1) {
+        F2 tk = {1.0, 0.0};
+        for (int b = 0; b < 6; b++) {
+            const F2 cb = chal[nrounds - 6 + b];
+            const int bit = (lane >> (5 - b)) & 1;
+            F2 f = bit ? cb : x_sub<FQ2>(F2{1.0, 0.0}, cb, M);
+            tk = x_mul<FQ2, SNR>(tk, f, nr, M);
         }
-        double vals[4] = {reduce(a0.a, M), reduce(a0.b, M), reduce(a1.a, M), reduce(a1.b, M)};
-        block_sum<4>(vals, red, M);  // synchronises: folded cur is complete
-        if (tid < 4) msg_local[t - t0][tid] = red[tid];
-        cl = half;
-        t++;
+        // conj: base[0] = T[0], base[i] = -T[d - i]; lane owns T[lane]
+        base[lane == 0 ? 0 : kD - lane] = lane == 0 ? tk : x_sub<FQ2>(F2{0.0, 0.0}, tk, M);
     }
     __syncthreads();
     const F2 ev = cur[0];
-
-    // ---- norm_check_3: rhs = conj(tensor(last log2 d challenges)) * V
-    if (want_rhs) {
-        if (tid < kD) {
-            F2 tk = {1.0, 0.0};
-            for (int b = 0; b < 6; b++) {
-                const F2 cb = chal[nrounds - 6 + b];
-                const int bit = (tid >> (5 - b)) & 1;
-                F2 f = bit ? cb : x_sub<FQ2>(F2{1.0, 0.0}, cb, M);
-                tk = x_mul<FQ2, SNR>(tk, f, nr, M);
-            }
-            // conj: base[0] = T[0], base[i] = -T[d - i]; thread tid owns T[tid]
-            const int dst = tid == 0 ? 0 : kD - tid;
-            F2 bv = tid == 0 ? tk : x_sub<FQ2>(F2{0.0, 0.0}, tk, M);
-            base[dst] = bv;
+    // the rounds' messages: wave w sums rounds tl + w, tl + w + 4, ..
+    for (int r = wave; r < nrounds - tl; r += SC_THREADS / 64) {
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = wave_sum_res(msgp[r][k][lane], M);
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) msg[tl + r][k] = v[k];
         }
-        __syncthreads();
-        if (tid < kD) {
-            // negacyclic products, coefficient tid
-            double r0 = 0.0, r1 = 0.0;
-            for (int i = 0; i < kD; i++) {
-                const int j = tid - i;
-                const bool wrap = j < 0;
-                const F2 b = base[i];
-                const F2 v = V[wrap ? j + kD : j];
-                double p0, p1 = 0.0;
-                if (!FQ2) {
-                    p0 = mul_lazy(b.a, v.a, M);
-                } else {
-                    p0 = mul_lazy(b.a, v.a, M) + nr_times<SNR>(nr, mul_lazy(b.b, v.b, M), M);
-                    p1 = mul_lazy(b.a, v.b, M) + mul_lazy(b.b, v.a, M);
-                }
-                r0 = reduce_lazy(wrap ? r0 - p0 : r0 + p0, M);
-                r1 = reduce_lazy(wrap ? r1 - p1 : r1 + p1, M);
+    }
+
+    // ---- norm_check_3: rhs = base * V (negacyclic), thread (k, i) summing
+    // the 16 terms base[j] V[i - j], 16 k <= j < 16 k + 16, of coefficient i
+    if (want_rhs) {
+        const int i = lane, k = wave;
+        double r0 = 0.0, r1 = 0.0;
+#pragma unroll 4
+        for (int jj = 0; jj < 16; jj++) {
+            const int j = 16 * k + jj, vi = i - j;
+            const bool wrap = vi < 0;
+            const F2 b = base[j];
+            const F2 v = V[wrap ? vi + kD : vi];
+            double p0, p1 = 0.0;
+            if (!FQ2) {
+                p0 = mul_lazy(b.a, v.a, M);
+            } else {
+                p0 = mul_lazy(b.a, v.a, M) + nr_times<SNR>(nr, mul_lazy(b.b, v.b, M), M);
+                p1 = mul_lazy(b.a, v.b, M) + mul_lazy(b.b, v.a, M);
             }
-            rc[0][tid] = reduce(r0, M);
-            rc[1][tid] = reduce(r1, M);
+            r0 = reduce_lazy(wrap ? r0 - p0 : r0 + p0, M);
+            r1 = reduce_lazy(wrap ? r1 - p1 : r1 + p1, M);
+        }
+        part[k][i] = {reduce(r0, M), reduce(r1, M)};
+        __syncthreads();
+        if (tid < 2 * kD) {
+            const int comp = tid >> 6;
+            double s = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) s = addmod(s, comp ? part[kk][lane].b : part[kk][lane].a, M);
+            rc[comp][lane] = s;
         }
         __syncthreads();
         if (tid < 4) {
@@ -858,47 +1022,64 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
             const int comp = tid >> 1, p = tid & 1;
             double v[32];
 #pragma unroll
-            for (int i = 0; i < 32; i++) v[i] = rc[comp][2 * i + p];
+            for (int ii = 0; ii < 32; ii++) v[ii] = rc[comp][2 * ii + p];
             ntt_fwd_regs<32>(v, tab + TAB_FWD32, M);
             uint64_t *dst = o.rhs + rep * o.rhs_rep + row * o.rhs_row + comp * kD + 32 * p;
 #pragma unroll
-            for (int i = 0; i < 32; i++) dst[i] = d2u(v[i]);
+            for (int ii = 0; ii < 32; ii++) dst[ii] = d2u(v[ii]);
         }
     }
 
-    // ---- verifier replay (fast_norm_check.rs:352-399, 443-462): first reduce
-    // the accumulated messages of the earlier passes in parallel
-    __shared__ double msg_pre[48][6];
-    __shared__ double claim_red;
-    for (int x = tid; x < t0 * 6; x += SC_THREADS) {
-        const int tt = x / 6, k = x % 6;
-        msg_pre[tt][k] = u64_mod(macc[macc_idx(rep, tt, row, nrounds, rows) + k], M);
-    }
-    if (tid == SC_THREADS - 1) claim_red = FIRST ? claim_local : u64_mod(cacc[row], M);
-    __syncthreads();
-    // Replay (thread 0).  Each round's f2 is the one value that satisfies the
-    // round identity 2 f0 + 2 f1 + f2 = claim (f2 = sum (w1 - w0)^2 equals it
-    // exactly, so the messages are those of a direct evaluation); the
+    // ---- verifier replay (fast_norm_check.rs:352-399, 443-462).  Round t's
+    // f2 is the one value with 2 f0 + 2 f1 + f2 = claim_t (f2 = sum (w1 - w0)^2
+    // equals it exactly, so the messages are those of a direct evaluation),
+    // and then claim_{t+1} = f0 + 2 f1 c + f2 c^2 = alpha_t claim_t + beta_t
+    // with alpha_t = c^2, beta_t = f0 (1 - 2 c^2) + 2 f1 (c - c^2): an affine
+    // recurrence, evaluated by a prefix scan of the maps (alpha, beta).  The
     // accumulated f0 and f1 of every round are then checked at the end, where
     // the folded evaluation must square to the final claim.
-    __shared__ double f2s[48][2];
-    if (tid == 0) {
-        const double claim0 = claim_red;
-        if (rep == 0 && o.claims) o.claims[row * o.claim_row] = d2u(claim0);
-        F2 cl2 = {claim0, 0.0};
-        int good = 1;
-        for (int tt = 0; tt < nrounds; tt++) {
-            const double *f = tt < t0 ? msg_pre[tt] : msg_local[tt - t0];
-            const F2 f0 = {f[0], f[1]}, f1 = {f[2], f[3]};
-            const F2 f2 = x_sub<true>(cl2, x_add<true>(x_add<true>(f0, f0, M),
-                                                         x_add<true>(f1, f1, M), M), M);
-            f2s[tt][0] = f2.a;
-            f2s[tt][1] = f2.b;
-            const F2 c = chal[tt];
-            const F2 c2 = x_mul<true, SNR>(c, c, nr, M);
-            cl2 = x_add<true>(x_add<true>(f0, x_mul<true, SNR>(x_add<true>(f1, f1, M), c, nr, M), M),
-                              x_mul<true, SNR>(f2, c2, nr, M), M);
+    const F2 one = {1.0, 0.0};
+    F2 f0 = {0.0, 0.0}, f1 = {0.0, 0.0};
+    if (tid < nrounds) {
+        f0 = {msg[tid][0], msg[tid][1]};
+        f1 = {msg[tid][2], msg[tid][3]};
+        const F2 c = chal[tid];
+        const F2 c2 = x_mul<true, SNR>(c, c, nr, M);
+        const F2 f1x2 = x_add<true>(f1, f1, M);
+        sa_[tid] = c2;
+        sb_[tid] = x_add<true>(x_mul<true, SNR>(f0, x_sub<true>(one, x_add<true>(c2, c2, M), M), nr, M),
+                               x_mul<true, SNR>(f1x2, x_sub<true>(c, c2, M), nr, M), M);
+    }
+    __syncthreads();
+    for (int d = 1; d < nrounds; d <<= 1) {
+        F2 na = {0.0, 0.0}, nb = {0.0, 0.0};
+        const bool act = tid < nrounds && tid >= d;
+        if (act) {
+            const F2 a2 = sa_[tid];
+            na = x_mul<true, SNR>(a2, sa_[tid - d], nr, M);
+            nb = x_add<true>(x_mul<true, SNR>(a2, sb_[tid - d], nr, M), sb_[tid], M);
         }
+        __syncthreads();
+        if (act) {
+            sa_[tid] = na;
+            sb_[tid] = nb;
+        }
+        __syncthreads();
+    }
+    const F2 claim0 = {claim_sh, 0.0};
+    if (tid < nrounds)  // the claim after round tid
+        clv[tid] = x_add<true>(x_mul<true, SNR>(sa_[tid], claim0, nr, M), sb_[tid], M);
+    __syncthreads();
+    if (tid < nrounds) {
+        const F2 clt = tid == 0 ? claim0 : clv[tid - 1];
+        const F2 f2 = x_sub<true>(clt, x_add<true>(x_add<true>(f0, f0, M), x_add<true>(f1, f1, M), M), M);
+        f2s[tid][0] = f2.a;
+        f2s[tid][1] = f2.b;
+    }
+    if (tid == 0) {
+        if (rep == 0 && o.claims) o.claims[row * o.claim_row] = d2u(claim_sh);
+        const F2 cl2 = clv[nrounds - 1];
+        int good = 1;
         const F2 ev2 = x_mul<true, SNR>(ev, ev, nr, M);
         if (ev2.a != cl2.a || ev2.b != cl2.b) good = 0;
         if (want_rhs) {
@@ -923,7 +1104,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
     if (o.msgs) {
         for (int x = tid; x < nrounds * 6; x += SC_THREADS) {
             const int tt = x / 6, k = x % 6;
-            const double f = k >= 4 ? f2s[tt][k - 4] : (tt < t0 ? msg_pre[tt][k] : msg_local[tt - t0][k]);
+            const double f = k >= 4 ? f2s[tt][k - 4] : msg[tt][k];
             o.msgs[rep * o.msg_rep + tt * o.msg_t + row * o.msg_row + k] = d2u(f);
         }
     }
@@ -934,9 +1115,6 @@ __global__ void k_zero_u64(unsigned long long *p, size_t n) {
     if (i < n) p[i] = 0ULL;
 }
 
-__global__ void k_add_one(unsigned long long *p) {
-    if (threadIdx.x == 0) *p += 1ULL;
-}
 
 template <bool FQ2, bool SNR>
 int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *wrow, size_t rows,
@@ -948,28 +1126,31 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
     const int want_rhs = o.rhs != nullptr;
     size_t need_m, need_a, need_b;
     nc_scratch_need(rows, L, reps, &need_m, &need_a, &need_b);
-    unsigned long long *macc = nullptr, *cacc = nullptr;
+    unsigned long long *macc = nullptr;
     const size_t nmacc = reps * (size_t)n * rows * 6;
     if (ext) {
         SALSAA_CHECK(ext->macc_bytes >= need_m && ext->bufA_bytes >= need_a &&
                          ext->bufB_bytes >= need_b,
                      "norm check scratch too small");
         macc = ext->macc;
-    } else {
+    } else if (L > (size_t)SC_FINAL_MAX) {
         SALSAA_TRY(scratch_get(ctx, 0, need_m, (void **)&macc));
     }
-    cacc = macc + nmacc;  // then the rows' Gram sums (rows x 256)
-    const size_t nzero = nmacc + rows + (L > (size_t)SC_FINAL_MAX ? rows * 256 : 0);
-    hipLaunchKernelGGL(k_zero_u64, dim3(ceil_div(nzero, 256)), dim3(256), 0, ctx->stream, macc,
-                       nzero);
-    SALSAA_HIP(hipGetLastError());
     if (L <= (size_t)SC_FINAL_MAX) {
+        // one block per (row, rep) from the witness: no accumulators
         hipLaunchKernelGGL((k_sc_final<FQ2, true, SNR>), dim3((unsigned)rows, (unsigned)reps),
-                           dim3(SC_THREADS), 0, ctx->stream, w, wrow, nullptr, L, (int)rows, 0, cs, cs_row,
-                           n, nr, macc, cacc, o, want_rhs, ctx->q, ctx->d_tables, M);
+                           dim3(SC_THREADS), 0, ctx->stream, w, wrow, nullptr, L, (int)rows, 0, cs,
+                           cs_row, n, nr, nullptr, nullptr, 0, o, want_rhs, ctx->q, ctx->d_tables, M);
         SALSAA_HIP(hipGetLastError());
         return SALSAA_OK;
     }
+    // the passes' message accumulators, the rows' Gram sums (rows x 256) and
+    // the flag of first-pass sub-blocks left to k_sc_gram_rest start at zero
+    unsigned long long *gacc = macc + nmacc, *need_long = gacc + rows * 256;
+    const size_t nzero = nmacc + rows * 256 + 1;
+    hipLaunchKernelGGL(k_zero_u64, dim3(ceil_div(nzero, 256)), dim3(256), 0, ctx->stream, macc,
+                       nzero);
+    SALSAA_HIP(hipGetLastError());
     // ping-pong buffers of folded witnesses (pairs of doubles)
     const size_t len1 = L >> 4;
     double *bufA = nullptr, *bufB = nullptr;
@@ -981,11 +1162,16 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         SALSAA_TRY(scratch_get(ctx, 2, need_b, (void **)&bufB));
     }
     {
-        // one streaming pass: the folds after round 3 and the Gram matrix,
-        // then rounds 0..3's messages from it
+        // one streaming pass: the folds after round 3 and the Gram matrix
+        // (rounds 0..3's messages follow from it in the final block)
         const size_t S = len1;
-        const int cpt = cpt_first > 0 ? std::min(cpt_first, 8)
-                                      : (S >= (size_t)GR_X * 64 ? 4 : 1);
+        // columns per thread: the batch tuner's choice, else the fewest that
+        // keep the grid within one round of resident blocks (k_sc_gram holds
+        // three per CU): more blocks add per-block setup and Gram atomics,
+        // fewer leave CUs idle
+        int cpt = cpt_first > 0 ? std::min(cpt_first, 8) : 1;
+        if (cpt_first <= 0)
+            while (cpt < 8 && rows * ceil_div(S, (size_t)GR_X * cpt) > (size_t)ctx->ncu * 3) cpt *= 2;
         const size_t nsub = ceil_div(S, (size_t)GR_X * cpt);
         // With a short copy, a block takes up to 8 sub-blocks of cpt GR_X
         // columns (each kernel skips the other's): the fewer, longer blocks
@@ -996,13 +1182,17 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
             for (sbn = 8; sbn > 1 && ceil_div(nsub, (size_t)sbn) * rows < (size_t)ctx->ncu * 4;)
                 sbn >>= 1;
         const unsigned bx = (unsigned)ceil_div(nsub, (size_t)sbn);
-        unsigned long long *gacc = cacc + rows;
         dim3 grid(bx, (unsigned)rows);
-        // With a short copy the general kernel only takes the blocks the copy
-        // cannot serve, usually none: a grid of about one block per CU walks
-        // the super-blocks instead of one heavy block each.
+        // With a short copy k_sc_gram_rest takes the sub-blocks the copy
+        // cannot serve, usually none (then each block returns after one
+        // load).  A block walks up to `walk` super-blocks, at most 256
+        // iterations of 64 columns per wave: its int32 digit sums, up to
+        // 7 x 2^14 per column, stay below 2^31.
         dim3 ggrid = grid;
-        if (sc) ggrid.x = (unsigned)std::min<size_t>(bx, std::max<size_t>(1, (size_t)ctx->ncu / rows));
+        if (sc) {
+            const size_t walk = std::max<size_t>(1, 256 / ((size_t)sbn * cpt));
+            ggrid.x = (unsigned)ceil_div((size_t)bx, walk);
+        }
         for (size_t r0 = 0; r0 < reps; r0 += GR_MAXREP) {
             const int nrep = (int)std::min<size_t>(GR_MAXREP, reps - r0);
             // with a short copy, its short blocks first, then the others
@@ -1010,31 +1200,37 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
             if (sc && r0 == 0)
                 hipLaunchKernelGGL((k_sc_gram_short<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn);
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn, need_long);
             else if (sc)
                 hipLaunchKernelGGL((k_sc_gram_short<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn);
-            if (r0 == 0)
-                hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, true>), ggrid, dim3(SC_THREADS), 0,
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn, need_long);
+            if (sc && r0 == 0)
+                hipLaunchKernelGGL((k_sc_gram_rest<FQ2, SNR, true>), ggrid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
-                                   nr, cpt, bufA, gacc, ctx->d_tables, M, shf, sbn, (int)bx);
+                                   nr, cpt, bufA, gacc, ctx->d_tables, M, shf, sbn, (int)bx,
+                                   need_long);
+            else if (sc)
+                hipLaunchKernelGGL((k_sc_gram_rest<FQ2, SNR, false>), ggrid, dim3(SC_THREADS), 0,
+                                   ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
+                                   nr, cpt, bufA, gacc, ctx->d_tables, M, shf, sbn, (int)bx,
+                                   need_long);
+            else if (r0 == 0)
+                hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
+                                   ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
+                                   nr, cpt, bufA, gacc, ctx->d_tables, M);
             else
-                hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, false>), ggrid, dim3(SC_THREADS), 0,
+                hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
-                                   nr, cpt, bufA, gacc, ctx->d_tables, M, shf, sbn, (int)bx);
+                                   nr, cpt, bufA, gacc, ctx->d_tables, M);
             SALSAA_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL((k_sc_gram_msgs<FQ2, SNR>), dim3((unsigned)rows, (unsigned)reps), dim3(128),
-                           0, ctx->stream, gacc, (int)rows, cs, cs_row, n, nr, macc, cacc, M);
-        if (fault_claim)
-            hipLaunchKernelGGL(k_add_one, dim3(1), dim3(64), 0, ctx->stream, cacc);
-        SALSAA_HIP(hipGetLastError());
     }
     size_t len = len1;
     int t0 = 4;
     double *src = bufA, *dst = bufB;
-    while (len > (size_t)SC_FINAL_MAX) {
+    // streaming passes while the final block cannot take the rest
+    while (len > (size_t)SC_TAIL_MAX) {
         const size_t S = len >> 4;
         const int cpt = S >= (size_t)SC_THREADS * 64 ? 4 : 1;
         dim3 grid(ceil_div(S, (size_t)SC_THREADS * cpt), (unsigned)rows, (unsigned)reps);
@@ -1049,8 +1245,7 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
     }
     hipLaunchKernelGGL((k_sc_final<FQ2, false, SNR>), dim3((unsigned)rows, (unsigned)reps),
                        dim3(SC_THREADS), 0, ctx->stream, nullptr, nullptr, src, len, (int)rows, t0, cs,
-                       cs_row,
-                       n, nr, macc, cacc, o, want_rhs, ctx->q, ctx->d_tables, M);
+                       cs_row, n, nr, macc, gacc, fault_claim, o, want_rhs, ctx->q, ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }
@@ -1061,7 +1256,7 @@ void nc_scratch_need(size_t rows, size_t L, size_t reps, size_t *macc_bytes, siz
                      size_t *bufB_bytes) {
     const size_t n = (L >= 2) ? (size_t)__builtin_ctzll(L) : 0;
     const size_t len1 = L >> 4;
-    *macc_bytes = (reps * n * rows * 6 + rows + (L > (size_t)SC_FINAL_MAX ? rows * 256 : 0)) *
+    *macc_bytes = (L > (size_t)SC_FINAL_MAX ? reps * n * rows * 6 + rows * 256 + 1 : 0) *
                   sizeof(unsigned long long);
     *bufA_bytes = L > (size_t)SC_FINAL_MAX ? reps * rows * len1 * 2 * sizeof(double) : 0;
     *bufB_bytes =

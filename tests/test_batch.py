@@ -229,6 +229,31 @@ def test_batch_short_copy_mixed(sa, ctx, orc):
     key.close()
 
 
+def test_batch_wide_rows_long_walk(sa, ctx, orc):
+    # A class of many long rows whose residues are all wide (no tile short):
+    # the first pass's blocks that take the rows from the witnesses walk many
+    # sub-blocks, and digit rows of 127 (residue 0x7F7F7F7F7F7F) make the
+    # int32 digit sums of the Gram matrix grow fastest (6 x 127^2 per column
+    # for E_5).  The step must equal the plain per-call path.
+    say("case: batch, 130 wide rows of m = 2^15")
+    n, m = 130, 1 << 15
+    A_h = orc.sample_uniform(12, sa.STREAM_A, (8, m, 64), Q)
+    key = ctx.key(ctx.to_device(A_h))
+    b = sa.Batch(ctx, key, [m] * n, [8] * n, reps=2, fq2=1, tune=False)
+    fill(sa, ctx, b, 1300)
+    wide = np.full((m, 64), 0x7F7F7F7F7F7F, np.uint64)
+    for i in range(n):
+        b.slot(i)["witness"].upload(wide)
+    b.step_plain()
+    plain = snapshot(b, True)
+    b.step(graph=False)
+    ctx.sync()
+    same(snapshot(b, False), plain)
+    assert all(p["ok"].all() for p in plain)
+    b.close()
+    key.close()
+
+
 def test_batch_split_pass_equals_single_pass(sa, ctx, orc):
     # The step's NTT pass runs in two launches (SALSAA_STEP_SPLIT=1: the
     # shorter classes first, 2: the longer ones first; the first part's norm
