@@ -426,6 +426,14 @@ int salsaa_batch_tuning_ntt(const salsaa_batch *b, int *xch, float *lds_ms, floa
 /* Sets the NTT pass's schedule (0 LDS, 1 DPP); the step is captured again on
  * its next graph replay.  Waits for the context's stream. */
 int salsaa_batch_set_ntt_schedule(salsaa_batch *b, int xch);
+/* The NTT pass's persistent blocks the tuner chose (blocks: the fastest of
+ * one per CU, 7/8 and 3/4 of that), the candidates and its times of the
+ * whole pass with each (cand[3], ms[3]; zero without tuning).  Any pointer
+ * may be NULL. */
+int salsaa_batch_tuning_grid(const salsaa_batch *b, unsigned *blocks, unsigned *cand, float *ms);
+/* Sets the NTT pass's persistent blocks (0: one per CU); the step is captured
+ * again on its next graph replay.  Waits for the context's stream. */
+int salsaa_batch_set_ntt_grid(salsaa_batch *b, unsigned blocks);
 
 /* ------------------------------------------------------------------ folding
  * The folding step of n accumulated instances (the protocol's folding round,

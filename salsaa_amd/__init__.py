@@ -176,6 +176,10 @@ def _load():
                                            ctypes.POINTER(ctypes.c_float),
                                            ctypes.POINTER(ctypes.c_float)]),
         "salsaa_batch_set_ntt_schedule": (_int, [_vp, _int]),
+        "salsaa_batch_tuning_grid": (_int, [_vp, ctypes.POINTER(ctypes.c_uint),
+                                            ctypes.POINTER(ctypes.c_uint),
+                                            ctypes.POINTER(ctypes.c_float)]),
+        "salsaa_batch_set_ntt_grid": (_int, [_vp, ctypes.c_uint]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -762,13 +766,20 @@ class Batch:
         x, tl, td = ctypes.c_int(), ctypes.c_float(), ctypes.c_float()
         _check(lib.salsaa_batch_tuning_ntt(self.h, ctypes.byref(x), ctypes.byref(tl),
                                            ctypes.byref(td)))
+        g, gc, gt = ctypes.c_uint(), (ctypes.c_uint * 3)(), (ctypes.c_float * 3)()
+        _check(lib.salsaa_batch_tuning_grid(self.h, ctypes.byref(g), gc, gt))
         return dict(kchunk=kc.value, cpt_first=cpt.value, commit_ms=tc.value,
                     norm_check_ms=tn.value, ntt_schedule=("lds", "dpp")[x.value],
-                    ntt_pass_ms={"lds": tl.value, "dpp": td.value})
+                    ntt_pass_ms={"lds": tl.value, "dpp": td.value}, ntt_blocks=g.value,
+                    ntt_blocks_ms={str(gc[i]): gt[i] for i in range(3) if gc[i]})
 
     def set_ntt_schedule(self, name):
         """The NTT pass's exchange: "lds" or "dpp" (quad permutes)."""
         _check(lib.salsaa_batch_set_ntt_schedule(self.h, {"lds": 0, "dpp": 1}[name]))
+
+    def set_ntt_grid(self, blocks):
+        """The NTT pass's persistent blocks (0: one per CU)."""
+        _check(lib.salsaa_batch_set_ntt_grid(self.h, int(blocks)))
 
     def close(self):
         if self.h:

@@ -142,6 +142,11 @@ struct salsaa_batch {
     // permutes) and the tuner's times of the whole pass with each
     int ntt_xch = 0;
     float t_ntt[2] = {0.f, 0.f};
+    // the NTT pass's persistent blocks (0: one per CU) and the tuner's
+    // times of the whole pass at each candidate
+    unsigned ntt_grid = 0;
+    unsigned grid_cand[3] = {0, 0, 0};
+    float t_grid[3] = {0.f, 0.f, 0.f};
     // plain path
     std::vector<salsaa_key *> views;
     // graph
@@ -314,7 +319,7 @@ int run_norm_checks(salsaa_batch *b, int cpt, const ShortCopy *sc) {
 
 int run_fwd_planes(salsaa_batch *b, int roundtrip) {
     return launch_ntt_fwd_planes(&b->ctx->c, b->d_wit, b->tot_w, b->d_planes, b->d_ptiles,
-                                 roundtrip, roundtrip ? &b->sc : nullptr, b->ntt_xch);
+                                 roundtrip, roundtrip ? &b->sc : nullptr, b->ntt_xch, b->ntt_grid);
 }
 
 // the round trip over arena tiles [t0, t1) (the pass's tables and short copy
@@ -323,7 +328,7 @@ int run_rt_range(salsaa_batch *b, size_t t0, size_t t1) {
     ShortCopy sc{b->sc.t + t0 * kShortTile, b->sc.flag + t0};
     return launch_ntt_fwd_planes(&b->ctx->c, b->d_wit + t0 * kShortTile,
                                  (t1 - t0) * kPlaneTile, b->d_planes, b->d_ptiles + t0, 1, &sc,
-                                 b->ntt_xch);
+                                 b->ntt_xch, b->ntt_grid);
 }
 
 // one pass: forward NTT, the commit's digit planes, the inverse NTT and the
@@ -405,6 +410,20 @@ int autotune(salsaa_batch *b) {
         SALSAA_TRY(time_ms(b, [&] { return run_fwd_planes(b, 1); }, 2, &b->t_ntt[x]));
     }
     b->ntt_xch = b->t_ntt[1] < b->t_ntt[0] ? 1 : 0;
+    // its persistent blocks: one per CU, 7/8 or 3/4 of that (the write-heavy
+    // pass has run faster on fewer CUs than on all of them)
+    {
+        const unsigned ncu = (unsigned)b->ctx->c.ncu;
+        const unsigned cand[3] = {ncu, ncu * 7 / 8, ncu * 3 / 4};
+        int best_g = 0;
+        for (int g = 0; g < 3; g++) {
+            b->grid_cand[g] = cand[g];
+            b->ntt_grid = cand[g];
+            SALSAA_TRY(time_ms(b, [&] { return run_fwd_planes(b, 1); }, 2, &b->t_grid[g]));
+            if (b->t_grid[g] < b->t_grid[best_g]) best_g = g;
+        }
+        b->ntt_grid = cand[best_g];
+    }
     // the norm checks as the step runs them: after a round trip (of the
     // still empty, short witnesses), from the short copy
     SALSAA_TRY(run_fwd_planes(b, 1));
@@ -838,6 +857,30 @@ int salsaa_batch_set_ntt_schedule(salsaa_batch *b, int xch) {
     b->exec = nullptr;
     b->graph = nullptr;
     b->ntt_xch = xch;
+    return SALSAA_OK;
+}
+
+int salsaa_batch_tuning_grid(const salsaa_batch *b, unsigned *blocks, unsigned *cand, float *ms) {
+    SALSAA_CHECK(b != nullptr, "null batch");
+    if (blocks) *blocks = b->ntt_grid ? b->ntt_grid : (unsigned)b->ctx->c.ncu;
+    for (int g = 0; g < 3; g++) {
+        if (cand) cand[g] = b->grid_cand[g];
+        if (ms) ms[g] = b->t_grid[g];
+    }
+    return SALSAA_OK;
+}
+
+int salsaa_batch_set_ntt_grid(salsaa_batch *b, unsigned blocks) {
+    SALSAA_CHECK(b != nullptr, "null batch");
+    SALSAA_CHECK(blocks < 65536, "too many persistent blocks");
+    CTX_OK(b->ctx);
+    // the captured step launches the old grid: capture again on next use
+    SALSAA_HIP(hipStreamSynchronize(b->ctx->c.stream));
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->ntt_grid = blocks;
     return SALSAA_OK;
 }
 

@@ -382,10 +382,11 @@ __device__ __forceinline__ uint32_t short_bad(uint64_t t) {
 // elements kPlaneTile i ..), written under `planes`.  roundtrip: the inverse
 // IncompleteNTT follows in the same pass, so the arena ends in Coefficient
 // form again (the forward + inverse NTT pair of a prover step); with `sc`
-// the pass also writes the arena's short copy.
+// the pass also writes the arena's short copy.  grid: persistent blocks (0:
+// as many as the CUs hold).
 int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *planes,
                           const PlaneTile *tiles, int roundtrip, const ShortCopy *sc = nullptr,
-                          int xch = 0);
+                          int xch = 0, unsigned grid = 0);
 int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
                        size_t a_ktiles, const int8_t *WB, size_t r, size_t m, size_t ktW,
                        uint64_t *out);

@@ -1025,7 +1025,8 @@ int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse
 }
 
 int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *planes,
-                          const PlaneTile *tiles, int roundtrip, const ShortCopy *sc, int xch) {
+                          const PlaneTile *tiles, int roundtrip, const ShortCopy *sc, int xch,
+                          unsigned grid) {
     if (count == 0) return SALSAA_OK;
     SALSAA_CHECK(count % kPlaneTile == 0, "fused planes need whole tiles");
     SALSAA_CHECK(sc == nullptr || (roundtrip && sc->t && sc->flag),
@@ -1048,7 +1049,7 @@ int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *pl
             n = 1;
         o = n;
     }
-    const unsigned g = (unsigned)std::min<size_t>(nt, (size_t)ctx->ncu * o);
+    const unsigned g = (unsigned)std::min<size_t>(nt, grid ? (size_t)grid : (size_t)ctx->ncu * o);
     if (roundtrip && sc && dpp)
         hipLaunchKernelGGL((k_ntt_rt_planes<true, true>), dim3(g), dim3(NT), 0, ctx->stream, data,
                            nt, ctx->d_tables, M, planes, tiles, sc->t, sc->flag);

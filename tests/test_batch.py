@@ -287,6 +287,29 @@ def test_batch_split_pass_equals_single_pass(sa, ctx, orc):
     key.close()
 
 
+def test_batch_ntt_grid(sa, ctx, orc):
+    # The step's NTT pass runs on any number of persistent blocks (the tuner
+    # picks the fastest of one per CU, 7/8 and 3/4 of that): every grid gives
+    # the plain path's outputs.
+    say("case: batch NTT pass grids")
+    m = [2048, 1024, 1024, 512, 256, 256, 128, 64, 16, 2]
+    kappa = [32, 8, 16, 32, 8, 16, 32, 8, 16, 8]
+    A_h = orc.sample_uniform(13, sa.STREAM_A, (32, 2048, 64), Q)
+    key = ctx.key(ctx.to_device(A_h))
+    b = sa.Batch(ctx, key, m, kappa, reps=2, fq2=1, tune=False)
+    fill(sa, ctx, b, 1700)
+    b.step_plain()
+    plain = snapshot(b, True)
+    for blocks in (0, 200, 7, 1):
+        b.set_ntt_grid(blocks)
+        for graph in (False, True, True):
+            b.step(graph=graph)
+            ctx.sync()
+            same(snapshot(b, False), plain)
+    b.close()
+    key.close()
+
+
 def test_batch_with_generated_crs_key(sa, ctx, orc):
     # the batched step over a CRS key kept as its module only (the key rows
     # made in the batched commit kernel): eager step and graph replays = the
