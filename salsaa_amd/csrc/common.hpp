@@ -80,7 +80,7 @@ struct Context {
     hipStream_t stream = nullptr;
     double *d_tables = nullptr;     // TAB_SIZE doubles
     std::vector<double> h_tables;   // host copy
-    ScratchBuf scratch[12];         // grown on demand, owned by the context
+    ScratchBuf scratch[14];         // grown on demand, owned by the context
     bool mfma_commit = true;        // int8 MFMA commit (false: fp64 VALU commit)
     bool mfma_project = true;       // int8 MFMA projection (false: VALU signed sums)
     int ntt_xch = 0;                // IncompleteNTT exchange between register phases:
@@ -406,6 +406,9 @@ int launch_commit_finalize(Context *ctx, const double *partial, size_t ksplit, s
                            size_t r, uint64_t *out);
 int launch_radix(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
                  uint64_t radix, uint64_t *d_radix);
+// the radix of each of ninst instances (instance i at W + i r m d) into d_radix[i]
+int launch_radix_n(Context *ctx, const uint64_t *W, size_t ninst, size_t r, size_t m,
+                   size_t parts, uint64_t radix, uint64_t *d_radix);
 int launch_commit_mac(Context *ctx, const double *A_sm, size_t kappa, size_t a_rows, size_t mA,
                       const double *W_sm, size_t r, size_t m, uint64_t *out);
 // One finalize job: partial [ks][64][kappa][rows]; (j, k) -> out + j osj + k osk

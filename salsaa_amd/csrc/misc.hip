@@ -37,10 +37,13 @@ __global__ void k_challenges(uint64_t *__restrict__ cs, size_t reps, size_t nrou
 // ---- decomposition (arithmetic.rs:1461-1593)
 // OR of |x| (pos / neg split) over W; one atomic per block (a grid of a few
 // hundred blocks, so the single accumulator sees little contention)
+// instance blockIdx.y: W + y n, OR into acc[y]
 __global__ __launch_bounds__(256) void k_bitor(const uint64_t *__restrict__ W, size_t n,
                                                uint64_t q, unsigned long long *__restrict__ acc) {
     __shared__ uint64_t part[4];
     uint64_t a = 0;
+    W += (size_t)blockIdx.y * n;
+    acc += blockIdx.y;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         const uint64_t x = W[i];
@@ -55,7 +58,10 @@ __global__ __launch_bounds__(256) void k_bitor(const uint64_t *__restrict__ W, s
     }
 }
 
-__global__ void k_zero1(unsigned long long *p) { *p = 0ULL; }
+__global__ void k_zero_n(unsigned long long *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ULL;
+}
 
 // gadget weight 2^(lg * t) mod q; 0 once the shift leaves 64 bits, as the
 // reference's wrapping 2u64.pow does (arithmetic.rs:1739)
@@ -64,18 +70,20 @@ __device__ __forceinline__ double gadget_weight(unsigned lg, size_t t, uint64_t 
     return s >= 64 ? 0.0 : u2d((1ULL << s) % q);
 }
 
-// radix = 2^ceil(bw / parts) from the OR accumulator (or the explicit radix)
-__global__ void k_set_radix(unsigned long long *__restrict__ acc, uint64_t explicit_radix,
-                            size_t parts, uint64_t *__restrict__ d_radix) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// radix = 2^ceil(bw / parts) from the OR accumulator (or the explicit radix),
+// one per instance
+__global__ void k_set_radix(const unsigned long long *__restrict__ acc, size_t ninst,
+                            uint64_t explicit_radix, size_t parts, uint64_t *__restrict__ d_radix) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ninst) return;
     if (explicit_radix) {
-        *d_radix = explicit_radix;
+        d_radix[i] = explicit_radix;
         return;
     }
-    const uint64_t a = *acc;
+    const uint64_t a = acc[i];
     const unsigned bw = a ? 64u - (unsigned)__clzll((long long)a) : 0u;
     const unsigned lg = (unsigned)((bw + parts - 1) / parts);
-    *d_radix = 1ULL << lg;
+    d_radix[i] = 1ULL << lg;
 }
 
 __global__ void k_decompose(const uint64_t *__restrict__ W, size_t r, size_t md, size_t parts,
@@ -229,22 +237,34 @@ int launch_challenges(Context *ctx, uint64_t *cs, size_t reps, size_t nrounds, i
     return SALSAA_OK;
 }
 
-int launch_radix(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
-                 uint64_t radix, uint64_t *d_radix) {
+int launch_radix_n(Context *ctx, const uint64_t *W, size_t ninst, size_t r, size_t m,
+                   size_t parts, uint64_t radix, uint64_t *d_radix) {
     SALSAA_CHECK(parts >= 1 && parts <= 64, "parts must be in [1, 64]");
     SALSAA_CHECK(radix == 0 || (radix & (radix - 1)) == 0, "radix must be a power of two");
-    const size_t n = r * m * kD;
+    SALSAA_CHECK(ninst >= 1 && ninst < 65536, "instances out of range");
+    const size_t n = r * m * kD;  // per instance
     unsigned long long *acc = nullptr;
-    SALSAA_TRY(scratch_get(ctx, 3, sizeof(unsigned long long), (void **)&acc));
+    SALSAA_TRY(scratch_get(ctx, 3, ninst * sizeof(unsigned long long), (void **)&acc));
     if (radix == 0) {
-        // the OR accumulator starts at zero (kernel write, safe under capture)
-        hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, ctx->stream, acc);
-        const unsigned g = (unsigned)std::min<size_t>(512, (n + 4095) / 4096);
-        hipLaunchKernelGGL(k_bitor, dim3(g ? g : 1), dim3(256), 0, ctx->stream, W, n, ctx->q, acc);
+        // the OR accumulators start at zero (kernel writes, safe under capture)
+        hipLaunchKernelGGL(k_zero_n, dim3(ceil_div(ninst, 256)), dim3(256), 0, ctx->stream, acc,
+                           ninst);
+        // about four blocks per CU over all instances, each taking at least
+        // 4096 residues
+        const unsigned g = (unsigned)std::min<size_t>(
+            std::max<size_t>(1, (size_t)ctx->ncu * 4 / ninst), std::max<size_t>(1, (n + 4095) / 4096));
+        hipLaunchKernelGGL(k_bitor, dim3(g, (unsigned)ninst), dim3(256), 0, ctx->stream, W, n,
+                           ctx->q, acc);
     }
-    hipLaunchKernelGGL(k_set_radix, dim3(1), dim3(1), 0, ctx->stream, acc, radix, parts, d_radix);
+    hipLaunchKernelGGL(k_set_radix, dim3(ceil_div(ninst, 256)), dim3(256), 0, ctx->stream, acc,
+                       ninst, radix, parts, d_radix);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
+}
+
+int launch_radix(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
+                 uint64_t radix, uint64_t *d_radix) {
+    return launch_radix_n(ctx, W, 1, r, m, parts, radix, d_radix);
 }
 
 int launch_decompose(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,

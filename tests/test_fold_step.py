@@ -28,14 +28,29 @@ def ctx(sa):
     c.close()
 
 
-@pytest.mark.parametrize("n,r,m,parts,r_out,bound,kappa", [(3, 8, 128, 4, 8, 1 << 12, 16),
-                                                          (2, 3, 256, 5, 5, 1000, 9)])
+# C: "ternary" (the reference's fold challenges: the matrix-core fold when the
+# radix is at most 128), "uniform" (the NTT-domain fold), "mixed" (instance 0
+# uniform, the others ternary: both kernels in one step).  parts = 1 with
+# 2^12-bounded witnesses gives radix 2^13, which also takes the NTT domain.
+@pytest.mark.parametrize("n,r,m,parts,r_out,bound,kappa,cmode", [
+    (3, 8, 128, 4, 8, 1 << 12, 16, "ternary"),
+    (2, 3, 256, 5, 5, 1000, 9, "ternary"),
+    (2, 3, 256, 5, 5, 1000, 9, "uniform"),
+    (3, 4, 128, 2, 3, 1 << 12, 8, "mixed"),
+    (2, 2, 128, 1, 4, 1 << 12, 8, "ternary"),
+    (2, 8, 256, 8, 8, (1 << 20) - 1, 8, "ternary"),
+])
 def test_fold_step_vs_separate_calls_and_oracle(sa, ctx, orc, n, r, m, parts, r_out, bound,
-                                                kappa):
-    say("case: fold step n=%d r=%d m=%d parts=%d r_out=%d" % (n, r, m, parts, r_out))
+                                                kappa, cmode):
+    say("case: fold step n=%d r=%d m=%d parts=%d r_out=%d C %s" % (n, r, m, parts, r_out, cmode))
     reps, nrounds = 2, (m * 64).bit_length() - 1
     W_h = orc.sample_bounded(21, sa.STREAM_W, (n * r, m, 64), bound, Q)
-    C_h = orc.sample_bounded(22, sa.STREAM_FOLD, (n, r_out, r * parts, 64), 1, Q)
+    if cmode == "uniform":
+        C_h = orc.sample_uniform(22, sa.STREAM_FOLD, (n, r_out, r * parts, 64), Q)
+    else:
+        C_h = orc.sample_bounded(22, sa.STREAM_FOLD, (n, r_out, r * parts, 64), 1, Q)
+    if cmode == "mixed":
+        C_h[0] = orc.sample_uniform(24, sa.STREAM_FOLD, C_h[0].shape, Q)
     A_h = orc.sample_uniform(23, sa.STREAM_A, (kappa, m, 64), Q)
     cs_h = np.stack([np.stack([orc.sample_uniform(30 + i, 0x5C000 + rep, (nrounds * 2,), Q)
                                .reshape(nrounds, 2) for rep in range(reps)]) for i in range(n)])
@@ -65,13 +80,16 @@ def test_fold_step_vs_separate_calls_and_oracle(sa, ctx, orc, n, r, m, parts, r_
     assert np.array_equal(coeff, g["folded_coeff"])
     com = ctx.parallel_dot_series_matrix(key, ctx.to_device(coeff)).download()
     assert np.array_equal(com, g["rhs"])
-    # oracle: decomposition + fold of instance 0, commitment of its rows
-    limbs, radix = orc.decompose(W_h[:r], parts, Q)
-    assert radix == int(g["radix"][0])
-    want_fold = orc.fold(limbs, C_h[0], Q, threads=8)
-    assert np.array_equal(want_fold, g["folded"][:r_out])
-    want_com = orc.commit(A_h, coeff[:r_out], Q, threads=8)
-    assert np.array_equal(want_com, g["rhs"][:, :r_out])
+    # oracle: decomposition + fold of the first and last instance, commitment
+    # of their rows
+    for i in sorted({0, n - 1}):
+        limbs, radix = orc.decompose(W_h[i * r:(i + 1) * r], parts, Q)
+        assert radix == int(g["radix"][i])
+        want_fold = orc.fold(limbs, C_h[i], Q, threads=8)
+        sl = slice(i * r_out, (i + 1) * r_out)
+        assert np.array_equal(want_fold, g["folded"][sl]), i
+        want_com = orc.commit(A_h, coeff[sl], Q, threads=8)
+        assert np.array_equal(want_com, g["rhs"][:, sl]), i
     key.close()
 
 
