@@ -37,18 +37,37 @@ __global__ void k_challenges(uint64_t *__restrict__ cs, size_t reps, size_t nrou
 // ---- decomposition (arithmetic.rs:1461-1593)
 // OR of |x| (pos / neg split) over W; one atomic per block (a grid of a few
 // hundred blocks, so the single accumulator sees little contention)
-// instance blockIdx.y: W + y n, OR into acc[y]
+// instance blockIdx.y: W + y n, OR into acc[y] (of the magnitudes min(x, q - x)).
+// Each thread keeps four 16-byte loads in flight per iteration.
+__device__ __forceinline__ uint64_t mag(uint64_t x, uint64_t q) { return (2 * x > q) ? q - x : x; }
 __global__ __launch_bounds__(256) void k_bitor(const uint64_t *__restrict__ W, size_t n,
                                                uint64_t q, unsigned long long *__restrict__ acc) {
     __shared__ uint64_t part[4];
     uint64_t a = 0;
     W += (size_t)blockIdx.y * n;
     acc += blockIdx.y;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const uint64_t x = W[i];
-        a |= (2 * x > q) ? q - x : x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // whole 16-byte pairs when the instance starts 16-byte aligned
+    const bool al = ((reinterpret_cast<uintptr_t>(W) & 15) == 0);
+    size_t done = 0;
+    if (al) {
+        const ulonglong2 *W2 = reinterpret_cast<const ulonglong2 *>(W);
+        const size_t n2 = n / 2;
+        size_t i = t0;
+        for (; i + 3 * stride < n2; i += 4 * stride) {
+            const ulonglong2 v0 = W2[i], v1 = W2[i + stride], v2 = W2[i + 2 * stride],
+                             v3 = W2[i + 3 * stride];
+            a |= mag(v0.x, q) | mag(v0.y, q) | mag(v1.x, q) | mag(v1.y, q) | mag(v2.x, q) |
+                 mag(v2.y, q) | mag(v3.x, q) | mag(v3.y, q);
+        }
+        for (; i < n2; i += stride) {
+            const ulonglong2 v = W2[i];
+            a |= mag(v.x, q) | mag(v.y, q);
+        }
+        done = 2 * n2;
     }
+    for (size_t i = done + t0; i < n; i += stride) a |= mag(W[i], q);
     for (int off = 32; off >= 1; off >>= 1) a |= __shfl_xor(a, off, 64);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = a;
     __syncthreads();
