@@ -448,7 +448,11 @@ int salsaa_batch_set_ntt_grid(salsaa_batch *b, unsigned blocks);
  * (parallel_dot_series_matrix, arithmetic.rs:459-505) and norm-checked row by
  * row, instance i with its own challenges cs [n][reps][log2(m d)][2]
  * (fast_norm_check.rs:282-462).  Decomposition and fold are one fused pass:
- * the limbs never reach HBM.  Requires r <= 8, r*parts <= 64, r_out <= 8 and
+ * the limbs never reach HBM.  An instance whose C has coefficients in
+ * {-1, 0, 1} (the reference's fold challenges, CyclotomicRing::random_biased,
+ * cyclotomic_ring.rs:319-339) and whose radix is at most 128 folds on the int8
+ * matrix cores in the Coefficient domain; any other instance in the NTT
+ * domain (the same outputs).  Requires r <= 8, r*parts <= 64, r_out <= 8 and
  * m a power of two, at least 128. */
 typedef struct {
     uint64_t *folded;       /* [n*r_out][m][d] IncompleteNTT (the fold output)  */
