@@ -229,9 +229,6 @@ __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) 
 typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int GR_X = SC_THREADS;  // columns per iteration
 constexpr int GR_LD = GR_X + 2;   // LDS row stride (u64): fragment reads hit distinct banks
-// k_sc_gram_rest's per-wave staging row stride (u32): unpadded, so that its
-// LDS stays below k_sc_gram_short's and fits beside the batched step's NTT pass
-constexpr int GW_LD = 64;
 constexpr int GR_MAXREP = 4;      // repetitions folded per launch
 constexpr uint64_t GR_BIAS = 0x0080808080808080ULL;
 constexpr uint32_t GR_SMALL = kShortOff;  // short path: balanced |b| < 2^14, t = b + 2^14
@@ -446,13 +443,9 @@ __device__ __forceinline__ void gr_wide_frags(const uint64_t *src, v4i (&f)[7]) 
     }
 }
 
-// The first pass over a row's columns, shared by k_sc_gram (NARROW = false:
-// one 16 x GR_LD u64 staging block, for calls without a short copy) and
-// k_sc_gram_rest (NARROW = true: per wave 16 rows of GW_LD u32, which hold the
-// short path's t or half of the wave's columns of wide residues, so that it
-// fits beside the batched step's NTT pass).  Tall: the staging rows, reused
-// for the Gram partials after the loop.
-template <bool FQ2, bool SNR, bool GRAM, bool NARROW>
+// The first pass over a row's columns (k_sc_gram): one 16 x GR_LD u64
+// staging block (Tall), reused for the Gram partials after the loop.
+template <bool FQ2, bool SNR, bool GRAM>
 __device__ __forceinline__ void sc_gram_body(
     uint32_t *Tall, GrCoef &C, const uint64_t *__restrict__ w,
     const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
@@ -464,12 +457,11 @@ __device__ __forceinline__ void sc_gram_body(
     const size_t base = wrow ? wrow[row] : (size_t)row * L;
     const size_t span = (size_t)cpt * GR_X;
     const uint64_t *wr = w + base;
-    uint64_t *T = reinterpret_cast<uint64_t *>(Tall);  // NARROW = false: row y at T + GR_LD y
-    // the short path's u32 rows of the wave's columns: NARROW = false, row y
-    // at T32 + 2 GR_LD y (the first half of the row's 64 u64 slots)
-    const int ld32 = NARROW ? GW_LD : 2 * GR_LD;
-    uint32_t *T32 = NARROW ? Tall + 16 * GW_LD * wave : Tall + 128 * wave;
-    uint64_t *T64 = reinterpret_cast<uint64_t *>(T32);  // NARROW: a half's u64 rows
+    uint64_t *T = reinterpret_cast<uint64_t *>(Tall);  // row y at T + GR_LD y
+    // the short path's u32 rows of the wave's columns: row y at T32 + 2 GR_LD y
+    // (the first half of the row's 64 u64 slots)
+    const int ld32 = 2 * GR_LD;
+    uint32_t *T32 = Tall + 128 * wave;
     const int fr = lane & 15, fg = lane >> 4;
     const uint64_t qi = (uint64_t)M.q, half = qi >> 1;
     const uint64_t off_neg = (uint64_t)GR_SMALL - qi;
@@ -517,7 +509,7 @@ __device__ __forceinline__ void sc_gram_body(
             // both paths then run the same 49 products (kept unconditional:
             // accumulators that only one branch updates cost copies instead)
             v4i f[7];
-            if (small || !NARROW) {
+            {
                 if (small) {
 #pragma unroll
                     for (int y = 0; y < 16; y++)
@@ -542,37 +534,11 @@ __device__ __forceinline__ void sc_gram_body(
 #pragma unroll
                     for (int i = 2; i < 7; i++) f[i] = (v4i){0, 0, 0, 0};
                 }
-            } else if (!NARROW) {
+            } else {
                 if (live) gr_wide_fold<FQ2>(C, T + tid, GR_LD, nrep, rep0, rows, row, S, x, out, M);
                 // lane 16 g + r takes digit row (i, r) of the wave's columns
                 // 16 g .. 16 g + 15: only the wave has to be in step around T
                 if (GRAM) gr_wide_frags(T + fr * GR_LD + 64 * wave + 16 * fg, f);
-            } else {
-                // The wave's 64 columns in two halves of 32: lanes 32 h ..
-                // 32 h + 31 stage their columns and fold them from the rows,
-                // then lane 16 g + r (g = 2 h, 2 h + 1) takes digit row (i, r)
-                // of columns 16 g .. 16 g + 15.  The next loads go out once the
-                // second half is staged.
-#pragma unroll
-                for (int hf = 0; hf < 2; hf++) {
-                    const bool mine = (lane >> 5) == hf;
-                    if (mine) {
-#pragma unroll
-                        for (int y = 0; y < 16; y++)
-                            T64[(GW_LD / 2) * y + (lane & 31)] = live ? v[y] : 0;
-                    }
-                    if (hf == 1) {
-#pragma unroll
-                        for (int y = 0; y < 16; y++) v[y] = wr[(size_t)y * S + xn];
-                    }
-                    wave_sync();
-                    if (mine && live)
-                        gr_wide_fold<FQ2>(C, T64 + (lane & 31), GW_LD / 2, nrep, rep0, rows, row, S,
-                                          x, out, M);
-                    if (GRAM && (fg >> 1) == hf)
-                        gr_wide_frags(T64 + fr * (GW_LD / 2) + 16 * (fg & 1), f);
-                    if (hf == 0) wave_sync();  // the rows are restaged
-                }
             }
             wave_sync();  // the rows are rewritten by the next iteration
             if (GRAM) {
@@ -603,26 +569,90 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     const double *__restrict__ tab, Mod M) {
     __shared__ __align__(16) uint64_t T[16 * GR_LD];
     __shared__ GrCoef C;
-    sc_gram_body<FQ2, SNR, GRAM, false>(reinterpret_cast<uint32_t *>(T), C, w, wrow, L, rows, rep0,
-                                        nrep, cs0, cs_row, nrounds, nr, cpt, out, gacc, tab, M,
-                                        nullptr, 1, (int)gridDim.x);
+    sc_gram_body<FQ2, SNR, GRAM>(reinterpret_cast<uint32_t *>(T), C, w, wrow, L, rows, rep0, nrep,
+                                 cs0, cs_row, nrounds, nr, cpt, out, gacc, tab, M, nullptr, 1,
+                                 (int)gridDim.x);
 }
 
 // The sub-blocks of a batch arena that k_sc_gram_short leaves (some tile not
-// short), on a grid that walks the super-blocks; it returns at once when
-// k_sc_gram_short has flagged none (need_long zero), the usual case.
+// short), on a grid whose blocks walk up to 64 iterations of 256 columns.  It
+// returns at once when k_sc_gram_short has flagged none (need_long zero), the
+// usual case, and it is kept small in registers so that its blocks are
+// dispatched beside the other norm checks' kernels: each column folds from
+// registers and adds its 136 Gram products v_y v_y' (y <= y', exact mod q)
+// into the block's u64 sums in LDS (at most 2^14 columns per block, so each
+// sum of residues below 2^50 stays below 2^64), which go to gacc at the end.
 template <bool FQ2, bool SNR, bool GRAM>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_rest(
     const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
     int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
-    double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc,
-    const double *__restrict__ tab, Mod M, const uint32_t *__restrict__ shf, int sbn, int nbx,
+    double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc, Mod M,
+    const uint32_t *__restrict__ shf, int sbn, int nbx,
     const unsigned long long *__restrict__ need_long) {
-    __shared__ __align__(16) uint32_t Tall[4 * 16 * GW_LD];
     __shared__ GrCoef C;
+    __shared__ unsigned long long Gs[16 * 16];
     if (*need_long == 0) return;  // uniform
-    sc_gram_body<FQ2, SNR, GRAM, true>(Tall, C, w, wrow, L, rows, rep0, nrep, cs0, cs_row, nrounds,
-                                       nr, cpt, out, gacc, tab, M, shf, sbn, nbx);
+    const int row = blockIdx.y, tid = threadIdx.x;
+    const size_t S = L >> 4;
+    const size_t base = wrow ? wrow[row] : (size_t)row * L;
+    const size_t span = (size_t)cpt * GR_X;
+    const uint64_t *wr = w + base;
+    bool ready = false;
+    for (size_t bxi = blockIdx.x; bxi < (size_t)nbx; bxi += gridDim.x) {
+        const size_t sb0 = bxi * sbn;
+        uint32_t todo = 0;
+        for (int k = 0; k < sbn && (sb0 + k) * span < S; k++)
+            if (!gr_compact(shf, base, S, (sb0 + k) * span, cpt)) todo |= 1u << k;
+        if (todo == 0) continue;  // uniform
+        if (!ready) {
+            gr_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
+            Gs[tid] = 0ULL;
+            __syncthreads();
+            ready = true;
+        }
+        for (int k = 0; k < sbn; k++) {
+            if (!((todo >> k) & 1)) continue;
+            for (int cc = 0; cc < cpt; cc++) {
+                const size_t x = (sb0 + k) * span + (size_t)cc * GR_X + tid;
+                if (x >= S) continue;
+                uint64_t v[16];
+#pragma unroll
+                for (int y = 0; y < 16; y++) v[y] = wr[(size_t)y * S + x];
+                for (int r = 0; r < nrep; r++) {
+                    // products |coef v| <= 0.69 q (balanced coef, canonical v):
+                    // eight of them stay below 5.6 q, then two lazy reductions
+                    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+#pragma unroll
+                    for (int y = 0; y < 8; y++) {
+                        const F2 c0 = C.coef[r][y], c1 = C.coef[r][y + 8];
+                        const double v0 = u2d(v[y]), v1 = u2d(v[y + 8]);
+                        a0 += mul_lazy(c0.a, v0, M);
+                        a1 += mul_lazy(c1.a, v1, M);
+                        if (FQ2) {
+                            b0 += mul_lazy(c0.b, v0, M);
+                            b1 += mul_lazy(c1.b, v1, M);
+                        }
+                    }
+                    double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
+                    o[0] = reduce_lazy(reduce_lazy(a0, M) + a1, M);
+                    o[1] = FQ2 ? reduce_lazy(reduce_lazy(b0, M) + b1, M) : 0.0;
+                }
+                if (GRAM) {
+                    for (int y = 0; y < 16; y++) {
+                        const double vy = u2d(v[y]);
+                        for (int y2 = y; y2 < 16; y2++)
+                            atomicAdd(&Gs[16 * y + y2],
+                                      (unsigned long long)d2u(mulmod(vy, u2d(v[y2]), M)));
+                    }
+                }
+            }
+        }
+    }
+    if (!ready || !GRAM) return;  // uniform
+    __syncthreads();
+    const int y = tid >> 4, y2 = tid & 15;
+    const unsigned long long g = Gs[y <= y2 ? 16 * y + y2 : 16 * y2 + y];
+    atomicAdd(&gacc[(size_t)row * 256 + tid], (unsigned long long)d2u(u64_mod(g, M)));
 }
 
 // The short copy's blocks: the same columns and outputs as k_sc_gram, read
@@ -1185,12 +1215,11 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         dim3 grid(bx, (unsigned)rows);
         // With a short copy k_sc_gram_rest takes the sub-blocks the copy
         // cannot serve, usually none (then each block returns after one
-        // load).  A block walks up to `walk` super-blocks, at most 256
-        // iterations of 64 columns per wave: its int32 digit sums, up to
-        // 7 x 2^14 per column, stay below 2^31.
+        // load).  A block walks up to `walk` super-blocks, at most 64
+        // iterations of 256 columns: its u64 Gram sums stay exact.
         dim3 ggrid = grid;
         if (sc) {
-            const size_t walk = std::max<size_t>(1, 256 / ((size_t)sbn * cpt));
+            const size_t walk = std::max<size_t>(1, 64 / ((size_t)sbn * cpt));
             ggrid.x = (unsigned)ceil_div((size_t)bx, walk);
         }
         for (size_t r0 = 0; r0 < reps; r0 += GR_MAXREP) {
@@ -1208,13 +1237,11 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
             if (sc && r0 == 0)
                 hipLaunchKernelGGL((k_sc_gram_rest<FQ2, SNR, true>), ggrid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
-                                   nr, cpt, bufA, gacc, ctx->d_tables, M, shf, sbn, (int)bx,
-                                   need_long);
+                                   nr, cpt, bufA, gacc, M, shf, sbn, (int)bx, need_long);
             else if (sc)
                 hipLaunchKernelGGL((k_sc_gram_rest<FQ2, SNR, false>), ggrid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
-                                   nr, cpt, bufA, gacc, ctx->d_tables, M, shf, sbn, (int)bx,
-                                   need_long);
+                                   nr, cpt, bufA, gacc, M, shf, sbn, (int)bx, need_long);
             else if (r0 == 0)
                 hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
