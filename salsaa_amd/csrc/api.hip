@@ -99,27 +99,51 @@ bool is_prime_u64(uint64_t n) {
 }
 
 // HEXL-convention tables for n = 32 and n = 64 plus the slot moduli omega.
+// The length-n negacyclic NTT's twiddles (HEXL convention: psi the minimal
+// primitive 2n-th root, tab[fwd_off + i] = psi^brv(i)), their inverses with
+// the last inverse stage's twiddle pre-scaled by n^{-1}, and n^{-1}.
+static std::vector<uint64_t> fill_roots(uint64_t q, int n, std::vector<double> &tab, int fwd_off,
+                                        int inv_off, int ninv_off) {
+    int bits = 0;
+    while ((1 << bits) < n) bits++;
+    uint64_t psi = h_min_root(2 * (uint64_t)n, q);
+    std::vector<uint64_t> roots(n);
+    uint64_t p = 1;
+    for (int i = 0; i < n; i++) {
+        roots[h_brv(i, bits)] = p;
+        p = h_mulmod(p, psi, q);
+    }
+    uint64_t ninv = h_invmod((uint64_t)n, q);
+    for (int i = 0; i < n; i++) {
+        tab[fwd_off + i] = (double)roots[i];
+        tab[inv_off + i] = (double)h_invmod(roots[i], q);
+    }
+    // last inverse stage uses inv(roots[1]) * n^{-1}
+    tab[inv_off + 0] = (double)h_mulmod(h_invmod(roots[1], q), ninv, q);
+    tab[ninv_off] = (double)ninv;
+    return roots;
+}
+
+// Tables of a degree-d context, d > 64 (WideTab layout); *full_ok says
+// whether the length-d Full transform exists (2d | q - 1).
+void build_wide_tables(uint64_t q, int d, std::vector<double> &tab, bool *full_ok) {
+    const WideTab W(d);
+    tab.assign(W.size, 0.0);
+    fill_roots(q, W.n, tab, W.fwd_h, W.inv_h, W.ninv_h);
+    *full_ok = (q - 1) % (2 * (uint64_t)d) == 0;
+    if (*full_ok) fill_roots(q, d, tab, W.fwd_f, W.inv_f, W.ninv_f);
+    // slot moduli: NTT_n of Y evaluates Y at the slot roots psi_d^(2 brv(j) + 1)
+    int bits = 0;
+    while ((1 << bits) < W.n) bits++;
+    const uint64_t psi = h_min_root((uint64_t)d, q);
+    for (int j = 0; j < W.n; j++)
+        tab[W.omega + j] = (double)h_powmod(psi, 2 * h_brv(j, bits) + 1, q);
+}
+
 void build_tables(uint64_t q, std::vector<double> &tab) {
     tab.assign(TAB_SIZE, 0.0);
     auto fill = [&](int n, int fwd_off, int inv_off, int ninv_off) {
-        int bits = 0;
-        while ((1 << bits) < n) bits++;
-        uint64_t psi = h_min_root(2 * (uint64_t)n, q);
-        std::vector<uint64_t> roots(n);
-        uint64_t p = 1;
-        for (int i = 0; i < n; i++) {
-            roots[h_brv(i, bits)] = p;
-            p = h_mulmod(p, psi, q);
-        }
-        uint64_t ninv = h_invmod((uint64_t)n, q);
-        for (int i = 0; i < n; i++) {
-            tab[fwd_off + i] = (double)roots[i];
-            tab[inv_off + i] = (double)h_invmod(roots[i], q);
-        }
-        // last inverse stage uses inv(roots[1]) * n^{-1}
-        tab[inv_off + 0] = (double)h_mulmod(h_invmod(roots[1], q), ninv, q);
-        tab[ninv_off] = (double)ninv;
-        return roots;
+        return fill_roots(q, n, tab, fwd_off, inv_off, ninv_off);
     };
     std::vector<uint64_t> r32 = fill(32, TAB_FWD32, TAB_INV32, TAB_NINV32);
     fill(64, TAB_FWD64, TAB_INV64, TAB_NINV64);
@@ -222,9 +246,11 @@ const char *salsaa_version(void) { return "salsaa_amd 0.1 gfx950"; }
 
 int salsaa_ctx_create(uint64_t q, uint32_t d, salsaa_ctx **out) {
     SALSAA_CHECK(out != nullptr, "null output pointer");
-    SALSAA_CHECK(d == (uint32_t)kD, "device kernels support d = 64 only");
+    SALSAA_CHECK(d == 64 || d == 128 || d == 256 || d == 512,
+                 "device kernels support d = 64, 128, 256 and 512");
     SALSAA_CHECK(q > 2 && q < (1ULL << 50), "q must be below 2^50");
-    SALSAA_CHECK((q - 1) % 128 == 0, "q must be 1 mod 128 (2d-th roots of unity)");
+    SALSAA_CHECK((q - 1) % 128 == 0, "q must be 1 mod 128 (128-th roots of unity)");
+    SALSAA_CHECK((q - 1) % d == 0, "q must be 1 mod d (the IncompleteNTT's d-th roots of unity)");
     SALSAA_CHECK(is_prime_u64(q), "q must be prime");
     salsaa_ctx *c = new (std::nothrow) salsaa_ctx();
     if (!c) return SALSAA_ERR_NOMEM;
@@ -251,13 +277,20 @@ int salsaa_ctx_create(uint64_t q, uint32_t d, salsaa_ctx **out) {
     const char *xpath = getenv("SALSAA_NTT_XCH");
     x.ntt_xch = (xpath && std::string(xpath) == "dpp") ? 1 : 0;
     build_tables(q, x.h_tables);
+    std::vector<double> wtab;
+    if (d > (uint32_t)kD) build_wide_tables(q, (int)d, wtab, &x.full_ok);
     if (hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&x.d_tables, TAB_SIZE * sizeof(double)) != hipSuccess ||
         hipMemcpy(x.d_tables, x.h_tables.data(), TAB_SIZE * sizeof(double),
-                  hipMemcpyHostToDevice) != hipSuccess) {
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        (!wtab.empty() &&
+         (hipMalloc(&x.d_wtab, wtab.size() * sizeof(double)) != hipSuccess ||
+          hipMemcpy(x.d_wtab, wtab.data(), wtab.size() * sizeof(double), hipMemcpyHostToDevice) !=
+              hipSuccess))) {
         set_error("context setup failed: " + std::string(hipGetErrorString(hipGetLastError())));
         if (x.stream) (void)hipStreamDestroy(x.stream);
         if (x.d_tables) (void)hipFree(x.d_tables);
+        if (x.d_wtab) (void)hipFree(x.d_wtab);
         delete c;
         return SALSAA_ERR_HIP;
     }
@@ -265,6 +298,7 @@ int salsaa_ctx_create(uint64_t q, uint32_t d, salsaa_ctx **out) {
     if (st != SALSAA_OK) {
         (void)hipStreamDestroy(x.stream);
         (void)hipFree(x.d_tables);
+        if (x.d_wtab) (void)hipFree(x.d_wtab);
         delete c;
         return st;
     }
@@ -281,6 +315,7 @@ int salsaa_ctx_destroy(salsaa_ctx *ctx) {
     if (ctx->c.zeros.ptr) (void)hipFree(ctx->c.zeros.ptr);
     for (auto &b : ctx->c.slabs) (void)hipFree(b.ptr);
     (void)hipFree(ctx->c.d_tables);
+    if (ctx->c.d_wtab) (void)hipFree(ctx->c.d_wtab);
     (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
     return SALSAA_OK;
@@ -396,8 +431,8 @@ int salsaa_convert(salsaa_ctx *ctx, uint64_t *d_data, size_t count, int from_rep
     // NTT <-> IncompleteNTT goes through Coefficient, as the reference does
     // (cyclotomic_ring.rs:488-490, 519-521)
     if (from_rep != SALSAA_REP_COEFF)
-        SALSAA_TRY(launch_ntt(&ctx->c, d_data, count, from_rep, 1));
-    if (to_rep != SALSAA_REP_COEFF) SALSAA_TRY(launch_ntt(&ctx->c, d_data, count, to_rep, 0));
+        SALSAA_TRY(launch_ntt_any(&ctx->c, d_data, count, from_rep, 1));
+    if (to_rep != SALSAA_REP_COEFF) SALSAA_TRY(launch_ntt_any(&ctx->c, d_data, count, to_rep, 0));
     return SALSAA_OK;
 }
 
@@ -424,7 +459,23 @@ static int key_make(salsaa_ctx *ctx, size_t kappa, size_t m, salsaa_key **out, F
     k->k.a_rows = kappa;
     k->k.a_m = m;
     int s = SALSAA_OK;
-    if (ctx->c.mfma_commit && kappa <= kMfmaMaxKappa) {
+    if (ctx->c.d != kD) {
+        // degree d > 64: digit planes of the doubled rows over the d/2 slot pairs
+        if (2 * kappa > kMfmaMaxKappa) {
+            delete k;
+            set_error("a key at degree d > 64 needs kappa <= 18");
+            return SALSAA_ERR_ARG;
+        }
+        k->k.a_kt = (2 * kappa * 7 + 15) / 16;
+        k->k.a_ktiles = plane_ktiles(m);
+        hipError_t e = hipMalloc(&k->k.d_AB, wide_plane_bytes(ctx->c.d, kappa, m));
+        if (e != hipSuccess) {
+            delete k;
+            set_error(std::string("key allocation: ") + hipGetErrorString(e));
+            return SALSAA_ERR_NOMEM;
+        }
+        s = fill(true, k->k);
+    } else if (ctx->c.mfma_commit && kappa <= kMfmaMaxKappa) {
         k->k.a_kt = (kappa * 7 + 15) / 16;
         k->k.a_ktiles = plane_ktiles(m);
         hipError_t e = hipMalloc(&k->k.d_AB, plane_bytes(kappa, m));
@@ -458,6 +509,7 @@ extern "C" {
 int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t m,
                       salsaa_key **out) {
     return key_make(ctx, kappa, m, out, [&](bool planes, Key &k) {
+        if (ctx->c.d != kD) return launch_wide_planes(&ctx->c, d_A, kappa, m, k.d_AB, k.a_ktiles);
         return planes ? launch_ntt_full_to_planes(&ctx->c, d_A, kappa, m, kappa, m * kD, 0, k.d_AB,
                                                   k.a_ktiles)
                       : launch_ntt_full_to_slotmajor(&ctx->c, d_A, kappa, m, kappa, m * kD, 0,
@@ -467,6 +519,7 @@ int salsaa_key_create(salsaa_ctx *ctx, const uint64_t *d_A, size_t kappa, size_t
 
 int salsaa_key_create_crs(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t m,
                           salsaa_key **out) {
+    CTX_D64(ctx, "salsaa_key_create_crs");
     return key_make(ctx, kappa, m, out, [&](bool planes, Key &k) {
         return planes ? launch_crs(&ctx->c, d_module, kappa, m, 2, k.d_AB, k.a_ktiles)
                       : launch_crs(&ctx->c, d_module, kappa, m, 1, k.d_A, 0);
@@ -475,7 +528,7 @@ int salsaa_key_create_crs(salsaa_ctx *ctx, const uint64_t *d_module, size_t kapp
 
 int salsaa_key_crs_implicit(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa, size_t m,
                             salsaa_key **out) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_key_crs_implicit");
     SALSAA_CHECK(out != nullptr, "null output pointer");
     SALSAA_CHECK(kappa >= 1 && kappa <= kMfmaMaxKappa && m >= 1,
                  "a generated CRS key needs 1 <= kappa <= 36 and m >= 1");
@@ -510,7 +563,7 @@ int salsaa_key_crs_implicit(salsaa_ctx *ctx, const uint64_t *d_module, size_t ka
 
 int salsaa_compute_commitment_keys(salsaa_ctx *ctx, const uint64_t *d_module, size_t kappa,
                                    size_t len, uint64_t *d_out) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_compute_commitment_keys");
     SALSAA_TRY(launch_crs(&ctx->c, d_module, kappa, len, 0, d_out, 0));
     return launch_ntt(&ctx->c, d_out, kappa * len, SALSAA_REP_FULL, 1);
 }
@@ -548,6 +601,18 @@ static int commit_rows(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d
     SALSAA_CHECK(key->k.ctx == &ctx->c, "key belongs to another context");
     SALSAA_CHECK(m >= 1 && m <= key->k.m, "witness longer than the key");
     SALSAA_CHECK(rows >= 1, "commit needs at least one witness row");
+    if (ctx->c.d != kD) {
+        // degree d > 64: the slot-pair commit over plain [rows][m][d] rows
+        SALSAA_CHECK(key->k.d_AB != nullptr, "key has no digit planes");
+        SALSAA_CHECK(row_lo == rows && stride_lo == m * (size_t)ctx->c.d && stride_hi == 0,
+                     "only whole matrices commit at degree d > 64");
+        const size_t ps = plane_ktiles(m);
+        int8_t *wb = nullptr;
+        SALSAA_TRY(scratch_get(&ctx->c, 6, wide_plane_bytes(ctx->c.d, rows, m), (void **)&wb));
+        SALSAA_TRY(launch_wide_planes(&ctx->c, d_W, rows, m, wb, ps));
+        return launch_commit_mfma_pairs(&ctx->c, key->k.d_AB, 2 * key->k.kappa, key->k.a_kt,
+                                        key->k.a_ktiles, wb, 2 * rows, m, ps, d_out);
+    }
     if (key->k.d_mhat) {
         // a generated CRS key: its rows are made inside the commit kernel
         const size_t ps = plane_ktiles(m);
@@ -577,12 +642,12 @@ static int commit_rows(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d
 int salsaa_parallel_dot_series_matrix(salsaa_ctx *ctx, const salsaa_key *key,
                                       const uint64_t *d_W, size_t r, size_t m, uint64_t *d_out) {
     CTX_OK(ctx);
-    return commit_rows(ctx, key, d_W, r, m, r, m * kD, 0, d_out);
+    return commit_rows(ctx, key, d_W, r, m, r, m * (size_t)ctx->c.d, 0, d_out);
 }
 
 int salsaa_complete_power_series(salsaa_ctx *ctx, const uint64_t *d_base, size_t m0,
                                  const uint64_t *d_tensors, size_t n, uint64_t *d_out) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_complete_power_series");
     SALSAA_CHECK(m0 >= 1 && n <= 40 && ((m0 << n) >> n) == m0, "power series size out of range");
     // copies of the base and the tensors in NTT form, where each product is pointwise
     uint64_t *tmp = nullptr;
@@ -600,7 +665,7 @@ int salsaa_complete_power_series(salsaa_ctx *ctx, const uint64_t *d_base, size_t
 
 int salsaa_parallel_dot_single_series_matrix(salsaa_ctx *ctx, const uint64_t *d_layer, size_t m,
                                              const uint64_t *d_W, size_t r, uint64_t *d_out) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_parallel_dot_single_series_matrix");
     SALSAA_CHECK(m >= 1 && r >= 1, "inner product needs m, r >= 1");
     // up to 16 rows: one pass over the layer and the rows in the NTT domain
     if (r <= 16) return launch_dot_series(&ctx->c, d_layer, m, d_W, r, d_out);
@@ -614,7 +679,7 @@ int salsaa_parallel_dot_single_series_matrix(salsaa_ctx *ctx, const uint64_t *d_
 
 int salsaa_project(salsaa_ctx *ctx, const uint64_t *d_W, size_t r, size_t m,
                    const uint8_t *d_is_zero, const uint8_t *d_sign, uint64_t *d_out) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_project");
     return launch_project(&ctx->c, d_W, r, m, d_is_zero, d_sign, d_out);
 }
 
@@ -625,7 +690,7 @@ int salsaa_batch_projections(salsaa_ctx *ctx, const uint64_t *d_P, const uint64_
                              const uint64_t *d_c0, uint64_t *d_projected_lhs,
                              uint64_t *d_witness_lhs, uint64_t *d_projection_rhs,
                              uint64_t *d_witness_rhs, int32_t *d_ok) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_batch_projections");
     SALSAA_CHECK(d_projection_rhs && d_witness_rhs, "null output pointer");
     const size_t cw = r * 256;
     // the reference's series have layers of lengths cw 2^t and 256 2^t only,
@@ -665,7 +730,7 @@ int salsaa_send_cross_terms_before_join(salsaa_ctx *ctx, const uint64_t *d_proje
                                         const uint64_t *d_witness_lhs, size_t m,
                                         const uint64_t *d_W, const uint64_t *d_P, uint64_t *d_l0,
                                         uint64_t *d_l1) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_send_cross_terms_before_join");
     SALSAA_CHECK(d_l0 && d_l1, "null output pointer");
     // the reference takes entry 0 of each inner product: row 0 of each matrix
     SALSAA_TRY(salsaa_parallel_dot_single_series_matrix(ctx, d_projected_lhs, m, d_W, 1, d_l0));
@@ -674,7 +739,7 @@ int salsaa_send_cross_terms_before_join(salsaa_ctx *ctx, const uint64_t *d_proje
 
 int salsaa_split(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_W, size_t r, size_t m,
                  uint64_t *d_out) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_split");
     SALSAA_CHECK(m >= 2 && m % 2 == 0, "split needs an even witness length");
     // row k < r is L_k = W[k][0 .. m/2), row r + k is R_k = W[k][m/2 .. m)
     return commit_rows(ctx, key, d_W, 2 * r, m / 2, r, m * kD, (m / 2) * kD, d_out);
@@ -694,6 +759,8 @@ int salsaa_norm_check_noalloc(salsaa_ctx *ctx, const uint64_t *d_w, size_t rows,
                               uint64_t *d_claims, uint64_t *d_msgs, uint64_t *d_finals,
                               uint64_t *d_evals, uint64_t *d_rhs, int32_t *d_ok) {
     CTX_OK(ctx);
+    SALSAA_CHECK(ctx->c.d == kD || d_rhs == nullptr,
+                 "norm_check_noalloc: at degree d > 64 only without rhs (d_rhs = NULL)");
     SALSAA_CHECK(rows > 0 && reps > 0 && rows < 65536 && reps < 65536,
                  "norm_check_noalloc: rows and reps must be in 1..65535");
     SALSAA_CHECK(L >= 2 && (L & (L - 1)) == 0 && __builtin_ctzll(L) <= 48,
@@ -742,6 +809,12 @@ int salsaa_decomp_ell(salsaa_ctx *ctx, const salsaa_key *key, const uint64_t *d_
     SALSAA_CHECK(key->k.ctx == &ctx->c, "key belongs to another context");
     SALSAA_CHECK(m >= 1 && m <= key->k.m, "witness longer than the key");
     SALSAA_CHECK(r >= 1, "decomp_ell needs at least one witness row");
+    if (ctx->c.d != kD) {
+        // degree d > 64: the limbs, then their slot-pair commit
+        SALSAA_TRY(launch_decompose(&ctx->c, d_W, r, m, parts, radix, d_new_W, d_radix));
+        return commit_rows(ctx, key, d_new_W, r * parts, m, r * parts, m * (size_t)ctx->c.d, 0,
+                           d_rhs);
+    }
     SALSAA_TRY(launch_radix(&ctx->c, d_W, r, m, parts, radix, d_radix));
     // one pass over W writes the limb witness and the limbs' slot-major NTT
     const size_t rows = r * parts;

@@ -723,7 +723,7 @@ extern "C" {
 int salsaa_batch_create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m,
                         const size_t *kappa, size_t reps, int fq2, uint64_t nr, int tune,
                         salsaa_batch **out) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_batch_create");
     SALSAA_CHECK(out != nullptr, "null output pointer");
     salsaa_batch *b = new (std::nothrow) salsaa_batch();
     if (!b) return SALSAA_ERR_NOMEM;

@@ -65,6 +65,30 @@ enum TableOffset : int {
     TAB_SIZE = TAB_ZJ + 64,
 };
 
+// Tables of a context of degree d > 64 (doubles inside ctx->d_wtab), n = d/2:
+// the IncompleteNTT's two length-n transforms, the Full transform of length d
+// (present when 2d divides q - 1) and the slot moduli X^2 - omega_j.
+struct WideTab {
+    int d = 0, n = 0;
+    int fwd_h = 0, inv_h = 0;  // [n] psi_d^brv(i); inverses, [0] pre-scaled by n^{-1}
+    int fwd_f = 0, inv_f = 0;  // [d] psi_2d^brv(i); inverses, [0] pre-scaled by d^{-1}
+    int omega = 0;             // [n] omega_j = NTT_n(Y)[j] = psi_d^(2 brv(j) + 1)
+    int ninv_h = 0, ninv_f = 0;
+    int size = 0;
+    __host__ __device__ explicit WideTab(int deg = 0) {
+        d = deg;
+        n = deg / 2;
+        fwd_h = 0;
+        inv_h = n;
+        fwd_f = 2 * n;
+        inv_f = 2 * n + d;
+        omega = 2 * n + 2 * d;
+        ninv_h = omega + n;
+        ninv_f = ninv_h + 1;
+        size = ninv_f + 1;
+    }
+};
+
 // ---------------------------------------------------------------- context
 struct ScratchBuf {
     void *ptr = nullptr;
@@ -80,7 +104,9 @@ struct Context {
     hipStream_t stream = nullptr;
     double *d_tables = nullptr;     // TAB_SIZE doubles
     std::vector<double> h_tables;   // host copy
-    ScratchBuf scratch[14];         // grown on demand, owned by the context
+    double *d_wtab = nullptr;       // degree-d tables (WideTab(d)) when d > 64
+    bool full_ok = true;            // the Full representation exists at degree d (2d | q - 1)
+    ScratchBuf scratch[16];         // grown on demand, owned by the context
     bool mfma_commit = true;        // int8 MFMA commit (false: fp64 VALU commit)
     bool mfma_project = true;       // int8 MFMA projection (false: VALU signed sums)
     int ntt_xch = 0;                // IncompleteNTT exchange between register phases:
@@ -100,6 +126,8 @@ inline void share_context(const Context &from, Context &to) {
     to.qd = from.qd;
     to.qinv = from.qinv;
     to.d_tables = from.d_tables;
+    to.d_wtab = from.d_wtab;
+    to.full_ok = from.full_ok;
     to.mfma_commit = from.mfma_commit;
     to.mfma_project = from.mfma_project;
     to.ntt_xch = from.ntt_xch;
@@ -298,6 +326,11 @@ __device__ __forceinline__ F2 f2_mul(F2 x, F2 y, double nr, Mod M) {
 
 // ---------------------------------------------------------------- launch helpers
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+inline int log2_deg(const Context *ctx) { return __builtin_ctz((unsigned)ctx->d); }
+// the IncompleteNTT's slot moduli X^2 - omega_j at the context's degree
+inline const double *slot_omega(const Context *ctx) {
+    return ctx->d == kD ? ctx->d_tables + TAB_OMEGA : ctx->d_wtab + WideTab(ctx->d).omega;
+}
 
 // entry points implemented in the kernel files
 int launch_ntt(Context *ctx, uint64_t *data, size_t count, int mode, int inverse);
@@ -390,6 +423,33 @@ int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *pl
 int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
                        size_t a_ktiles, const int8_t *WB, size_t r, size_t m, size_t ktW,
                        uint64_t *out);
+// ---- degrees d > 64 (ring_wide.hip)
+// representation change src -> dst (may alias) of count degree-d elements
+int launch_wide_ntt(Context *ctx, const uint64_t *src, uint64_t *dst, size_t count, int mode,
+                    int inverse);
+// launch_ntt at the context's degree (in place)
+int launch_ntt_any(Context *ctx, uint64_t *data, size_t count, int mode, int inverse);
+// the commit's digit planes of rows [rows][m][d] (Coefficient) at degree d:
+// d/2 slots, value row 2 row + h holding half h of each IncompleteNTT slot
+int launch_wide_planes(Context *ctx, const uint64_t *src, size_t rows, size_t m, int8_t *dst,
+                       size_t ktiles);
+inline size_t wide_plane_bytes(size_t d, size_t rows, size_t m) {
+    return d / 2 * ((2 * rows * 7 + 15) / 16) * plane_ktiles(m) * 1024;
+}
+// the commit at degree d over the d/2 slot pairs: key planes of 2 kappa and
+// witness planes of 2 r value rows (launch_wide_planes); out [kappa][r][d]
+// IncompleteNTT
+int launch_commit_mfma_pairs(Context *ctx, const int8_t *AB, size_t kappa2, size_t a_kt,
+                             size_t a_ktiles, const int8_t *WB, size_t r2, size_t m, size_t ktW,
+                             uint64_t *out);
+int launch_commit_finalize_pairs(Context *ctx, const double *partial, size_t ksplit,
+                                 size_t kappa2, size_t r2, uint64_t *out);
+// norm_check_3's rhs [reps][rows][2][d] at degree d > 64 from plain witness
+// rows, and the verifier's check of its constant term against the finals
+// (clears ok[rep ok_rep + row ok_row] on a mismatch)
+int launch_nc3_wide(Context *ctx, const uint64_t *w, size_t rows, size_t L, const uint64_t *cs,
+                    size_t reps, double nr, bool fq2, const uint64_t *finals, size_t fe_rep,
+                    size_t fe_row, uint64_t *rhs, int32_t *ok, size_t ok_rep, size_t ok_row);
 // table[b] = (job, z * 64 + s) for each of nblk logical blocks; every job's
 // partial sums are left for launch_commit_finalize_batch
 int launch_commit_mfma_batch(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,

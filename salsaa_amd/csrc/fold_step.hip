@@ -457,7 +457,7 @@ int salsaa_fold_step(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const uin
                      size_t r, size_t m, size_t parts, uint64_t radix, const uint64_t *d_C,
                      size_t r_out, const uint64_t *d_cs, size_t reps, int fq2, uint64_t nr,
                      const salsaa_fold_outputs *o) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_fold_step");
     Context *c = &ctx->c;
     SALSAA_CHECK(key && o && d_W && d_C && d_cs, "null argument");
     SALSAA_CHECK(key->k.ctx == c && (key->k.d_AB || key->k.d_mhat),

@@ -213,25 +213,26 @@ __device__ __forceinline__ unsigned logical_block() {
 }
 
 // One commit: W planes with rtB = ceil(7 r / 16) tiles per K step; block
-// (nb, s, z) owns witness rows 16 nb .. 16 nb + 15 and K chunk z.
+// (nb, s, z) owns witness rows 16 nb .. 16 nb + 15, slot s < nslot and K
+// chunk z.
 template <int TM, int TN, int WN>
 __global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) void k_commit_mfma(
     const int8_t *AB, int kappa, int a_kt, int a_ktiles, const int8_t *WB, int r, int ktW,
-    int kchunk, int ktend, int nbx, double *__restrict__ partial, const double *__restrict__ tab,
-    Mod M) {
+    int kchunk, int ktend, int nbx, int nslot, double *__restrict__ partial,
+    const double *__restrict__ tab, Mod M) {
     __shared__ __align__(16) char lds[CommitShape<TM, TN, WN>::LDS_BYTES];
     const unsigned lid = logical_block();
     const int nb = (int)(lid % (unsigned)nbx);
     const unsigned rest = lid / (unsigned)nbx;
-    const int s = (int)(rest % 64u);
-    const int z = (int)(rest / 64u);
+    const int s = (int)(rest % (unsigned)nslot);
+    const int z = (int)(rest / (unsigned)nslot);
     const int rtB = (7 * r + 15) / 16;
     const int ktb = z * kchunk;
     commit_body<TM, TN, WN>(lds, AB, (size_t)a_ktiles * a_kt * FRAG, a_kt, kappa,
                             WB + (size_t)GT * nb * FRAG, (size_t)ktW * rtB * FRAG, rtB,
                             min(GT, rtB - GT * nb), min(16, r - 16 * nb), s, ktb,
                             min(ktend, ktb + kchunk) - ktb,
-                            partial + ((size_t)z * 64 + s) * kappa * r + 16 * nb, r, tab, M);
+                            partial + ((size_t)z * nslot + s) * kappa * r + 16 * nb, r, tab, M);
 }
 
 // A batch of commits against one key: job j is a block of at most 16 witness
@@ -266,9 +267,13 @@ int occupancy() {
     return occ;
 }
 
+// nslot slots; wide_d = 0: the 64 full-NTT slots of d = 64 (finalized into
+// IncompleteNTT form), else the slot pairs of degree wide_d (kappa and r
+// count the doubled rows, see ring_wide.hip)
 template <int TM, int TN, int WN>
 int run(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt, size_t a_ktiles,
-        const int8_t *WB, size_t r, size_t ktW, size_t ktend, uint64_t *out) {
+        const int8_t *WB, size_t r, size_t ktW, size_t ktend, uint64_t *out, size_t nslot,
+        int wide_d) {
     Mod M{ctx->qd, ctx->qinv};
     const size_t nbx = (r + 15) / 16;
     // K chunks: at most 1024 tiles (2^16 columns, exact int32 sums), an even
@@ -281,7 +286,7 @@ int run(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt, size_t a_ktil
         size_t kc = (ktend + ks - 1) / ks;
         kc += kc & 1;
         const size_t ksn = (ktend + kc - 1) / kc;
-        const size_t nblk = nbx * 64 * ksn;
+        const size_t nblk = nbx * nslot * ksn;
         const double cost = (double)((nblk + slots - 1) / slots) * (double)(kc + 8);
         if (best_ks == 0 || cost < best) {
             best = cost;
@@ -290,13 +295,15 @@ int run(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt, size_t a_ktil
         }
     }
     double *partial = nullptr;
-    SALSAA_TRY(scratch_get(ctx, 7, best_ks * 64 * kappa * r * sizeof(double), (void **)&partial));
-    const size_t nblk = nbx * 64 * best_ks;
+    SALSAA_TRY(scratch_get(ctx, 7, best_ks * nslot * kappa * r * sizeof(double),
+                           (void **)&partial));
+    const size_t nblk = nbx * nslot * best_ks;
     SALSAA_CHECK(nblk < (1u << 31), "commit grid too large");
     hipLaunchKernelGGL((k_commit_mfma<TM, TN, WN>), dim3((unsigned)nblk), dim3(256 * WN), 0,
                        ctx->stream, AB, (int)kappa, (int)a_kt, (int)a_ktiles, WB, (int)r, (int)ktW,
-                       (int)best_kc, (int)ktend, (int)nbx, partial, ctx->d_tables, M);
+                       (int)best_kc, (int)ktend, (int)nbx, (int)nslot, partial, ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());
+    if (wide_d) return launch_commit_finalize_pairs(ctx, partial, best_ks, kappa, r, out);
     return launch_commit_finalize(ctx, partial, best_ks, kappa, r, out);
 }
 
@@ -557,11 +564,16 @@ int run_crs_n(Context *ctx, const uint64_t *mhat, size_t kappa, const int8_t *WB
 
 template <int TM>
 int run_n(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt, size_t a_ktiles,
-          const int8_t *WB, size_t r, size_t ktW, size_t ktend, uint64_t *out) {
+          const int8_t *WB, size_t r, size_t ktW, size_t ktend, uint64_t *out, size_t nslot,
+          int wide_d) {
     const size_t rtB = (7 * r + 15) / 16;
-    if (rtB <= 2) return run<TM, 1, 2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
-    if (rtB <= 4) return run<TM, 2, 2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
-    return run<TM, 4, 2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
+    if (rtB <= 2)
+        return run<TM, 1, 2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out, nslot,
+                             wide_d);
+    if (rtB <= 4)
+        return run<TM, 2, 2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out, nslot,
+                             wide_d);
+    return run<TM, 4, 2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out, nslot, wide_d);
 }
 
 template <int TM, int TN, int WN>
@@ -588,18 +600,32 @@ int run_batch_n(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt, size_
 
 }  // namespace
 
-int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
-                       size_t a_ktiles, const int8_t *WB, size_t r, size_t m, size_t ktW,
-                       uint64_t *out) {
+static int commit_mfma_any(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
+                           size_t a_ktiles, const int8_t *WB, size_t r, size_t m, size_t ktW,
+                           uint64_t *out, size_t nslot, int wide_d) {
     const size_t ktend = plane_ktiles(m);
     SALSAA_CHECK(kappa >= 1 && kappa <= kMfmaMaxKappa, "MFMA commit needs kappa <= 36");
     SALSAA_CHECK(ktend == ktW && ktend <= a_ktiles, "plane K-tiles do not cover the witness");
     SALSAA_CHECK(a_ktiles < (1u << 24) && r < (1u << 20), "commit too large");
     const size_t rtA = (7 * kappa + 15) / 16;
     SALSAA_CHECK(rtA <= a_kt, "key planes hold fewer rows than kappa");
-    if (rtA <= 4) return run_n<1>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
-    if (rtA <= 8) return run_n<2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
-    return run_n<4>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out);
+    if (rtA <= 4) return run_n<1>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out, nslot, wide_d);
+    if (rtA <= 8) return run_n<2>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out, nslot, wide_d);
+    return run_n<4>(ctx, AB, kappa, a_kt, a_ktiles, WB, r, ktW, ktend, out, nslot, wide_d);
+}
+
+int launch_commit_mfma(Context *ctx, const int8_t *AB, size_t kappa, size_t a_kt,
+                       size_t a_ktiles, const int8_t *WB, size_t r, size_t m, size_t ktW,
+                       uint64_t *out) {
+    return commit_mfma_any(ctx, AB, kappa, a_kt, a_ktiles, WB, r, m, ktW, out, 64, 0);
+}
+
+int launch_commit_mfma_pairs(Context *ctx, const int8_t *AB, size_t kappa2, size_t a_kt,
+                             size_t a_ktiles, const int8_t *WB, size_t r2, size_t m, size_t ktW,
+                             uint64_t *out) {
+    SALSAA_CHECK(ctx->d > kD, "slot-pair commit is for degrees above 64");
+    return commit_mfma_any(ctx, AB, kappa2, a_kt, a_ktiles, WB, r2, m, ktW, out,
+                           (size_t)ctx->d / 2, ctx->d);
 }
 
 int launch_commit_crs(Context *ctx, const uint64_t *mhat, size_t kappa, const int8_t *WB, size_t r,

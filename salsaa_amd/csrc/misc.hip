@@ -127,20 +127,20 @@ __global__ void k_decompose(const uint64_t *__restrict__ W, size_t r, size_t md,
     }
 }
 
-// compose_with_radix_mod: in [rows][n*parts][d] -> out [rows][n][d]
+// compose_with_radix_mod: in [rows][n*parts][d] -> out [rows][n][d], d = 2^logd
 __global__ void k_compose(const uint64_t *__restrict__ in, size_t rows, size_t n, size_t parts,
                           const uint64_t *__restrict__ d_radix, uint64_t *__restrict__ out,
-                          uint64_t q, Mod M) {
+                          uint64_t q, Mod M, int logd) {
     const uint64_t radix = *d_radix;
     const unsigned lg = (unsigned)__ffsll((long long)radix) - 1u;
-    const size_t total = rows * n * kD;
+    const size_t total = rows * n << logd;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (size_t)gridDim.x * blockDim.x) {
-        const size_t c = idx % kD, ri = idx / kD;
+        const size_t c = idx & ((1u << logd) - 1), ri = idx >> logd;
         const size_t row = ri / n, i = ri % n;
         double acc = 0.0;
         for (size_t t = 0; t < parts; t++) {
-            const double x = u2d(in[((row * n * parts) + i * parts + t) * kD + c]);
+            const double x = u2d(in[(((row * n * parts) + i * parts + t) << logd) + c]);
             acc = addmod(acc, mulmod(x, gadget_weight(lg, t, q), M), M);
         }
         out[idx] = d2u(acc);
@@ -153,19 +153,19 @@ __global__ void k_compose(const uint64_t *__restrict__ in, size_t rows, size_t n
 __global__ void k_compose_check(const uint64_t *__restrict__ in, size_t rows, size_t n,
                                 size_t parts, const uint64_t *__restrict__ d_radix,
                                 const uint64_t *__restrict__ expect, int rows_mode,
-                                uint32_t *__restrict__ mismatch, uint64_t q, Mod M) {
+                                uint32_t *__restrict__ mismatch, uint64_t q, Mod M, int logd) {
     const uint64_t radix = *d_radix;
     const unsigned lg = (unsigned)__ffsll((long long)radix) - 1u;
-    const size_t total = rows * n * kD;
+    const size_t total = rows * n << logd;
     uint32_t bad = 0;
     for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (size_t)gridDim.x * blockDim.x) {
-        const size_t c = idx % kD, ri = idx / kD;
+        const size_t c = idx & ((1u << logd) - 1), ri = idx >> logd;
         const size_t row = ri / n, i = ri % n;
         double acc = 0.0;
         for (size_t t = 0; t < parts; t++) {
-            const size_t src = rows_mode ? (((row * parts + t) * n + i) * kD + c)
-                                         : (((row * n * parts) + i * parts + t) * kD + c);
+            const size_t src = rows_mode ? ((((row * parts + t) * n + i) << logd) + c)
+                                         : ((((row * n * parts) + i * parts + t) << logd) + c);
             acc = addmod(acc, mulmod(u2d(in[src]), gadget_weight(lg, t, q), M), M);
         }
         if (d2u(acc) != expect[idx]) bad++;
@@ -174,49 +174,53 @@ __global__ void k_compose_check(const uint64_t *__restrict__ in, size_t rows, si
 }
 
 // ---- IncompleteNTT slot products
-// out[e] = a[e] * b[e], all IncompleteNTT; thread per (element, slot)
+// out[e] = a[e] * b[e], all IncompleteNTT with h = 2^logh slots (d = 2h),
+// slot moduli X^2 - omega[j]; thread per (element, slot)
 __global__ void k_slot_mul(const uint64_t *__restrict__ a, const uint64_t *__restrict__ b,
-                           uint64_t *__restrict__ out, size_t count,
-                           const double *__restrict__ tab, Mod M) {
+                           uint64_t *__restrict__ out, size_t count, int logh,
+                           const double *__restrict__ omega, Mod M) {
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= count * kH) return;
-    const size_t e = gid / kH;
-    const int j = (int)(gid % kH);
-    const uint64_t *ae = a + e * kD, *be = b + e * kD;
-    const double a0 = u2d(ae[j]), a1 = u2d(ae[kH + j]);
-    const double b0 = u2d(be[j]), b1 = u2d(be[kH + j]);
-    const double om = tab[TAB_OMEGA + j];
+    const size_t h = (size_t)1 << logh;
+    if (gid >= count * h) return;
+    const size_t e = gid >> logh;
+    const size_t j = gid & (h - 1);
+    const uint64_t *ae = a + 2 * h * e, *be = b + 2 * h * e;
+    const double a0 = u2d(ae[j]), a1 = u2d(ae[h + j]);
+    const double b0 = u2d(be[j]), b1 = u2d(be[h + j]);
+    const double om = omega[j];
     const double re = reduce(mul_lazy(a0, b0, M) + mul_lazy(mulmod(a1, b1, M), om, M), M);
     const double ro = reduce(mul_lazy(a1, b0, M) + mul_lazy(a0, b1, M), M);
-    out[e * kD + j] = d2u(re);
-    out[e * kD + kH + j] = d2u(ro);
+    out[2 * h * e + j] = d2u(re);
+    out[2 * h * e + h + j] = d2u(ro);
 }
 
-// fold MAC in the IncompleteNTT domain:
+// fold MAC in the IncompleteNTT domain (h = 2^logh slots, d = 2h):
 // out[j][i] = sum_k Wn[k][i] * Cn[j][k]; thread per (j, i, slot)
 __global__ void k_fold_mac(const uint64_t *__restrict__ Wn, size_t r_in, size_t m,
                            const uint64_t *__restrict__ Cn, size_t r_out,
-                           uint64_t *__restrict__ out, const double *__restrict__ tab, Mod M) {
+                           uint64_t *__restrict__ out, int logh, const double *__restrict__ omega,
+                           Mod M) {
     const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= r_out * m * kH) return;
-    const int s = (int)(gid % kH);
-    const size_t ji = gid / kH;
+    const size_t h = (size_t)1 << logh, d = 2 * h;
+    if (gid >= r_out * m * h) return;
+    const size_t s = gid & (h - 1);
+    const size_t ji = gid >> logh;
     const size_t j = ji / m, i = ji % m;
-    const double om = tab[TAB_OMEGA + s];
+    const double om = omega[s];
     double re = 0.0, ro = 0.0;
     for (size_t k = 0; k < r_in; k++) {
-        const uint64_t *we = Wn + (k * m + i) * kD;
-        const uint64_t *ce = Cn + (j * r_in + k) * kD;
-        const double w0 = u2d(we[s]), w1 = u2d(we[kH + s]);
-        const double c0 = u2d(ce[s]), c1 = u2d(ce[kH + s]);
+        const uint64_t *we = Wn + (k * m + i) * d;
+        const uint64_t *ce = Cn + (j * r_in + k) * d;
+        const double w0 = u2d(we[s]), w1 = u2d(we[h + s]);
+        const double c0 = u2d(ce[s]), c1 = u2d(ce[h + s]);
         re += mul_lazy(w0, c0, M) + mul_lazy(mulmod(w1, c1, M), om, M);
         ro += mul_lazy(w1, c0, M) + mul_lazy(w0, c1, M);
         re = reduce_lazy(re, M);
         ro = reduce_lazy(ro, M);
     }
-    uint64_t *o = out + (j * m + i) * kD;
+    uint64_t *o = out + (j * m + i) * d;
     o[s] = d2u(reduce(re, M));
-    o[kH + s] = d2u(reduce(ro, M));
+    o[h + s] = d2u(reduce(ro, M));
 }
 
 unsigned grid_for(size_t n, unsigned block) {
@@ -261,7 +265,7 @@ int launch_radix_n(Context *ctx, const uint64_t *W, size_t ninst, size_t r, size
     SALSAA_CHECK(parts >= 1 && parts <= 64, "parts must be in [1, 64]");
     SALSAA_CHECK(radix == 0 || (radix & (radix - 1)) == 0, "radix must be a power of two");
     SALSAA_CHECK(ninst >= 1 && ninst < 65536, "instances out of range");
-    const size_t n = r * m * kD;  // per instance
+    const size_t n = r * m * (size_t)ctx->d;  // per instance
     unsigned long long *acc = nullptr;
     SALSAA_TRY(scratch_get(ctx, 3, ninst * sizeof(unsigned long long), (void **)&acc));
     if (radix == 0) {
@@ -289,7 +293,7 @@ int launch_radix(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t par
 int launch_decompose(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t parts,
                      uint64_t radix, uint64_t *out, uint64_t *d_radix) {
     SALSAA_TRY(launch_radix(ctx, W, r, m, parts, radix, d_radix));
-    const size_t md = m * kD, n = r * md;
+    const size_t md = m * (size_t)ctx->d, n = r * md;
     hipLaunchKernelGGL(k_decompose, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, W, r, md,
                        parts, d_radix, ctx->q, out);
     SALSAA_HIP(hipGetLastError());
@@ -299,10 +303,10 @@ int launch_decompose(Context *ctx, const uint64_t *W, size_t r, size_t m, size_t
 int launch_compose(Context *ctx, const uint64_t *in, size_t rows, size_t n, size_t parts,
                    const uint64_t *d_radix, uint64_t *out) {
     Mod M{ctx->qd, ctx->qinv};
-    const size_t total = rows * n * kD;
+    const size_t total = rows * n * (size_t)ctx->d;
     if (!total) return SALSAA_OK;
     hipLaunchKernelGGL(k_compose, dim3(grid_for(total, 256)), dim3(256), 0, ctx->stream, in, rows, n,
-                       parts, d_radix, out, ctx->q, M);
+                       parts, d_radix, out, ctx->q, M, log2_deg(ctx));
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }
@@ -311,27 +315,34 @@ int launch_compose_check(Context *ctx, const uint64_t *in, size_t rows, size_t n
                          const uint64_t *d_radix, const uint64_t *expect, int rows_mode,
                          uint32_t *d_mismatch) {
     Mod M{ctx->qd, ctx->qinv};
-    const size_t total = rows * n * kD;
+    const size_t total = rows * n * (size_t)ctx->d;
     if (!total) return SALSAA_OK;
     hipLaunchKernelGGL(k_compose_check, dim3(grid_for(total, 256)), dim3(256), 0, ctx->stream, in,
-                       rows, n, parts, d_radix, expect, rows_mode, d_mismatch, ctx->q, M);
+                       rows, n, parts, d_radix, expect, rows_mode, d_mismatch, ctx->q, M,
+                       log2_deg(ctx));
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
+}
+
+// the IncompleteNTT of count Coefficient elements src into dst
+static int ntt_copy(Context *ctx, const uint64_t *src, uint64_t *dst, size_t count) {
+    if (ctx->d != kD) return launch_wide_ntt(ctx, src, dst, count, SALSAA_REP_INCOMPLETE, 0);
+    SALSAA_HIP(hipMemcpyAsync(dst, src, count * kD * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    return launch_ntt(ctx, dst, count, SALSAA_REP_INCOMPLETE, 0);
 }
 
 int launch_ring_mul(Context *ctx, const uint64_t *a, const uint64_t *b, uint64_t *out,
                     size_t count) {
     if (!count) return SALSAA_OK;
     Mod M{ctx->qd, ctx->qinv};
+    const size_t d = (size_t)ctx->d;
     uint64_t *ta = nullptr;
-    SALSAA_TRY(scratch_get(ctx, 4, 2 * count * kD * sizeof(uint64_t), (void **)&ta));
-    uint64_t *tb = ta + count * kD;
-    SALSAA_HIP(hipMemcpyAsync(ta, a, count * kD * 8, hipMemcpyDeviceToDevice, ctx->stream));
-    SALSAA_HIP(hipMemcpyAsync(tb, b, count * kD * 8, hipMemcpyDeviceToDevice, ctx->stream));
-    SALSAA_TRY(launch_ntt(ctx, ta, count, SALSAA_REP_INCOMPLETE, 0));
-    SALSAA_TRY(launch_ntt(ctx, tb, count, SALSAA_REP_INCOMPLETE, 0));
-    hipLaunchKernelGGL(k_slot_mul, dim3(ceil_div(count * kH, 256)), dim3(256), 0, ctx->stream, ta,
-                       tb, out, count, ctx->d_tables, M);
+    SALSAA_TRY(scratch_get(ctx, 4, 2 * count * d * sizeof(uint64_t), (void **)&ta));
+    uint64_t *tb = ta + count * d;
+    SALSAA_TRY(ntt_copy(ctx, a, ta, count));
+    SALSAA_TRY(ntt_copy(ctx, b, tb, count));
+    hipLaunchKernelGGL(k_slot_mul, dim3(ceil_div(count * d / 2, 256)), dim3(256), 0, ctx->stream,
+                       ta, tb, out, count, log2_deg(ctx) - 1, slot_omega(ctx), M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }
@@ -341,17 +352,16 @@ int launch_fold(Context *ctx, const uint64_t *W, size_t r_in, size_t m, const ui
     if (!r_out || !m) return SALSAA_OK;
     SALSAA_CHECK(r_in >= 1, "fold needs r_in >= 1");
     Mod M{ctx->qd, ctx->qinv};
+    const size_t d = (size_t)ctx->d;
     uint64_t *wn = nullptr;
     const size_t nw = r_in * m, nc = r_out * r_in;
-    SALSAA_TRY(scratch_get(ctx, 5, (nw + nc) * kD * sizeof(uint64_t), (void **)&wn));
-    uint64_t *cn = wn + nw * kD;
-    SALSAA_HIP(hipMemcpyAsync(wn, W, nw * kD * 8, hipMemcpyDeviceToDevice, ctx->stream));
-    SALSAA_HIP(hipMemcpyAsync(cn, C, nc * kD * 8, hipMemcpyDeviceToDevice, ctx->stream));
-    SALSAA_TRY(launch_ntt(ctx, wn, nw, SALSAA_REP_INCOMPLETE, 0));
-    SALSAA_TRY(launch_ntt(ctx, cn, nc, SALSAA_REP_INCOMPLETE, 0));
-    const size_t threads = r_out * m * kH;
+    SALSAA_TRY(scratch_get(ctx, 5, (nw + nc) * d * sizeof(uint64_t), (void **)&wn));
+    uint64_t *cn = wn + nw * d;
+    SALSAA_TRY(ntt_copy(ctx, W, wn, nw));
+    SALSAA_TRY(ntt_copy(ctx, C, cn, nc));
+    const size_t threads = r_out * m * d / 2;
     hipLaunchKernelGGL(k_fold_mac, dim3(ceil_div(threads, 256)), dim3(256), 0, ctx->stream, wn, r_in,
-                       m, cn, r_out, out, ctx->d_tables, M);
+                       m, cn, r_out, out, log2_deg(ctx) - 1, slot_omega(ctx), M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }

@@ -758,7 +758,7 @@ extern "C" int salsaa_protocol(salsaa_ctx *ctx, const uint64_t *d_module, size_t
                                const size_t *parts, int folding, uint64_t seed,
                                uint64_t *d_W_out, size_t w_cap, uint64_t *d_rhs_out,
                                size_t rhs_cap, size_t *shape, int32_t *status, double *times) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_protocol");
     return run_protocol(ctx, d_module, kappa, m, d_W, r, nrounds, parts, folding, seed, d_W_out,
                         w_cap, d_rhs_out, rhs_cap, shape, status, times, nullptr);
 }
@@ -769,7 +769,7 @@ extern "C" int salsaa_protocol_ex(salsaa_ctx *ctx, const uint64_t *d_module, siz
                                   uint64_t *d_W_out, size_t w_cap, uint64_t *d_rhs_out,
                                   size_t rhs_cap, size_t *shape, int32_t *status, double *times,
                                   const salsaa_protocol_opts *opts) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_protocol_ex");
     return run_protocol(ctx, d_module, kappa, m, d_W, r, nrounds, parts, folding, seed, d_W_out,
                         w_cap, d_rhs_out, rhs_cap, shape, status, times, opts);
 }

@@ -584,7 +584,7 @@ extern "C" {
 
 int salsaa_server_create(salsaa_ctx *ctx, const salsaa_key *key, size_t kappa,
                          const salsaa_server_opts *opts, salsaa_server **out) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_server_create");
     SALSAA_CHECK(key && opts && out, "null key, options or output");
     SALSAA_CHECK(key->k.ctx == &ctx->c, "key belongs to another context");
     SALSAA_CHECK(key->k.d_AB != nullptr || key->k.d_mhat != nullptr,

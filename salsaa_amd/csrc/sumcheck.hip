@@ -1322,11 +1322,28 @@ int launch_norm_check_ex(Context *ctx, const uint64_t *w, const unsigned long lo
     SALSAA_CHECK(L >= 2 && (L & (L - 1)) == 0, "norm check length must be a power of two >= 2");
     SALSAA_CHECK(rows < 65536 && reps < 65536, "too many rows or repetitions");
     SALSAA_CHECK(__builtin_ctzll(L) <= 48, "norm check length too large");
-    SALSAA_CHECK(o.rhs == nullptr || L >= (size_t)kD, "norm_check_3 needs L >= d");
+    SALSAA_CHECK(o.rhs == nullptr || L >= (size_t)ctx->d, "norm_check_3 needs L >= d");
     SALSAA_CHECK(nr < ctx->q, "non-residue must be reduced mod q");
     SALSAA_CHECK(o.ok != nullptr, "ok output is required");
     SALSAA_CHECK(cpt_first >= 0 && cpt_first <= 64, "columns per thread out of range");
     SALSAA_CHECK(!fault_claim || L > (size_t)SC_FINAL_MAX, "claim fault needs L > 1024");
+    if (ctx->d != kD && o.rhs != nullptr) {
+        // degree d > 64: the sumcheck, then norm_check_3's rhs and its check
+        SALSAA_CHECK(wrow == nullptr && sc == nullptr && cs_row == 0 && ext == nullptr,
+                     "norm_check_3 at degree d > 64 runs on plain witness rows");
+        NcOut o2 = o;
+        o2.rhs = nullptr;
+        if (!o2.finals) {
+            SALSAA_TRY(scratch_get(ctx, 14, reps * rows * 2 * sizeof(uint64_t),
+                                   (void **)&o2.finals));
+            o2.fe_rep = rows * 2;
+            o2.fe_row = 2;
+        }
+        SALSAA_TRY(launch_norm_check_ex(ctx, w, wrow, rows, L, cs, cs_row, reps, fq2, nr, o2,
+                                        cpt_first, ext, sc, fault_claim));
+        return launch_nc3_wide(ctx, w, rows, L, cs, reps, (double)nr, fq2 != 0, o2.finals,
+                               o2.fe_rep, o2.fe_row, o.rhs, o.ok, o.ok_rep, o.ok_row);
+    }
     if (fq2 && nr < 8)
         return norm_check_impl<true, true>(ctx, w, wrow, rows, L, cs, cs_row, reps, (double)nr, o,
                                            cpt_first, ext, sc, fault_claim);

@@ -560,7 +560,7 @@ int salsaa_norm_check_fs(salsaa_ctx *ctx, const uint64_t *d_w, size_t rows, size
                          int fq2, uint64_t nr, const uint8_t *seed, uint64_t *d_cs,
                          uint64_t *d_claims, uint64_t *d_msgs, uint64_t *d_finals,
                          uint64_t *d_evals, uint64_t *d_rhs, int32_t *d_ok) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_norm_check_fs");
     return norm_check_fs(&ctx->c, d_w, rows, L, reps, fq2, nr, seed, d_cs, d_claims, d_msgs,
                          d_finals, d_evals, d_rhs, d_ok);
 }
@@ -571,7 +571,7 @@ int salsaa_transcript_norm_check(salsaa_ctx *ctx, size_t rows, size_t L, size_t 
                                  const uint64_t *d_cs, const uint64_t *d_finals,
                                  const uint64_t *d_evals, const uint64_t *d_rhs, uint8_t *d_out,
                                  size_t cap, size_t *len) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_transcript_norm_check");
     SALSAA_CHECK(L >= 2 && (L & (L - 1)) == 0, "transcript: length must be a power of two");
     const size_t n = (size_t)__builtin_ctzll(L);
     const uint32_t flags = (fs ? 1u : 0u) | (fq2 ? 2u : 0u) | (d_rhs ? 4u : 0u);
@@ -589,7 +589,7 @@ int salsaa_transcript_prover_step(salsaa_ctx *ctx, size_t kappa, const uint64_t 
                                   const uint64_t *d_finals, const uint64_t *d_evals,
                                   const uint64_t *d_rhs, uint8_t *d_out, size_t cap,
                                   size_t *len) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_transcript_prover_step");
     SALSAA_CHECK(L >= 2 && (L & (L - 1)) == 0, "transcript: length must be a power of two");
     const size_t n = (size_t)__builtin_ctzll(L);
     const uint32_t flags = (fs ? 1u : 0u) | (fq2 ? 2u : 0u) | (d_rhs ? 4u : 0u);
@@ -606,7 +606,7 @@ int salsaa_transcript_protocol(salsaa_ctx *ctx, size_t kappa, size_t nrounds, in
                                uint64_t seed, int32_t status, const size_t *shape,
                                const uint64_t *d_rhs, const uint64_t *d_W, uint8_t *d_out,
                                size_t cap, size_t *len) {
-    CTX_OK(ctx);
+    CTX_D64(ctx, "salsaa_transcript_protocol");
     SALSAA_CHECK(shape != nullptr, "transcript: null shape");
     const size_t r = shape[0], m = shape[1], R = shape[2];
     uint8_t sd[32] = {0};

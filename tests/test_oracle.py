@@ -7,7 +7,7 @@ import random
 import numpy as np
 import pytest
 
-from conftest import D, NR, Q
+from conftest import D, NR, Q, REF_RINGS
 import pyref
 
 Q62 = 4546383823830515713  # reference test modulus (cyclotomic_ring.rs:612)
@@ -87,7 +87,9 @@ def test_roots_table_bitreversed(orc):
             assert roots[j] == pow(psi, i, Q)
 
 
-@pytest.mark.parametrize("n,q", [(2, 17), (4, 17), (8, 65537), (32, Q), (64, Q), (16, Q62)])
+@pytest.mark.parametrize("n,q", [(2, 17), (4, 17), (8, 65537), (32, Q), (64, Q), (16, Q62),
+                                 (64, REF_RINGS[0][1]), (128, REF_RINGS[0][1]),
+                                 (128, REF_RINGS[1][1]), (256, REF_RINGS[2][1])])
 def test_ntt_matches_closed_form(orc, n, q):
     rng = random.Random(n)
     x = rnd(rng, n, q)
@@ -140,7 +142,8 @@ def test_multiplication_same_representation_kat(orc):
     assert orc.from_incomplete(c, 17).tolist() == [0, 3, 4, 3]
 
 
-@pytest.mark.parametrize("d,q", [(4, 17), (8, Q62), (64, Q), (32, Q62)])
+@pytest.mark.parametrize("d,q", [(4, 17), (8, Q62), (64, Q), (32, Q62)] +
+                         [(d, q) for d, q, _ in REF_RINGS])
 def test_incomplete_product_is_negacyclic(orc, d, q):
     rng = random.Random(d)
     for _ in range(4):
@@ -196,20 +199,21 @@ def test_split_commit_kat(orc):
     assert orc.from_incomplete(rhs2[0, 1], Q62).tolist()[0] == 109
 
 
-def test_commit_matches_schoolbook(orc):
+@pytest.mark.parametrize("kappa,r,m,d,q", [(2, 3, 4, 16, Q), (2, 2, 2, 128, REF_RINGS[0][1]),
+                                           (1, 2, 2, 256, REF_RINGS[1][1])])
+def test_commit_matches_schoolbook(orc, kappa, r, m, d, q):
     rng = random.Random(6)
-    kappa, r, m, d = 2, 3, 4, 16
-    A = np.array(rnd(rng, kappa * m * d, Q), np.uint64).reshape(kappa, m, d)
-    W = np.array([rng.randrange(-8, 9) % Q for _ in range(r * m * d)],
+    A = np.array(rnd(rng, kappa * m * d, q), np.uint64).reshape(kappa, m, d)
+    W = np.array([rng.randrange(-8, 9) % q for _ in range(r * m * d)],
                  np.uint64).reshape(r, m, d)
-    out = orc.commit(A, W, Q)
+    out = orc.commit(A, W, q)
     for j in range(kappa):
         for k in range(r):
             acc = [0] * d
             for i in range(m):
-                p = pyref.negacyclic_mul(A[j, i].tolist(), W[k, i].tolist(), Q)
-                acc = [(x + y) % Q for x, y in zip(acc, p)]
-            assert orc.from_incomplete(out[j, k], Q).tolist() == acc
+                p = pyref.negacyclic_mul(A[j, i].tolist(), W[k, i].tolist(), q)
+                acc = [(x + y) % q for x, y in zip(acc, p)]
+            assert orc.from_incomplete(out[j, k], q).tolist() == acc
 
 
 # ---------------------------------------------------------------- sumcheck
@@ -291,24 +295,26 @@ def test_fq2_is_a_field_for_nr5():
 
 
 # ---------------------------------------------------------------- norm_check_3
-def test_norm_check_3_constant_term_is_evaluation(orc):
+@pytest.mark.parametrize("d,q", [(16, Q), (128, REF_RINGS[0][1])])
+def test_norm_check_3_constant_term_is_evaluation(orc, d, q):
     rng = random.Random(12)
-    rows, m, d = 2, 8, 16
-    W = np.array([rng.randrange(-16, 17) % Q for _ in range(rows * m * d)],
+    rows, m = 2, 8
+    nc = d.bit_length() - 1
+    W = np.array([rng.randrange(-16, 17) % q for _ in range(rows * m * d)],
                  np.uint64).reshape(rows, m, d)
-    cs = rnd(rng, 3 + 4, Q)
-    rhs = orc.norm_check_3(W, cs, Q)
-    tens = pyref.eq_weights(cs, Q)
+    cs = rnd(rng, 3 + nc, q)
+    rhs = orc.norm_check_3(W, cs, q)
+    tens = pyref.eq_weights(cs, q)
     for j in range(rows):
-        coeff = orc.from_incomplete(rhs[j], Q).tolist()
-        ip = sum(int(x) * t for x, t in zip(W[j].reshape(-1), tens)) % Q
+        coeff = orc.from_incomplete(rhs[j], q).tolist()
+        ip = sum(int(x) * t for x, t in zip(W[j].reshape(-1), tens)) % q
         assert coeff[0] == ip
         # full ring element: base * sum_i T_ring[i] W_j[i]
-        tr = pyref.eq_weights(cs[:3], Q)
-        tc = pyref.eq_weights(cs[3:], Q)
-        base = orc.conjugate(tc, Q).tolist()
-        v = [sum(tr[i] * int(W[j, i, c]) for i in range(m)) % Q for c in range(d)]
-        assert coeff == pyref.negacyclic_mul(base, v, Q)
+        tr = pyref.eq_weights(cs[:3], q)
+        tc = pyref.eq_weights(cs[3:], q)
+        base = orc.conjugate(tc, q).tolist()
+        v = [sum(tr[i] * int(W[j, i, c]) for i in range(m)) % q for c in range(d)]
+        assert coeff == pyref.negacyclic_mul(base, v, q)
 
 
 def test_norm_check_3_fq2_matches_evaluation(orc):
@@ -678,3 +684,27 @@ def test_full_slots_from_incomplete_halves(orc):
             j = zj[s]
             want = (int(inc[e, j]) + int(zeta[s]) * int(inc[e, 32 + j])) % Q
             assert int(full[e, s]) == want
+
+
+# ---------------------------------------------------------------- the reference's rings
+@pytest.mark.parametrize("d,q,nr", REF_RINGS)
+def test_reference_moduli(d, q, nr):
+    # protocol.rs:28-47: each MOD_Q is a prime below 2^50 with d | q - 1 (the
+    # IncompleteNTT's two length-d/2 negacyclic transforms need a primitive
+    # d-th root of unity); nr is its smallest quadratic non-residue
+    assert q < 1 << 50 and pow(2, q - 1, q) == 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x, e = q - 1, 0
+        while x % 2 == 0:
+            x, e = x // 2, e + 1
+        y = pow(a, x, q)
+        if y in (1, q - 1):
+            continue
+        for _ in range(e - 1):
+            y = y * y % q
+            if y == q - 1:
+                break
+        assert y == q - 1, "composite witness %d" % a
+    assert (q - 1) % d == 0
+    assert all(pow(a, (q - 1) // 2, q) == 1 for a in range(2, nr))
+    assert pow(nr, (q - 1) // 2, q) == q - 1
