@@ -561,6 +561,9 @@ __device__ __forceinline__ void sc_gram_body(
     }
 }
 
+// blocks per row that add to one u64 Gram sum: 2^14 residues below 2^50
+constexpr size_t kGramBlocksMax = 16384;
+
 template <bool FQ2, bool SNR, bool GRAM>
 __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_sc_gram(
     const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
@@ -1015,6 +1018,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
             for (int k = 0; k < 4; k++) msg[tl + r][k] = v[k];
         }
     }
+    __syncthreads();  // every round's message before the verifier replay reads them
 
     // ---- norm_check_3: rhs = base * V (negacyclic), thread (k, i) summing
     // the 16 terms base[j] V[i - j], 16 k <= j < 16 k + 16, of coefficient i
@@ -1202,14 +1206,21 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         int cpt = cpt_first > 0 ? std::min(cpt_first, 8) : 1;
         if (cpt_first <= 0)
             while (cpt < 8 && rows * ceil_div(S, (size_t)GR_X * cpt) > (size_t)ctx->ncu * 3) cpt *= 2;
+        // Every first-pass block adds one residue (< 2^50) to each of its
+        // row's u64 Gram sums, so a row takes at most 2^14 blocks; longer rows
+        // get more columns per thread (a wave's int32 digit sums stay exact
+        // over 64 iterations of 64 columns)
+        while (cpt < 64 && ceil_div(S, (size_t)GR_X * cpt) > kGramBlocksMax) cpt *= 2;
         const size_t nsub = ceil_div(S, (size_t)GR_X * cpt);
         // With a short copy, a block takes up to 8 sub-blocks of cpt GR_X
         // columns (each kernel skips the other's): the fewer, longer blocks
         // amortise the skipped blocks and the per-block setup.  A wave then
-        // sums at most 8 x 8 x 64 columns: its int32 digit sums stay exact.
+        // sums at most 64 iterations of 64 columns: its int32 digit sums stay
+        // exact.
         int sbn = 1;
         if (sc)
-            for (sbn = 8; sbn > 1 && ceil_div(nsub, (size_t)sbn) * rows < (size_t)ctx->ncu * 4;)
+            for (sbn = std::max(1, std::min(8, 64 / cpt));
+                 sbn > 1 && ceil_div(nsub, (size_t)sbn) * rows < (size_t)ctx->ncu * 4;)
                 sbn >>= 1;
         const unsigned bx = (unsigned)ceil_div(nsub, (size_t)sbn);
         dim3 grid(bx, (unsigned)rows);
@@ -1222,6 +1233,8 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
             const size_t walk = std::max<size_t>(1, 64 / ((size_t)sbn * cpt));
             ggrid.x = (unsigned)ceil_div((size_t)bx, walk);
         }
+        SALSAA_CHECK((size_t)bx + (sc ? (size_t)ggrid.x : 0) <= kGramBlocksMax,
+                     "norm check row too long (at most 2^32 coefficients)");
         for (size_t r0 = 0; r0 < reps; r0 += GR_MAXREP) {
             const int nrep = (int)std::min<size_t>(GR_MAXREP, reps - r0);
             // with a short copy, its short blocks first, then the others

@@ -483,6 +483,36 @@ def test_norm_check_mixed_magnitudes(ctx, orc, fq2, L):
             assert np.array_equal(rhs[rep, :, 0], orc.norm_check_3(W3, csd[rep, :, 0], Q))
 
 
+def test_norm_check_row_of_2_30(sa, ctx):
+    # one row of 2^30 coefficients: more first-pass blocks than one u64 Gram
+    # sum can take at the default columns per thread.  A uniform row passes
+    # the verifier's checks; a constant row c has claim L c^2, evaluation c
+    # (the tensor weights sum to 1) and final claim c^2, for F_q and F_{q^2}
+    L = 1 << 30
+    say("case: norm_check one row of L=2^30")
+    w = ctx.uniform((1, L), 41, 0xB)
+    for fq2 in (1, 0):
+        cs = ctx.challenges(1, 30, fq2, 91)
+        outs = ctx.norm_check(w, cs, fq2)
+        assert outs["ok"].download().all(), fq2
+    c = 123456789
+    chunk = ctx.to_device(np.full(1 << 26, c, np.uint64))
+    for k in range(L >> 26):
+        part = sa.DeviceArray.view(ctx, w.ptr + (k << 26) * 8, (1 << 26,))
+        ctx.copy(part, chunk)
+    for fq2 in (1, 0):
+        cs = ctx.challenges(2, 30, fq2, 92)
+        outs = ctx.norm_check(w, cs, fq2)
+        assert outs["ok"].download().all(), fq2
+        assert int(outs["claims"].download()[0]) == L * c * c % Q
+        ev, fin = outs["evals"].download(), outs["finals"].download()
+        for rep in range(2):
+            assert (int(ev[rep, 0, 0]), int(ev[rep, 0, 1])) == (c, 0)
+            assert (int(fin[rep, 0, 0]), int(fin[rep, 0, 1])) == (c * c % Q, 0)
+    w.free()
+    chunk.free()
+
+
 def test_norm_check_small_lengths(ctx, orc):
     for L in (2, 4, 32):
         say("case: norm_check L=%d (no rhs)" % L)
