@@ -681,6 +681,99 @@ def protocol(sa, ctx, runs=2):
     return res
 
 
+# The reference's own parameter sets at their ring degree and modulus
+# (protocol.rs:28-120), with the published CPU times of their commit and
+# norm-check phases in ms (experiments/{A0,A1,A2,B0,B1,B2,C2}.log,
+# results.md:4-12): (name, N, MOD_Q, MODULE_SIZE, WIT_LEN, commit,
+# (norm_check_1, norm_check_2, norm_check_3) or None)
+REF_SETS = (("A0", 128, 1125899904679937, 12, 1 << 19, 79.60, (19.18, 69.51, 30.70)),
+            ("A1", 256, 1125899906840833, 7, 1 << 18, 53.93, None),
+            ("A2", 512, 1125899906822657, 4, 1 << 17, 34.07, None),
+            ("B0", 128, 1125899904679937, 12, 1 << 21, 348.22, (None, 232.61, None)),
+            ("B1", 256, 1125899906840833, 7, 1 << 20, 211.56, None),
+            ("B2", 512, 1125899906822657, 4, 1 << 19, 135.02, None),
+            ("C2", 512, 1125899906822657, 4, 1 << 21, 480.72, (271.60, 897.29, 392.52)))
+
+
+def reference_sets(sa, steps):
+    """At each reference set's ring degree and modulus: the commit of one
+    witness row of m ring elements with coefficients in [-2, 2]
+    (sample_random_short_mat(INIT_WIT_REP = 1, WIT_LEN, 2) and its commit,
+    protocol.rs:132-138) under a uniform key of MODULE_SIZE rows; the forward
+    and inverse IncompleteNTT of that row; and, where the reference published
+    them, the norm check of the row (norm_check_1, one repetition of
+    norm_check_2 with its verifier, norm_check_3; F_q challenges, one row of
+    L = m N coefficients).  The published CPU times are the reference's own
+    (2 x 48-core Xeon 8468, 64 threads); they are context, not the same
+    machine."""
+    res = {}
+    n = max(steps // 4, 3)
+    for name, d, q, kappa, m, pub_commit, pub_nc in REF_SETS:
+        ctx = sa.Context(q, d)
+        A = ctx.uniform((kappa, m, d), 31, sa.STREAM_A)
+        key = ctx.key(A)
+        A.free()
+        W = ctx.bounded((1, m, d), 2, 31, sa.STREAM_W)
+        com = ctx.empty((kappa, 1, d))
+        ctx.parallel_dot_series_matrix(key, W, out=com)
+        ctx.sync()
+        _, ctot, _ = timed(ctx, lambda: ctx.parallel_dot_series_matrix(key, W, out=com), n)
+        cms = ctot / n
+        ktiles = (m + 127) // 128 * 2
+        key_bytes = d // 2 * ((14 * kappa + 15) // 16) * 1024 * ktiles
+        # the forward and inverse IncompleteNTT of the row (round trip)
+        X = ctx.empty((m, d))
+        ctx.copy(X, W)
+        fwd, inv = [], []
+        for _ in range(n):
+            e0, e1, e2 = ctx.event(), ctx.event(), ctx.event()
+            ctx.record(e0)
+            ctx.convert(X, sa.REP_COEFF, sa.REP_INCOMPLETE)
+            ctx.record(e1)
+            ctx.convert(X, sa.REP_INCOMPLETE, sa.REP_COEFF)
+            ctx.record(e2)
+            ctx.sync()
+            fwd.append(ctx.elapsed_ms(e0, e1))
+            inv.append(ctx.elapsed_ms(e1, e2))
+        ntt_bytes = 2 * m * d * 8
+        r = {"N": d, "q": q, "kappa": kappa, "m": m,
+             "commit_ms": round(cms, 3),
+             "commit_key_planes_GB": round(key_bytes / 1e9, 2),
+             "commit_key_planes_GB/s": round(key_bytes / (cms * 1e-3) / 1e9, 1),
+             "published_cpu_commit_ms": pub_commit,
+             "commit_vs_published_cpu": round(pub_commit / cms, 1),
+             "ntt_fwd_ms": round(float(np.median(fwd)), 4),
+             "ntt_inv_ms": round(float(np.median(inv)), 4),
+             "ntt_fwd_hbm_frac": round(ntt_bytes / (float(np.median(fwd)) * 1e-3) / 8e12, 3),
+             "ntt_inv_hbm_frac": round(ntt_bytes / (float(np.median(inv)) * 1e-3) / 8e12, 3),
+             "ntt_round_trip_exact": bool(np.array_equal(X.download()[:4096], W.download()[0, :4096]))}
+        X.free()
+        if pub_nc:
+            L = m * d
+            cs = ctx.challenges(1, L.bit_length() - 1, 0, 5)
+            outs = ctx.norm_check(W, cs, 0)
+            ctx.sync()
+            _, ntot, _ = timed(ctx, lambda: ctx.norm_check(W, cs, 0, outs=outs), n)
+            r["norm_check_L"] = L
+            r["norm_check_ms"] = round(ntot / n, 3)
+            r["norm_check_GB/s"] = round(L * 8 / (ntot / n * 1e-3) / 1e9, 1)
+            r["norm_check_ok"] = bool(outs["ok"].download().all())
+            r["published_cpu_ms"] = {k: v for k, v in zip(
+                ("norm_check_1", "norm_check_2", "norm_check_3"), pub_nc) if v is not None}
+            if all(v is not None for v in pub_nc):
+                r["norm_check_vs_published_cpu"] = round(sum(pub_nc) / (ntot / n), 1)
+            for v in outs.values():
+                if v is not None:
+                    v.free()
+            cs.free()
+        key.close()
+        for x in (W, com):
+            x.free()
+        ctx.close()
+        res[name] = r
+    return res
+
+
 def timed(ctx, fn, k):
     evs = [ctx.event() for _ in range(k + 1)]
     ctx.sync()
@@ -785,6 +878,8 @@ def main():
                     help="skip the projection / batching / cross terms")
     ap.add_argument("--no-crs", action="store_true", help="skip the CRS key generation")
     ap.add_argument("--no-protocol", action="store_true", help="skip the protocol runs")
+    ap.add_argument("--no-ref-sets", action="store_true",
+                    help="skip the reference's parameter sets at their own ring degree")
     ap.add_argument("--eager", action="store_true",
                     help="run the batched steps eagerly instead of replaying the hipGraph "
                          "(for counter collection)")
@@ -931,6 +1026,9 @@ def main():
     if not args.no_protocol:
         progress("protocol runs")
         out["protocol"] = protocol(sa, ctx)
+    if not args.no_ref_sets:
+        progress("reference sets")
+        out["reference_sets"] = reference_sets(sa, args.steps)
     if not args.no_single:
         progress("NTT schedules")
         out["ntt_schedules"] = ntt_schedules(sa, args.steps)

@@ -106,7 +106,7 @@ struct Context {
     std::vector<double> h_tables;   // host copy
     double *d_wtab = nullptr;       // degree-d tables (WideTab(d)) when d > 64
     bool full_ok = true;            // the Full representation exists at degree d (2d | q - 1)
-    ScratchBuf scratch[16];         // grown on demand, owned by the context
+    ScratchBuf scratch[17];         // grown on demand, owned by the context
     bool mfma_commit = true;        // int8 MFMA commit (false: fp64 VALU commit)
     bool mfma_project = true;       // int8 MFMA projection (false: VALU signed sums)
     int ntt_xch = 0;                // IncompleteNTT exchange between register phases:

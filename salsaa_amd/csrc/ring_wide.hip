@@ -176,48 +176,48 @@ int run_wide_ntt(Context *ctx, const uint64_t *src, uint64_t *dst, size_t count,
 // as in the d = 64 plane writers.
 constexpr uint64_t WIDE_DIGIT_BIAS = 0x0080808080808080ULL;
 
-// Fragment-tiled digit planes (the layout of store_planes in ntt8.hip) of
-// the IncompleteNTT of rows [rows][m][d] (Coefficient), over nslot = d/2
-// slots and 2 rows value rows: value row 2 row + h of slot j holds half h of
-// slot j.  Block: 8 consecutive elements i0 .. i0 + 7 of one row, d threads;
-// grid (8 ktiles, rows) covers the K tiles, the columns past m as zeros.
+// Fragment-tiled digit planes (the layout of store_planes in ntt8.hip) from
+// IncompleteNTT elements, over nslot = d/2 slots and 2 rows value rows:
+// value row 2 row + h of slot j holds half h of slot j.  Block (K tile kt0 +
+// x, group of PK_SG slots, row0 + z): the group's two halves of the tile's
+// 64 elements staged in LDS, then thread (slot, half, 8-element run) writes
+// the 8 bytes of its run in each of the 7 digit rows, so 8 lanes fill each
+// 64-byte fragment row.  Element i of the row sits at inc + ((z inc_rs) +
+// i - i_base) d; columns past m are zeros.
+constexpr int PK_SG = 32;
 template <int LOGD>
-__global__ __launch_bounds__(512) void k_wide_planes(const uint64_t *__restrict__ src, size_t m,
-                                                     size_t rows, size_t ktiles,
-                                                     int8_t *__restrict__ dst,
-                                                     const double *__restrict__ wt, Mod M) {
-    constexpr int d = 1 << LOGD, N = d / 2, LOGN = LOGD - 1, TPE = d / 8;
-    __shared__ double x[8 * d];
-    const WideTab W(d);
-    const int tid = threadIdx.x, el = tid / TPE, l = tid % TPE, h = l & 1, u = l >> 1;
-    const size_t row = blockIdx.y, i0 = (size_t)blockIdx.x * 8, i = i0 + el;
-    const bool live = i < m;
-    double v[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = live ? u2d(src[(row * m + i) * d + l + k * TPE]) : 0.0;
-    double *xs = x + el * d + h * N;
-    wide_fwd<LOGN>(v, xs, u, wt + W.fwd_h, M);
-#pragma unroll
-    for (int k = 0; k < 8; k++) xs[8 * u + k] = v[k];
+__global__ __launch_bounds__(512) void k_wide_pack(const uint64_t *__restrict__ inc, size_t inc_rs,
+                                                   size_t i_base, size_t m, size_t rows,
+                                                   size_t row0, size_t kt0, size_t ktiles,
+                                                   int8_t *__restrict__ dst) {
+    constexpr int d = 1 << LOGD, N = d / 2;
+    __shared__ uint64_t st[64][2 * PK_SG + 1];
+    const int tid = threadIdx.x;
+    const size_t kt = kt0 + blockIdx.x, row = row0 + blockIdx.z, i0 = kt * 64;
+    const int j0 = blockIdx.y * PK_SG;
+    const uint64_t *src = inc + (size_t)blockIdx.z * inc_rs * d;
+    for (int c = tid; c < 64 * 2 * PK_SG; c += 512) {
+        const int e = c / (2 * PK_SG), hj = c % (2 * PK_SG);
+        const size_t i = i0 + e;
+        st[e][hj] = i < m ? src[(i - i_base) * d + (hj / PK_SG) * N + j0 + hj % PK_SG] : 0;
+    }
     __syncthreads();
-    // thread (slot j, half hh): the 8 elements' values -> 7 digit rows of 8 bytes
-    const int j = tid >> 1, hh = tid & 1;
+    const int eg = tid & 7, hj = tid >> 3, h = hj / PK_SG, jl = hj % PK_SG;
     uint32_t dg[2][7];
 #pragma unroll
     for (int g = 0; g < 2; g++) {
         uint32_t lo[4], hi[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const uint64_t y =
-                (d2u(x[(4 * g + q) * d + hh * N + j]) + WIDE_DIGIT_BIAS) ^ 0x8080808080808080ULL;
+            const uint64_t y = (st[8 * eg + 4 * g + q][hj] + WIDE_DIGIT_BIAS) ^ 0x8080808080808080ULL;
             lo[q] = (uint32_t)y;
             hi[q] = (uint32_t)(y >> 32);
         }
         bytes_transpose4(lo, hi, dg[g]);
     }
     const size_t rt = (2 * rows * 7 + 15) / 16;
-    int8_t *base = dst + ((size_t)j * ktiles + i0 / 64) * rt * 1024 +
-                   (size_t)(7 * (2 * row + hh)) * 64 + (i0 % 64);
+    int8_t *base = dst + ((size_t)(j0 + jl) * ktiles + kt) * rt * 1024 +
+                   (size_t)(7 * (2 * row + h)) * 64 + 8 * eg;
 #pragma unroll
     for (int q = 0; q < 7; q++)
         __builtin_nontemporal_store((unsigned long long)dg[0][q] | ((unsigned long long)dg[1][q] << 32),
@@ -225,13 +225,35 @@ __global__ __launch_bounds__(512) void k_wide_planes(const uint64_t *__restrict_
 }
 
 template <int LOGD>
-int run_wide_planes(Context *ctx, const uint64_t *src, size_t rows, size_t m, int8_t *dst,
-                    size_t ktiles) {
-    Mod M{ctx->qd, ctx->qinv};
-    hipLaunchKernelGGL((k_wide_planes<LOGD>), dim3((unsigned)(ktiles * 8), (unsigned)rows),
-                       dim3(1 << LOGD), 0, ctx->stream, src, m, rows, ktiles, dst, ctx->d_wtab, M);
+int run_wide_pack(Context *ctx, const uint64_t *inc, size_t inc_rs, size_t i_base, size_t m,
+                  size_t rows, size_t row0, size_t nrows, size_t kt0, size_t nkt, size_t ktiles,
+                  int8_t *dst) {
+    if (nkt == 0) return SALSAA_OK;
+    hipLaunchKernelGGL((k_wide_pack<LOGD>),
+                       dim3((unsigned)nkt, (unsigned)((1 << LOGD) / 2 / PK_SG), (unsigned)nrows),
+                       dim3(512), 0, ctx->stream, inc, inc_rs, i_base, m, rows, row0, kt0, ktiles,
+                       dst);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
+}
+
+int run_pack_any(Context *ctx, const uint64_t *inc, size_t inc_rs, size_t i_base, size_t m,
+                 size_t rows, size_t row0, size_t nrows, size_t kt0, size_t nkt, size_t ktiles,
+                 int8_t *dst) {
+    switch (ctx->d) {
+        case 128:
+            return run_wide_pack<7>(ctx, inc, inc_rs, i_base, m, rows, row0, nrows, kt0, nkt,
+                                    ktiles, dst);
+        case 256:
+            return run_wide_pack<8>(ctx, inc, inc_rs, i_base, m, rows, row0, nrows, kt0, nkt,
+                                    ktiles, dst);
+        case 512:
+            return run_wide_pack<9>(ctx, inc, inc_rs, i_base, m, rows, row0, nrows, kt0, nkt,
+                                    ktiles, dst);
+        default: break;
+    }
+    set_error("no plane writer for this degree");
+    return SALSAA_ERR_ARG;
 }
 
 // A commitment's slots from the per-slot products of the doubled rows,
@@ -288,15 +310,30 @@ int launch_wide_planes(Context *ctx, const uint64_t *src, size_t rows, size_t m,
                        size_t ktiles) {
     if (rows == 0 || m == 0) return SALSAA_OK;
     SALSAA_CHECK(ktiles == plane_ktiles(m), "plane K tiles do not match the length");
-    SALSAA_CHECK(rows < 65536 && ktiles * 8 < (1u << 31), "plane grid out of range");
-    switch (ctx->d) {
-        case 128: return run_wide_planes<7>(ctx, src, rows, m, dst, ktiles);
-        case 256: return run_wide_planes<8>(ctx, src, rows, m, dst, ktiles);
-        case 512: return run_wide_planes<9>(ctx, src, rows, m, dst, ktiles);
-        default: break;
+    SALSAA_CHECK(rows < 65536 && ktiles < (1u << 31), "plane grid out of range");
+    const size_t d = (size_t)ctx->d;
+    // the IncompleteNTT into scratch, then the pack: all rows at once when they
+    // fit a chunk of kWideChunk elements, else row by row in chunks of whole K tiles
+    constexpr size_t kWideChunk = (size_t)1 << 17;
+    uint64_t *inc = nullptr;
+    if (rows * m <= kWideChunk) {
+        SALSAA_TRY(scratch_get(ctx, 16, rows * m * d * sizeof(uint64_t), (void **)&inc));
+        SALSAA_TRY(launch_wide_ntt(ctx, src, inc, rows * m, SALSAA_REP_INCOMPLETE, 0));
+        return run_pack_any(ctx, inc, m, 0, m, rows, 0, rows, 0, ktiles, ktiles, dst);
     }
-    set_error("no plane writer for this degree");
-    return SALSAA_ERR_ARG;
+    SALSAA_TRY(scratch_get(ctx, 16, kWideChunk * d * sizeof(uint64_t), (void **)&inc));
+    for (size_t row = 0; row < rows; row++) {
+        for (size_t i0 = 0; i0 < m; i0 += kWideChunk) {
+            const size_t n = std::min(kWideChunk, m - i0);
+            SALSAA_TRY(launch_wide_ntt(ctx, src + (row * m + i0) * d, inc, n,
+                                       SALSAA_REP_INCOMPLETE, 0));
+            // the row's last chunk also writes the zero columns up to 64 ktiles
+            const size_t kt0 = i0 / 64;
+            const size_t nkt = i0 + n < m ? n / 64 : ktiles - kt0;
+            SALSAA_TRY(run_pack_any(ctx, inc, 0, i0, m, rows, row, 1, kt0, nkt, ktiles, dst));
+        }
+    }
+    return SALSAA_OK;
 }
 
 int launch_commit_finalize_pairs(Context *ctx, const double *partial, size_t ksplit,

@@ -130,6 +130,30 @@ def test_wide_commit_vs_oracle(ring, orc, kappa, r, m, bound):
     key.close()
 
 
+def test_wide_commit_long_witness(ring, orc):
+    # 2^17 ring elements, the commit's K range split into chunks: a witness
+    # whose nonzero elements are the first and the last 128 commits like the
+    # oracle's commit of those 256 elements under the matching key columns
+    ctx, d, q, _ = ring
+    kappa = {128: 12, 256: 7, 512: 4}[d]
+    m, e = 1 << 17, 128
+    say("case: wide commit long witness d=%d kappa=%d m=%d" % (d, kappa, m))
+    dA = ctx.uniform((kappa, m, d), 3, 0xA)
+    key = ctx.key(dA)
+    dA.free()
+    W = np.zeros((1, m, d), np.uint64)
+    W[0, :e] = bounded(orc, (e, d), q, 3, 0xB, 2)
+    W[0, m - e:] = bounded(orc, (e, d), q, 4, 0xB, 2)
+    got = ctx.parallel_dot_series_matrix(key, ctx.to_device(W)).download()
+    A_sel = np.zeros((kappa, 2 * e, d), np.uint64)
+    for j in range(kappa):
+        A_sel[j, :e] = orc.sample_uniform_at(3, 0xA, j * m * d, (e, d), q)
+        A_sel[j, e:] = orc.sample_uniform_at(3, 0xA, (j * m + m - e) * d, (e, d), q)
+    W_sel = np.concatenate([W[:, :e], W[:, m - e:]], axis=1)
+    assert np.array_equal(got, orc.commit(A_sel, W_sel, q, threads=8))
+    key.close()
+
+
 def test_wide_commit_kappa_limit(ring, orc):
     ctx, d, q, _ = ring
     A = ctx.uniform((19, 8, d), 1, 1)
