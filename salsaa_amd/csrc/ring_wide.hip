@@ -113,35 +113,49 @@ __device__ __forceinline__ void wide_inv(double (&v)[8], double *x, int u,
 // Representation change of `count` degree-2^LOGD elements, src -> dst (may
 // be the same buffer): INC the IncompleteNTT, else the Full NTT; INV from
 // that representation to Coefficient.  Each element is handled by d/8
-// threads of one wavefront.
+// threads of one wavefront, a block takes WT/(d/8) elements.  Both
+// directions move data lane-contiguously (element word l + k d/8 for lane
+// l): the forward's last phase hands its outputs over in LDS, the inverse's
+// first phase takes its inputs from there.  (Persistent blocks that load the
+// next group during this one's transform measured slower.)
 template <int LOGD, bool INV, bool INC>
 __global__ __launch_bounds__(WT) void k_wide_ntt(const uint64_t *src, uint64_t *dst, size_t count,
                                                  const double *__restrict__ wt, Mod M) {
     constexpr int d = 1 << LOGD, LOGN = INC ? LOGD - 1 : LOGD, N = 1 << LOGN;
     constexpr int TPE = d / 8, EPB = WT / TPE;
     __shared__ double x[EPB * d];
+    __shared__ double tws[N];  // the transform's twiddles (per-lane indices)
     const WideTab W(d);
     const int tid = threadIdx.x, el = tid / TPE, l = tid % TPE;
     const int h = INC ? (l & 1) : 0, u = INC ? (l >> 1) : l;
     const size_t e = (size_t)blockIdx.x * EPB + el;
     const bool live = e < count;
-    double *xs = x + el * d + h * N;
+    {
+        const double *tw = wt + (INC ? (INV ? W.inv_h : W.fwd_h) : (INV ? W.inv_f : W.fwd_f));
+        for (int i = tid; i < N; i += WT) tws[i] = tw[i];
+        __syncthreads();
+    }
+    double *xe = x + el * d, *xs = xe + h * N;
     double v[8];
     if (!INV) {
 #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = live ? u2d(src[e * d + l + k * TPE]) : 0.0;
-        wide_fwd<LOGN>(v, xs, u, wt + (INC ? W.fwd_h : W.fwd_f), M);
-        if (live) {
-            uint64_t *o = dst + e * d + h * N + 8 * u;
+        wide_fwd<LOGN>(v, xs, u, tws, M);
 #pragma unroll
-            for (int k = 0; k < 8; k++) o[k] = d2u(v[k]);
+        for (int k = 0; k < 8; k++) xs[8 * u + k] = v[k];
+        wave_sync();
+        if (live) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) dst[e * d + l + k * TPE] = d2u(xe[l + k * TPE]);
         }
     } else {
-        const uint64_t *in = src + e * d + h * N + 8 * u;
 #pragma unroll
-        for (int k = 0; k < 8; k++) v[k] = live ? u2d(in[k]) : 0.0;
-        wide_inv<LOGN>(v, xs, u, wt + (INC ? W.inv_h : W.inv_f), wt[INC ? W.ninv_h : W.ninv_f],
-                       M);
+        for (int k = 0; k < 8; k++) xe[l + k * TPE] = live ? u2d(src[e * d + l + k * TPE]) : 0.0;
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = xs[8 * u + k];
+        wave_sync();
+        wide_inv<LOGN>(v, xs, u, tws, wt[INC ? W.ninv_h : W.ninv_f], M);
         if (live) {
 #pragma unroll
             for (int k = 0; k < 8; k++) dst[e * d + l + k * TPE] = d2u(v[k]);
@@ -149,26 +163,24 @@ __global__ __launch_bounds__(WT) void k_wide_ntt(const uint64_t *src, uint64_t *
     }
 }
 
-template <int LOGD>
-int run_wide_ntt(Context *ctx, const uint64_t *src, uint64_t *dst, size_t count, bool inc,
-                 bool inv) {
+template <int LOGD, bool INV, bool INC>
+int launch_wide_kernel(Context *ctx, const uint64_t *src, uint64_t *dst, size_t count) {
     constexpr int EPB = WT / ((1 << LOGD) / 8);
     const unsigned g = ceil_div(count, EPB);
     Mod M{ctx->qd, ctx->qinv};
-    if (inc && !inv)
-        hipLaunchKernelGGL((k_wide_ntt<LOGD, false, true>), dim3(g), dim3(WT), 0, ctx->stream, src,
-                           dst, count, ctx->d_wtab, M);
-    else if (inc)
-        hipLaunchKernelGGL((k_wide_ntt<LOGD, true, true>), dim3(g), dim3(WT), 0, ctx->stream, src,
-                           dst, count, ctx->d_wtab, M);
-    else if (!inv)
-        hipLaunchKernelGGL((k_wide_ntt<LOGD, false, false>), dim3(g), dim3(WT), 0, ctx->stream,
-                           src, dst, count, ctx->d_wtab, M);
-    else
-        hipLaunchKernelGGL((k_wide_ntt<LOGD, true, false>), dim3(g), dim3(WT), 0, ctx->stream,
-                           src, dst, count, ctx->d_wtab, M);
+    hipLaunchKernelGGL((k_wide_ntt<LOGD, INV, INC>), dim3(g), dim3(WT), 0, ctx->stream, src, dst,
+                       count, ctx->d_wtab, M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
+}
+
+template <int LOGD>
+int run_wide_ntt(Context *ctx, const uint64_t *src, uint64_t *dst, size_t count, bool inc,
+                 bool inv) {
+    if (inc && !inv) return launch_wide_kernel<LOGD, false, true>(ctx, src, dst, count);
+    if (inc) return launch_wide_kernel<LOGD, true, true>(ctx, src, dst, count);
+    if (!inv) return launch_wide_kernel<LOGD, false, false>(ctx, src, dst, count);
+    return launch_wide_kernel<LOGD, true, false>(ctx, src, dst, count);
 }
 
 // ---------------------------------------------------------------- commit planes
