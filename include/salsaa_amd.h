@@ -24,7 +24,12 @@
  * SALSAA_REP_FULL (NTT: NTT_d of the coefficients)
  * (reference: src/cyclotomic_ring.rs:13-18, 483-556).
  *
- * The device kernels support d = 64 and any prime q < 2^50 with q = 1 mod 128.
+ * The device kernels support any prime q < 2^50 with q = 1 mod 128 at d = 64,
+ * and at the reference's own degrees d = 128, 256, 512 (protocol.rs:28-120)
+ * when also q = 1 mod d.  At d > 64 the conversions (the NTT representation
+ * only when q = 1 mod 2d), the ring product, keys and the commit (kappa <= 18),
+ * decomposition, decomp_ell, compose / verify_decomp, the fold and the norm
+ * check run; every other entry point returns SALSAA_ERR_ARG for such a context.
  */
 #ifndef SALSAA_AMD_H
 #define SALSAA_AMD_H
@@ -121,7 +126,8 @@ const char *salsaa_version(void);
 
 /* Context for modulus q and degree d on the current HIP device: builds the
  * NTT tables (the role of the reference's NTTCache, hexl_wrapper.cpp:11-32)
- * and one default-priority stream.  d must be 64. */
+ * and one default-priority stream.  d = 64, or 128 / 256 / 512 with
+ * q = 1 mod d (the entry points listed above). */
 int salsaa_ctx_create(uint64_t q, uint32_t d, salsaa_ctx **out);
 int salsaa_ctx_destroy(salsaa_ctx *ctx);
 int salsaa_ctx_sync(salsaa_ctx *ctx);
