@@ -788,6 +788,27 @@ def timed(ctx, fn, k):
     return per, ctx.elapsed_ms(evs[0], evs[-1]), wall
 
 
+def mix_ceilings(ctx):
+    """The HBM rate a plain streaming kernel (salsaa_memstream) reaches at the
+    read/write mixes of the kernels the roofline objects report: the batched
+    step's NTT pass reads 512 B and writes 1088 B per ring element, one
+    stream of it in place (7 read : 15 write streams); the standalone NTT
+    rewrites its input in place (1 : 1).  Best over 4, 8 and 16 blocks per CU
+    of the mean of 5 launches, 1 GiB per stream."""
+    n16 = 1 << 26
+    res = {}
+    for name, rn, wn in (("ntt_pass", 7, 15), ("in_place_copy", 1, 1)):
+        buf = ctx.empty(((rn + wn - 1) * n16 * 2,))
+        best = 0.0
+        for bpc in (4, 8, 16):
+            ctx.memstream(buf, n16, rn, wn, True, bpc)
+            _, tot, _ = timed(ctx, lambda: ctx.memstream(buf, n16, rn, wn, True, bpc), 5)
+            best = max(best, (rn + wn) * n16 * 16 / (tot / 5 * 1e-3) / 1e9)
+        buf.free()
+        res[name] = round(best, 1)
+    return res
+
+
 def ntt_schedules(sa, steps):
     """Config 2's NTT (forward + inverse IncompleteNTT of 2^20 ring elements)
     under each radix-exchange schedule: between the register phases of each
@@ -1008,6 +1029,15 @@ def main():
     }
     b.close()
     bw.key.close()
+    ceil = mix_ceilings(ctx)
+    roofline["mix_ceiling"] = {
+        "GB/s": ceil["ntt_pass"], "frac": round(roofline["achieved"] / ceil["ntt_pass"], 4),
+        "mix": "plain streaming kernel (salsaa_memstream), 7 read : 15 write streams, one in "
+               "place: the pass's 512 B read and 1088 B written per ring element"}
+    roofline_ntt["mix_ceiling"] = {
+        "GB/s": ceil["in_place_copy"],
+        "frac": round(roofline_ntt["achieved"] / ceil["in_place_copy"], 4),
+        "mix": "plain streaming kernel (salsaa_memstream), in-place copy (1 read : 1 write)"}
     if cpu_batched is not None:
         out["cpu_baseline"] = cpu_batched
     if not args.no_served:

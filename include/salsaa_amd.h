@@ -148,6 +148,16 @@ int salsaa_memcpy_d2h(salsaa_ctx *ctx, void *h_dst, const void *d_src, size_t by
 /* Stream-ordered device copy (also the plain copy kernel the traffic
  * counters are calibrated against). */
 int salsaa_memcpy_d2d(salsaa_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
+/* Bandwidth probe, no counterpart in the reference: a plain streaming kernel
+ * with rn read streams and wn write streams of n16 16-byte vectors each, laid
+ * out back to back in d_buf (reads first, so d_buf holds (rn + wn) * n16 * 16
+ * bytes; with inplace, write stream 0 overwrites read stream 0 and d_buf
+ * holds (rn + wn - 1) * n16 * 16 bytes), on blocks_per_cu 256-thread blocks
+ * per CU.  It measures the HBM
+ * rate a given read/write mix reaches, which the benchmark reports beside the
+ * fused passes that have that mix. */
+int salsaa_memstream(salsaa_ctx *ctx, void *d_buf, size_t n16, int rn, int wn, int inplace,
+                     int blocks_per_cu);
 
 /* Events on the context stream, for timing enqueued work. */
 int salsaa_event_create(void **ev);
@@ -440,6 +450,16 @@ int salsaa_batch_tuning_grid(const salsaa_batch *b, unsigned *blocks, unsigned *
 /* Sets the NTT pass's persistent blocks (0: one per CU); the step is captured
  * again on its next graph replay.  Waits for the context's stream. */
 int salsaa_batch_set_ntt_grid(salsaa_batch *b, unsigned blocks);
+/* The commit schedule the tuner chose (mixed 0: one launch per class of key
+ * row tiles, 1: one launch over every class, in which the blocks of all
+ * classes that read one (slot, K chunk) of a stored key run side by side and
+ * share it through L2) and its best times of the commit with each
+ * (milliseconds; mixed_ms 0 when the mixed launch does not apply: a generated
+ * CRS key or a single class). */
+int salsaa_batch_tuning_commit(const salsaa_batch *b, int *mixed, float *class_ms, float *mixed_ms);
+/* Sets the commit schedule (mixed 1 needs a stored key); the step is captured
+ * again on its next graph replay.  Waits for the context's stream. */
+int salsaa_batch_set_commit_schedule(salsaa_batch *b, int mixed);
 
 /* ------------------------------------------------------------------ folding
  * The folding step of n accumulated instances (the protocol's folding round,

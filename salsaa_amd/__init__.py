@@ -105,6 +105,7 @@ def _load():
         "salsaa_memcpy_h2d": (_int, [_vp, _vp, _vp, _sz]),
         "salsaa_memcpy_d2h": (_int, [_vp, _vp, _vp, _sz]),
         "salsaa_memcpy_d2d": (_int, [_vp, _vp, _vp, _sz]),
+        "salsaa_memstream": (_int, [_vp, _vp, _sz, _int, _int, _int, _int]),
         "salsaa_event_create": (_int, [_PP]),
         "salsaa_event_destroy": (_int, [_vp]),
         "salsaa_event_record": (_int, [_vp, _vp]),
@@ -180,6 +181,10 @@ def _load():
                                             ctypes.POINTER(ctypes.c_uint),
                                             ctypes.POINTER(ctypes.c_float)]),
         "salsaa_batch_set_ntt_grid": (_int, [_vp, ctypes.c_uint]),
+        "salsaa_batch_tuning_commit": (_int, [_vp, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_float),
+                                              ctypes.POINTER(ctypes.c_float)]),
+        "salsaa_batch_set_commit_schedule": (_int, [_vp, _int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -337,6 +342,13 @@ class Context:
         assert dst.nbytes == src.nbytes
         _check(lib.salsaa_memcpy_d2d(self.h, dst.ptr, src.ptr, src.nbytes))
         return dst
+
+    def memstream(self, buf, n16, rn, wn, inplace=False, blocks_per_cu=8):
+        """Bandwidth probe (salsaa_memstream): rn read and wn write streams of
+        n16 16-byte vectors in buf."""
+        need = (rn + wn - (1 if inplace else 0)) * n16 * 16
+        assert buf.nbytes >= need, "stream buffer too small"
+        _check(lib.salsaa_memstream(self.h, buf.ptr, n16, rn, wn, int(inplace), blocks_per_cu))
 
     def uniform(self, shape, seed, stream):
         a = self.empty(shape)
@@ -768,10 +780,15 @@ class Batch:
                                            ctypes.byref(td)))
         g, gc, gt = ctypes.c_uint(), (ctypes.c_uint * 3)(), (ctypes.c_float * 3)()
         _check(lib.salsaa_batch_tuning_grid(self.h, ctypes.byref(g), gc, gt))
+        cm, cc, cx = ctypes.c_int(), ctypes.c_float(), ctypes.c_float()
+        _check(lib.salsaa_batch_tuning_commit(self.h, ctypes.byref(cm), ctypes.byref(cc),
+                                              ctypes.byref(cx)))
         return dict(kchunk=kc.value, cpt_first=cpt.value, commit_ms=tc.value,
                     norm_check_ms=tn.value, ntt_schedule=("lds", "dpp")[x.value],
                     ntt_pass_ms={"lds": tl.value, "dpp": td.value}, ntt_blocks=g.value,
-                    ntt_blocks_ms={str(gc[i]): gt[i] for i in range(3) if gc[i]})
+                    ntt_blocks_ms={str(gc[i]): gt[i] for i in range(3) if gc[i]},
+                    commit_schedule=("per_class", "mixed")[cm.value],
+                    commit_schedule_ms={"per_class": cc.value, "mixed": cx.value})
 
     def set_ntt_schedule(self, name):
         """The NTT pass's exchange: "lds" or "dpp" (quad permutes)."""
@@ -780,6 +797,10 @@ class Batch:
     def set_ntt_grid(self, blocks):
         """The NTT pass's persistent blocks (0: one per CU)."""
         _check(lib.salsaa_batch_set_ntt_grid(self.h, int(blocks)))
+
+    def set_commit_schedule(self, name):
+        """The commit's launches: "per_class" or "mixed" (one launch, stored key)."""
+        _check(lib.salsaa_batch_set_commit_schedule(self.h, {"per_class": 0, "mixed": 1}[name]))
 
     def close(self):
         if self.h:

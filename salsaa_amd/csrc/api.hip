@@ -213,6 +213,29 @@ __global__ void k_fill_zero(uint64_t *p, size_t n) {
     if (i < n) p[i] = 0;
 }
 
+// rn read streams and wn write streams of n 16-byte vectors, back to back in
+// buf (reads first); with inplace the first write stream is read stream 0.
+// Vector i of every write stream gets the xor of vector i of the read streams
+// plus the stream index, so no load or store can be dropped.
+__global__ __launch_bounds__(256) void k_memstream(ulonglong2 *buf, size_t n, int rn, int wn,
+                                                   int inplace) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    // write stream w >= 1 follows the read streams (w >= 0 without inplace)
+    ulonglong2 *wb = buf + ((size_t)rn - (inplace ? 1 : 0)) * n;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        unsigned long long a = 0, b = 0;
+        for (int r = 0; r < rn; ++r) {
+            const ulonglong2 v = buf[(size_t)r * n + i];
+            a ^= v.x;
+            b ^= v.y;
+        }
+        for (int w = 0; w < wn; ++w) {
+            ulonglong2 *p = (inplace && w == 0) ? buf + i : wb + (size_t)w * n + i;
+            *p = make_ulonglong2(a + (unsigned)w, b);
+        }
+    }
+}
+
 int zero_row(Context *ctx, size_t bytes, const int8_t **out) {
     ScratchBuf &b = ctx->zeros;
     if (b.bytes < bytes) {
@@ -377,6 +400,21 @@ int salsaa_memcpy_d2h(salsaa_ctx *ctx, void *h_dst, const void *d_src, size_t by
 int salsaa_memcpy_d2d(salsaa_ctx *ctx, void *d_dst, const void *d_src, size_t bytes) {
     CTX_OK(ctx);
     SALSAA_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->c.stream));
+    return SALSAA_OK;
+}
+
+int salsaa_memstream(salsaa_ctx *ctx, void *d_buf, size_t n16, int rn, int wn, int inplace,
+                     int blocks_per_cu) {
+    CTX_OK(ctx);
+    SALSAA_CHECK(d_buf != nullptr && n16 > 0, "empty stream buffer");
+    SALSAA_CHECK(rn >= 0 && rn <= 16 && wn >= 0 && wn <= 16 && rn + wn > 0,
+                 "stream counts must be 0..16, not both zero");
+    SALSAA_CHECK(!inplace || (rn > 0 && wn > 0), "an in-place write stream needs a read stream");
+    SALSAA_CHECK(blocks_per_cu >= 1 && blocks_per_cu <= 64, "blocks per CU must be 1..64");
+    const unsigned g = (unsigned)(ctx->c.ncu * blocks_per_cu);
+    hipLaunchKernelGGL(k_memstream, dim3(g), dim3(256), 0, ctx->c.stream,
+                       static_cast<ulonglong2 *>(d_buf), n16, rn, wn, inplace);
+    SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }
 

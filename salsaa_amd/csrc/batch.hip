@@ -119,6 +119,13 @@ struct salsaa_batch {
     CommitJob *d_cjobs = nullptr;
     uint2 *d_table = nullptr;
     size_t table_cap = 0;
+    // commit schedule: one launch per class (0) or one launch over every
+    // class (1, stored keys: the classes share each key chunk through L2);
+    // the mixed launch's table follows the per-class tables; the tuner's
+    // best times of each schedule
+    int commit_mixed = 0;
+    size_t mixed_t0 = 0, mixed_nblk = 0;
+    float t_commit_mode[2] = {0.f, 0.f};
     FinJob *d_fjobs = nullptr;
     // norm check
     struct Group {
@@ -200,6 +207,9 @@ int build_commit_tables(salsaa_batch *b, size_t kc) {
             c.rows = b->job_rows[j];
             c.rtiles = (7 * c.rows + 15) / 16;
             c.ntile = c.rtiles;
+            c.kappa = (int)cc.kappa;
+            const size_t rtA = (7 * cc.kappa + 15) / 16;
+            c.tm = rtA <= 4 ? 1 : rtA <= 8 ? 2 : 4;
             c.partial = b->d_partial + poff;
             FinJob &f = fj[j];
             f.partial = c.partial;
@@ -223,6 +233,23 @@ int build_commit_tables(salsaa_batch *b, size_t kc) {
         cc.nblk = table.size() - cc.t0;
         cc.fnblk = fblk;
     }
+    // the mixed launch: block order (z, s, class, job), so every block of one
+    // key chunk is consecutive whatever its class
+    b->mixed_t0 = table.size();
+    {
+        int maxks = 0;
+        std::vector<int> ks(b->njobs);
+        for (size_t j = 0; j < b->njobs; j++) {
+            ks[j] = (int)((b->job_ktiles[j] + kc - 1) / kc);
+            maxks = std::max(maxks, ks[j]);
+        }
+        for (int z = 0; z < maxks; z++)
+            for (unsigned s = 0; s < 64; s++)
+                for (const auto &cc : b->cclass)
+                    for (size_t jj = 0; jj < cc.njobs && ks[cc.job0 + jj] > z; jj++)
+                        table.push_back(make_uint2((unsigned)(cc.job0 + jj), ((unsigned)z << 6) | s));
+    }
+    b->mixed_nblk = table.size() - b->mixed_t0;
     SALSAA_CHECK(poff <= b->partial_cap, "commit partial buffer too small");
     SALSAA_CHECK(table.size() <= b->table_cap, "commit table too small");
     SALSAA_TRY(upload(ctx, b->d_cjobs, cj));
@@ -237,9 +264,15 @@ int run_planes(salsaa_batch *b) {
                                    b->pjobs.size(), b->pnblk);
 }
 
-// one commit launch per class (its kappa and kernel shape), then one finalize each
+// one commit launch per class (its kappa and kernel shape) or, with a stored
+// key and the mixed schedule, one launch over every class; then one finalize
+// per class
 int run_commit(salsaa_batch *b) {
     const Key &k = b->key->k;
+    if (b->commit_mixed && !k.d_mhat)
+        SALSAA_TRY(launch_commit_mfma_mixed(&b->ctx->c, k.d_AB, k.a_kt, k.a_ktiles, b->d_cjobs,
+                                            b->d_table + b->mixed_t0, b->mixed_nblk, b->kchunk));
+    else
     for (const auto &cc : b->cclass) {
         if (k.d_mhat)  // a CRS key generated inside the commit
             SALSAA_TRY(launch_commit_crs_batch(&b->ctx->c, k.d_mhat, cc.kappa, b->d_cjobs + cc.job0,
@@ -388,19 +421,30 @@ int time_ms(salsaa_batch *b, F fn, int runs, float *ms_out) {
 int autotune(salsaa_batch *b) {
     size_t max_kt = 0;
     for (int kt : b->job_ktiles) max_kt = std::max(max_kt, (size_t)kt);
+    // the commit: K chunk and schedule (a launch per key row class, or one
+    // launch over all classes when the key is stored)
+    const int nmode = b->key->k.d_mhat || b->cclass.size() < 2 ? 1 : 2;
     size_t best_kc = 0;
+    int best_mode = 0;
     float best = 0.f;
+    b->t_commit_mode[0] = b->t_commit_mode[1] = 0.f;
     for (size_t kc : {16, 32, 64, 128, 256}) {
         if (kc > 2 * max_kt && best_kc) break;
         SALSAA_TRY(build_commit_tables(b, kc));
-        float ms = 0.f;
-        SALSAA_TRY(time_ms(b, [&] { return run_commit(b); }, 2, &ms));
-        if (!best_kc || ms < best) {
-            best = ms;
-            best_kc = kc;
+        for (int mode = 0; mode < nmode; mode++) {
+            b->commit_mixed = mode;
+            float ms = 0.f;
+            SALSAA_TRY(time_ms(b, [&] { return run_commit(b); }, 2, &ms));
+            if (b->t_commit_mode[mode] == 0.f || ms < b->t_commit_mode[mode]) b->t_commit_mode[mode] = ms;
+            if (!best_kc || ms < best) {
+                best = ms;
+                best_kc = kc;
+                best_mode = mode;
+            }
         }
     }
     SALSAA_TRY(build_commit_tables(b, best_kc));
+    b->commit_mixed = best_mode;
     b->t_commit = best;
     // the NTT pass's radix-exchange schedule: the whole round trip (the
     // witnesses come back unchanged) with the exchange through LDS and
@@ -629,7 +673,7 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
         for (size_t j = cc.job0; j < cc.job0 + cc.njobs; j++) {
             const size_t ks = (b->job_ktiles[j] + 15) / 16;
             pcap += ks * 64 * cc.kappa * b->job_rows[j];
-            tcap += ks * 64;
+            tcap += 2 * ks * 64;  // the per-class tables and the mixed one
         }
     b->partial_cap = pcap;
     b->table_cap = tcap;
@@ -651,7 +695,7 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
             if (load[j] < load[k]) k = j;
         load[k] += (double)g.rows * (double)g.L + 1e6;  // a fixed cost per class
         size_t a, bb, mm;
-        nc_scratch_need(g.rows, g.L, reps, &mm, &a, &bb);
+        nc_scratch_need(g.rows, g.L, reps, &mm, &a, &bb, true);
         need_m[k] = std::max(need_m[k], mm);
         need_a[k] = std::max(need_a[k], a);
         need_b[k] = std::max(need_b[k], bb);
@@ -881,6 +925,29 @@ int salsaa_batch_set_ntt_grid(salsaa_batch *b, unsigned blocks) {
     b->exec = nullptr;
     b->graph = nullptr;
     b->ntt_grid = blocks;
+    return SALSAA_OK;
+}
+
+int salsaa_batch_tuning_commit(const salsaa_batch *b, int *mixed, float *class_ms, float *mixed_ms) {
+    SALSAA_CHECK(b != nullptr, "null batch");
+    if (mixed) *mixed = b->commit_mixed;
+    if (class_ms) *class_ms = b->t_commit_mode[0];
+    if (mixed_ms) *mixed_ms = b->t_commit_mode[1];
+    return SALSAA_OK;
+}
+
+int salsaa_batch_set_commit_schedule(salsaa_batch *b, int mixed) {
+    SALSAA_CHECK(b != nullptr, "null batch");
+    SALSAA_CHECK(mixed == 0 || mixed == 1, "commit schedule must be 0 (per class) or 1 (mixed)");
+    SALSAA_CHECK(!mixed || !b->key->k.d_mhat, "the mixed commit launch needs a stored key");
+    CTX_OK(b->ctx);
+    // the captured step launches the other schedule: capture again on next use
+    SALSAA_HIP(hipStreamSynchronize(b->ctx->c.stream));
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->commit_mixed = mixed;
     return SALSAA_OK;
 }
 

@@ -150,6 +150,9 @@ struct CommitJob {
     const int8_t *WB = nullptr;
     double *partial = nullptr;  // [ceil(ktiles / kchunk)][64][kappa][rows]
     int ktiles = 0, rtiles = 0, ntile = 0, rows = 0;
+    // the job's key rows and key row tiles per wave (1, 2 or 4), read by the
+    // launch that mixes jobs of different kappa
+    int kappa = 0, tm = 0;
 };
 
 // The MFMA commit keeps all of a key's digit rows in one block (16 row tiles).
@@ -456,6 +459,11 @@ int launch_commit_mfma_batch(Context *ctx, const int8_t *AB, size_t kappa, size_
                              size_t a_ktiles, const CommitJob *jobs, const uint2 *table,
                              size_t nblk, size_t kchunk, size_t max_tiles);
 int commit_batch_occupancy(size_t kappa, size_t max_tiles);
+// One launch over jobs of any kappa (each job's kappa and tm from its
+// CommitJob, job indices in the table relative to `jobs`).
+int launch_commit_mfma_mixed(Context *ctx, const int8_t *AB, size_t a_kt, size_t a_ktiles,
+                             const CommitJob *jobs, const uint2 *table, size_t nblk,
+                             size_t kchunk);
 // the commit against the power-series CRS with NTT-form module mhat [kappa][d],
 // the key generated in the kernel (WB: the witness planes)
 int launch_commit_crs(Context *ctx, const uint64_t *mhat, size_t kappa, const int8_t *WB, size_t r,
@@ -504,8 +512,10 @@ struct NcScratch {
     double *bufA = nullptr, *bufB = nullptr;
     size_t macc_bytes = 0, bufA_bytes = 0, bufB_bytes = 0;
 };
+// short_copy: the norm check reads a short copy (its first pass may then keep
+// a 64 x 64 Gram matrix per row instead of 16 x 16)
 void nc_scratch_need(size_t rows, size_t L, size_t reps, size_t *macc_bytes, size_t *bufA_bytes,
-                     size_t *bufB_bytes);
+                     size_t *bufB_bytes, bool short_copy = false);
 // wrow: per-row offsets (u64) of the rows from w (null: row r at w + r L);
 // cs_row: u64 between consecutive rows' challenge blocks (0: shared);
 // cpt_first: columns per thread of the first pass (0: automatic);

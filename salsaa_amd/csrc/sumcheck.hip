@@ -737,6 +737,327 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_short(
     }
 }
 
+// ---- six rounds per first pass (batch arenas with a short copy).  With 64
+// row segments (S = L / 64, v[y S + x] for y < 64) rounds 0..5 follow from the
+// row's 64 x 64 Gram matrix exactly as rounds 0..3 follow from the 16 x 16
+// one (the fold vectors of round t < 6 have the closed form
+// A_t[z][y] = [y mod 64/2^t = z] e_t(y), e_t(y) = prod_{s<t} (bit 5 - s of y ?
+// c_s : 1 - c_s)), and the witness after round 5 is the single 64-term
+// combination sum_y e_6(y) v[y S + x].  So the first pass over the short copy
+// costs what the 16-segment one does per coefficient (the same short fold,
+// with 64 terms) and no F_{q^2} round work, and writes L / 64 values per
+// repetition instead of L / 16.  The Gram matrix is kept as its 10 upper 16 x
+// 16 tile pairs (a <= b), gacc[row][p][256].
+constexpr int G64_MAXREP = 2;    // repetitions folded per launch
+constexpr int G64_PAIRS = 10;    // tile pairs (a <= b) of the 64 x 64 Gram matrix
+constexpr size_t kGram64Entries = (size_t)G64_PAIRS * 256;
+
+__device__ __forceinline__ void g64_pair(int p, int &a, int &b) {
+    // p -> (a, b), a <= b < 4, in the order (0,0) (0,1) (0,2) (0,3) (1,1) ...
+    a = p < 4 ? 0 : p < 7 ? 1 : p < 9 ? 2 : 3;
+    b = a + (p - (a == 0 ? 0 : a == 1 ? 4 : a == 2 ? 7 : 9));
+}
+
+// The block's fold weights e_6(y) for repetitions rep0 .. rep0 + nrep - 1,
+// split as in GrCoef (spare repetition slots zero), and -128 sum_y e_6(y)
+// (the short pass folds z = b + 128).
+struct G64Coef {
+    F2 coef[G64_MAXREP][64];
+    __align__(32) double csplit[G64_MAXREP][64][4];
+    double coff[G64_MAXREP][2];
+};
+template <bool FQ2, bool SNR>
+__device__ __forceinline__ void g64_coefs(G64Coef &C, const uint64_t *cs, int rep0, int nrep,
+                                          int nrounds, double nr, Mod M, int tid) {
+    if (tid < 64 * G64_MAXREP) {
+        const int r = tid >> 6, y = tid & 63;
+        if (r < nrep) {
+            F2 a = {1.0, 0.0};
+            for (int t = 0; t < 6; t++) {
+                const F2 c = load_chal<FQ2>(cs, rep0 + r, nrounds, t);
+                a = x_mul<FQ2, SNR>(a, ((y >> (5 - t)) & 1) ? c : x_sub<FQ2>(F2{1.0, 0.0}, c, M), nr, M);
+            }
+            const double ca = balance(a.a, M), cb = balance(a.b, M);
+            C.coef[r][y] = {ca, cb};
+            const double ha = __builtin_rint(ca * 0x1p-25), hb = __builtin_rint(cb * 0x1p-25);
+            C.csplit[r][y][0] = ha;
+            C.csplit[r][y][1] = ca - ha * 0x1p25;
+            C.csplit[r][y][2] = hb;
+            C.csplit[r][y][3] = cb - hb * 0x1p25;
+        } else {
+            C.coef[r][y] = {0.0, 0.0};
+            C.csplit[r][y][0] = C.csplit[r][y][1] = C.csplit[r][y][2] = C.csplit[r][y][3] = 0.0;
+        }
+    }
+    __syncthreads();
+    if (tid < 2 * nrep) {
+        const int r = tid >> 1, comp = tid & 1;
+        double sum = 0.0;  // 64 balanced terms, reduced every 8 (below 4.5 q + 16)
+        for (int y = 0; y < 64; y++) {
+            sum += comp ? C.coef[r][y].b : C.coef[r][y].a;
+            if ((y & 7) == 7) sum = reduce_lazy(sum, M);
+        }
+        C.coff[r][comp] = reduce_lazy(mul_lazy(sum, -128.0, M), M);
+    }
+    __syncthreads();
+}
+
+// Whether every short-copy tile the block's columns [xb, xb + cpt GR_X) meet
+// in the 64 segments is flagged short (each range meets at most two).
+__device__ __forceinline__ bool g64_compact(const uint32_t *__restrict__ shf, size_t base, size_t S,
+                                            size_t xb, int cpt) {
+    const size_t xe = min(xb + (size_t)cpt * GR_X, S) - 1;
+    uint32_t any = 0;
+    for (int y = 0; y < 64; y++) {
+        const size_t s0 = base + (size_t)y * S;
+        any |= shf[(s0 + xb) / kShortTile] | shf[(s0 + xe) / kShortTile];
+    }
+    return any == 0;
+}
+
+// Digits 0 and 1 of 16 consecutive staged entries z = b + 128 (int16, b the
+// balanced residue) as matrix-core fragments: the low byte of z is d0 + 128,
+// the high byte d1.
+__device__ __forceinline__ void g64_frags(const uint16_t *src, v4i &f0, v4i &f1) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(src);
+    const uint4 u0 = p[0], u1 = p[1];
+    const uint32_t w[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        f0[k] = (int)(__builtin_amdgcn_perm(w[2 * k + 1], w[2 * k], 0x06040200u) ^ 0x80808080u);
+        f1[k] = (int)__builtin_amdgcn_perm(w[2 * k + 1], w[2 * k], 0x07050301u);
+    }
+}
+
+// The first pass over the short copy with 64 segments: the sub-blocks of cpt
+// GR_X columns x < S whose tiles in all 64 segments are short.  Per
+// iteration, wave w stages the 64 segments of columns 64 w .. 64 w + 63 (one
+// per lane, prefetched a whole iteration ahead) into the block's u16 rows and
+// folds its lane's column; after a block barrier wave w multiplies the tile
+// pairs p = w, w + 4, w + 8 over all 256 columns on the matrix cores (so each
+// wave keeps at most 3 pairs of E_0..E_2 accumulators), and a second barrier
+// frees the rows.  Each wave adds its pairs' sums to gacc at the end.
+constexpr int G64_LD = GR_X + 8;  // u16 row stride of the staging rows
+template <bool FQ2, bool SNR, bool GRAM>
+__global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_sc_gram64_short(
+    const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
+    const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr, int cpt,
+    double *__restrict__ out, unsigned long long *__restrict__ gacc, const double *__restrict__ tab,
+    Mod M, const uint16_t *__restrict__ sht, const uint32_t *__restrict__ shf, int sbn,
+    unsigned long long *__restrict__ need_long) {
+    __shared__ __align__(16) uint16_t T16[64 * G64_LD];  // T16[y][c]: segment y, block column c
+    __shared__ G64Coef C;
+    const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const size_t S = L >> 6;
+    const size_t base = wrow ? wrow[row] : (size_t)row * L;
+    const size_t span = (size_t)cpt * GR_X, sb0 = (size_t)blockIdx.x * sbn;
+    uint32_t todo = 0;
+    bool left = false;  // a sub-block for k_sc_gram64_rest
+    for (int k = 0; k < sbn && (sb0 + k) * span < S; k++) {
+        if (g64_compact(shf, base, S, (sb0 + k) * span, cpt))
+            todo |= 1u << k;
+        else
+            left = true;
+    }
+    if (left && tid == 0) *need_long = 1ULL;
+    if (todo == 0) return;
+    g64_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
+    const int fr = lane & 15, fg = lane >> 4;
+    const uint16_t *tr = sht + base;
+    v4i acc[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) acc[i][k] = (v4i){0, 0, 0, 0};
+    // entry (y, x) at the uniform tr plus the 32-bit offset y S + x (S <= 2^26:
+    // one base address, not 64 of them in scalar registers)
+    const uint32_t S32 = (uint32_t)S;
+    auto fetch = [&](uint32_t(&t)[64], size_t xc) {
+        const uint32_t xo = (uint32_t)min(xc, S - 1);
+#pragma unroll
+        for (int y = 0; y < 64; y++) t[y] = tr[(uint32_t)y * S32 + xo];
+    };
+    uint16_t *mycol = T16 + 64 * wave + lane;
+    for (int k = 0; k < sbn; k++) {
+        if (!((todo >> k) & 1)) continue;
+        size_t xb = (sb0 + k) * span;  // the block's columns (uniform)
+        uint32_t tv[64];
+        fetch(tv, xb + 64 * wave + lane);
+        for (int cc = 0;; cc++) {
+            const size_t x = xb + 64 * wave + lane;
+            const bool live = x < S;
+            const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform in the block
+            // z = t - 2^14 + 128 = b + 128 (a dead column stages b = 0)
+#pragma unroll
+            for (int y = 0; y < 64; y++)
+                mycol[G64_LD * y] = (uint16_t)(live ? tv[y] - (GR_SMALL - 128u) : 128u);
+            fetch(tv, more ? x + GR_X : x);
+            // the short fold of the lane's column over the 64 segments, of
+            // z = b + 128 (|z| < 2^15: the sums of c_h z and c_l z stay below
+            // 2^45; the repetitions in a runtime loop, so the weights are read
+            // per column)
+            for (int r = 0; r < nrep; r += 2) {
+                double h0 = 0.0, l0 = 0.0, h1 = 0.0, l1 = 0.0;  // repetition r: a, b
+                double h2 = 0.0, l2 = 0.0, h3 = 0.0, l3 = 0.0;  // repetition r + 1
+#pragma unroll 16
+                for (int y = 0; y < 64; y++) {
+                    const double tv_y = (double)(int16_t)mycol[G64_LD * y];
+                    const double4 c0 = *reinterpret_cast<const double4 *>(&C.csplit[r][y][0]);
+                    const double4 c1 = *reinterpret_cast<const double4 *>(&C.csplit[r + 1][y][0]);
+                    h0 = __fma_rn(c0.x, tv_y, h0);
+                    l0 = __fma_rn(c0.y, tv_y, l0);
+                    h2 = __fma_rn(c1.x, tv_y, h2);
+                    l2 = __fma_rn(c1.y, tv_y, l2);
+                    if (FQ2) {
+                        h1 = __fma_rn(c0.z, tv_y, h1);
+                        l1 = __fma_rn(c0.w, tv_y, l1);
+                        h3 = __fma_rn(c1.z, tv_y, h3);
+                        l3 = __fma_rn(c1.w, tv_y, l3);
+                    }
+                }
+                if (live) {
+                    double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
+                    o[0] = gr_small_sum(h0, l0, C.coff[r][0], M);
+                    o[1] = FQ2 ? gr_small_sum(h1, l1, C.coff[r][1], M) : 0.0;
+                    if (r + 1 < nrep) {
+                        o += (size_t)rows * S * 2;
+                        o[0] = gr_small_sum(h2, l2, C.coff[r + 1][0], M);
+                        o[1] = FQ2 ? gr_small_sum(h3, l3, C.coff[r + 1][1], M) : 0.0;
+                    }
+                }
+            }
+            if (GRAM) {
+                __syncthreads();  // every wave's columns are staged
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const int q = wave + 4 * i;  // wave-uniform
+                    if (q >= G64_PAIRS) break;
+                    int a, b;
+                    g64_pair(q, a, b);
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; k4++) {
+                        v4i da0, da1, db0, db1;
+                        g64_frags(T16 + G64_LD * (16 * a + fr) + 64 * k4 + 16 * fg, da0, da1);
+                        g64_frags(T16 + G64_LD * (16 * b + fr) + 64 * k4 + 16 * fg, db0, db1);
+                        acc[i][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(da0, db0, acc[i][0], 0, 0, 0);
+                        acc[i][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(da0, db1, acc[i][1], 0, 0, 0);
+                        acc[i][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(da1, db0, acc[i][1], 0, 0, 0);
+                        acc[i][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(da1, db1, acc[i][2], 0, 0, 0);
+                    }
+                }
+                __syncthreads();  // the rows are rewritten by the next iteration
+            }
+            if (!more) break;
+            xb += GR_X;
+        }
+    }
+    if (GRAM) {
+        // lane (r, g) holds E_k[4 g + e][r] of each of the wave's pairs
+        const double *wt = tab + TAB_W256;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int q = wave + 4 * i;
+            if (q >= G64_PAIRS) break;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                double xg = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) xg += mul_lazy((double)acc[i][k][e], wt[k], M);
+                atomicAdd(&gacc[((size_t)row * G64_PAIRS + q) * 256 + (4 * fg + e) * 16 + fr],
+                          (unsigned long long)d2u(reduce(xg, M)));
+            }
+        }
+    }
+}
+
+// The sub-blocks k_sc_gram64_short leaves (some tile not short), from the
+// witness, on a grid whose blocks walk up to 64 iterations of 64 columns.  It
+// returns at once when none was flagged (need_long zero), the usual case.
+// Per iteration the block stages 64 segments x 64 columns of residues in LDS;
+// threads r * 64 + c (r < nrep) fold column c with the 64 balanced weights,
+// and thread e owns Gram entries e + 256 k (k < 10: tile pair k) as u64 sums
+// of exact products (at most 4096 residues below 2^50 per block).
+constexpr int G64R_LD = 64 + 1;
+template <bool FQ2, bool SNR, bool GRAM>
+__global__ __launch_bounds__(SC_THREADS) void k_sc_gram64_rest(
+    const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
+    int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
+    double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc, Mod M,
+    const uint32_t *__restrict__ shf, int sbn, int nbx,
+    const unsigned long long *__restrict__ need_long) {
+    __shared__ G64Coef C;
+    __shared__ uint64_t V[64 * G64R_LD];  // V[y][c]: segment y, column xb + c
+    if (*need_long == 0) return;  // uniform
+    const int row = blockIdx.y, tid = threadIdx.x;
+    const size_t S = L >> 6;
+    const size_t base = wrow ? wrow[row] : (size_t)row * L;
+    const size_t span = (size_t)cpt * GR_X;
+    const uint64_t *wr = w + base;
+    unsigned long long gs[G64_PAIRS];
+#pragma unroll
+    for (int q = 0; q < G64_PAIRS; q++) gs[q] = 0ULL;
+    bool ready = false;
+    for (size_t bxi = blockIdx.x; bxi < (size_t)nbx; bxi += gridDim.x) {
+        const size_t sb0 = bxi * sbn;
+        uint32_t todo = 0;
+        for (int k = 0; k < sbn && (sb0 + k) * span < S; k++)
+            if (!g64_compact(shf, base, S, (sb0 + k) * span, cpt)) todo |= 1u << k;
+        if (todo == 0) continue;  // uniform
+        if (!ready) {
+            g64_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
+            ready = true;
+        }
+        for (int k = 0; k < sbn; k++) {
+            if (!((todo >> k) & 1)) continue;
+            for (size_t xb = (sb0 + k) * span; xb < min((sb0 + k + 1) * span, S); xb += 64) {
+                __syncthreads();  // V is free
+                for (int e = tid; e < 64 * 64; e += SC_THREADS) {
+                    const int y = e >> 6, c = e & 63;
+                    V[y * G64R_LD + c] = xb + c < S ? wr[(size_t)y * S + xb + c] : 0;
+                }
+                __syncthreads();
+                const int c = tid & 63, r = tid >> 6;
+                if (r < nrep && xb + c < S) {
+                    // products |coef v| <= 0.69 q (balanced coef, canonical v):
+                    // eight of them stay below 5.6 q, then a lazy reduction
+                    double a = 0.0, bsum = 0.0;
+                    for (int y0 = 0; y0 < 64; y0 += 8) {
+                        double a8 = 0.0, b8 = 0.0;
+#pragma unroll
+                        for (int y = y0; y < y0 + 8; y++) {
+                            const double v = u2d(V[y * G64R_LD + c]);
+                            a8 += mul_lazy(C.coef[r][y].a, v, M);
+                            if (FQ2) b8 += mul_lazy(C.coef[r][y].b, v, M);
+                        }
+                        a = reduce_lazy(a + reduce_lazy(a8, M), M);
+                        bsum = reduce_lazy(bsum + reduce_lazy(b8, M), M);
+                    }
+                    double *o = out + (((size_t)(rep0 + r) * rows + row) * S + xb + c) * 2;
+                    o[0] = a;
+                    o[1] = FQ2 ? bsum : 0.0;
+                }
+                if (GRAM) {
+#pragma unroll
+                    for (int q = 0; q < G64_PAIRS; q++) {
+                        int pa, pb;
+                        g64_pair(q, pa, pb);
+                        const int y = 16 * pa + (tid >> 4), y2 = 16 * pb + (tid & 15);
+                        for (int cc = 0; cc < 64; cc++)
+                            gs[q] += (unsigned long long)d2u(
+                                mulmod(u2d(V[y * G64R_LD + cc]), u2d(V[y2 * G64R_LD + cc]), M));
+                    }
+                }
+            }
+        }
+    }
+    if (!ready || !GRAM) return;  // uniform
+#pragma unroll
+    for (int q = 0; q < G64_PAIRS; q++)
+        atomicAdd(&gacc[((size_t)row * G64_PAIRS + q) * 256 + tid],
+                  (unsigned long long)d2u(u64_mod(gs[q], M)));
+}
+
 // Messages of rounds 0..3 (and the claim) from G, one block per (row, rep):
 // thread (z, y) (z < 8, y < 16) forms column y of G A[z] and its share of
 // both forms; the shares are summed across the block.
@@ -852,6 +1173,62 @@ __device__ __forceinline__ void gram_msgs(unsigned long long graw, int row, cons
     if (tid == 0) *claim = claim_fault && row == 0 ? addmod(red[16], 1.0, M) : red[16];
 }
 
+// Rounds 0..5's messages of (row, rep) from the row's 64 x 64 Gram sums (g:
+// this thread's entry tid of each tile pair, reduced) into msg[0..5] and the
+// claim tr G, as gram_msgs does for 16 segments: after t rounds (n = 64 >> t,
+// h = n / 2) round t pairs y, y' with y = y' or y = y' + h (mod n), y' mod n <
+// h, weighted by e_t(y) e_t(y'), e_t(y) = prod_{s<t} (bit 5 - s of y ? c_s :
+// 1 - c_s).  A pair a < b stands for both G[y][y'] and G[y'][y].
+template <bool FQ2, bool SNR>
+__device__ __forceinline__ void gram64_msgs(const double (&g)[G64_PAIRS], int row, const F2 *chal,
+                                            double (*msg)[4], double *claim, int claim_fault,
+                                            double *red, double nr, Mod M) {
+    static_assert(SC_THREADS == 256, "one thread per tile entry");
+    __shared__ F2 E[6][64];
+    const int tid = threadIdx.x, r16 = tid >> 4, c16 = tid & 15;
+    for (int k = tid; k < 6 * 64; k += SC_THREADS) {
+        const int t = k >> 6, yy = k & 63;
+        F2 e = {1.0, 0.0};
+        for (int s = 0; s < t; s++) {
+            const F2 c = chal[s];
+            e = x_mul<FQ2, SNR>(e, ((yy >> (5 - s)) & 1) ? c : x_sub<FQ2>(F2{1.0, 0.0}, c, M), nr, M);
+        }
+        E[t][yy] = e;
+    }
+    __syncthreads();
+    double vals[25];
+#pragma unroll
+    for (int k = 0; k < 25; k++) vals[k] = 0.0;
+    for (int q = 0; q < G64_PAIRS; q++) {
+        int a, b;
+        g64_pair(q, a, b);
+        const double gv = g[q];
+        for (int o = 0; o < (a == b ? 1 : 2); o++) {
+            const int y = o ? 16 * b + c16 : 16 * a + r16, y2 = o ? 16 * a + r16 : 16 * b + c16;
+#pragma unroll
+            for (int t = 0; t < 6; t++) {
+                const int n = 64 >> t, h = n >> 1, ry = y & (n - 1), ry2 = y2 & (n - 1);
+                if (ry2 < h && (ry == ry2 || ry == ry2 + h)) {
+                    const F2 p = x_mul<FQ2, SNR>(E[t][y], E[t][y2], nr, M);
+                    const int k = 4 * t + (ry == ry2 ? 0 : 2);
+                    vals[k] = addmod(vals[k], mulmod(p.a, gv, M), M);
+                    if (FQ2) vals[k + 1] = addmod(vals[k + 1], mulmod(p.b, gv, M), M);
+                }
+            }
+            if (o == 0 && y == y2) vals[24] = addmod(vals[24], gv, M);  // tr G
+        }
+    }
+    block_sum<25>(vals, red, M);
+    if (tid < 6) {
+        const double f0a = red[4 * tid], f0b = red[4 * tid + 1];
+        msg[tid][0] = f0a;
+        msg[tid][1] = f0b;
+        msg[tid][2] = submod(red[4 * tid + 2], f0a, M);
+        msg[tid][3] = submod(red[4 * tid + 3], f0b, M);
+    }
+    if (tid == 0) *claim = claim_fault && row == 0 ? addmod(red[24], 1.0, M) : red[24];
+}
+
 // ---------------------------------------------------------------- final block
 // One block per (row, rep) runs the rest of the norm check.  FIRST: the whole
 // row (L <= SC_FINAL_MAX) from the witness.  Otherwise: rounds 0..3 from the
@@ -861,7 +1238,7 @@ __device__ __forceinline__ void gram_msgs(unsigned long long graw, int row, cons
 // entries per thread, at most 256 columns) into LDS.  The remaining rounds run
 // in LDS, V (length d) is captured for norm_check_3, and the verifier is
 // replayed over all rounds.
-template <bool FQ2, bool FIRST, bool SNR>
+template <bool FQ2, bool FIRST, bool SNR, bool G64 = false>
 __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
     const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow,
     const double *__restrict__ in, size_t len, int rows, int t0,
@@ -908,7 +1285,15 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
         const size_t S = len >> 4;
         // this thread's Gram entry, then (with the pass below) its column's 16
         // entries: the loads stay in flight through the Gram messages
-        const unsigned long long graw = gacc[(size_t)row * 256 + tid];
+        // G64: rounds 0..5 from the 64 x 64 Gram sums (10 tile pairs)
+        constexpr int TG = G64 ? 6 : 4;
+        const unsigned long long graw = G64 ? 0ULL : gacc[(size_t)row * 256 + tid];
+        double g64v[G64 ? G64_PAIRS : 1];
+        if constexpr (G64) {
+#pragma unroll
+            for (int q = 0; q < G64_PAIRS; q++)
+                g64v[q] = u64_mod(gacc[((size_t)row * G64_PAIRS + q) * 256 + tid], M);
+        }
         F2 col[16];
         if (pre && (size_t)tid < S) {
 #pragma unroll
@@ -918,9 +1303,12 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_final(
                 col[y].b = FQ2 ? p[1] : 0.0;
             }
         }
-        gram_msgs<FQ2, SNR>(graw, row, chal, msg, &claim_sh, claim_fault, red, nr, M);
-        for (int x = tid; x < (t0 - 4) * 4; x += SC_THREADS) {
-            const int tt = 4 + x / 4, k = x % 4;
+        if constexpr (G64)
+            gram64_msgs<FQ2, SNR>(g64v, row, chal, msg, &claim_sh, claim_fault, red, nr, M);
+        else
+            gram_msgs<FQ2, SNR>(graw, row, chal, msg, &claim_sh, claim_fault, red, nr, M);
+        for (int x = tid; x < (t0 - TG) * 4; x += SC_THREADS) {
+            const int tt = TG + x / 4, k = x % 4;
             msg[tt][k] = u64_mod(macc[macc_idx(rep, tt, row, nrounds, rows) + k], M);
         }
         if (pre) {
@@ -1159,7 +1547,7 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
     const int n = __builtin_ctzll(L);
     const int want_rhs = o.rhs != nullptr;
     size_t need_m, need_a, need_b;
-    nc_scratch_need(rows, L, reps, &need_m, &need_a, &need_b);
+    nc_scratch_need(rows, L, reps, &need_m, &need_a, &need_b, sc != nullptr);
     unsigned long long *macc = nullptr;
     const size_t nmacc = reps * (size_t)n * rows * 6;
     if (ext) {
@@ -1180,8 +1568,10 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
     }
     // the passes' message accumulators, the rows' Gram sums (rows x 256) and
     // the flag of first-pass sub-blocks left to k_sc_gram_rest start at zero
-    unsigned long long *gacc = macc + nmacc, *need_long = gacc + rows * 256;
-    const size_t nzero = nmacc + rows * 256 + 1;
+    // (with a short copy the first pass may keep 10 tile pairs of 256 sums)
+    const size_t gram = sc != nullptr ? kGram64Entries : 256;
+    unsigned long long *gacc = macc + nmacc, *need_long = gacc + rows * gram;
+    const size_t nzero = nmacc + rows * gram + 1;
     hipLaunchKernelGGL(k_zero_u64, dim3(ceil_div(nzero, 256)), dim3(256), 0, ctx->stream, macc,
                        nzero);
     SALSAA_HIP(hipGetLastError());
@@ -1195,7 +1585,48 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         SALSAA_TRY(scratch_get(ctx, 1, need_a, (void **)&bufA));
         SALSAA_TRY(scratch_get(ctx, 2, need_b, (void **)&bufB));
     }
-    {
+    // With a short copy and L >= 2^16 the first pass runs six rounds from a
+    // 64 x 64 Gram matrix (its folded output, L / 64 per repetition, still
+    // fills the final block)
+    const bool g64 = sc != nullptr && (L >> 6) >= (size_t)SC_FINAL_MAX;
+    if (g64) {
+        const size_t S = L >> 6;
+        // columns per thread and blocks as in the 16-segment pass below
+        int cpt = cpt_first > 0 ? std::min(cpt_first, 8) : 1;
+        if (cpt_first <= 0)
+            while (cpt < 8 && rows * ceil_div(S, (size_t)GR_X * cpt) > (size_t)ctx->ncu * 3) cpt *= 2;
+        while (cpt < 64 && ceil_div(S, (size_t)GR_X * cpt) > kGramBlocksMax) cpt *= 2;
+        const size_t nsub = ceil_div(S, (size_t)GR_X * cpt);
+        int sbn = std::max(1, std::min(8, 64 / cpt));
+        while (sbn > 1 && ceil_div(nsub, (size_t)sbn) * rows < (size_t)ctx->ncu * 2) sbn >>= 1;
+        const unsigned bx = (unsigned)ceil_div(nsub, (size_t)sbn);
+        dim3 grid(bx, (unsigned)rows);
+        dim3 ggrid = grid;
+        const size_t walk = std::max<size_t>(1, 64 / ((size_t)sbn * cpt));
+        ggrid.x = (unsigned)ceil_div((size_t)bx, walk);
+        SALSAA_CHECK((size_t)bx + (size_t)ggrid.x <= kGramBlocksMax,
+                     "norm check row too long (at most 2^32 coefficients)");
+        const uint32_t *shf = sc->flag;
+        for (size_t r0 = 0; r0 < reps; r0 += G64_MAXREP) {
+            const int nrep = (int)std::min<size_t>(G64_MAXREP, reps - r0);
+            if (r0 == 0) {
+                hipLaunchKernelGGL((k_sc_gram64_short<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
+                                   ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn, need_long);
+                hipLaunchKernelGGL((k_sc_gram64_rest<FQ2, SNR, true>), ggrid, dim3(SC_THREADS), 0,
+                                   ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
+                                   nr, cpt, bufA, gacc, M, shf, sbn, (int)bx, need_long);
+            } else {
+                hipLaunchKernelGGL((k_sc_gram64_short<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0,
+                                   ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn, need_long);
+                hipLaunchKernelGGL((k_sc_gram64_rest<FQ2, SNR, false>), ggrid, dim3(SC_THREADS), 0,
+                                   ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
+                                   nr, cpt, bufA, gacc, M, shf, sbn, (int)bx, need_long);
+            }
+            SALSAA_HIP(hipGetLastError());
+        }
+    } else {
         // one streaming pass: the folds after round 3 and the Gram matrix
         // (rounds 0..3's messages follow from it in the final block)
         const size_t S = len1;
@@ -1266,8 +1697,8 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
             SALSAA_HIP(hipGetLastError());
         }
     }
-    size_t len = len1;
-    int t0 = 4;
+    size_t len = g64 ? L >> 6 : len1;
+    int t0 = g64 ? 6 : 4;
     double *src = bufA, *dst = bufB;
     // streaming passes while the final block cannot take the rest
     while (len > (size_t)SC_TAIL_MAX) {
@@ -1283,9 +1714,16 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         src = dst;
         dst = tmp;
     }
-    hipLaunchKernelGGL((k_sc_final<FQ2, false, SNR>), dim3((unsigned)rows, (unsigned)reps),
-                       dim3(SC_THREADS), 0, ctx->stream, nullptr, nullptr, src, len, (int)rows, t0, cs,
-                       cs_row, n, nr, macc, gacc, fault_claim, o, want_rhs, ctx->q, ctx->d_tables, M);
+    if (g64)
+        hipLaunchKernelGGL((k_sc_final<FQ2, false, SNR, true>), dim3((unsigned)rows, (unsigned)reps),
+                           dim3(SC_THREADS), 0, ctx->stream, nullptr, nullptr, src, len, (int)rows, t0,
+                           cs, cs_row, n, nr, macc, gacc, fault_claim, o, want_rhs, ctx->q,
+                           ctx->d_tables, M);
+    else
+        hipLaunchKernelGGL((k_sc_final<FQ2, false, SNR>), dim3((unsigned)rows, (unsigned)reps),
+                           dim3(SC_THREADS), 0, ctx->stream, nullptr, nullptr, src, len, (int)rows, t0,
+                           cs, cs_row, n, nr, macc, gacc, fault_claim, o, want_rhs, ctx->q,
+                           ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }
@@ -1293,10 +1731,11 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
 }  // namespace
 
 void nc_scratch_need(size_t rows, size_t L, size_t reps, size_t *macc_bytes, size_t *bufA_bytes,
-                     size_t *bufB_bytes) {
+                     size_t *bufB_bytes, bool short_copy) {
+    const size_t gram = short_copy ? kGram64Entries : 256;  // Gram sums per row
     const size_t n = (L >= 2) ? (size_t)__builtin_ctzll(L) : 0;
     const size_t len1 = L >> 4;
-    *macc_bytes = (L > (size_t)SC_FINAL_MAX ? reps * n * rows * 6 + rows * 256 + 1 : 0) *
+    *macc_bytes = (L > (size_t)SC_FINAL_MAX ? reps * n * rows * 6 + rows * gram + 1 : 0) *
                   sizeof(unsigned long long);
     *bufA_bytes = L > (size_t)SC_FINAL_MAX ? reps * rows * len1 * 2 * sizeof(double) : 0;
     *bufB_bytes =

@@ -287,6 +287,43 @@ def test_batch_split_pass_equals_single_pass(sa, ctx, orc):
     key.close()
 
 
+def test_batch_commit_schedules(sa, ctx, orc):
+    # The commit runs as one launch per class of key row tiles or as one
+    # launch over every class (blocks of all classes that read one key chunk
+    # side by side); each class here holds several jobs of 16 instances, the
+    # last one partial, and kappa values inside each class.  Both schedules
+    # give the plain path's outputs, eagerly and from graph replays, and the
+    # oracle's commitments.
+    say("case: batch commit schedules")
+    rng = np.random.default_rng(17)
+    n = 75
+    m = [int(1 << rng.integers(0, 10)) for _ in range(n)]
+    kappa = [int(rng.choice([3, 8, 9, 12, 16, 18, 20, 32])) for _ in range(n)]
+    A_h = orc.sample_uniform(15, sa.STREAM_A, (32, 512, 64), Q)
+    key = ctx.key(ctx.to_device(A_h))
+    b = sa.Batch(ctx, key, m, kappa, reps=1, fq2=1, tune=False)
+    fill(sa, ctx, b, 2100)
+    W0 = [b.slot(i)["witness"].download() for i in range(n)]
+    b.step_plain()
+    plain = snapshot(b, True)
+    for sched in ("mixed", "per_class", "mixed"):
+        say("  schedule", sched)
+        b.set_commit_schedule(sched)
+        for graph in (False, True, True):
+            for i in range(n):
+                c = b.slot(i)["commitment"]
+                c.upload(np.zeros(c.shape, np.uint64))
+            b.step(graph=graph)
+            ctx.sync()
+            same(snapshot(b, False), plain)
+    for i in sorted(set([int(np.argmax(m)), 0, 1, 2, n - 1])):
+        Wi = W0[i].reshape(1, m[i], 64)
+        want = orc.commit(np.ascontiguousarray(A_h[: kappa[i], : m[i]]), Wi, Q, threads=8)
+        assert np.array_equal(plain[i]["commitment"], want), i
+    b.close()
+    key.close()
+
+
 def test_batch_ntt_grid(sa, ctx, orc):
     # The step's NTT pass runs on any number of persistent blocks (the tuner
     # picks the fastest of one per CU, 7/8 and 3/4 of that): every grid gives
@@ -321,6 +358,8 @@ def test_batch_with_generated_crs_key(sa, ctx, orc):
     module = orc.sample_uniform(13, sa.STREAM_A, (32, 64), Q)
     key = ctx.crs_key(ctx.to_device(module), 2048, implicit=True)
     b = sa.Batch(ctx, key, m, kappa, reps=2, fq2=1, tune=True)
+    with pytest.raises(sa.SalsaaError):
+        b.set_commit_schedule("mixed")  # the mixed launch reads a stored key
     fill(sa, ctx, b, 1300)
     W0 = [b.slot(i)["witness"].download() for i in range(n)]
     b.step_plain()
