@@ -8,6 +8,7 @@ the reference's function names (parallel_dot_series_matrix, norm_check,
 decomp_ell, parallel_dot_matrix_matrix, split) on HBM-resident matrices.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -240,7 +241,7 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr is not None and self.owner:
+        if self.ptr is not None and self.owner and self.ctx.h:
             lib.salsaa_dev_free(self.ctx.h, self.ptr)
         self.ptr = None
 
@@ -254,11 +255,14 @@ class DeviceArray:
 class Key:
     """Resident commitment key (A in the slot-major full-NTT layout)."""
 
+    _close_order = 3
+
     def __init__(self, ctx, A_dev, kappa, m):
         self.ctx, self.kappa, self.m = ctx, kappa, m
         p = ctypes.c_void_p()
         _check(lib.salsaa_key_create(ctx.h, A_dev.ptr, kappa, m, ctypes.byref(p)))
         self.h = p.value
+        ctx._own(self)
 
     @classmethod
     def crs_implicit(cls, ctx, module, m):
@@ -269,6 +273,7 @@ class Key:
         p = ctypes.c_void_p()
         _check(lib.salsaa_key_crs_implicit(ctx.h, module.ptr, k.kappa, m, ctypes.byref(p)))
         k.h = p.value
+        ctx._own(k)
         return k
 
     @classmethod
@@ -281,6 +286,7 @@ class Key:
         p = ctypes.c_void_p()
         _check(lib.salsaa_key_create_crs(ctx.h, module.ptr, k.kappa, m, ctypes.byref(p)))
         k.h = p.value
+        ctx._own(k)
         return k
 
     def view(self, kappa, m):
@@ -291,6 +297,8 @@ class Key:
         p = ctypes.c_void_p()
         _check(lib.salsaa_key_view(self.h, kappa, m, ctypes.byref(p)))
         v.h = p.value
+        v._close_order = 2  # views before keys
+        self.ctx._own(v)
         return v
 
     def close(self):
@@ -317,9 +325,18 @@ class Context:
         p = ctypes.c_void_p()
         _check(lib.salsaa_ctx_create(q, d, ctypes.byref(p)))
         self.h, self.q, self.d = p.value, q, d
+        # keys, batches and servers made on this context: closed before it
+        self._deps = weakref.WeakSet()
+
+    def _own(self, obj):
+        self._deps.add(obj)
 
     def close(self):
         if self.h:
+            # servers and batches first, then key views, then keys (a native
+            # object must not outlive its context)
+            for obj in sorted(list(self._deps), key=lambda o: o._close_order):
+                obj.close()
             lib.salsaa_ctx_destroy(self.h)
             self.h = None
 
@@ -708,6 +725,8 @@ class Batch:
     """Many independent prover steps (NTT forward + inverse, commit, norm
     check) over one resident key, in grouped launches (salsaa_batch_*)."""
 
+    _close_order = 1
+
     def __init__(self, ctx, key, m, kappa, reps=2, fq2=True, nr=NR, tune=True):
         self.ctx, self.key = ctx, key
         self.n = len(m)
@@ -720,6 +739,7 @@ class Batch:
         _check(lib.salsaa_batch_create(ctx.h, key.h, self.n, M, K, reps, int(fq2), nr, int(tune),
                                        ctypes.byref(p)))
         self.h = p.value
+        ctx._own(self)
 
     def slot(self, i, plain=False):
         """Device views of instance i's witness, challenges and outputs
@@ -818,6 +838,8 @@ class Server:
     """Continuous-batching commit + norm-check service (salsaa_server_*).
     max_batch_jobs=1, inflight=1 serves each job alone (the plain path)."""
 
+    _close_order = 0
+
     def __init__(self, ctx, key, kappa, arena_elems, max_batch_jobs, max_batch_elems,
                  inflight=2, reps=2, fq2=True, nr=NR, kchunk=64):
         self.ctx, self.key, self.kappa, self.reps = ctx, key, kappa, reps
@@ -826,6 +848,7 @@ class Server:
         p = ctypes.c_void_p()
         _check(lib.salsaa_server_create(ctx.h, key.h, kappa, ctypes.byref(o), ctypes.byref(p)))
         self.h = p.value
+        ctx._own(self)
         offs = (ctypes.c_size_t * 8)()
         _check(lib.salsaa_server_layout(self.h, offs))
         self.off = list(offs)

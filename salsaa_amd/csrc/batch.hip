@@ -31,7 +31,11 @@
 namespace salsaa {
 namespace {
 
-constexpr int kNcStreams = 3;
+// Two: HIP maps streams onto a few hardware queues (four by default); the
+// context's stream and two norm-check streams get queues of their own, while
+// a third norm-check stream shared one with the second and serialised its
+// classes behind it.
+constexpr int kNcStreams = 2;
 
 __global__ void k_batch_zero(uint64_t *p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -759,45 +759,24 @@ __device__ __forceinline__ void g64_pair(int p, int &a, int &b) {
 }
 
 // The block's fold weights e_6(y) for repetitions rep0 .. rep0 + nrep - 1,
-// split as in GrCoef (spare repetition slots zero), and -128 sum_y e_6(y)
-// (the short pass folds z = b + 128).
+// balanced (spare repetition slots zero).
 struct G64Coef {
     F2 coef[G64_MAXREP][64];
-    __align__(32) double csplit[G64_MAXREP][64][4];
-    double coff[G64_MAXREP][2];
 };
 template <bool FQ2, bool SNR>
 __device__ __forceinline__ void g64_coefs(G64Coef &C, const uint64_t *cs, int rep0, int nrep,
                                           int nrounds, double nr, Mod M, int tid) {
     if (tid < 64 * G64_MAXREP) {
         const int r = tid >> 6, y = tid & 63;
+        F2 a = {0.0, 0.0};
         if (r < nrep) {
-            F2 a = {1.0, 0.0};
+            a = {1.0, 0.0};
             for (int t = 0; t < 6; t++) {
                 const F2 c = load_chal<FQ2>(cs, rep0 + r, nrounds, t);
                 a = x_mul<FQ2, SNR>(a, ((y >> (5 - t)) & 1) ? c : x_sub<FQ2>(F2{1.0, 0.0}, c, M), nr, M);
             }
-            const double ca = balance(a.a, M), cb = balance(a.b, M);
-            C.coef[r][y] = {ca, cb};
-            const double ha = __builtin_rint(ca * 0x1p-25), hb = __builtin_rint(cb * 0x1p-25);
-            C.csplit[r][y][0] = ha;
-            C.csplit[r][y][1] = ca - ha * 0x1p25;
-            C.csplit[r][y][2] = hb;
-            C.csplit[r][y][3] = cb - hb * 0x1p25;
-        } else {
-            C.coef[r][y] = {0.0, 0.0};
-            C.csplit[r][y][0] = C.csplit[r][y][1] = C.csplit[r][y][2] = C.csplit[r][y][3] = 0.0;
         }
-    }
-    __syncthreads();
-    if (tid < 2 * nrep) {
-        const int r = tid >> 1, comp = tid & 1;
-        double sum = 0.0;  // 64 balanced terms, reduced every 8 (below 4.5 q + 16)
-        for (int y = 0; y < 64; y++) {
-            sum += comp ? C.coef[r][y].b : C.coef[r][y].a;
-            if ((y & 7) == 7) sum = reduce_lazy(sum, M);
-        }
-        C.coff[r][comp] = reduce_lazy(mul_lazy(sum, -128.0, M), M);
+        C.coef[r][y] = {balance(a.a, M), balance(a.b, M)};
     }
     __syncthreads();
 }
@@ -829,23 +808,61 @@ __device__ __forceinline__ void g64_frags(const uint16_t *src, v4i &f0, v4i &f1)
     }
 }
 
+// The fold weights of one block as matrix-core A fragments: row 4 g + e of
+// tile mt is digit u = 4 mt + e (u < 7; row u = 7 is zero) of the weight of
+// (repetition, component) g = 2 r + comp, the K index the segment y.  Lane
+// (row, kg) holds segments 16 kg .. 16 kg + 15.  A balanced weight w
+// (|w| < 2^49) has w + 0x0080808080808080 in [0, 2^56) as an integer, whose
+// byte u is a_u + 128 for the signed digits w = sum_{u<7} a_u 256^u.
+__device__ __forceinline__ void g64_wfrags(const G64Coef &C, int lane, v4i (&A)[2]) {
+    const int row = lane & 15, kg = lane >> 4, g = row >> 2, e = row & 3;
+    const int r = g >> 1, comp = g & 1;
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) {
+        const int u = 4 * mt + e;
+#pragma unroll
+        for (int k4 = 0; k4 < 4; k4++) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int bq = 0; bq < 4; bq++) {
+                const int y = 16 * kg + 4 * k4 + bq;
+                const double w = comp ? C.coef[r][y].b : C.coef[r][y].a;
+                const uint64_t t = (uint64_t)(long long)w + GR_BIAS;
+                const uint32_t d = u < 7 ? (((uint32_t)(t >> (8 * u)) & 0xFFu) ^ 0x80u) : 0u;
+                word |= d << (8 * bq);
+            }
+            A[mt][k4] = (int)word;
+        }
+    }
+}
+
 // The first pass over the short copy with 64 segments: the sub-blocks of cpt
 // GR_X columns x < S whose tiles in all 64 segments are short.  Per
 // iteration, wave w stages the 64 segments of columns 64 w .. 64 w + 63 (one
-// per lane, prefetched a whole iteration ahead) into the block's u16 rows and
-// folds its lane's column; after a block barrier wave w multiplies the tile
-// pairs p = w, w + 4, w + 8 over all 256 columns on the matrix cores (so each
-// wave keeps at most 3 pairs of E_0..E_2 accumulators), and a second barrier
-// frees the rows.  Each wave adds its pairs' sums to gacc at the end.
-constexpr int G64_LD = GR_X + 8;  // u16 row stride of the staging rows
+// per lane, prefetched a whole iteration ahead) twice: into the block's rows
+// (segment-major, for the Gram matrix) and into its own columns
+// (column-major, for the fold), as z = b + 128, whose low byte xor 128 and
+// high byte are the signed base-256 digits of b.  The fold of 16 columns is
+// one set of matrix-core products: the 7 digits of the weights of both
+// repetitions and components (A, 32 rows) times the 2 digits of b (B),
+// K = the 64 segments, each product sum exact in int32 (|sum| <= 2^20); lane
+// (n, g) then holds all 14 digit sums of one output, (repetition,
+// component) g of column n, and recombines them with the weights 2^(8k) mod
+// q.  After a block barrier wave w multiplies the Gram tile pairs p = w,
+// w + 4, w + 8 over all 256 columns (so each wave keeps at most 3 pairs of
+// E_0..E_2 accumulators), and a second barrier frees the rows.  Each wave
+// adds its pairs' sums to gacc at the end.
+constexpr int G64_LD = GR_X + 8;  // u16 row stride of the segment-major rows
+constexpr int G64_CLD = 64 + 8;   // u16 stride of the column-major columns
 template <bool FQ2, bool SNR, bool GRAM>
-__global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void k_sc_gram64_short(
+__global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_sc_gram64_short(
     const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
     const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr, int cpt,
     double *__restrict__ out, unsigned long long *__restrict__ gacc, const double *__restrict__ tab,
     Mod M, const uint16_t *__restrict__ sht, const uint32_t *__restrict__ shf, int sbn,
     unsigned long long *__restrict__ need_long) {
-    __shared__ __align__(16) uint16_t T16[64 * G64_LD];  // T16[y][c]: segment y, block column c
+    __shared__ __align__(16) uint16_t T16[64 * G64_LD];   // T16[y][c]: segment y, block column c
+    __shared__ __align__(16) uint16_t TC[GR_X * G64_CLD];  // TC[c][y]
     __shared__ G64Coef C;
     const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -864,6 +881,14 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     if (todo == 0) return;
     g64_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
     const int fr = lane & 15, fg = lane >> 4;
+    v4i Aw[2];
+    g64_wfrags(C, lane, Aw);
+    const double *wt = tab + TAB_W256;
+    // this lane's output of the fold: (repetition, component) fg and its
+    // offset in the [rep][row][S][2] output
+    const int fo_r = fg >> 1, fo_c = fg & 1;
+    const bool fo_live = fo_r < nrep && (FQ2 || fo_c == 0);
+    double *fo_base = out + ((size_t)(rep0 + fo_r) * rows + row) * S * 2 + fo_c;
     const uint16_t *tr = sht + base;
     v4i acc[3][3];
 #pragma unroll
@@ -879,6 +904,7 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
         for (int y = 0; y < 64; y++) t[y] = tr[(uint32_t)y * S32 + xo];
     };
     uint16_t *mycol = T16 + 64 * wave + lane;
+    uint16_t *myTC = TC + (64 * wave + lane) * G64_CLD;
     for (int k = 0; k < sbn; k++) {
         if (!((todo >> k) & 1)) continue;
         size_t xb = (sb0 + k) * span;  // the block's columns (uniform)
@@ -888,44 +914,52 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
             const size_t x = xb + 64 * wave + lane;
             const bool live = x < S;
             const bool more = cc + 1 < cpt && xb + GR_X < S;  // uniform in the block
-            // z = t - 2^14 + 128 = b + 128 (a dead column stages b = 0)
+            // z = t - 2^14 + 128 = b + 128 (a dead column stages b = 0), into
+            // the segment-major rows and the lane's own column
 #pragma unroll
-            for (int y = 0; y < 64; y++)
-                mycol[G64_LD * y] = (uint16_t)(live ? tv[y] - (GR_SMALL - 128u) : 128u);
+            for (int y = 0; y < 64; y++) tv[y] = live ? tv[y] - (GR_SMALL - 128u) : 128u;
+#pragma unroll
+            for (int y = 0; y < 64; y++) mycol[G64_LD * y] = (uint16_t)tv[y];
+#pragma unroll
+            for (int q8 = 0; q8 < 8; q8++) {
+                uint4 v;
+                v.x = (tv[8 * q8 + 0] & 0xFFFFu) | (tv[8 * q8 + 1] << 16);
+                v.y = (tv[8 * q8 + 2] & 0xFFFFu) | (tv[8 * q8 + 3] << 16);
+                v.z = (tv[8 * q8 + 4] & 0xFFFFu) | (tv[8 * q8 + 5] << 16);
+                v.w = (tv[8 * q8 + 6] & 0xFFFFu) | (tv[8 * q8 + 7] << 16);
+                reinterpret_cast<uint4 *>(myTC)[q8] = v;
+            }
             fetch(tv, more ? x + GR_X : x);
-            // the short fold of the lane's column over the 64 segments, of
-            // z = b + 128 (|z| < 2^15: the sums of c_h z and c_l z stay below
-            // 2^45; the repetitions in a runtime loop, so the weights are read
-            // per column)
-            for (int r = 0; r < nrep; r += 2) {
-                double h0 = 0.0, l0 = 0.0, h1 = 0.0, l1 = 0.0;  // repetition r: a, b
-                double h2 = 0.0, l2 = 0.0, h3 = 0.0, l3 = 0.0;  // repetition r + 1
-#pragma unroll 16
-                for (int y = 0; y < 64; y++) {
-                    const double tv_y = (double)(int16_t)mycol[G64_LD * y];
-                    const double4 c0 = *reinterpret_cast<const double4 *>(&C.csplit[r][y][0]);
-                    const double4 c1 = *reinterpret_cast<const double4 *>(&C.csplit[r + 1][y][0]);
-                    h0 = __fma_rn(c0.x, tv_y, h0);
-                    l0 = __fma_rn(c0.y, tv_y, l0);
-                    h2 = __fma_rn(c1.x, tv_y, h2);
-                    l2 = __fma_rn(c1.y, tv_y, l2);
-                    if (FQ2) {
-                        h1 = __fma_rn(c0.z, tv_y, h1);
-                        l1 = __fma_rn(c0.w, tv_y, l1);
-                        h3 = __fma_rn(c1.z, tv_y, h3);
-                        l3 = __fma_rn(c1.w, tv_y, l3);
-                    }
+            wave_sync();  // the wave's columns are in TC
+            // the fold of the wave's 64 columns, 16 at a time
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                v4i b0, b1;
+                g64_frags(TC + (64 * wave + 16 * j + fr) * G64_CLD + 16 * fg, b0, b1);
+                const v4i z4 = (v4i){0, 0, 0, 0};
+                const v4i s00 = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aw[0], b0, z4, 0, 0, 0);
+                const v4i s01 = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aw[0], b1, z4, 0, 0, 0);
+                const v4i s10 = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aw[1], b0, z4, 0, 0, 0);
+                const v4i s11 = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aw[1], b1, z4, 0, 0, 0);
+                // digit sums S[u][v]: u < 4 from tile 0 (s00, s01), u = 4..6
+                // from tile 1 (s10, s11); E_k = sum_{u + v = k} S[u][v]
+                const double E[8] = {(double)s00[0],
+                                     (double)s00[1] + (double)s01[0],
+                                     (double)s00[2] + (double)s01[1],
+                                     (double)s00[3] + (double)s01[2],
+                                     (double)s10[0] + (double)s01[3],
+                                     (double)s10[1] + (double)s11[0],
+                                     (double)s10[2] + (double)s11[1],
+                                     (double)s11[2]};
+                // the digits of z - 128 = b: sum_k E_k 2^(8k) = sum_y w(y) b(y)
+                double v = 0.0;
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++) {
+                    v += mul_lazy(E[kk], wt[kk], M);
+                    if (kk == 3) v = reduce_lazy(v, M);
                 }
-                if (live) {
-                    double *o = out + (((size_t)(rep0 + r) * rows + row) * S + x) * 2;
-                    o[0] = gr_small_sum(h0, l0, C.coff[r][0], M);
-                    o[1] = FQ2 ? gr_small_sum(h1, l1, C.coff[r][1], M) : 0.0;
-                    if (r + 1 < nrep) {
-                        o += (size_t)rows * S * 2;
-                        o[0] = gr_small_sum(h2, l2, C.coff[r + 1][0], M);
-                        o[1] = FQ2 ? gr_small_sum(h3, l3, C.coff[r + 1][1], M) : 0.0;
-                    }
-                }
+                const size_t xo = xb + 64 * wave + 16 * j + fr;
+                if (fo_live && xo < S) fo_base[xo * 2] = reduce_lazy(v, M);
             }
             if (GRAM) {
                 __syncthreads();  // every wave's columns are staged
@@ -947,6 +981,8 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
                     }
                 }
                 __syncthreads();  // the rows are rewritten by the next iteration
+            } else {
+                wave_sync();  // the wave's TC columns are rewritten by the next iteration
             }
             if (!more) break;
             xb += GR_X;
@@ -954,7 +990,6 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     }
     if (GRAM) {
         // lane (r, g) holds E_k[4 g + e][r] of each of the wave's pairs
-        const double *wt = tab + TAB_W256;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             const int q = wave + 4 * i;
