@@ -904,6 +904,8 @@ def main():
     ap.add_argument("--eager", action="store_true",
                     help="run the batched steps eagerly instead of replaying the hipGraph "
                          "(for counter collection)")
+    ap.add_argument("--hbm-mix", action="store_true",
+                    help="print the HBM rate of a plain streaming kernel at several read/write mixes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -915,6 +917,21 @@ def main():
 
     import salsaa_amd as sa
     ctx = sa.Context(sa.Q, 64)
+    if args.hbm_mix:
+        n16 = 1 << 26  # 1 GiB per stream
+        res = []
+        for rn, wn, ip in ((1, 0, False), (0, 1, False), (1, 1, False), (1, 1, True), (1, 2, False),
+                           (1, 2, True), (7, 15, False), (7, 15, True)):
+            buf = ctx.empty(((rn + wn - (1 if ip else 0)) * n16 * 2,))
+            for bpc in (4, 8, 16):
+                ctx.memstream(buf, n16, rn, wn, ip, bpc)
+                _, tot, _ = timed(ctx, lambda: ctx.memstream(buf, n16, rn, wn, ip, bpc), 5)
+                res.append({"read_streams": rn, "write_streams": wn, "in_place": ip,
+                            "blocks_per_cu": bpc,
+                            "GB/s": round((rn + wn) * n16 * 16 / (tot / 5 * 1e-3) / 1e9, 1)})
+            buf.free()
+        print(json.dumps({"hbm_mix": res, "stream_bytes": n16 * 16, "launches": 5}), flush=True)
+        return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 8
         src, dst = ctx.uniform((n,), 0, 1), ctx.empty((n,))
