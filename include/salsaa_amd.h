@@ -450,16 +450,18 @@ int salsaa_batch_tuning_grid(const salsaa_batch *b, unsigned *blocks, unsigned *
 /* Sets the NTT pass's persistent blocks (0: one per CU); the step is captured
  * again on its next graph replay.  Waits for the context's stream. */
 int salsaa_batch_set_ntt_grid(salsaa_batch *b, unsigned blocks);
-/* The commit schedule the tuner chose (mixed 0: one launch per class of key
- * row tiles, 1: one launch over every class, in which the blocks of all
- * classes that read one (slot, K chunk) of a stored key run side by side and
- * share it through L2) and its best times of the commit with each
- * (milliseconds; mixed_ms 0 when the mixed launch does not apply: a generated
- * CRS key or a single class). */
-int salsaa_batch_tuning_commit(const salsaa_batch *b, int *mixed, float *class_ms, float *mixed_ms);
-/* Sets the commit schedule (mixed 1 needs a stored key); the step is captured
+/* The commit schedule the tuner chose (0: one launch per class of key row
+ * tiles; 1: one launch over every class, in which the blocks of all classes
+ * that read one (slot, K chunk) of a stored key run side by side and share it
+ * through L2; 2: one such launch over the classes of at most 8 key row tiles
+ * (kappa <= 18) and the other classes' own launches) and its best times of
+ * the commit with each, ms[3] (milliseconds; 0 for a schedule that does not
+ * apply: a generated CRS key, or fewer than two classes to mix).  Either
+ * pointer may be NULL. */
+int salsaa_batch_tuning_commit(const salsaa_batch *b, int *schedule, float *ms);
+/* Sets the commit schedule (1 and 2 need a stored key); the step is captured
  * again on its next graph replay.  Waits for the context's stream. */
-int salsaa_batch_set_commit_schedule(salsaa_batch *b, int mixed);
+int salsaa_batch_set_commit_schedule(salsaa_batch *b, int schedule);
 
 /* ------------------------------------------------------------------ folding
  * The folding step of n accumulated instances (the protocol's folding round,

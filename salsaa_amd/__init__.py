@@ -183,7 +183,6 @@ def _load():
                                             ctypes.POINTER(ctypes.c_float)]),
         "salsaa_batch_set_ntt_grid": (_int, [_vp, ctypes.c_uint]),
         "salsaa_batch_tuning_commit": (_int, [_vp, ctypes.POINTER(ctypes.c_int),
-                                              ctypes.POINTER(ctypes.c_float),
                                               ctypes.POINTER(ctypes.c_float)]),
         "salsaa_batch_set_commit_schedule": (_int, [_vp, _int]),
     }
@@ -721,6 +720,10 @@ class Context:
                     times=dict(zip(PROTOCOL_PHASES, list(times))))
 
 
+#: the batched commit's schedules, in salsaa_batch_set_commit_schedule's order
+COMMIT_SCHEDULES = ("per_class", "mixed", "mixed_small")
+
+
 class Batch:
     """Many independent prover steps (NTT forward + inverse, commit, norm
     check) over one resident key, in grouped launches (salsaa_batch_*)."""
@@ -800,15 +803,14 @@ class Batch:
                                            ctypes.byref(td)))
         g, gc, gt = ctypes.c_uint(), (ctypes.c_uint * 3)(), (ctypes.c_float * 3)()
         _check(lib.salsaa_batch_tuning_grid(self.h, ctypes.byref(g), gc, gt))
-        cm, cc, cx = ctypes.c_int(), ctypes.c_float(), ctypes.c_float()
-        _check(lib.salsaa_batch_tuning_commit(self.h, ctypes.byref(cm), ctypes.byref(cc),
-                                              ctypes.byref(cx)))
+        cm, cms = ctypes.c_int(), (ctypes.c_float * 3)()
+        _check(lib.salsaa_batch_tuning_commit(self.h, ctypes.byref(cm), cms))
         return dict(kchunk=kc.value, cpt_first=cpt.value, commit_ms=tc.value,
                     norm_check_ms=tn.value, ntt_schedule=("lds", "dpp")[x.value],
                     ntt_pass_ms={"lds": tl.value, "dpp": td.value}, ntt_blocks=g.value,
                     ntt_blocks_ms={str(gc[i]): gt[i] for i in range(3) if gc[i]},
-                    commit_schedule=("per_class", "mixed")[cm.value],
-                    commit_schedule_ms={"per_class": cc.value, "mixed": cx.value})
+                    commit_schedule=COMMIT_SCHEDULES[cm.value],
+                    commit_schedule_ms={n: cms[i] for i, n in enumerate(COMMIT_SCHEDULES)})
 
     def set_ntt_schedule(self, name):
         """The NTT pass's exchange: "lds" or "dpp" (quad permutes)."""
@@ -819,8 +821,10 @@ class Batch:
         _check(lib.salsaa_batch_set_ntt_grid(self.h, int(blocks)))
 
     def set_commit_schedule(self, name):
-        """The commit's launches: "per_class" or "mixed" (one launch, stored key)."""
-        _check(lib.salsaa_batch_set_commit_schedule(self.h, {"per_class": 0, "mixed": 1}[name]))
+        """The commit's launches (COMMIT_SCHEDULES): "per_class", "mixed" (one
+        launch over every class) or "mixed_small" (one launch over the classes
+        of kappa <= 18 beside the others'); the mixed ones need a stored key."""
+        _check(lib.salsaa_batch_set_commit_schedule(self.h, COMMIT_SCHEDULES.index(name)))
 
     def close(self):
         if self.h:

@@ -123,13 +123,16 @@ struct salsaa_batch {
     CommitJob *d_cjobs = nullptr;
     uint2 *d_table = nullptr;
     size_t table_cap = 0;
-    // commit schedule: one launch per class (0) or one launch over every
-    // class (1, stored keys: the classes share each key chunk through L2);
-    // the mixed launch's table follows the per-class tables; the tuner's
-    // best times of each schedule
+    // commit schedule (stored keys; the classes of a mixed launch share each
+    // key chunk through L2): 0 one launch per class; 1 one launch over every
+    // class at the largest shape; 2 one launch over the classes of at most 8
+    // key row tiles at their largest shape, and the others' own launches.
+    // The mixed tables (all classes, then the small ones) follow the
+    // per-class tables; the tuner's best time of each schedule
     int commit_mixed = 0;
-    size_t mixed_t0 = 0, mixed_nblk = 0;
-    float t_commit_mode[2] = {0.f, 0.f};
+    size_t mixed_t0[2] = {0, 0}, mixed_nblk[2] = {0, 0};
+    int mixed_tm[2] = {4, 2};
+    float t_commit_mode[3] = {0.f, 0.f, 0.f};
     FinJob *d_fjobs = nullptr;
     // norm check
     struct Group {
@@ -237,9 +240,9 @@ int build_commit_tables(salsaa_batch *b, size_t kc) {
         cc.nblk = table.size() - cc.t0;
         cc.fnblk = fblk;
     }
-    // the mixed launch: block order (z, s, class, job), so every block of one
-    // key chunk is consecutive whatever its class
-    b->mixed_t0 = table.size();
+    // the mixed launches: block order (z, s, class, job), so every block of
+    // one key chunk is consecutive whatever its class; [0] every class, [1]
+    // the classes of at most 8 key row tiles
     {
         int maxks = 0;
         std::vector<int> ks(b->njobs);
@@ -247,13 +250,18 @@ int build_commit_tables(salsaa_batch *b, size_t kc) {
             ks[j] = (int)((b->job_ktiles[j] + kc - 1) / kc);
             maxks = std::max(maxks, ks[j]);
         }
-        for (int z = 0; z < maxks; z++)
-            for (unsigned s = 0; s < 64; s++)
-                for (const auto &cc : b->cclass)
-                    for (size_t jj = 0; jj < cc.njobs && ks[cc.job0 + jj] > z; jj++)
-                        table.push_back(make_uint2((unsigned)(cc.job0 + jj), ((unsigned)z << 6) | s));
+        for (int mt = 0; mt < 2; mt++) {
+            b->mixed_t0[mt] = table.size();
+            for (int z = 0; z < maxks; z++)
+                for (unsigned s = 0; s < 64; s++)
+                    for (const auto &cc : b->cclass) {
+                        if (mt == 1 && (7 * cc.kappa + 15) / 16 > 8) continue;
+                        for (size_t jj = 0; jj < cc.njobs && ks[cc.job0 + jj] > z; jj++)
+                            table.push_back(make_uint2((unsigned)(cc.job0 + jj), ((unsigned)z << 6) | s));
+                    }
+            b->mixed_nblk[mt] = table.size() - b->mixed_t0[mt];
+        }
     }
-    b->mixed_nblk = table.size() - b->mixed_t0;
     SALSAA_CHECK(poff <= b->partial_cap, "commit partial buffer too small");
     SALSAA_CHECK(table.size() <= b->table_cap, "commit table too small");
     SALSAA_TRY(upload(ctx, b->d_cjobs, cj));
@@ -269,15 +277,20 @@ int run_planes(salsaa_batch *b) {
 }
 
 // one commit launch per class (its kappa and kernel shape) or, with a stored
-// key and the mixed schedule, one launch over every class; then one finalize
-// per class
+// key and a mixed schedule, one launch over every class (1) or over the
+// classes of at most 8 key row tiles beside the others' own launches (2);
+// then one finalize per class
 int run_commit(salsaa_batch *b) {
     const Key &k = b->key->k;
-    if (b->commit_mixed && !k.d_mhat)
+    const int mode = k.d_mhat ? 0 : b->commit_mixed;
+    if (mode && b->mixed_nblk[mode - 1])
         SALSAA_TRY(launch_commit_mfma_mixed(&b->ctx->c, k.d_AB, k.a_kt, k.a_ktiles, b->d_cjobs,
-                                            b->d_table + b->mixed_t0, b->mixed_nblk, b->kchunk));
-    else
+                                            b->d_table + b->mixed_t0[mode - 1],
+                                            b->mixed_nblk[mode - 1], b->kchunk,
+                                            b->mixed_tm[mode - 1]));
     for (const auto &cc : b->cclass) {
+        // a class in the mixed launch
+        if (mode == 1 || (mode == 2 && (7 * cc.kappa + 15) / 16 <= 8)) continue;
         if (k.d_mhat)  // a CRS key generated inside the commit
             SALSAA_TRY(launch_commit_crs_batch(&b->ctx->c, k.d_mhat, cc.kappa, b->d_cjobs + cc.job0,
                                                b->d_table + cc.t0, cc.nblk, b->kchunk,
@@ -427,15 +440,19 @@ int autotune(salsaa_batch *b) {
     for (int kt : b->job_ktiles) max_kt = std::max(max_kt, (size_t)kt);
     // the commit: K chunk and schedule (a launch per key row class, or one
     // launch over all classes when the key is stored)
-    const int nmode = b->key->k.d_mhat || b->cclass.size() < 2 ? 1 : 2;
+    size_t nsmall = 0;
+    for (const auto &cc : b->cclass) nsmall += (7 * cc.kappa + 15) / 16 <= 8;
+    const bool stored = b->key->k.d_mhat == nullptr;
+    const bool modes[3] = {true, stored && b->cclass.size() >= 2, stored && nsmall >= 2};
     size_t best_kc = 0;
     int best_mode = 0;
     float best = 0.f;
-    b->t_commit_mode[0] = b->t_commit_mode[1] = 0.f;
+    b->t_commit_mode[0] = b->t_commit_mode[1] = b->t_commit_mode[2] = 0.f;
     for (size_t kc : {16, 32, 64, 128, 256}) {
         if (kc > 2 * max_kt && best_kc) break;
         SALSAA_TRY(build_commit_tables(b, kc));
-        for (int mode = 0; mode < nmode; mode++) {
+        for (int mode = 0; mode < 3; mode++) {
+            if (!modes[mode]) continue;
             b->commit_mixed = mode;
             float ms = 0.f;
             SALSAA_TRY(time_ms(b, [&] { return run_commit(b); }, 2, &ms));
@@ -677,7 +694,7 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
         for (size_t j = cc.job0; j < cc.job0 + cc.njobs; j++) {
             const size_t ks = (b->job_ktiles[j] + 15) / 16;
             pcap += ks * 64 * cc.kappa * b->job_rows[j];
-            tcap += 2 * ks * 64;  // the per-class tables and the mixed one
+            tcap += 3 * ks * 64;  // the per-class tables and the two mixed ones
         }
     b->partial_cap = pcap;
     b->table_cap = tcap;
@@ -932,18 +949,18 @@ int salsaa_batch_set_ntt_grid(salsaa_batch *b, unsigned blocks) {
     return SALSAA_OK;
 }
 
-int salsaa_batch_tuning_commit(const salsaa_batch *b, int *mixed, float *class_ms, float *mixed_ms) {
+int salsaa_batch_tuning_commit(const salsaa_batch *b, int *schedule, float *ms) {
     SALSAA_CHECK(b != nullptr, "null batch");
-    if (mixed) *mixed = b->commit_mixed;
-    if (class_ms) *class_ms = b->t_commit_mode[0];
-    if (mixed_ms) *mixed_ms = b->t_commit_mode[1];
+    if (schedule) *schedule = b->commit_mixed;
+    if (ms)
+        for (int k = 0; k < 3; k++) ms[k] = b->t_commit_mode[k];
     return SALSAA_OK;
 }
 
 int salsaa_batch_set_commit_schedule(salsaa_batch *b, int mixed) {
     SALSAA_CHECK(b != nullptr, "null batch");
-    SALSAA_CHECK(mixed == 0 || mixed == 1, "commit schedule must be 0 (per class) or 1 (mixed)");
-    SALSAA_CHECK(!mixed || !b->key->k.d_mhat, "the mixed commit launch needs a stored key");
+    SALSAA_CHECK(mixed >= 0 && mixed <= 2, "commit schedule must be 0, 1 or 2");
+    SALSAA_CHECK(!mixed || !b->key->k.d_mhat, "the mixed commit launches need a stored key");
     CTX_OK(b->ctx);
     // the captured step launches the other schedule: capture again on next use
     SALSAA_HIP(hipStreamSynchronize(b->ctx->c.stream));

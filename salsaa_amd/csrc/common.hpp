@@ -459,11 +459,12 @@ int launch_commit_mfma_batch(Context *ctx, const int8_t *AB, size_t kappa, size_
                              size_t a_ktiles, const CommitJob *jobs, const uint2 *table,
                              size_t nblk, size_t kchunk, size_t max_tiles);
 int commit_batch_occupancy(size_t kappa, size_t max_tiles);
-// One launch over jobs of any kappa (each job's kappa and tm from its
-// CommitJob, job indices in the table relative to `jobs`).
+// One launch over jobs of several kappa (each job's kappa from its CommitJob,
+// job indices in the table relative to `jobs`) at the shape of tm (2 or 4)
+// key tiles per wave, which must cover every job's (job.tm <= tm).
 int launch_commit_mfma_mixed(Context *ctx, const int8_t *AB, size_t a_kt, size_t a_ktiles,
                              const CommitJob *jobs, const uint2 *table, size_t nblk,
-                             size_t kchunk);
+                             size_t kchunk, int tm);
 // the commit against the power-series CRS with NTT-form module mhat [kappa][d],
 // the key generated in the kernel (WB: the witness planes)
 int launch_commit_crs(Context *ctx, const uint64_t *mhat, size_t kappa, const int8_t *WB, size_t r,

@@ -254,26 +254,26 @@ __global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) v
                             jb.partial + ((size_t)z * 64 + s) * kappa * jb.rows, jb.rows, tab, M);
 }
 
-// The same over jobs of every key row class in one launch: the blocks that
-// share one (slot, K chunk) of the key, whatever their jobs' kappa, are
+// The same over jobs of several key row classes in one launch: the blocks
+// that share one (slot, K chunk) of the key, whatever their jobs' kappa, are
 // consecutive in the table and so run side by side on one XCD, which reads
 // that key chunk from HBM once for all of them (a launch per class reads it
-// once per class).  Every block runs the largest shape (4 key tiles per wave)
-// with its job's kappa: it loads only its job's key tiles and skips the
-// products of the others.
-template <int WN>
+// once per class).  Every block runs the shape of TM key tiles per wave (the
+// largest class's) with its job's kappa: it loads only its job's key tiles
+// and skips the products of the others.
+template <int TM, int WN>
 __global__ __launch_bounds__(256 * WN) __attribute__((amdgpu_waves_per_eu(4))) void k_commit_mfma_mixed(
     const int8_t *AB, int a_kt, int a_ktiles, const CommitJob *__restrict__ jobs,
     const uint2 *__restrict__ table, int kchunk, const double *__restrict__ tab, Mod M) {
-    __shared__ __align__(16) char lds[CommitShape<4, 4, WN>::LDS_BYTES];
+    __shared__ __align__(16) char lds[CommitShape<TM, 4, WN>::LDS_BYTES];
     const uint2 e = table[logical_block()];
     const CommitJob jb = jobs[e.x];
     const int s = (int)(e.y & 63u), z = (int)(e.y >> 6);
     const int ktb = z * kchunk;
-    commit_body<4, 4, WN>(lds, AB, (size_t)a_ktiles * a_kt * FRAG, a_kt, jb.kappa, jb.WB,
-                          (size_t)jb.ktiles * jb.rtiles * FRAG, jb.rtiles, jb.ntile, jb.rows, s,
-                          ktb, min(jb.ktiles, ktb + kchunk) - ktb,
-                          jb.partial + ((size_t)z * 64 + s) * jb.kappa * jb.rows, jb.rows, tab, M);
+    commit_body<TM, 4, WN>(lds, AB, (size_t)a_ktiles * a_kt * FRAG, a_kt, jb.kappa, jb.WB,
+                           (size_t)jb.ktiles * jb.rtiles * FRAG, jb.rtiles, jb.ntile, jb.rows, s,
+                           ktb, min(jb.ktiles, ktb + kchunk) - ktb,
+                           jb.partial + ((size_t)z * 64 + s) * jb.kappa * jb.rows, jb.rows, tab, M);
 }
 
 template <int TM, int TN, int WN, bool BATCH>
@@ -693,13 +693,20 @@ int launch_commit_mfma_batch(Context *ctx, const int8_t *AB, size_t kappa, size_
 
 int launch_commit_mfma_mixed(Context *ctx, const int8_t *AB, size_t a_kt, size_t a_ktiles,
                              const CommitJob *jobs, const uint2 *table, size_t nblk,
-                             size_t kchunk) {
+                             size_t kchunk, int tm) {
     SALSAA_CHECK(a_kt >= 1 && a_kt <= 16, "key planes hold 1..16 row tiles per K step");
     SALSAA_CHECK(kchunk >= 2 && kchunk <= 1024 && kchunk % 2 == 0, "K chunk must be even, <= 1024");
     SALSAA_CHECK(nblk >= 1 && nblk < (1u << 31), "commit grid size out of range");
+    SALSAA_CHECK(tm == 2 || tm == 4, "mixed commit shape must be 2 or 4 key tiles per wave");
     Mod M{ctx->qd, ctx->qinv};
-    hipLaunchKernelGGL((k_commit_mfma_mixed<2>), dim3((unsigned)nblk), dim3(512), 0, ctx->stream, AB,
-                       (int)a_kt, (int)a_ktiles, jobs, table, (int)kchunk, ctx->d_tables, M);
+    if (tm == 4)
+        hipLaunchKernelGGL((k_commit_mfma_mixed<4, 2>), dim3((unsigned)nblk), dim3(512), 0,
+                           ctx->stream, AB, (int)a_kt, (int)a_ktiles, jobs, table, (int)kchunk,
+                           ctx->d_tables, M);
+    else
+        hipLaunchKernelGGL((k_commit_mfma_mixed<2, 2>), dim3((unsigned)nblk), dim3(512), 0,
+                           ctx->stream, AB, (int)a_kt, (int)a_ktiles, jobs, table, (int)kchunk,
+                           ctx->d_tables, M);
     SALSAA_HIP(hipGetLastError());
     return SALSAA_OK;
 }

@@ -306,7 +306,7 @@ def test_batch_commit_schedules(sa, ctx, orc):
     W0 = [b.slot(i)["witness"].download() for i in range(n)]
     b.step_plain()
     plain = snapshot(b, True)
-    for sched in ("mixed", "per_class", "mixed"):
+    for sched in ("mixed", "per_class", "mixed_small", "mixed"):
         say("  schedule", sched)
         b.set_commit_schedule(sched)
         for graph in (False, True, True):
