@@ -419,11 +419,12 @@ int salsaa_batch_step(salsaa_batch *b, int use_graph);
 /* The same work as a native loop of per-instance calls (the plain path). */
 int salsaa_batch_step_plain(salsaa_batch *b);
 /* One phase, eagerly (for per-phase timing): 0 NTT forward, the commit's
- * digit planes, NTT inverse and the witnesses' short copy (each residue b
- * with |b| < 2^14 as two bytes, per tile of 128 ring elements a flag for
- * any longer one) in one pass, 3 commit + finalize, 7 norm checks whose
- * first pass reads the short copy where its tiles are short (valid after
- * phase 0 over the current witnesses); phases 0, 3, 7 in order equal one
+ * digit planes, NTT inverse and the witnesses' short copy (per slice of 8
+ * ring elements, each balanced residue b as one byte when all lie in
+ * [-128, 127], as two bytes when all lie in [-2^14, 2^14), else not kept)
+ * in one pass, 3 commit + finalize, 7 norm checks whose first pass reads the
+ * short copy where its slices are kept (valid after phase 0 over the
+ * current witnesses); phases 0, 3, 7 in order equal one
  * salsaa_batch_step.  4: the norm checks from the witnesses alone.  The
  * unfused alternatives of phase 0, not part of the step: 5 NTT forward,
  * 6 NTT forward with the digit planes, 1 NTT inverse, 2 digit planes from

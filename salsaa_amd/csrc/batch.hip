@@ -41,9 +41,9 @@ __global__ void k_batch_zero(uint64_t *p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
-__global__ void k_batch_ones(uint32_t *p, size_t n) {
+__global__ void k_batch_fill8(uint8_t *p, size_t n, uint8_t v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 1;
+    if (i < n) p[i] = v;
 }
 
 struct Outputs {
@@ -375,7 +375,8 @@ int run_fwd_planes(salsaa_batch *b, int roundtrip) {
 // the round trip over arena tiles [t0, t1) (the pass's tables and short copy
 // are indexed from the range's first tile)
 int run_rt_range(salsaa_batch *b, size_t t0, size_t t1) {
-    ShortCopy sc{b->sc.t + t0 * kShortTile, b->sc.flag + t0};
+    ShortCopy sc{b->sc.t + t0 * kShortTile, b->sc.t8 + t0 * kShortTile,
+                 b->sc.fmt + t0 * (kShortTile / kShortSlice)};
     return launch_ntt_fwd_planes(&b->ctx->c, b->d_wit + t0 * kShortTile,
                                  (t1 - t0) * kPlaneTile, b->d_planes, b->d_ptiles + t0, 1, &sc,
                                  b->ntt_xch, b->ntt_grid);
@@ -518,7 +519,8 @@ void destroy(salsaa_batch *b) {
     (void)hipFree(b->d_wit);
     (void)hipFree(b->d_cs);
     (void)hipFree(b->sc.t);
-    (void)hipFree(b->sc.flag);
+    (void)hipFree(b->sc.t8);
+    (void)hipFree(b->sc.fmt);
     (void)hipFree(b->d_pjobs);
     (void)hipFree(b->d_off);
     (void)hipFree(b->d_m);
@@ -588,11 +590,12 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
     SALSAA_TRY(dmalloc(&b->d_wit, b->tot_w * kD));
     SALSAA_TRY(dmalloc(&b->d_cs, b->tot_c));
     SALSAA_TRY(dmalloc(&b->sc.t, b->tot_w * kD));
-    SALSAA_TRY(dmalloc(&b->sc.flag, b->tot_w / kPlaneTile));
+    SALSAA_TRY(dmalloc(&b->sc.t8, b->tot_w * kD));
+    SALSAA_TRY(dmalloc(&b->sc.fmt, b->tot_w * kD / kShortSlice));
     hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div(b->tot_w * kD, 256)), dim3(256), 0, c->stream,
                        b->d_wit, b->tot_w * kD);
-    hipLaunchKernelGGL(k_batch_ones, dim3(ceil_div(b->tot_w / kPlaneTile, 256)), dim3(256), 0,
-                       c->stream, b->sc.flag, b->tot_w / kPlaneTile);
+    hipLaunchKernelGGL(k_batch_fill8, dim3(ceil_div(b->tot_w * kD / kShortSlice, 256)), dim3(256),
+                       0, c->stream, b->sc.fmt, b->tot_w * kD / kShortSlice, (uint8_t)2);
     hipLaunchKernelGGL(k_batch_zero, dim3(ceil_div(b->tot_c, 256)), dim3(256), 0, c->stream,
                        b->d_cs, b->tot_c);
     SALSAA_HIP(hipGetLastError());

@@ -395,16 +395,20 @@ struct PlaneTile {
     unsigned sstride = 0, kstride = 0;
 };
 // The short copy of a witness arena, for the norm check's first pass: the
-// norm check proves a witness short, and a residue x whose balanced form b has
-// |b| < 2^14 is kept as the u16 t = b + 2^14 (t[i] for arena coefficient i).
-// flag[k] is nonzero when some coefficient of tile k (kShortTile coefficients,
-// one kPlaneTile-element tile of the arena) is not short; such a tile's t
-// entries are meaningless.
+// norm check proves a witness short, and residues x whose balanced forms b
+// lie in [-2^14, 2^14) are kept in one of two widths per slice of
+// kShortSlice coefficients (one NTT-pass wave's 8 ring elements),
+// fmt[slice]: 0 when every b of the slice lies in [-128, 127], t8[i] = b + 128
+// (one byte for arena coefficient i); 1 when every b lies in [-2^14, 2^14),
+// t[i] = b + 2^14 (two bytes); 2 otherwise (the slice's entries are
+// meaningless; the format bytes start at 2).
 constexpr uint32_t kShortOff = 1u << 14;
 constexpr size_t kShortTile = (size_t)kPlaneTile * kD;
+constexpr size_t kShortSlice = 512;
 struct ShortCopy {
     uint16_t *t = nullptr;
-    uint32_t *flag = nullptr;
+    uint8_t *t8 = nullptr;
+    uint8_t *fmt = nullptr;
 };
 // t = b + 2^14 (mod 2^64) for canonical x; x is short iff short_bad(t) == 0
 __device__ __forceinline__ uint64_t short_t(uint64_t x, uint64_t half, uint64_t off_neg) {
@@ -412,6 +416,10 @@ __device__ __forceinline__ uint64_t short_t(uint64_t x, uint64_t half, uint64_t 
 }
 __device__ __forceinline__ uint32_t short_bad(uint64_t t) {
     return (uint32_t)(t >> 32) | ((uint32_t)t & ~(2 * kShortOff - 1));
+}
+// whether the t = b + 2^14 of short_t is not byte-short (b outside [-128, 127])
+__device__ __forceinline__ uint32_t byte_bad(uint64_t t) {
+    return (uint32_t)((t - (kShortOff - 128)) >> 8 != 0);
 }
 // Forward IncompleteNTT of `count` (a multiple of kPlaneTile) arena elements in
 // place, fused with the full-NTT digit planes of every tile (tiles[i] for
@@ -521,7 +529,7 @@ void nc_scratch_need(size_t rows, size_t L, size_t reps, size_t *macc_bytes, siz
 // cs_row: u64 between consecutive rows' challenge blocks (0: shared);
 // cpt_first: columns per thread of the first pass (0: automatic);
 // sc: the short copy of the arena starting at w (the first pass reads it
-// where every tile it covers is flagged short);
+// where the slices a sub-block covers are all of one width);
 // fault_claim: adds 1 to row 0's claim (norm_check_1) before the verifier
 // replay reads it, as a prover sending a wrong claim would (needs L > 1024);
 // for tests of the verifier

@@ -442,12 +442,11 @@ template <bool SHORT, bool DPP>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_ntt_rt_planes(
     uint64_t *__restrict__ data, size_t ntiles, const double *__restrict__ tab, Mod M,
     int8_t *__restrict__ planes, const PlaneTile *__restrict__ ptiles, uint16_t *__restrict__ sht,
-    uint32_t *__restrict__ shf) {
+    uint8_t *__restrict__ sht8, uint8_t *__restrict__ shfmt) {
     __shared__ uint64_t rows_all[(NT / 64) * WROWS];
     __shared__ uint64_t work[64 * EL];
     __shared__ double twl[32], twi[33], zt[32];
     __shared__ int zo[32];
-    __shared__ uint32_t wbad[NT / 64];  // per wave: a coefficient of its slice is not short
     size_t tile = blockIdx.x;
     if (tile >= ntiles) return;  // uniform in the block
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -472,9 +471,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     PlaneTile dsc = ptiles[tile];
     const uint64_t qi = (uint64_t)M.q, half = qi >> 1, off_neg = (uint64_t)kShortOff - qi;
     // the slice of tile tl into the wave's rows; with a short copy, also its
-    // u16 entries (chunk c holds coefficients 2c, 2c + 1: one u32 store) and
-    // the wave's flag, which thread 0 folds into the tile's after the next
-    // block barrier
+    // entries (chunk c holds coefficients 2c, 2c + 1): one byte each when the
+    // whole slice is byte-short (one u16 store per chunk), else two bytes each
+    // when it is short (one u32 store), and its format byte
     auto put = [&](size_t tl) {
         *reinterpret_cast<ulonglong2 *>(&rows[(lane >> 5) * SROW + (lane & 31) * 2]) = r0;
         *reinterpret_cast<ulonglong2 *>(&rows[(2 + (lane >> 5)) * SROW + (lane & 31) * 2]) = r1;
@@ -482,16 +481,32 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         *reinterpret_cast<ulonglong2 *>(&rows[(6 + (lane >> 5)) * SROW + (lane & 31) * 2]) = r3;
         if (SHORT) {
             const ulonglong2 rr[4] = {r0, r1, r2, r3};
-            uint32_t *dst = reinterpret_cast<uint32_t *>(sht + (tl * EL + WEL * w) * kD) + lane;
-            uint32_t bad = 0;
+            const size_t sl = tl * EL + WEL * w;  // the slice's first ring element
+            uint32_t ts[4];
+            uint32_t bad = 0, bad8 = 0;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const uint64_t ta = short_t(rr[i].x, half, off_neg), tb = short_t(rr[i].y, half, off_neg);
                 bad |= short_bad(ta) | short_bad(tb);
-                dst[64 * i] = ((uint32_t)ta & 0xFFFFu) | ((uint32_t)tb << 16);
+                bad8 |= byte_bad(ta) | byte_bad(tb);
+                ts[i] = ((uint32_t)ta & 0xFFFFu) | ((uint32_t)tb << 16);
             }
             const bool any = __builtin_amdgcn_ballot_w64(bad != 0) != 0;
-            if (lane == 0) wbad[w] = any;
+            const bool any8 = __builtin_amdgcn_ballot_w64(bad8 != 0) != 0;
+            if (!any8) {
+                // t - 2^14 + 128 = b + 128 in each byte
+                uint16_t *dst = reinterpret_cast<uint16_t *>(sht8 + sl * kD) + lane;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t z = ts[i] - ((kShortOff - 128u) | ((kShortOff - 128u) << 16));
+                    dst[64 * i] = (uint16_t)((z & 0xFFu) | ((z >> 8) & 0xFF00u));
+                }
+            } else if (!any) {
+                uint32_t *dst = reinterpret_cast<uint32_t *>(sht + sl * kD) + lane;
+#pragma unroll
+                for (int i = 0; i < 4; i++) dst[64 * i] = ts[i];
+            }
+            if (lane == 0) shfmt[sl * kD / kShortSlice] = any ? 2 : any8 ? 1 : 0;
         }
     };
     put(tile);
@@ -537,12 +552,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         }
         ntt32_inv_b_lazy(v, twi, t, M);  // v: this lane's forward outputs 32p + 8t + x
         __syncthreads();  // the previous tile's planes have been read out of work
-        if (SHORT && tid == 0) {
-            uint32_t f = 0;
-#pragma unroll
-            for (int k = 0; k < NT / 64; k++) f |= wbad[k];
-            shf[tile] = f;
-        }
 #pragma unroll
         for (int x = 0; x < 8; x++) work[wpos_w(8 * h + x, e)] = dbits(a[x]);
         __syncthreads();
@@ -1029,7 +1038,7 @@ int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *pl
                           unsigned grid) {
     if (count == 0) return SALSAA_OK;
     SALSAA_CHECK(count % kPlaneTile == 0, "fused planes need whole tiles");
-    SALSAA_CHECK(sc == nullptr || (roundtrip && sc->t && sc->flag),
+    SALSAA_CHECK(sc == nullptr || (roundtrip && sc->t && sc->t8 && sc->fmt),
                  "the short copy comes with the round trip");
     Mod M{ctx->qd, ctx->qinv};
     const size_t nt = count / kPlaneTile;
@@ -1052,16 +1061,16 @@ int launch_ntt_fwd_planes(Context *ctx, uint64_t *data, size_t count, int8_t *pl
     const unsigned g = (unsigned)std::min<size_t>(nt, grid ? (size_t)grid : (size_t)ctx->ncu * o);
     if (roundtrip && sc && dpp)
         hipLaunchKernelGGL((k_ntt_rt_planes<true, true>), dim3(g), dim3(NT), 0, ctx->stream, data,
-                           nt, ctx->d_tables, M, planes, tiles, sc->t, sc->flag);
+                           nt, ctx->d_tables, M, planes, tiles, sc->t, sc->t8, sc->fmt);
     else if (roundtrip && sc)
         hipLaunchKernelGGL((k_ntt_rt_planes<true, false>), dim3(g), dim3(NT), 0, ctx->stream, data,
-                           nt, ctx->d_tables, M, planes, tiles, sc->t, sc->flag);
+                           nt, ctx->d_tables, M, planes, tiles, sc->t, sc->t8, sc->fmt);
     else if (roundtrip && dpp)
         hipLaunchKernelGGL((k_ntt_rt_planes<false, true>), dim3(g), dim3(NT), 0, ctx->stream, data,
-                           nt, ctx->d_tables, M, planes, tiles, nullptr, nullptr);
+                           nt, ctx->d_tables, M, planes, tiles, nullptr, nullptr, nullptr);
     else if (roundtrip)
         hipLaunchKernelGGL((k_ntt_rt_planes<false, false>), dim3(g), dim3(NT), 0, ctx->stream,
-                           data, nt, ctx->d_tables, M, planes, tiles, nullptr, nullptr);
+                           data, nt, ctx->d_tables, M, planes, tiles, nullptr, nullptr, nullptr);
     else
         hipLaunchKernelGGL((k_ntt_fwd_planes<kPlaneTile>), dim3(g), dim3(8 * kPlaneTile), 0,
                            ctx->stream, data, nt, ctx->d_tables, M, planes, tiles);

@@ -224,8 +224,9 @@ __device__ __forceinline__ void atomic_add_res(unsigned long long *p, double v) 
 // needs only b's two balanced base-256 digits (E_0, E_1, E_2: 4 products).
 // k_sc_gram takes this path per wave and iteration after checking the
 // residues; with the round trip's short copy of a batch arena (ShortCopy),
-// k_sc_gram_short reads the copy's t directly for every block whose tiles
-// are all flagged short, and k_sc_gram skips those blocks.
+// k_sc_gram_short reads the copy (one or two bytes per entry) for every
+// sub-block whose slices all have one width, and k_sc_gram_rest takes the
+// others from the witness.
 typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int GR_X = SC_THREADS;  // columns per iteration
 constexpr int GR_LD = GR_X + 2;   // LDS row stride (u64): fragment reads hit distinct banks
@@ -281,17 +282,27 @@ __device__ __forceinline__ void gr_coefs(GrCoef &C, const uint64_t *cs, int rep0
     __syncthreads();
 }
 
-// Whether every tile of the short copy that the block's columns
-// [xb, xb + cpt GR_X) of the 16 row segments meet is flagged short (the
-// range, at most 2048 < kShortTile columns, meets at most two per segment).
-__device__ __forceinline__ bool gr_compact(const uint32_t *__restrict__ shf, size_t base, size_t S,
-                                           size_t xb, int cpt) {
-    const size_t xe = min(xb + (size_t)cpt * GR_X, S) - 1;
-    uint32_t any = 0;
-    for (int y = 0; y < 16; y++)
-        any |= shf[(base + (size_t)y * S + xb) / kShortTile] |
-               shf[(base + (size_t)y * S + xe) / kShortTile];
-    return any == 0;
+// The width of a batch's short copy over a first-pass sub-block: the block's
+// columns [xb, xe] of the NSEG row segments (S apart) meet slices whose
+// format bytes are all 0 (0: read the one-byte copy), all 1 (1: the two-byte
+// copy), or neither (2: a mix of widths, or a slice that is not short; the
+// rest kernel reads the sub-block from the witness).  Block-uniform: every
+// thread of the block calls it.
+template <int NSEG>
+__device__ __forceinline__ int sc_classify(const uint8_t *__restrict__ fmt, size_t base, size_t S,
+                                           size_t xb, size_t xe) {
+    constexpr int PARTS = SC_THREADS / NSEG;
+    const int y = threadIdx.x % NSEG, part = threadIdx.x / NSEG;
+    const size_t s0 = (base + (size_t)y * S + xb) / kShortSlice;
+    const size_t s1 = (base + (size_t)y * S + xe) / kShortSlice;
+    int n0 = 0, n1 = 0;
+    for (size_t sl = s0 + part; sl <= s1; sl += PARTS) {
+        const uint8_t f = fmt[sl];
+        n0 |= f != 0;
+        n1 |= f != 1;
+    }
+    const int any0 = __syncthreads_or(n0), any1 = __syncthreads_or(n1);
+    return !any0 ? 0 : !any1 ? 1 : 2;
 }
 
 // The short fold of column x from its 16 entries t (PAIR: two repetitions per
@@ -451,7 +462,7 @@ __device__ __forceinline__ void sc_gram_body(
     const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
     const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr, int cpt,
     double *__restrict__ out, unsigned long long *__restrict__ gacc, const double *__restrict__ tab,
-    Mod M, const uint32_t *__restrict__ shf, int sbn, int nbx) {
+    Mod M, int sbn, int nbx) {
     const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t S = L >> 4;
     const size_t base = wrow ? wrow[row] : (size_t)row * L;
@@ -477,7 +488,7 @@ __device__ __forceinline__ void sc_gram_body(
     const size_t sb0 = bxi * sbn;
     uint32_t todo = 0;
     for (int k = 0; k < sbn && (sb0 + k) * span < S; k++)
-        if (shf == nullptr || !gr_compact(shf, base, S, (sb0 + k) * span, cpt)) todo |= 1u << k;
+        todo |= 1u << k;
     if (todo == 0) continue;
     if (!ready) {
         gr_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
@@ -573,7 +584,7 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(3)))
     __shared__ __align__(16) uint64_t T[16 * GR_LD];
     __shared__ GrCoef C;
     sc_gram_body<FQ2, SNR, GRAM>(reinterpret_cast<uint32_t *>(T), C, w, wrow, L, rows, rep0, nrep,
-                                 cs0, cs_row, nrounds, nr, cpt, out, gacc, tab, M, nullptr, 1,
+                                 cs0, cs_row, nrounds, nr, cpt, out, gacc, tab, M, 1,
                                  (int)gridDim.x);
 }
 
@@ -590,7 +601,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_rest(
     const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
     int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
     double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc, Mod M,
-    const uint32_t *__restrict__ shf, int sbn, int nbx,
+    const uint8_t *__restrict__ fmt, int sbn, int nbx,
     const unsigned long long *__restrict__ need_long) {
     __shared__ GrCoef C;
     __shared__ unsigned long long Gs[16 * 16];
@@ -604,8 +615,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_rest(
     for (size_t bxi = blockIdx.x; bxi < (size_t)nbx; bxi += gridDim.x) {
         const size_t sb0 = bxi * sbn;
         uint32_t todo = 0;
-        for (int k = 0; k < sbn && (sb0 + k) * span < S; k++)
-            if (!gr_compact(shf, base, S, (sb0 + k) * span, cpt)) todo |= 1u << k;
+        for (int k = 0; k < sbn && (sb0 + k) * span < S; k++) {
+            const size_t xb = (sb0 + k) * span;
+            if (sc_classify<16>(fmt, base, S, xb, min(xb + span, S) - 1) == 2) todo |= 1u << k;
+        }
         if (todo == 0) continue;  // uniform
         if (!ready) {
             gr_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
@@ -666,22 +679,27 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_short(
     const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
     const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr, int cpt,
     double *__restrict__ out, unsigned long long *__restrict__ gacc, const double *__restrict__ tab,
-    Mod M, const uint16_t *__restrict__ sht, const uint32_t *__restrict__ shf, int sbn,
-    unsigned long long *__restrict__ need_long) {
+    Mod M, const uint16_t *__restrict__ sht, const uint8_t *__restrict__ sht8,
+    const uint8_t *__restrict__ fmt, int sbn, unsigned long long *__restrict__ need_long) {
     __shared__ __align__(16) uint32_t T32all[16 * GS_LD];  // then the Gram partials
     __shared__ GrCoef C;
     const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const size_t S = L >> 4;
     const size_t base = wrow ? wrow[row] : (size_t)row * L;
-    // the sub-blocks (as in k_sc_gram) that lie in short tiles (uniform)
+    // the sub-blocks (as in k_sc_gram) whose slices are all of one width
+    // (uniform); wide8 bit k: sub-block k reads the two-byte copy
     const size_t span = (size_t)cpt * GR_X, sb0 = (size_t)blockIdx.x * sbn;
-    uint32_t todo = 0;
-    bool left = false;  // a sub-block for k_sc_gram
+    uint32_t todo = 0, wide8 = 0;
+    bool left = false;  // a sub-block for k_sc_gram_rest
     for (int k = 0; k < sbn && (sb0 + k) * span < S; k++) {
-        if (gr_compact(shf, base, S, (sb0 + k) * span, cpt))
+        const size_t xb = (sb0 + k) * span;
+        const int mode = sc_classify<16>(fmt, base, S, xb, min(xb + span, S) - 1);
+        if (mode < 2) {
             todo |= 1u << k;
-        else
+            wide8 |= (uint32_t)mode << k;
+        } else {
             left = true;
+        }
     }
     if (left && tid == 0) *need_long = 1ULL;
     if (todo == 0) return;
@@ -689,17 +707,26 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_short(
     uint32_t *T32 = T32all + 64 * wave;
     const int fr = lane & 15, fg = lane >> 4;
     const uint16_t *tr = sht + base;
+    const uint8_t *tr8 = sht8 + base;
     v4i acc[3] = {(v4i){0, 0, 0, 0}, (v4i){0, 0, 0, 0}, (v4i){0, 0, 0, 0}};
     // row segment y at the uniform tr + y S, the lane's column as a 32-bit
-    // offset (one address register for all 16 loads)
+    // offset (one address register for all 16 loads); t = b + 2^14 from
+    // either copy
+    bool wide = false;  // the current sub-block reads the two-byte copy
     auto fetch = [&](uint32_t(&t)[16], size_t xc) {
         const uint32_t xo = (uint32_t)min(xc, S - 1);
+        if (wide) {
 #pragma unroll
-        for (int y = 0; y < 16; y++) t[y] = (tr + (size_t)y * S)[xo];
+            for (int y = 0; y < 16; y++) t[y] = (tr + (size_t)y * S)[xo];
+        } else {
+#pragma unroll
+            for (int y = 0; y < 16; y++) t[y] = (uint32_t)(tr8 + (size_t)y * S)[xo] + (kShortOff - 128u);
+        }
     };
     for (int k = 0; k < sbn; k++) {
         if (!((todo >> k) & 1)) continue;
         size_t xb = (sb0 + k) * span;
+        wide = (wide8 >> k) & 1;
         uint32_t tv[16];
         fetch(tv, xb + tid);
         for (int cc = 0;; cc++) {
@@ -781,19 +808,6 @@ __device__ __forceinline__ void g64_coefs(G64Coef &C, const uint64_t *cs, int re
     __syncthreads();
 }
 
-// Whether every short-copy tile the block's columns [xb, xb + cpt GR_X) meet
-// in the 64 segments is flagged short (each range meets at most two).
-__device__ __forceinline__ bool g64_compact(const uint32_t *__restrict__ shf, size_t base, size_t S,
-                                            size_t xb, int cpt) {
-    const size_t xe = min(xb + (size_t)cpt * GR_X, S) - 1;
-    uint32_t any = 0;
-    for (int y = 0; y < 64; y++) {
-        const size_t s0 = base + (size_t)y * S;
-        any |= shf[(s0 + xb) / kShortTile] | shf[(s0 + xe) / kShortTile];
-    }
-    return any == 0;
-}
-
 // Digits 0 and 1 of 16 consecutive staged entries z = b + 128 (int16, b the
 // balanced residue) as matrix-core fragments: the low byte of z is d0 + 128,
 // the high byte d1.
@@ -859,8 +873,8 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
     const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr, int cpt,
     double *__restrict__ out, unsigned long long *__restrict__ gacc, const double *__restrict__ tab,
-    Mod M, const uint16_t *__restrict__ sht, const uint32_t *__restrict__ shf, int sbn,
-    unsigned long long *__restrict__ need_long) {
+    Mod M, const uint16_t *__restrict__ sht, const uint8_t *__restrict__ sht8,
+    const uint8_t *__restrict__ fmt, int sbn, unsigned long long *__restrict__ need_long) {
     __shared__ __align__(16) uint16_t T16[64 * G64_LD];   // T16[y][c]: segment y, block column c
     __shared__ __align__(16) uint16_t TC[GR_X * G64_CLD];  // TC[c][y]
     __shared__ G64Coef C;
@@ -869,13 +883,17 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     const size_t S = L >> 6;
     const size_t base = wrow ? wrow[row] : (size_t)row * L;
     const size_t span = (size_t)cpt * GR_X, sb0 = (size_t)blockIdx.x * sbn;
-    uint32_t todo = 0;
+    uint32_t todo = 0, wide8 = 0;  // wide8 bit k: sub-block k reads the two-byte copy
     bool left = false;  // a sub-block for k_sc_gram64_rest
     for (int k = 0; k < sbn && (sb0 + k) * span < S; k++) {
-        if (g64_compact(shf, base, S, (sb0 + k) * span, cpt))
+        const size_t xb = (sb0 + k) * span;
+        const int mode = sc_classify<64>(fmt, base, S, xb, min(xb + span, S) - 1);
+        if (mode < 2) {
             todo |= 1u << k;
-        else
+            wide8 |= (uint32_t)mode << k;
+        } else {
             left = true;
+        }
     }
     if (left && tid == 0) *need_long = 1ULL;
     if (todo == 0) return;
@@ -890,6 +908,7 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     const bool fo_live = fo_r < nrep && (FQ2 || fo_c == 0);
     double *fo_base = out + ((size_t)(rep0 + fo_r) * rows + row) * S * 2 + fo_c;
     const uint16_t *tr = sht + base;
+    const uint8_t *tr8 = sht8 + base;
     v4i acc[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -898,16 +917,23 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     // entry (y, x) at the uniform tr plus the 32-bit offset y S + x (S <= 2^26:
     // one base address, not 64 of them in scalar registers)
     const uint32_t S32 = (uint32_t)S;
+    bool wide = false;  // the current sub-block reads the two-byte copy
     auto fetch = [&](uint32_t(&t)[64], size_t xc) {
         const uint32_t xo = (uint32_t)min(xc, S - 1);
+        if (wide) {
 #pragma unroll
-        for (int y = 0; y < 64; y++) t[y] = tr[(uint32_t)y * S32 + xo];
+            for (int y = 0; y < 64; y++) t[y] = tr[(uint32_t)y * S32 + xo];
+        } else {
+#pragma unroll
+            for (int y = 0; y < 64; y++) t[y] = (uint32_t)tr8[(uint32_t)y * S32 + xo] + (kShortOff - 128u);
+        }
     };
     uint16_t *mycol = T16 + 64 * wave + lane;
     uint16_t *myTC = TC + (64 * wave + lane) * G64_CLD;
     for (int k = 0; k < sbn; k++) {
         if (!((todo >> k) & 1)) continue;
         size_t xb = (sb0 + k) * span;  // the block's columns (uniform)
+        wide = (wide8 >> k) & 1;
         uint32_t tv[64];
         fetch(tv, xb + 64 * wave + lane);
         for (int cc = 0;; cc++) {
@@ -1019,7 +1045,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram64_rest(
     const uint64_t *__restrict__ w, const unsigned long long *__restrict__ wrow, size_t L,
     int rows, int rep0, int nrep, const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds,
     double nr, int cpt, double *__restrict__ out, unsigned long long *__restrict__ gacc, Mod M,
-    const uint32_t *__restrict__ shf, int sbn, int nbx,
+    const uint8_t *__restrict__ fmt, int sbn, int nbx,
     const unsigned long long *__restrict__ need_long) {
     __shared__ G64Coef C;
     __shared__ uint64_t V[64 * G64R_LD];  // V[y][c]: segment y, column xb + c
@@ -1036,8 +1062,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram64_rest(
     for (size_t bxi = blockIdx.x; bxi < (size_t)nbx; bxi += gridDim.x) {
         const size_t sb0 = bxi * sbn;
         uint32_t todo = 0;
-        for (int k = 0; k < sbn && (sb0 + k) * span < S; k++)
-            if (!g64_compact(shf, base, S, (sb0 + k) * span, cpt)) todo |= 1u << k;
+        for (int k = 0; k < sbn && (sb0 + k) * span < S; k++) {
+            const size_t xb = (sb0 + k) * span;
+            if (sc_classify<64>(fmt, base, S, xb, min(xb + span, S) - 1) == 2) todo |= 1u << k;
+        }
         if (todo == 0) continue;  // uniform
         if (!ready) {
             g64_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
@@ -1641,23 +1669,22 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         ggrid.x = (unsigned)ceil_div((size_t)bx, walk);
         SALSAA_CHECK((size_t)bx + (size_t)ggrid.x <= kGramBlocksMax,
                      "norm check row too long (at most 2^32 coefficients)");
-        const uint32_t *shf = sc->flag;
         for (size_t r0 = 0; r0 < reps; r0 += G64_MAXREP) {
             const int nrep = (int)std::min<size_t>(G64_MAXREP, reps - r0);
             if (r0 == 0) {
                 hipLaunchKernelGGL((k_sc_gram64_short<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn, need_long);
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, sc->t8, sc->fmt, sbn, need_long);
                 hipLaunchKernelGGL((k_sc_gram64_rest<FQ2, SNR, true>), ggrid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
-                                   nr, cpt, bufA, gacc, M, shf, sbn, (int)bx, need_long);
+                                   nr, cpt, bufA, gacc, M, sc->fmt, sbn, (int)bx, need_long);
             } else {
                 hipLaunchKernelGGL((k_sc_gram64_short<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn, need_long);
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, sc->t8, sc->fmt, sbn, need_long);
                 hipLaunchKernelGGL((k_sc_gram64_rest<FQ2, SNR, false>), ggrid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
-                                   nr, cpt, bufA, gacc, M, shf, sbn, (int)bx, need_long);
+                                   nr, cpt, bufA, gacc, M, sc->fmt, sbn, (int)bx, need_long);
             }
             SALSAA_HIP(hipGetLastError());
         }
@@ -1704,23 +1731,22 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         for (size_t r0 = 0; r0 < reps; r0 += GR_MAXREP) {
             const int nrep = (int)std::min<size_t>(GR_MAXREP, reps - r0);
             // with a short copy, its short blocks first, then the others
-            const uint32_t *shf = sc ? sc->flag : nullptr;
             if (sc && r0 == 0)
                 hipLaunchKernelGGL((k_sc_gram_short<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn, need_long);
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, sc->t8, sc->fmt, sbn, need_long);
             else if (sc)
                 hipLaunchKernelGGL((k_sc_gram_short<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, shf, sbn, need_long);
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, sc->t8, sc->fmt, sbn, need_long);
             if (sc && r0 == 0)
                 hipLaunchKernelGGL((k_sc_gram_rest<FQ2, SNR, true>), ggrid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
-                                   nr, cpt, bufA, gacc, M, shf, sbn, (int)bx, need_long);
+                                   nr, cpt, bufA, gacc, M, sc->fmt, sbn, (int)bx, need_long);
             else if (sc)
                 hipLaunchKernelGGL((k_sc_gram_rest<FQ2, SNR, false>), ggrid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
-                                   nr, cpt, bufA, gacc, M, shf, sbn, (int)bx, need_long);
+                                   nr, cpt, bufA, gacc, M, sc->fmt, sbn, (int)bx, need_long);
             else if (r0 == 0)
                 hipLaunchKernelGGL((k_sc_gram<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
@@ -1805,7 +1831,7 @@ int launch_norm_check_ex(Context *ctx, const uint64_t *w, const unsigned long lo
                          int fq2, uint64_t nr, const NcOut &o, int cpt_first,
                          const NcScratch *ext, const ShortCopy *sc, int fault_claim) {
     SALSAA_CHECK(rows > 0 && reps > 0, "norm check needs rows > 0 and reps > 0");
-    SALSAA_CHECK(sc == nullptr || (sc->t && sc->flag), "incomplete short copy");
+    SALSAA_CHECK(sc == nullptr || (sc->t && sc->t8 && sc->fmt), "incomplete short copy");
     SALSAA_CHECK(L >= 2 && (L & (L - 1)) == 0, "norm check length must be a power of two >= 2");
     SALSAA_CHECK(rows < 65536 && reps < 65536, "too many rows or repetitions");
     SALSAA_CHECK(__builtin_ctzll(L) <= 48, "norm check length too large");

@@ -229,6 +229,74 @@ def test_batch_short_copy_mixed(sa, ctx, orc):
     key.close()
 
 
+def test_batch_short_copy_widths(sa, ctx, orc):
+    # The round trip keeps each slice of 512 coefficients (8 ring elements)
+    # one byte wide when its balanced residues lie in [-128, 127], two bytes
+    # wide when they lie in [-2^14, 2^14), and the norm check's first pass
+    # reads a sub-block from the copy only when its slices share one width.
+    # Instances: the byte edges -128 and 127 (64-segment pass); byte slices
+    # with every other slice widened by one 128 or -129; a byte half and a
+    # two-byte half; random slices of either width (16-segment pass); and
+    # only the two-byte edges -129 and 128.  Eager and graph steps must equal
+    # the plain per-call path, and the plain path the oracle.
+    say("case: batch short copy, byte and two-byte slices")
+    m = [4096, 2048, 1024, 512, 256]
+    kappa = [8, 16, 8, 32, 8]
+    n = len(m)
+    A_h = orc.sample_uniform(16, sa.STREAM_A, (32, 4096, 64), Q)
+    key = ctx.key(ctx.to_device(A_h))
+    b = sa.Batch(ctx, key, m, kappa, reps=2, fq2=1, tune=False)
+    fill(sa, ctx, b, 1600)
+    rng = np.random.default_rng(16)
+
+    def res(v):
+        return np.where(v < 0, v + Q, v).astype(np.uint64)
+
+    def byte_vals(k):
+        return rng.integers(-128, 128, k)
+
+    W0 = []
+    v = rng.choice(np.array([-128, 127, 0]), (m[0], 64))
+    W0.append(res(v))
+    v = byte_vals(m[1] * 64)
+    for sl in range(0, m[1] * 64 // 512, 2):
+        v[sl * 512 + rng.integers(0, 512)] = 128 if sl % 4 == 0 else -129
+    W0.append(res(v).reshape(m[1], 64))
+    v = byte_vals(m[2] * 64)
+    h = m[2] * 32
+    v[h:] = rng.integers(-(1 << 14), 1 << 14, h)
+    W0.append(res(v).reshape(m[2], 64))
+    v = byte_vals(m[3] * 64).reshape(-1, 512)
+    wide = rng.random(v.shape[0]) < 0.5
+    v[wide] = rng.integers(-(1 << 14), 1 << 14, (int(wide.sum()), 512))
+    W0.append(res(v).reshape(m[3], 64))
+    v = rng.choice(np.array([-129, 128, -(1 << 14), (1 << 14) - 1]), (m[4], 64))
+    W0.append(res(v))
+    for i in range(n):
+        b.slot(i)["witness"].upload(W0[i])
+    b.step_plain()
+    plain = snapshot(b, True)
+    for i in range(n):
+        s = b.slot(i)
+        want = orc.sumcheck(W0[i].reshape(1, m[i] * 64), s["challenges"].download(), Q,
+                            fq2=True, nr=NR, threads=8)
+        for k in ("claims", "msgs", "finals", "evals"):
+            assert np.array_equal(plain[i][k], want[k]), (i, k)
+        assert plain[i]["ok"].all()
+    b.step(graph=False)
+    same(snapshot(b, False), plain)
+    for i in range(n):
+        assert np.array_equal(b.slot(i)["witness"].download(), W0[i]), i
+    for rep in range(2):
+        b.step(graph=True)
+        ctx.sync()
+        same(snapshot(b, False), plain)
+        b.step_plain()
+        ctx.sync()
+    b.close()
+    key.close()
+
+
 def test_batch_wide_rows_long_walk(sa, ctx, orc):
     # A class of many long rows whose residues are all wide (no tile short):
     # the first pass's blocks that take the rows from the witnesses walk many
