@@ -748,9 +748,11 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram_short(
                 gr_short_frags(T32, GS_LD, fr, fg, f0, f1);
                 wave_sync();  // T32 is rewritten by the next iteration
                 acc[0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f0, f0, acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f0, f1, acc[1], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f1, f0, acc[1], 0, 0, 0);
-                acc[2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f1, f1, acc[2], 0, 0, 0);
+                if (wide) {  // byte slices: b's high digit is 0, E_1 = E_2 = 0
+                    acc[1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f0, f1, acc[1], 0, 0, 0);
+                    acc[1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f1, f0, acc[1], 0, 0, 0);
+                    acc[2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(f1, f1, acc[2], 0, 0, 0);
+                }
             } else {
                 wave_sync();
             }
@@ -828,6 +830,9 @@ __device__ __forceinline__ void g64_frags(const uint16_t *src, v4i &f0, v4i &f1)
 // (row, kg) holds segments 16 kg .. 16 kg + 15.  A balanced weight w
 // (|w| < 2^49) has w + 0x0080808080808080 in [0, 2^56) as an integer, whose
 // byte u is a_u + 128 for the signed digits w = sum_{u<7} a_u 256^u.
+// PERM: K position p = 16 kg + 4 k4 + bq holds segment bq + 4 kg + 16 k4
+// (the byte kernel's column order) instead of segment p.
+template <bool PERM = false>
 __device__ __forceinline__ void g64_wfrags(const G64Coef &C, int lane, v4i (&A)[2]) {
     const int row = lane & 15, kg = lane >> 4, g = row >> 2, e = row & 3;
     const int r = g >> 1, comp = g & 1;
@@ -839,7 +844,7 @@ __device__ __forceinline__ void g64_wfrags(const G64Coef &C, int lane, v4i (&A)[
             uint32_t word = 0;
 #pragma unroll
             for (int bq = 0; bq < 4; bq++) {
-                const int y = 16 * kg + 4 * k4 + bq;
+                const int y = PERM ? bq + 4 * kg + 16 * k4 : 16 * kg + 4 * k4 + bq;
                 const double w = comp ? C.coef[r][y].b : C.coef[r][y].a;
                 const uint64_t t = (uint64_t)(long long)w + GR_BIAS;
                 const uint32_t d = u < 7 ? (((uint32_t)(t >> (8 * u)) & 0xFFu) ^ 0x80u) : 0u;
@@ -851,7 +856,8 @@ __device__ __forceinline__ void g64_wfrags(const G64Coef &C, int lane, v4i (&A)[
 }
 
 // The first pass over the short copy with 64 segments: the sub-blocks of cpt
-// GR_X columns x < S whose tiles in all 64 segments are short.  Per
+// GR_X columns x < S whose slices in all 64 segments are kept two bytes wide
+// (k_sc_gram64_byte takes the byte-wide ones, k_sc_gram64_rest the rest).  Per
 // iteration, wave w stages the 64 segments of columns 64 w .. 64 w + 63 (one
 // per lane, prefetched a whole iteration ahead) twice: into the block's rows
 // (segment-major, for the Gram matrix) and into its own columns
@@ -873,8 +879,8 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
     const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr, int cpt,
     double *__restrict__ out, unsigned long long *__restrict__ gacc, const double *__restrict__ tab,
-    Mod M, const uint16_t *__restrict__ sht, const uint8_t *__restrict__ sht8,
-    const uint8_t *__restrict__ fmt, int sbn, unsigned long long *__restrict__ need_long) {
+    Mod M, const uint16_t *__restrict__ sht, const uint8_t *__restrict__ fmt, int sbn,
+    unsigned long long *__restrict__ need_long) {
     __shared__ __align__(16) uint16_t T16[64 * G64_LD];   // T16[y][c]: segment y, block column c
     __shared__ __align__(16) uint16_t TC[GR_X * G64_CLD];  // TC[c][y]
     __shared__ G64Coef C;
@@ -883,17 +889,15 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     const size_t S = L >> 6;
     const size_t base = wrow ? wrow[row] : (size_t)row * L;
     const size_t span = (size_t)cpt * GR_X, sb0 = (size_t)blockIdx.x * sbn;
-    uint32_t todo = 0, wide8 = 0;  // wide8 bit k: sub-block k reads the two-byte copy
+    uint32_t todo = 0;
     bool left = false;  // a sub-block for k_sc_gram64_rest
     for (int k = 0; k < sbn && (sb0 + k) * span < S; k++) {
         const size_t xb = (sb0 + k) * span;
         const int mode = sc_classify<64>(fmt, base, S, xb, min(xb + span, S) - 1);
-        if (mode < 2) {
+        if (mode == 1)
             todo |= 1u << k;
-            wide8 |= (uint32_t)mode << k;
-        } else {
-            left = true;
-        }
+        else if (mode == 2)
+            left = true;  // (mode 0: k_sc_gram64_byte's)
     }
     if (left && tid == 0) *need_long = 1ULL;
     if (todo == 0) return;
@@ -908,7 +912,6 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     const bool fo_live = fo_r < nrep && (FQ2 || fo_c == 0);
     double *fo_base = out + ((size_t)(rep0 + fo_r) * rows + row) * S * 2 + fo_c;
     const uint16_t *tr = sht + base;
-    const uint8_t *tr8 = sht8 + base;
     v4i acc[3][3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
@@ -917,23 +920,16 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
     // entry (y, x) at the uniform tr plus the 32-bit offset y S + x (S <= 2^26:
     // one base address, not 64 of them in scalar registers)
     const uint32_t S32 = (uint32_t)S;
-    bool wide = false;  // the current sub-block reads the two-byte copy
     auto fetch = [&](uint32_t(&t)[64], size_t xc) {
         const uint32_t xo = (uint32_t)min(xc, S - 1);
-        if (wide) {
 #pragma unroll
-            for (int y = 0; y < 64; y++) t[y] = tr[(uint32_t)y * S32 + xo];
-        } else {
-#pragma unroll
-            for (int y = 0; y < 64; y++) t[y] = (uint32_t)tr8[(uint32_t)y * S32 + xo] + (kShortOff - 128u);
-        }
+        for (int y = 0; y < 64; y++) t[y] = tr[(uint32_t)y * S32 + xo];
     };
     uint16_t *mycol = T16 + 64 * wave + lane;
     uint16_t *myTC = TC + (64 * wave + lane) * G64_CLD;
     for (int k = 0; k < sbn; k++) {
         if (!((todo >> k) & 1)) continue;
         size_t xb = (sb0 + k) * span;  // the block's columns (uniform)
-        wide = (wide8 >> k) & 1;
         uint32_t tv[64];
         fetch(tv, xb + 64 * wave + lane);
         for (int cc = 0;; cc++) {
@@ -1028,6 +1024,158 @@ __global__ __launch_bounds__(SC_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
                 atomicAdd(&gacc[((size_t)row * G64_PAIRS + q) * 256 + (4 * fg + e) * 16 + fr],
                           (unsigned long long)d2u(reduce(xg, M)));
             }
+        }
+    }
+}
+
+// The same first pass over the sub-blocks whose slices are all one byte wide
+// (b in [-128, 127]: one signed digit, so the Gram matrix is E_0 alone and
+// the fold is the weights' 7 digits times b).  S = L / 64 is a power of two
+// of at least 1024, so every iteration's 256 columns lie in the row.
+// Lane (sg, cg) = (lane & 3, lane >> 2) of wave w loads columns
+// 64 w + 4 cg .. + 3 (one u32) of the 16 segments y_i = (i & 3) + 4 sg +
+// 16 (i >> 2), from one uniform base per i plus a 32-bit lane offset, and
+// stages them as signed bytes twice: into the rows T8[y][c] (one u32 store
+// per segment) and, after a 4 x 4 byte transpose, into the columns
+// TC8[c][16 sg + i] (one 16-byte store per column).  Column position
+// p = 16 sg + i holds segment y_i, so the fold's weights take K in that order
+// (g64_wfrags<true>); both are matrix-core fragments as they lie in LDS.
+// The fold's digit sums recombine exactly in int64 (|sum_y w(y) b(y)| <
+// 64 2^49 2^7 = 2^62), and the Gram tile sums stay below 2^31 (at most 2^14
+// columns per block).  Row strides are chosen so that the stores and the
+// fragment reads of 16 lanes meet distinct LDS banks.
+constexpr int GB_LD = GR_X + 16;  // byte stride of the segment rows (68 words)
+constexpr int GB_CLD = 64 + 16;   // byte stride of the columns (20 words)
+template <bool FQ2, bool SNR, bool GRAM>
+__global__ __launch_bounds__(SC_THREADS) void k_sc_gram64_byte(
+    const unsigned long long *__restrict__ wrow, size_t L, int rows, int rep0, int nrep,
+    const uint64_t *__restrict__ cs0, size_t cs_row, int nrounds, double nr, int cpt,
+    double *__restrict__ out, unsigned long long *__restrict__ gacc, const double *__restrict__ tab,
+    Mod M, const uint8_t *__restrict__ sht8, const uint8_t *__restrict__ fmt, int sbn) {
+    __shared__ __align__(16) uint8_t T8[64 * GB_LD];
+    __shared__ __align__(16) uint8_t TC8[GR_X * GB_CLD];
+    __shared__ G64Coef C;
+    const int row = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const size_t S = L >> 6;
+    const size_t base = wrow ? wrow[row] : (size_t)row * L;
+    const size_t span = (size_t)cpt * GR_X, sb0 = (size_t)blockIdx.x * sbn;
+    uint32_t todo = 0;
+    for (int k = 0; k < sbn && (sb0 + k) * span < S; k++) {
+        const size_t xb = (sb0 + k) * span;
+        if (sc_classify<64>(fmt, base, S, xb, min(xb + span, S) - 1) == 0) todo |= 1u << k;
+    }
+    if (todo == 0) return;
+    g64_coefs<FQ2, SNR>(C, cs0 + row * cs_row, rep0, nrep, nrounds, nr, M, tid);
+    const int fr = lane & 15, fg = lane >> 4;
+    const int sg = lane & 3, cg = lane >> 2;
+    v4i Aw[2];
+    g64_wfrags<true>(C, lane, Aw);
+    const double w32 = tab[TAB_W256 + 4];  // 2^32 mod q
+    const int fo_r = fg >> 1, fo_c = fg & 1;
+    const bool fo_live = fo_r < nrep && (FQ2 || fo_c == 0);
+    double *fo_base = out + ((size_t)(rep0 + fo_r) * rows + row) * S * 2 + fo_c;
+    const uint8_t *tr8 = sht8 + base;
+    const uint32_t S32 = (uint32_t)S;
+    const uint32_t wcol = 64u * wave + 4u * cg;  // the lane's first column in the block
+    v4i acc[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) acc[i] = (v4i){0, 0, 0, 0};
+    // the 16 segments' u32 at block column xb
+    auto fetch = [&](uint32_t(&r)[16], uint32_t xb) {
+        const uint32_t off = 4u * sg * S32 + xb + wcol;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint8_t *sb = tr8 + (size_t)((i & 3) + 16 * (i >> 2)) * S;
+            r[i] = *reinterpret_cast<const uint32_t *>(sb + off);
+        }
+    };
+    uint8_t *t8w = T8 + GB_LD * 4 * sg + wcol;
+    uint8_t *tcw = TC8 + wcol * GB_CLD + 16 * sg;
+    for (int k = 0; k < sbn; k++) {
+        if (!((todo >> k) & 1)) continue;
+        uint32_t xb = (uint32_t)((sb0 + k) * span);  // the block's columns (uniform)
+        uint32_t rv[16];
+        fetch(rv, xb);
+        for (int cc = 0;; cc++) {
+            const bool more = cc + 1 < cpt && xb + GR_X < S32;  // uniform in the block
+            // t8 = b + 128 per byte -> b as a signed byte
+            uint32_t d[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) d[i] = rv[i] ^ 0x80808080u;
+            fetch(rv, more ? xb + GR_X : xb);
+            if (GRAM)
+#pragma unroll
+                for (int i = 0; i < 16; i++)
+                    *reinterpret_cast<uint32_t *>(t8w + GB_LD * ((i & 3) + 16 * (i >> 2))) = d[i];
+            // columns: o[c][q] = bytes c of d[4 q .. 4 q + 3]
+            uint32_t o[4][4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t l01 = __builtin_amdgcn_perm(d[4 * q + 1], d[4 * q], 0x05010400u);
+                const uint32_t h01 = __builtin_amdgcn_perm(d[4 * q + 1], d[4 * q], 0x07030602u);
+                const uint32_t l23 = __builtin_amdgcn_perm(d[4 * q + 3], d[4 * q + 2], 0x05010400u);
+                const uint32_t h23 = __builtin_amdgcn_perm(d[4 * q + 3], d[4 * q + 2], 0x07030602u);
+                o[0][q] = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
+                o[1][q] = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
+                o[2][q] = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
+                o[3][q] = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                *reinterpret_cast<uint4 *>(tcw + c * GB_CLD) = make_uint4(o[c][0], o[c][1], o[c][2], o[c][3]);
+            wave_sync();  // the wave's columns are in TC8
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const v4i b = *reinterpret_cast<const v4i *>(TC8 + (64 * wave + 16 * j + fr) * GB_CLD + 16 * fg);
+                const v4i z4 = (v4i){0, 0, 0, 0};
+                const v4i s0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aw[0], b, z4, 0, 0, 0);
+                const v4i s1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aw[1], b, z4, 0, 0, 0);
+                // sum_k E_k 2^(8k), exact in wrapping int64 (the sum fits)
+                const unsigned long long tot =
+                    (unsigned long long)(long long)s0[0] + ((unsigned long long)(long long)s0[1] << 8) +
+                    ((unsigned long long)(long long)s0[2] << 16) +
+                    ((unsigned long long)(long long)s0[3] << 24) +
+                    ((unsigned long long)(long long)s1[0] << 32) +
+                    ((unsigned long long)(long long)s1[1] << 40) +
+                    ((unsigned long long)(long long)s1[2] << 48);
+                const long long ts = (long long)tot;
+                const double v = mul_lazy((double)(ts >> 32), w32, M) + (double)(uint32_t)tot;
+                const size_t xo = (size_t)xb + 64 * wave + 16 * j + fr;
+                if (fo_live) fo_base[xo * 2] = reduce_lazy(v, M);
+            }
+            if (GRAM) {
+                __syncthreads();  // every wave's rows are staged
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const int q = wave + 4 * i;  // wave-uniform
+                    if (q >= G64_PAIRS) break;
+                    int a, bq;
+                    g64_pair(q, a, bq);
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; k4++) {
+                        const v4i da = *reinterpret_cast<const v4i *>(T8 + GB_LD * (16 * a + fr) + 64 * k4 + 16 * fg);
+                        const v4i db = *reinterpret_cast<const v4i *>(T8 + GB_LD * (16 * bq + fr) + 64 * k4 + 16 * fg);
+                        acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8(da, db, acc[i], 0, 0, 0);
+                    }
+                }
+                __syncthreads();  // the rows are rewritten by the next iteration
+            } else {
+                wave_sync();  // the wave's columns are rewritten by the next iteration
+            }
+            if (!more) break;
+            xb += GR_X;
+        }
+    }
+    if (GRAM) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int q = wave + 4 * i;
+            if (q >= G64_PAIRS) break;
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                atomicAdd(&gacc[((size_t)row * G64_PAIRS + q) * 256 + (4 * fg + e) * 16 + fr],
+                          (unsigned long long)d2u(reduce((double)acc[i][e], M)));
         }
     }
 }
@@ -1669,19 +1817,31 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         ggrid.x = (unsigned)ceil_div((size_t)bx, walk);
         SALSAA_CHECK((size_t)bx + (size_t)ggrid.x <= kGramBlocksMax,
                      "norm check row too long (at most 2^32 coefficients)");
+        // the byte-wide sub-blocks on k_sc_gram64_byte, the two-byte ones on
+        // k_sc_gram64_short, the others on k_sc_gram64_rest (L is a power of
+        // two, so S is a multiple of the byte kernel's 256 columns)
+        SALSAA_CHECK(S % GR_X == 0, "64-segment pass needs whole 256-column iterations");
         for (size_t r0 = 0; r0 < reps; r0 += G64_MAXREP) {
             const int nrep = (int)std::min<size_t>(G64_MAXREP, reps - r0);
+            if (r0 == 0)
+                hipLaunchKernelGGL((k_sc_gram64_byte<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
+                                   ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t8, sc->fmt, sbn);
+            else
+                hipLaunchKernelGGL((k_sc_gram64_byte<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0,
+                                   ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t8, sc->fmt, sbn);
             if (r0 == 0) {
                 hipLaunchKernelGGL((k_sc_gram64_short<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, sc->t8, sc->fmt, sbn, need_long);
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, sc->fmt, sbn, need_long);
                 hipLaunchKernelGGL((k_sc_gram64_rest<FQ2, SNR, true>), ggrid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
                                    nr, cpt, bufA, gacc, M, sc->fmt, sbn, (int)bx, need_long);
             } else {
                 hipLaunchKernelGGL((k_sc_gram64_short<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, sc->t8, sc->fmt, sbn, need_long);
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t, sc->fmt, sbn, need_long);
                 hipLaunchKernelGGL((k_sc_gram64_rest<FQ2, SNR, false>), ggrid, dim3(SC_THREADS), 0,
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
                                    nr, cpt, bufA, gacc, M, sc->fmt, sbn, (int)bx, need_long);
