@@ -1823,14 +1823,6 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
         SALSAA_CHECK(S % GR_X == 0, "64-segment pass needs whole 256-column iterations");
         for (size_t r0 = 0; r0 < reps; r0 += G64_MAXREP) {
             const int nrep = (int)std::min<size_t>(G64_MAXREP, reps - r0);
-            if (r0 == 0)
-                hipLaunchKernelGGL((k_sc_gram64_byte<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
-                                   ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t8, sc->fmt, sbn);
-            else
-                hipLaunchKernelGGL((k_sc_gram64_byte<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0,
-                                   ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
-                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t8, sc->fmt, sbn);
             if (r0 == 0) {
                 hipLaunchKernelGGL((k_sc_gram64_short<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
                                    ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
@@ -1846,6 +1838,17 @@ int norm_check_impl(Context *ctx, const uint64_t *w, const unsigned long long *w
                                    ctx->stream, w, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n,
                                    nr, cpt, bufA, gacc, M, sc->fmt, sbn, (int)bx, need_long);
             }
+            // the byte kernel last: the two kernels above usually find
+            // nothing to do, and their blocks (large in LDS) get CUs most
+            // easily before the byte kernel's fill them
+            if (r0 == 0)
+                hipLaunchKernelGGL((k_sc_gram64_byte<FQ2, SNR, true>), grid, dim3(SC_THREADS), 0,
+                                   ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t8, sc->fmt, sbn);
+            else
+                hipLaunchKernelGGL((k_sc_gram64_byte<FQ2, SNR, false>), grid, dim3(SC_THREADS), 0,
+                                   ctx->stream, wrow, L, (int)rows, (int)r0, nrep, cs, cs_row, n, nr,
+                                   cpt, bufA, gacc, ctx->d_tables, M, sc->t8, sc->fmt, sbn);
             SALSAA_HIP(hipGetLastError());
         }
     } else {
