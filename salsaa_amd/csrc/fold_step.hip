@@ -33,6 +33,15 @@ namespace {
 constexpr int FNT = 1024;    // threads per block
 constexpr int FROW = 72;     // LDS row stride (u64), skewed layout
 constexpr int FCOLS = 16;    // columns per block
+
+// The column block of a (m / FCOLS) x n grid, remapped so that the four
+// blocks whose 16-column plane stores share each 64-byte line run on one XCD
+// (blocks are dealt to the 8 XCDs in turn): their partial lines then merge in
+// that XCD's L2 before they reach HBM.
+__device__ __forceinline__ size_t fold_col_block() {
+    const unsigned gx = gridDim.x, bx = blockIdx.x;
+    return gx % 8 == 0 ? (size_t)(bx % 8) * (gx / 8) + bx / 8 : bx;
+}
 constexpr uint64_t DIGIT_BIAS8 = 0x0080808080808080ULL;
 
 __device__ __forceinline__ int sk(int k) { return k + (k >> 3); }
@@ -304,7 +313,7 @@ __global__ __launch_bounds__(FNT) void k_decomp_fold(
     __shared__ double cl[8 * 8 * kD];  // this limb's challenges C[jo][rho * parts + tl]
     const int inst = blockIdx.y;
     if (fold_mfma_ok(bad, d_radix, inst)) return;  // uniform
-    const size_t i0 = (size_t)blockIdx.x * FCOLS;
+    const size_t i0 = fold_col_block() * FCOLS;
     const int tid = threadIdx.x;
     // NTT role: element e, half p, quarter t (positions 8k + 2t + p)
     const int e = tid >> 3, h = tid & 7, p = h >> 2, t = h & 3;
@@ -401,7 +410,7 @@ __global__ __launch_bounds__(FNT) void k_fold_coeff_out(
     __shared__ uint64_t lds[128 * FROW];
     const int inst = blockIdx.y;
     if (!fold_mfma_ok(bad, d_radix, inst)) return;  // uniform
-    const size_t i0 = (size_t)blockIdx.x * FCOLS;
+    const size_t i0 = fold_col_block() * FCOLS;
     const int tid = threadIdx.x, nout = r_out * FCOLS;
     const int e = tid >> 3, h = tid & 7, p = h >> 2, t = h & 3;
     for (int c = tid; c < nout * 32; c += FNT) {
