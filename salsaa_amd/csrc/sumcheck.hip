@@ -1092,18 +1092,30 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram64_byte(
     };
     uint8_t *t8w = T8 + GB_LD * 4 * sg + wcol;
     uint8_t *tcw = TC8 + wcol * GB_CLD + 16 * sg;
-    for (int k = 0; k < sbn; k++) {
-        if (!((todo >> k) & 1)) continue;
-        uint32_t xb = (uint32_t)((sb0 + k) * span);  // the block's columns (uniform)
-        uint32_t rv[16];
-        fetch(rv, xb);
-        for (int cc = 0;; cc++) {
-            const bool more = cc + 1 < cpt && xb + GR_X < S32;  // uniform in the block
+    // the iterations of every sub-block to do, in one loop, so that each
+    // iteration's loads (the next sub-block's first ones included) are in
+    // flight while the one before is staged and multiplied
+    int k = __builtin_ctz(todo), cc = 0;
+    uint32_t xb = (uint32_t)((sb0 + k) * span);  // the block's columns (uniform)
+    uint32_t rv[16];
+    fetch(rv, xb);
+    {
+        for (;;) {
+            // the next iteration (uniform in the block)
+            int nk = k, ncc = cc + 1;
+            uint32_t nxb = xb + GR_X;
+            if (ncc >= cpt || nxb >= S32) {
+                nk = k + 1;
+                while (nk < sbn && !((todo >> nk) & 1)) nk++;
+                ncc = 0;
+                nxb = nk < sbn ? (uint32_t)((sb0 + nk) * span) : xb;
+            }
+            const bool more = nk < sbn;
             // t8 = b + 128 per byte -> b as a signed byte
             uint32_t d[16];
 #pragma unroll
             for (int i = 0; i < 16; i++) d[i] = rv[i] ^ 0x80808080u;
-            fetch(rv, more ? xb + GR_X : xb);
+            fetch(rv, nxb);
             if (GRAM)
 #pragma unroll
                 for (int i = 0; i < 16; i++)
@@ -1164,7 +1176,9 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc_gram64_byte(
                 wave_sync();  // the wave's columns are rewritten by the next iteration
             }
             if (!more) break;
-            xb += GR_X;
+            k = nk;
+            cc = ncc;
+            xb = nxb;
         }
     }
     if (GRAM) {
