@@ -444,7 +444,7 @@ int salsaa_batch_tuning_ntt(const salsaa_batch *b, int *xch, float *lds_ms, floa
  * its next graph replay.  Waits for the context's stream. */
 int salsaa_batch_set_ntt_schedule(salsaa_batch *b, int xch);
 /* The NTT pass's persistent blocks the tuner chose (blocks: the fastest of
- * one per CU, 7/8 and 3/4 of that), the candidates and its times of the
+ * 29/32, 7/8 and 13/16 of the CUs), the candidates and its times of the
  * whole pass with each (cand[3], ms[3]; zero without tuning).  Any pointer
  * may be NULL. */
 int salsaa_batch_tuning_grid(const salsaa_batch *b, unsigned *blocks, unsigned *cand, float *ms);

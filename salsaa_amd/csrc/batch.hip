@@ -476,11 +476,13 @@ int autotune(salsaa_batch *b) {
         SALSAA_TRY(time_ms(b, [&] { return run_fwd_planes(b, 1); }, 2, &b->t_ntt[x]));
     }
     b->ntt_xch = b->t_ntt[1] < b->t_ntt[0] ? 1 : 0;
-    // its persistent blocks: one per CU, 7/8 or 3/4 of that (the write-heavy
-    // pass has run faster on fewer CUs than on all of them)
+    // its persistent blocks: 29/32, 7/8 or 13/16 of the CUs (the write-heavy
+    // pass has run fastest on 224..232 of 256 CUs, 8.5-8.6 ms, against 9.0-9.1
+    // on all of them and 9.1-9.2 on 192; where the optimum lies varies from
+    // box to box)
     {
         const unsigned ncu = (unsigned)b->ctx->c.ncu;
-        const unsigned cand[3] = {ncu, ncu * 7 / 8, ncu * 3 / 4};
+        const unsigned cand[3] = {ncu * 29 / 32, ncu * 7 / 8, ncu * 13 / 16};
         int best_g = 0;
         for (int g = 0; g < 3; g++) {
             b->grid_cand[g] = cand[g];
