@@ -448,6 +448,13 @@ int salsaa_batch_set_ntt_schedule(salsaa_batch *b, int xch);
  * whole pass with each (cand[3], ms[3]; zero without tuning).  Any pointer
  * may be NULL. */
 int salsaa_batch_tuning_grid(const salsaa_batch *b, unsigned *blocks, unsigned *cand, float *ms);
+/* The persistent blocks of a split step's second NTT launch, which runs
+ * beside the first part's norm checks (blocks: the tuner's fastest whole step
+ * with the pass's own blocks, 3/4 and 11/16 of the CUs), the candidates and
+ * the step times with each (cand[3], ms[3]; zero without tuning or without a
+ * split).  Any pointer may be NULL. */
+int salsaa_batch_tuning_step_grid(const salsaa_batch *b, unsigned *blocks, unsigned *cand,
+                                  float *ms);
 /* Sets the NTT pass's persistent blocks (0: one per CU); the step is captured
  * again on its next graph replay.  Waits for the context's stream. */
 int salsaa_batch_set_ntt_grid(salsaa_batch *b, unsigned blocks);

@@ -182,6 +182,9 @@ def _load():
                                             ctypes.POINTER(ctypes.c_uint),
                                             ctypes.POINTER(ctypes.c_float)]),
         "salsaa_batch_set_ntt_grid": (_int, [_vp, ctypes.c_uint]),
+        "salsaa_batch_tuning_step_grid": (_int, [_vp, ctypes.POINTER(ctypes.c_uint),
+                                                 ctypes.POINTER(ctypes.c_uint),
+                                                 ctypes.POINTER(ctypes.c_float)]),
         "salsaa_batch_tuning_commit": (_int, [_vp, ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_float)]),
         "salsaa_batch_set_commit_schedule": (_int, [_vp, _int]),
@@ -805,10 +808,15 @@ class Batch:
         _check(lib.salsaa_batch_tuning_grid(self.h, ctypes.byref(g), gc, gt))
         cm, cms = ctypes.c_int(), (ctypes.c_float * 3)()
         _check(lib.salsaa_batch_tuning_commit(self.h, ctypes.byref(cm), cms))
+        g2, g2c, g2t = ctypes.c_uint(), (ctypes.c_uint * 3)(), (ctypes.c_float * 3)()
+        _check(lib.salsaa_batch_tuning_step_grid(self.h, ctypes.byref(g2), g2c, g2t))
         return dict(kchunk=kc.value, cpt_first=cpt.value, commit_ms=tc.value,
                     norm_check_ms=tn.value, ntt_schedule=("lds", "dpp")[x.value],
                     ntt_pass_ms={"lds": tl.value, "dpp": td.value}, ntt_blocks=g.value,
                     ntt_blocks_ms={str(gc[i]): gt[i] for i in range(3) if gc[i]},
+                    step_second_pass_blocks=g2.value,
+                    step_ms_by_second_pass_blocks={str(g2c[i]): g2t[i] for i in range(3)
+                                                   if g2c[i]},
                     commit_schedule=COMMIT_SCHEDULES[cm.value],
                     commit_schedule_ms={n: cms[i] for i, n in enumerate(COMMIT_SCHEDULES)})
 

@@ -159,6 +159,12 @@ struct salsaa_batch {
     // the NTT pass's persistent blocks (0: one per CU) and the tuner's
     // times of the whole pass at each candidate
     unsigned ntt_grid = 0;
+    // the second launch's persistent blocks in a split step (0: ntt_grid);
+    // fewer leave CUs to the first part's norm checks beside it.  The tuner's
+    // candidates and its times of the whole step with each
+    unsigned ntt_grid2 = 0;
+    unsigned grid2_cand[3] = {0, 0, 0};
+    float t_grid2[3] = {0.f, 0.f, 0.f};
     unsigned grid_cand[3] = {0, 0, 0};
     float t_grid[3] = {0.f, 0.f, 0.f};
     // plain path
@@ -374,12 +380,12 @@ int run_fwd_planes(salsaa_batch *b, int roundtrip) {
 
 // the round trip over arena tiles [t0, t1) (the pass's tables and short copy
 // are indexed from the range's first tile)
-int run_rt_range(salsaa_batch *b, size_t t0, size_t t1) {
+int run_rt_range(salsaa_batch *b, size_t t0, size_t t1, unsigned grid) {
     ShortCopy sc{b->sc.t + t0 * kShortTile, b->sc.t8 + t0 * kShortTile,
                  b->sc.fmt + t0 * (kShortTile / kShortSlice)};
     return launch_ntt_fwd_planes(&b->ctx->c, b->d_wit + t0 * kShortTile,
                                  (t1 - t0) * kPlaneTile, b->d_planes, b->d_ptiles + t0, 1, &sc,
-                                 b->ntt_xch, b->ntt_grid);
+                                 b->ntt_xch, grid);
 }
 
 // one pass: forward NTT, the commit's digit planes, the inverse NTT and the
@@ -393,16 +399,16 @@ int run_step(salsaa_batch *b) {
         SALSAA_TRY(run_fwd_planes(b, 1));
         SALSAA_TRY(norm_checks_fork(b, b->cpt_first, &b->sc));
     } else if (b->split_mode == 1) {
-        SALSAA_TRY(run_rt_range(b, b->split_tile, nt));
+        SALSAA_TRY(run_rt_range(b, b->split_tile, nt, b->ntt_grid));
         SALSAA_TRY(norm_checks_fork_range(b, b->cpt_first, &b->sc, b->split_group, G, b->ev_fork));
-        SALSAA_TRY(run_rt_range(b, 0, b->split_tile));
+        SALSAA_TRY(run_rt_range(b, 0, b->split_tile, b->ntt_grid2 ? b->ntt_grid2 : b->ntt_grid));
         SALSAA_TRY(norm_checks_fork_range(b, b->cpt_first, &b->sc, 0, b->split_group,
                                           b->ev_fork2));
         SALSAA_TRY(norm_checks_mark_done(b));
     } else {
-        SALSAA_TRY(run_rt_range(b, 0, b->split_tile));
+        SALSAA_TRY(run_rt_range(b, 0, b->split_tile, b->ntt_grid));
         SALSAA_TRY(norm_checks_fork_range(b, b->cpt_first, &b->sc, 0, b->split_group, b->ev_fork));
-        SALSAA_TRY(run_rt_range(b, b->split_tile, nt));
+        SALSAA_TRY(run_rt_range(b, b->split_tile, nt, b->ntt_grid2 ? b->ntt_grid2 : b->ntt_grid));
         SALSAA_TRY(norm_checks_fork_range(b, b->cpt_first, &b->sc, b->split_group, G,
                                           b->ev_fork2));
         SALSAA_TRY(norm_checks_mark_done(b));
@@ -507,6 +513,21 @@ int autotune(salsaa_batch *b) {
     }
     b->cpt_first = best_cpt;
     b->t_nc = best_nc;
+    // a split step's second launch runs beside the first part's norm checks:
+    // the whole step timed with it on the pass's blocks, 3/4 and 11/16 of the
+    // CUs (fewer leave CUs to the norm checks)
+    if (b->split_mode && !b->ntt_grid2) {
+        const unsigned ncu = (unsigned)b->ctx->c.ncu;
+        const unsigned cand[3] = {0, ncu * 3 / 4, ncu * 11 / 16};
+        int best_g = 0;
+        for (int g = 0; g < 3; g++) {
+            b->grid2_cand[g] = cand[g] ? cand[g] : b->ntt_grid;
+            b->ntt_grid2 = cand[g];
+            SALSAA_TRY(time_ms(b, [&] { return run_step(b); }, 2, &b->t_grid2[g]));
+            if (b->t_grid2[g] < b->t_grid2[best_g]) best_g = g;
+        }
+        b->ntt_grid2 = cand[best_g];
+    }
     return SALSAA_OK;
 }
 
@@ -763,6 +784,7 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
             b->split_group = gsplit;
             b->split_tile = b->off_w[b->groups[gsplit].first] / kPlaneTile;
         }
+
     }
     for (int k = 0; k < kNcStreams; k++) {
         share_context(*c, b->nc_ctx[k]);
@@ -936,6 +958,18 @@ int salsaa_batch_tuning_grid(const salsaa_batch *b, unsigned *blocks, unsigned *
     for (int g = 0; g < 3; g++) {
         if (cand) cand[g] = b->grid_cand[g];
         if (ms) ms[g] = b->t_grid[g];
+    }
+    return SALSAA_OK;
+}
+
+int salsaa_batch_tuning_step_grid(const salsaa_batch *b, unsigned *blocks, unsigned *cand,
+                                  float *ms) {
+    SALSAA_CHECK(b != nullptr, "null batch");
+    if (blocks)
+        *blocks = b->ntt_grid2 ? b->ntt_grid2 : b->ntt_grid ? b->ntt_grid : (unsigned)b->ctx->c.ncu;
+    for (int g = 0; g < 3; g++) {
+        if (cand) cand[g] = b->grid2_cand[g];
+        if (ms) ms[g] = b->t_grid2[g];
     }
     return SALSAA_OK;
 }
