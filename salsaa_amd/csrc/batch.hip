@@ -144,10 +144,11 @@ struct salsaa_batch {
     NcScratch nc[kNcStreams];
     hipEvent_t ev_fork = nullptr, ev_fork2 = nullptr, ev_done[kNcStreams] = {};
     // the step's NTT pass in two launches split at arena tile split_tile (the
-    // start of length class split_group; the arena runs longest first):
-    // split_mode 1 transforms the shorter classes (tiles from split_tile on)
-    // first, 2 the longer ones (tiles before split_tile); the first part's
-    // norm checks run beside the second launch (split_mode 0: one launch)
+    // start of norm-check group split_group; the arena runs longest first):
+    // split_mode 1 transforms the shorter classes and part of the next one
+    // (tiles from split_tile on) first, 2 the longer classes (tiles before
+    // split_tile); the first part's norm checks run beside the second launch
+    // (split_mode 0: one launch)
     size_t split_tile = 0, split_group = 0;
     int split_mode = 0;
     int cpt_first = 0;
@@ -752,10 +753,14 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
     }
     SALSAA_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
     SALSAA_HIP(hipEventCreateWithFlags(&b->ev_fork2, hipEventDisableTiming));
-    // The split: the first part takes whole classes while they hold at most
-    // 70 % of the arena (their norm checks, at about 0.4 of the pass's time
-    // per element, then fit beside the pass over the rest).  Mode from
-    // SALSAA_STEP_SPLIT: 0 one pass, 1 shorter classes first, 2 longer first.
+    // The split.  Mode 1 (the default): the first part takes the shorter
+    // classes up to 80 % of the arena and splits the next class by rows at
+    // that mark (both halves as norm checks of their own, on its stream), so
+    // that the norm checks left beside the commit are those of a part of the
+    // longest class; their norm checks, at about a sixth of the pass's time
+    // per element, fit beside the pass over the rest.  Mode 2: the longer
+    // classes first, whole classes up to 70 %.  Mode 0: one pass.  The mode
+    // comes from SALSAA_STEP_SPLIT.
     {
         const char *env = getenv("SALSAA_STEP_SPLIT");
         const int mode = env ? atoi(env) : 1;
@@ -766,9 +771,17 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
             while (gsplit > 1) {
                 const auto &g = b->groups[gsplit - 1];
                 const size_t e = g.rows * (g.L / kD);
-                if (10 * (acc + e) > 7 * b->tot_w) break;
+                if (10 * (acc + e) > 8 * b->tot_w) break;
                 acc += e;
                 gsplit--;
+            }
+            const salsaa_batch::Group g = b->groups[gsplit - 1];
+            const size_t rb = (8 * b->tot_w / 10 - acc) / (g.L / kD);  // rows to the mark
+            if (rb >= 1 && rb < g.rows) {
+                b->groups[gsplit - 1].rows = g.rows - rb;
+                b->groups.insert(b->groups.begin() + gsplit,
+                                 salsaa_batch::Group{g.first + g.rows - rb, rb, g.L});
+                b->group_stream.insert(b->group_stream.begin() + gsplit, b->group_stream[gsplit - 1]);
             }
         } else if (mode == 2) {
             while (gsplit + 1 < G) {
@@ -779,12 +792,11 @@ int create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m, co
                 gsplit++;
             }
         }
-        if ((mode == 1 || mode == 2) && gsplit > 0 && gsplit < G) {
+        if ((mode == 1 || mode == 2) && gsplit > 0 && gsplit < b->groups.size()) {
             b->split_mode = mode;
             b->split_group = gsplit;
             b->split_tile = b->off_w[b->groups[gsplit].first] / kPlaneTile;
         }
-
     }
     for (int k = 0; k < kNcStreams; k++) {
         share_context(*c, b->nc_ctx[k]);

@@ -324,9 +324,11 @@ def test_batch_wide_rows_long_walk(sa, ctx, orc):
 
 def test_batch_split_pass_equals_single_pass(sa, ctx, orc):
     # The step's NTT pass runs in two launches (SALSAA_STEP_SPLIT=1: the
-    # shorter classes first, 2: the longer ones first; the first part's norm
-    # checks overlap the rest of the pass) or one (0): every order gives the
-    # plain path's outputs.
+    # shorter classes first and, up to 80 % of the arena, rows of the next
+    # class, whose norm check then runs as two groups (here the 1024 class,
+    # one row in each part); 2: the longer classes first; the first part's
+    # norm checks overlap the rest of the pass) or one (0): every order gives
+    # the plain path's outputs.
     import os
     say("case: batch split pass vs single pass")
     m = [1024, 1024, 512, 256, 256, 128, 64, 32, 8, 1]
