@@ -455,8 +455,9 @@ int salsaa_batch_tuning_grid(const salsaa_batch *b, unsigned *blocks, unsigned *
  * split).  Any pointer may be NULL. */
 int salsaa_batch_tuning_step_grid(const salsaa_batch *b, unsigned *blocks, unsigned *cand,
                                   float *ms);
-/* Sets the NTT pass's persistent blocks (0: one per CU); the step is captured
- * again on its next graph replay.  Waits for the context's stream. */
+/* Sets the NTT pass's persistent blocks (0: one per CU), for both launches of
+ * a split step; the step is captured again on its next graph replay.  Waits
+ * for the context's stream. */
 int salsaa_batch_set_ntt_grid(salsaa_batch *b, unsigned blocks);
 /* The commit schedule the tuner chose (0: one launch per class of key row
  * tiles; 1: one launch over every class, in which the blocks of all classes

@@ -997,6 +997,7 @@ int salsaa_batch_set_ntt_grid(salsaa_batch *b, unsigned blocks) {
     b->exec = nullptr;
     b->graph = nullptr;
     b->ntt_grid = blocks;
+    b->ntt_grid2 = 0;  // both launches of a split step
     return SALSAA_OK;
 }
 
