@@ -403,8 +403,11 @@ typedef struct {
 
 /* Plan a batch: allocates the witness/challenge arenas and both output sets
  * on the device, sorts the instances by length and builds the grouped
- * launches.  tune = 1 times candidate commit K chunks and norm check
- * first-pass widths on the device and keeps the fastest. */
+ * launches.  tune = 1 times on the device, and keeps the fastest of, the
+ * commit's K chunks and schedules, the NTT pass's exchange and block count,
+ * the norm check's first-pass widths and a split step's second-launch block
+ * count, over synthetic short witnesses written into the new arena, which
+ * is zeroed again before the call returns. */
 int salsaa_batch_create(salsaa_ctx *ctx, const salsaa_key *key, size_t n, const size_t *m,
                         const size_t *kappa, size_t reps, int fq2, uint64_t nr, int tune,
                         salsaa_batch **out);
