@@ -70,7 +70,15 @@ def test_batch_matches_plain_and_oracle(sa, ctx, orc, fq2, reps):
     A_h = orc.sample_uniform(3, sa.STREAM_A, (kmax, mmax, 64), Q)
     key = ctx.key(ctx.to_device(A_h))
     b = sa.Batch(ctx, key, m, kappa, reps=reps, fq2=fq2, tune=True)
-    say("  tuning", b.tuning())
+    t = b.tuning()
+    say("  tuning", t)
+    # the tuner's choices are among its timed candidates, and its synthetic
+    # witnesses are gone (the commitments below also need the arena's padding
+    # zero again)
+    assert str(t["ntt_blocks"]) in t["ntt_blocks_ms"]
+    if t["step_ms_by_second_pass_blocks"]:
+        assert str(t["step_second_pass_blocks"]) in t["step_ms_by_second_pass_blocks"]
+    assert not any(b.slot(i)["witness"].download().any() for i in range(n))
     fill(sa, ctx, b, 100)
     W0 = [b.slot(i)["witness"].download() for i in range(n)]
     b.step_plain()
