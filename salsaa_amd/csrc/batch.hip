@@ -102,7 +102,7 @@ struct salsaa_batch {
     int8_t *d_planes = nullptr;
     size_t planes_bytes = 0;
     // the arena's short copy, written by every round trip and read by the
-    // norm checks of the same step (its tile flags start at "not short")
+    // norm checks of the same step (its format bytes start at 2, not kept)
     ShortCopy sc;
     // commit
     // Jobs of 16 instances in commit order: instances grouped by the key row
